@@ -1,0 +1,106 @@
+"""Synthetic R1CS workloads (harness side, in the role of exp_cell's builders).
+
+The reference benchmarks its own x*x=y circuit (snark-ring/examples/
+exp_cell.rs:38-65).  The configurations measured here use random R1CS
+instead, drawn from an instance seed with SplitMix64 (not `rand`):
+
+* square n x n matrices, column 0 is the constant 1;
+* columns [1, n/2) are uniform limbs of Z/2^k (the inputs);
+* row i < n/2 gets A and B rows with 1..3 nonzeros (A: 1..4 for the MIMD
+  serving mix) at uniform columns in [0, n/2) with uniform coefficients,
+  and C[i, n/2+i] = 1 with the auxiliary z[n/2+i] = (Az)_i * (Bz)_i mod 2^k;
+* rows >= n/2 are empty.
+
+Matrices are COO triples (rows u32, cols u32, vals u64 reduced mod 2^k);
+witnesses are u64 arrays of limbs.  Duplicated (row, col) entries are legal
+and sum, as in the reference's COO matvecs (reduction/src/matrix.rs:11-18).
+"""
+import numpy as np
+
+_G = np.uint64(0x9E3779B97F4A7C15)
+
+
+def _mix(z):
+    z = np.asarray(z, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def stream(seed, tag, count):
+    """`count` SplitMix64 outputs for (seed, tag)."""
+    with np.errstate(over="ignore"):
+        base = _mix(np.uint64(seed & (2**64 - 1)) ^ (np.uint64(tag) * np.uint64(0xD1B54A32D192ED03)))
+        idx = np.arange(1, count + 1, dtype=np.uint64)
+        return _mix(base + idx * _G)
+
+
+def mask(k):
+    return np.uint64((1 << k) - 1) if k < 64 else np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def random_matrices(n, k, seed, max_a=3, max_b=3):
+    """A, B, C of the half-constrained random layout (n a power of two >= 4)."""
+    half = n // 2
+    msk = mask(k)
+    out = []
+    for tag, mx in ((1, max_a), (2, max_b)):
+        cnt = (stream(seed, tag, half) % np.uint64(mx)).astype(np.int64) + 1
+        rows = np.repeat(np.arange(half, dtype=np.uint32), cnt)
+        nnz = len(rows)
+        cols = (stream(seed, tag + 10, nnz) % np.uint64(half)).astype(np.uint32)
+        vals = stream(seed, tag + 20, nnz) & msk
+        out.append((rows, cols, vals))
+    rows = np.arange(half, dtype=np.uint32)
+    out.append((rows, (rows + half).astype(np.uint32), np.ones(half, np.uint64)))
+    return tuple(out)
+
+
+def _matvec(m, z, n, k):
+    r, c, v = m
+    msk = mask(k)
+    with np.errstate(over="ignore"):
+        prod = (v * z[c]) & msk
+    out = np.zeros(n, np.uint64)
+    # wrapping accumulation: reduce per row with np.add.at on uint64 (wraps mod 2^64)
+    np.add.at(out, r, prod)
+    return out & msk
+
+
+def random_witness(n, k, A, B, seed, tag=0):
+    """Satisfying witness for (A, B, C) from random_matrices."""
+    half = n // 2
+    msk = mask(k)
+    z = np.zeros(n, np.uint64)
+    z[0] = 1
+    z[1:half] = stream(seed, 100 + tag, half - 1) & msk
+    az = _matvec(A, z, half, k)
+    bz = _matvec(B, z, half, k)
+    with np.errstate(over="ignore"):
+        z[half:] = (az * bz) & msk
+    return z
+
+
+def is_satisfied(A, B, Cm, z, n, k):
+    msk = mask(k)
+    az, bz, cz = (_matvec(m, z, n, k) for m in (A, B, Cm))
+    with np.errstate(over="ignore"):
+        return bool(np.all(((az * bz) & msk) == cz))
+
+
+def general_instance(log_n, k, seed):
+    """Config-1 style General instance: n = 2^log_n (log_blocks = log_n - 5)."""
+    n = 1 << log_n
+    A, B, Cm = random_matrices(n, k, seed)
+    w = random_witness(n, k, A, B, seed)
+    return A, B, Cm, w
+
+
+def simd_instance(log, k, seed):
+    """Config-0/2 style SIMD batch: shared n x n matrices (n = 2^log) and 32
+    witnesses with independent inputs; returns (A, B, C, wit[32, n])."""
+    n = 1 << log
+    A, B, Cm = random_matrices(n, k, seed)
+    wit = np.stack([random_witness(n, k, A, B, seed, tag=1 + s) for s in range(32)])
+    return A, B, Cm, wit
