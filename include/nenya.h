@@ -1,0 +1,91 @@
+/*
+ * nenya.h -- C ABI of the MI355X (gfx950) Nenya prover.
+ *
+ * Drop-in replacement for the reference prover API (tnyuzg/nenya,
+ * crates/nenya/snark-ring/src/lib.rs:23-36): R1CS instance + witness in,
+ * serialized proof bytes out, same BLAKE3 Fiat-Shamir transcript, so the
+ * reference verifier accepts the bytes.
+ *
+ * Conventions
+ *   - k in {8, 16, 32, 64}: the base ring Z/2^k.  Every limb buffer
+ *     (`vals`, witnesses) holds one little-endian k-bit integer per element,
+ *     i.e. arrays of uint8_t / uint16_t / uint32_t / uint64_t.
+ *   - Sparse matrices are COO triples (rows[i], cols[i], vals[i]), i < nnz;
+ *     duplicates sum (reduction/src/matrix.rs:11-18).
+ *   - Return 0 on success or a NENYA_E_* code; nenya_last_error() describes
+ *     the failure (thread-local).  Nothing panics or unwinds across the ABI
+ *     (the reference asserts on shape misuse: simd.rs:224-231,
+ *     reduce.rs:463-467).
+ *   - Proof buffers are library-allocated; release them with nenya_free().
+ *   - A params object is immutable after creation and owns its device memory
+ *     and HIP stream; calls on one params object are serialized internally.
+ */
+#ifndef NENYA_H
+#define NENYA_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NENYA_OK 0
+#define NENYA_E_SHAPE 1  /* a shape/size mismatch (the reference's assert!) */
+#define NENYA_E_OOM 2    /* device memory exhausted */
+#define NENYA_E_DEVICE 3 /* HIP runtime error or internal self-check */
+#define NENYA_E_ARG 4    /* invalid argument (width, null pointer, ...) */
+#define NENYA_E_NOGPU 5  /* no HIP device visible */
+
+typedef struct nenya_params nenya_params;
+
+/* SimdSnarkParams::secure(log_n, log_m, seed_w, seed_q)
+ * (snark-ring/src/simd.rs:112-119): Pcs::setup for the packed witness and
+ * R2Pcs::setup for the kernel error at delta=0.19, lambda=100, rate 1/4. */
+int nenya_params_simd_secure(int k, int log_n, int log_m, uint64_t seed_w, uint64_t seed_q, nenya_params** out);
+
+/* prove_simd(&SimdR1csInstance, &params) (snark-ring/src/simd.rs:212-217):
+ * A, B, C are n x m (n = 2^log_n, m = 2^log_m); `witnesses` holds the 32
+ * witnesses back to back (32 * m limbs).  Returns SimdSnarkProof::serialize. */
+int nenya_prove_simd(const nenya_params* p,
+                     const uint32_t* a_rows, const uint32_t* a_cols, const void* a_vals, size_t a_nnz,
+                     const uint32_t* b_rows, const uint32_t* b_cols, const void* b_vals, size_t b_nnz,
+                     const uint32_t* c_rows, const uint32_t* c_cols, const void* c_vals, size_t c_nnz,
+                     const void* witnesses, uint8_t** proof, size_t* proof_len);
+
+/* GeneralSnarkParams::secure(log_blocks, pack_general_matrices(a, b, c),
+ * seed_w, seed_e) (snark-ring/src/general.rs:69-95,
+ * reduction/src/reduce.rs:618-651): the cyclic-diagonal packing of the
+ * square n x n matrices (n = 32 * 2^log_blocks) is done here, once, on setup. */
+int nenya_params_general_secure(int k, int log_blocks,
+                                const uint32_t* a_rows, const uint32_t* a_cols, const void* a_vals, size_t a_nnz,
+                                const uint32_t* b_rows, const uint32_t* b_cols, const void* b_vals, size_t b_nnz,
+                                const uint32_t* c_rows, const uint32_t* c_cols, const void* c_vals, size_t c_nnz,
+                                uint64_t seed_w, uint64_t seed_e, nenya_params** out);
+
+/* prove_general(w, &params) (snark-ring/src/general.rs:367-372): w has
+ * n = 32 * 2^log_blocks limbs in host memory.  Returns
+ * GeneralSnarkProof::serialize. */
+int nenya_prove_general(const nenya_params* p, const void* w, size_t n, uint8_t** proof, size_t* proof_len);
+
+/* Same as nenya_prove_general with `w` already resident in device memory
+ * (a hipMalloc'd pointer on the current device).  No reference counterpart:
+ * lets a caller that produced the witness on the GPU skip the host copy. */
+int nenya_prove_general_device(const nenya_params* p, const void* d_w, size_t n, uint8_t** proof, size_t* proof_len);
+
+/* Per-stage wall-clock timing (ms) of the last proof made with `p` when
+ * enabled (GeneralProveTiming / SimdProveTiming, general.rs:153-173,
+ * simd.rs:177-192).  Enabling adds stream synchronizations at stage
+ * boundaries.  nenya_last_timing writes up to `cap` entries and returns the
+ * count; names are separated by '\n' in `names` (up to names_cap bytes). */
+int nenya_set_timing(nenya_params* p, int enabled);
+int nenya_last_timing(const nenya_params* p, double* ms, int cap, char* names, size_t names_cap);
+
+void nenya_params_free(nenya_params* p);
+void nenya_free(void* ptr);
+const char* nenya_last_error(void);
+int nenya_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NENYA_H */

@@ -1,0 +1,20 @@
+"""MI355X-native Nenya prover (gfx950 HIP kernels behind a C ABI).
+
+Python mirror of the reference prover interface (tnyuzg/nenya,
+crates/nenya/snark-ring/src/lib.rs:23-36):
+
+    params = SimdSnarkParams.secure(log_n, log_m, seed_w, seed_q, k=32)
+    proof  = prove_simd(SimdR1csInstance(a, b, c, witnesses), params)
+
+    params = GeneralSnarkParams.secure(log_blocks, (a, b, c), seed_w, seed_e, k=32)
+    proof  = prove_general(w, params)
+
+Matrices are COO triples (rows u32, cols u32, vals) with vals limbs of Z/2^k;
+witnesses are limb arrays.  Proofs are the reference's serialized bytes.
+Every call runs the HIP library nenya_amd/libnenya_amd.so; there is no CPU
+fallback: without the library or a GPU the calls raise.
+"""
+from ._lib import (  # noqa: F401
+    NenyaError, GeneralSnarkParams, SimdR1csInstance, SimdSnarkParams, device_count, lib_path,
+    prove_general, prove_general_device, prove_simd,
+)

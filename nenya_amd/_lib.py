@@ -1,0 +1,169 @@
+"""ctypes binding of include/nenya.h (the library is loaded lazily)."""
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_DT = {8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}
+_CODES = {1: "shape", 2: "out of device memory", 3: "device", 4: "argument", 5: "no GPU"}
+
+
+class NenyaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"nenya error {code} ({_CODES.get(code, '?')}): {msg}")
+        self.code = code
+
+
+def lib_path():
+    return os.path.join(HERE, "libnenya_amd.so")
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        p = lib_path()
+        if not os.path.exists(p):
+            raise NenyaError(3, f"{p} is not built; run `python -m nenya_amd.build`")
+        L = C.CDLL(p)
+        vp, sz, u64 = C.c_void_p, C.c_size_t, C.c_uint64
+        coo = [vp, vp, vp, sz]
+        L.nenya_params_simd_secure.argtypes = [C.c_int, C.c_int, C.c_int, u64, u64, C.POINTER(vp)]
+        L.nenya_prove_simd.argtypes = [vp] + coo * 3 + [vp, C.POINTER(vp), C.POINTER(sz)]
+        L.nenya_params_general_secure.argtypes = [C.c_int, C.c_int] + coo * 3 + [u64, u64, C.POINTER(vp)]
+        L.nenya_prove_general.argtypes = [vp, vp, sz, C.POINTER(vp), C.POINTER(sz)]
+        L.nenya_prove_general_device.argtypes = [vp, vp, sz, C.POINTER(vp), C.POINTER(sz)]
+        L.nenya_set_timing.argtypes = [vp, C.c_int]
+        L.nenya_last_timing.argtypes = [vp, C.POINTER(C.c_double), C.c_int, C.c_char_p, sz]
+        L.nenya_params_free.argtypes = [vp]
+        L.nenya_free.argtypes = [vp]
+        L.nenya_last_error.restype = C.c_char_p
+        L.nenya_device_count.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def device_count():
+    return int(_lib().nenya_device_count())
+
+
+def _check(rc):
+    if rc != 0:
+        raise NenyaError(rc, _lib().nenya_last_error().decode(errors="replace"))
+
+
+def _coo(m, k):
+    r, c, v = m
+    r = np.ascontiguousarray(r, np.uint32)
+    c = np.ascontiguousarray(c, np.uint32)
+    v = np.ascontiguousarray(np.asarray(v).astype(_DT[k]))
+    if not (len(r) == len(c) == len(v)):
+        raise NenyaError(1, "COO arrays differ in length")
+    return (r, c, v)
+
+
+def _coo_args(m):
+    r, c, v = m
+    return [r.ctypes.data, c.ctypes.data, v.ctypes.data, len(r)]
+
+
+def _take_proof(buf, n):
+    try:
+        return C.string_at(buf, n.value)
+    finally:
+        _lib().nenya_free(buf)
+
+
+class _Params:
+    kind = None
+
+    def __init__(self, handle, k):
+        self._h = handle
+        self.k = k
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _LIB is not None:
+            _LIB.nenya_params_free(h)
+            self._h = None
+
+    def set_timing(self, on=True):
+        _check(_lib().nenya_set_timing(self._h, 1 if on else 0))
+
+    def last_timing(self):
+        ms = (C.c_double * 64)()
+        names = C.create_string_buffer(4096)
+        n = _lib().nenya_last_timing(self._h, ms, 64, names, 4096)
+        labels = names.value.decode().split("\n")[:n]
+        return {labels[i]: ms[i] for i in range(n)}
+
+
+class SimdSnarkParams(_Params):
+    """SimdSnarkParams::secure (snark-ring/src/simd.rs:112-119)."""
+
+    @classmethod
+    def secure(cls, log_n, log_m, seed_w=0xDEAD, seed_q=0xCAFE, k=32):
+        h = C.c_void_p()
+        _check(_lib().nenya_params_simd_secure(k, log_n, log_m, seed_w, seed_q, C.byref(h)))
+        p = cls(h, k)
+        p.log_n, p.log_m = log_n, log_m
+        return p
+
+
+@dataclass
+class SimdR1csInstance:
+    """reduction/src/instance.rs:11-21: shared (A, B, C) and 32 witnesses."""
+    a: tuple
+    b: tuple
+    c: tuple
+    witnesses: np.ndarray  # shape (32, m)
+
+
+def prove_simd(inst, params):
+    """prove_simd(&SimdR1csInstance, &SimdSnarkParams) -> proof bytes."""
+    k = params.k
+    mats = [_coo(m, k) for m in (inst.a, inst.b, inst.c)]
+    wit = np.ascontiguousarray(np.asarray(inst.witnesses).astype(_DT[k]))
+    if wit.shape != (32, 1 << params.log_m):
+        raise NenyaError(1, f"witnesses must have shape (32, {1 << params.log_m})")
+    out, n = C.c_void_p(), C.c_size_t()
+    args = [params._h]
+    for m in mats:
+        args += _coo_args(m)
+    _check(_lib().nenya_prove_simd(*args, wit.ctypes.data, C.byref(out), C.byref(n)))
+    return _take_proof(out, n)
+
+
+class GeneralSnarkParams(_Params):
+    """GeneralSnarkParams::secure(log_blocks, pack_general_matrices(a,b,c), seed_w, seed_e)
+    (snark-ring/src/general.rs:69-95)."""
+
+    @classmethod
+    def secure(cls, log_blocks, matrices, seed_w=0xDEAD, seed_e=0xCAFE, k=32):
+        mats = [_coo(m, k) for m in matrices]
+        args = []
+        for m in mats:
+            args += _coo_args(m)
+        h = C.c_void_p()
+        _check(_lib().nenya_params_general_secure(k, log_blocks, *args, seed_w, seed_e, C.byref(h)))
+        p = cls(h, k)
+        p.log_blocks = log_blocks
+        return p
+
+
+def prove_general(w, params):
+    """prove_general(w, &GeneralSnarkParams) -> proof bytes."""
+    w = np.ascontiguousarray(np.asarray(w).astype(_DT[params.k]))
+    out, n = C.c_void_p(), C.c_size_t()
+    _check(_lib().nenya_prove_general(params._h, w.ctypes.data, len(w), C.byref(out), C.byref(n)))
+    return _take_proof(out, n)
+
+
+def prove_general_device(dev_ptr, n_limbs, params):
+    """prove_general with the witness already in device memory (e.g. a torch
+    tensor's data_ptr())."""
+    out, n = C.c_void_p(), C.c_size_t()
+    _check(_lib().nenya_prove_general_device(params._h, C.c_void_p(dev_ptr), n_limbs, C.byref(out), C.byref(n)))
+    return _take_proof(out, n)

@@ -1,0 +1,220 @@
+// extern "C" boundary of the MI355X Nenya prover (include/nenya.h).
+#include <cstdlib>
+#include <mutex>
+
+#include "../../include/nenya.h"
+#include "snark.hpp"
+
+using namespace nenya;
+
+struct nenya_params {
+  ParamsBase* impl = nullptr;
+  std::mutex mu;
+};
+
+static thread_local std::string g_err;
+
+template <class F> static int guard(F&& f) {
+  try {
+    g_err.clear();
+    return f();
+  } catch (const Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return NENYA_E_OOM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return NENYA_E_DEVICE;
+  }
+}
+
+static void need_gpu() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    throw Error(NENYA_E_NOGPU, "no HIP device visible");
+  }
+}
+static void need_width(int k) {
+  if (k != 8 && k != 16 && k != 32 && k != 64) throw Error(NENYA_E_ARG, "k must be 8, 16, 32 or 64");
+}
+static uint8_t* emit(const std::vector<uint8_t>& v, uint8_t** proof, size_t* len) {
+  uint8_t* b = (uint8_t*)std::malloc(v.size() ? v.size() : 1);
+  if (!b) throw std::bad_alloc();
+  std::memcpy(b, v.data(), v.size());
+  *proof = b;
+  *len = v.size();
+  return b;
+}
+
+template <class T>
+static std::vector<uint8_t> simd_prove_t(ParamsBase* b, const HostCoo* m3, const void* wit) {
+  auto* sp = static_cast<SimdParams<T>*>(b);
+  return sp->prove(m3[0], m3[1], m3[2], (const typename T::L*)wit);
+}
+
+template <class T> static std::vector<uint8_t> general_t(ParamsBase* b, const void* w, size_t n, bool dev) {
+  auto* gp = static_cast<GeneralParams<T>*>(b);
+  if (dev) {
+    if (n != gp->N) throw Error(NENYA_E_SHAPE, "witness length must be 32 * 2^log_blocks");
+    return gp->prove_device((const typename T::L*)w);
+  }
+  return gp->prove_host((const typename T::L*)w, n);
+}
+
+static int general_common(const nenya_params* p, const void* w, size_t n, uint8_t** proof, size_t* len, bool dev) {
+  return guard([&] {
+    if (!p || !p->impl || p->impl->kind != 1 || !w || !proof || !len) throw Error(NENYA_E_ARG, "bad argument");
+    auto* np = const_cast<nenya_params*>(p);
+    std::lock_guard<std::mutex> g(np->mu);
+    std::vector<uint8_t> v;
+    switch (p->impl->k) {
+      case 8: v = general_t<W<8>>(p->impl, w, n, dev); break;
+      case 16: v = general_t<W<16>>(p->impl, w, n, dev); break;
+      case 32: v = general_t<W<32>>(p->impl, w, n, dev); break;
+      default: v = general_t<W<64>>(p->impl, w, n, dev); break;
+    }
+    emit(v, proof, len);
+    return NENYA_OK;
+  });
+}
+
+extern "C" {
+
+int nenya_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+const char* nenya_last_error(void) { return g_err.c_str(); }
+
+void nenya_free(void* p) { std::free(p); }
+
+void nenya_params_free(nenya_params* p) {
+  if (!p) return;
+  delete p->impl;
+  delete p;
+}
+
+int nenya_params_simd_secure(int k, int log_n, int log_m, uint64_t seed_w, uint64_t seed_q, nenya_params** out) {
+  return guard([&] {
+    if (!out) throw Error(NENYA_E_ARG, "null out");
+    need_width(k);
+    if (log_n < 2 || log_m < 2 || log_n > 30 || log_m > 30) throw Error(NENYA_E_SHAPE, "log_n/log_m out of range");
+    need_gpu();
+    auto* p = new nenya_params;
+    try {
+      switch (k) {
+        case 8: p->impl = new SimdParams<W<8>>(log_n, log_m, seed_w, seed_q); break;
+        case 16: p->impl = new SimdParams<W<16>>(log_n, log_m, seed_w, seed_q); break;
+        case 32: p->impl = new SimdParams<W<32>>(log_n, log_m, seed_w, seed_q); break;
+        default: p->impl = new SimdParams<W<64>>(log_n, log_m, seed_w, seed_q); break;
+      }
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *out = p;
+    return NENYA_OK;
+  });
+}
+
+int nenya_prove_simd(const nenya_params* p, const uint32_t* ar, const uint32_t* ac, const void* av, size_t an,
+                     const uint32_t* br, const uint32_t* bc, const void* bv, size_t bn, const uint32_t* cr,
+                     const uint32_t* cc, const void* cv, size_t cn, const void* witnesses, uint8_t** proof,
+                     size_t* proof_len) {
+  return guard([&] {
+    if (!p || !p->impl || p->impl->kind != 0 || !witnesses || !proof || !proof_len) throw Error(NENYA_E_ARG, "bad argument");
+    auto* np = const_cast<nenya_params*>(p);
+    std::lock_guard<std::mutex> g(np->mu);
+    ParamsBase* b = p->impl;
+    int ln = 0, lm = 0;
+    switch (b->k) {
+      case 8: ln = static_cast<SimdParams<W<8>>*>(b)->log_n; lm = static_cast<SimdParams<W<8>>*>(b)->log_m; break;
+      case 16: ln = static_cast<SimdParams<W<16>>*>(b)->log_n; lm = static_cast<SimdParams<W<16>>*>(b)->log_m; break;
+      case 32: ln = static_cast<SimdParams<W<32>>*>(b)->log_n; lm = static_cast<SimdParams<W<32>>*>(b)->log_m; break;
+      default: ln = static_cast<SimdParams<W<64>>*>(b)->log_n; lm = static_cast<SimdParams<W<64>>*>(b)->log_m; break;
+    }
+    size_t n = size_t(1) << ln, m = size_t(1) << lm;
+    if ((an && (!ar || !ac || !av)) || (bn && (!br || !bc || !bv)) || (cn && (!cr || !cc || !cv)))
+      throw Error(NENYA_E_ARG, "null matrix array");
+    HostCoo m3[3] = {{n, m, an, ar, ac, av}, {n, m, bn, br, bc, bv}, {n, m, cn, cr, cc, cv}};
+    std::vector<uint8_t> v;
+    switch (b->k) {
+      case 8: v = simd_prove_t<W<8>>(b, m3, witnesses); break;
+      case 16: v = simd_prove_t<W<16>>(b, m3, witnesses); break;
+      case 32: v = simd_prove_t<W<32>>(b, m3, witnesses); break;
+      default: v = simd_prove_t<W<64>>(b, m3, witnesses); break;
+    }
+    emit(v, proof, proof_len);
+    return NENYA_OK;
+  });
+}
+
+int nenya_params_general_secure(int k, int log_blocks, const uint32_t* ar, const uint32_t* ac, const void* av, size_t an,
+                                const uint32_t* br, const uint32_t* bc, const void* bv, size_t bn, const uint32_t* cr,
+                                const uint32_t* cc, const void* cv, size_t cn, uint64_t seed_w, uint64_t seed_e,
+                                nenya_params** out) {
+  return guard([&] {
+    if (!out) throw Error(NENYA_E_ARG, "null out");
+    need_width(k);
+    if (log_blocks < 2 || log_blocks > 26) throw Error(NENYA_E_SHAPE, "log_blocks out of range");
+    if ((an && (!ar || !ac || !av)) || (bn && (!br || !bc || !bv)) || (cn && (!cr || !cc || !cv)))
+      throw Error(NENYA_E_ARG, "null matrix array");
+    need_gpu();
+    size_t n = size_t(32) << log_blocks;
+    HostCoo m3[3] = {{n, n, an, ar, ac, av}, {n, n, bn, br, bc, bv}, {n, n, cn, cr, cc, cv}};
+    auto* p = new nenya_params;
+    try {
+      switch (k) {
+        case 8: p->impl = new GeneralParams<W<8>>(log_blocks, m3, seed_w, seed_e); break;
+        case 16: p->impl = new GeneralParams<W<16>>(log_blocks, m3, seed_w, seed_e); break;
+        case 32: p->impl = new GeneralParams<W<32>>(log_blocks, m3, seed_w, seed_e); break;
+        default: p->impl = new GeneralParams<W<64>>(log_blocks, m3, seed_w, seed_e); break;
+      }
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *out = p;
+    return NENYA_OK;
+  });
+}
+
+int nenya_prove_general(const nenya_params* p, const void* w, size_t n, uint8_t** proof, size_t* len) {
+  return general_common(p, w, n, proof, len, false);
+}
+int nenya_prove_general_device(const nenya_params* p, const void* d_w, size_t n, uint8_t** proof, size_t* len) {
+  return general_common(p, d_w, n, proof, len, true);
+}
+
+int nenya_set_timing(nenya_params* p, int enabled) {
+  if (!p || !p->impl) return NENYA_E_ARG;
+  p->impl->timing = enabled != 0;
+  return NENYA_OK;
+}
+int nenya_last_timing(const nenya_params* p, double* ms, int cap, char* names, size_t names_cap) {
+  if (!p || !p->impl) return 0;
+  const auto& t = p->impl->last_timing;
+  int n = (int)std::min<size_t>(t.size(), cap > 0 ? (size_t)cap : 0);
+  std::string all;
+  for (int i = 0; i < n; i++) {
+    if (ms) ms[i] = t[i].second;
+    all += t[i].first;
+    all += '\n';
+  }
+  if (names && names_cap) {
+    size_t c = std::min(all.size(), names_cap - 1);
+    std::memcpy(names, all.data(), c);
+    names[c] = 0;
+  }
+  return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,352 @@
+// Host-side building blocks: device memory pool, per-width constants,
+// PCS parameters, Fiat-Shamir transcript.
+#pragma once
+#include <cmath>
+#include <map>
+#include <memory>
+#include <mutex>
+
+#include "blake3.hpp"
+#include "common.hpp"
+#include "rng.hpp"
+
+namespace nenya {
+
+// ---------------------------------------------------------------------------
+// Caching device allocator: blocks are returned to per-size free lists and
+// reused by later proofs with the same parameters (no hipMalloc in steady state).
+// ---------------------------------------------------------------------------
+struct Pool {
+  std::multimap<size_t, void*> free_;
+  std::vector<std::pair<void*, size_t>> all_;
+  size_t live = 0, peak = 0;
+  std::mutex mu;
+  void* get(size_t bytes) {
+    bytes = (bytes + 255) & ~size_t(255);
+    if (bytes == 0) bytes = 256;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = free_.find(bytes);
+    if (it != free_.end()) {
+      void* p = it->second;
+      free_.erase(it);
+      live += bytes;
+      peak = std::max(peak, live);
+      return p;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      // release cached blocks and retry once
+      for (auto& kv : free_) {
+        (void)hipFree(kv.second);
+        for (auto& a : all_)
+          if (a.first == kv.second) a.first = nullptr;
+      }
+      free_.clear();
+      (void)hipGetLastError();
+      e = hipMalloc(&p, bytes);
+      if (e != hipSuccess) throw Error(NENYA_E_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    all_.push_back({p, bytes});
+    live += bytes;
+    peak = std::max(peak, live);
+    return p;
+  }
+  void put(void* p, size_t bytes) {
+    if (!p) return;
+    bytes = (bytes + 255) & ~size_t(255);
+    if (bytes == 0) bytes = 256;
+    std::lock_guard<std::mutex> g(mu);
+    free_.insert({bytes, p});
+    live -= bytes;
+  }
+  ~Pool() {
+    for (auto& a : all_)
+      if (a.first) (void)hipFree(a.first);
+  }
+};
+
+template <class X> struct DBuf {
+  Pool* pool = nullptr;
+  X* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(Pool* pl, size_t cnt) : pool(pl), n(cnt) { p = (X*)pool->get(cnt * sizeof(X)); }
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : pool(o.pool), p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    release();
+    pool = o.pool; p = o.p; n = o.n;
+    o.p = nullptr; o.n = 0;
+    return *this;
+  }
+  void release() {
+    if (p) pool->put(p, n * sizeof(X));
+    p = nullptr;
+    n = 0;
+  }
+  ~DBuf() { release(); }
+  X* get() const { return p; }
+};
+
+// ---------------------------------------------------------------------------
+// Per-width packing constants (reduction/src/constants.rs:198-302):
+//   T1 (teichmuller.rs:65-76), l_beta_i(t) = 1 - t^15 (i = 0) or
+//   15^-1 prod_{j != i} (t - t_j) evaluated at beta, and the 32 images
+//   basis[2i] = l_beta_i (1 - alpha), basis[2i+1] = l_beta_i alpha
+//   (= pack_r2(e_n), pcs.rs:49-55).  All coefficients lie in R1 and the
+//   polynomials have degree <= 15 in beta, so evaluation at beta needs no
+//   reduction mod f2: slot c of l_beta_i is the t^c coefficient.
+// ---------------------------------------------------------------------------
+template <class T> struct G4 {
+  using C = typename T::C;
+  C v[4];
+  static G4 mul(const G4& a, const G4& b) {
+    C p[7], o[4];
+    g4_raw(a.v, b.v, p);
+    g4_reduce(p, o);
+    G4 r;
+    for (int i = 0; i < 4; i++) r.v[i] = (typename T::L)o[i];
+    return r;
+  }
+};
+
+template <class T> struct Consts {
+  using L = typename T::L;
+  using C = typename T::C;
+  G4<T> t1[16];
+  Gr<T> lbeta[16];
+  Gr<T> basis[32];
+  std::vector<L> psi_pw;  // [16][32][4]: t_i^k
+  L inv15;
+
+  Consts() {
+    const int K = T::K;
+    // inverse of 15 by 6 Newton doublings (limb.rs:91-101)
+    L x = 1;
+    for (int i = 0; i < 6; i++) x = (L)(x * (L)(2 - (L)(15 * x)));
+    inv15 = x;
+    // T1: iterate x -> x^16 K times from the natural lifts of alpha^(i-1)
+    G4<T> alpha{{0, 1, 0, 0}}, pw{{1, 0, 0, 0}};
+    t1[0] = G4<T>{{0, 0, 0, 0}};
+    for (int i = 1; i < 16; i++) {
+      G4<T> y = pw;
+      for (int it = 0; it < K; it++)
+        for (int s = 0; s < 4; s++) y = G4<T>::mul(y, y);
+      t1[i] = y;
+      pw = G4<T>::mul(pw, alpha);
+    }
+    // Lagrange basis polynomials in t with R1 coefficients
+    for (int i = 0; i < 16; i++) {
+      std::vector<G4<T>> poly(17, G4<T>{{0, 0, 0, 0}});
+      if (i == 0) {
+        poly[0].v[0] = 1;
+        poly[15].v[0] = (L)(0 - 1);
+      } else {
+        poly[0].v[0] = 1;
+        int deg = 0;
+        for (int j = 0; j < 16; j++) {
+          if (j == i) continue;
+          // poly *= (t - t_j)
+          std::vector<G4<T>> np(17, G4<T>{{0, 0, 0, 0}});
+          for (int d = 0; d <= deg; d++) {
+            for (int c = 0; c < 4; c++) np[d + 1].v[c] = (L)(np[d + 1].v[c] + poly[d].v[c]);
+            G4<T> p = G4<T>::mul(poly[d], t1[j]);
+            for (int c = 0; c < 4; c++) np[d].v[c] = (L)(np[d].v[c] - p.v[c]);
+          }
+          poly = np;
+          deg++;
+        }
+        for (auto& g : poly)
+          for (int c = 0; c < 4; c++) g.v[c] = (L)(g.v[c] * inv15);
+      }
+      Gr<T> e = Gr<T>::zero();
+      for (int d = 0; d < 16; d++)
+        for (int c = 0; c < 4; c++) e.v[4 * d + c] = (L)poly[d].v[c];
+      lbeta[i] = e;
+    }
+    G4<T> la0{{1, (C)(L)(0 - 1), 0, 0}}, la1{{0, 1, 0, 0}};
+    for (int i = 0; i < 16; i++)
+      for (int h = 0; h < 2; h++) {
+        Gr<T> b = Gr<T>::zero();
+        for (int d = 0; d < 32; d++) {
+          G4<T> s{{lbeta[i].v[4 * d], lbeta[i].v[4 * d + 1], lbeta[i].v[4 * d + 2], lbeta[i].v[4 * d + 3]}};
+          G4<T> p = G4<T>::mul(s, h ? la1 : la0);
+          for (int c = 0; c < 4; c++) b.v[4 * d + c] = (L)p.v[c];
+        }
+        basis[2 * i + h] = b;
+      }
+    psi_pw.resize(16 * 32 * 4);
+    for (int i = 0; i < 16; i++) {
+      G4<T> p{{1, 0, 0, 0}};
+      for (int k = 0; k < 32; k++) {
+        for (int c = 0; c < 4; c++) psi_pw[(i * 32 + k) * 4 + c] = (L)p.v[c];
+        p = G4<T>::mul(p, t1[i]);
+      }
+    }
+  }
+  static const Consts& get() {
+    static Consts c;
+    return c;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// PCS parameters (pcs.rs:45-293; r2_pcs.rs:41-62)
+// ---------------------------------------------------------------------------
+inline size_t recommended_query_size(double delta, size_t lambda) {
+  double lam = (double)lambda, bt = 1.0 - delta / 3.0, bo = 1.0 - 2.0 * delta / 3.0;
+  size_t t = (size_t)std::ceil(lam / (-std::log2(bt)));
+  double target = std::exp2(-lam);
+  for (;;) {
+    double pt = 1.0, po = 1.0;  // f64::powi: repeated products
+    for (size_t i = 0; i < t; i++) { pt *= bt; po *= bo; }
+    if (pt + 2.0 * po > target) t++; else break;
+  }
+  return t;
+}
+inline size_t proof_size_bytes_for(size_t k1, size_t k2, size_t n, size_t t, size_t mb, size_t cb) {
+  size_t np = 1;
+  while (np < n) np <<= 1;
+  size_t pl = (size_t)__builtin_ctzll(np);
+  return 32 + k2 * mb + t * (8 + k1 * cb) + t * (8 + pl * 32) + t * 4;
+}
+inline bool best_shape(size_t nv, size_t q, size_t t, size_t mb, size_t cb, bool floor_t, size_t& k1, size_t& k2) {
+  bool found = false;
+  size_t best = 0;
+  for (size_t l = 1; l < nv; l++) {
+    size_t a = size_t(1) << l, b = size_t(1) << (nv - l);
+    if (floor_t && q * b < t) continue;
+    size_t s = proof_size_bytes_for(a, b, q * b, t, mb, cb);
+    if (!found || s < best) { best = s; k1 = a; k2 = b; found = true; }
+  }
+  return found;
+}
+
+struct PcsParams {
+  size_t k1 = 0, k2 = 0, q = 4, n = 0, ell = 0;
+  bool packed = true;
+  std::vector<uint32_t> perm1, perm2;
+  std::vector<uint64_t> d1, d2;
+
+  // RaaParams::setup (raa.rs:44-64): two Fisher-Yates permutations then 2n odd units
+  void raa_setup(uint64_t seed, int K) {
+    n = k2 * q;
+    StdRng r = StdRng::seed_from_u64(seed);
+    for (auto* p : {&perm1, &perm2}) {
+      p->resize(n);
+      for (size_t i = 0; i < n; i++) (*p)[i] = (uint32_t)i;
+      for (size_t i = n - 1; i >= 1; i--) {
+        size_t j = (size_t)r.gen_range_incl(0, i);
+        std::swap((*p)[i], (*p)[j]);
+      }
+    }
+    uint64_t mask = K == 64 ? ~0ull : ((1ull << K) - 1);
+    d1.resize(n);
+    d2.resize(n);
+    for (size_t i = 0; i < n; i++) d1[i] = (r.next_u64() | 1) & mask;
+    for (size_t i = 0; i < n; i++) d2[i] = (r.next_u64() | 1) & mask;
+  }
+  static PcsParams make(size_t k1, size_t k2, size_t q, size_t ell, uint64_t seed, bool packed, int K) {
+    PcsParams p;
+    p.k1 = k1; p.k2 = k2; p.q = q; p.ell = ell; p.packed = packed;
+    p.raa_setup(seed, K);
+    return p;
+  }
+  // Pcs::setup / setup_batched / setup_stacked and R2Pcs::setup
+  static PcsParams setup(size_t nv, uint64_t seed, int K) {
+    size_t lb = K / 8, t = recommended_query_size(0.19, 100), k1 = 0, k2 = 0;
+    best_shape(nv, 4, t, 128 * lb, 32 * lb, false, k1, k2);
+    return make(k1, k2, 4, t, seed, true, K);
+  }
+  static PcsParams setup_batched(size_t nv, uint64_t seed, size_t nopen, int K) {
+    size_t lb = K / 8, t = recommended_query_size(0.19, 100), k1 = 0, k2 = 0;
+    if (!best_shape(nv, 4, t, nopen * 128 * lb, 32 * lb, true, k1, k2)) best_shape(nv, 4, t, 128 * lb, 32 * lb, false, k1, k2);
+    return make(k1, k2, 4, t, seed, true, K);
+  }
+  static PcsParams setup_stacked(size_t nv, uint64_t seed, size_t stack, int K) {
+    size_t lb = K / 8, t = recommended_query_size(0.19, 100), k1 = 0, k2 = 0;
+    if (!best_shape(nv, 4, t, 128 * lb, stack * 32 * lb, true, k1, k2))
+      best_shape(nv, 4, t, 128 * lb, stack * 32 * lb, false, k1, k2);
+    return make(k1, k2, 4, t, seed, true, K);
+  }
+  static PcsParams setup_r2(size_t nv, uint64_t seed, int K) {
+    size_t lb = K / 8, t = recommended_query_size(0.19, 100), k1 = 0, k2 = 0;
+    best_shape(nv, 4, t, 128 * lb, 128 * lb, false, k1, k2);
+    return make(k1, k2, 4, t, seed, false, K);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Fiat-Shamir transcript (snark-ring/src/transcript.rs:23-105)
+// ---------------------------------------------------------------------------
+struct Transcript {
+  b3::Hasher h;
+  explicit Transcript(const char* label) {
+    h.update("snark-ring/v0", 13);
+    u32(std::strlen(label));
+    h.update(label, std::strlen(label));
+  }
+  void u32(uint32_t x) {
+    uint8_t b[4] = {(uint8_t)x, (uint8_t)(x >> 8), (uint8_t)(x >> 16), (uint8_t)(x >> 24)};
+    h.update(b, 4);
+  }
+  void label(const char* l) {
+    h.update("L", 1);
+    u32(std::strlen(l));
+    h.update(l, std::strlen(l));
+  }
+  void root(const uint8_t r[32]) {
+    h.update("R", 1);
+    h.update(r, 32);
+  }
+  template <class T> void elems(const char* l, const Gr<T>* xs, size_t n) {
+    label(l);
+    u32((uint32_t)n);
+    for (size_t i = 0; i < n; i++) {
+      h.update("E", 1);
+      h.update(xs[i].v, sizeof xs[i].v);
+    }
+  }
+  // challenge_sampler: squeeze 8 bytes of XOF from a clone, absorb "S"||bytes
+  StdRng sampler(const char* l) {
+    label(l);
+    uint8_t b[8];
+    h.finalize(0, b, 8);
+    uint64_t seed = 0;
+    for (int i = 0; i < 8; i++) seed |= (uint64_t)b[i] << (8 * i);
+    h.update("S", 1);
+    h.update(b, 8);
+    return StdRng::seed_from_u64(seed);
+  }
+};
+
+// XOF query indices (pcs.rs:1130-1156; r2_pcs.rs:220-246): BLAKE3(domain || root
+// || m-bar limbs), 4-byte LE words masked to next_pow2(n)-1, kept if < n.
+inline std::vector<uint32_t> derive_queries(const char* domain, const uint8_t root[32], const void* mbar_bytes,
+                                            size_t nbytes, size_t n, size_t t) {
+  b3::Hasher h;
+  h.update(domain, std::strlen(domain));
+  h.update(root, 32);
+  h.update(mbar_bytes, nbytes);
+  size_t np = 1;
+  while (np < n) np <<= 1;
+  uint32_t mask = (uint32_t)(np - 1);
+  std::vector<uint32_t> out;
+  out.reserve(t);
+  uint64_t pos = 0;
+  while (out.size() < t) {
+    uint8_t buf[256];
+    h.finalize(pos, buf, sizeof buf);
+    pos += sizeof buf;
+    for (size_t o = 0; o + 4 <= sizeof buf && out.size() < t; o += 4) {
+      uint32_t v = ((uint32_t)buf[o] | ((uint32_t)buf[o + 1] << 8) | ((uint32_t)buf[o + 2] << 16) |
+                    ((uint32_t)buf[o + 3] << 24)) & mask;
+      if (v < (uint32_t)n) out.push_back(v);
+    }
+  }
+  return out;
+}
+
+}  // namespace nenya

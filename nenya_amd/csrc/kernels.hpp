@@ -1,0 +1,780 @@
+// gfx950 kernels for the Nenya prover, templated on the limb width.
+//
+// Core primitive: a 32-lane group computes GR(2^k,128) products.  Lane t owns
+// R1 coefficient t (4 limbs) of the result.  For a = sum a_i Y^i and
+// b = sum b_j Y^j it accumulates the raw alpha-convolutions of
+//   circ_t = sum_i a_i b_{(t-i) mod 32}      (all 32 terms, uniform per lane)
+//   hi_t   = sum_{i>t} a_i b_{t+32-i}        (the wrap-around part = Y^(32+t))
+// Sums of products (round polynomials, matvecs, m-bar) accumulate these raw
+// values and reduce once: alpha-reduce (X^4 = -X - 1), lo = circ - hi, then
+// Y^32 = -Y^3 - Y - alpha with the cascade of positions 61, 62 into 32, 33.
+// The result equals the reference's schoolbook + top-down reduction
+// (galois-ring/src/gr128.rs:215-296) because both compute the unique
+// representative of a*b mod f2.
+//
+// Operands are staged in LDS by the group; every kernel uses 64-thread
+// blocks (one wave, two groups) so __syncthreads is a single-wave barrier.
+#pragma once
+#include "blake3.hpp"
+#include "common.hpp"
+
+namespace nenya {
+
+constexpr int GB = 64;  // threads per block for group kernels
+constexpr int GPB = GB / 32;
+
+template <class T> using LT = typename T::L;
+template <class T> using CT = typename T::C;
+
+template <class T> __device__ __forceinline__ void ld4(const LT<T>* p, CT<T> o[4]) {
+  using L = LT<T>;
+  if constexpr (sizeof(L) == 4) {
+    uint4 v = *reinterpret_cast<const uint4*>(p);
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+  } else if constexpr (sizeof(L) == 8) {
+    const ulonglong2* q = reinterpret_cast<const ulonglong2*>(p);
+    ulonglong2 a = q[0], b = q[1];
+    o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+  } else if constexpr (sizeof(L) == 2) {
+    uint2 v = *reinterpret_cast<const uint2*>(p);
+    o[0] = v.x & 0xffffu; o[1] = v.x >> 16; o[2] = v.y & 0xffffu; o[3] = v.y >> 16;
+  } else {
+    uint32_t v = *reinterpret_cast<const uint32_t*>(p);
+    o[0] = v & 0xff; o[1] = (v >> 8) & 0xff; o[2] = (v >> 16) & 0xff; o[3] = v >> 24;
+  }
+}
+template <class T> __device__ __forceinline__ void st4(LT<T>* p, const CT<T> v[4]) {
+  using L = LT<T>;
+  if constexpr (sizeof(L) == 4) {
+    *reinterpret_cast<uint4*>(p) = make_uint4(v[0], v[1], v[2], v[3]);
+  } else if constexpr (sizeof(L) == 8) {
+    ulonglong2* q = reinterpret_cast<ulonglong2*>(p);
+    q[0] = make_ulonglong2(v[0], v[1]);
+    q[1] = make_ulonglong2(v[2], v[3]);
+  } else if constexpr (sizeof(L) == 2) {
+    *reinterpret_cast<uint2*>(p) = make_uint2((v[0] & 0xffffu) | (v[1] << 16), (v[2] & 0xffffu) | (v[3] << 16));
+  } else {
+    *reinterpret_cast<uint32_t*>(p) = (v[0] & 0xff) | ((v[1] & 0xff) << 8) | ((v[2] & 0xff) << 16) | (v[3] << 24);
+  }
+}
+
+template <class C> __device__ __forceinline__ void d_raw(const C a[4], const C b[4], C p[7]) {
+  p[0] = a[0] * b[0];
+  p[1] = a[0] * b[1] + a[1] * b[0];
+  p[2] = a[0] * b[2] + a[1] * b[1] + a[2] * b[0];
+  p[3] = a[0] * b[3] + a[1] * b[2] + a[2] * b[1] + a[3] * b[0];
+  p[4] = a[1] * b[3] + a[2] * b[2] + a[3] * b[1];
+  p[5] = a[2] * b[3] + a[3] * b[2];
+  p[6] = a[3] * b[3];
+}
+template <class C> __device__ __forceinline__ void d_red(const C p[7], C o[4]) {
+  o[0] = p[0] - p[4];
+  o[1] = p[1] - p[4] - p[5];
+  o[2] = p[2] - p[5] - p[6];
+  o[3] = p[3] - p[6];
+}
+
+template <class T> struct Acc {
+  CT<T> c[7];
+  CT<T> h[7];
+  __device__ __forceinline__ void clear() {
+#pragma unroll
+    for (int i = 0; i < 7; i++) c[i] = h[i] = 0;
+  }
+};
+
+// acc += A * B for the group; A has beta-degree < NA (16 for packed images).
+template <class T, int NA = 32>
+__device__ __forceinline__ void acc_mul(Acc<T>& acc, const LT<T>* A, const LT<T>* B, int t) {
+  using C = CT<T>;
+#pragma unroll 4
+  for (int i = 0; i < NA; i++) {
+    C a[4], b[4], p[7];
+    ld4<T>(A + 4 * i, a);
+    ld4<T>(B + 4 * ((t - i) & 31), b);
+    d_raw(a, b, p);
+    const C m = (i > t) ? (C)~(C)0 : (C)0;
+#pragma unroll
+    for (int c = 0; c < 7; c++) {
+      acc.c[c] += p[c];
+      acc.h[c] += p[c] & m;
+    }
+  }
+}
+
+// Reduce the group's accumulated raw sums; lane t gets coefficient t.
+template <class T> __device__ __forceinline__ void acc_final(const Acc<T>& acc, int t, CT<T> out[4]) {
+  using C = CT<T>;
+  C cl[4], H[4];
+  d_red(acc.c, cl);
+  d_red(acc.h, H);
+  C h[4], hm1[4], hm3[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    C x = __shfl(H[c], (t < 2) ? t + 29 : t, 32);
+    h[c] = (t < 2) ? H[c] - x : H[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    C a = __shfl(h[c], (t >= 1) ? t - 1 : 0, 32);
+    C b = __shfl(h[c], (t >= 3) ? t - 3 : 0, 32);
+    hm1[c] = (t >= 1) ? a : (C)0;
+    hm3[c] = (t >= 3) ? b : (C)0;
+  }
+  C lo[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) lo[c] = cl[c] - H[c] - hm1[c] - hm3[c];
+  out[0] = lo[0] + h[3];
+  out[1] = lo[1] - (h[0] - h[3]);
+  out[2] = lo[2] - h[1];
+  out[3] = lo[3] - h[2];
+}
+
+// Stage one element (128 limbs) into group LDS: lane t copies coefficient t.
+template <class T> __device__ __forceinline__ void stage(LT<T>* dst, const LT<T>* src, int t) {
+  CT<T> v[4];
+  ld4<T>(src + 4 * t, v);
+  st4<T>(dst + 4 * t, v);
+}
+
+// ============================================================================
+// Element-wise group kernels
+// ============================================================================
+
+// out[i] = A[i]*B[i] - (S ? S[i] : 0)   (S may alias out)
+template <class T, int NA>
+__global__ void __launch_bounds__(GB) k_mul_sub(const LT<T>* A, const LT<T>* B, const LT<T>* S, LT<T>* out, uint64_t n) {
+  __shared__ __attribute__((aligned(16))) LT<T> sh[GPB][2][E];
+  int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < n; base += (uint64_t)gridDim.x * GPB) {
+    uint64_t i = base + g;
+    bool ok = i < n;
+    if (ok) { stage<T>(sh[g][0], A + i * E, t); stage<T>(sh[g][1], B + i * E, t); }
+    __syncthreads();
+    Acc<T> acc;
+    acc.clear();
+    if (ok) acc_mul<T, NA>(acc, sh[g][0], sh[g][1], t);
+    CT<T> o[4];
+    acc_final<T>(acc, t, o);
+    if (ok) {
+      if (S) {
+        CT<T> s[4];
+        ld4<T>(S + i * E + 4 * t, s);
+        for (int c = 0; c < 4; c++) o[c] -= s[c];
+      }
+      st4<T>(out + i * E + 4 * t, o);
+    }
+    __syncthreads();
+  }
+}
+
+// out[i] = s * X[i]  (s one shared element)
+template <class T>
+__global__ void __launch_bounds__(GB) k_scale(const LT<T>* s, const LT<T>* X, LT<T>* out, uint64_t n) {
+  __shared__ __attribute__((aligned(16))) LT<T> ss[E];
+  __shared__ __attribute__((aligned(16))) LT<T> sh[GPB][E];
+  int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  if (threadIdx.x < 32) stage<T>(ss, s, t);
+  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < n; base += (uint64_t)gridDim.x * GPB) {
+    uint64_t i = base + g;
+    bool ok = i < n;
+    if (ok) stage<T>(sh[g], X + i * E, t);
+    __syncthreads();
+    Acc<T> acc;
+    acc.clear();
+    if (ok) acc_mul<T>(acc, ss, sh[g], t);
+    CT<T> o[4];
+    acc_final<T>(acc, t, o);
+    if (ok) st4<T>(out + i * E + 4 * t, o);
+    __syncthreads();
+  }
+}
+
+// Mle::fold_in_place (mle.rs:42-54): dst[i] = lo[i] + r*(hi[i] - lo[i]),
+// lo = src[i], hi = src[i+m].  dst may equal src (each i reads then writes i).
+template <class T>
+__global__ void __launch_bounds__(GB) k_fold(const LT<T>* r, const LT<T>* src, LT<T>* dst, uint64_t m, uint64_t ntab,
+                                             uint64_t src_stride, uint64_t dst_stride) {
+  __shared__ __attribute__((aligned(16))) LT<T> sr[E];
+  __shared__ __attribute__((aligned(16))) LT<T> sh[GPB][E];
+  int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  if (threadIdx.x < 32) stage<T>(sr, r, t);
+  uint64_t total = m * ntab;
+  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < total; base += (uint64_t)gridDim.x * GPB) {
+    uint64_t w = base + g;
+    bool ok = w < total;
+    uint64_t tab = ok ? w / m : 0, i = ok ? w % m : 0;
+    const LT<T>* s = src + tab * src_stride * E;
+    CT<T> lo[4], hi[4];
+    if (ok) {
+      ld4<T>(s + i * E + 4 * t, lo);
+      ld4<T>(s + (i + m) * E + 4 * t, hi);
+      CT<T> d[4] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], hi[3] - lo[3]};
+      st4<T>(sh[g] + 4 * t, d);
+    }
+    __syncthreads();
+    Acc<T> acc;
+    acc.clear();
+    if (ok) acc_mul<T>(acc, sr, sh[g], t);
+    CT<T> o[4];
+    acc_final<T>(acc, t, o);
+    if (ok) {
+      for (int c = 0; c < 4; c++) o[c] += lo[c];
+      st4<T>(dst + tab * dst_stride * E + i * E + 4 * t, o);
+    }
+    __syncthreads();
+  }
+}
+
+// marginalize_and_scale (mult_check.rs:302-313): eg[i] = scale*(eg[i]+eg[i+m])
+template <class T>
+__global__ void __launch_bounds__(GB) k_marg(const LT<T>* scale, LT<T>* eg, uint64_t m) {
+  __shared__ __attribute__((aligned(16))) LT<T> ss[E];
+  __shared__ __attribute__((aligned(16))) LT<T> sh[GPB][E];
+  int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  if (threadIdx.x < 32) stage<T>(ss, scale, t);
+  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < m; base += (uint64_t)gridDim.x * GPB) {
+    uint64_t i = base + g;
+    bool ok = i < m;
+    if (ok) {
+      CT<T> a[4], b[4];
+      ld4<T>(eg + i * E + 4 * t, a);
+      ld4<T>(eg + (i + m) * E + 4 * t, b);
+      for (int c = 0; c < 4; c++) a[c] += b[c];
+      st4<T>(sh[g] + 4 * t, a);
+    }
+    __syncthreads();
+    Acc<T> acc;
+    acc.clear();
+    if (ok) acc_mul<T>(acc, ss, sh[g], t);
+    CT<T> o[4];
+    acc_final<T>(acc, t, o);
+    if (ok) st4<T>(eg + i * E + 4 * t, o);
+    __syncthreads();
+  }
+}
+
+// One doubling level of build_eq_table (eq.rs:35-58): from prev (len) to
+// next (2 len): next[2k+1] = prev[k]*r, next[2k] = prev[k] - next[2k+1].
+template <class T>
+__global__ void __launch_bounds__(GB) k_eq_level(const LT<T>* r, const LT<T>* prev, LT<T>* next, uint64_t len) {
+  __shared__ __attribute__((aligned(16))) LT<T> sr[E];
+  __shared__ __attribute__((aligned(16))) LT<T> sh[GPB][E];
+  int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  if (threadIdx.x < 32) stage<T>(sr, r, t);
+  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < len; base += (uint64_t)gridDim.x * GPB) {
+    uint64_t k = base + g;
+    bool ok = k < len;
+    CT<T> p[4];
+    if (ok) {
+      ld4<T>(prev + k * E + 4 * t, p);
+      st4<T>(sh[g] + 4 * t, p);
+    }
+    __syncthreads();
+    Acc<T> acc;
+    acc.clear();
+    if (ok) acc_mul<T>(acc, sr, sh[g], t);
+    CT<T> o[4];
+    acc_final<T>(acc, t, o);
+    if (ok) {
+      CT<T> lo[4] = {p[0] - o[0], p[1] - o[1], p[2] - o[2], p[3] - o[3]};
+      st4<T>(next + (2 * k + 1) * E + 4 * t, o);
+      st4<T>(next + (2 * k) * E + 4 * t, lo);
+    }
+    __syncthreads();
+  }
+}
+
+// ============================================================================
+// Sumcheck round kernels: per-block raw partial sums -> reduced partials.
+// ============================================================================
+
+// Write the block's reduced sum of nsum accumulators.  Groups' raw sums are
+// added through LDS (linear), then group 0 reduces.
+template <class T, int NS>
+__device__ __forceinline__ void block_emit(Acc<T> (&acc)[NS], LT<T>* partials, int t, int g) {
+  __shared__ CT<T> red[GPB][NS][14][32];
+  for (int s = 0; s < NS; s++)
+    for (int c = 0; c < 7; c++) {
+      red[g][s][c][t] = acc[s].c[c];
+      red[g][s][7 + c][t] = acc[s].h[c];
+    }
+  __syncthreads();
+  if (g == 0) {
+    for (int s = 0; s < NS; s++) {
+      Acc<T> a = acc[s];
+      for (int gg = 1; gg < GPB; gg++)
+        for (int c = 0; c < 7; c++) { a.c[c] += red[gg][s][c][t]; a.h[c] += red[gg][s][7 + c][t]; }
+      CT<T> o[4];
+      acc_final<T>(a, t, o);
+      st4<T>(partials + ((uint64_t)blockIdx.x * NS + s) * E + 4 * t, o);
+    }
+  }
+}
+
+// Gruen mult-check round (mult_check.rs:250-296, sub form 426-461):
+//   s0 += eg*(a_lo b_lo - c_lo - e_lo), s1 += eg*(a_hi b_hi - c_hi - e_hi),
+//   sinf += eg*(da db).  c, e may be null (plain form).
+template <class T>
+__global__ void __launch_bounds__(GB) k_mult_round(const LT<T>* eg, const LT<T>* a, const LT<T>* b, const LT<T>* cc,
+                                                   const LT<T>* ee, uint64_t m, LT<T>* partials) {
+  __shared__ __attribute__((aligned(16))) LT<T> sh[GPB][3][E];
+  int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  Acc<T> acc[3];
+  for (int s = 0; s < 3; s++) acc[s].clear();
+  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < m; base += (uint64_t)gridDim.x * GPB) {
+    uint64_t i = base + g;
+    bool ok = i < m;
+    CT<T> alo[4] = {0, 0, 0, 0}, ahi[4] = {0, 0, 0, 0}, blo[4] = {0, 0, 0, 0}, bhi[4] = {0, 0, 0, 0};
+    if (ok) {
+      ld4<T>(a + i * E + 4 * t, alo);
+      ld4<T>(a + (i + m) * E + 4 * t, ahi);
+      ld4<T>(b + i * E + 4 * t, blo);
+      ld4<T>(b + (i + m) * E + 4 * t, bhi);
+      stage<T>(sh[g][2], eg + i * E, t);
+    }
+    // three products of the varying tables, each then weighted by eg
+    for (int s = 0; s < 3; s++) {
+      CT<T> x[4], y[4];
+      for (int c = 0; c < 4; c++) {
+        x[c] = s == 0 ? alo[c] : s == 1 ? ahi[c] : ahi[c] - alo[c];
+        y[c] = s == 0 ? blo[c] : s == 1 ? bhi[c] : bhi[c] - blo[c];
+      }
+      st4<T>(sh[g][0] + 4 * t, x);
+      st4<T>(sh[g][1] + 4 * t, y);
+      __syncthreads();
+      Acc<T> pa;
+      pa.clear();
+      if (ok) acc_mul<T>(pa, sh[g][0], sh[g][1], t);
+      CT<T> p[4];
+      acc_final<T>(pa, t, p);
+      if (ok && s < 2 && cc) {
+        CT<T> c4[4], e4[4];
+        uint64_t off = (s == 0 ? i : i + m) * E + 4 * t;
+        ld4<T>(cc + off, c4);
+        ld4<T>(ee + off, e4);
+        for (int c = 0; c < 4; c++) p[c] -= c4[c] + e4[c];
+      }
+      __syncthreads();
+      st4<T>(sh[g][0] + 4 * t, p);
+      __syncthreads();
+      if (ok) acc_mul<T>(acc[s], sh[g][2], sh[g][0], t);
+      __syncthreads();
+    }
+  }
+  block_emit<T, 3>(acc, partials, t, g);
+}
+
+// Linear-check round over `ntab` table pairs (linear_check_simd.rs:264-286,
+// linear_check_general.rs:289-334): s0 += f_lo g_lo, sinf += df dg.
+// f, g: ntab tables of stride `stride` elements, current length 2m.
+template <class T, int NAF>
+__global__ void __launch_bounds__(GB) k_lin_round(const LT<T>* f, uint64_t fstride, const LT<T>* gt, uint64_t gstride,
+                                                  uint64_t ntab, uint64_t m, LT<T>* partials) {
+  __shared__ __attribute__((aligned(16))) LT<T> sh[GPB][2][E];
+  int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  Acc<T> acc[2];
+  acc[0].clear();
+  acc[1].clear();
+  uint64_t total = m * ntab;
+  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < total; base += (uint64_t)gridDim.x * GPB) {
+    uint64_t w = base + g;
+    bool ok = w < total;
+    uint64_t tab = ok ? w / m : 0, i = ok ? w % m : 0;
+    const LT<T>* F = f + tab * fstride * E;
+    const LT<T>* G = gt + tab * gstride * E;
+    CT<T> flo[4] = {0, 0, 0, 0}, fhi[4] = {0, 0, 0, 0}, glo[4] = {0, 0, 0, 0}, ghi[4] = {0, 0, 0, 0};
+    if (ok) {
+      ld4<T>(F + i * E + 4 * t, flo);
+      ld4<T>(F + (i + m) * E + 4 * t, fhi);
+      ld4<T>(G + i * E + 4 * t, glo);
+      ld4<T>(G + (i + m) * E + 4 * t, ghi);
+    }
+    st4<T>(sh[g][0] + 4 * t, flo);
+    st4<T>(sh[g][1] + 4 * t, glo);
+    __syncthreads();
+    if (ok) acc_mul<T, NAF>(acc[0], sh[g][0], sh[g][1], t);
+    __syncthreads();
+    for (int c = 0; c < 4; c++) { fhi[c] -= flo[c]; ghi[c] -= glo[c]; }
+    st4<T>(sh[g][0] + 4 * t, fhi);
+    st4<T>(sh[g][1] + 4 * t, ghi);
+    __syncthreads();
+    if (ok) acc_mul<T, NAF>(acc[1], sh[g][0], sh[g][1], t);
+    __syncthreads();
+  }
+  block_emit<T, 2>(acc, partials, t, g);
+}
+
+// partial sums of sum_i A[i]*B[i] (A low-degree if NA = 16)
+template <class T, int NA>
+__global__ void __launch_bounds__(GB) k_dot(const LT<T>* A, const LT<T>* B, uint64_t n, LT<T>* partials) {
+  __shared__ __attribute__((aligned(16))) LT<T> sh[GPB][2][E];
+  int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  Acc<T> acc[1];
+  acc[0].clear();
+  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < n; base += (uint64_t)gridDim.x * GPB) {
+    uint64_t i = base + g;
+    bool ok = i < n;
+    if (ok) {
+      stage<T>(sh[g][0], A + i * E, t);
+      stage<T>(sh[g][1], B + i * E, t);
+    }
+    __syncthreads();
+    if (ok) acc_mul<T, NA>(acc[0], sh[g][0], sh[g][1], t);
+    __syncthreads();
+  }
+  block_emit<T, 1>(acc, partials, t, g);
+}
+
+// out[s] = sum_b partials[b][s]   (thread per limb)
+template <class T>
+__global__ void k_sum_partials(const LT<T>* partials, uint64_t nblocks, int ns, LT<T>* out) {
+  int s = blockIdx.x, l = threadIdx.x;
+  if (l >= E) return;
+  CT<T> acc = 0;
+  for (uint64_t b = 0; b < nblocks; b++) acc += partials[(b * ns + s) * E + l];
+  out[s * E + l] = (LT<T>)acc;
+}
+
+// ============================================================================
+// Teichmuller lift: x <- x^(2^128) (one Frobenius pass; exact lift for k<=64,
+// teichmuller.rs:104-114 iterates the same fixed point).  One 256-thread
+// block per element: 8 groups split the 32 beta terms of each squaring.
+// ============================================================================
+template <class T>
+__global__ void __launch_bounds__(256) k_lift(LT<T>* x, int squarings) {
+  __shared__ __attribute__((aligned(16))) LT<T> sx[E];
+  __shared__ CT<T> red[8][14][32];
+  int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  LT<T>* el = x + (uint64_t)blockIdx.x * E;
+  if (g == 0) stage<T>(sx, el, t);
+  __syncthreads();
+  for (int it = 0; it < squarings; it++) {
+    using C = CT<T>;
+    Acc<T> acc;
+    acc.clear();
+    for (int ii = 0; ii < 4; ii++) {
+      int i = 4 * g + ii;
+      C a[4], b[4], p[7];
+      ld4<T>(sx + 4 * i, a);
+      ld4<T>(sx + 4 * ((t - i) & 31), b);
+      d_raw(a, b, p);
+      const C m = (i > t) ? (C)~(C)0 : (C)0;
+      for (int c = 0; c < 7; c++) { acc.c[c] += p[c]; acc.h[c] += p[c] & m; }
+    }
+    for (int c = 0; c < 7; c++) { red[g][c][t] = acc.c[c]; red[g][7 + c][t] = acc.h[c]; }
+    __syncthreads();
+    if (g == 0) {
+      for (int gg = 1; gg < 8; gg++)
+        for (int c = 0; c < 7; c++) { acc.c[c] += red[gg][c][t]; acc.h[c] += red[gg][7 + c][t]; }
+      C o[4];
+      acc_final<T>(acc, t, o);
+      st4<T>(sx + 4 * t, o);
+    }
+    __syncthreads();
+  }
+  if (g == 0) stage<T>(el, sx, t);
+}
+
+// ============================================================================
+// Packing (phi), unpacking (psi) and matvecs
+// ============================================================================
+
+// out[rot*n + e] = pack_r2(rot_rot(slots[e]))  for rot in [0,nrot)
+// (embedding.rs:63-65 as the R0-linear map with basis pack_r2(e_s), pcs.rs:49-55;
+//  rot_l from reduce.rs:352-358).  basis: 32 x 128 limbs.
+template <class T>
+__global__ void __launch_bounds__(GB) k_pack(const LT<T>* slots, const LT<T>* basis, LT<T>* out, uint64_t n, int nrot,
+                                             int rot0) {
+  __shared__ __attribute__((aligned(16))) LT<T> sb[32 * E];
+  __shared__ CT<T> ssl[GPB][32];
+  int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  for (int i = threadIdx.x; i < 32 * E / 4; i += GB) {
+    CT<T> v[4];
+    ld4<T>(basis + 4 * i, v);
+    st4<T>(sb + 4 * i, v);
+  }
+  uint64_t total = n * (uint64_t)nrot;
+  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < total; base += (uint64_t)gridDim.x * GPB) {
+    uint64_t w = base + g;
+    bool ok = w < total;
+    uint64_t rot = ok ? w / n : 0, e = ok ? w % n : 0;
+    __syncthreads();
+    if (ok) ssl[g][t] = slots[e * 32 + ((t + rot + rot0) & 31)];
+    __syncthreads();
+    if (ok) {
+      CT<T> acc[4] = {0, 0, 0, 0};
+      for (int s = 0; s < 32; s++) {
+        CT<T> bv[4];
+        ld4<T>(sb + s * E + 4 * t, bv);
+        CT<T> x = ssl[g][s];
+        for (int c = 0; c < 4; c++) acc[c] += x * bv[c];
+      }
+      st4<T>(out + w * E + 4 * t, acc);
+    }
+  }
+}
+
+// psi (embedding.rs:247-261, flattened as general.rs:192-206):
+// slots[2i] = g_i(0), slots[2i+1] = g_i(1), g_i = sum_k x_k t_i^k in R1.
+// pw: powers t_i^k as [16][32][4] limbs.  Lanes t and t+16 split the 32 k's.
+template <class T>
+__global__ void __launch_bounds__(GB) k_psi(const LT<T>* x, const LT<T>* pw, LT<T>* slots, uint64_t n) {
+  __shared__ __attribute__((aligned(16))) LT<T> sp[16 * 32 * 4];
+  __shared__ __attribute__((aligned(16))) LT<T> sx[GPB][E];
+  int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  for (int i = threadIdx.x; i < 16 * 32; i += GB) {
+    CT<T> v[4];
+    ld4<T>(pw + 4 * i, v);
+    st4<T>(sp + 4 * i, v);
+  }
+  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < n; base += (uint64_t)gridDim.x * GPB) {
+    uint64_t e = base + g;
+    bool ok = e < n;
+    __syncthreads();
+    if (ok) stage<T>(sx[g], x + e * E, t);
+    __syncthreads();
+    int pt = t & 15, half = t >> 4;
+    CT<T> raw[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int kk = 0; kk < 16; kk++) {
+      int k = 16 * half + kk;
+      CT<T> a[4], b[4], p[7];
+      ld4<T>(sx[g] + 4 * k, a);
+      ld4<T>(sp + (pt * 32 + k) * 4, b);
+      d_raw(a, b, p);
+      for (int c = 0; c < 7; c++) raw[c] += p[c];
+    }
+    for (int c = 0; c < 7; c++) raw[c] += __shfl_xor(raw[c], 16, 32);
+    CT<T> gg[4];
+    d_red(raw, gg);
+    if (ok && half == 0) {
+      slots[e * 32 + 2 * pt] = (LT<T>)gg[0];
+      slots[e * 32 + 2 * pt + 1] = (LT<T>)(gg[0] + gg[1] + gg[2] + gg[3]);
+    }
+  }
+}
+
+// R0 CSR matvec over nvec vectors: out[r*ostride + v*ovstride] = sum A[r,c] w[v*wstride + c]
+template <class T>
+__global__ void k_r0_matvec(const uint32_t* ptr, const uint32_t* col, const LT<T>* val, const LT<T>* w, uint64_t wstride,
+                            LT<T>* out, uint64_t nrows, int nvec, uint64_t ostride, uint64_t ovstride) {
+  uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nrows * (uint64_t)nvec) return;
+  uint64_t r = id / nvec;
+  int v = (int)(id % nvec);
+  CT<T> acc = 0;
+  for (uint32_t k = ptr[r]; k < ptr[r + 1]; k++) acc += (CT<T>)val[k] * (CT<T>)w[v * wstride + col[k]];
+  out[r * ostride + v * ovstride] = (LT<T>)acc;
+}
+
+// out[o] (+)= sum_{k in ptr[o]..ptr[o+1]} val[k] * X[idx[k]]   (R0 scalar x R2,
+// the mul_by_gr1 matvecs of matrix.rs:93-105 and simd.rs:570-580)
+template <class T>
+__global__ void __launch_bounds__(GB) k_spmv_gr1(const uint32_t* ptr, const uint32_t* idx, const LT<T>* val,
+                                                 const LT<T>* X, LT<T>* out, uint64_t nout, int accumulate) {
+  int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  for (uint64_t o = (uint64_t)blockIdx.x * GPB + g; o < nout; o += (uint64_t)gridDim.x * GPB) {
+    CT<T> acc[4] = {0, 0, 0, 0};
+    if (accumulate) ld4<T>(out + o * E + 4 * t, acc);
+    for (uint32_t k = ptr[o]; k < ptr[o + 1]; k++) {
+      CT<T> x[4];
+      ld4<T>(X + (uint64_t)idx[k] * E + 4 * t, x);
+      CT<T> v = val[k];
+      for (int c = 0; c < 4; c++) acc[c] += v * x[c];
+    }
+    st4<T>(out + o * E + 4 * t, acc);
+  }
+}
+
+// Sparse R2 x R2 accumulation: out[o] = sum_{k in ptr[o]..} D[k] * X[xsel[k]][xidx[k]]
+// with D low-degree (packed diagonal cells).  Used for layered_matvec
+// (general.rs:211-230; X = rho layers) and build_layer_rows (general.rs:233-271;
+// X = {eq, gamma*eq, gamma^2*eq}).
+template <class T>
+__global__ void __launch_bounds__(GB) k_spmv_gr(const uint64_t* ptr, const uint32_t* dk, const uint32_t* xsel,
+                                                const uint32_t* xidx, const LT<T>* D, const LT<T>* X,
+                                                uint64_t xstride, LT<T>* out, uint64_t nout) {
+  __shared__ __attribute__((aligned(16))) LT<T> sh[GPB][2][E];
+  int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < nout; base += (uint64_t)gridDim.x * GPB) {
+    uint64_t o = base + g;
+    bool ok = o < nout;
+    uint64_t k0 = ok ? ptr[o] : 0, k1 = ok ? ptr[o + 1] : 0;
+    // both groups of the wave iterate the max count to keep barriers uniform
+    uint64_t cnt = k1 - k0, other = __shfl_xor(cnt, 32, 64), mx = cnt > other ? cnt : other;
+    Acc<T> acc;
+    acc.clear();
+    for (uint64_t j = 0; j < mx; j++) {
+      bool act = j < cnt;
+      if (act) {
+        uint64_t k = k0 + j;
+        stage<T>(sh[g][0], D + (uint64_t)dk[k] * E, t);
+        stage<T>(sh[g][1], X + (uint64_t)xsel[k] * xstride * E + (uint64_t)xidx[k] * E, t);
+      }
+      __syncthreads();
+      if (act) acc_mul<T, 16>(acc, sh[g][0], sh[g][1], t);
+      __syncthreads();
+    }
+    CT<T> r[4];
+    acc_final<T>(acc, t, r);
+    if (ok) st4<T>(out + o * E + 4 * t, r);
+  }
+}
+
+// m-bar (pcs.rs:554-561): mbar[j] = sum_{i<rows} e0s[i] * M[i*k2 + j]  (M low-degree if NA=16)
+template <class T, int NA>
+__global__ void __launch_bounds__(GB) k_mbar(const LT<T>* e0s, const LT<T>* M, uint64_t rows, uint64_t k2,
+                                             LT<T>* mbar, uint64_t mstride) {
+  __shared__ __attribute__((aligned(16))) LT<T> sh[GPB][2][E];
+  int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < k2; base += (uint64_t)gridDim.x * GPB) {
+    uint64_t j = base + g;
+    bool ok = j < k2;
+    Acc<T> acc;
+    acc.clear();
+    for (uint64_t i = 0; i < rows; i++) {
+      if (ok) {
+        stage<T>(sh[g][0], M + (i * k2 + j) * E, t);
+        stage<T>(sh[g][1], e0s + i * E, t);
+      }
+      __syncthreads();
+      if (ok) acc_mul<T, NA>(acc, sh[g][0], sh[g][1], t);
+      __syncthreads();
+    }
+    CT<T> r[4];
+    acc_final<T>(acc, t, r);
+    if (ok) st4<T>(mbar + j * E + 4 * t, r);
+    (void)mstride;
+  }
+}
+
+// Rotated m-bars (pcs.rs:984-1001): mbar[l][j] = sum_i e0[i] * rho_l[i*k2+j]
+template <class T>
+__global__ void __launch_bounds__(GB) k_rot_mbar(const LT<T>* e0, const LT<T>* rho, uint64_t nb, uint64_t k1,
+                                                 uint64_t k2, int nl, LT<T>* mbar) {
+  __shared__ __attribute__((aligned(16))) LT<T> sh[GPB][2][E];
+  int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  uint64_t total = k2 * (uint64_t)nl;
+  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < total; base += (uint64_t)gridDim.x * GPB) {
+    uint64_t w = base + g;
+    bool ok = w < total;
+    uint64_t l = ok ? w / k2 : 0, j = ok ? w % k2 : 0;
+    Acc<T> acc;
+    acc.clear();
+    for (uint64_t i = 0; i < k1; i++) {
+      if (ok) {
+        stage<T>(sh[g][0], rho + (l * nb + i * k2 + j) * E, t);
+        stage<T>(sh[g][1], e0 + i * E, t);
+      }
+      __syncthreads();
+      if (ok) acc_mul<T, 16>(acc, sh[g][0], sh[g][1], t);
+      __syncthreads();
+    }
+    CT<T> r[4];
+    acc_final<T>(acc, t, r);
+    if (ok) st4<T>(mbar + w * E + 4 * t, r);
+  }
+}
+
+// ============================================================================
+// RAA encoder (raa.rs:67-153): one block per row, W limbs per entry.
+//   t[i] = msg[perm1[i]/q] * d1[i];  prefix-sum;  z[i] = t[perm2[i]] * d2[i];  prefix-sum.
+// rows: `msg` rows of k2 entries -> `out` rows of n entries; `tmp` is n*W scratch per row.
+// ============================================================================
+template <class T, int W>
+__device__ void block_scan(LT<T>* buf, uint64_t n) {
+  // inclusive prefix sum over i of buf[i*W + s] for every limb s
+  constexpr int NSEG = 1024 / W;
+  __shared__ CT<T> tot[NSEG][W];
+  int s = threadIdx.x % W, seg = threadIdx.x / W;
+  uint64_t per = (n + NSEG - 1) / NSEG;
+  uint64_t b = seg * per, e = b + per < n ? b + per : n;
+  CT<T> sum = 0;
+  for (uint64_t i = b; i < e; i++) sum += buf[i * W + s];
+  tot[seg][s] = sum;
+  __syncthreads();
+  if (seg == 0) {
+    CT<T> run = 0;
+    for (int q = 0; q < NSEG; q++) { CT<T> v = tot[q][s]; tot[q][s] = run; run += v; }
+  }
+  __syncthreads();
+  CT<T> run = tot[seg][s];
+  for (uint64_t i = b; i < e; i++) { run += buf[i * W + s]; buf[i * W + s] = (LT<T>)run; }
+  __syncthreads();
+}
+
+template <class T, int W>
+__global__ void __launch_bounds__(1024) k_raa(const LT<T>* msg, uint64_t k2, const uint32_t* perm1,
+                                              const uint32_t* perm2, const LT<T>* d1, const LT<T>* d2, uint64_t q,
+                                              uint64_t n, LT<T>* tmp, LT<T>* out) {
+  uint64_t row = blockIdx.x;
+  const LT<T>* m = msg + row * k2 * W;
+  LT<T>* tb = tmp + row * n * W;
+  LT<T>* ob = out + row * n * W;
+  for (uint64_t x = threadIdx.x; x < n * W; x += blockDim.x) {
+    uint64_t i = x / W, s = x % W;
+    tb[x] = (LT<T>)((CT<T>)m[(perm1[i] / q) * W + s] * (CT<T>)d1[i]);
+  }
+  __syncthreads();
+  block_scan<T, W>(tb, n);
+  for (uint64_t x = threadIdx.x; x < n * W; x += blockDim.x) {
+    uint64_t i = x / W, s = x % W;
+    ob[x] = (LT<T>)((CT<T>)tb[(uint64_t)perm2[i] * W + s] * (CT<T>)d2[i]);
+  }
+  __syncthreads();
+  block_scan<T, W>(ob, n);
+}
+
+// ============================================================================
+// Merkle commitment over columns (merkle.rs:20-87, r2_pcs.rs:194-218)
+// ============================================================================
+// leaf[j] = BLAKE3(concat_i entry(i, j)); entry bytes es = W*sizeof(L); rows x n row-major.
+__global__ void k_hash_cols(const uint8_t* enc, uint64_t rows, uint64_t n, uint64_t es, uint32_t* leaves) {
+  uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint8_t* base = enc + j * es;
+  uint64_t rowb = n * es;
+  auto word = [&](uint64_t w) -> uint32_t {
+    uint64_t o = 4 * w;
+    uint64_t r = o / es, off = o - r * es;
+    return *reinterpret_cast<const uint32_t*>(base + r * rowb + off);
+  };
+  uint32_t out[8];
+  b3::hash_words(rows * es, word, out);
+  for (int i = 0; i < 8; i++) leaves[j * 8 + i] = out[i];
+}
+// nodes: level-order, leaves first; next level = BLAKE3(left || right)
+__global__ void k_merkle_level(uint32_t* nodes, uint64_t start, uint64_t len) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len / 2) return;
+  const uint32_t* l = nodes + (start + 2 * i) * 8;
+  uint32_t out[8];
+  b3::hash64(l, l + 8, out);
+  uint32_t* o = nodes + (start + len + i) * 8;
+  for (int k = 0; k < 8; k++) o[k] = out[k];
+}
+
+// out[q][i][:] = enc[i][idx[q]][:]   (W limbs per entry)
+template <class T>
+__global__ void k_gather_cols(const LT<T>* enc, const uint32_t* idx, uint64_t nq, uint64_t rows, uint64_t n, int W,
+                              LT<T>* out) {
+  uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t per = rows * W;
+  if (id >= nq * per) return;
+  uint64_t q = id / per, r = (id % per) / W, s = id % W;
+  out[id] = enc[(r * n + idx[q]) * W + s];
+}
+
+// small element-wise helpers (thread per limb)
+template <class T>
+__global__ void k_sub(const LT<T>* a, const LT<T>* b, LT<T>* o, uint64_t nl) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nl) o[i] = (LT<T>)(a[i] - b[i]);
+}
+template <class T>
+__global__ void k_hadamard(const LT<T>* a, const LT<T>* b, LT<T>* o, uint64_t nl) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nl) o[i] = (LT<T>)((CT<T>)a[i] * (CT<T>)b[i]);
+}
+
+}  // namespace nenya

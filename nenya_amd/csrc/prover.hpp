@@ -1,0 +1,440 @@
+// Prover orchestration: device-resident tables, host-side Fiat-Shamir,
+// proof serialization in the reference's byte order.
+//
+// Reference entry points mirrored: prove_simd (snark-ring/src/simd.rs:212-373)
+// and prove_general (snark-ring/src/general.rs:367-537).  Every value is exact
+// arithmetic mod 2^k, so the algebraically equal device schedules below emit
+// the reference's bytes (see the identities cited at each step).
+#pragma once
+#include <chrono>
+
+#include "host.hpp"
+#include "kernels.hpp"
+
+namespace nenya {
+
+inline unsigned grid_for(uint64_t items, uint64_t per_block, unsigned cap = 16384) {
+  uint64_t b = (items + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (unsigned)b;
+}
+
+// Stage timer (GeneralProveTiming / SimdProveTiming analogue); synchronizes the
+// stream at stage boundaries only when enabled.
+struct Stages {
+  bool on = false;
+  hipStream_t st = nullptr;
+  std::vector<std::pair<std::string, double>> t;
+  std::chrono::steady_clock::time_point last;
+  void start() {
+    if (!on) return;
+    (void)hipStreamSynchronize(st);
+    last = std::chrono::steady_clock::now();
+  }
+  void mark(const char* name) {
+    if (!on) return;
+    (void)hipStreamSynchronize(st);
+    auto now = std::chrono::steady_clock::now();
+    t.push_back({name, std::chrono::duration<double, std::milli>(now - last).count()});
+    last = now;
+  }
+};
+
+template <class T> struct PcsDev {
+  using L = typename T::L;
+  PcsParams P;
+  DBuf<uint32_t> perm1, perm2;
+  DBuf<L> d1, d2;
+  void upload(Pool* pool, hipStream_t st) {
+    perm1 = DBuf<uint32_t>(pool, P.n);
+    perm2 = DBuf<uint32_t>(pool, P.n);
+    d1 = DBuf<L>(pool, P.n);
+    d2 = DBuf<L>(pool, P.n);
+    std::vector<L> a(P.n), b(P.n);
+    for (size_t i = 0; i < P.n; i++) { a[i] = (L)P.d1[i]; b[i] = (L)P.d2[i]; }
+    HIP_CHECK(hipMemcpyAsync(perm1.p, P.perm1.data(), P.n * 4, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(perm2.p, P.perm2.data(), P.n * 4, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(d1.p, a.data(), P.n * sizeof(L), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(d2.p, b.data(), P.n * sizeof(L), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+  }
+};
+
+template <class T> struct CommitDev {
+  using L = typename T::L;
+  uint8_t root[32];
+  size_t rows = 0;
+  DBuf<L> enc;          // rows x n x W
+  DBuf<uint32_t> nodes;  // (2n-1) x 8 words, level order
+  const L* msg = nullptr;  // rows x k2 lifted message (borrowed)
+};
+
+struct OpenOut {
+  std::vector<uint8_t> bytes;
+};
+
+template <class T> struct Engine {
+  using L = typename T::L;
+  using C = typename T::C;
+  using G = Gr<T>;
+  Pool* pool;
+  hipStream_t st;
+  const Consts<T>& K;
+  DBuf<L> d_basis, d_pw;
+  DBuf<L> d_one;
+  Stages* stages = nullptr;
+
+  Engine(Pool* p, hipStream_t s) : pool(p), st(s), K(Consts<T>::get()) {
+    d_basis = DBuf<L>(pool, 32 * E);
+    d_pw = DBuf<L>(pool, K.psi_pw.size());
+    d_one = DBuf<L>(pool, E);
+    std::vector<L> b(32 * E);
+    for (int i = 0; i < 32; i++) std::memcpy(&b[i * E], K.basis[i].v, E * sizeof(L));
+    G one = G::one();
+    HIP_CHECK(hipMemcpyAsync(d_basis.p, b.data(), b.size() * sizeof(L), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(d_pw.p, K.psi_pw.data(), K.psi_pw.size() * sizeof(L), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(d_one.p, one.v, E * sizeof(L), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+  }
+
+  void check() { HIP_CHECK(hipGetLastError()); }
+  void sync() { HIP_CHECK(hipStreamSynchronize(st)); }
+
+  void h2d(void* d, const void* h, size_t bytes) { HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st)); }
+  void d2h(void* h, const void* d, size_t bytes) {
+    HIP_CHECK(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st));
+    sync();
+  }
+  void d2d(void* d, const void* s, size_t bytes) { HIP_CHECK(hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, st)); }
+  G get(const L* d) {
+    G g;
+    d2h(g.v, d, sizeof g.v);
+    return g;
+  }
+  DBuf<L> put(const G* xs, size_t n) {
+    DBuf<L> b(pool, n * E);
+    h2d(b.p, xs, n * sizeof(G));
+    return b;
+  }
+
+  // ---------------- element kernels ----------------
+  void pack(const L* slots, L* out, uint64_t n, int nrot, int rot0 = 0) {
+    hipLaunchKernelGGL((k_pack<T>), dim3(grid_for(n * nrot, GPB)), dim3(GB), 0, st, slots, d_basis.p, out, n, nrot, rot0);
+    check();
+  }
+  void psi(const L* x, L* slots, uint64_t n) {
+    hipLaunchKernelGGL((k_psi<T>), dim3(grid_for(n, GPB)), dim3(GB), 0, st, x, d_pw.p, slots, n);
+    check();
+  }
+  template <int NA> void mul_sub(const L* A, const L* B, const L* S, L* out, uint64_t n) {
+    hipLaunchKernelGGL((k_mul_sub<T, NA>), dim3(grid_for(n, GPB)), dim3(GB), 0, st, A, B, S, out, n);
+    check();
+  }
+  void scale(const L* s, const L* X, L* out, uint64_t n) {
+    hipLaunchKernelGGL((k_scale<T>), dim3(grid_for(n, GPB)), dim3(GB), 0, st, s, X, out, n);
+    check();
+  }
+  void sub(const L* a, const L* b, L* o, uint64_t nl) {
+    hipLaunchKernelGGL((k_sub<T>), dim3(grid_for(nl, 256, 1u << 20)), dim3(256), 0, st, a, b, o, nl);
+    check();
+  }
+  void fold(const L* r, const L* src, L* dst, uint64_t m, uint64_t ntab, uint64_t sstride, uint64_t dstride) {
+    hipLaunchKernelGGL((k_fold<T>), dim3(grid_for(m * ntab, GPB)), dim3(GB), 0, st, r, src, dst, m, ntab, sstride, dstride);
+    check();
+  }
+  void marg(const L* s, L* eg, uint64_t m) {
+    hipLaunchKernelGGL((k_marg<T>), dim3(grid_for(m, GPB)), dim3(GB), 0, st, s, eg, m);
+    check();
+  }
+  // sum of partial sums -> ns elements on device
+  DBuf<L> reduce(const DBuf<L>& partials, unsigned nblocks, int ns) {
+    DBuf<L> out(pool, ns * E);
+    hipLaunchKernelGGL((k_sum_partials<T>), dim3(ns), dim3(E), 0, st, partials.p, (uint64_t)nblocks, ns, out.p);
+    check();
+    return out;
+  }
+  template <int NA> G dot(const L* A, const L* B, uint64_t n) {
+    unsigned nb = grid_for(n, GPB, 2048);
+    DBuf<L> part(pool, (size_t)nb * E);
+    hipLaunchKernelGGL((k_dot<T, NA>), dim3(nb), dim3(GB), 0, st, A, B, n, part.p);
+    check();
+    DBuf<L> r = reduce(part, nb, 1);
+    return get(r.p);
+  }
+
+  // build_eq_table (eq.rs:35-58) for host points r[0..n): 2^n entries
+  DBuf<L> eq_table(const G* r, size_t n) {
+    size_t len = size_t(1) << n;
+    DBuf<L> a(pool, len * E), b(pool, len * E);
+    d2d(a.p, d_one.p, E * sizeof(L));
+    if (n == 0) return a;
+    DBuf<L> rr = put(r, n);
+    L* prev = a.p;
+    L* next = b.p;
+    size_t cur = 1;
+    for (size_t l = 0; l < n; l++) {
+      hipLaunchKernelGGL((k_eq_level<T>), dim3(grid_for(cur, GPB)), dim3(GB), 0, st, rr.p + l * E, prev, next, (uint64_t)cur);
+      check();
+      std::swap(prev, next);
+      cur *= 2;
+    }
+    if (prev == a.p) return a;
+    return b;
+  }
+
+  // ---------------- Fiat-Shamir challenges ----------------
+  // R2Sampler::sample (sampling.rs:57-68): 128 L::random draws then the
+  // Teichmuller lift; the lift runs on the device.
+  std::vector<G> sample(StdRng& rng, size_t cnt, DBuf<L>* dev_out = nullptr) {
+    std::vector<G> nat(cnt);
+    for (size_t i = 0; i < cnt; i++)
+      for (int l = 0; l < E; l++) nat[i].v[l] = (L)rng.limb(T::K);
+    DBuf<L> d = put(nat.data(), cnt);
+    hipLaunchKernelGGL((k_lift<T>), dim3(cnt), dim3(256), 0, st, d.p, 128);
+    check();
+    d2h(nat.data(), d.p, cnt * sizeof(G));
+    if (dev_out) *dev_out = std::move(d);
+    return nat;
+  }
+  G challenge(Transcript& tr, const char* label, DBuf<L>* dev = nullptr) {
+    StdRng rng = tr.sampler(label);
+    return sample(rng, 1, dev)[0];
+  }
+
+  // ---------------- PCS ----------------
+  // Commit (pcs.rs:489-543, 670-730; r2_pcs.rs:259-294, 452-494): encode the
+  // `rows` message rows (k2 entries of W limbs, contiguous), hash the n
+  // columns, build the Merkle tree.
+  CommitDev<T> commit(const PcsDev<T>& pc, const L* rowsbuf, size_t rows, const L* lifted) {
+    const PcsParams& P = pc.P;
+    const int W = P.packed ? 32 : E;
+    CommitDev<T> c;
+    c.rows = rows;
+    c.msg = lifted;
+    DBuf<L> tmp(pool, rows * P.n * W);
+    c.enc = DBuf<L>(pool, rows * P.n * W);
+    if (P.packed)
+      hipLaunchKernelGGL((k_raa<T, 32>), dim3(rows), dim3(1024), 0, st, rowsbuf, (uint64_t)P.k2, pc.perm1.p, pc.perm2.p,
+                         pc.d1.p, pc.d2.p, (uint64_t)P.q, (uint64_t)P.n, tmp.p, c.enc.p);
+    else
+      hipLaunchKernelGGL((k_raa<T, E>), dim3(rows), dim3(1024), 0, st, rowsbuf, (uint64_t)P.k2, pc.perm1.p, pc.perm2.p,
+                         pc.d1.p, pc.d2.p, (uint64_t)P.q, (uint64_t)P.n, tmp.p, c.enc.p);
+    check();
+    size_t n = P.n;  // a power of two
+    c.nodes = DBuf<uint32_t>(pool, (2 * n - 1) * 8);
+    hipLaunchKernelGGL(k_hash_cols, dim3(grid_for(n, 64)), dim3(64), 0, st, (const uint8_t*)c.enc.p, (uint64_t)rows,
+                       (uint64_t)n, (uint64_t)W * sizeof(L), c.nodes.p);
+    check();
+    size_t start = 0, len = n;
+    while (len > 1) {
+      hipLaunchKernelGGL(k_merkle_level, dim3(grid_for(len / 2, 128)), dim3(128), 0, st, c.nodes.p, (uint64_t)start,
+                         (uint64_t)len);
+      check();
+      start += len;
+      len /= 2;
+    }
+    d2h(c.root, c.nodes.p + (2 * n - 2) * 8, 32);
+    return c;
+  }
+
+  // Shared tail of every opening: query indices, columns, paths.
+  void open_tail(const PcsDev<T>& pc, const CommitDev<T>& c, const std::vector<uint8_t>& mbar_bytes,
+                 std::vector<uint8_t>& out) {
+    const PcsParams& P = pc.P;
+    const int W = P.packed ? 32 : E;
+    const char* dom = P.packed ? "pcs-ring/fs/queries" : "snark-ring/r2-pcs/fs/queries";
+    std::vector<uint32_t> idx = derive_queries(dom, c.root, mbar_bytes.data(), mbar_bytes.size(), P.n, P.ell);
+    put_u64(out, idx.size());
+    for (uint32_t j : idx) put_u32(out, j);
+    DBuf<uint32_t> didx(pool, idx.size());
+    h2d(didx.p, idx.data(), idx.size() * 4);
+    size_t per = c.rows * W;
+    DBuf<L> cols(pool, idx.size() * per);
+    hipLaunchKernelGGL((k_gather_cols<T>), dim3(grid_for(idx.size() * per, 256, 1u << 20)), dim3(256), 0, st, c.enc.p,
+                       didx.p, (uint64_t)idx.size(), (uint64_t)c.rows, (uint64_t)P.n, W, cols.p);
+    check();
+    std::vector<L> hc(idx.size() * per);
+    d2h(hc.data(), cols.p, hc.size() * sizeof(L));
+    put_u64(out, idx.size());
+    for (size_t q = 0; q < idx.size(); q++) {
+      put_u64(out, c.rows);
+      const uint8_t* b = reinterpret_cast<const uint8_t*>(&hc[q * per]);
+      out.insert(out.end(), b, b + per * sizeof(L));
+    }
+    size_t n = P.n;
+    std::vector<uint32_t> nodes((2 * n - 1) * 8);
+    d2h(nodes.data(), c.nodes.p, nodes.size() * 4);
+    size_t plen = (size_t)__builtin_ctzll(n);
+    put_u64(out, idx.size());
+    for (uint32_t j : idx) {
+      put_u64(out, plen);
+      size_t id = j, start = 0, len = n;
+      while (len > 1) {
+        const uint8_t* h = reinterpret_cast<const uint8_t*>(&nodes[(start + (id ^ 1)) * 8]);
+        out.insert(out.end(), h, h + 32);
+        id /= 2;
+        start += len;
+        len /= 2;
+      }
+    }
+  }
+
+  // Pcs::open / open_stacked (pcs.rs:547-591, 734-779) and R2Pcs::open /
+  // open_stacked (r2_pcs.rs:297-338, 512-559).  weights: s elements or null (s=1).
+  G open(const PcsDev<T>& pc, const CommitDev<T>& c, const G* z, const G* weights, size_t s, std::vector<uint8_t>& out) {
+    const PcsParams& P = pc.P;
+    size_t lk1 = (size_t)__builtin_ctzll(P.k1), lk2 = (size_t)__builtin_ctzll(P.k2);
+    DBuf<L> e0 = eq_table(z, lk1), e1 = eq_table(z + lk1, lk2);
+    DBuf<L> e0s(pool, s * P.k1 * E);
+    for (size_t t = 0; t < s; t++) {
+      if (!weights) d2d(e0s.p + t * P.k1 * E, e0.p, P.k1 * E * sizeof(L));
+      else {
+        DBuf<L> w = put(&weights[t], 1);
+        scale(w.p, e0.p, e0s.p + t * P.k1 * E, P.k1);
+      }
+    }
+    DBuf<L> mbar(pool, P.k2 * E);
+    uint64_t rows = s * P.k1;
+    if (P.packed)
+      hipLaunchKernelGGL((k_mbar<T, 16>), dim3(grid_for(P.k2, GPB)), dim3(GB), 0, st, e0s.p, c.msg, rows, (uint64_t)P.k2,
+                         mbar.p, (uint64_t)0);
+    else
+      hipLaunchKernelGGL((k_mbar<T, 32>), dim3(grid_for(P.k2, GPB)), dim3(GB), 0, st, e0s.p, c.msg, rows, (uint64_t)P.k2,
+                         mbar.p, (uint64_t)0);
+    check();
+    G y = dot<32>(mbar.p, e1.p, P.k2);
+    std::vector<uint8_t> mb(P.k2 * E * sizeof(L));
+    d2h(mb.data(), mbar.p, mb.size());
+    put_u64(out, P.k2);
+    out.insert(out.end(), mb.begin(), mb.end());
+    open_tail(pc, c, mb, out);
+    return y;
+  }
+
+  // Pcs::rotated_open_batch (pcs.rs:973-1040) over the unfolded rho layers
+  // (rho_l[q] = pack_r2(rot_l(slots_q)), the lifted messages of the rotations).
+  std::vector<G> rotated_open(const PcsDev<T>& pc, const CommitDev<T>& c, const L* rho, uint64_t nb, const G* z, int nl,
+                              std::vector<uint8_t>& out) {
+    const PcsParams& P = pc.P;
+    size_t lk1 = (size_t)__builtin_ctzll(P.k1), lk2 = (size_t)__builtin_ctzll(P.k2);
+    DBuf<L> e0 = eq_table(z, lk1), e1 = eq_table(z + lk1, lk2);
+    DBuf<L> mbar(pool, (size_t)nl * P.k2 * E);
+    hipLaunchKernelGGL((k_rot_mbar<T>), dim3(grid_for((uint64_t)nl * P.k2, GPB)), dim3(GB), 0, st, e0.p, rho, nb,
+                       (uint64_t)P.k1, (uint64_t)P.k2, nl, mbar.p);
+    check();
+    std::vector<G> ys(nl);
+    for (int l = 0; l < nl; l++) ys[l] = dot<32>(mbar.p + (size_t)l * P.k2 * E, e1.p, P.k2);
+    std::vector<uint8_t> mb((size_t)nl * P.k2 * E * sizeof(L));
+    d2h(mb.data(), mbar.p, mb.size());
+    put_u64(out, nl);
+    size_t per = P.k2 * E * sizeof(L);
+    for (int l = 0; l < nl; l++) {
+      put_u64(out, P.k2);
+      out.insert(out.end(), mb.begin() + l * per, mb.begin() + (l + 1) * per);
+    }
+    open_tail(pc, c, mb, out);
+    return ys;
+  }
+
+  // ---------------- sumchecks ----------------
+  static G eq_one(const G& t, const G& r) { return t * r + (G::one() - t) * (G::one() - r); }
+
+  // Gruen mult check (mult_check.rs:180-241) and its sub form (426-514).
+  // tabs: a, b (, c, e) device tables of 2^nv entries; round 0 folds
+  // out-of-place into `work` so the caller's tables survive.
+  struct SumcheckOut {
+    std::vector<G> rounds;  // 3 per round
+    std::vector<G> chal;
+    std::vector<G> finals;
+  };
+  SumcheckOut mult_check(const L* const* tabs, int ntab, size_t nv, const std::vector<G>& rt, Transcript& tr) {
+    SumcheckOut o;
+    size_t N = size_t(1) << nv;
+    G one = G::one();
+    DBuf<L> eg = nv > 1 ? eq_table(rt.data() + 1, nv - 1) : put(&one, 1);
+    DBuf<L> work(pool, (size_t)ntab * (N / 2 > 0 ? N / 2 : 1) * E);
+    size_t len = N;
+    const L* cur[4];
+    for (int t = 0; t < ntab; t++) cur[t] = tabs[t];
+    for (size_t r = 0; r < nv; r++) {
+      size_t m = len / 2;
+      unsigned nbk = grid_for(m, GPB, 2048);
+      DBuf<L> part(pool, (size_t)nbk * 3 * E);
+      hipLaunchKernelGGL((k_mult_round<T>), dim3(nbk), dim3(GB), 0, st, eg.p, cur[0], cur[1], ntab > 2 ? cur[2] : nullptr,
+                         ntab > 2 ? cur[3] : nullptr, (uint64_t)m, part.p);
+      check();
+      DBuf<L> s = reduce(part, nbk, 3);
+      G sv[3];
+      d2h(sv, s.p, sizeof sv);
+      G q[3] = {sv[0], sv[1] - sv[0] - sv[2], sv[2]};
+      tr.elems("sc/round", q, 3);
+      for (int i = 0; i < 3; i++) o.rounds.push_back(q[i]);
+      DBuf<L> dr;
+      G rc = challenge(tr, "sc/challenge", &dr);
+      o.chal.push_back(rc);
+      for (int t = 0; t < ntab; t++) {
+        L* dst = work.p + (size_t)t * (N / 2) * E;
+        fold(dr.p, cur[t], dst, m, 1, 0, 0);
+        cur[t] = dst;
+      }
+      len = m;
+      if (r + 1 < nv) {
+        G sc = eq_one(rt[r], rc);
+        DBuf<L> ds = put(&sc, 1);
+        marg(ds.p, eg.p, m / 2);
+      }
+    }
+    for (int t = 0; t < ntab; t++) o.finals.push_back(get(cur[t]));
+    return o;
+  }
+
+  // {0,1,inf} linear sumcheck over `ntab` pairs (f_t, g_t)
+  // (linear_check_simd.rs:196-286 for ntab = 1, linear_check_general.rs:214-350
+  // for ntab = 32).  Round 0 reads the caller's tables and folds into fresh
+  // buffers, so both inputs survive.  f0 may be low-degree (NAF0 = 16).
+  template <int NAF0>
+  SumcheckOut linear(const L* f0, uint64_t fstride0, const L* g0, uint64_t gstride0, int ntab, size_t nv, G claim,
+                     Transcript& tr) {
+    SumcheckOut o;
+    size_t N = size_t(1) << nv;
+    size_t H = N / 2 > 0 ? N / 2 : 1;
+    DBuf<L> fw(pool, (size_t)ntab * H * E), gw(pool, (size_t)ntab * H * E);
+    const L* f = f0;
+    const L* g = g0;
+    uint64_t fs = fstride0, gs = gstride0;
+    G c = claim;
+    size_t len = N;
+    for (size_t r = 0; r < nv; r++) {
+      size_t m = len / 2;
+      unsigned nbk = grid_for(m * ntab, GPB, 2048);
+      DBuf<L> part(pool, (size_t)nbk * 2 * E);
+      if (r == 0 && NAF0 == 16)
+        hipLaunchKernelGGL((k_lin_round<T, 16>), dim3(nbk), dim3(GB), 0, st, f, fs, g, gs, (uint64_t)ntab, (uint64_t)m, part.p);
+      else
+        hipLaunchKernelGGL((k_lin_round<T, 32>), dim3(nbk), dim3(GB), 0, st, f, fs, g, gs, (uint64_t)ntab, (uint64_t)m, part.p);
+      check();
+      DBuf<L> s = reduce(part, nbk, 2);
+      G sv[2];
+      d2h(sv, s.p, sizeof sv);
+      G q[3] = {sv[0], c - (sv[0] + sv[0]) - sv[1], sv[1]};
+      tr.elems("sc/round", q, 3);
+      for (int i = 0; i < 3; i++) o.rounds.push_back(q[i]);
+      DBuf<L> dr;
+      G rc = challenge(tr, "sc/challenge", &dr);
+      o.chal.push_back(rc);
+      c = q[0] + rc * (q[1] + rc * q[2]);
+      fold(dr.p, f, fw.p, m, ntab, fs, H);
+      fold(dr.p, g, gw.p, m, ntab, gs, H);
+      f = fw.p;
+      g = gw.p;
+      fs = gs = H;
+      len = m;
+    }
+    for (int t = 0; t < ntab; t++) o.finals.push_back(get(f + (size_t)t * fs * E));
+    for (int t = 0; t < ntab; t++) o.finals.push_back(get(g + (size_t)t * gs * E));
+    return o;
+  }
+};
+
+}  // namespace nenya

@@ -1,0 +1,479 @@
+// SIMD and General SNARK provers on the device (the drop-in prove_simd /
+// prove_general of snark-ring/src/{simd,general}.rs).
+#pragma once
+#include <algorithm>
+#include <numeric>
+
+#include "prover.hpp"
+
+namespace nenya {
+
+struct ParamsBase {
+  int k = 32;
+  int kind = 0;  // 0 simd, 1 general
+  bool timing = false;
+  std::vector<std::pair<std::string, double>> last_timing;
+  virtual ~ParamsBase() = default;
+};
+
+struct HostCoo {
+  size_t rows, cols, nnz;
+  const uint32_t* r;
+  const uint32_t* c;
+  const void* v;  // nnz limbs of k bits (native width)
+};
+
+template <class T> struct Csr {
+  DBuf<uint32_t> ptr, idx;
+  DBuf<typename T::L> val;
+  // group COO by `major` (rows if !by_col), stable
+  void build(Pool* pool, hipStream_t st, const HostCoo& m, bool by_col) {
+    using L = typename T::L;
+    size_t nmaj = by_col ? m.cols : m.rows;
+    std::vector<uint32_t> p(nmaj + 1, 0), ix(m.nnz);
+    std::vector<L> v(m.nnz);
+    const L* vals = (const L*)m.v;
+    for (size_t k = 0; k < m.nnz; k++) p[(by_col ? m.c[k] : m.r[k]) + 1]++;
+    for (size_t i = 0; i < nmaj; i++) p[i + 1] += p[i];
+    std::vector<uint32_t> pos(p.begin(), p.end() - 1);
+    for (size_t k = 0; k < m.nnz; k++) {
+      uint32_t maj = by_col ? m.c[k] : m.r[k];
+      uint32_t o = pos[maj]++;
+      ix[o] = by_col ? m.r[k] : m.c[k];
+      v[o] = vals[k];
+    }
+    ptr = DBuf<uint32_t>(pool, nmaj + 1);
+    idx = DBuf<uint32_t>(pool, std::max<size_t>(m.nnz, 1));
+    val = DBuf<L>(pool, std::max<size_t>(m.nnz, 1));
+    HIP_CHECK(hipMemcpyAsync(ptr.p, p.data(), p.size() * 4, hipMemcpyHostToDevice, st));
+    if (m.nnz) {
+      HIP_CHECK(hipMemcpyAsync(idx.p, ix.data(), ix.size() * 4, hipMemcpyHostToDevice, st));
+      HIP_CHECK(hipMemcpyAsync(val.p, v.data(), v.size() * sizeof(L), hipMemcpyHostToDevice, st));
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+  }
+};
+
+inline void check_coo(const HostCoo& m, size_t rows, size_t cols) {
+  if (m.rows != rows || m.cols != cols) throw Error(NENYA_E_SHAPE, "matrix shape mismatch");
+  for (size_t k = 0; k < m.nnz; k++)
+    if (m.r[k] >= rows || m.c[k] >= cols) throw Error(NENYA_E_SHAPE, "matrix entry out of range");
+}
+
+// ============================================================================
+// SIMD (simd.rs:220-373)
+// ============================================================================
+template <class T> struct SimdParams : ParamsBase {
+  using L = typename T::L;
+  using G = Gr<T>;
+  int log_n = 0, log_m = 0;
+  Pool pool;
+  hipStream_t st = nullptr;
+  std::unique_ptr<Engine<T>> eng;
+  PcsDev<T> pw, pq;
+
+  SimdParams(int ln, int lm, uint64_t seed_w, uint64_t seed_q) {
+    k = T::K;
+    kind = 0;
+    log_n = ln;
+    log_m = lm;
+    HIP_CHECK(hipStreamCreate(&st));
+    eng.reset(new Engine<T>(&pool, st));
+    pw.P = PcsParams::setup(lm, seed_w, T::K);
+    pq.P = PcsParams::setup_r2(ln, seed_q, T::K);
+    pw.upload(&pool, st);
+    pq.upload(&pool, st);
+  }
+  ~SimdParams() override {
+    eng.reset();
+    pw = PcsDev<T>();
+    pq = PcsDev<T>();
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  std::vector<uint8_t> prove(const HostCoo& A, const HostCoo& B, const HostCoo& Cm, const L* wit /*32 x m*/) {
+    Engine<T>& e = *eng;
+    size_t n = size_t(1) << log_n, m = size_t(1) << log_m;
+    check_coo(A, n, m);
+    check_coo(B, n, m);
+    check_coo(Cm, n, m);
+    Stages stg;
+    stg.on = timing;
+    stg.st = st;
+    stg.start();
+    Csr<T> ar, br, cr, ac, bc, cc;
+    ar.build(&pool, st, A, false);
+    br.build(&pool, st, B, false);
+    cr.build(&pool, st, Cm, false);
+    ac.build(&pool, st, A, true);
+    bc.build(&pool, st, B, true);
+    cc.build(&pool, st, Cm, true);
+    // witnesses [32][m] and their slot transpose [m][32]
+    DBuf<L> dw(&pool, 32 * m), slots(&pool, m * 32);
+    {
+      std::vector<L> tr(m * 32);
+      for (size_t j = 0; j < m; j++)
+        for (int kk = 0; kk < 32; kk++) tr[j * 32 + kk] = wit[kk * m + j];
+      e.h2d(dw.p, wit, 32 * m * sizeof(L));
+      e.h2d(slots.p, tr.data(), tr.size() * sizeof(L));
+    }
+    stg.mark("upload");
+    // 1. simd_to_packed (reduce.rs:33-104): w_phi = phi(slots); x_i = (A w_k)_i, y_i = (B w_k)_i
+    //    and e_i = phi(x_i) phi(y_i) - phi(x_i . y_i)  (= q1 v_alpha + q2 v_beta exactly,
+    //    embedding.rs:147-167).
+    DBuf<L> wphi(&pool, m * E);
+    e.pack(slots.p, wphi.p, m, 1);
+    DBuf<L> xs(&pool, n * 32), ys(&pool, n * 32), xy(&pool, n * 32);
+    unsigned g0 = grid_for(n * 32, 256, 1u << 20);
+    hipLaunchKernelGGL((k_r0_matvec<T>), dim3(g0), dim3(256), 0, st, ar.ptr.p, ar.idx.p, ar.val.p, dw.p, (uint64_t)m, xs.p,
+                       (uint64_t)n, 32, (uint64_t)32, (uint64_t)1);
+    hipLaunchKernelGGL((k_r0_matvec<T>), dim3(g0), dim3(256), 0, st, br.ptr.p, br.idx.p, br.val.p, dw.p, (uint64_t)m, ys.p,
+                       (uint64_t)n, 32, (uint64_t)32, (uint64_t)1);
+    hipLaunchKernelGGL((k_hadamard<T>), dim3(g0), dim3(256), 0, st, xs.p, ys.p, xy.p, (uint64_t)n * 32);
+    e.check();
+    DBuf<L> px(&pool, n * E), py(&pool, n * E), ev(&pool, n * E);
+    e.pack(xs.p, px.p, n, 1);
+    e.pack(ys.p, py.p, n, 1);
+    e.pack(xy.p, ev.p, n, 1);
+    e.template mul_sub<16>(px.p, py.p, ev.p, ev.p, n);
+    stg.mark("reduce");
+    // 2. commits
+    CommitDev<T> cw = e.commit(pw, slots.p, pw.P.k1, wphi.p);
+    CommitDev<T> ce = e.commit(pq, ev.p, pq.P.k1, ev.p);
+    stg.mark("commit");
+    // 3. transcript
+    Transcript tr("simd");
+    tr.root(cw.root);
+    tr.root(ce.root);
+    StdRng rs = tr.sampler("mult_check_target");
+    std::vector<G> rt = e.sample(rs, log_n);
+    // 4. a_phi, b_phi, c_phi (matrix.rs:93-105)
+    DBuf<L> aw(&pool, n * E), bw(&pool, n * E), cwv(&pool, n * E);
+    unsigned gs = grid_for(n, GPB);
+    hipLaunchKernelGGL((k_spmv_gr1<T>), dim3(gs), dim3(GB), 0, st, ar.ptr.p, ar.idx.p, ar.val.p, wphi.p, aw.p, (uint64_t)n, 0);
+    hipLaunchKernelGGL((k_spmv_gr1<T>), dim3(gs), dim3(GB), 0, st, br.ptr.p, br.idx.p, br.val.p, wphi.p, bw.p, (uint64_t)n, 0);
+    hipLaunchKernelGGL((k_spmv_gr1<T>), dim3(gs), dim3(GB), 0, st, cr.ptr.p, cr.idx.p, cr.val.p, wphi.p, cwv.p, (uint64_t)n, 0);
+    e.check();
+    // 5. claim_mult = c_phi(r) + e(r)
+    DBuf<L> eq_r = e.eq_table(rt.data(), log_n);
+    G cw_at_r = e.template dot<32>(eq_r.p, cwv.p, n);
+    G e_at_r = e.template dot<32>(eq_r.p, ev.p, n);
+    G claim_mult = cw_at_r + e_at_r;
+    tr.elems("claim_mult", &claim_mult, 1);
+    stg.mark("matvec");
+    // 6. mult check (Gruen)
+    tr.label("mult_check_sumcheck");
+    const L* mt[2] = {aw.p, bw.p};
+    auto mp = e.mult_check(mt, 2, log_n, rt, tr);
+    G fin_ab[2] = {mp.finals[0], mp.finals[1]};
+    tr.elems("mult/finals", fin_ab, 2);
+    stg.mark("mult_sumcheck");
+    // 7. gamma
+    G gam = e.challenge(tr, "linear_check_gamma");
+    G gam2 = gam * gam;
+    // 8. M_gamma row (simd.rs:539-580) = A_row(s1) + gamma B_row(s1) + gamma^2 C_row(r),
+    //    accumulated as sum over columns of (eq_s1 | gamma eq_s1 | gamma^2 eq_r)[row] * v.
+    DBuf<L> eq_s1 = e.eq_table(mp.chal.data(), log_n);
+    DBuf<L> geq(&pool, n * E), g2eq(&pool, n * E), mrow(&pool, m * E);
+    DBuf<L> dg = e.put(&gam, 1), dg2 = e.put(&gam2, 1);
+    e.scale(dg.p, eq_s1.p, geq.p, n);
+    e.scale(dg2.p, eq_r.p, g2eq.p, n);
+    unsigned gm = grid_for(m, GPB);
+    hipLaunchKernelGGL((k_spmv_gr1<T>), dim3(gm), dim3(GB), 0, st, ac.ptr.p, ac.idx.p, ac.val.p, eq_s1.p, mrow.p, (uint64_t)m, 0);
+    hipLaunchKernelGGL((k_spmv_gr1<T>), dim3(gm), dim3(GB), 0, st, bc.ptr.p, bc.idx.p, bc.val.p, geq.p, mrow.p, (uint64_t)m, 1);
+    hipLaunchKernelGGL((k_spmv_gr1<T>), dim3(gm), dim3(GB), 0, st, cc.ptr.p, cc.idx.p, cc.val.p, g2eq.p, mrow.p, (uint64_t)m, 1);
+    e.check();
+    // 9. claim_lin
+    G claim_lin = mp.finals[0] + gam * mp.finals[1] + gam2 * cw_at_r;
+    tr.elems("claim_lin", &claim_lin, 1);
+    stg.mark("linear_rows");
+    // 10. linear check on (w_phi, M_gamma row); w_phi is low-degree
+    tr.label("linear_check_sumcheck");
+    auto lp = e.template linear<16>(wphi.p, 0, mrow.p, 0, 1, log_m, claim_lin, tr);
+    G final_w = lp.finals[0], final_row = lp.finals[1];
+    G fin_lin[2] = {final_row, final_w};
+    tr.elems("lin/finals", fin_lin, 2);
+    stg.mark("linear_sumcheck");
+    // 11. serialize (simd.rs:150-174) with the two openings
+    std::vector<uint8_t> out;
+    out.insert(out.end(), cw.root, cw.root + 32);
+    out.insert(out.end(), ce.root, ce.root + 32);
+    put_gr(out, claim_mult);
+    put_u64(out, mp.chal.size());
+    for (auto& g : mp.rounds) put_gr(out, g);
+    put_gr(out, mp.finals[0]);
+    put_gr(out, mp.finals[1]);
+    put_gr(out, claim_lin);
+    put_u64(out, lp.chal.size());
+    for (auto& g : lp.rounds) put_gr(out, g);
+    put_gr(out, final_row);
+    put_gr(out, final_w);
+    G yw = e.open(pw, cw, lp.chal.data(), nullptr, 1, out);
+    G ye = e.open(pq, ce, rt.data(), nullptr, 1, out);
+    stg.mark("open");
+    if (!(yw == final_w) || !(ye == e_at_r)) throw Error(NENYA_E_DEVICE, "internal opening self-check failed");
+    last_timing = stg.t;
+    return out;
+  }
+};
+
+// ============================================================================
+// General (general.rs:374-537)
+// ============================================================================
+template <class T> struct GeneralParams : ParamsBase {
+  using L = typename T::L;
+  using G = Gr<T>;
+  int log_b = 0;
+  size_t nb = 0, N = 0;
+  Pool pool;
+  hipStream_t st = nullptr;
+  std::unique_ptr<Engine<T>> eng;
+  PcsDev<T> pw, pabc, pe;
+  Csr<T> mr[3];  // A, B, C by row (base-ring matvecs)
+  DBuf<L> dhat;  // packed diagonal cells, A then B then C
+  size_t ncell[3] = {0, 0, 0};
+  DBuf<uint64_t> lptr[3];
+  DBuf<uint32_t> ldk[3], lsel[3], lidx[3];
+  DBuf<uint64_t> rptr;
+  DBuf<uint32_t> rdk, rsel, ridx;
+
+  // GeneralSnarkParams::secure (general.rs:69-95) with pack_general_matrices
+  // (reduce.rs:372-438, 618-651) done once here (setup, not prover time).
+  GeneralParams(int lb, const HostCoo* m3, uint64_t seed_w, uint64_t seed_e) {
+    k = T::K;
+    kind = 1;
+    log_b = lb;
+    nb = size_t(1) << lb;
+    N = 32 * nb;
+    HIP_CHECK(hipStreamCreate(&st));
+    eng.reset(new Engine<T>(&pool, st));
+    pw.P = PcsParams::setup_batched(lb, seed_w, 32, T::K);
+    pabc.P = PcsParams::setup_stacked(lb, seed_w ^ 0xABC, 3, T::K);
+    pe.P = PcsParams::setup_r2(lb, seed_e, T::K);
+    pw.upload(&pool, st);
+    pabc.upload(&pool, st);
+    pe.upload(&pool, st);
+    for (int w = 0; w < 3; w++) {
+      check_coo(m3[w], N, N);
+      mr[w].build(&pool, st, m3[w], false);
+    }
+    // diagonal cells: nonzero (r,c) -> layer l = (c%32 - r%32) mod 32, p = r/32,
+    // q = c/32, slot i = r%32 (reduce.rs:387-397)
+    struct Cell { uint64_t key; uint32_t slot; uint64_t val; };
+    std::vector<std::vector<uint32_t>> cp(3), cq(3), cl(3);
+    std::vector<L> cslots;
+    size_t base = 0;
+    std::vector<size_t> cell_base(3);
+    for (int w = 0; w < 3; w++) {
+      const HostCoo& m = m3[w];
+      const L* vals = (const L*)m.v;
+      std::vector<Cell> items(m.nnz);
+      for (size_t k2 = 0; k2 < m.nnz; k2++) {
+        uint64_t r = m.r[k2], c = m.c[k2];
+        uint64_t l = ((c % 32) + 32 - (r % 32)) % 32;
+        items[k2] = {(l * nb + r / 32) * nb + c / 32, (uint32_t)(r % 32), (uint64_t)vals[k2]};
+      }
+      std::sort(items.begin(), items.end(), [](const Cell& a, const Cell& b) { return a.key < b.key; });
+      cell_base[w] = base;
+      for (size_t i = 0; i < items.size(); i++) {
+        if (i == 0 || items[i].key != items[i - 1].key) {
+          uint64_t key = items[i].key;
+          cl[w].push_back((uint32_t)(key / (nb * nb)));
+          cp[w].push_back((uint32_t)((key / nb) % nb));
+          cq[w].push_back((uint32_t)(key % nb));
+          cslots.resize(cslots.size() + 32, (L)0);
+        }
+        L& s = cslots[cslots.size() - 32 + items[i].slot];
+        s = (L)(s + (L)items[i].val);
+      }
+      ncell[w] = cl[w].size();
+      base += ncell[w];
+    }
+    size_t total = base;
+    DBuf<L> dsl(&pool, std::max<size_t>(total, 1) * 32);
+    dhat = DBuf<L>(&pool, std::max<size_t>(total, 1) * E);
+    if (total) {
+      eng->h2d(dsl.p, cslots.data(), cslots.size() * sizeof(L));
+      eng->pack(dsl.p, dhat.p, total, 1);
+    }
+    // layered_matvec CSR per matrix, grouped by output block row p
+    for (int w = 0; w < 3; w++) {
+      std::vector<uint64_t> ptr(nb + 1, 0);
+      for (size_t i = 0; i < ncell[w]; i++) ptr[cp[w][i] + 1]++;
+      for (size_t p = 0; p < nb; p++) ptr[p + 1] += ptr[p];
+      std::vector<uint64_t> pos(ptr.begin(), ptr.end() - 1);
+      std::vector<uint32_t> dk(ncell[w]), sel(ncell[w]), idx(ncell[w]);
+      for (size_t i = 0; i < ncell[w]; i++) {
+        uint64_t o = pos[cp[w][i]]++;
+        dk[o] = (uint32_t)(cell_base[w] + i);
+        sel[o] = cl[w][i];
+        idx[o] = cq[w][i];
+      }
+      upload_csr(ptr, dk, sel, idx, lptr[w], ldk[w], lsel[w], lidx[w]);
+    }
+    // build_layer_rows CSR over (l, q) merging A, B, C (xsel = matrix id)
+    {
+      std::vector<uint64_t> ptr(32 * nb + 1, 0);
+      for (int w = 0; w < 3; w++)
+        for (size_t i = 0; i < ncell[w]; i++) ptr[(size_t)cl[w][i] * nb + cq[w][i] + 1]++;
+      for (size_t o = 0; o < 32 * nb; o++) ptr[o + 1] += ptr[o];
+      std::vector<uint64_t> pos(ptr.begin(), ptr.end() - 1);
+      std::vector<uint32_t> dk(total), sel(total), idx(total);
+      for (int w = 0; w < 3; w++)
+        for (size_t i = 0; i < ncell[w]; i++) {
+          uint64_t o = pos[(size_t)cl[w][i] * nb + cq[w][i]]++;
+          dk[o] = (uint32_t)(cell_base[w] + i);
+          sel[o] = (uint32_t)w;
+          idx[o] = cp[w][i];
+        }
+      upload_csr(ptr, dk, sel, idx, rptr, rdk, rsel, ridx);
+    }
+    eng->sync();
+  }
+  void upload_csr(const std::vector<uint64_t>& ptr, const std::vector<uint32_t>& dk, const std::vector<uint32_t>& sel,
+                  const std::vector<uint32_t>& idx, DBuf<uint64_t>& dp, DBuf<uint32_t>& ddk, DBuf<uint32_t>& dsel,
+                  DBuf<uint32_t>& didx) {
+    dp = DBuf<uint64_t>(&pool, ptr.size());
+    size_t nn = std::max<size_t>(dk.size(), 1);
+    ddk = DBuf<uint32_t>(&pool, nn);
+    dsel = DBuf<uint32_t>(&pool, nn);
+    didx = DBuf<uint32_t>(&pool, nn);
+    eng->h2d(dp.p, ptr.data(), ptr.size() * 8);
+    if (!dk.empty()) {
+      eng->h2d(ddk.p, dk.data(), dk.size() * 4);
+      eng->h2d(dsel.p, sel.data(), sel.size() * 4);
+      eng->h2d(didx.p, idx.data(), idx.size() * 4);
+    }
+    eng->sync();
+  }
+  ~GeneralParams() override {
+    eng.reset();
+    pw = PcsDev<T>();
+    pabc = PcsDev<T>();
+    pe = PcsDev<T>();
+    for (auto& m : mr) m = Csr<T>();
+    dhat.release();
+    for (int w = 0; w < 3; w++) { lptr[w].release(); ldk[w].release(); lsel[w].release(); lidx[w].release(); }
+    rptr.release(); rdk.release(); rsel.release(); ridx.release();
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  std::vector<uint8_t> prove_device(const L* w /* device, N limbs */) {
+    Engine<T>& e = *eng;
+    Stages stg;
+    stg.on = timing;
+    stg.st = st;
+    stg.start();
+    // 1. rho_l[q] = pack_r2(rot_l(w_q)) for all 32 l (reduce.rs:690-707)
+    DBuf<L> rho(&pool, 32 * nb * E);
+    e.pack(w, rho.p, nb, 32);
+    // 2. base-ring matvecs: slots of a_hat, b_hat are A w, B w blocks
+    //    (psi(a_hat) = A w since a_hat = sum D rho - e_A = phi(A w) exactly)
+    DBuf<L> abc_slots(&pool, 3 * N);
+    unsigned g0 = grid_for(N, 256, 1u << 20);
+    for (int m = 0; m < 2; m++)
+      hipLaunchKernelGGL((k_r0_matvec<T>), dim3(g0), dim3(256), 0, st, mr[m].ptr.p, mr[m].idx.p, mr[m].val.p, w, (uint64_t)0,
+                         abc_slots.p + m * N, (uint64_t)N, 1, (uint64_t)1, (uint64_t)0);
+    e.check();
+    // 3. layered matvecs sum_l D_l rho_l (general.rs:211-230) for A, B, C
+    DBuf<L> sums(&pool, 3 * nb * E);
+    for (int m = 0; m < 3; m++)
+      hipLaunchKernelGGL((k_spmv_gr<T>), dim3(grid_for(nb, GPB)), dim3(GB), 0, st, lptr[m].p, ldk[m].p, lsel[m].p, lidx[m].p,
+                         dhat.p, rho.p, (uint64_t)nb, sums.p + m * nb * E, (uint64_t)nb);
+    e.check();
+    // 4. c_hat = pack_r2(psi(c_sum)) (general.rs:410-414); a_hat, b_hat = phi(slots)
+    e.psi(sums.p + 2 * nb * E, abc_slots.p + 2 * N, nb);
+    DBuf<L> hats(&pool, 3 * nb * E);
+    e.pack(abc_slots.p, hats.p, 3 * nb, 1);
+    // 5. e_A, e_B, e_C = sums - hats (stacked error), e_mult = a_hat b_hat - c_hat
+    DBuf<L> estack(&pool, 3 * nb * E), emult(&pool, nb * E);
+    e.sub(sums.p, hats.p, estack.p, 3 * nb * E);
+    e.template mul_sub<16>(hats.p, hats.p + nb * E, hats.p + 2 * nb * E, emult.p, nb);
+    sums.release();
+    stg.mark("reduce_matvec");
+    // 6. commits (general.rs:434-451)
+    CommitDev<T> cw = e.commit(pw, w, pw.P.k1, rho.p);
+    CommitDev<T> cabc = e.commit(pabc, abc_slots.p, 3 * pabc.P.k1, hats.p);
+    CommitDev<T> cem = e.commit(pe, emult.p, pe.P.k1, emult.p);
+    CommitDev<T> ces = e.commit(pe, estack.p, 3 * pe.P.k1, estack.p);
+    stg.mark("commit");
+    Transcript tr("general");
+    tr.root(cw.root);
+    tr.root(cabc.root);
+    tr.root(cem.root);
+    tr.root(ces.root);
+    StdRng rs = tr.sampler("mult_check_target");
+    std::vector<G> rt = e.sample(rs, log_b);
+    // 7. mult check, sub form, on (a_hat, b_hat, c_hat, e_mult)
+    tr.label("mult_check_sumcheck");
+    const L* mt[4] = {hats.p, hats.p + nb * E, hats.p + 2 * nb * E, emult.p};
+    auto mp = e.mult_check(mt, 4, log_b, rt, tr);
+    tr.elems("mult/finals", mp.finals.data(), 4);
+    G gam = e.challenge(tr, "linear_check_gamma");
+    G gam2 = gam * gam;
+    stg.mark("mult_sumcheck");
+    // 8. openings at s1 (general.rs:475-478)
+    G wts[3] = {G::one(), gam, gam2};
+    std::vector<uint8_t> o_es, o_abc, o_em;
+    G estack_val = e.open(pe, ces, mp.chal.data(), wts, 3, o_es);
+    G abc_val = e.open(pabc, cabc, mp.chal.data(), wts, 3, o_abc);
+    G em_val = e.open(pe, cem, mp.chal.data(), nullptr, 1, o_em);
+    stg.mark("open_s1");
+    // 9. layer rows (general.rs:233-271) as sum over cells of
+    //    {eq_s1, gamma eq_s1, gamma^2 eq_s1}[p] * D[cell]
+    DBuf<L> eqs(&pool, 3 * nb * E);
+    {
+      DBuf<L> eq1 = e.eq_table(mp.chal.data(), log_b);
+      e.d2d(eqs.p, eq1.p, nb * E * sizeof(L));
+      DBuf<L> dg = e.put(&gam, 1), dg2 = e.put(&gam2, 1);
+      e.scale(dg.p, eq1.p, eqs.p + nb * E, nb);
+      e.scale(dg2.p, eq1.p, eqs.p + 2 * nb * E, nb);
+    }
+    DBuf<L> rows(&pool, 32 * nb * E);
+    hipLaunchKernelGGL((k_spmv_gr<T>), dim3(grid_for(32 * nb, GPB)), dim3(GB), 0, st, rptr.p, rdk.p, rsel.p, ridx.p, dhat.p,
+                       eqs.p, (uint64_t)nb, rows.p, (uint64_t)(32 * nb));
+    e.check();
+    eqs.release();
+    stg.mark("layer_rows");
+    // 10. linear check over the 32 (rho_l, row_l) pairs; rho is low-degree
+    G claim_lin = mp.finals[0] + gam * mp.finals[1] + gam2 * mp.finals[2] + estack_val;
+    tr.label("linear_check_sumcheck");
+    auto lp = e.template linear<16>(rho.p, nb, rows.p, nb, 32, log_b, claim_lin, tr);
+    rows.release();
+    stg.mark("linear_sumcheck");
+    // 11. rotated openings of w at s2 (pcs.rs:973-1040)
+    std::vector<uint8_t> o_rho;
+    std::vector<G> ys = e.rotated_open(pw, cw, rho.p, nb, lp.chal.data(), 32, o_rho);
+    stg.mark("open_rho");
+    // serialize (general.rs:120-150)
+    std::vector<uint8_t> out;
+    for (auto* r : {cw.root, cabc.root, cem.root, ces.root}) out.insert(out.end(), r, r + 32);
+    put_u64(out, mp.chal.size());
+    for (auto& g : mp.rounds) put_gr(out, g);
+    for (int i = 0; i < 4; i++) put_gr(out, mp.finals[i]);
+    put_gr(out, claim_lin);
+    put_u64(out, lp.chal.size());
+    for (auto& g : lp.rounds) put_gr(out, g);
+    for (int l = 0; l < 32; l++) put_gr(out, lp.finals[32 + l]);  // final_row
+    for (int l = 0; l < 32; l++) put_gr(out, lp.finals[l]);       // final_rho
+    out.insert(out.end(), o_em.begin(), o_em.end());
+    out.insert(out.end(), o_abc.begin(), o_abc.end());
+    out.insert(out.end(), o_es.begin(), o_es.end());
+    out.insert(out.end(), o_rho.begin(), o_rho.end());
+    bool ok = em_val == mp.finals[3] &&
+              abc_val == mp.finals[0] + gam * mp.finals[1] + gam2 * mp.finals[2];
+    for (int l = 0; l < 32; l++) ok = ok && ys[l] == lp.finals[l];
+    if (!ok) throw Error(NENYA_E_DEVICE, "internal opening self-check failed");
+    last_timing = stg.t;
+    return out;
+  }
+
+  std::vector<uint8_t> prove_host(const L* w, size_t n) {
+    if (n != N) throw Error(NENYA_E_SHAPE, "witness length must be 32 * 2^log_blocks");
+    DBuf<L> dw(&pool, N);
+    eng->h2d(dw.p, w, N * sizeof(L));
+    return prove_device(dw.p);
+  }
+};
+
+}  // namespace nenya
