@@ -1,0 +1,146 @@
+#!/usr/bin/env python3
+"""Headline benchmark: prover seconds for a General-model proof over Z/2^32 at
+2^20 R1CS constraints (log_blocks = 15) on one MI355X.
+
+A step is one full proof (prove_general on a witness already resident in
+HBM), returning the serialized proof bytes to the host.  Instance: the
+random half-constrained R1CS of nenya_amd.instances (seed 1), PCS seeds
+0xDEAD / 0xCAFE, delta = 0.19, lambda = 100, rate 1/4, t = 1060.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(k, log_n, seed):
+    """Oracle (C restatement of the reference prover, single thread) timed on
+    a bounded sample: one proof at 2^log_n constraints, scaled linearly to
+    2^20 (the prover is linear in n up to log factors)."""
+    from oracle import load
+    from nenya_amd.instances import general_instance
+    o = load()
+    A, B, C, w = general_instance(log_n, k, seed)
+    t = time.perf_counter()
+    o.general_prove(k, log_n - 5, A, B, C, w)
+    dt = time.perf_counter() - t
+    scale = 2 ** (20 - log_n)
+    return {"value": dt * scale, "unit": "s", "cores": 1, "kind": "port",
+            "sample": f"oracle prove_general at 2^{log_n} u{k} constraints took {dt:.2f} s; scaled x{scale} to 2^20"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--log-n", type=int, default=20)
+    ap.add_argument("--k", type=int, default=32)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-log-n", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "independent proofs shard trivially across GPUs, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import nenya_amd as na
+    from nenya_amd.instances import general_instance
+
+    k, log_n = args.k, args.log_n
+    t0 = time.perf_counter()
+    A, B, C, w = general_instance(log_n, k, args.seed)
+    log(f"instance 2^{log_n} u{k}: nnz A={len(A[0])} B={len(B[0])} C={len(C[0])} ({time.perf_counter() - t0:.1f}s)")
+    t0 = time.perf_counter()
+    params = na.GeneralSnarkParams.secure(log_n - 5, (A, B, C), 0xDEAD, 0xCAFE, k=k)
+    log(f"setup (diagonal packing + PCS params) {time.perf_counter() - t0:.1f}s")
+    dt = {8: torch.uint8, 16: torch.int16, 32: torch.int32, 64: torch.int64}[k]
+    npdt = {8: np.uint8, 16: np.int16, 32: np.int32, 64: np.int64}[k]
+    wd = torch.from_numpy(w.astype({8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}[k]).view(npdt)).to("cuda:0")
+    torch.cuda.synchronize()
+    n = len(w)
+
+    proofs = []
+    for i in range(args.warmup):
+        proofs.append(na.prove_general_device(wd.data_ptr(), n, params))
+        log(f"warmup {i}: proof {len(proofs[-1])} bytes")
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for i in range(args.steps):
+        p = na.prove_general_device(wd.data_ptr(), n, params)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t
+    ms = elapsed / args.steps * 1e3
+    log(f"timed {args.steps} proofs: {ms:.2f} ms/proof")
+    if proofs and p != proofs[0]:
+        raise SystemExit("non-deterministic proof bytes")
+
+    # profiled proof: per-kernel HIP-event durations + stage breakdown
+    params.set_profile(True)
+    params.set_timing(True)
+    na.prove_general_device(wd.data_ptr(), n, params)
+    stats = params.kernel_stats()
+    stages = params.last_timing()
+    params.set_profile(False)
+    params.set_timing(False)
+    for name, s in sorted(stats.items(), key=lambda kv: -kv[1]["ms"]):
+        log(f"  {name:16s} {s['ms']:9.3f} ms  x{s['count']:<5d} {s['bytes'] / max(s['ms'], 1e-9) / 1e6:9.1f} GB/s "
+            f"{s['macs'] / max(s['ms'], 1e-9) / 1e9:9.1f} Gmac/s")
+    for name, v in stages.items():
+        log(f"  stage {name:16s} {v:9.3f} ms")
+    dom_name, dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
+    per_launch_ms = dom["ms"] / dom["count"]
+    per_launch_bytes = dom["bytes"] / dom["count"]
+    achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom_name,
+                "avg_launch_ms": round(per_launch_ms, 4), "launches_per_proof": dom["count"],
+                "valu_gmacs": round(dom["macs"] / dom["ms"] / 1e6, 1)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log(f"cpu baseline: oracle at 2^{args.cpu_log_n} ...")
+        cpu = cpu_baseline(k, args.cpu_log_n, args.seed)
+        log(f"cpu baseline {cpu['value']:.1f} s (scaled)")
+
+    out = {
+        "metric": "prover sec @2^20 u32 constraints; batched proofs/s + p99 vs plain GPU path",
+        "value": round(ms / 1e3, 5),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"u{k}",
+        "data": "synthetic",
+        "config": {"workload": f"General-model proof, Z/2^{k}, 2^{log_n} constraints (log_blocks {log_n - 5}), "
+                               "random R1CS seed 1, witness resident in HBM",
+                   "constraints": 2 ** log_n, "log_blocks": log_n - 5, "k": k, "proof_bytes": len(p),
+                   "proofs_per_s": round(1e3 / ms, 3)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "stages_ms": {k2: round(v, 3) for k2, v in stages.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -80,6 +80,15 @@ int nenya_prove_general_device(const nenya_params* p, const void* d_w, size_t n,
 int nenya_set_timing(nenya_params* p, int enabled);
 int nenya_last_timing(const nenya_params* p, double* ms, int cap, char* names, size_t names_cap);
 
+/* Per-kernel device timing for roofline figures (no reference counterpart):
+ * when enabled, every launch on the params' stream is bracketed by HIP
+ * events.  nenya_kernel_stats returns, per kernel name, the summed duration
+ * (ms), launch count, algorithmic bytes and limb multiply-accumulates over
+ * the proofs made since profiling was (re)enabled. */
+int nenya_set_profile(nenya_params* p, int enabled);
+int nenya_kernel_stats(const nenya_params* p, int cap, char* names, size_t names_cap, double* ms, double* count,
+                       double* bytes, double* macs);
+
 void nenya_params_free(nenya_params* p);
 void nenya_free(void* ptr);
 const char* nenya_last_error(void);

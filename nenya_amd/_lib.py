@@ -37,6 +37,9 @@ def _lib():
         L.nenya_prove_general_device.argtypes = [vp, vp, sz, C.POINTER(vp), C.POINTER(sz)]
         L.nenya_set_timing.argtypes = [vp, C.c_int]
         L.nenya_last_timing.argtypes = [vp, C.POINTER(C.c_double), C.c_int, C.c_char_p, sz]
+        L.nenya_set_profile.argtypes = [vp, C.c_int]
+        dp = C.POINTER(C.c_double)
+        L.nenya_kernel_stats.argtypes = [vp, C.c_int, C.c_char_p, sz, dp, dp, dp, dp]
         L.nenya_params_free.argtypes = [vp]
         L.nenya_free.argtypes = [vp]
         L.nenya_last_error.restype = C.c_char_p
@@ -91,6 +94,18 @@ class _Params:
 
     def set_timing(self, on=True):
         _check(_lib().nenya_set_timing(self._h, 1 if on else 0))
+
+    def set_profile(self, on=True):
+        _check(_lib().nenya_set_profile(self._h, 1 if on else 0))
+
+    def kernel_stats(self):
+        cap = 64
+        arr = [(C.c_double * cap)() for _ in range(4)]
+        names = C.create_string_buffer(8192)
+        n = _lib().nenya_kernel_stats(self._h, cap, names, 8192, *arr)
+        labels = names.value.decode().split("\n")[:n]
+        return {labels[i]: {"ms": arr[0][i], "count": int(arr[1][i]), "bytes": arr[2][i], "macs": arr[3][i]}
+                for i in range(n)}
 
     def last_timing(self):
         ms = (C.c_double * 64)()

@@ -199,6 +199,34 @@ int nenya_set_timing(nenya_params* p, int enabled) {
   p->impl->timing = enabled != 0;
   return NENYA_OK;
 }
+int nenya_set_profile(nenya_params* p, int enabled) {
+  if (!p || !p->impl) return NENYA_E_ARG;
+  p->impl->prof.on = enabled != 0;
+  p->impl->prof.stats.clear();
+  return NENYA_OK;
+}
+int nenya_kernel_stats(const nenya_params* p, int cap, char* names, size_t names_cap, double* ms, double* count,
+                       double* bytes, double* macs) {
+  if (!p || !p->impl) return 0;
+  std::string all;
+  int i = 0;
+  for (const auto& kv : p->impl->prof.stats) {
+    if (i >= cap) break;
+    if (ms) ms[i] = kv.second.ms;
+    if (count) count[i] = (double)kv.second.count;
+    if (bytes) bytes[i] = kv.second.bytes;
+    if (macs) macs[i] = kv.second.macs;
+    all += kv.first;
+    all += '\n';
+    i++;
+  }
+  if (names && names_cap) {
+    size_t c = std::min(all.size(), names_cap - 1);
+    std::memcpy(names, all.data(), c);
+    names[c] = 0;
+  }
+  return i;
+}
 int nenya_last_timing(const nenya_params* p, double* ms, int cap, char* names, size_t names_cap) {
   if (!p || !p->impl) return 0;
   const auto& t = p->impl->last_timing;

@@ -324,6 +324,25 @@ struct Transcript {
 
 // XOF query indices (pcs.rs:1130-1156; r2_pcs.rs:220-246): BLAKE3(domain || root
 // || m-bar limbs), 4-byte LE words masked to next_pow2(n)-1, kept if < n.
+// `root_node` are the inputs of the message's root compression (cv, block,
+// blen, flags), computed on the device; the XOF runs that node with counters.
+inline std::vector<uint32_t> queries_from_root(const uint32_t root_node[26], size_t n, size_t t) {
+  size_t np = 1;
+  while (np < n) np <<= 1;
+  uint32_t mask = (uint32_t)(np - 1);
+  std::vector<uint32_t> out;
+  out.reserve(t);
+  for (uint64_t blk = 0; out.size() < t; blk++) {
+    uint32_t o[16];
+    b3::compress(root_node, root_node + 8, blk, root_node[24], root_node[25] | b3::ROOT, o);
+    for (int i = 0; i < 16 && out.size() < t; i++) {
+      uint32_t v = o[i] & mask;
+      if (v < (uint32_t)n) out.push_back(v);
+    }
+  }
+  return out;
+}
+
 inline std::vector<uint32_t> derive_queries(const char* domain, const uint8_t root[32], const void* mbar_bytes,
                                             size_t nbytes, size_t n, size_t t) {
   b3::Hasher h;

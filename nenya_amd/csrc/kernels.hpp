@@ -754,6 +754,75 @@ __global__ void k_merkle_level(uint32_t* nodes, uint64_t start, uint64_t len) {
   for (int k = 0; k < 8; k++) o[k] = out[k];
 }
 
+// ---- one long message (Fiat-Shamir query derivation over m-bar) ----
+// Chunk chaining values of a byte message (len bytes; bytes past len read as 0
+// from a padded buffer).  Thread per 1024-byte chunk; for a one-chunk message
+// the root node inputs are written instead (cv, block, blen, flags).
+__global__ void k_b3_chunks(const uint8_t* msg, uint64_t len, uint64_t nchunks, uint32_t* cvs, uint32_t* rootnode) {
+  uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nchunks) return;
+  uint64_t cbeg = c * 1024;
+  uint64_t clen = len - cbeg < 1024 ? len - cbeg : 1024;
+  uint32_t nblk = clen == 0 ? 1 : (uint32_t)((clen + 63) / 64);
+  uint32_t cv[8], m[16], o[16];
+  b3::iv(cv);
+  auto word = [&](uint64_t byte) -> uint32_t {
+    uint32_t w = 0;
+    for (int i = 0; i < 4; i++) {
+      uint64_t p = byte + i;
+      uint32_t v = p < len ? msg[p] : 0u;
+      w |= v << (8 * i);
+    }
+    return w;
+  };
+  for (uint32_t b = 0; b + 1 < nblk; b++) {
+    for (int i = 0; i < 16; i++) m[i] = word(cbeg + 64 * (uint64_t)b + 4 * i);
+    b3::compress(cv, m, c, 64, b == 0 ? b3::CHUNK_START : 0, o);
+    for (int i = 0; i < 8; i++) cv[i] = o[i];
+  }
+  uint32_t lb = nblk - 1;
+  uint32_t blen = (uint32_t)(clen - 64 * (uint64_t)lb);
+  for (int i = 0; i < 16; i++) m[i] = word(cbeg + 64 * (uint64_t)lb + 4 * i);
+  uint32_t fl = (lb == 0 ? b3::CHUNK_START : 0) | b3::CHUNK_END;
+  if (nchunks == 1) {
+    for (int i = 0; i < 8; i++) rootnode[i] = cv[i];
+    for (int i = 0; i < 16; i++) rootnode[8 + i] = m[i];
+    rootnode[24] = blen;
+    rootnode[25] = fl;
+    return;
+  }
+  b3::compress(cv, m, c, blen, fl, o);
+  for (int i = 0; i < 8; i++) cvs[c * 8 + i] = o[i];
+}
+// One merge level of the left-complete BLAKE3 tree: pairs become parents, an
+// odd last node is carried up unchanged.
+__global__ void k_b3_merge(const uint32_t* in, uint64_t cnt, uint32_t* out) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t half = cnt / 2;
+  if (i < half) {
+    uint32_t key[8], m[16], o[16];
+    b3::iv(key);
+    for (int k = 0; k < 8; k++) { m[k] = in[(2 * i) * 8 + k]; m[8 + k] = in[(2 * i + 1) * 8 + k]; }
+    b3::compress(key, m, 0, 64, b3::PARENT, o);
+    for (int k = 0; k < 8; k++) out[i * 8 + k] = o[k];
+  } else if (i == half && (cnt & 1)) {
+    for (int k = 0; k < 8; k++) out[i * 8 + k] = in[(cnt - 1) * 8 + k];
+  }
+}
+
+// Merkle authentication paths: out[q][lvl] = sibling of idx[q] at level lvl.
+__global__ void k_merkle_paths(const uint32_t* nodes, uint64_t n, const uint32_t* idx, uint64_t nq, int plen,
+                               uint32_t* out) {
+  uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nq * plen) return;
+  uint64_t q = id / plen;
+  int lvl = (int)(id % plen);
+  uint64_t start = 0, len = n;
+  for (int l = 0; l < lvl; l++) { start += len; len >>= 1; }
+  uint64_t node = start + ((idx[q] >> lvl) ^ 1);
+  for (int k = 0; k < 8; k++) out[id * 8 + k] = nodes[node * 8 + k];
+}
+
 // out[q][i][:] = enc[i][idx[q]][:]   (W limbs per entry)
 template <class T>
 __global__ void k_gather_cols(const LT<T>* enc, const uint32_t* idx, uint64_t nq, uint64_t rows, uint64_t n, int W,

@@ -7,6 +7,7 @@
 // the reference's bytes (see the identities cited at each step).
 #pragma once
 #include <chrono>
+#include <map>
 
 #include "host.hpp"
 #include "kernels.hpp"
@@ -38,6 +39,31 @@ struct Stages {
     auto now = std::chrono::steady_clock::now();
     t.push_back({name, std::chrono::duration<double, std::milli>(now - last).count()});
     last = now;
+  }
+};
+
+// Per-kernel HIP-event timing with the algorithmic bytes and limb-MACs of
+// each launch (for roofline figures); off unless enabled on the params.
+struct KProf {
+  bool on = false;
+  struct Rec { std::string name; hipEvent_t a, b; double bytes, macs; };
+  struct Stat { double ms = 0, bytes = 0, macs = 0; long count = 0; };
+  std::vector<Rec> recs;
+  std::map<std::string, Stat> stats;
+  void collect() {
+    for (auto& r : recs) {
+      (void)hipEventSynchronize(r.b);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, r.a, r.b);
+      Stat& s = stats[r.name];
+      s.ms += ms;
+      s.bytes += r.bytes;
+      s.macs += r.macs;
+      s.count++;
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    recs.clear();
   }
 };
 
@@ -83,7 +109,26 @@ template <class T> struct Engine {
   const Consts<T>& K;
   DBuf<L> d_basis, d_pw;
   DBuf<L> d_one;
-  Stages* stages = nullptr;
+  KProf* prof = nullptr;
+
+  // Launch on the engine's stream, bracketed by events when profiling.
+  template <class Kern, class... Args>
+  void launch(const char* name, double bytes, double macs, Kern kern, dim3 grid, dim3 block, Args... args) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (prof && prof->on) {
+      HIP_CHECK(hipEventCreate(&a));
+      HIP_CHECK(hipEventCreate(&b));
+      HIP_CHECK(hipEventRecord(a, st));
+    }
+    hipLaunchKernelGGL(kern, grid, block, 0, st, args...);
+    HIP_CHECK(hipGetLastError());
+    if (prof && prof->on) {
+      HIP_CHECK(hipEventRecord(b, st));
+      prof->recs.push_back({name, a, b, bytes, macs});
+    }
+  }
+  static constexpr double EB = E * sizeof(typename T::L);  // bytes per element
+  static constexpr double PM = 512.0;                      // limb-MACs per beta term of a group product
 
   Engine(Pool* p, hipStream_t s) : pool(p), st(s), K(Consts<T>::get()) {
     d_basis = DBuf<L>(pool, 32 * E);
@@ -120,45 +165,40 @@ template <class T> struct Engine {
 
   // ---------------- element kernels ----------------
   void pack(const L* slots, L* out, uint64_t n, int nrot, int rot0 = 0) {
-    hipLaunchKernelGGL((k_pack<T>), dim3(grid_for(n * nrot, GPB)), dim3(GB), 0, st, slots, d_basis.p, out, n, nrot, rot0);
-    check();
+    launch("k_pack", n * nrot * (32.0 * sizeof(L) + EB), n * nrot * 4096.0, (k_pack<T>), dim3(grid_for(n * nrot, GPB)),
+           dim3(GB), slots, d_basis.p, out, n, nrot, rot0);
   }
   void psi(const L* x, L* slots, uint64_t n) {
-    hipLaunchKernelGGL((k_psi<T>), dim3(grid_for(n, GPB)), dim3(GB), 0, st, x, d_pw.p, slots, n);
-    check();
+    launch("k_psi", n * (EB + 32.0 * sizeof(L)), n * 16 * 32 * 16.0, (k_psi<T>), dim3(grid_for(n, GPB)), dim3(GB), x,
+           d_pw.p, slots, n);
   }
   template <int NA> void mul_sub(const L* A, const L* B, const L* S, L* out, uint64_t n) {
-    hipLaunchKernelGGL((k_mul_sub<T, NA>), dim3(grid_for(n, GPB)), dim3(GB), 0, st, A, B, S, out, n);
-    check();
+    launch("k_mul_sub", n * 4 * EB, n * NA * PM, (k_mul_sub<T, NA>), dim3(grid_for(n, GPB)), dim3(GB), A, B, S, out, n);
   }
   void scale(const L* s, const L* X, L* out, uint64_t n) {
-    hipLaunchKernelGGL((k_scale<T>), dim3(grid_for(n, GPB)), dim3(GB), 0, st, s, X, out, n);
-    check();
+    launch("k_scale", n * 2 * EB, n * 32 * PM, (k_scale<T>), dim3(grid_for(n, GPB)), dim3(GB), s, X, out, n);
   }
   void sub(const L* a, const L* b, L* o, uint64_t nl) {
-    hipLaunchKernelGGL((k_sub<T>), dim3(grid_for(nl, 256, 1u << 20)), dim3(256), 0, st, a, b, o, nl);
-    check();
+    launch("k_sub", nl * 3.0 * sizeof(L), 0, (k_sub<T>), dim3(grid_for(nl, 256, 1u << 20)), dim3(256), a, b, o, nl);
   }
   void fold(const L* r, const L* src, L* dst, uint64_t m, uint64_t ntab, uint64_t sstride, uint64_t dstride) {
-    hipLaunchKernelGGL((k_fold<T>), dim3(grid_for(m * ntab, GPB)), dim3(GB), 0, st, r, src, dst, m, ntab, sstride, dstride);
-    check();
+    launch("k_fold", m * ntab * 3 * EB, m * ntab * 32 * PM, (k_fold<T>), dim3(grid_for(m * ntab, GPB)), dim3(GB), r, src,
+           dst, m, ntab, sstride, dstride);
   }
   void marg(const L* s, L* eg, uint64_t m) {
-    hipLaunchKernelGGL((k_marg<T>), dim3(grid_for(m, GPB)), dim3(GB), 0, st, s, eg, m);
-    check();
+    launch("k_marg", m * 3 * EB, m * 32 * PM, (k_marg<T>), dim3(grid_for(m, GPB)), dim3(GB), s, eg, m);
   }
   // sum of partial sums -> ns elements on device
   DBuf<L> reduce(const DBuf<L>& partials, unsigned nblocks, int ns) {
     DBuf<L> out(pool, ns * E);
-    hipLaunchKernelGGL((k_sum_partials<T>), dim3(ns), dim3(E), 0, st, partials.p, (uint64_t)nblocks, ns, out.p);
-    check();
+    launch("k_sum_partials", (double)nblocks * ns * EB, 0, (k_sum_partials<T>), dim3(ns), dim3(E), partials.p,
+           (uint64_t)nblocks, ns, out.p);
     return out;
   }
   template <int NA> G dot(const L* A, const L* B, uint64_t n) {
     unsigned nb = grid_for(n, GPB, 2048);
     DBuf<L> part(pool, (size_t)nb * E);
-    hipLaunchKernelGGL((k_dot<T, NA>), dim3(nb), dim3(GB), 0, st, A, B, n, part.p);
-    check();
+    launch("k_dot", n * 2 * EB, n * NA * PM, (k_dot<T, NA>), dim3(nb), dim3(GB), A, B, n, part.p);
     DBuf<L> r = reduce(part, nb, 1);
     return get(r.p);
   }
@@ -174,8 +214,8 @@ template <class T> struct Engine {
     L* next = b.p;
     size_t cur = 1;
     for (size_t l = 0; l < n; l++) {
-      hipLaunchKernelGGL((k_eq_level<T>), dim3(grid_for(cur, GPB)), dim3(GB), 0, st, rr.p + l * E, prev, next, (uint64_t)cur);
-      check();
+      launch("k_eq_level", cur * 3 * EB, cur * 32 * PM, (k_eq_level<T>), dim3(grid_for(cur, GPB)), dim3(GB), rr.p + l * E,
+             prev, next, (uint64_t)cur);
       std::swap(prev, next);
       cur *= 2;
     }
@@ -191,8 +231,7 @@ template <class T> struct Engine {
     for (size_t i = 0; i < cnt; i++)
       for (int l = 0; l < E; l++) nat[i].v[l] = (L)rng.limb(T::K);
     DBuf<L> d = put(nat.data(), cnt);
-    hipLaunchKernelGGL((k_lift<T>), dim3(cnt), dim3(256), 0, st, d.p, 128);
-    check();
+    launch("k_lift", cnt * 2 * EB, cnt * 128 * 32 * PM, (k_lift<T>), dim3(cnt), dim3(256), d.p, 128);
     d2h(nat.data(), d.p, cnt * sizeof(G));
     if (dev_out) *dev_out = std::move(d);
     return nat;
@@ -214,23 +253,22 @@ template <class T> struct Engine {
     c.msg = lifted;
     DBuf<L> tmp(pool, rows * P.n * W);
     c.enc = DBuf<L>(pool, rows * P.n * W);
+    // algorithmic bytes: read the k2-entry row, write/read t, write/read z (two passes)
+    double rb = (double)rows * W * sizeof(L) * (P.k2 + 4.0 * P.n);
     if (P.packed)
-      hipLaunchKernelGGL((k_raa<T, 32>), dim3(rows), dim3(1024), 0, st, rowsbuf, (uint64_t)P.k2, pc.perm1.p, pc.perm2.p,
-                         pc.d1.p, pc.d2.p, (uint64_t)P.q, (uint64_t)P.n, tmp.p, c.enc.p);
+      launch("k_raa", rb, 0, (k_raa<T, 32>), dim3(rows), dim3(1024), rowsbuf, (uint64_t)P.k2, pc.perm1.p, pc.perm2.p,
+             pc.d1.p, pc.d2.p, (uint64_t)P.q, (uint64_t)P.n, tmp.p, c.enc.p);
     else
-      hipLaunchKernelGGL((k_raa<T, E>), dim3(rows), dim3(1024), 0, st, rowsbuf, (uint64_t)P.k2, pc.perm1.p, pc.perm2.p,
-                         pc.d1.p, pc.d2.p, (uint64_t)P.q, (uint64_t)P.n, tmp.p, c.enc.p);
-    check();
+      launch("k_raa", rb, 0, (k_raa<T, E>), dim3(rows), dim3(1024), rowsbuf, (uint64_t)P.k2, pc.perm1.p, pc.perm2.p,
+             pc.d1.p, pc.d2.p, (uint64_t)P.q, (uint64_t)P.n, tmp.p, c.enc.p);
     size_t n = P.n;  // a power of two
     c.nodes = DBuf<uint32_t>(pool, (2 * n - 1) * 8);
-    hipLaunchKernelGGL(k_hash_cols, dim3(grid_for(n, 64)), dim3(64), 0, st, (const uint8_t*)c.enc.p, (uint64_t)rows,
-                       (uint64_t)n, (uint64_t)W * sizeof(L), c.nodes.p);
-    check();
+    launch("k_hash_cols", (double)rows * n * W * sizeof(L), 0, k_hash_cols, dim3(grid_for(n, 64)), dim3(64),
+           (const uint8_t*)c.enc.p, (uint64_t)rows, (uint64_t)n, (uint64_t)W * sizeof(L), c.nodes.p);
     size_t start = 0, len = n;
     while (len > 1) {
-      hipLaunchKernelGGL(k_merkle_level, dim3(grid_for(len / 2, 128)), dim3(128), 0, st, c.nodes.p, (uint64_t)start,
-                         (uint64_t)len);
-      check();
+      launch("k_merkle_level", len * 32.0 + len * 16.0, 0, k_merkle_level, dim3(grid_for(len / 2, 128)), dim3(128),
+             c.nodes.p, (uint64_t)start, (uint64_t)len);
       start += len;
       len /= 2;
     }
@@ -238,45 +276,80 @@ template <class T> struct Engine {
     return c;
   }
 
-  // Shared tail of every opening: query indices, columns, paths.
-  void open_tail(const PcsDev<T>& pc, const CommitDev<T>& c, const std::vector<uint8_t>& mbar_bytes,
-                 std::vector<uint8_t>& out) {
+  // BLAKE3 root node of prefix || device bytes, hashed on the device
+  // (chunk chaining values in parallel, then the left-complete merge levels).
+  void hash_root_node(const uint8_t* prefix, size_t plen, const void* dbytes, size_t dlen, uint32_t node[26]) {
+    size_t len = plen + dlen;
+    DBuf<uint8_t> msg(pool, len + 64);
+    h2d(msg.p, prefix, plen);
+    d2d(msg.p + plen, dbytes, dlen);
+    uint64_t nch = len == 0 ? 1 : (len + 1023) / 1024;
+    DBuf<uint32_t> cv(pool, nch * 8), cv2(pool, (nch / 2 + 1) * 8), rn(pool, 32);
+    launch("k_b3_chunks", (double)len, 0, k_b3_chunks, dim3(grid_for(nch, 64, 1u << 20)), dim3(64), (const uint8_t*)msg.p,
+           (uint64_t)len, nch, cv.p, rn.p);
+    if (nch == 1) {
+      d2h(node, rn.p, 26 * 4);
+      return;
+    }
+    uint32_t* a = cv.p;
+    uint32_t* b = cv2.p;
+    uint64_t cnt = nch;
+    while (cnt > 2) {
+      launch("k_b3_merge", cnt * 32.0, 0, k_b3_merge, dim3(grid_for(cnt / 2 + 1, 64, 1u << 20)), dim3(64),
+             (const uint32_t*)a, cnt, b);
+      std::swap(a, b);
+      cnt = (cnt + 1) / 2;
+    }
+    uint32_t kids[16];
+    d2h(kids, a, 64);
+    b3::iv(node);
+    for (int i = 0; i < 16; i++) node[8 + i] = kids[i];
+    node[24] = 64;
+    node[25] = b3::PARENT;
+  }
+
+  // Shared tail of every opening: query indices (derived from BLAKE3 of
+  // domain || root || m-bar), columns, Merkle paths.
+  void open_tail(const PcsDev<T>& pc, const CommitDev<T>& c, const L* d_mbar, size_t nmbar, std::vector<uint8_t>& out) {
     const PcsParams& P = pc.P;
     const int W = P.packed ? 32 : E;
     const char* dom = P.packed ? "pcs-ring/fs/queries" : "snark-ring/r2-pcs/fs/queries";
-    std::vector<uint32_t> idx = derive_queries(dom, c.root, mbar_bytes.data(), mbar_bytes.size(), P.n, P.ell);
+    std::vector<uint8_t> prefix(dom, dom + std::strlen(dom));
+    prefix.insert(prefix.end(), c.root, c.root + 32);
+    uint32_t node[26];
+    hash_root_node(prefix.data(), prefix.size(), d_mbar, nmbar * E * sizeof(L), node);
+    std::vector<uint32_t> idx = queries_from_root(node, P.n, P.ell);
     put_u64(out, idx.size());
     for (uint32_t j : idx) put_u32(out, j);
     DBuf<uint32_t> didx(pool, idx.size());
     h2d(didx.p, idx.data(), idx.size() * 4);
     size_t per = c.rows * W;
     DBuf<L> cols(pool, idx.size() * per);
-    hipLaunchKernelGGL((k_gather_cols<T>), dim3(grid_for(idx.size() * per, 256, 1u << 20)), dim3(256), 0, st, c.enc.p,
-                       didx.p, (uint64_t)idx.size(), (uint64_t)c.rows, (uint64_t)P.n, W, cols.p);
-    check();
+    launch("k_gather_cols", idx.size() * per * 2.0 * sizeof(L), 0, (k_gather_cols<T>),
+           dim3(grid_for(idx.size() * per, 256, 1u << 20)), dim3(256), (const L*)c.enc.p, (const uint32_t*)didx.p,
+           (uint64_t)idx.size(), (uint64_t)c.rows, (uint64_t)P.n, W, cols.p);
+    size_t n = P.n;
+    int plen = __builtin_ctzll(n);
+    DBuf<uint32_t> paths(pool, std::max<size_t>(idx.size() * plen, 1) * 8);
+    if (plen > 0)
+      launch("k_merkle_paths", 0, 0, k_merkle_paths, dim3(grid_for(idx.size() * plen, 64, 1u << 20)), dim3(64),
+             (const uint32_t*)c.nodes.p, (uint64_t)n, (const uint32_t*)didx.p, (uint64_t)idx.size(), plen, paths.p);
     std::vector<L> hc(idx.size() * per);
-    d2h(hc.data(), cols.p, hc.size() * sizeof(L));
+    std::vector<uint32_t> hp(idx.size() * plen * 8);
+    HIP_CHECK(hipMemcpyAsync(hc.data(), cols.p, hc.size() * sizeof(L), hipMemcpyDeviceToHost, st));
+    if (!hp.empty()) HIP_CHECK(hipMemcpyAsync(hp.data(), paths.p, hp.size() * 4, hipMemcpyDeviceToHost, st));
+    sync();
     put_u64(out, idx.size());
     for (size_t q = 0; q < idx.size(); q++) {
       put_u64(out, c.rows);
       const uint8_t* b = reinterpret_cast<const uint8_t*>(&hc[q * per]);
       out.insert(out.end(), b, b + per * sizeof(L));
     }
-    size_t n = P.n;
-    std::vector<uint32_t> nodes((2 * n - 1) * 8);
-    d2h(nodes.data(), c.nodes.p, nodes.size() * 4);
-    size_t plen = (size_t)__builtin_ctzll(n);
     put_u64(out, idx.size());
-    for (uint32_t j : idx) {
-      put_u64(out, plen);
-      size_t id = j, start = 0, len = n;
-      while (len > 1) {
-        const uint8_t* h = reinterpret_cast<const uint8_t*>(&nodes[(start + (id ^ 1)) * 8]);
-        out.insert(out.end(), h, h + 32);
-        id /= 2;
-        start += len;
-        len /= 2;
-      }
+    for (size_t q = 0; q < idx.size(); q++) {
+      put_u64(out, (uint64_t)plen);
+      const uint8_t* b = reinterpret_cast<const uint8_t*>(&hp[q * plen * 8]);
+      out.insert(out.end(), b, b + (size_t)plen * 32);
     }
   }
 
@@ -296,19 +369,19 @@ template <class T> struct Engine {
     }
     DBuf<L> mbar(pool, P.k2 * E);
     uint64_t rows = s * P.k1;
+    double mbb = (rows * P.k2 + P.k2) * EB;
     if (P.packed)
-      hipLaunchKernelGGL((k_mbar<T, 16>), dim3(grid_for(P.k2, GPB)), dim3(GB), 0, st, e0s.p, c.msg, rows, (uint64_t)P.k2,
-                         mbar.p, (uint64_t)0);
+      launch("k_mbar", mbb, rows * P.k2 * 16 * PM, (k_mbar<T, 16>), dim3(grid_for(P.k2, GPB)), dim3(GB), (const L*)e0s.p,
+             c.msg, rows, (uint64_t)P.k2, mbar.p, (uint64_t)0);
     else
-      hipLaunchKernelGGL((k_mbar<T, 32>), dim3(grid_for(P.k2, GPB)), dim3(GB), 0, st, e0s.p, c.msg, rows, (uint64_t)P.k2,
-                         mbar.p, (uint64_t)0);
-    check();
+      launch("k_mbar", mbb, rows * P.k2 * 32 * PM, (k_mbar<T, 32>), dim3(grid_for(P.k2, GPB)), dim3(GB), (const L*)e0s.p,
+             c.msg, rows, (uint64_t)P.k2, mbar.p, (uint64_t)0);
     G y = dot<32>(mbar.p, e1.p, P.k2);
-    std::vector<uint8_t> mb(P.k2 * E * sizeof(L));
-    d2h(mb.data(), mbar.p, mb.size());
+    size_t o0 = out.size();
     put_u64(out, P.k2);
-    out.insert(out.end(), mb.begin(), mb.end());
-    open_tail(pc, c, mb, out);
+    out.resize(out.size() + P.k2 * E * sizeof(L));
+    d2h(out.data() + o0 + 8, mbar.p, P.k2 * E * sizeof(L));
+    open_tail(pc, c, mbar.p, P.k2, out);
     return y;
   }
 
@@ -320,20 +393,22 @@ template <class T> struct Engine {
     size_t lk1 = (size_t)__builtin_ctzll(P.k1), lk2 = (size_t)__builtin_ctzll(P.k2);
     DBuf<L> e0 = eq_table(z, lk1), e1 = eq_table(z + lk1, lk2);
     DBuf<L> mbar(pool, (size_t)nl * P.k2 * E);
-    hipLaunchKernelGGL((k_rot_mbar<T>), dim3(grid_for((uint64_t)nl * P.k2, GPB)), dim3(GB), 0, st, e0.p, rho, nb,
-                       (uint64_t)P.k1, (uint64_t)P.k2, nl, mbar.p);
-    check();
+    launch("k_rot_mbar", (double)nl * P.k1 * P.k2 * EB, (double)nl * P.k1 * P.k2 * 16 * PM, (k_rot_mbar<T>),
+           dim3(grid_for((uint64_t)nl * P.k2, GPB)), dim3(GB), (const L*)e0.p, rho, nb, (uint64_t)P.k1, (uint64_t)P.k2, nl,
+           mbar.p);
     std::vector<G> ys(nl);
     for (int l = 0; l < nl; l++) ys[l] = dot<32>(mbar.p + (size_t)l * P.k2 * E, e1.p, P.k2);
-    std::vector<uint8_t> mb((size_t)nl * P.k2 * E * sizeof(L));
-    d2h(mb.data(), mbar.p, mb.size());
     put_u64(out, nl);
     size_t per = P.k2 * E * sizeof(L);
+    size_t o0 = out.size();
+    out.resize(o0 + (size_t)nl * (8 + per));  // sized once: async copies below target it
     for (int l = 0; l < nl; l++) {
-      put_u64(out, P.k2);
-      out.insert(out.end(), mb.begin() + l * per, mb.begin() + (l + 1) * per);
+      uint8_t* dst = out.data() + o0 + (size_t)l * (8 + per);
+      for (int i = 0; i < 8; i++) dst[i] = (uint8_t)((uint64_t)P.k2 >> (8 * i));
+      HIP_CHECK(hipMemcpyAsync(dst + 8, mbar.p + (size_t)l * P.k2 * E, per, hipMemcpyDeviceToHost, st));
     }
-    open_tail(pc, c, mb, out);
+    sync();
+    open_tail(pc, c, mbar.p, (size_t)nl * P.k2, out);
     return ys;
   }
 
@@ -361,9 +436,9 @@ template <class T> struct Engine {
       size_t m = len / 2;
       unsigned nbk = grid_for(m, GPB, 2048);
       DBuf<L> part(pool, (size_t)nbk * 3 * E);
-      hipLaunchKernelGGL((k_mult_round<T>), dim3(nbk), dim3(GB), 0, st, eg.p, cur[0], cur[1], ntab > 2 ? cur[2] : nullptr,
-                         ntab > 2 ? cur[3] : nullptr, (uint64_t)m, part.p);
-      check();
+      launch("k_mult_round", m * (2.0 * ntab + 1) * EB, m * 6 * 32 * PM, (k_mult_round<T>), dim3(nbk), dim3(GB),
+             (const L*)eg.p, cur[0], cur[1], ntab > 2 ? cur[2] : (const L*)nullptr, ntab > 2 ? cur[3] : (const L*)nullptr,
+             (uint64_t)m, part.p);
       DBuf<L> s = reduce(part, nbk, 3);
       G sv[3];
       d2h(sv, s.p, sizeof sv);
@@ -409,11 +484,13 @@ template <class T> struct Engine {
       size_t m = len / 2;
       unsigned nbk = grid_for(m * ntab, GPB, 2048);
       DBuf<L> part(pool, (size_t)nbk * 2 * E);
+      double lb = m * ntab * 4.0 * EB;
       if (r == 0 && NAF0 == 16)
-        hipLaunchKernelGGL((k_lin_round<T, 16>), dim3(nbk), dim3(GB), 0, st, f, fs, g, gs, (uint64_t)ntab, (uint64_t)m, part.p);
+        launch("k_lin_round", lb, m * ntab * 2 * 16 * PM, (k_lin_round<T, 16>), dim3(nbk), dim3(GB), f, fs, g, gs,
+               (uint64_t)ntab, (uint64_t)m, part.p);
       else
-        hipLaunchKernelGGL((k_lin_round<T, 32>), dim3(nbk), dim3(GB), 0, st, f, fs, g, gs, (uint64_t)ntab, (uint64_t)m, part.p);
-      check();
+        launch("k_lin_round", lb, m * ntab * 2 * 32 * PM, (k_lin_round<T, 32>), dim3(nbk), dim3(GB), f, fs, g, gs,
+               (uint64_t)ntab, (uint64_t)m, part.p);
       DBuf<L> s = reduce(part, nbk, 2);
       G sv[2];
       d2h(sv, s.p, sizeof sv);

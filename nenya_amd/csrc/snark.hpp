@@ -13,6 +13,7 @@ struct ParamsBase {
   int kind = 0;  // 0 simd, 1 general
   bool timing = false;
   std::vector<std::pair<std::string, double>> last_timing;
+  KProf prof;
   virtual ~ParamsBase() = default;
 };
 
@@ -93,6 +94,7 @@ template <class T> struct SimdParams : ParamsBase {
 
   std::vector<uint8_t> prove(const HostCoo& A, const HostCoo& B, const HostCoo& Cm, const L* wit /*32 x m*/) {
     Engine<T>& e = *eng;
+    e.prof = &prof;
     size_t n = size_t(1) << log_n, m = size_t(1) << log_m;
     check_coo(A, n, m);
     check_coo(B, n, m);
@@ -125,12 +127,14 @@ template <class T> struct SimdParams : ParamsBase {
     e.pack(slots.p, wphi.p, m, 1);
     DBuf<L> xs(&pool, n * 32), ys(&pool, n * 32), xy(&pool, n * 32);
     unsigned g0 = grid_for(n * 32, 256, 1u << 20);
-    hipLaunchKernelGGL((k_r0_matvec<T>), dim3(g0), dim3(256), 0, st, ar.ptr.p, ar.idx.p, ar.val.p, dw.p, (uint64_t)m, xs.p,
-                       (uint64_t)n, 32, (uint64_t)32, (uint64_t)1);
-    hipLaunchKernelGGL((k_r0_matvec<T>), dim3(g0), dim3(256), 0, st, br.ptr.p, br.idx.p, br.val.p, dw.p, (uint64_t)m, ys.p,
-                       (uint64_t)n, 32, (uint64_t)32, (uint64_t)1);
-    hipLaunchKernelGGL((k_hadamard<T>), dim3(g0), dim3(256), 0, st, xs.p, ys.p, xy.p, (uint64_t)n * 32);
-    e.check();
+    e.launch("k_r0_matvec", 0, 0, (k_r0_matvec<T>), dim3(g0), dim3(256), (const uint32_t*)ar.ptr.p,
+             (const uint32_t*)ar.idx.p, (const L*)ar.val.p, (const L*)dw.p, (uint64_t)m, xs.p, (uint64_t)n, 32, (uint64_t)32,
+             (uint64_t)1);
+    e.launch("k_r0_matvec", 0, 0, (k_r0_matvec<T>), dim3(g0), dim3(256), (const uint32_t*)br.ptr.p,
+             (const uint32_t*)br.idx.p, (const L*)br.val.p, (const L*)dw.p, (uint64_t)m, ys.p, (uint64_t)n, 32, (uint64_t)32,
+             (uint64_t)1);
+    e.launch("k_hadamard", 0, 0, (k_hadamard<T>), dim3(g0), dim3(256), (const L*)xs.p, (const L*)ys.p, xy.p,
+             (uint64_t)n * 32);
     DBuf<L> px(&pool, n * E), py(&pool, n * E), ev(&pool, n * E);
     e.pack(xs.p, px.p, n, 1);
     e.pack(ys.p, py.p, n, 1);
@@ -150,10 +154,9 @@ template <class T> struct SimdParams : ParamsBase {
     // 4. a_phi, b_phi, c_phi (matrix.rs:93-105)
     DBuf<L> aw(&pool, n * E), bw(&pool, n * E), cwv(&pool, n * E);
     unsigned gs = grid_for(n, GPB);
-    hipLaunchKernelGGL((k_spmv_gr1<T>), dim3(gs), dim3(GB), 0, st, ar.ptr.p, ar.idx.p, ar.val.p, wphi.p, aw.p, (uint64_t)n, 0);
-    hipLaunchKernelGGL((k_spmv_gr1<T>), dim3(gs), dim3(GB), 0, st, br.ptr.p, br.idx.p, br.val.p, wphi.p, bw.p, (uint64_t)n, 0);
-    hipLaunchKernelGGL((k_spmv_gr1<T>), dim3(gs), dim3(GB), 0, st, cr.ptr.p, cr.idx.p, cr.val.p, wphi.p, cwv.p, (uint64_t)n, 0);
-    e.check();
+    e.launch("k_spmv_gr1", 0, 0, (k_spmv_gr1<T>), dim3(gs), dim3(GB), (const uint32_t*)ar.ptr.p, (const uint32_t*)ar.idx.p, (const L*)ar.val.p, (const L*)wphi.p, aw.p, (uint64_t)n, 0);
+    e.launch("k_spmv_gr1", 0, 0, (k_spmv_gr1<T>), dim3(gs), dim3(GB), (const uint32_t*)br.ptr.p, (const uint32_t*)br.idx.p, (const L*)br.val.p, (const L*)wphi.p, bw.p, (uint64_t)n, 0);
+    e.launch("k_spmv_gr1", 0, 0, (k_spmv_gr1<T>), dim3(gs), dim3(GB), (const uint32_t*)cr.ptr.p, (const uint32_t*)cr.idx.p, (const L*)cr.val.p, (const L*)wphi.p, cwv.p, (uint64_t)n, 0);
     // 5. claim_mult = c_phi(r) + e(r)
     DBuf<L> eq_r = e.eq_table(rt.data(), log_n);
     G cw_at_r = e.template dot<32>(eq_r.p, cwv.p, n);
@@ -179,10 +182,9 @@ template <class T> struct SimdParams : ParamsBase {
     e.scale(dg.p, eq_s1.p, geq.p, n);
     e.scale(dg2.p, eq_r.p, g2eq.p, n);
     unsigned gm = grid_for(m, GPB);
-    hipLaunchKernelGGL((k_spmv_gr1<T>), dim3(gm), dim3(GB), 0, st, ac.ptr.p, ac.idx.p, ac.val.p, eq_s1.p, mrow.p, (uint64_t)m, 0);
-    hipLaunchKernelGGL((k_spmv_gr1<T>), dim3(gm), dim3(GB), 0, st, bc.ptr.p, bc.idx.p, bc.val.p, geq.p, mrow.p, (uint64_t)m, 1);
-    hipLaunchKernelGGL((k_spmv_gr1<T>), dim3(gm), dim3(GB), 0, st, cc.ptr.p, cc.idx.p, cc.val.p, g2eq.p, mrow.p, (uint64_t)m, 1);
-    e.check();
+    e.launch("k_spmv_gr1", 0, 0, (k_spmv_gr1<T>), dim3(gm), dim3(GB), (const uint32_t*)ac.ptr.p, (const uint32_t*)ac.idx.p, (const L*)ac.val.p, (const L*)eq_s1.p, mrow.p, (uint64_t)m, 0);
+    e.launch("k_spmv_gr1", 0, 0, (k_spmv_gr1<T>), dim3(gm), dim3(GB), (const uint32_t*)bc.ptr.p, (const uint32_t*)bc.idx.p, (const L*)bc.val.p, (const L*)geq.p, mrow.p, (uint64_t)m, 1);
+    e.launch("k_spmv_gr1", 0, 0, (k_spmv_gr1<T>), dim3(gm), dim3(GB), (const uint32_t*)cc.ptr.p, (const uint32_t*)cc.idx.p, (const L*)cc.val.p, (const L*)g2eq.p, mrow.p, (uint64_t)m, 1);
     // 9. claim_lin
     G claim_lin = mp.finals[0] + gam * mp.finals[1] + gam2 * cw_at_r;
     tr.elems("claim_lin", &claim_lin, 1);
@@ -213,6 +215,7 @@ template <class T> struct SimdParams : ParamsBase {
     stg.mark("open");
     if (!(yw == final_w) || !(ye == e_at_r)) throw Error(NENYA_E_DEVICE, "internal opening self-check failed");
     last_timing = stg.t;
+    if (prof.on) prof.collect();
     return out;
   }
 };
@@ -360,6 +363,7 @@ template <class T> struct GeneralParams : ParamsBase {
 
   std::vector<uint8_t> prove_device(const L* w /* device, N limbs */) {
     Engine<T>& e = *eng;
+    e.prof = &prof;
     Stages stg;
     stg.on = timing;
     stg.st = st;
@@ -372,15 +376,16 @@ template <class T> struct GeneralParams : ParamsBase {
     DBuf<L> abc_slots(&pool, 3 * N);
     unsigned g0 = grid_for(N, 256, 1u << 20);
     for (int m = 0; m < 2; m++)
-      hipLaunchKernelGGL((k_r0_matvec<T>), dim3(g0), dim3(256), 0, st, mr[m].ptr.p, mr[m].idx.p, mr[m].val.p, w, (uint64_t)0,
-                         abc_slots.p + m * N, (uint64_t)N, 1, (uint64_t)1, (uint64_t)0);
-    e.check();
+      e.launch("k_r0_matvec", 0, 0, (k_r0_matvec<T>), dim3(g0), dim3(256), (const uint32_t*)mr[m].ptr.p,
+               (const uint32_t*)mr[m].idx.p, (const L*)mr[m].val.p, w, (uint64_t)0, abc_slots.p + m * N, (uint64_t)N, 1,
+               (uint64_t)1, (uint64_t)0);
     // 3. layered matvecs sum_l D_l rho_l (general.rs:211-230) for A, B, C
     DBuf<L> sums(&pool, 3 * nb * E);
     for (int m = 0; m < 3; m++)
-      hipLaunchKernelGGL((k_spmv_gr<T>), dim3(grid_for(nb, GPB)), dim3(GB), 0, st, lptr[m].p, ldk[m].p, lsel[m].p, lidx[m].p,
-                         dhat.p, rho.p, (uint64_t)nb, sums.p + m * nb * E, (uint64_t)nb);
-    e.check();
+      e.launch("k_spmv_gr", ncell[m] * 2.0 * Engine<T>::EB, ncell[m] * 16 * Engine<T>::PM, (k_spmv_gr<T>),
+               dim3(grid_for(nb, GPB)), dim3(GB), (const uint64_t*)lptr[m].p, (const uint32_t*)ldk[m].p,
+               (const uint32_t*)lsel[m].p, (const uint32_t*)lidx[m].p, (const L*)dhat.p, (const L*)rho.p, (uint64_t)nb,
+               sums.p + m * nb * E, (uint64_t)nb);
     // 4. c_hat = pack_r2(psi(c_sum)) (general.rs:410-414); a_hat, b_hat = phi(slots)
     e.psi(sums.p + 2 * nb * E, abc_slots.p + 2 * N, nb);
     DBuf<L> hats(&pool, 3 * nb * E);
@@ -430,9 +435,10 @@ template <class T> struct GeneralParams : ParamsBase {
       e.scale(dg2.p, eq1.p, eqs.p + 2 * nb * E, nb);
     }
     DBuf<L> rows(&pool, 32 * nb * E);
-    hipLaunchKernelGGL((k_spmv_gr<T>), dim3(grid_for(32 * nb, GPB)), dim3(GB), 0, st, rptr.p, rdk.p, rsel.p, ridx.p, dhat.p,
-                       eqs.p, (uint64_t)nb, rows.p, (uint64_t)(32 * nb));
-    e.check();
+    double tc = (double)(ncell[0] + ncell[1] + ncell[2]);
+    e.launch("k_spmv_gr", tc * 2.0 * Engine<T>::EB, tc * 16 * Engine<T>::PM, (k_spmv_gr<T>), dim3(grid_for(32 * nb, GPB)),
+             dim3(GB), (const uint64_t*)rptr.p, (const uint32_t*)rdk.p, (const uint32_t*)rsel.p, (const uint32_t*)ridx.p,
+             (const L*)dhat.p, (const L*)eqs.p, (uint64_t)nb, rows.p, (uint64_t)(32 * nb));
     eqs.release();
     stg.mark("layer_rows");
     // 10. linear check over the 32 (rho_l, row_l) pairs; rho is low-degree
@@ -465,6 +471,7 @@ template <class T> struct GeneralParams : ParamsBase {
     for (int l = 0; l < 32; l++) ok = ok && ys[l] == lp.finals[l];
     if (!ok) throw Error(NENYA_E_DEVICE, "internal opening self-check failed");
     last_timing = stg.t;
+    if (prof.on) prof.collect();
     return out;
   }
 

@@ -1,0 +1,45 @@
+"""CPU-side checks of the product boundary: the HIP library loads, exports
+every symbol include/nenya.h declares, and fails loudly (no CPU fallback)
+when no GPU is visible."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import nenya_amd as na
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    src = open(os.path.join(ROOT, "include", "nenya.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(nenya_\w+)\s*\(", src)))
+
+
+def test_header_declares_the_boundary():
+    names = declared()
+    for n in ("nenya_params_simd_secure", "nenya_prove_simd", "nenya_params_general_secure",
+              "nenya_prove_general", "nenya_free", "nenya_params_free", "nenya_last_error"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(na.lib_path())
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_fails_loudly_without_gpu():
+    if na.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(na.NenyaError) as e:
+        na.GeneralSnarkParams.secure(5, (([], [], []),) * 3)
+    assert e.value.code == 5
+
+
+def test_argument_errors_are_codes_not_crashes():
+    with pytest.raises(na.NenyaError) as e:
+        na.SimdSnarkParams.secure(5, 5, k=12)
+    assert e.value.code == 4
