@@ -72,6 +72,12 @@ int nenya_prove_general(const nenya_params* p, const void* w, size_t n, uint8_t*
  * lets a caller that produced the witness on the GPU skip the host copy. */
 int nenya_prove_general_device(const nenya_params* p, const void* d_w, size_t n, uint8_t** proof, size_t* proof_len);
 
+/* Same as nenya_prove_general_device, but *proof points into a pinned host
+ * buffer owned by `p` (no copy, nothing allocated per call); the bytes stay
+ * valid until the next proof made with `p` or nenya_params_free(p). */
+int nenya_prove_general_device_view(const nenya_params* p, const void* d_w, size_t n, const uint8_t** proof,
+                                    size_t* proof_len);
+
 /* Per-stage wall-clock timing (ms) of the last proof made with `p` when
  * enabled (GeneralProveTiming / SimdProveTiming, general.rs:153-173,
  * simd.rs:177-192).  Enabling adds stream synchronizations at stage

@@ -35,6 +35,7 @@ def _lib():
         L.nenya_params_general_secure.argtypes = [C.c_int, C.c_int] + coo * 3 + [u64, u64, C.POINTER(vp)]
         L.nenya_prove_general.argtypes = [vp, vp, sz, C.POINTER(vp), C.POINTER(sz)]
         L.nenya_prove_general_device.argtypes = [vp, vp, sz, C.POINTER(vp), C.POINTER(sz)]
+        L.nenya_prove_general_device_view.argtypes = [vp, vp, sz, C.POINTER(vp), C.POINTER(sz)]
         L.nenya_set_timing.argtypes = [vp, C.c_int]
         L.nenya_last_timing.argtypes = [vp, C.POINTER(C.c_double), C.c_int, C.c_char_p, sz]
         L.nenya_set_profile.argtypes = [vp, C.c_int]
@@ -176,9 +177,15 @@ def prove_general(w, params):
     return _take_proof(out, n)
 
 
-def prove_general_device(dev_ptr, n_limbs, params):
+def prove_general_device(dev_ptr, n_limbs, params, view=False):
     """prove_general with the witness already in device memory (e.g. a torch
-    tensor's data_ptr())."""
+    tensor's data_ptr()).  view=True returns a read-only memoryview of the
+    proof in the params' pinned buffer (no copy; valid until the next proof
+    with these params)."""
     out, n = C.c_void_p(), C.c_size_t()
+    if view:
+        _check(_lib().nenya_prove_general_device_view(params._h, C.c_void_p(dev_ptr), n_limbs, C.byref(out),
+                                                      C.byref(n)))
+        return memoryview((C.c_ubyte * n.value).from_address(out.value)).toreadonly()
     _check(_lib().nenya_prove_general_device(params._h, C.c_void_p(dev_ptr), n_limbs, C.byref(out), C.byref(n)))
     return _take_proof(out, n)

@@ -40,22 +40,22 @@ static void need_gpu() {
 static void need_width(int k) {
   if (k != 8 && k != 16 && k != 32 && k != 64) throw Error(NENYA_E_ARG, "k must be 8, 16, 32 or 64");
 }
-static uint8_t* emit(const std::vector<uint8_t>& v, uint8_t** proof, size_t* len) {
-  uint8_t* b = (uint8_t*)std::malloc(v.size() ? v.size() : 1);
+static uint8_t* emit(const ProofView& v, uint8_t** proof, size_t* len) {
+  uint8_t* b = (uint8_t*)std::malloc(v.second ? v.second : 1);
   if (!b) throw std::bad_alloc();
-  std::memcpy(b, v.data(), v.size());
+  std::memcpy(b, v.first, v.second);
   *proof = b;
-  *len = v.size();
+  *len = v.second;
   return b;
 }
 
 template <class T>
-static std::vector<uint8_t> simd_prove_t(ParamsBase* b, const HostCoo* m3, const void* wit) {
+static ProofView simd_prove_t(ParamsBase* b, const HostCoo* m3, const void* wit) {
   auto* sp = static_cast<SimdParams<T>*>(b);
   return sp->prove(m3[0], m3[1], m3[2], (const typename T::L*)wit);
 }
 
-template <class T> static std::vector<uint8_t> general_t(ParamsBase* b, const void* w, size_t n, bool dev) {
+template <class T> static ProofView general_t(ParamsBase* b, const void* w, size_t n, bool dev) {
   auto* gp = static_cast<GeneralParams<T>*>(b);
   if (dev) {
     if (n != gp->N) throw Error(NENYA_E_SHAPE, "witness length must be 32 * 2^log_blocks");
@@ -64,19 +64,25 @@ template <class T> static std::vector<uint8_t> general_t(ParamsBase* b, const vo
   return gp->prove_host((const typename T::L*)w, n);
 }
 
-static int general_common(const nenya_params* p, const void* w, size_t n, uint8_t** proof, size_t* len, bool dev) {
+static int general_common(const nenya_params* p, const void* w, size_t n, uint8_t** proof, size_t* len, bool dev,
+                          const uint8_t** view = nullptr) {
   return guard([&] {
-    if (!p || !p->impl || p->impl->kind != 1 || !w || !proof || !len) throw Error(NENYA_E_ARG, "bad argument");
+    if (!p || !p->impl || p->impl->kind != 1 || !w || !len || (!proof && !view)) throw Error(NENYA_E_ARG, "bad argument");
     auto* np = const_cast<nenya_params*>(p);
     std::lock_guard<std::mutex> g(np->mu);
-    std::vector<uint8_t> v;
+    ProofView v;
     switch (p->impl->k) {
       case 8: v = general_t<W<8>>(p->impl, w, n, dev); break;
       case 16: v = general_t<W<16>>(p->impl, w, n, dev); break;
       case 32: v = general_t<W<32>>(p->impl, w, n, dev); break;
       default: v = general_t<W<64>>(p->impl, w, n, dev); break;
     }
-    emit(v, proof, len);
+    if (view) {
+      *view = v.first;
+      *len = v.second;
+    } else {
+      emit(v, proof, len);
+    }
     return NENYA_OK;
   });
 }
@@ -145,7 +151,7 @@ int nenya_prove_simd(const nenya_params* p, const uint32_t* ar, const uint32_t* 
     if ((an && (!ar || !ac || !av)) || (bn && (!br || !bc || !bv)) || (cn && (!cr || !cc || !cv)))
       throw Error(NENYA_E_ARG, "null matrix array");
     HostCoo m3[3] = {{n, m, an, ar, ac, av}, {n, m, bn, br, bc, bv}, {n, m, cn, cr, cc, cv}};
-    std::vector<uint8_t> v;
+    ProofView v;
     switch (b->k) {
       case 8: v = simd_prove_t<W<8>>(b, m3, witnesses); break;
       case 16: v = simd_prove_t<W<16>>(b, m3, witnesses); break;
@@ -192,6 +198,10 @@ int nenya_prove_general(const nenya_params* p, const void* w, size_t n, uint8_t*
 }
 int nenya_prove_general_device(const nenya_params* p, const void* d_w, size_t n, uint8_t** proof, size_t* len) {
   return general_common(p, d_w, n, proof, len, true);
+}
+int nenya_prove_general_device_view(const nenya_params* p, const void* d_w, size_t n, const uint8_t** proof,
+                                    size_t* len) {
+  return general_common(p, d_w, n, nullptr, len, true, proof);
 }
 
 int nenya_set_timing(nenya_params* p, int enabled) {

@@ -405,35 +405,63 @@ __global__ void __launch_bounds__(GB) k_lin_round(const LT<T>* f, uint64_t fstri
   block_emit<T, 2>(acc, partials, t, g);
 }
 
-// partial sums of sum_i A[i]*B[i] (A low-degree if NA = 16)
+// partial sums of sum_i A_y[i]*B[i] for tables y = blockIdx.y (A_y = A + y*astride
+// elements; A low-degree if NA = 16); partials laid out [block][y].
 template <class T, int NA>
-__global__ void __launch_bounds__(GB) k_dot(const LT<T>* A, const LT<T>* B, uint64_t n, LT<T>* partials) {
+__global__ void __launch_bounds__(GB) k_dot(const LT<T>* A, uint64_t astride, const LT<T>* B, uint64_t n, LT<T>* partials) {
   __shared__ __attribute__((aligned(16))) LT<T> sh[GPB][2][E];
   int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  const LT<T>* Ay = A + (uint64_t)blockIdx.y * astride * E;
   Acc<T> acc[1];
   acc[0].clear();
   for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < n; base += (uint64_t)gridDim.x * GPB) {
     uint64_t i = base + g;
     bool ok = i < n;
     if (ok) {
-      stage<T>(sh[g][0], A + i * E, t);
+      stage<T>(sh[g][0], Ay + i * E, t);
       stage<T>(sh[g][1], B + i * E, t);
     }
     __syncthreads();
     if (ok) acc_mul<T, NA>(acc[0], sh[g][0], sh[g][1], t);
     __syncthreads();
   }
-  block_emit<T, 1>(acc, partials, t, g);
+  __shared__ CT<T> red[GPB][14][32];
+  for (int c = 0; c < 7; c++) { red[g][c][t] = acc[0].c[c]; red[g][7 + c][t] = acc[0].h[c]; }
+  __syncthreads();
+  if (g == 0) {
+    Acc<T> a = acc[0];
+    for (int gg = 1; gg < GPB; gg++)
+      for (int c = 0; c < 7; c++) { a.c[c] += red[gg][c][t]; a.h[c] += red[gg][7 + c][t]; }
+    CT<T> o[4];
+    acc_final<T>(a, t, o);
+    st4<T>(partials + ((uint64_t)blockIdx.x * gridDim.y + blockIdx.y) * E + 4 * t, o);
+  }
 }
 
-// out[s] = sum_b partials[b][s]   (thread per limb)
+// out[y] = src[y * stride]  (element copies, thread per limb)
 template <class T>
-__global__ void k_sum_partials(const LT<T>* partials, uint64_t nblocks, int ns, LT<T>* out) {
-  int s = blockIdx.x, l = threadIdx.x;
-  if (l >= E) return;
+__global__ void k_pick(const LT<T>* src, uint64_t stride, uint64_t cnt, LT<T>* out) {
+  uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= cnt * E) return;
+  uint64_t y = id / E, l = id % E;
+  out[id] = src[y * stride * E + l];
+}
+
+// out[s] = sum_b partials[b][s]: block (s, 32-limb slice) of 1024 threads,
+// 32 slices of the partial list per limb, then an LDS tree.
+template <class T>
+__global__ void __launch_bounds__(1024) k_sum_partials(const LT<T>* partials, uint64_t nblocks, int ns, LT<T>* out) {
+  __shared__ CT<T> red[32][33];
+  int s = blockIdx.x / 4, l0 = (blockIdx.x % 4) * 32;
+  int l = threadIdx.x & 31, sl = threadIdx.x >> 5;
   CT<T> acc = 0;
-  for (uint64_t b = 0; b < nblocks; b++) acc += partials[(b * ns + s) * E + l];
-  out[s * E + l] = (LT<T>)acc;
+  for (uint64_t b = sl; b < nblocks; b += 32) acc += partials[(b * ns + s) * E + l0 + l];
+  red[sl][l] = acc;
+  __syncthreads();
+  if (sl == 0) {
+    for (int k = 1; k < 32; k++) acc += red[k][l];
+    out[s * E + l0 + l] = (LT<T>)acc;
+  }
 }
 
 // ============================================================================
@@ -441,34 +469,56 @@ __global__ void k_sum_partials(const LT<T>* partials, uint64_t nblocks, int ns, 
 // teichmuller.rs:104-114 iterates the same fixed point).  One 256-thread
 // block per element: 8 groups split the 32 beta terms of each squaring.
 // ============================================================================
+// Squaring uses the symmetry of a*a: lane t's circular sum pairs term
+// (i, j) with (j, i), i + j = t mod 32.  With h0 = floor(t/2), h1 = t - h0 the
+// pairs are (h0 - d, h1 + d) for d = 0..16: weight 2, except the diagonals
+// (t even: d = 0 and d = 16, weight 1) and the repeat d = 16 for odd t
+// (weight 0).  A pair lies in the wrap-around part iff i > t.  Group g takes
+// d = g, g + 8 (and group 0 also d = 16); the 8 groups' 14 x 32 raw sums are
+// added by all 256 threads in parallel, then group 0 reduces.
 template <class T>
 __global__ void __launch_bounds__(256) k_lift(LT<T>* x, int squarings) {
+  using C = CT<T>;
   __shared__ __attribute__((aligned(16))) LT<T> sx[E];
-  __shared__ CT<T> red[8][14][32];
+  __shared__ C red[8][14][32];
+  __shared__ C sum[14][32];
   int g = threadIdx.x >> 5, t = threadIdx.x & 31;
   LT<T>* el = x + (uint64_t)blockIdx.x * E;
   if (g == 0) stage<T>(sx, el, t);
   __syncthreads();
+  const int h0 = t >> 1, h1 = t - h0;
   for (int it = 0; it < squarings; it++) {
-    using C = CT<T>;
     Acc<T> acc;
     acc.clear();
-    for (int ii = 0; ii < 4; ii++) {
-      int i = 4 * g + ii;
+    for (int d = g; d <= 16; d += 8) {
+      int i = (h0 - d) & 31, j = (h1 + d) & 31;
+      C w = (d == 0 || d == 16) ? ((t & 1) ? (d == 16 ? 0 : 2) : 1) : 2;
       C a[4], b[4], p[7];
       ld4<T>(sx + 4 * i, a);
-      ld4<T>(sx + 4 * ((t - i) & 31), b);
+      ld4<T>(sx + 4 * j, b);
       d_raw(a, b, p);
       const C m = (i > t) ? (C)~(C)0 : (C)0;
-      for (int c = 0; c < 7; c++) { acc.c[c] += p[c]; acc.h[c] += p[c] & m; }
+#pragma unroll
+      for (int c = 0; c < 7; c++) {
+        C v = p[c] * w;
+        acc.c[c] += v;
+        acc.h[c] += v & m;
+      }
     }
     for (int c = 0; c < 7; c++) { red[g][c][t] = acc.c[c]; red[g][7 + c][t] = acc.h[c]; }
     __syncthreads();
+    for (int id = threadIdx.x; id < 14 * 32; id += 256) {
+      int v = id >> 5, l = id & 31;
+      C s2 = 0;
+      for (int gg = 0; gg < 8; gg++) s2 += red[gg][v][l];
+      sum[v][l] = s2;
+    }
+    __syncthreads();
     if (g == 0) {
-      for (int gg = 1; gg < 8; gg++)
-        for (int c = 0; c < 7; c++) { acc.c[c] += red[gg][c][t]; acc.h[c] += red[gg][7 + c][t]; }
+      Acc<T> a2;
+      for (int c = 0; c < 7; c++) { a2.c[c] = sum[c][t]; a2.h[c] = sum[7 + c][t]; }
       C o[4];
-      acc_final<T>(acc, t, o);
+      acc_final<T>(a2, t, o);
       st4<T>(sx + 4 * t, o);
     }
     __syncthreads();
@@ -677,52 +727,81 @@ __global__ void __launch_bounds__(GB) k_rot_mbar(const LT<T>* e0, const LT<T>* r
 }
 
 // ============================================================================
-// RAA encoder (raa.rs:67-153): one block per row, W limbs per entry.
-//   t[i] = msg[perm1[i]/q] * d1[i];  prefix-sum;  z[i] = t[perm2[i]] * d2[i];  prefix-sum.
-// rows: `msg` rows of k2 entries -> `out` rows of n entries; `tmp` is n*W scratch per row.
+// RAA encoder (raa.rs:67-153), all rows of a commitment at once, W limbs per
+// entry.  Each pass is gather-and-scale then an inclusive prefix sum over the
+// codeword index, done as a segmented scan across many blocks:
+//   pass 1  v[i] = src[perm[i] / q] * d[i]  (+ per-segment sums)
+//   scan    exclusive scan of the segment sums (per row, per limb)
+//   pass 3  local inclusive scan of each segment from its offset
+// Encode = pass(msg -> t, perm1, d1, q) then pass(t -> z, perm2, d2, 1).
 // ============================================================================
-template <class T, int W>
-__device__ void block_scan(LT<T>* buf, uint64_t n) {
-  // inclusive prefix sum over i of buf[i*W + s] for every limb s
-  constexpr int NSEG = 1024 / W;
-  __shared__ CT<T> tot[NSEG][W];
-  int s = threadIdx.x % W, seg = threadIdx.x / W;
-  uint64_t per = (n + NSEG - 1) / NSEG;
-  uint64_t b = seg * per, e = b + per < n ? b + per : n;
-  CT<T> sum = 0;
-  for (uint64_t i = b; i < e; i++) sum += buf[i * W + s];
-  tot[seg][s] = sum;
-  __syncthreads();
-  if (seg == 0) {
-    CT<T> run = 0;
-    for (int q = 0; q < NSEG; q++) { CT<T> v = tot[q][s]; tot[q][s] = run; run += v; }
-  }
-  __syncthreads();
-  CT<T> run = tot[seg][s];
-  for (uint64_t i = b; i < e; i++) { run += buf[i * W + s]; buf[i * W + s] = (LT<T>)run; }
-  __syncthreads();
-}
+constexpr int RAA_SEG = 256;  // codeword entries per segment
+constexpr int RAA_TB = 256;   // threads per block
 
 template <class T, int W>
-__global__ void __launch_bounds__(1024) k_raa(const LT<T>* msg, uint64_t k2, const uint32_t* perm1,
-                                              const uint32_t* perm2, const LT<T>* d1, const LT<T>* d2, uint64_t q,
-                                              uint64_t n, LT<T>* tmp, LT<T>* out) {
-  uint64_t row = blockIdx.x;
-  const LT<T>* m = msg + row * k2 * W;
-  LT<T>* tb = tmp + row * n * W;
-  LT<T>* ob = out + row * n * W;
-  for (uint64_t x = threadIdx.x; x < n * W; x += blockDim.x) {
-    uint64_t i = x / W, s = x % W;
-    tb[x] = (LT<T>)((CT<T>)m[(perm1[i] / q) * W + s] * (CT<T>)d1[i]);
+__global__ void __launch_bounds__(RAA_TB) k_raa_gather(const LT<T>* src, uint64_t src_row, const uint32_t* perm,
+                                                       const LT<T>* d, uint64_t q, uint64_t n, LT<T>* dst,
+                                                       CT<T>* segsum, uint64_t nseg) {
+  constexpr int SUB = RAA_TB / W;
+  constexpr int PER = RAA_SEG / SUB;
+  __shared__ CT<T> red[SUB][W];
+  uint64_t row = blockIdx.y, seg = blockIdx.x;
+  int s = threadIdx.x % W, sub = threadIdx.x / W;
+  const LT<T>* sr = src + row * src_row * W;
+  LT<T>* dr = dst + row * n * W;
+  uint64_t b = seg * RAA_SEG + (uint64_t)sub * PER;
+  CT<T> acc = 0;
+  for (int k = 0; k < PER; k++) {
+    uint64_t i = b + k;
+    if (i < n) {
+      CT<T> v = (CT<T>)sr[(uint64_t)(perm[i] / q) * W + s] * (CT<T>)d[i];
+      dr[i * W + s] = (LT<T>)v;
+      acc += v;
+    }
   }
+  red[sub][s] = acc;
   __syncthreads();
-  block_scan<T, W>(tb, n);
-  for (uint64_t x = threadIdx.x; x < n * W; x += blockDim.x) {
-    uint64_t i = x / W, s = x % W;
-    ob[x] = (LT<T>)((CT<T>)tb[(uint64_t)perm2[i] * W + s] * (CT<T>)d2[i]);
+  if (sub == 0) {
+    for (int k = 1; k < SUB; k++) acc += red[k][s];
+    segsum[(row * nseg + seg) * W + s] = acc;
   }
+}
+template <class T, int W>
+__global__ void k_raa_segscan(CT<T>* segsum, uint64_t rows, uint64_t nseg) {
+  uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= rows * W) return;
+  uint64_t row = id / W, s = id % W;
+  CT<T> run = 0;
+  for (uint64_t g = 0; g < nseg; g++) {
+    CT<T>& x = segsum[(row * nseg + g) * W + s];
+    CT<T> v = x;
+    x = run;
+    run += v;
+  }
+}
+template <class T, int W>
+__global__ void __launch_bounds__(RAA_TB) k_raa_scan(LT<T>* buf, uint64_t n, const CT<T>* segsum, uint64_t nseg) {
+  constexpr int SUB = RAA_TB / W;
+  constexpr int PER = RAA_SEG / SUB;
+  __shared__ CT<T> red[SUB][W];
+  uint64_t row = blockIdx.y, seg = blockIdx.x;
+  int s = threadIdx.x % W, sub = threadIdx.x / W;
+  LT<T>* br = buf + row * n * W;
+  uint64_t b = seg * RAA_SEG + (uint64_t)sub * PER;
+  CT<T> tot = 0;
+  for (int k = 0; k < PER; k++)
+    if (b + k < n) tot += br[(b + k) * W + s];
+  red[sub][s] = tot;
   __syncthreads();
-  block_scan<T, W>(ob, n);
+  CT<T> run = segsum[(row * nseg + seg) * W + s];
+  for (int k = 0; k < sub; k++) run += red[k][s];
+  for (int k = 0; k < PER; k++) {
+    uint64_t i = b + k;
+    if (i < n) {
+      run += br[i * W + s];
+      br[i * W + s] = (LT<T>)run;
+    }
+  }
 }
 
 // ============================================================================
@@ -810,28 +889,35 @@ __global__ void k_b3_merge(const uint32_t* in, uint64_t cnt, uint32_t* out) {
   }
 }
 
-// Merkle authentication paths: out[q][lvl] = sibling of idx[q] at level lvl.
+// Serialized Merkle paths (pcs.rs:390-397): per query a u64 path length then
+// the sibling hashes from the leaf up.  Thread per (query, level).
 __global__ void k_merkle_paths(const uint32_t* nodes, uint64_t n, const uint32_t* idx, uint64_t nq, int plen,
-                               uint32_t* out) {
+                               uint8_t* out) {
   uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= nq * plen) return;
   uint64_t q = id / plen;
   int lvl = (int)(id % plen);
+  uint8_t* rec = out + q * (8 + (uint64_t)plen * 32);
+  if (lvl == 0) *reinterpret_cast<uint64_t*>(rec) = (uint64_t)plen;
   uint64_t start = 0, len = n;
   for (int l = 0; l < lvl; l++) { start += len; len >>= 1; }
   uint64_t node = start + ((idx[q] >> lvl) ^ 1);
-  for (int k = 0; k < 8; k++) out[id * 8 + k] = nodes[node * 8 + k];
+  uint32_t* dst = reinterpret_cast<uint32_t*>(rec + 8 + (uint64_t)lvl * 32);
+  for (int k = 0; k < 8; k++) dst[k] = nodes[node * 8 + k];
 }
 
-// out[q][i][:] = enc[i][idx[q]][:]   (W limbs per entry)
+// Serialized queried columns (pcs.rs:379-388): per query a u64 entry count,
+// then enc[i][idx[q]] for rows i (W limbs each).  Thread per limb.
 template <class T>
 __global__ void k_gather_cols(const LT<T>* enc, const uint32_t* idx, uint64_t nq, uint64_t rows, uint64_t n, int W,
-                              LT<T>* out) {
+                              uint8_t* out) {
   uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t per = rows * W;
   if (id >= nq * per) return;
   uint64_t q = id / per, r = (id % per) / W, s = id % W;
-  out[id] = enc[(r * n + idx[q]) * W + s];
+  uint8_t* rec = out + q * (8 + per * sizeof(LT<T>));
+  if (r == 0 && s == 0) *reinterpret_cast<uint64_t*>(rec) = rows;
+  reinterpret_cast<LT<T>*>(rec + 8)[r * W + s] = enc[(r * n + idx[q]) * W + s];
 }
 
 // small element-wise helpers (thread per limb)

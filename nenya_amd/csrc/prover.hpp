@@ -67,6 +67,57 @@ struct KProf {
   }
 };
 
+// Pinned host buffer the proof is assembled in (grow-only, owned by params).
+struct Pinned {
+  uint8_t* p = nullptr;
+  size_t cap = 0;
+  uint8_t* get(size_t n) {
+    if (n > cap) {
+      if (p) (void)hipHostFree(p);
+      p = nullptr;
+      cap = 0;
+      HIP_CHECK(hipHostMalloc((void**)&p, n ? n : 1, hipHostMallocDefault));
+      cap = n;
+    }
+    return p;
+  }
+  ~Pinned() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
+// Write cursor over a pre-sized section of the proof buffer.
+struct Sink {
+  uint8_t* p;
+  uint8_t* end;
+  Sink(uint8_t* b, size_t n) : p(b), end(b + n) {}
+  uint8_t* take(size_t n) {
+    if ((size_t)(end - p) < n) throw Error(NENYA_E_DEVICE, "proof section overflow");
+    uint8_t* r = p;
+    p += n;
+    return r;
+  }
+  void u64(uint64_t x) {
+    uint8_t* d = take(8);
+    for (int i = 0; i < 8; i++) d[i] = (uint8_t)(x >> (8 * i));
+  }
+  void u32(uint32_t x) {
+    uint8_t* d = take(4);
+    for (int i = 0; i < 4; i++) d[i] = (uint8_t)(x >> (8 * i));
+  }
+  void bytes(const void* s, size_t n) { std::memcpy(take(n), s, n); }
+  template <class G> void gr(const G& g) { bytes(g.v, sizeof g.v); }
+  bool full() const { return p == end; }
+};
+
+// Serialized size of an opening (OpeningProof / R2OpeningProof, pcs.rs:357-406,
+// r2_pcs.rs:109-160) or a rotated batch of nl m-bars (pcs.rs:426-471).
+inline size_t opening_bytes(const PcsParams& P, size_t rows, size_t limb_bytes, size_t nl) {
+  size_t W = P.packed ? 32 : 128, eb = 128 * limb_bytes, plen = (size_t)__builtin_ctzll(P.n);
+  size_t mb = nl == 0 ? 8 + P.k2 * eb : 8 + nl * (8 + P.k2 * eb);
+  return mb + 8 + 4 * P.ell + 8 + P.ell * (8 + rows * W * limb_bytes) + 8 + P.ell * (8 + plen * 32);
+}
+
 template <class T> struct PcsDev {
   using L = typename T::L;
   PcsParams P;
@@ -191,16 +242,29 @@ template <class T> struct Engine {
   // sum of partial sums -> ns elements on device
   DBuf<L> reduce(const DBuf<L>& partials, unsigned nblocks, int ns) {
     DBuf<L> out(pool, ns * E);
-    launch("k_sum_partials", (double)nblocks * ns * EB, 0, (k_sum_partials<T>), dim3(ns), dim3(E), partials.p,
+    launch("k_sum_partials", (double)nblocks * ns * EB, 0, (k_sum_partials<T>), dim3(ns * 4), dim3(1024), partials.p,
            (uint64_t)nblocks, ns, out.p);
     return out;
   }
-  template <int NA> G dot(const L* A, const L* B, uint64_t n) {
-    unsigned nb = grid_for(n, GPB, 2048);
-    DBuf<L> part(pool, (size_t)nb * E);
-    launch("k_dot", n * 2 * EB, n * NA * PM, (k_dot<T, NA>), dim3(nb), dim3(GB), A, B, n, part.p);
-    DBuf<L> r = reduce(part, nb, 1);
-    return get(r.p);
+  // ntab inner products <A_y, B> (A_y = A + y*astride elements), one D2H.
+  template <int NA> std::vector<G> dots(const L* A, uint64_t astride, int ntab, const L* B, uint64_t n) {
+    unsigned nb = grid_for(n, GPB, ntab > 1 ? 256 : 2048);
+    DBuf<L> part(pool, (size_t)nb * ntab * E);
+    launch("k_dot", n * (ntab + 1.0) * EB, n * ntab * NA * PM, (k_dot<T, NA>), dim3(nb, ntab), dim3(GB), A, astride, B, n,
+           part.p);
+    DBuf<L> r = reduce(part, nb, ntab);
+    std::vector<G> out(ntab);
+    d2h(out.data(), r.p, ntab * sizeof(G));
+    return out;
+  }
+  template <int NA> G dot(const L* A, const L* B, uint64_t n) { return dots<NA>(A, 0, 1, B, n)[0]; }
+  // element 0 of ntab tables spaced `stride` elements apart, one D2H
+  std::vector<G> pick(const L* src, uint64_t stride, int ntab) {
+    DBuf<L> o(pool, (size_t)ntab * E);
+    launch("k_pick", 0, 0, (k_pick<T>), dim3(grid_for((uint64_t)ntab * E, 256)), dim3(256), src, stride, (uint64_t)ntab, o.p);
+    std::vector<G> out(ntab);
+    d2h(out.data(), o.p, ntab * sizeof(G));
+    return out;
   }
 
   // build_eq_table (eq.rs:35-58) for host points r[0..n): 2^n entries
@@ -253,14 +317,8 @@ template <class T> struct Engine {
     c.msg = lifted;
     DBuf<L> tmp(pool, rows * P.n * W);
     c.enc = DBuf<L>(pool, rows * P.n * W);
-    // algorithmic bytes: read the k2-entry row, write/read t, write/read z (two passes)
-    double rb = (double)rows * W * sizeof(L) * (P.k2 + 4.0 * P.n);
-    if (P.packed)
-      launch("k_raa", rb, 0, (k_raa<T, 32>), dim3(rows), dim3(1024), rowsbuf, (uint64_t)P.k2, pc.perm1.p, pc.perm2.p,
-             pc.d1.p, pc.d2.p, (uint64_t)P.q, (uint64_t)P.n, tmp.p, c.enc.p);
-    else
-      launch("k_raa", rb, 0, (k_raa<T, E>), dim3(rows), dim3(1024), rowsbuf, (uint64_t)P.k2, pc.perm1.p, pc.perm2.p,
-             pc.d1.p, pc.d2.p, (uint64_t)P.q, (uint64_t)P.n, tmp.p, c.enc.p);
+    if (P.packed) raa<32>(pc, rowsbuf, rows, tmp.p, c.enc.p);
+    else raa<E>(pc, rowsbuf, rows, tmp.p, c.enc.p);
     size_t n = P.n;  // a power of two
     c.nodes = DBuf<uint32_t>(pool, (2 * n - 1) * 8);
     launch("k_hash_cols", (double)rows * n * W * sizeof(L), 0, k_hash_cols, dim3(grid_for(n, 64)), dim3(64),
@@ -274,6 +332,25 @@ template <class T> struct Engine {
     }
     d2h(c.root, c.nodes.p + (2 * n - 2) * 8, 32);
     return c;
+  }
+
+  // RAA encode of `rows` rows (raa.rs:106-153 / 67-98), W limbs per entry.
+  template <int W> void raa(const PcsDev<T>& pc, const L* msg, size_t rows, L* tmp, L* out) {
+    const PcsParams& P = pc.P;
+    uint64_t nseg = (P.n + RAA_SEG - 1) / RAA_SEG;
+    DBuf<C> seg(pool, rows * nseg * W);
+    dim3 grid((unsigned)nseg, (unsigned)rows);
+    double eb = W * sizeof(L), rowb = (double)rows * P.n * eb;
+    for (int pass = 0; pass < 2; pass++) {
+      const L* src = pass == 0 ? msg : tmp;
+      L* dst = pass == 0 ? tmp : out;
+      launch("k_raa_gather", rowb * 2.0, 0, (k_raa_gather<T, W>), grid, dim3(RAA_TB), src,
+             (uint64_t)(pass == 0 ? P.k2 : P.n), (const uint32_t*)(pass == 0 ? pc.perm1.p : pc.perm2.p),
+             (const L*)(pass == 0 ? pc.d1.p : pc.d2.p), (uint64_t)(pass == 0 ? P.q : 1), (uint64_t)P.n, dst, seg.p, nseg);
+      launch("k_raa_segscan", 0, 0, (k_raa_segscan<T, W>), dim3(grid_for(rows * W, 256)), dim3(256), seg.p, (uint64_t)rows,
+             nseg);
+      launch("k_raa_scan", rowb * 2.0, 0, (k_raa_scan<T, W>), grid, dim3(RAA_TB), dst, (uint64_t)P.n, (const C*)seg.p, nseg);
+    }
   }
 
   // BLAKE3 root node of prefix || device bytes, hashed on the device
@@ -309,8 +386,9 @@ template <class T> struct Engine {
   }
 
   // Shared tail of every opening: query indices (derived from BLAKE3 of
-  // domain || root || m-bar), columns, Merkle paths.
-  void open_tail(const PcsDev<T>& pc, const CommitDev<T>& c, const L* d_mbar, size_t nmbar, std::vector<uint8_t>& out) {
+  // domain || root || m-bar), columns, Merkle paths; D2H straight into the
+  // pinned proof buffer.
+  void open_tail(const PcsDev<T>& pc, const CommitDev<T>& c, const L* d_mbar, size_t nmbar, Sink& out) {
     const PcsParams& P = pc.P;
     const int W = P.packed ? 32 : E;
     const char* dom = P.packed ? "pcs-ring/fs/queries" : "snark-ring/r2-pcs/fs/queries";
@@ -319,43 +397,37 @@ template <class T> struct Engine {
     uint32_t node[26];
     hash_root_node(prefix.data(), prefix.size(), d_mbar, nmbar * E * sizeof(L), node);
     std::vector<uint32_t> idx = queries_from_root(node, P.n, P.ell);
-    put_u64(out, idx.size());
-    for (uint32_t j : idx) put_u32(out, j);
+    out.u64(idx.size());
+    for (uint32_t j : idx) out.u32(j);
     DBuf<uint32_t> didx(pool, idx.size());
     h2d(didx.p, idx.data(), idx.size() * 4);
-    size_t per = c.rows * W;
-    DBuf<L> cols(pool, idx.size() * per);
-    launch("k_gather_cols", idx.size() * per * 2.0 * sizeof(L), 0, (k_gather_cols<T>),
-           dim3(grid_for(idx.size() * per, 256, 1u << 20)), dim3(256), (const L*)c.enc.p, (const uint32_t*)didx.p,
-           (uint64_t)idx.size(), (uint64_t)c.rows, (uint64_t)P.n, W, cols.p);
+    size_t per = c.rows * W, nq = idx.size();
+    size_t colb = nq * (8 + per * sizeof(L));
+    DBuf<uint8_t> cols(pool, colb);
+    launch("k_gather_cols", nq * per * 2.0 * sizeof(L), 0, (k_gather_cols<T>), dim3(grid_for(nq * per, 256, 1u << 20)),
+           dim3(256), (const L*)c.enc.p, (const uint32_t*)didx.p, (uint64_t)nq, (uint64_t)c.rows, (uint64_t)P.n, W,
+           cols.p);
     size_t n = P.n;
     int plen = __builtin_ctzll(n);
-    DBuf<uint32_t> paths(pool, std::max<size_t>(idx.size() * plen, 1) * 8);
+    size_t pathb = nq * (8 + (size_t)plen * 32);
+    DBuf<uint8_t> paths(pool, std::max<size_t>(pathb, 8));
     if (plen > 0)
-      launch("k_merkle_paths", 0, 0, k_merkle_paths, dim3(grid_for(idx.size() * plen, 64, 1u << 20)), dim3(64),
-             (const uint32_t*)c.nodes.p, (uint64_t)n, (const uint32_t*)didx.p, (uint64_t)idx.size(), plen, paths.p);
-    std::vector<L> hc(idx.size() * per);
-    std::vector<uint32_t> hp(idx.size() * plen * 8);
-    HIP_CHECK(hipMemcpyAsync(hc.data(), cols.p, hc.size() * sizeof(L), hipMemcpyDeviceToHost, st));
-    if (!hp.empty()) HIP_CHECK(hipMemcpyAsync(hp.data(), paths.p, hp.size() * 4, hipMemcpyDeviceToHost, st));
+      launch("k_merkle_paths", 0, 0, k_merkle_paths, dim3(grid_for(nq * plen, 64, 1u << 20)), dim3(64),
+             (const uint32_t*)c.nodes.p, (uint64_t)n, (const uint32_t*)didx.p, (uint64_t)nq, plen, paths.p);
+    out.u64(nq);
+    HIP_CHECK(hipMemcpyAsync(out.take(colb), cols.p, colb, hipMemcpyDeviceToHost, st));
+    out.u64(nq);
+    if (plen > 0) {
+      HIP_CHECK(hipMemcpyAsync(out.take(pathb), paths.p, pathb, hipMemcpyDeviceToHost, st));
+    } else {
+      for (size_t q = 0; q < nq; q++) out.u64(0);
+    }
     sync();
-    put_u64(out, idx.size());
-    for (size_t q = 0; q < idx.size(); q++) {
-      put_u64(out, c.rows);
-      const uint8_t* b = reinterpret_cast<const uint8_t*>(&hc[q * per]);
-      out.insert(out.end(), b, b + per * sizeof(L));
-    }
-    put_u64(out, idx.size());
-    for (size_t q = 0; q < idx.size(); q++) {
-      put_u64(out, (uint64_t)plen);
-      const uint8_t* b = reinterpret_cast<const uint8_t*>(&hp[q * plen * 8]);
-      out.insert(out.end(), b, b + (size_t)plen * 32);
-    }
   }
 
   // Pcs::open / open_stacked (pcs.rs:547-591, 734-779) and R2Pcs::open /
   // open_stacked (r2_pcs.rs:297-338, 512-559).  weights: s elements or null (s=1).
-  G open(const PcsDev<T>& pc, const CommitDev<T>& c, const G* z, const G* weights, size_t s, std::vector<uint8_t>& out) {
+  G open(const PcsDev<T>& pc, const CommitDev<T>& c, const G* z, const G* weights, size_t s, Sink& out) {
     const PcsParams& P = pc.P;
     size_t lk1 = (size_t)__builtin_ctzll(P.k1), lk2 = (size_t)__builtin_ctzll(P.k2);
     DBuf<L> e0 = eq_table(z, lk1), e1 = eq_table(z + lk1, lk2);
@@ -377,10 +449,8 @@ template <class T> struct Engine {
       launch("k_mbar", mbb, rows * P.k2 * 32 * PM, (k_mbar<T, 32>), dim3(grid_for(P.k2, GPB)), dim3(GB), (const L*)e0s.p,
              c.msg, rows, (uint64_t)P.k2, mbar.p, (uint64_t)0);
     G y = dot<32>(mbar.p, e1.p, P.k2);
-    size_t o0 = out.size();
-    put_u64(out, P.k2);
-    out.resize(out.size() + P.k2 * E * sizeof(L));
-    d2h(out.data() + o0 + 8, mbar.p, P.k2 * E * sizeof(L));
+    out.u64(P.k2);
+    HIP_CHECK(hipMemcpyAsync(out.take(P.k2 * E * sizeof(L)), mbar.p, P.k2 * E * sizeof(L), hipMemcpyDeviceToHost, st));
     open_tail(pc, c, mbar.p, P.k2, out);
     return y;
   }
@@ -388,7 +458,7 @@ template <class T> struct Engine {
   // Pcs::rotated_open_batch (pcs.rs:973-1040) over the unfolded rho layers
   // (rho_l[q] = pack_r2(rot_l(slots_q)), the lifted messages of the rotations).
   std::vector<G> rotated_open(const PcsDev<T>& pc, const CommitDev<T>& c, const L* rho, uint64_t nb, const G* z, int nl,
-                              std::vector<uint8_t>& out) {
+                              Sink& out) {
     const PcsParams& P = pc.P;
     size_t lk1 = (size_t)__builtin_ctzll(P.k1), lk2 = (size_t)__builtin_ctzll(P.k2);
     DBuf<L> e0 = eq_table(z, lk1), e1 = eq_table(z + lk1, lk2);
@@ -396,18 +466,13 @@ template <class T> struct Engine {
     launch("k_rot_mbar", (double)nl * P.k1 * P.k2 * EB, (double)nl * P.k1 * P.k2 * 16 * PM, (k_rot_mbar<T>),
            dim3(grid_for((uint64_t)nl * P.k2, GPB)), dim3(GB), (const L*)e0.p, rho, nb, (uint64_t)P.k1, (uint64_t)P.k2, nl,
            mbar.p);
-    std::vector<G> ys(nl);
-    for (int l = 0; l < nl; l++) ys[l] = dot<32>(mbar.p + (size_t)l * P.k2 * E, e1.p, P.k2);
-    put_u64(out, nl);
+    std::vector<G> ys = dots<32>(mbar.p, P.k2, nl, e1.p, P.k2);
+    out.u64(nl);
     size_t per = P.k2 * E * sizeof(L);
-    size_t o0 = out.size();
-    out.resize(o0 + (size_t)nl * (8 + per));  // sized once: async copies below target it
     for (int l = 0; l < nl; l++) {
-      uint8_t* dst = out.data() + o0 + (size_t)l * (8 + per);
-      for (int i = 0; i < 8; i++) dst[i] = (uint8_t)((uint64_t)P.k2 >> (8 * i));
-      HIP_CHECK(hipMemcpyAsync(dst + 8, mbar.p + (size_t)l * P.k2 * E, per, hipMemcpyDeviceToHost, st));
+      out.u64(P.k2);
+      HIP_CHECK(hipMemcpyAsync(out.take(per), mbar.p + (size_t)l * P.k2 * E, per, hipMemcpyDeviceToHost, st));
     }
-    sync();
     open_tail(pc, c, mbar.p, (size_t)nl * P.k2, out);
     return ys;
   }
@@ -460,7 +525,7 @@ template <class T> struct Engine {
         marg(ds.p, eg.p, m / 2);
       }
     }
-    for (int t = 0; t < ntab; t++) o.finals.push_back(get(cur[t]));
+    o.finals = pick(work.p, N / 2, ntab);
     return o;
   }
 
@@ -508,8 +573,9 @@ template <class T> struct Engine {
       fs = gs = H;
       len = m;
     }
-    for (int t = 0; t < ntab; t++) o.finals.push_back(get(f + (size_t)t * fs * E));
-    for (int t = 0; t < ntab; t++) o.finals.push_back(get(g + (size_t)t * gs * E));
+    o.finals = pick(f, fs, ntab);
+    std::vector<G> gf = pick(g, gs, ntab);
+    o.finals.insert(o.finals.end(), gf.begin(), gf.end());
     return o;
   }
 };

@@ -14,8 +14,10 @@ struct ParamsBase {
   bool timing = false;
   std::vector<std::pair<std::string, double>> last_timing;
   KProf prof;
+  Pinned proofbuf;  // the last proof's bytes (valid until the next proof)
   virtual ~ParamsBase() = default;
 };
+using ProofView = std::pair<const uint8_t*, size_t>;
 
 struct HostCoo {
   size_t rows, cols, nnz;
@@ -92,7 +94,7 @@ template <class T> struct SimdParams : ParamsBase {
     if (st) (void)hipStreamDestroy(st);
   }
 
-  std::vector<uint8_t> prove(const HostCoo& A, const HostCoo& B, const HostCoo& Cm, const L* wit /*32 x m*/) {
+  ProofView prove(const HostCoo& A, const HostCoo& B, const HostCoo& Cm, const L* wit /*32 x m*/) {
     Engine<T>& e = *eng;
     e.prof = &prof;
     size_t n = size_t(1) << log_n, m = size_t(1) << log_m;
@@ -197,26 +199,32 @@ template <class T> struct SimdParams : ParamsBase {
     tr.elems("lin/finals", fin_lin, 2);
     stg.mark("linear_sumcheck");
     // 11. serialize (simd.rs:150-174) with the two openings
-    std::vector<uint8_t> out;
-    out.insert(out.end(), cw.root, cw.root + 32);
-    out.insert(out.end(), ce.root, ce.root + 32);
-    put_gr(out, claim_mult);
-    put_u64(out, mp.chal.size());
-    for (auto& g : mp.rounds) put_gr(out, g);
-    put_gr(out, mp.finals[0]);
-    put_gr(out, mp.finals[1]);
-    put_gr(out, claim_lin);
-    put_u64(out, lp.chal.size());
-    for (auto& g : lp.rounds) put_gr(out, g);
-    put_gr(out, final_row);
-    put_gr(out, final_w);
+    const size_t EBs = E * sizeof(L);
+    size_t s_mult = EBs + 8 + 3 * (size_t)log_n * EBs + 2 * EBs, s_lin = EBs + 8 + 3 * (size_t)log_m * EBs + 2 * EBs;
+    size_t s_ow = opening_bytes(pw.P, pw.P.k1, sizeof(L), 0), s_oe = opening_bytes(pq.P, pq.P.k1, sizeof(L), 0);
+    size_t total = 64 + s_mult + s_lin + s_ow + s_oe;
+    uint8_t* buf = proofbuf.get(total);
+    Sink out(buf, total);
+    out.bytes(cw.root, 32);
+    out.bytes(ce.root, 32);
+    out.gr(claim_mult);
+    out.u64(mp.chal.size());
+    for (auto& g : mp.rounds) out.gr(g);
+    out.gr(mp.finals[0]);
+    out.gr(mp.finals[1]);
+    out.gr(claim_lin);
+    out.u64(lp.chal.size());
+    for (auto& g : lp.rounds) out.gr(g);
+    out.gr(final_row);
+    out.gr(final_w);
     G yw = e.open(pw, cw, lp.chal.data(), nullptr, 1, out);
     G ye = e.open(pq, ce, rt.data(), nullptr, 1, out);
     stg.mark("open");
     if (!(yw == final_w) || !(ye == e_at_r)) throw Error(NENYA_E_DEVICE, "internal opening self-check failed");
+    if (!out.full()) throw Error(NENYA_E_DEVICE, "proof size mismatch");
     last_timing = stg.t;
     if (prof.on) prof.collect();
-    return out;
+    return {buf, total};
   }
 };
 
@@ -361,7 +369,7 @@ template <class T> struct GeneralParams : ParamsBase {
     if (st) (void)hipStreamDestroy(st);
   }
 
-  std::vector<uint8_t> prove_device(const L* w /* device, N limbs */) {
+  ProofView prove_device(const L* w /* device, N limbs */) {
     Engine<T>& e = *eng;
     e.prof = &prof;
     Stages stg;
@@ -417,12 +425,26 @@ template <class T> struct GeneralParams : ParamsBase {
     G gam = e.challenge(tr, "linear_check_gamma");
     G gam2 = gam * gam;
     stg.mark("mult_sumcheck");
+    // proof layout (general.rs:120-150): roots, mult, lin, open_emult,
+    // open_abc, open_estack, open_rho -- sections sized up front
+    const size_t EBs = E * sizeof(L);
+    size_t s_mult = 8 + 3 * (size_t)log_b * EBs + 4 * EBs, s_lin = EBs + 8 + 3 * (size_t)log_b * EBs + 64 * EBs;
+    size_t s_em = opening_bytes(pe.P, pe.P.k1, sizeof(L), 0), s_abc = opening_bytes(pabc.P, 3 * pabc.P.k1, sizeof(L), 0);
+    size_t s_es = opening_bytes(pe.P, 3 * pe.P.k1, sizeof(L), 0), s_rho = opening_bytes(pw.P, pw.P.k1, sizeof(L), 32);
+    size_t total = 128 + s_mult + s_lin + s_em + s_abc + s_es + s_rho;
+    uint8_t* buf = proofbuf.get(total);
+    uint8_t* o_mult = buf + 128;
+    uint8_t* o_lin = o_mult + s_mult;
+    uint8_t* o_em = o_lin + s_lin;
+    uint8_t* o_abc = o_em + s_em;
+    uint8_t* o_es = o_abc + s_abc;
+    uint8_t* o_rho = o_es + s_es;
     // 8. openings at s1 (general.rs:475-478)
     G wts[3] = {G::one(), gam, gam2};
-    std::vector<uint8_t> o_es, o_abc, o_em;
-    G estack_val = e.open(pe, ces, mp.chal.data(), wts, 3, o_es);
-    G abc_val = e.open(pabc, cabc, mp.chal.data(), wts, 3, o_abc);
-    G em_val = e.open(pe, cem, mp.chal.data(), nullptr, 1, o_em);
+    Sink k_es(o_es, s_es), k_abc(o_abc, s_abc), k_em(o_em, s_em);
+    G estack_val = e.open(pe, ces, mp.chal.data(), wts, 3, k_es);
+    G abc_val = e.open(pabc, cabc, mp.chal.data(), wts, 3, k_abc);
+    G em_val = e.open(pe, cem, mp.chal.data(), nullptr, 1, k_em);
     stg.mark("open_s1");
     // 9. layer rows (general.rs:233-271) as sum over cells of
     //    {eq_s1, gamma eq_s1, gamma^2 eq_s1}[p] * D[cell]
@@ -448,34 +470,31 @@ template <class T> struct GeneralParams : ParamsBase {
     rows.release();
     stg.mark("linear_sumcheck");
     // 11. rotated openings of w at s2 (pcs.rs:973-1040)
-    std::vector<uint8_t> o_rho;
-    std::vector<G> ys = e.rotated_open(pw, cw, rho.p, nb, lp.chal.data(), 32, o_rho);
+    Sink k_rho(o_rho, s_rho);
+    std::vector<G> ys = e.rotated_open(pw, cw, rho.p, nb, lp.chal.data(), 32, k_rho);
     stg.mark("open_rho");
-    // serialize (general.rs:120-150)
-    std::vector<uint8_t> out;
-    for (auto* r : {cw.root, cabc.root, cem.root, ces.root}) out.insert(out.end(), r, r + 32);
-    put_u64(out, mp.chal.size());
-    for (auto& g : mp.rounds) put_gr(out, g);
-    for (int i = 0; i < 4; i++) put_gr(out, mp.finals[i]);
-    put_gr(out, claim_lin);
-    put_u64(out, lp.chal.size());
-    for (auto& g : lp.rounds) put_gr(out, g);
-    for (int l = 0; l < 32; l++) put_gr(out, lp.finals[32 + l]);  // final_row
-    for (int l = 0; l < 32; l++) put_gr(out, lp.finals[l]);       // final_rho
-    out.insert(out.end(), o_em.begin(), o_em.end());
-    out.insert(out.end(), o_abc.begin(), o_abc.end());
-    out.insert(out.end(), o_es.begin(), o_es.end());
-    out.insert(out.end(), o_rho.begin(), o_rho.end());
+    Sink hd(buf, 128 + s_mult + s_lin);
+    for (auto* r : {cw.root, cabc.root, cem.root, ces.root}) hd.bytes(r, 32);
+    hd.u64(mp.chal.size());
+    for (auto& g : mp.rounds) hd.gr(g);
+    for (int i = 0; i < 4; i++) hd.gr(mp.finals[i]);
+    hd.gr(claim_lin);
+    hd.u64(lp.chal.size());
+    for (auto& g : lp.rounds) hd.gr(g);
+    for (int l = 0; l < 32; l++) hd.gr(lp.finals[32 + l]);  // final_row
+    for (int l = 0; l < 32; l++) hd.gr(lp.finals[l]);       // final_rho
+    if (!hd.full() || !k_es.full() || !k_abc.full() || !k_em.full() || !k_rho.full())
+      throw Error(NENYA_E_DEVICE, "proof size mismatch");
     bool ok = em_val == mp.finals[3] &&
               abc_val == mp.finals[0] + gam * mp.finals[1] + gam2 * mp.finals[2];
     for (int l = 0; l < 32; l++) ok = ok && ys[l] == lp.finals[l];
     if (!ok) throw Error(NENYA_E_DEVICE, "internal opening self-check failed");
     last_timing = stg.t;
     if (prof.on) prof.collect();
-    return out;
+    return {buf, total};
   }
 
-  std::vector<uint8_t> prove_host(const L* w, size_t n) {
+  ProofView prove_host(const L* w, size_t n) {
     if (n != N) throw Error(NENYA_E_SHAPE, "witness length must be 32 * 2^log_blocks");
     DBuf<L> dw(&pool, N);
     eng->h2d(dw.p, w, N * sizeof(L));
