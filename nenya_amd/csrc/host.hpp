@@ -119,6 +119,7 @@ template <class T> struct Consts {
   Gr<T> lbeta[16];
   Gr<T> basis[32];
   std::vector<L> psi_pw;  // [16][32][4]: t_i^k
+  std::vector<L> frob_t;  // [j][o]: matrix of sigma^-(k-1), transposed
   L inv15;
 
   Consts() {
@@ -185,10 +186,86 @@ template <class T> struct Consts {
         p = G4<T>::mul(p, t1[i]);
       }
     }
+    frobenius_setup();
   }
   static const Consts& get() {
     static Consts c;
     return c;
+  }
+
+  // Matrix of sigma^-n, n = k-1, where sigma is the Frobenius automorphism
+  // (sigma(t) = t^2 on Teichmuller elements, sigma^128 = id).  The elements
+  // b_j = Teich(X)^c Teich(Y)^i (limb j = 4i+c) are Teichmuller and congruent
+  // to the standard basis mod 2, so B = [b_j] is invertible and
+  // sigma^-n = P B^-1 with P = [sigma^-n b_j] = [b_j^(2^(128-n))].
+  void frobenius_setup() {
+    const int n = T::K - 1;
+    using G = Gr<T>;
+    auto lift128 = [](G g) {
+      for (int s = 0; s < 128; s++) g = g * g;
+      return g;
+    };
+    G tx = G::zero(), ty = G::zero();
+    tx.v[1] = 1;  // alpha
+    ty.v[4] = 1;  // beta
+    tx = lift128(tx);
+    ty = lift128(ty);
+    G px = tx, py = ty;
+    for (int s = 0; s < 128 - n; s++) { px = px * px; py = py * py; }
+    std::vector<C> Bm(E * E), Pm(E * E);  // [o][j]
+    G yi = G::one(), qi = G::one();
+    for (int i = 0; i < 32; i++) {
+      G b = yi, p = qi;
+      for (int c = 0; c < 4; c++) {
+        for (int o = 0; o < E; o++) {
+          Bm[o * E + 4 * i + c] = b.v[o];
+          Pm[o * E + 4 * i + c] = p.v[o];
+        }
+        b = b * tx;
+        p = p * px;
+      }
+      yi = yi * ty;
+      qi = qi * py;
+    }
+    auto matmul = [](const std::vector<C>& A, const std::vector<C>& Bq) {
+      std::vector<C> R(E * E, 0);
+      for (int r = 0; r < E; r++)
+        for (int m = 0; m < E; m++) {
+          C a = A[r * E + m];
+          if (!a) continue;
+          for (int c = 0; c < E; c++) R[r * E + c] += a * Bq[m * E + c];
+        }
+      return R;
+    };
+    // B^-1 by Newton's iteration X <- X (2I - B X) from X = I (B = I mod 2);
+    // each step doubles the 2-adic precision.
+    std::vector<C> X(E * E, 0);
+    for (int i = 0; i < E; i++) X[i * E + i] = 1;
+    for (int it = 0; (1 << it) < 2 * T::K; it++) {
+      std::vector<C> BX = matmul(Bm, X);
+      for (int i = 0; i < E * E; i++) BX[i] = (C)(0 - BX[i]);
+      for (int i = 0; i < E; i++) BX[i * E + i] += 2;
+      X = matmul(X, BX);
+    }
+    std::vector<C> M = matmul(Pm, X);
+    frob_t.assign(E * E, 0);
+    for (int o = 0; o < E; o++)
+      for (int j = 0; j < E; j++) frob_t[j * E + o] = (L)M[o * E + j];
+    // self-check against the plain definition on a fixed element
+    G x;
+    uint64_t s = 0x9E3779B97F4A7C15ull;
+    for (int l = 0; l < E; l++) {
+      s = s * 6364136223846793005ull + 1442695040888963407ull;
+      x.v[l] = (L)(s >> 17);
+    }
+    G z = G::zero();
+    for (int o = 0; o < E; o++) {
+      C acc = 0;
+      for (int j = 0; j < E; j++) acc += (C)frob_t[j * E + o] * (C)x.v[j];
+      z.v[o] = (L)acc;
+    }
+    for (int q = 0; q < n; q++) z = z * z;
+    if (!(z == lift128(x))) throw Error(NENYA_E_DEVICE, "Frobenius lift matrix failed its self-check");
   }
 };
 

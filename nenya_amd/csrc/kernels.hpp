@@ -465,65 +465,137 @@ __global__ void __launch_bounds__(1024) k_sum_partials(const LT<T>* partials, ui
 }
 
 // ============================================================================
-// Teichmuller lift: x <- x^(2^128) (one Frobenius pass; exact lift for k<=64,
-// teichmuller.rs:104-114 iterates the same fixed point).  One 256-thread
-// block per element: 8 groups split the 32 beta terms of each squaring.
+// Teichmuller lift of Fiat-Shamir challenges
 // ============================================================================
-// Squaring uses the symmetry of a*a: lane t's circular sum pairs term
-// (i, j) with (j, i), i + j = t mod 32.  With h0 = floor(t/2), h1 = t - h0 the
-// pairs are (h0 - d, h1 + d) for d = 0..16: weight 2, except the diagonals
-// (t even: d = 0 and d = 16, weight 1) and the repeat d = 16 for odd t
-// (weight 0).  A pair lies in the wrap-around part iff i > t.  Group g takes
-// d = g, g + 8 (and group 0 also d = 16); the 8 groups' 14 x 32 raw sums are
-// added by all 256 threads in parallel, then group 0 reduces.
+
+// Element of C-typed LDS as 4 values.
+template <class C> __device__ __forceinline__ void ldc4(const C* p, C o[4]) {
+  if constexpr (sizeof(C) == 4) {
+    uint4 v = *reinterpret_cast<const uint4*>(p);
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+  } else {
+    const ulonglong2* q = reinterpret_cast<const ulonglong2*>(p);
+    ulonglong2 a = q[0], b = q[1];
+    o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+  }
+}
+
+// Cross-lane move inside a row of 16 lanes (DPP; CTRL is the dpp_ctrl code).
+template <int CTRL, class C> __device__ __forceinline__ C dpp(C v) {
+  if constexpr (sizeof(C) == 4) {
+    return (C)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+  } else {
+    int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, 0xF, 0xF, false);
+    int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, 0xF, 0xF, false);
+    return ((C)(uint32_t)hi << 32) | (C)(uint32_t)lo;
+  }
+}
+constexpr int DPP_XOR1 = 0xB1;         // quad_perm [1,0,3,2]
+constexpr int DPP_XOR2 = 0x4E;         // quad_perm [2,3,0,1]
+constexpr int DPP_HALF_MIRROR = 0x141; // lane i <-> 7-i within 8 lanes
+
+// Teichmuller lift of `x[blockIdx.x]` in place.  The reference
+// (teichmuller.rs:104-114) squares (ceil(log2 k)+1)*128 times; its result is
+// the unique Teichmuller element congruent to x mod 2 (x^(2^128) already is
+// that element, as 128 >= k-1).  With sigma the Frobenius automorphism of
+// GR(2^k,128), the same element is (sigma^-(k-1) x)^(2^(k-1)): if
+// z = sigma^-n(x) = t(1+2w) with t Teichmuller, z^(2^n) = t^(2^n) = Teich(x)
+// once n+1 >= k.  sigma^-(k-1) is Z/2^k-linear; `frob_t` holds its 128x128
+// matrix transposed ([j][o], computed on the host), so the lift is one matvec
+// and `squarings` = k-1 squarings.
+//
+// Squaring with 256 threads: wave w owns result coefficients t = 8w..8w+7,
+// lane (t, dl) takes the unordered pairs {i, j}, i + j = t mod 32, numbered
+// d = dl and dl+8 (i = t/2 - d, j = t - i), plus d = 16 on dl = 0 for even t.
+// Off-diagonal pairs count twice.  The 8 lanes of a t sum their raw circ/hi
+// terms with a DPP transpose-reduction; the reduction Y^32 = -Y^3 - Y - alpha
+// is applied by 128 threads reading the sums from LDS.
 template <class T>
-__global__ void __launch_bounds__(256) k_lift(LT<T>* x, int squarings) {
+__global__ void __launch_bounds__(256) k_lift(LT<T>* x, const LT<T>* frob_t, int squarings) {
   using C = CT<T>;
-  __shared__ __attribute__((aligned(16))) LT<T> sx[E];
-  __shared__ C red[8][14][32];
-  __shared__ C sum[14][32];
-  int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  __shared__ __attribute__((aligned(16))) C sx[E];
+  __shared__ C part[E];
+  __shared__ C sred[2][32][4];  // [0] reduced circ, [1] reduced hi (before the cascade)
+  const int tid = threadIdx.x;
   LT<T>* el = x + (uint64_t)blockIdx.x * E;
-  if (g == 0) stage<T>(sx, el, t);
+  if (tid < E) sx[tid] = (C)el[tid];
   __syncthreads();
-  const int h0 = t >> 1, h1 = t - h0;
+  {
+    const int o = tid & (E - 1), h = tid >> 7;
+    C s = 0;
+#pragma unroll 8
+    for (int j = 64 * h; j < 64 * h + 64; j++) s += (C)frob_t[(size_t)j * E + o] * sx[j];
+    if (h) part[o] = s;
+    __syncthreads();
+    if (!h) sx[o] = s + part[o];
+    __syncthreads();
+  }
+  const int lane = tid & 63, dl = lane & 7;
+  const int t = 8 * (tid >> 6) + (lane >> 3), h0 = t >> 1, h1 = t - h0;
+  const bool even = !(t & 1);
+  const bool b2 = dl & 4, b1 = dl & 2, b0 = dl & 1;
   for (int it = 0; it < squarings; it++) {
-    Acc<T> acc;
-    acc.clear();
-    for (int d = g; d <= 16; d += 8) {
-      int i = (h0 - d) & 31, j = (h1 + d) & 31;
-      C w = (d == 0 || d == 16) ? ((t & 1) ? (d == 16 ? 0 : 2) : 1) : 2;
+    C ac[7], ah[7];
+#pragma unroll
+    for (int c = 0; c < 7; c++) ac[c] = ah[c] = 0;
+    auto pair = [&](int d) {
+      const int i = (h0 - d) & 31, j = (h1 + d) & 31;
       C a[4], b[4], p[7];
-      ld4<T>(sx + 4 * i, a);
-      ld4<T>(sx + 4 * j, b);
+      ldc4(sx + 4 * i, a);
+      ldc4(sx + 4 * j, b);
+      if (!(even && (d == 0 || d == 16))) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) a[c] += a[c];
+      }
       d_raw(a, b, p);
       const C m = (i > t) ? (C)~(C)0 : (C)0;
 #pragma unroll
       for (int c = 0; c < 7; c++) {
-        C v = p[c] * w;
-        acc.c[c] += v;
-        acc.h[c] += v & m;
+        ac[c] += p[c];
+        ah[c] += p[c] & m;
       }
+    };
+    pair(dl);
+    pair(dl + 8);
+    if (dl == 0 && even) pair(16);
+    C v[8];
+    d_red(ac, v);
+    d_red(ah, v + 4);
+    // lane dl ends with the sum over the 8 lanes of v[dl]
+    C k4[4], k2[2];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      C send = b2 ? v[c] : v[c + 4];
+      k4[c] = (b2 ? v[c + 4] : v[c]) + dpp<DPP_HALF_MIRROR>(send);
     }
-    for (int c = 0; c < 7; c++) { red[g][c][t] = acc.c[c]; red[g][7 + c][t] = acc.h[c]; }
-    __syncthreads();
-    for (int id = threadIdx.x; id < 14 * 32; id += 256) {
-      int v = id >> 5, l = id & 31;
-      C s2 = 0;
-      for (int gg = 0; gg < 8; gg++) s2 += red[gg][v][l];
-      sum[v][l] = s2;
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      C send = b1 ? k4[c] : k4[c + 2];
+      k2[c] = (b1 ? k4[c + 2] : k4[c]) + dpp<DPP_XOR2>(send);
     }
+    C sum = (b0 ? k2[1] : k2[0]) + dpp<DPP_XOR1>(b0 ? k2[0] : k2[1]);
+    sred[dl >> 2][t][dl & 3] = sum;
     __syncthreads();
-    if (g == 0) {
-      Acc<T> a2;
-      for (int c = 0; c < 7; c++) { a2.c[c] = sum[c][t]; a2.h[c] = sum[7 + c][t]; }
-      C o[4];
-      acc_final<T>(a2, t, o);
-      st4<T>(sx + 4 * t, o);
+    if (tid < E) {
+      const int u = tid >> 2, c = tid & 3;
+      auto hc = [&](int s, int cc) -> C {  // hi of position 32+s after the 61/62 cascade
+        C r = sred[1][s][cc];
+        if (s < 2) r -= sred[1][s + 29][cc];
+        return r;
+      };
+      C lo = sred[0][u][c] - sred[1][u][c];
+      if (u >= 1) lo -= hc(u - 1, c);
+      if (u >= 3) lo -= hc(u - 3, c);
+      C out;
+      if (c == 0) out = lo + hc(u, 3);
+      else if (c == 1) out = lo - hc(u, 0) + hc(u, 3);
+      else if (c == 2) out = lo - hc(u, 1);
+      else out = lo - hc(u, 2);
+      sx[tid] = out;
     }
     __syncthreads();
   }
-  if (g == 0) stage<T>(el, sx, t);
+  if (tid < E) el[tid] = (LT<T>)sx[tid];
 }
 
 // ============================================================================

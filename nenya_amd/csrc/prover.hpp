@@ -160,6 +160,7 @@ template <class T> struct Engine {
   const Consts<T>& K;
   DBuf<L> d_basis, d_pw;
   DBuf<L> d_one;
+  DBuf<L> d_frob;  // sigma^-(k-1), transposed [j][o]
   KProf* prof = nullptr;
 
   // Launch on the engine's stream, bracketed by events when profiling.
@@ -185,12 +186,14 @@ template <class T> struct Engine {
     d_basis = DBuf<L>(pool, 32 * E);
     d_pw = DBuf<L>(pool, K.psi_pw.size());
     d_one = DBuf<L>(pool, E);
+    d_frob = DBuf<L>(pool, E * E);
     std::vector<L> b(32 * E);
     for (int i = 0; i < 32; i++) std::memcpy(&b[i * E], K.basis[i].v, E * sizeof(L));
     G one = G::one();
     HIP_CHECK(hipMemcpyAsync(d_basis.p, b.data(), b.size() * sizeof(L), hipMemcpyHostToDevice, st));
     HIP_CHECK(hipMemcpyAsync(d_pw.p, K.psi_pw.data(), K.psi_pw.size() * sizeof(L), hipMemcpyHostToDevice, st));
     HIP_CHECK(hipMemcpyAsync(d_one.p, one.v, E * sizeof(L), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(d_frob.p, K.frob_t.data(), E * E * sizeof(L), hipMemcpyHostToDevice, st));
     HIP_CHECK(hipStreamSynchronize(st));
   }
 
@@ -295,7 +298,8 @@ template <class T> struct Engine {
     for (size_t i = 0; i < cnt; i++)
       for (int l = 0; l < E; l++) nat[i].v[l] = (L)rng.limb(T::K);
     DBuf<L> d = put(nat.data(), cnt);
-    launch("k_lift", cnt * 2 * EB, cnt * 128 * 32 * PM, (k_lift<T>), dim3(cnt), dim3(256), d.p, 128);
+    launch("k_lift", cnt * 2 * EB, cnt * (E * E + (T::K - 1) * 16.5 * PM), (k_lift<T>), dim3(cnt), dim3(256), d.p,
+           d_frob.p, T::K - 1);
     d2h(nat.data(), d.p, cnt * sizeof(G));
     if (dev_out) *dev_out = std::move(d);
     return nat;
