@@ -11,6 +11,8 @@ FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # never emit packed-FP32 instructions (no floating point here anyway)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in ordinary VGPRs (the integer Horner steps shift them)
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
 ]
 
 

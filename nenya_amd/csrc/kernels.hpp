@@ -510,14 +510,14 @@ constexpr int DPP_HALF_MIRROR = 0x141; // lane i <-> 7-i within 8 lanes
 // Off-diagonal pairs count twice.  The 8 lanes of a t sum their raw circ/hi
 // terms with a DPP transpose-reduction; the reduction Y^32 = -Y^3 - Y - alpha
 // is applied by 128 threads reading the sums from LDS.
+// Lifts el in place with a 256-thread block; the lifted value is left in sx
+// (LDS, C-typed, 16-byte aligned) as well.
 template <class T>
-__global__ void __launch_bounds__(256) k_lift(LT<T>* x, const LT<T>* frob_t, int squarings) {
+__device__ __forceinline__ void teich_lift_block(LT<T>* el, const LT<T>* frob_t, int squarings, CT<T>* sx) {
   using C = CT<T>;
-  __shared__ __attribute__((aligned(16))) C sx[E];
   __shared__ C part[E];
   __shared__ C sred[2][32][4];  // [0] reduced circ, [1] reduced hi (before the cascade)
   const int tid = threadIdx.x;
-  LT<T>* el = x + (uint64_t)blockIdx.x * E;
   if (tid < E) sx[tid] = (C)el[tid];
   __syncthreads();
   {
@@ -596,6 +596,12 @@ __global__ void __launch_bounds__(256) k_lift(LT<T>* x, const LT<T>* frob_t, int
     __syncthreads();
   }
   if (tid < E) el[tid] = (LT<T>)sx[tid];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_lift(LT<T>* x, const LT<T>* frob_t, int squarings) {
+  __shared__ __attribute__((aligned(16))) CT<T> sx[E];
+  teich_lift_block<T>(x + (uint64_t)blockIdx.x * E, frob_t, squarings, sx);
 }
 
 // ============================================================================

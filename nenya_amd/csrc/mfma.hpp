@@ -1,0 +1,253 @@
+// Products by a fixed element on the gfx950 matrix cores.
+//
+// For fixed r in GR(2^k,128) the map x -> r*x is Z/2^k-linear: r*x = M_r x
+// with M_r the 128x128 matrix whose column j is r*e_j (e_{4i+c} = X^c Y^i,
+// the limb order of an element).  Applying r to many elements is therefore a
+// GEMM over Z/2^k, which int8 MFMA computes exactly through balanced digits:
+// u = sum_p 256^p u_p (mod 2^k) with u_p in [-128, 127]; the bytes of
+// (u + 0x80..80) ^ 0x80..80 are those digits as int8.  With ND = k/8,
+//   M x = sum_{s<ND} 256^s sum_{p+q=s} M_p x_q   (mod 2^k),
+// every M_p x_q is an exact small-integer product (|.| <= 128 * 2^14), and
+// the i32 accumulators wrap mod 2^32, so Horner in 256 over s leaves M x mod
+// 2^32 in one accumulator set: acc = ((Y_3 * 256 + Y_2) * 256 + Y_1) * 256 + Y_0.
+//
+// MFMA used: v_mfma_i32_32x32x32_i8.  Lane l holds A[row l%32][k] and
+// B[k][col l%32] for the 16 k of its half h = l/32 in the bytes of its
+// 16-byte operands, and C[8(i/4) + 4h + i%4][l%32] in accumulator i.  Rows
+// are limbs of the product, columns are elements, k runs over the limbs of
+// the input.  Any assignment of the 32 k of a block to (h, byte) gives the
+// same product as long as A and B share it; byte b of half h carries
+// k = kappa(h, b) = 8(b/4) + 4h + b%4, which makes a lane's input limbs the
+// same as its output rows (the fold adds lo at exactly the limbs it loaded).
+#pragma once
+#include "kernels.hpp"
+
+namespace nenya {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+template <class T> constexpr bool kMx = T::K <= 32;  // 64-bit limbs stay on the vector ALUs
+template <class T> constexpr int kND = T::K / 8;     // balanced base-256 digits per limb
+
+// Balanced digits of v (mod 2^(8 ND)): byte p of the result is digit p.
+template <int ND> __device__ __forceinline__ uint32_t bal_digits(uint32_t v) {
+  if constexpr (ND == 1) return v & 0xffu;
+  else if constexpr (ND == 2) return ((v + 0x8080u) ^ 0x8080u) & 0xffffu;
+  else return (v + 0x80808080u) ^ 0x80808080u;
+}
+
+// 16 values (k = 16 consecutive limbs) -> ND digit planes of 16 int8 each.
+template <int ND> __device__ __forceinline__ void planes16(const uint32_t v[16], v4i out[ND]) {
+  uint32_t z[16];
+#pragma unroll
+  for (int b = 0; b < 16; b++) z[b] = bal_digits<ND>(v[b]);
+#pragma unroll
+  for (int p = 0; p < ND; p++) {
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+      uint32_t x = 0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) x |= ((z[4 * w + j] >> (8 * p)) & 0xffu) << (8 * j);
+      out[p][w] = (int)x;
+    }
+  }
+}
+
+// Fragment index of M_r digit plane p, k-block kb, row-block rb (16 bytes per lane).
+__device__ __forceinline__ int frag_at(int p, int kb, int rb) { return (p * 4 + kb) * 4 + rb; }
+
+// v[i] = Y^i v[0] for i < 32 by Y^32 = -Y^3 - Y - alpha (all threads of the
+// block call this; threads < 128 own one limb each).
+template <class C> __device__ __forceinline__ void mulmat_chain(C (*v)[E]) {
+  const int tid = threadIdx.x;
+  for (int i = 1; i < 32; i++) {
+    if (tid < E) {
+      const int t = tid >> 2, c = tid & 3;
+      const C* o = v[i - 1];
+      C x;
+      if (t == 0) {  // -alpha * o_31 = (h3, h3 - h0, -h1, -h2)
+        const C h0 = o[124], h1 = o[125], h2 = o[126], h3 = o[127];
+        x = (c == 0) ? h3 : (c == 1) ? (C)(h3 - h0) : (c == 2) ? (C)(0 - h1) : (C)(0 - h2);
+      } else {
+        x = o[tid - 4];
+        if (t == 1 || t == 3) x -= o[124 + c];
+      }
+      v[i][tid] = x;
+    }
+    __syncthreads();
+  }
+}
+
+// A-fragments of M_r from v[i] = Y^i r: column 4i+c of M_r is X^c v_i with
+// X*(a0,a1,a2,a3) = (-a3, a0-a3, a1, a2) (X^4 = -X - 1).  Item (row, kb, h)
+// writes the 16 bytes of lane (row%32)+32h in fragment (p, kb, row/32): word
+// j holds columns 4i..4i+3 with i = 8kb + 2j + h (k = 32kb + kappa(h, 4j+c)).
+template <class T> __device__ __forceinline__ void mulmat_frag(const CT<T> (*v)[E], uint32_t* frag) {
+  using C = CT<T>;
+  constexpr int ND = kND<T>;
+  for (int id = threadIdx.x; id < E * 8; id += blockDim.x) {
+    const int row = id & (E - 1), kb = (id >> 7) & 3, h = id >> 9;
+    const int t = row >> 2, cp = row & 3;
+    uint32_t words[ND][4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int i = 8 * kb + 2 * j + h;
+      C a[4] = {v[i][4 * t], v[i][4 * t + 1], v[i][4 * t + 2], v[i][4 * t + 3]};
+      uint32_t z[4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        z[c] = bal_digits<ND>((uint32_t)a[cp]);
+        C n0 = 0 - a[3], n1 = a[0] - a[3], n2 = a[1], n3 = a[2];
+        a[0] = n0; a[1] = n1; a[2] = n2; a[3] = n3;
+      }
+#pragma unroll
+      for (int p = 0; p < ND; p++) {
+        uint32_t wd = 0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) wd |= ((z[c] >> (8 * p)) & 0xffu) << (8 * c);
+        words[p][j] = wd;
+      }
+    }
+    const int rb = row >> 5, lane = (row & 31) + 32 * h;
+#pragma unroll
+    for (int p = 0; p < ND; p++)
+      reinterpret_cast<uint4*>(frag)[frag_at(p, kb, rb) * 64 + lane] =
+          make_uint4(words[p][0], words[p][1], words[p][2], words[p][3]);
+  }
+}
+
+// Standalone M_r fragments for an r already in device memory (one block).
+template <class T>
+__global__ void __launch_bounds__(256) k_mulmat(const LT<T>* r, uint32_t* frag) {
+  using C = CT<T>;
+  __shared__ C v[32][E];
+  if (threadIdx.x < E) v[0][threadIdx.x] = (C)r[threadIdx.x];
+  __syncthreads();
+  mulmat_chain<C>(v);
+  mulmat_frag<T>(v, frag);
+}
+
+// Lift one challenge (as k_lift) and emit the M_r fragments of the result.
+template <class T>
+__global__ void __launch_bounds__(256) k_lift_mulmat(LT<T>* x, const LT<T>* frob_t, int squarings, uint32_t* frag) {
+  using C = CT<T>;
+  __shared__ __attribute__((aligned(16))) C v[32][E];
+  teich_lift_block<T>(x, frob_t, squarings, v[0]);
+  __syncthreads();
+  mulmat_chain<C>(v);
+  mulmat_frag<T>(v, frag);
+}
+
+// The 16 limbs of half h in k-block kb (kappa order): groups of 4 at 8j + 4h.
+template <class T> __device__ __forceinline__ void ld16k(const LT<T>* el, int kb, int h, uint32_t v[16]) {
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    CT<T> x[4];
+    ld4<T>(el + 32 * kb + 8 * j + 4 * h, x);
+#pragma unroll
+    for (int c = 0; c < 4; c++) v[4 * j + c] = (uint32_t)x[c];
+  }
+}
+
+// One 32-element fold tile: loads lo/hi, forms the digit planes of hi - lo,
+// then row-blocks rb0 .. rb0+NRB-1 of lo + M_r (hi - lo).  A points at the
+// fragments (LDS or global).
+template <class T, int NRB>
+__device__ __forceinline__ void fold_tile(const v4i* A, const LT<T>* src, LT<T>* dst, uint64_t m, uint64_t total,
+                                          uint64_t src_stride, uint64_t dst_stride, uint64_t tile, int rb0) {
+  using C = CT<T>;
+  using L = LT<T>;
+  constexpr int ND = kND<T>;
+  const int l = threadIdx.x & 63, col = l & 31, h = l >> 5;
+  const uint64_t e = tile * 32 + col;
+  const bool ok = e < total;
+  const uint64_t tab = ok ? e / m : 0, i = ok ? e % m : 0;
+  const L* lo = src + (tab * src_stride + i) * E;
+  const L* hi = lo + m * E;
+  uint32_t a[4][16], d[4][16];
+  if (ok) {
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++) {
+      ld16k<T>(lo, kb, h, a[kb]);
+      ld16k<T>(hi, kb, h, d[kb]);
+    }
+  } else {
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++)
+#pragma unroll
+      for (int b = 0; b < 16; b++) a[kb][b] = d[kb][b] = 0;
+  }
+  v4i B[4][ND];
+#pragma unroll
+  for (int kb = 0; kb < 4; kb++) {
+#pragma unroll
+    for (int b = 0; b < 16; b++) d[kb][b] -= a[kb][b];
+    planes16<ND>(d[kb], B[kb]);
+  }
+  constexpr int NU = NRB >= 2 ? 2 : 1;  // row-blocks per accumulator set
+#pragma unroll
+  for (int rp = 0; rp < NRB; rp += NU) {
+    v16i acc[NU];
+#pragma unroll
+    for (int u = 0; u < NU; u++) acc[u] = (v16i)0;
+#pragma unroll
+    for (int s = ND - 1; s >= 0; s--) {
+      if (s != ND - 1) {
+#pragma unroll
+        for (int u = 0; u < NU; u++) acc[u] <<= 8;
+      }
+#pragma unroll
+      for (int p = 0; p <= s; p++)
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++)
+#pragma unroll
+          for (int u = 0; u < NU; u++)
+            acc[u] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[frag_at(p, kb, rb0 + rp + u) * 64 + l], B[kb][s - p],
+                                                           acc[u], 0, 0, 0);
+    }
+    if (ok) {
+      L* out = dst + (tab * dst_stride + i) * E;
+#pragma unroll
+      for (int u = 0; u < NU; u++) {
+        const int rb = rb0 + rp + u;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          C x[4];
+#pragma unroll
+          for (int c = 0; c < 4; c++) x[c] = (C)(a[rb][4 * j + c] + (uint32_t)acc[u][4 * j + c]);
+          st4<T>(out + 32 * rb + 8 * j + 4 * h, x);
+        }
+      }
+    }
+  }
+}
+
+// Sumcheck fold on the matrix cores (same result as k_fold):
+//   dst[tab][i] = lo + r (hi - lo),  lo = src[tab][i], hi = src[tab][i+m].
+// Bulk mode: persistent blocks of 4 waves, a wave folds 32 elements per
+// tile, the M_r fragments (ND*16 KB) sit in LDS for the block's lifetime.
+// Split mode (few tiles): block b takes tile b and wave w its row-block w,
+// with the fragments read from global memory (L2-resident).
+template <class T>
+__global__ void __launch_bounds__(256) k_fold_mx(const uint32_t* frag, const LT<T>* src, LT<T>* dst, uint64_t m,
+                                                 uint64_t ntab, uint64_t src_stride, uint64_t dst_stride, int split) {
+  constexpr int ND = kND<T>;
+  __shared__ v4i sA[ND * 16 * 64];
+  const v4i* gA = reinterpret_cast<const v4i*>(frag);
+  const uint64_t total = m * ntab, ntiles = (total + 31) / 32;
+  if (split) {
+    fold_tile<T, 1>(gA, src, dst, m, total, src_stride, dst_stride, blockIdx.x, threadIdx.x >> 6);
+    return;
+  }
+  for (int x = threadIdx.x; x < ND * 16 * 64; x += 256) sA[x] = gA[x];
+  __syncthreads();
+  for (uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntiles; tile += (uint64_t)gridDim.x * 4) {
+    // keeps the A-fragment reads inside the loop (hoisted, they would pin
+    // ND*64 registers)
+    asm volatile("" ::: "memory");
+    fold_tile<T, 4>(sA, src, dst, m, total, src_stride, dst_stride, tile, 0);
+  }
+}
+
+}  // namespace nenya

@@ -11,6 +11,7 @@
 
 #include "host.hpp"
 #include "kernels.hpp"
+#include "mfma.hpp"
 
 namespace nenya {
 
@@ -239,6 +240,32 @@ template <class T> struct Engine {
     launch("k_fold", m * ntab * 3 * EB, m * ntab * 32 * PM, (k_fold<T>), dim3(grid_for(m * ntab, GPB)), dim3(GB), r, src,
            dst, m, ntab, sstride, dstride);
   }
+  // x -> r*x as matrix-core fragments (k <= 32); folds by the same
+  // challenge share one.
+  struct MulMat {
+    DBuf<uint32_t> frag;
+    const L* r = nullptr;
+  };
+  MulMat mulmat(const L* r) {
+    MulMat mm;
+    mm.r = r;
+    if constexpr (kMx<T>) {
+      mm.frag = DBuf<uint32_t>(pool, (size_t)kND<T> * 4096);
+      launch("k_mulmat", EB + kND<T> * 16384.0, 0, (k_mulmat<T>), dim3(1), dim3(256), r, mm.frag.p);
+    }
+    return mm;
+  }
+  void fold(const MulMat& mm, const L* src, L* dst, uint64_t m, uint64_t ntab, uint64_t sstride, uint64_t dstride) {
+    if constexpr (kMx<T>) {
+      uint64_t ntiles = (m * ntab + 31) / 32;
+      const int split = ntiles <= 256;  // few tiles: 4 waves per tile, fragments from L2
+      launch("k_fold_mx", m * ntab * 3 * EB, m * ntab * 32 * PM, (k_fold_mx<T>),
+             dim3(split ? (unsigned)ntiles : grid_for(ntiles, 4, 512)), dim3(256), (const uint32_t*)mm.frag.p, src, dst,
+             m, ntab, sstride, dstride, split);
+    } else {
+      fold(mm.r, src, dst, m, ntab, sstride, dstride);
+    }
+  }
   void marg(const L* s, L* eg, uint64_t m) {
     launch("k_marg", m * 3 * EB, m * 32 * PM, (k_marg<T>), dim3(grid_for(m, GPB)), dim3(GB), s, eg, m);
   }
@@ -293,20 +320,34 @@ template <class T> struct Engine {
   // ---------------- Fiat-Shamir challenges ----------------
   // R2Sampler::sample (sampling.rs:57-68): 128 L::random draws then the
   // Teichmuller lift; the lift runs on the device.
-  std::vector<G> sample(StdRng& rng, size_t cnt, DBuf<L>* dev_out = nullptr) {
+  std::vector<G> sample(StdRng& rng, size_t cnt, DBuf<L>* dev_out = nullptr, MulMat* mm = nullptr) {
     std::vector<G> nat(cnt);
     for (size_t i = 0; i < cnt; i++)
       for (int l = 0; l < E; l++) nat[i].v[l] = (L)rng.limb(T::K);
     DBuf<L> d = put(nat.data(), cnt);
-    launch("k_lift", cnt * 2 * EB, cnt * (E * E + (T::K - 1) * 16.5 * PM), (k_lift<T>), dim3(cnt), dim3(256), d.p,
-           d_frob.p, T::K - 1);
+    const double lm = E * E + (T::K - 1) * 16.5 * PM;
+    bool fused = false;
+    if constexpr (kMx<T>) {
+      if (mm && cnt == 1) {
+        mm->frag = DBuf<uint32_t>(pool, (size_t)kND<T> * 4096);
+        launch("k_lift_mulmat", 2 * EB, lm, (k_lift_mulmat<T>), dim3(1), dim3(256), d.p, d_frob.p, T::K - 1,
+               mm->frag.p);
+        fused = true;
+      }
+    }
+    if (!fused)
+      launch("k_lift", cnt * 2 * EB, cnt * lm, (k_lift<T>), dim3(cnt), dim3(256), d.p, d_frob.p, T::K - 1);
     d2h(nat.data(), d.p, cnt * sizeof(G));
+    if (mm) {
+      mm->r = d.p;
+      if (!fused) *mm = mulmat(d.p);
+    }
     if (dev_out) *dev_out = std::move(d);
     return nat;
   }
-  G challenge(Transcript& tr, const char* label, DBuf<L>* dev = nullptr) {
+  G challenge(Transcript& tr, const char* label, DBuf<L>* dev = nullptr, MulMat* mm = nullptr) {
     StdRng rng = tr.sampler(label);
-    return sample(rng, 1, dev)[0];
+    return sample(rng, 1, dev, mm)[0];
   }
 
   // ---------------- PCS ----------------
@@ -515,11 +556,12 @@ template <class T> struct Engine {
       tr.elems("sc/round", q, 3);
       for (int i = 0; i < 3; i++) o.rounds.push_back(q[i]);
       DBuf<L> dr;
-      G rc = challenge(tr, "sc/challenge", &dr);
+      MulMat mr;
+      G rc = challenge(tr, "sc/challenge", &dr, &mr);
       o.chal.push_back(rc);
       for (int t = 0; t < ntab; t++) {
         L* dst = work.p + (size_t)t * (N / 2) * E;
-        fold(dr.p, cur[t], dst, m, 1, 0, 0);
+        fold(mr, cur[t], dst, m, 1, 0, 0);
         cur[t] = dst;
       }
       len = m;
@@ -567,11 +609,12 @@ template <class T> struct Engine {
       tr.elems("sc/round", q, 3);
       for (int i = 0; i < 3; i++) o.rounds.push_back(q[i]);
       DBuf<L> dr;
-      G rc = challenge(tr, "sc/challenge", &dr);
+      MulMat mr;
+      G rc = challenge(tr, "sc/challenge", &dr, &mr);
       o.chal.push_back(rc);
       c = q[0] + rc * (q[1] + rc * q[2]);
-      fold(dr.p, f, fw.p, m, ntab, fs, H);
-      fold(dr.p, g, gw.p, m, ntab, gs, H);
+      fold(mr, f, fw.p, m, ntab, fs, H);
+      fold(mr, g, gw.p, m, ntab, gs, H);
       f = fw.p;
       g = gw.p;
       fs = gs = H;
