@@ -112,6 +112,35 @@ template <class T> struct G4 {
   }
 };
 
+// Balanced base-256 digits of v mod 2^(8 nd): byte p is digit p as int8.
+inline uint32_t bal_digits_host(uint32_t v, int nd) {
+  if (nd == 1) return v & 0xffu;
+  if (nd == 2) return ((v + 0x8080u) ^ 0x8080u) & 0xffffu;
+  return (v + 0x80808080u) ^ 0x80808080u;
+}
+
+// int8 MFMA A-fragments (v_mfma_i32_32x32x32_i8) of a 128 x 32*kblocks matrix
+// over Z/2^(8 nd) given by at(row, k): fragment (p, kb, rb) = digit plane p,
+// k-block kb, row-block rb; lane (row%32) + 32h, byte b carries
+// k = 32 kb + 8(b/4) + 4h + b%4.  Layout [p][kb][rb][lane][16 bytes].
+template <class F> std::vector<uint32_t> mx_frags(int nd, int kblocks, F at) {
+  std::vector<uint32_t> fr((size_t)nd * kblocks * 4 * 64 * 4, 0);
+  for (int kb = 0; kb < kblocks; kb++)
+    for (int rb = 0; rb < 4; rb++)
+      for (int lane = 0; lane < 64; lane++)
+        for (int wd = 0; wd < 4; wd++) {
+          const int h = lane >> 5, row = 32 * rb + (lane & 31);
+          uint32_t z[4];
+          for (int c = 0; c < 4; c++) z[c] = bal_digits_host(at(row, 32 * kb + 8 * wd + 4 * h + c), nd);
+          for (int p = 0; p < nd; p++) {
+            uint32_t word = 0;
+            for (int c = 0; c < 4; c++) word |= ((z[c] >> (8 * p)) & 0xffu) << (8 * c);
+            fr[((((size_t)p * kblocks + kb) * 4 + rb) * 64 + lane) * 4 + wd] = word;
+          }
+        }
+  return fr;
+}
+
 template <class T> struct Consts {
   using L = typename T::L;
   using C = typename T::C;
@@ -120,6 +149,7 @@ template <class T> struct Consts {
   Gr<T> basis[32];
   std::vector<L> psi_pw;  // [16][32][4]: t_i^k
   std::vector<L> frob_t;  // [j][o]: matrix of sigma^-(k-1), transposed
+  std::vector<uint32_t> b2frag;  // MFMA fragments of [b_s b_s'] (128 x 1024), k <= 32
   L inv15;
 
   Consts() {
@@ -187,6 +217,13 @@ template <class T> struct Consts {
       }
     }
     frobenius_setup();
+    if (K <= 32) {
+      // products b_s b_s' of the packing basis, column k = 32 s + s'
+      std::vector<Gr<T>> b2(1024);
+      for (int a = 0; a < 32; a++)
+        for (int b = a; b < 32; b++) b2[32 * a + b] = b2[32 * b + a] = basis[a] * basis[b];
+      b2frag = mx_frags(K / 8, 32, [&](int row, int k) { return (uint32_t)b2[k].v[row]; });
+    }
   }
   static const Consts& get() {
     static Consts c;

@@ -250,4 +250,86 @@ __global__ void __launch_bounds__(256) k_fold_mx(const uint32_t* frag, const LT<
   }
 }
 
+// ---------------------------------------------------------------------------
+// Layered matvec on the matrix cores (k <= 32).
+//
+// layered_matvec (general.rs:211-230) sums D_hat(l,p,q) * rho_l[q] over the
+// diagonal cells of block row p.  pack_r2 is Z/2^k-linear with basis
+// b_s = pack_r2(e_s), so with D_hat = sum_s D[s] b_s and
+// rho_l[q] = sum_s' w[32q + (s'+l) mod 32] b_s' (l = (c - r) mod 32 of the
+// entry, D[s] = A[32p+s][c]):
+//   sum_cells D_hat rho = sum_{s,s'} C_p[s][s'] (b_s b_s'),
+//   C_p[s][s'] = sum_{entries (32p+s, c, v)} v * w[32(c/32) + (s'+c-s) mod 32].
+// k_cmat forms C_p from the row CSR of the matrix; k_cgemm multiplies by
+// the fixed 128 x 1024 matrix of the b_s b_s' (fragments built on the host).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(256) k_cmat(const uint32_t* ptr, const uint32_t* col, const LT<T>* val,
+                                              const LT<T>* w, LT<T>* cmat, uint64_t nb) {
+  using C = CT<T>;
+  const uint32_t lane = threadIdx.x & 31;
+  for (uint64_t p = (uint64_t)blockIdx.x * 8 + (threadIdx.x >> 5); p < nb; p += (uint64_t)gridDim.x * 8) {
+    for (uint32_t s = 0; s < 32; s++) {
+      const uint64_t r = 32 * p + s;
+      C acc = 0;
+      for (uint32_t k = ptr[r]; k < ptr[r + 1]; k++) {
+        const uint32_t c = col[k];
+        acc += (C)val[k] * (C)w[(c & ~31u) + ((lane + c - s) & 31u)];
+      }
+      cmat[p * 1024 + 32 * s + lane] = (LT<T>)acc;
+    }
+  }
+}
+
+// out[p] = sum_k B2[.][k] C_p[k] over k = 32s + s' (1024).  Block: 4 waves on
+// the same 32 block rows, wave w computing output rows 32w..32w+31; one
+// accumulator per digit class (K streams, so no Horner), combined at the end.
+template <class T>
+__global__ void __launch_bounds__(256) k_cgemm(const uint32_t* frag, const LT<T>* cmat, LT<T>* out, uint64_t nb) {
+  using C = CT<T>;
+  constexpr int ND = kND<T>;
+  const v4i* A = reinterpret_cast<const v4i*>(frag);
+  const int l = threadIdx.x & 63, col = l & 31, h = l >> 5, rb = threadIdx.x >> 6;
+  const uint64_t ntiles = (nb + 31) / 32;
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint64_t p = tile * 32 + col;
+    const bool ok = p < nb;
+    const LT<T>* cp = cmat + (ok ? p : 0) * 1024;
+    v16i acc[ND];
+#pragma unroll
+    for (int d = 0; d < ND; d++) acc[d] = (v16i)0;
+    for (int kb = 0; kb < 32; kb++) {
+      uint32_t v[16];
+      if (ok) {
+        ld16k<T>(cp, kb, h, v);
+      } else {
+#pragma unroll
+        for (int b = 0; b < 16; b++) v[b] = 0;
+      }
+      v4i B[ND];
+      planes16<ND>(v, B);
+#pragma unroll
+      for (int pd = 0; pd < ND; pd++) {
+        const v4i a = A[((pd * 32 + kb) * 4 + rb) * 64 + l];
+#pragma unroll
+        for (int q = 0; pd + q < ND; q++) acc[pd + q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, B[q], acc[pd + q], 0, 0, 0);
+      }
+    }
+    if (ok) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        C x[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          uint32_t y = 0;
+#pragma unroll
+          for (int d = 0; d < ND; d++) y += (uint32_t)acc[d][4 * j + c] << (8 * d);
+          x[c] = (C)y;
+        }
+        st4<T>(out + p * E + 32 * rb + 8 * j + 4 * h, x);
+      }
+    }
+  }
+}
+
 }  // namespace nenya

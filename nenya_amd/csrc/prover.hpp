@@ -162,6 +162,7 @@ template <class T> struct Engine {
   DBuf<L> d_basis, d_pw;
   DBuf<L> d_one;
   DBuf<L> d_frob;  // sigma^-(k-1), transposed [j][o]
+  DBuf<uint32_t> d_b2frag;  // MFMA fragments of the packing-basis products
   KProf* prof = nullptr;
 
   // Launch on the engine's stream, bracketed by events when profiling.
@@ -195,6 +196,10 @@ template <class T> struct Engine {
     HIP_CHECK(hipMemcpyAsync(d_pw.p, K.psi_pw.data(), K.psi_pw.size() * sizeof(L), hipMemcpyHostToDevice, st));
     HIP_CHECK(hipMemcpyAsync(d_one.p, one.v, E * sizeof(L), hipMemcpyHostToDevice, st));
     HIP_CHECK(hipMemcpyAsync(d_frob.p, K.frob_t.data(), E * E * sizeof(L), hipMemcpyHostToDevice, st));
+    if (!K.b2frag.empty()) {
+      d_b2frag = DBuf<uint32_t>(pool, K.b2frag.size());
+      HIP_CHECK(hipMemcpyAsync(d_b2frag.p, K.b2frag.data(), K.b2frag.size() * 4, hipMemcpyHostToDevice, st));
+    }
     HIP_CHECK(hipStreamSynchronize(st));
   }
 

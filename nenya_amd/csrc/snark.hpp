@@ -389,11 +389,24 @@ template <class T> struct GeneralParams : ParamsBase {
                (uint64_t)1, (uint64_t)0);
     // 3. layered matvecs sum_l D_l rho_l (general.rs:211-230) for A, B, C
     DBuf<L> sums(&pool, 3 * nb * E);
-    for (int m = 0; m < 3; m++)
-      e.launch("k_spmv_gr", ncell[m] * 2.0 * Engine<T>::EB, ncell[m] * 16 * Engine<T>::PM, (k_spmv_gr<T>),
-               dim3(grid_for(nb, GPB)), dim3(GB), (const uint64_t*)lptr[m].p, (const uint32_t*)ldk[m].p,
-               (const uint32_t*)lsel[m].p, (const uint32_t*)lidx[m].p, (const L*)dhat.p, (const L*)rho.p, (uint64_t)nb,
-               sums.p + m * nb * E, (uint64_t)nb);
+    if constexpr (kMx<T>) {
+      // sum_{s,s'} C_p[s][s'] b_s b_s' on the matrix cores
+      DBuf<L> cm(&pool, nb * 1024);
+      for (int m = 0; m < 3; m++) {
+        e.launch("k_cmat", (double)nb * 1024 * sizeof(L), 0, (k_cmat<T>), dim3(grid_for(nb, 8, 4096)), dim3(256),
+                 (const uint32_t*)mr[m].ptr.p, (const uint32_t*)mr[m].idx.p, (const L*)mr[m].val.p, w, cm.p,
+                 (uint64_t)nb);
+        e.launch("k_cgemm", (double)nb * (1024 * sizeof(L) + Engine<T>::EB), (double)nb * 1024 * E, (k_cgemm<T>),
+                 dim3(grid_for(nb, 32, 1024)), dim3(256), (const uint32_t*)e.d_b2frag.p, (const L*)cm.p,
+                 sums.p + m * nb * E, (uint64_t)nb);
+      }
+    } else {
+      for (int m = 0; m < 3; m++)
+        e.launch("k_spmv_gr", ncell[m] * 2.0 * Engine<T>::EB, ncell[m] * 16 * Engine<T>::PM, (k_spmv_gr<T>),
+                 dim3(grid_for(nb, GPB)), dim3(GB), (const uint64_t*)lptr[m].p, (const uint32_t*)ldk[m].p,
+                 (const uint32_t*)lsel[m].p, (const uint32_t*)lidx[m].p, (const L*)dhat.p, (const L*)rho.p,
+                 (uint64_t)nb, sums.p + m * nb * E, (uint64_t)nb);
+    }
     // 4. c_hat = pack_r2(psi(c_sum)) (general.rs:410-414); a_hat, b_hat = phi(slots)
     e.psi(sums.p + 2 * nb * E, abc_slots.p + 2 * N, nb);
     DBuf<L> hats(&pool, 3 * nb * E);
