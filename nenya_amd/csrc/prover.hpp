@@ -271,6 +271,21 @@ template <class T> struct Engine {
       fold(mm.r, src, dst, m, ntab, sstride, dstride);
     }
   }
+  // Linear-check round sums {sum f_lo g_lo, sum df dg} as two Gram matrices
+  // on the matrix cores; x rows past `xrows` limbs are zero.
+  DBuf<L> gram_lin(const L* f, uint64_t fs, const L* g, uint64_t gs, uint64_t ntab, uint64_t m, int xrows) {
+    const uint64_t nch = (m * ntab + 31) / 32;
+    const unsigned nwg = (unsigned)std::min<uint64_t>(nch, 128);
+    DBuf<uint32_t> part(pool, (size_t)2 * nwg * 16384);
+    launch("k_gram_lin", m * ntab * 4.0 * EB, m * ntab * 2.0 * E * E, (k_gram_lin<T>), dim3(2 * nwg), dim3(1024), f, fs,
+           g, gs, ntab, m, xrows, part.p);
+    DBuf<uint32_t> gsum(pool, 2 * 16384);
+    launch("k_gram_sum", 2.0 * nwg * 65536, 0, k_gram_sum, dim3(2 * 16384 / 256), dim3(256), (const uint32_t*)part.p,
+           (int)(2 * nwg), 2, gsum.p);
+    DBuf<L> out(pool, 2 * E);
+    launch("k_gram_contract", 2 * 65536.0, 0, (k_gram_contract<T>), dim3(2), dim3(512), (const uint32_t*)gsum.p, out.p);
+    return out;
+  }
   void marg(const L* s, L* eg, uint64_t m) {
     launch("k_marg", m * 3 * EB, m * 32 * PM, (k_marg<T>), dim3(grid_for(m, GPB)), dim3(GB), s, eg, m);
   }
@@ -598,16 +613,21 @@ template <class T> struct Engine {
     size_t len = N;
     for (size_t r = 0; r < nv; r++) {
       size_t m = len / 2;
-      unsigned nbk = grid_for(m * ntab, GPB, 2048);
-      DBuf<L> part(pool, (size_t)nbk * 2 * E);
-      double lb = m * ntab * 4.0 * EB;
-      if (r == 0 && NAF0 == 16)
-        launch("k_lin_round", lb, m * ntab * 2 * 16 * PM, (k_lin_round<T, 16>), dim3(nbk), dim3(GB), f, fs, g, gs,
-               (uint64_t)ntab, (uint64_t)m, part.p);
-      else
-        launch("k_lin_round", lb, m * ntab * 2 * 32 * PM, (k_lin_round<T, 32>), dim3(nbk), dim3(GB), f, fs, g, gs,
-               (uint64_t)ntab, (uint64_t)m, part.p);
-      DBuf<L> s = reduce(part, nbk, 2);
+      DBuf<L> s;
+      if constexpr (kMx<T>) {
+        s = gram_lin(f, fs, g, gs, (uint64_t)ntab, (uint64_t)m, (r == 0 && NAF0 == 16) ? 64 : 128);
+      } else {
+        unsigned nbk = grid_for(m * ntab, GPB, 2048);
+        DBuf<L> part(pool, (size_t)nbk * 2 * E);
+        double lb = m * ntab * 4.0 * EB;
+        if (r == 0 && NAF0 == 16)
+          launch("k_lin_round", lb, m * ntab * 2 * 16 * PM, (k_lin_round<T, 16>), dim3(nbk), dim3(GB), f, fs, g, gs,
+                 (uint64_t)ntab, (uint64_t)m, part.p);
+        else
+          launch("k_lin_round", lb, m * ntab * 2 * 32 * PM, (k_lin_round<T, 32>), dim3(nbk), dim3(GB), f, fs, g, gs,
+                 (uint64_t)ntab, (uint64_t)m, part.p);
+        s = reduce(part, nbk, 2);
+      }
       G sv[2];
       d2h(sv, s.p, sizeof sv);
       G q[3] = {sv[0], c - (sv[0] + sv[0]) - sv[1], sv[1]};
