@@ -454,8 +454,14 @@ __global__ void __launch_bounds__(1024) k_sum_partials(const LT<T>* partials, ui
   __shared__ CT<T> red[32][33];
   int s = blockIdx.x / 4, l0 = (blockIdx.x % 4) * 32;
   int l = threadIdx.x & 31, sl = threadIdx.x >> 5;
-  CT<T> acc = 0;
-  for (uint64_t b = sl; b < nblocks; b += 32) acc += partials[(b * ns + s) * E + l0 + l];
+  // eight independent loads in flight per thread
+  CT<T> a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint64_t b = sl;
+  for (; b + 7 * 32 < nblocks; b += 8 * 32)
+#pragma unroll
+    for (int u = 0; u < 8; u++) a[u] += partials[((b + 32 * u) * ns + s) * E + l0 + l];
+  for (; b < nblocks; b += 32) a[0] += partials[(b * ns + s) * E + l0 + l];
+  CT<T> acc = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   red[sl][l] = acc;
   __syncthreads();
   if (sl == 0) {

@@ -340,20 +340,66 @@ __global__ void __launch_bounds__(256) k_cgemm(const uint32_t* frag, const LT<T>
 // meet at beta^(i+i') alpha^(c+c').
 // ---------------------------------------------------------------------------
 
+// Contract a 128 x 128 Gram matrix held in LDS (G[j][j'], row-major) into
+// the 128 limbs of sum_e x_e y_e: raw[i+i'][c+c'] += G[4i+c][4i'+c'], the
+// alpha reduction (X^4 = -X - 1), then Y^32 = -Y^3 - Y - alpha from the top.
+// All threads of the block call it; the result is left in q (LDS).
+__device__ __forceinline__ void gram_contract_lds(const uint32_t* G, uint32_t (*raw)[7], uint32_t (*q)[4]) {
+  const int tid = threadIdx.x;
+  for (int o = tid; o < 63 * 7; o += blockDim.x) {
+    const int mpos = o / 7, d = o % 7;
+    uint32_t s = 0;
+    const int i0 = mpos > 31 ? mpos - 31 : 0, i1 = mpos < 31 ? mpos : 31;
+    const int c0 = d > 3 ? d - 3 : 0, c1 = d < 3 ? d : 3;
+    for (int i = i0; i <= i1; i++)
+      for (int c = c0; c <= c1; c++) s += G[(4 * i + c) * 128 + 4 * (mpos - i) + (d - c)];
+    raw[mpos][d] = s;
+  }
+  __syncthreads();
+  if (tid < 63) {
+    const uint32_t* p = raw[tid];
+    q[tid][0] = p[0] - p[4];
+    q[tid][1] = p[1] - p[4] - p[5];
+    q[tid][2] = p[2] - p[5] - p[6];
+    q[tid][3] = p[3] - p[6];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int sp = 62; sp >= 32; sp--) {
+      const uint32_t h0 = q[sp][0], h1 = q[sp][1], h2 = q[sp][2], h3 = q[sp][3];
+      const int i = sp - 32;
+      for (int c = 0; c < 4; c++) {
+        const uint32_t hc = q[sp][c];
+        q[i + 3][c] -= hc;
+        q[i + 1][c] -= hc;
+      }
+      q[i][0] += h3;
+      q[i][1] -= h0 - h3;
+      q[i][2] -= h1;
+      q[i][3] -= h2;
+    }
+  }
+  __syncthreads();
+}
+
 // Linear-check round (as k_lin_round): gram 0 over (f_lo, g_lo), gram 1
 // over (f_hi - f_lo, g_hi - g_lo).  Block b (16 waves) handles gram b&1 over
 // chunks of 32 elements: its threads convert the two 128 x 32 limb matrices
 // to digit-plane fragments in LDS (thread (j, eg): limb j of elements
 // 4eg..4eg+3, which are k = kappa(eg&1, 4(eg>>1) + c)), then wave w adds
 // Gram block (w/4, w%4) with one accumulator per digit class.  x rows at
-// limbs >= xrows are zero (low-degree f) and skipped.  Each block writes its
-// 128 x 128 partial (u32) to partial[blockIdx.x].
+// limbs >= xrows are zero (low-degree f) and skipped.  The block contracts
+// its Gram matrix and writes the sum as partials[blockIdx.x] (one element;
+// partials laid out [block/2][gram]).
 template <class T>
 __global__ void __launch_bounds__(1024) k_gram_lin(const LT<T>* f, uint64_t fstride, const LT<T>* gt, uint64_t gstride,
-                                                   uint64_t ntab, uint64_t m, int xrows, uint32_t* partial) {
+                                                   uint64_t ntab, uint64_t m, int xrows, LT<T>* partials) {
   using L = LT<T>;
   constexpr int ND = kND<T>;
-  __shared__ __attribute__((aligned(16))) uint32_t sF[2][ND][4][64][4];  // [x|y][plane][rb][lane][word]
+  // fragments [x|y][plane][rb][lane][word] during the loop, the Gram matrix after
+  __shared__ __attribute__((aligned(16))) uint32_t lds[16384];
+  __shared__ uint32_t raw[63][7], q[63][4];
+  uint32_t (*sF)[ND][4][64][4] = reinterpret_cast<uint32_t (*)[ND][4][64][4]>(lds);
   const int gram = blockIdx.x & 1;
   const uint64_t wgs = gridDim.x >> 1, wg = blockIdx.x >> 1;
   const int tid = threadIdx.x, j = tid & 127, eg = tid >> 7;
@@ -401,76 +447,25 @@ __global__ void __launch_bounds__(1024) k_gram_lin(const LT<T>* f, uint64_t fstr
       for (int pd = 0; pd < ND; pd++) {
         const v4i a = *reinterpret_cast<const v4i*>(sF[0][pd][bx][l]);
 #pragma unroll
-        for (int q = 0; pd + q < ND; q++) {
-          const v4i b = *reinterpret_cast<const v4i*>(sF[1][q][by][l]);
-          acc[pd + q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, acc[pd + q], 0, 0, 0);
+        for (int q2 = 0; pd + q2 < ND; q2++) {
+          const v4i b = *reinterpret_cast<const v4i*>(sF[1][q2][by][l]);
+          acc[pd + q2] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, acc[pd + q2], 0, 0, 0);
         }
       }
     }
     __syncthreads();
   }
-  uint32_t* P = partial + (size_t)blockIdx.x * 16384;
 #pragma unroll
   for (int i = 0; i < 16; i++) {
     uint32_t y = 0;
 #pragma unroll
     for (int d = 0; d < ND; d++) y += (uint32_t)acc[d][i] << (8 * d);
     const int row = 32 * bx + 8 * (i >> 2) + 4 * (l >> 5) + (i & 3), col = 32 * by + (l & 31);
-    P[row * 128 + col] = active ? y : 0u;
-  }
-}
-
-// gsum[g][x] = sum over blocks b with b % ng == g of partial[b][x].
-__global__ void __launch_bounds__(256) k_gram_sum(const uint32_t* partial, int nblocks, int ng, uint32_t* gsum) {
-  const int x = blockIdx.x * 256 + threadIdx.x;  // < ng * 16384
-  const int g = x >> 14, o = x & 16383;
-  uint32_t s = 0;
-  for (int b = g; b < nblocks; b += ng) s += partial[(size_t)b * 16384 + o];
-  gsum[x] = s;
-}
-
-// out[g] = contract(gsum[g]): raw[i+i'][c+c'] += G[4i+c][4i'+c'], then the
-// alpha reduction (X^4 = -X - 1) and Y^32 = -Y^3 - Y - alpha from the top.
-template <class T> __global__ void __launch_bounds__(512) k_gram_contract(const uint32_t* gsum, LT<T>* out) {
-  __shared__ uint32_t raw[63][7];
-  __shared__ uint32_t q[63][4];
-  const uint32_t* G = gsum + (size_t)blockIdx.x * 16384;
-  const int tid = threadIdx.x;
-  if (tid < 63 * 7) {
-    const int mpos = tid / 7, d = tid % 7;
-    uint32_t s = 0;
-    const int i0 = mpos > 31 ? mpos - 31 : 0, i1 = mpos < 31 ? mpos : 31;
-    const int c0 = d > 3 ? d - 3 : 0, c1 = d < 3 ? d : 3;
-    for (int i = i0; i <= i1; i++)
-      for (int c = c0; c <= c1; c++) s += G[(4 * i + c) * 128 + 4 * (mpos - i) + (d - c)];
-    raw[mpos][d] = s;
+    lds[row * 128 + col] = active ? y : 0u;
   }
   __syncthreads();
-  if (tid < 63) {
-    uint32_t* p = raw[tid];
-    q[tid][0] = p[0] - p[4];
-    q[tid][1] = p[1] - p[4] - p[5];
-    q[tid][2] = p[2] - p[5] - p[6];
-    q[tid][3] = p[3] - p[6];
-  }
-  __syncthreads();
-  if (tid == 0) {
-    for (int sp = 62; sp >= 32; sp--) {
-      const uint32_t h0 = q[sp][0], h1 = q[sp][1], h2 = q[sp][2], h3 = q[sp][3];
-      const int i = sp - 32;
-      for (int c = 0; c < 4; c++) {
-        const uint32_t hc = q[sp][c];
-        q[i + 3][c] -= hc;
-        q[i + 1][c] -= hc;
-      }
-      q[i][0] += h3;
-      q[i][1] -= h0 - h3;
-      q[i][2] -= h1;
-      q[i][3] -= h2;
-    }
-  }
-  __syncthreads();
-  if (tid < E) out[(size_t)blockIdx.x * E + tid] = (LT<T>)q[tid >> 2][tid & 3];
+  gram_contract_lds(lds, raw, q);
+  if (tid < E) partials[(size_t)blockIdx.x * E + tid] = (L)q[tid >> 2][tid & 3];
 }
 
 }  // namespace nenya

@@ -276,15 +276,10 @@ template <class T> struct Engine {
   DBuf<L> gram_lin(const L* f, uint64_t fs, const L* g, uint64_t gs, uint64_t ntab, uint64_t m, int xrows) {
     const uint64_t nch = (m * ntab + 31) / 32;
     const unsigned nwg = (unsigned)std::min<uint64_t>(nch, 128);
-    DBuf<uint32_t> part(pool, (size_t)2 * nwg * 16384);
+    DBuf<L> part(pool, (size_t)2 * nwg * E);
     launch("k_gram_lin", m * ntab * 4.0 * EB, m * ntab * 2.0 * E * E, (k_gram_lin<T>), dim3(2 * nwg), dim3(1024), f, fs,
            g, gs, ntab, m, xrows, part.p);
-    DBuf<uint32_t> gsum(pool, 2 * 16384);
-    launch("k_gram_sum", 2.0 * nwg * 65536, 0, k_gram_sum, dim3(2 * 16384 / 256), dim3(256), (const uint32_t*)part.p,
-           (int)(2 * nwg), 2, gsum.p);
-    DBuf<L> out(pool, 2 * E);
-    launch("k_gram_contract", 2 * 65536.0, 0, (k_gram_contract<T>), dim3(2), dim3(512), (const uint32_t*)gsum.p, out.p);
-    return out;
+    return reduce(part, nwg, 2);
   }
   void marg(const L* s, L* eg, uint64_t m) {
     launch("k_marg", m * 3 * EB, m * 32 * PM, (k_marg<T>), dim3(grid_for(m, GPB)), dim3(GB), s, eg, m);
