@@ -471,6 +471,67 @@ __global__ void __launch_bounds__(1024) k_sum_partials(const LT<T>* partials, ui
 }
 
 // ============================================================================
+// Sumcheck scalar updates kept on the device (one 32-lane group)
+// ============================================================================
+
+// out = t r + (1 - t)(1 - r) = 2 t r - t - r + 1   (eq in one variable)
+template <class T> __global__ void __launch_bounds__(64) k_eq_one(const LT<T>* t, const LT<T>* r, LT<T>* out) {
+  __shared__ __attribute__((aligned(16))) LT<T> st[E], sr[E];
+  const int g = threadIdx.x >> 5, l = threadIdx.x & 31;
+  if (g == 0) {
+    stage<T>(st, t, l);
+    stage<T>(sr, r, l);
+  }
+  __syncthreads();
+  if (g == 0) {
+    Acc<T> acc;
+    acc.clear();
+    acc_mul<T>(acc, st, sr, l);
+    CT<T> o[4], a[4], b[4];
+    acc_final<T>(acc, l, o);
+    ld4<T>(st + 4 * l, a);
+    ld4<T>(sr + 4 * l, b);
+    for (int c = 0; c < 4; c++) o[c] = o[c] + o[c] - a[c] - b[c];
+    if (l == 0) o[0] += 1;
+    st4<T>(out + 4 * l, o);
+  }
+}
+
+// Linear-check round message from the sums s = (s0, s_inf) and claim c:
+// q = (s0, c - 2 s0 - s_inf, s_inf)   (linear_check_general.rs:289-334).
+template <class T> __global__ void __launch_bounds__(128) k_lin_q(const LT<T>* s, const LT<T>* c, LT<T>* q) {
+  const int i = threadIdx.x;
+  const LT<T> s0 = s[i], s1 = s[E + i];
+  q[i] = s0;
+  q[E + i] = (LT<T>)(c[i] - s0 - s0 - s1);
+  q[2 * E + i] = s1;
+}
+
+// Next claim c = q0 + r (q1 + r q2).
+template <class T> __global__ void __launch_bounds__(64) k_lin_claim(const LT<T>* q, const LT<T>* r, LT<T>* c) {
+  __shared__ __attribute__((aligned(16))) LT<T> sr[E], sx[E];
+  const int g = threadIdx.x >> 5, l = threadIdx.x & 31;
+  if (g == 0) {
+    stage<T>(sr, r, l);
+    stage<T>(sx, q + 2 * E, l);
+  }
+  for (int step = 1; step >= 0; step--) {
+    __syncthreads();
+    CT<T> o[4], a[4];
+    if (g == 0) {
+      Acc<T> acc;
+      acc.clear();
+      acc_mul<T>(acc, sr, sx, l);
+      acc_final<T>(acc, l, o);
+      ld4<T>(q + step * E + 4 * l, a);
+      for (int k = 0; k < 4; k++) o[k] += a[k];
+    }
+    __syncthreads();
+    if (g == 0) st4<T>(step ? sx + 4 * l : c + 4 * l, o);
+  }
+}
+
+// ============================================================================
 // Teichmuller lift of Fiat-Shamir challenges
 // ============================================================================
 

@@ -204,7 +204,28 @@ template <class T> struct Engine {
   }
 
   void check() { HIP_CHECK(hipGetLastError()); }
-  void sync() { HIP_CHECK(hipStreamSynchronize(st)); }
+  void sync() {
+    HIP_CHECK(hipStreamSynchronize(st));
+    up_off = 0;
+  }
+
+  // Host-to-device copies through pinned staging: a slot stays untouched
+  // until the next stream synchronization, which recycles the ring.
+  static constexpr size_t UPCAP = 1 << 16;
+  Pinned upbuf;
+  size_t up_off = 0;
+  void put_async(L* dst, const void* src, size_t bytes) {
+    if (bytes > UPCAP) {
+      h2d(dst, src, bytes);
+      sync();
+      return;
+    }
+    if (up_off + bytes > UPCAP) sync();
+    uint8_t* slot = upbuf.get(UPCAP) + up_off;
+    std::memcpy(slot, src, bytes);
+    up_off += (bytes + 255) & ~size_t(255);
+    HIP_CHECK(hipMemcpyAsync(dst, slot, bytes, hipMemcpyHostToDevice, st));
+  }
 
   void h2d(void* d, const void* h, size_t bytes) { HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st)); }
   void d2h(void* h, const void* d, size_t bytes) {
@@ -359,6 +380,28 @@ template <class T> struct Engine {
     }
     if (dev_out) *dev_out = std::move(d);
     return nat;
+  }
+  // Samples one challenge and lifts it in device memory at d without waiting
+  // for it; mm (if given) receives its multiplication fragments.
+  void challenge_into(Transcript& tr, const char* label, L* d, MulMat* mm) {
+    StdRng rng = tr.sampler(label);
+    G nat;
+    for (int l = 0; l < E; l++) nat.v[l] = (L)rng.limb(T::K);
+    put_async(d, nat.v, sizeof nat.v);
+    const double lm = E * E + (T::K - 1) * 16.5 * PM;
+    bool fused = false;
+    if constexpr (kMx<T>) {
+      if (mm) {
+        mm->frag = DBuf<uint32_t>(pool, (size_t)kND<T> * 4096);
+        launch("k_lift_mulmat", 2 * EB, lm, (k_lift_mulmat<T>), dim3(1), dim3(256), d, d_frob.p, T::K - 1, mm->frag.p);
+        fused = true;
+      }
+    }
+    if (!fused) launch("k_lift", 2 * EB, lm, (k_lift<T>), dim3(1), dim3(256), d, d_frob.p, T::K - 1);
+    if (mm) {
+      mm->r = d;
+      if (!fused) *mm = mulmat(d);
+    }
   }
   G challenge(Transcript& tr, const char* label, DBuf<L>* dev = nullptr, MulMat* mm = nullptr) {
     StdRng rng = tr.sampler(label);
@@ -554,6 +597,8 @@ template <class T> struct Engine {
     G one = G::one();
     DBuf<L> eg = nv > 1 ? eq_table(rt.data() + 1, nv - 1) : put(&one, 1);
     DBuf<L> work(pool, (size_t)ntab * (N / 2 > 0 ? N / 2 : 1) * E);
+    // challenges and the point rt stay on the device; one sync per round
+    DBuf<L> drt = put(rt.data(), nv), dch(pool, nv * E), dsc(pool, E);
     size_t len = N;
     const L* cur[4];
     for (int t = 0; t < ntab; t++) cur[t] = tabs[t];
@@ -570,10 +615,9 @@ template <class T> struct Engine {
       G q[3] = {sv[0], sv[1] - sv[0] - sv[2], sv[2]};
       tr.elems("sc/round", q, 3);
       for (int i = 0; i < 3; i++) o.rounds.push_back(q[i]);
-      DBuf<L> dr;
+      L* rc = dch.p + r * E;
       MulMat mr;
-      G rc = challenge(tr, "sc/challenge", &dr, &mr);
-      o.chal.push_back(rc);
+      challenge_into(tr, "sc/challenge", rc, &mr);
       for (int t = 0; t < ntab; t++) {
         L* dst = work.p + (size_t)t * (N / 2) * E;
         fold(mr, cur[t], dst, m, 1, 0, 0);
@@ -581,11 +625,13 @@ template <class T> struct Engine {
       }
       len = m;
       if (r + 1 < nv) {
-        G sc = eq_one(rt[r], rc);
-        DBuf<L> ds = put(&sc, 1);
-        marg(ds.p, eg.p, m / 2);
+        launch("k_eq_one", 3 * EB, 32 * PM, (k_eq_one<T>), dim3(1), dim3(64), (const L*)(drt.p + r * E), (const L*)rc,
+               dsc.p);
+        marg(dsc.p, eg.p, m / 2);
       }
     }
+    o.chal.resize(nv);
+    d2h(o.chal.data(), dch.p, nv * sizeof(G));
     o.finals = pick(work.p, N / 2, ntab);
     return o;
   }
@@ -604,7 +650,8 @@ template <class T> struct Engine {
     const L* f = f0;
     const L* g = g0;
     uint64_t fs = fstride0, gs = gstride0;
-    G c = claim;
+    // the running claim and the challenges stay on the device; one sync per round
+    DBuf<L> dc = put(&claim, 1), dq(pool, 3 * E), dch(pool, nv * E);
     size_t len = N;
     for (size_t r = 0; r < nv; r++) {
       size_t m = len / 2;
@@ -623,16 +670,15 @@ template <class T> struct Engine {
                  (uint64_t)ntab, (uint64_t)m, part.p);
         s = reduce(part, nbk, 2);
       }
-      G sv[2];
-      d2h(sv, s.p, sizeof sv);
-      G q[3] = {sv[0], c - (sv[0] + sv[0]) - sv[1], sv[1]};
+      launch("k_lin_q", 5 * EB, 0, (k_lin_q<T>), dim3(1), dim3(128), (const L*)s.p, (const L*)dc.p, dq.p);
+      G q[3];
+      d2h(q, dq.p, sizeof q);
       tr.elems("sc/round", q, 3);
       for (int i = 0; i < 3; i++) o.rounds.push_back(q[i]);
-      DBuf<L> dr;
+      L* rc = dch.p + r * E;
       MulMat mr;
-      G rc = challenge(tr, "sc/challenge", &dr, &mr);
-      o.chal.push_back(rc);
-      c = q[0] + rc * (q[1] + rc * q[2]);
+      challenge_into(tr, "sc/challenge", rc, &mr);
+      launch("k_lin_claim", 5 * EB, 64 * PM, (k_lin_claim<T>), dim3(1), dim3(64), (const L*)dq.p, (const L*)rc, dc.p);
       fold(mr, f, fw.p, m, ntab, fs, H);
       fold(mr, g, gw.p, m, ntab, gs, H);
       f = fw.p;
@@ -640,6 +686,8 @@ template <class T> struct Engine {
       fs = gs = H;
       len = m;
     }
+    o.chal.resize(nv);
+    d2h(o.chal.data(), dch.p, nv * sizeof(G));
     o.finals = pick(f, fs, ntab);
     std::vector<G> gf = pick(g, gs, ntab);
     o.finals.insert(o.finals.end(), gf.begin(), gf.end());
