@@ -150,21 +150,54 @@ template <class T> __device__ __forceinline__ void ld16k(const LT<T>* el, int kb
   }
 }
 
+// Up to two groups of tables folded by one launch (same challenge, same m):
+// group g has ntab[g] tables, table t at src[g] + t*sstride[g] elements,
+// output at dst[g] + t*dstride[g].
+template <class T> struct FoldJob {
+  const LT<T>* src[2];
+  LT<T>* dst[2];
+  uint64_t ntab[2], sstride[2], dstride[2];
+  uint64_t m;
+};
+
+// Element e of the job: lo pointer (hi = lo + m) and output pointer.
+template <class T>
+__device__ __forceinline__ bool fold_locate(const FoldJob<T>& J, uint64_t e, const LT<T>*& lo, LT<T>*& out) {
+  const uint64_t n0 = J.m * J.ntab[0];
+  const int g = e < n0 ? 0 : 1;
+  const uint64_t x = g ? e - n0 : e;
+  if (x >= J.m * J.ntab[g]) {
+    lo = J.src[0];
+    out = nullptr;
+    return false;
+  }
+  const uint64_t tab = x / J.m, i = x % J.m;
+  lo = J.src[g] + (tab * J.sstride[g] + i) * E;
+  out = J.dst[g] + (tab * J.dstride[g] + i) * E;
+  return true;
+}
+
 // One 32-element fold tile: loads lo/hi, forms the digit planes of hi - lo,
 // then row-blocks rb0 .. rb0+NRB-1 of lo + M_r (hi - lo).  A points at the
-// fragments (LDS or global).
-template <class T, int NRB>
-__device__ __forceinline__ void fold_tile(const v4i* A, const LT<T>* src, LT<T>* dst, uint64_t m, uint64_t total,
-                                          uint64_t src_stride, uint64_t dst_stride, uint64_t tile, int rb0) {
+// fragments (LDS or global); with PRE the wave first loads every fragment it
+// needs into registers (global source: one round of latency, not one per MFMA).
+template <class T, int NRB, bool PRE>
+__device__ __forceinline__ void fold_tile(const v4i* A, const FoldJob<T>& J, uint64_t tile, int rb0) {
   using C = CT<T>;
   using L = LT<T>;
   constexpr int ND = kND<T>;
   const int l = threadIdx.x & 63, col = l & 31, h = l >> 5;
-  const uint64_t e = tile * 32 + col;
-  const bool ok = e < total;
-  const uint64_t tab = ok ? e / m : 0, i = ok ? e % m : 0;
-  const L* lo = src + (tab * src_stride + i) * E;
-  const L* hi = lo + m * E;
+  const L* lo;
+  L* out;
+  const bool ok = fold_locate<T>(J, tile * 32 + col, lo, out);
+  const L* hi = lo + J.m * E;
+  v4i Apre[PRE ? ND : 1][PRE ? 4 : 1];
+  if constexpr (PRE) {
+#pragma unroll
+    for (int p = 0; p < ND; p++)
+#pragma unroll
+      for (int kb = 0; kb < 4; kb++) Apre[p][kb] = A[frag_at(p, kb, rb0) * 64 + l];
+  }
   uint32_t a[4][16], d[4][16];
   if (ok) {
 #pragma unroll
@@ -202,12 +235,12 @@ __device__ __forceinline__ void fold_tile(const v4i* A, const LT<T>* src, LT<T>*
 #pragma unroll
         for (int kb = 0; kb < 4; kb++)
 #pragma unroll
-          for (int u = 0; u < NU; u++)
-            acc[u] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[frag_at(p, kb, rb0 + rp + u) * 64 + l], B[kb][s - p],
-                                                           acc[u], 0, 0, 0);
+          for (int u = 0; u < NU; u++) {
+            const v4i af = PRE ? Apre[PRE ? p : 0][PRE ? kb : 0] : A[frag_at(p, kb, rb0 + rp + u) * 64 + l];
+            acc[u] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af, B[kb][s - p], acc[u], 0, 0, 0);
+          }
     }
     if (ok) {
-      L* out = dst + (tab * dst_stride + i) * E;
 #pragma unroll
       for (int u = 0; u < NU; u++) {
         const int rb = rb0 + rp + u;
@@ -228,16 +261,15 @@ __device__ __forceinline__ void fold_tile(const v4i* A, const LT<T>* src, LT<T>*
 // Bulk mode: persistent blocks of 4 waves, a wave folds 32 elements per
 // tile, the M_r fragments (ND*16 KB) sit in LDS for the block's lifetime.
 // Split mode (few tiles): block b takes tile b and wave w its row-block w,
-// with the fragments read from global memory (L2-resident).
+// with its fragments loaded from global memory (L2-resident) up front.
 template <class T>
-__global__ void __launch_bounds__(256) k_fold_mx(const uint32_t* frag, const LT<T>* src, LT<T>* dst, uint64_t m,
-                                                 uint64_t ntab, uint64_t src_stride, uint64_t dst_stride, int split) {
+__global__ void __launch_bounds__(256) k_fold_mx(const uint32_t* frag, FoldJob<T> J, int split) {
   constexpr int ND = kND<T>;
   __shared__ v4i sA[ND * 16 * 64];
   const v4i* gA = reinterpret_cast<const v4i*>(frag);
-  const uint64_t total = m * ntab, ntiles = (total + 31) / 32;
+  const uint64_t total = J.m * (J.ntab[0] + J.ntab[1]), ntiles = (total + 31) / 32;
   if (split) {
-    fold_tile<T, 1>(gA, src, dst, m, total, src_stride, dst_stride, blockIdx.x, threadIdx.x >> 6);
+    fold_tile<T, 1, true>(gA, J, blockIdx.x, threadIdx.x >> 6);
     return;
   }
   for (int x = threadIdx.x; x < ND * 16 * 64; x += 256) sA[x] = gA[x];
@@ -246,7 +278,7 @@ __global__ void __launch_bounds__(256) k_fold_mx(const uint32_t* frag, const LT<
     // keeps the A-fragment reads inside the loop (hoisted, they would pin
     // ND*64 registers)
     asm volatile("" ::: "memory");
-    fold_tile<T, 4>(sA, src, dst, m, total, src_stride, dst_stride, tile, 0);
+    fold_tile<T, 4, false>(sA, J, tile, 0);
   }
 }
 

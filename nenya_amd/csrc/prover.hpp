@@ -205,7 +205,12 @@ template <class T> struct Engine {
 
   void check() { HIP_CHECK(hipGetLastError()); }
   void sync() {
-    HIP_CHECK(hipStreamSynchronize(st));
+    // spin on the stream instead of sleeping on its completion signal: the
+    // sumcheck rounds wait for a few hundred bytes about 30 times per proof
+    hipError_t e;
+    while ((e = hipStreamQuery(st)) == hipErrorNotReady) {
+    }
+    HIP_CHECK(e);
     up_off = 0;
   }
 
@@ -282,14 +287,20 @@ template <class T> struct Engine {
     return mm;
   }
   void fold(const MulMat& mm, const L* src, L* dst, uint64_t m, uint64_t ntab, uint64_t sstride, uint64_t dstride) {
+    fold2(mm, m, src, dst, ntab, sstride, dstride, nullptr, nullptr, 0, 0, 0);
+  }
+  // Two groups of tables folded by the same challenge in one launch.
+  void fold2(const MulMat& mm, uint64_t m, const L* s0, L* d0, uint64_t n0, uint64_t ss0, uint64_t ds0, const L* s1,
+             L* d1, uint64_t n1, uint64_t ss1, uint64_t ds1) {
     if constexpr (kMx<T>) {
-      uint64_t ntiles = (m * ntab + 31) / 32;
+      FoldJob<T> J{{s0, s1 ? s1 : s0}, {d0, d1 ? d1 : d0}, {n0, s1 ? n1 : 0}, {ss0, ss1}, {ds0, ds1}, m};
+      const uint64_t ntiles = (m * (J.ntab[0] + J.ntab[1]) + 31) / 32;
       const int split = ntiles <= 256;  // few tiles: 4 waves per tile, fragments from L2
-      launch("k_fold_mx", m * ntab * 3 * EB, m * ntab * 32 * PM, (k_fold_mx<T>),
-             dim3(split ? (unsigned)ntiles : grid_for(ntiles, 4, 512)), dim3(256), (const uint32_t*)mm.frag.p, src, dst,
-             m, ntab, sstride, dstride, split);
+      launch("k_fold_mx", m * (J.ntab[0] + J.ntab[1]) * 3 * EB, m * (J.ntab[0] + J.ntab[1]) * 32 * PM, (k_fold_mx<T>),
+             dim3(split ? (unsigned)ntiles : grid_for(ntiles, 4, 512)), dim3(256), (const uint32_t*)mm.frag.p, J, split);
     } else {
-      fold(mm.r, src, dst, m, ntab, sstride, dstride);
+      fold(mm.r, s0, d0, m, n0, ss0, ds0);
+      if (s1) fold(mm.r, s1, d1, m, n1, ss1, ds1);
     }
   }
   // Linear-check round sums {sum f_lo g_lo, sum df dg} as two Gram matrices
@@ -618,11 +629,16 @@ template <class T> struct Engine {
       L* rc = dch.p + r * E;
       MulMat mr;
       challenge_into(tr, "sc/challenge", rc, &mr);
-      for (int t = 0; t < ntab; t++) {
-        L* dst = work.p + (size_t)t * (N / 2) * E;
-        fold(mr, cur[t], dst, m, 1, 0, 0);
-        cur[t] = dst;
+      // tables at a common stride fold in one launch
+      bool uniform = true;
+      uniform = ntab > 1 && cur[1] > cur[0] && (cur[1] - cur[0]) % E == 0;
+      for (int t = 2; t < ntab; t++) uniform = uniform && cur[t] == cur[0] + (cur[1] - cur[0]) * t;
+      if (uniform && ntab > 1) {
+        fold(mr, cur[0], work.p, m, ntab, (uint64_t)(cur[1] - cur[0]) / E, N / 2);
+      } else {
+        for (int t = 0; t < ntab; t++) fold(mr, cur[t], work.p + (size_t)t * (N / 2) * E, m, 1, 0, 0);
       }
+      for (int t = 0; t < ntab; t++) cur[t] = work.p + (size_t)t * (N / 2) * E;
       len = m;
       if (r + 1 < nv) {
         launch("k_eq_one", 3 * EB, 32 * PM, (k_eq_one<T>), dim3(1), dim3(64), (const L*)(drt.p + r * E), (const L*)rc,
@@ -679,8 +695,7 @@ template <class T> struct Engine {
       MulMat mr;
       challenge_into(tr, "sc/challenge", rc, &mr);
       launch("k_lin_claim", 5 * EB, 64 * PM, (k_lin_claim<T>), dim3(1), dim3(64), (const L*)dq.p, (const L*)rc, dc.p);
-      fold(mr, f, fw.p, m, ntab, fs, H);
-      fold(mr, g, gw.p, m, ntab, gs, H);
+      fold2(mr, m, f, fw.p, ntab, fs, H, g, gw.p, ntab, gs, H);
       f = fw.p;
       g = gw.p;
       fs = gs = H;

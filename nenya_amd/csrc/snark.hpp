@@ -409,18 +409,21 @@ template <class T> struct GeneralParams : ParamsBase {
     }
     // 4. c_hat = pack_r2(psi(c_sum)) (general.rs:410-414); a_hat, b_hat = phi(slots)
     e.psi(sums.p + 2 * nb * E, abc_slots.p + 2 * N, nb);
-    DBuf<L> hats(&pool, 3 * nb * E);
+    // hats holds a_hat, b_hat, c_hat and then e_mult: the mult check folds
+    // the four tables at one stride
+    DBuf<L> hats(&pool, 4 * nb * E);
     e.pack(abc_slots.p, hats.p, 3 * nb, 1);
     // 5. e_A, e_B, e_C = sums - hats (stacked error), e_mult = a_hat b_hat - c_hat
-    DBuf<L> estack(&pool, 3 * nb * E), emult(&pool, nb * E);
+    DBuf<L> estack(&pool, 3 * nb * E);
+    L* emult = hats.p + 3 * nb * E;
     e.sub(sums.p, hats.p, estack.p, 3 * nb * E);
-    e.template mul_sub<16>(hats.p, hats.p + nb * E, hats.p + 2 * nb * E, emult.p, nb);
+    e.template mul_sub<16>(hats.p, hats.p + nb * E, hats.p + 2 * nb * E, emult, nb);
     sums.release();
     stg.mark("reduce_matvec");
     // 6. commits (general.rs:434-451)
     CommitDev<T> cw = e.commit(pw, w, pw.P.k1, rho.p);
     CommitDev<T> cabc = e.commit(pabc, abc_slots.p, 3 * pabc.P.k1, hats.p);
-    CommitDev<T> cem = e.commit(pe, emult.p, pe.P.k1, emult.p);
+    CommitDev<T> cem = e.commit(pe, emult, pe.P.k1, emult);
     CommitDev<T> ces = e.commit(pe, estack.p, 3 * pe.P.k1, estack.p);
     stg.mark("commit");
     Transcript tr("general");
@@ -432,7 +435,7 @@ template <class T> struct GeneralParams : ParamsBase {
     std::vector<G> rt = e.sample(rs, log_b);
     // 7. mult check, sub form, on (a_hat, b_hat, c_hat, e_mult)
     tr.label("mult_check_sumcheck");
-    const L* mt[4] = {hats.p, hats.p + nb * E, hats.p + 2 * nb * E, emult.p};
+    const L* mt[4] = {hats.p, hats.p + nb * E, hats.p + 2 * nb * E, emult};
     auto mp = e.mult_check(mt, 4, log_b, rt, tr);
     tr.elems("mult/finals", mp.finals.data(), 4);
     G gam = e.challenge(tr, "linear_check_gamma");
