@@ -499,12 +499,17 @@ template <class T> __global__ void __launch_bounds__(64) k_eq_one(const LT<T>* t
 
 // Linear-check round message from the sums s = (s0, s_inf) and claim c:
 // q = (s0, c - 2 s0 - s_inf, s_inf)   (linear_check_general.rs:289-334).
-template <class T> __global__ void __launch_bounds__(128) k_lin_q(const LT<T>* s, const LT<T>* c, LT<T>* q) {
+// q is written to device memory and to qh (host-pinned) for the transcript.
+template <class T>
+__global__ void __launch_bounds__(128) k_lin_q(const LT<T>* s, const LT<T>* c, LT<T>* q, LT<T>* qh) {
   const int i = threadIdx.x;
-  const LT<T> s0 = s[i], s1 = s[E + i];
+  const LT<T> s0 = s[i], s1 = s[E + i], q1 = (LT<T>)(c[i] - s0 - s0 - s1);
   q[i] = s0;
-  q[E + i] = (LT<T>)(c[i] - s0 - s0 - s1);
+  q[E + i] = q1;
   q[2 * E + i] = s1;
+  qh[i] = s0;
+  qh[E + i] = q1;
+  qh[2 * E + i] = s1;
 }
 
 // Next claim c = q0 + r (q1 + r q2).
@@ -577,15 +582,16 @@ constexpr int DPP_HALF_MIRROR = 0x141; // lane i <-> 7-i within 8 lanes
 // Off-diagonal pairs count twice.  The 8 lanes of a t sum their raw circ/hi
 // terms with a DPP transpose-reduction; the reduction Y^32 = -Y^3 - Y - alpha
 // is applied by 128 threads reading the sums from LDS.
-// Lifts el in place with a 256-thread block; the lifted value is left in sx
-// (LDS, C-typed, 16-byte aligned) as well.
+// Lifts src (device or host-pinned memory) into el with a 256-thread block;
+// the lifted value is left in sx (LDS, C-typed, 16-byte aligned) as well.
 template <class T>
-__device__ __forceinline__ void teich_lift_block(LT<T>* el, const LT<T>* frob_t, int squarings, CT<T>* sx) {
+__device__ __forceinline__ void teich_lift_block(const LT<T>* src, LT<T>* el, const LT<T>* frob_t, int squarings,
+                                                 CT<T>* sx) {
   using C = CT<T>;
   __shared__ C part[E];
   __shared__ C sred[2][32][4];  // [0] reduced circ, [1] reduced hi (before the cascade)
   const int tid = threadIdx.x;
-  if (tid < E) sx[tid] = (C)el[tid];
+  if (tid < E) sx[tid] = (C)src[tid];
   __syncthreads();
   {
     const int o = tid & (E - 1), h = tid >> 7;
@@ -666,9 +672,15 @@ __device__ __forceinline__ void teich_lift_block(LT<T>* el, const LT<T>* frob_t,
 }
 
 template <class T>
+__global__ void __launch_bounds__(256) k_lift_from(const LT<T>* src, LT<T>* x, const LT<T>* frob_t, int squarings) {
+  __shared__ __attribute__((aligned(16))) CT<T> sx[E];
+  teich_lift_block<T>(src, x, frob_t, squarings, sx);
+}
+
+template <class T>
 __global__ void __launch_bounds__(256) k_lift(LT<T>* x, const LT<T>* frob_t, int squarings) {
   __shared__ __attribute__((aligned(16))) CT<T> sx[E];
-  teich_lift_block<T>(x + (uint64_t)blockIdx.x * E, frob_t, squarings, sx);
+  teich_lift_block<T>(x + (uint64_t)blockIdx.x * E, x + (uint64_t)blockIdx.x * E, frob_t, squarings, sx);
 }
 
 // ============================================================================

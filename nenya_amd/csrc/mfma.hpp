@@ -128,12 +128,14 @@ __global__ void __launch_bounds__(256) k_mulmat(const LT<T>* r, uint32_t* frag) 
   mulmat_frag<T>(v, frag);
 }
 
-// Lift one challenge (as k_lift) and emit the M_r fragments of the result.
+// Lift one challenge (natural sample src, may be host-pinned) into x and
+// emit the M_r fragments of the result.
 template <class T>
-__global__ void __launch_bounds__(256) k_lift_mulmat(LT<T>* x, const LT<T>* frob_t, int squarings, uint32_t* frag) {
+__global__ void __launch_bounds__(256) k_lift_mulmat(const LT<T>* src, LT<T>* x, const LT<T>* frob_t, int squarings,
+                                                     uint32_t* frag) {
   using C = CT<T>;
   __shared__ __attribute__((aligned(16))) C v[32][E];
-  teich_lift_block<T>(x, frob_t, squarings, v[0]);
+  teich_lift_block<T>(src, x, frob_t, squarings, v[0]);
   __syncthreads();
   mulmat_chain<C>(v);
   mulmat_frag<T>(v, frag);

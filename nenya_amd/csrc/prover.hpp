@@ -219,6 +219,27 @@ template <class T> struct Engine {
   static constexpr size_t UPCAP = 1 << 16;
   Pinned upbuf;
   size_t up_off = 0;
+  // Copies src into the pinned ring and returns its device-side address, for
+  // kernels that read it directly (valid until the next synchronization).
+  const void* stage_host(const void* src, size_t bytes) {
+    if (bytes > UPCAP) throw Error(NENYA_E_DEVICE, "staging request too large");
+    if (up_off + bytes > UPCAP) sync();
+    uint8_t* slot = upbuf.get(UPCAP) + up_off;
+    std::memcpy(slot, src, bytes);
+    up_off += (bytes + 255) & ~size_t(255);
+    void* dptr = nullptr;
+    HIP_CHECK(hipHostGetDevicePointer(&dptr, slot, 0));
+    return dptr;
+  }
+  // Pinned area (up to 64 elements) that small result kernels write directly;
+  // the host reads it after sync().
+  Pinned resbuf;
+  L* res_host() { return (L*)resbuf.get(64 * E * sizeof(L)); }
+  L* res_dev() {
+    void* dptr = nullptr;
+    HIP_CHECK(hipHostGetDevicePointer(&dptr, res_host(), 0));
+    return (L*)dptr;
+  }
   void put_async(L* dst, const void* src, size_t bytes) {
     if (bytes > UPCAP) {
       h2d(dst, src, bytes);
@@ -317,6 +338,10 @@ template <class T> struct Engine {
     launch("k_marg", m * 3 * EB, m * 32 * PM, (k_marg<T>), dim3(grid_for(m, GPB)), dim3(GB), s, eg, m);
   }
   // sum of partial sums -> ns elements on device
+  void reduce_into(const DBuf<L>& partials, unsigned nblocks, int ns, L* out) {
+    launch("k_sum_partials", (double)nblocks * ns * EB, 0, (k_sum_partials<T>), dim3(ns * 4), dim3(1024), partials.p,
+           (uint64_t)nblocks, ns, out);
+  }
   DBuf<L> reduce(const DBuf<L>& partials, unsigned nblocks, int ns) {
     DBuf<L> out(pool, ns * E);
     launch("k_sum_partials", (double)nblocks * ns * EB, 0, (k_sum_partials<T>), dim3(ns * 4), dim3(1024), partials.p,
@@ -377,8 +402,8 @@ template <class T> struct Engine {
     if constexpr (kMx<T>) {
       if (mm && cnt == 1) {
         mm->frag = DBuf<uint32_t>(pool, (size_t)kND<T> * 4096);
-        launch("k_lift_mulmat", 2 * EB, lm, (k_lift_mulmat<T>), dim3(1), dim3(256), d.p, d_frob.p, T::K - 1,
-               mm->frag.p);
+        launch("k_lift_mulmat", 2 * EB, lm, (k_lift_mulmat<T>), dim3(1), dim3(256), (const L*)d.p, d.p, d_frob.p,
+               T::K - 1, mm->frag.p);
         fused = true;
       }
     }
@@ -398,17 +423,19 @@ template <class T> struct Engine {
     StdRng rng = tr.sampler(label);
     G nat;
     for (int l = 0; l < E; l++) nat.v[l] = (L)rng.limb(T::K);
-    put_async(d, nat.v, sizeof nat.v);
+    // the lift kernel reads the natural sample from pinned memory (no copy)
+    const L* src = (const L*)stage_host(nat.v, sizeof nat.v);
     const double lm = E * E + (T::K - 1) * 16.5 * PM;
     bool fused = false;
     if constexpr (kMx<T>) {
       if (mm) {
         mm->frag = DBuf<uint32_t>(pool, (size_t)kND<T> * 4096);
-        launch("k_lift_mulmat", 2 * EB, lm, (k_lift_mulmat<T>), dim3(1), dim3(256), d, d_frob.p, T::K - 1, mm->frag.p);
+        launch("k_lift_mulmat", 2 * EB, lm, (k_lift_mulmat<T>), dim3(1), dim3(256), src, d, d_frob.p, T::K - 1,
+               mm->frag.p);
         fused = true;
       }
     }
-    if (!fused) launch("k_lift", 2 * EB, lm, (k_lift<T>), dim3(1), dim3(256), d, d_frob.p, T::K - 1);
+    if (!fused) launch("k_lift", 2 * EB, lm, (k_lift_from<T>), dim3(1), dim3(256), src, d, d_frob.p, T::K - 1);
     if (mm) {
       mm->r = d;
       if (!fused) *mm = mulmat(d);
@@ -620,9 +647,10 @@ template <class T> struct Engine {
       launch("k_mult_round", m * (2.0 * ntab + 1) * EB, m * 6 * 32 * PM, (k_mult_round<T>), dim3(nbk), dim3(GB),
              (const L*)eg.p, cur[0], cur[1], ntab > 2 ? cur[2] : (const L*)nullptr, ntab > 2 ? cur[3] : (const L*)nullptr,
              (uint64_t)m, part.p);
-      DBuf<L> s = reduce(part, nbk, 3);
+      reduce_into(part, nbk, 3, res_dev());
+      sync();
       G sv[3];
-      d2h(sv, s.p, sizeof sv);
+      std::memcpy(sv, res_host(), sizeof sv);
       G q[3] = {sv[0], sv[1] - sv[0] - sv[2], sv[2]};
       tr.elems("sc/round", q, 3);
       for (int i = 0; i < 3; i++) o.rounds.push_back(q[i]);
@@ -686,9 +714,10 @@ template <class T> struct Engine {
                  (uint64_t)ntab, (uint64_t)m, part.p);
         s = reduce(part, nbk, 2);
       }
-      launch("k_lin_q", 5 * EB, 0, (k_lin_q<T>), dim3(1), dim3(128), (const L*)s.p, (const L*)dc.p, dq.p);
+      launch("k_lin_q", 5 * EB, 0, (k_lin_q<T>), dim3(1), dim3(128), (const L*)s.p, (const L*)dc.p, dq.p, res_dev());
+      sync();
       G q[3];
-      d2h(q, dq.p, sizeof q);
+      std::memcpy(q, res_host(), sizeof q);
       tr.elems("sc/round", q, 3);
       for (int i = 0; i < 3; i++) o.rounds.push_back(q[i]);
       L* rc = dch.p + r * E;
