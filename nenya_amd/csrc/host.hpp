@@ -141,6 +141,32 @@ template <class F> std::vector<uint32_t> mx_frags(int nd, int kblocks, F at) {
   return fr;
 }
 
+// Matrix of x -> r x over Z/2^k: M[row][k] = (r e_k)[row], e_{4i+c} = X^c Y^i,
+// built from Y^i r (Y^32 = -Y^3 - Y - alpha) and X (X^4 = -X - 1).
+template <class T> std::vector<typename T::C> mulmat_host(const Gr<T>& r) {
+  using C = typename T::C;
+  std::vector<C> M((size_t)E * E);
+  C v[E], nv[E];
+  for (int l = 0; l < E; l++) v[l] = r.v[l];
+  for (int i = 0; i < 32; i++) {
+    for (int t = 0; t < 32; t++) {
+      C a[4] = {v[4 * t], v[4 * t + 1], v[4 * t + 2], v[4 * t + 3]};
+      for (int c = 0; c < 4; c++) {
+        for (int cp = 0; cp < 4; cp++) M[(size_t)(4 * t + cp) * E + 4 * i + c] = a[cp];
+        C n0 = 0 - a[3], n1 = a[0] - a[3], n2 = a[1], n3 = a[2];
+        a[0] = n0; a[1] = n1; a[2] = n2; a[3] = n3;
+      }
+    }
+    // v <- Y v
+    const C h0 = v[124], h1 = v[125], h2 = v[126], h3 = v[127];
+    for (int l = E - 1; l >= 4; l--) nv[l] = v[l - 4];
+    nv[0] = h3; nv[1] = h3 - h0; nv[2] = 0 - h1; nv[3] = 0 - h2;
+    for (int c = 0; c < 4; c++) { nv[4 + c] -= v[124 + c]; nv[12 + c] -= v[124 + c]; }
+    for (int l = 0; l < E; l++) v[l] = nv[l];
+  }
+  return M;
+}
+
 template <class T> struct Consts {
   using L = typename T::L;
   using C = typename T::C;
@@ -150,6 +176,7 @@ template <class T> struct Consts {
   std::vector<L> psi_pw;  // [16][32][4]: t_i^k
   std::vector<L> frob_t;  // [j][o]: matrix of sigma^-(k-1), transposed
   std::vector<uint32_t> b2frag;  // MFMA fragments of [b_s b_s'] (128 x 1024), k <= 32
+  std::vector<uint32_t> lbfrag;  // MFMA fragments of x -> l_beta_i x for i < 16, k <= 32
   L inv15;
 
   Consts() {
@@ -223,6 +250,27 @@ template <class T> struct Consts {
       for (int a = 0; a < 32; a++)
         for (int b = a; b < 32; b++) b2[32 * a + b] = b2[32 * b + a] = basis[a] * basis[b];
       b2frag = mx_frags(K / 8, 32, [&](int row, int k) { return (uint32_t)b2[k].v[row]; });
+      {  // self-check of mulmat_host against the product on a fixed element
+        Gr<T> x;
+        uint64_t sd = 0xD1B54A32D192ED03ull;
+        for (int l = 0; l < E; l++) {
+          sd = sd * 6364136223846793005ull + 1442695040888963407ull;
+          x.v[l] = (L)(sd >> 23);
+        }
+        std::vector<C> M = mulmat_host<T>(lbeta[3]);
+        Gr<T> want = lbeta[3] * x, got;
+        for (int o = 0; o < E; o++) {
+          C acc = 0;
+          for (int k2 = 0; k2 < E; k2++) acc += M[(size_t)o * E + k2] * (C)x.v[k2];
+          got.v[o] = (L)acc;
+        }
+        if (!(got == want)) throw Error(NENYA_E_DEVICE, "multiplication matrix failed its self-check");
+      }
+      for (int i = 0; i < 16; i++) {
+        std::vector<C> M = mulmat_host<T>(lbeta[i]);
+        std::vector<uint32_t> f = mx_frags(K / 8, 4, [&](int row, int k) { return (uint32_t)M[(size_t)row * E + k]; });
+        lbfrag.insert(lbfrag.end(), f.begin(), f.end());
+      }
     }
   }
   static const Consts& get() {

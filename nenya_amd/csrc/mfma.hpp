@@ -284,6 +284,108 @@ __global__ void __launch_bounds__(256) k_fold_mx(const uint32_t* frag, FoldJob<T
   }
 }
 
+// y = M x for one 32-element tile (x[e] at x + e*E, e < n), all four
+// row-blocks, written to out + e*ostride*E (A: fragments in LDS).
+template <class T>
+__device__ __forceinline__ void mul_tile(const v4i* A, const LT<T>* x, uint64_t n, LT<T>* out, uint64_t ostride,
+                                         uint64_t tile) {
+  using C = CT<T>;
+  constexpr int ND = kND<T>;
+  const int l = threadIdx.x & 63, col = l & 31, h = l >> 5;
+  const uint64_t e = tile * 32 + col;
+  const bool ok = e < n;
+  v4i B[4][ND];
+#pragma unroll
+  for (int kb = 0; kb < 4; kb++) {
+    uint32_t d[16];
+    if (ok) {
+      ld16k<T>(x + e * E, kb, h, d);
+    } else {
+#pragma unroll
+      for (int b = 0; b < 16; b++) d[b] = 0;
+    }
+    planes16<ND>(d, B[kb]);
+  }
+#pragma unroll
+  for (int rp = 0; rp < 4; rp += 2) {
+    v16i acc[2] = {(v16i)0, (v16i)0};
+#pragma unroll
+    for (int s = ND - 1; s >= 0; s--) {
+      if (s != ND - 1) {
+        acc[0] <<= 8;
+        acc[1] <<= 8;
+      }
+#pragma unroll
+      for (int p = 0; p <= s; p++)
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++)
+#pragma unroll
+          for (int u = 0; u < 2; u++)
+            acc[u] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[frag_at(p, kb, rp + u) * 64 + l], B[kb][s - p], acc[u], 0,
+                                                           0, 0);
+    }
+    if (ok) {
+#pragma unroll
+      for (int u = 0; u < 2; u++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          C y[4];
+#pragma unroll
+          for (int c = 0; c < 4; c++) y[c] = (C)(uint32_t)acc[u][4 * j + c];
+          st4<T>(out + e * ostride * E + 32 * (rp + u) + 8 * j + 4 * h, y);
+        }
+    }
+  }
+}
+
+// out[e*nmat + i] = M_i x[e] for the nmat fixed matrices of frag (matrix i
+// at fragment offset i*ND*16); block (bx, i) keeps matrix i in LDS.
+template <class T>
+__global__ void __launch_bounds__(256) k_mulfix_mx(const uint32_t* frag, const LT<T>* x, uint64_t n, LT<T>* out) {
+  constexpr int ND = kND<T>;
+  __shared__ v4i sA[ND * 16 * 64];
+  const int i = blockIdx.y, nmat = gridDim.y;
+  const v4i* gA = reinterpret_cast<const v4i*>(frag) + (size_t)i * ND * 16 * 64;
+  for (int q = threadIdx.x; q < ND * 16 * 64; q += 256) sA[q] = gA[q];
+  __syncthreads();
+  const uint64_t ntiles = (n + 31) / 32;
+  for (uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntiles; tile += (uint64_t)gridDim.x * 4) {
+    asm volatile("" ::: "memory");
+    mul_tile<T>(sA, x, n, out + (size_t)i * E, (uint64_t)nmat, tile);
+  }
+}
+
+// build_layer_rows (general.rs:233-271) from the products G[w][p][i] =
+// eqs[w][p] * l_beta_i: the packing basis is b_{2i+h} = l_beta_i u_h with
+// u_0 = 1 - alpha, u_1 = alpha (R1 scalars acting per coefficient), so an
+// entry (row 32p+s of matrix w, value v) adds v * G[w][p][s/2] * u_{s%2} to
+// rows[l][q].  One 32-lane group per output (l, q); entries CSR by output,
+// code = p << 7 | w << 5 | s.
+template <class T>
+__global__ void __launch_bounds__(256) k_rows_gather(const uint64_t* ptr, const uint32_t* code, const LT<T>* val,
+                                                     const LT<T>* G, uint64_t nb, LT<T>* rows, uint64_t nout) {
+  using C = CT<T>;
+  const int t = threadIdx.x & 31;
+  for (uint64_t o = (uint64_t)blockIdx.x * 8 + (threadIdx.x >> 5); o < nout; o += (uint64_t)gridDim.x * 8) {
+    C acc[4] = {0, 0, 0, 0};
+    for (uint64_t k = ptr[o]; k < ptr[o + 1]; k++) {
+      const uint32_t cd = code[k];
+      const uint64_t p = cd >> 7, w = (cd >> 5) & 3, s = cd & 31;
+      C g[4], f[4];
+      ld4<T>(G + ((w * nb + p) * 16 + (s >> 1)) * E + 4 * t, g);
+      if (s & 1) {  // alpha g
+        f[0] = 0 - g[3]; f[1] = g[0] - g[3]; f[2] = g[1]; f[3] = g[2];
+      } else {      // (1 - alpha) g
+        f[0] = g[0] + g[3]; f[1] = g[1] - g[0] + g[3]; f[2] = g[2] - g[1]; f[3] = g[3] - g[2];
+      }
+      const C v = (C)val[k];
+#pragma unroll
+      for (int c = 0; c < 4; c++) acc[c] += v * f[c];
+    }
+    st4<T>(rows + o * E + 4 * t, acc);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Layered matvec on the matrix cores (k <= 32).
 //

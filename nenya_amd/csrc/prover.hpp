@@ -163,6 +163,7 @@ template <class T> struct Engine {
   DBuf<L> d_one;
   DBuf<L> d_frob;  // sigma^-(k-1), transposed [j][o]
   DBuf<uint32_t> d_b2frag;  // MFMA fragments of the packing-basis products
+  DBuf<uint32_t> d_lbfrag;  // MFMA fragments of x -> l_beta_i x
   KProf* prof = nullptr;
 
   // Launch on the engine's stream, bracketed by events when profiling.
@@ -199,6 +200,8 @@ template <class T> struct Engine {
     if (!K.b2frag.empty()) {
       d_b2frag = DBuf<uint32_t>(pool, K.b2frag.size());
       HIP_CHECK(hipMemcpyAsync(d_b2frag.p, K.b2frag.data(), K.b2frag.size() * 4, hipMemcpyHostToDevice, st));
+      d_lbfrag = DBuf<uint32_t>(pool, K.lbfrag.size());
+      HIP_CHECK(hipMemcpyAsync(d_lbfrag.p, K.lbfrag.data(), K.lbfrag.size() * 4, hipMemcpyHostToDevice, st));
     }
     HIP_CHECK(hipStreamSynchronize(st));
   }

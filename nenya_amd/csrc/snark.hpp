@@ -29,10 +29,12 @@ struct HostCoo {
 template <class T> struct Csr {
   DBuf<uint32_t> ptr, idx;
   DBuf<typename T::L> val;
+  size_t nnz = 0;
   // group COO by `major` (rows if !by_col), stable
   void build(Pool* pool, hipStream_t st, const HostCoo& m, bool by_col) {
     using L = typename T::L;
     size_t nmaj = by_col ? m.cols : m.rows;
+    nnz = m.nnz;
     std::vector<uint32_t> p(nmaj + 1, 0), ix(m.nnz);
     std::vector<L> v(m.nnz);
     const L* vals = (const L*)m.v;
@@ -247,6 +249,9 @@ template <class T> struct GeneralParams : ParamsBase {
   DBuf<uint32_t> ldk[3], lsel[3], lidx[3];
   DBuf<uint64_t> rptr;
   DBuf<uint32_t> rdk, rsel, ridx;
+  DBuf<uint64_t> eptr;  // layer-row entries by output (matrix-core path)
+  DBuf<uint32_t> ecode;
+  DBuf<L> eval;
 
   // GeneralSnarkParams::secure (general.rs:69-95) with pack_general_matrices
   // (reduce.rs:372-438, 618-651) done once here (setup, not prover time).
@@ -339,6 +344,38 @@ template <class T> struct GeneralParams : ParamsBase {
         }
       upload_csr(ptr, dk, sel, idx, rptr, rdk, rsel, ridx);
     }
+    if (kMx<T>) {
+      // matrix-core layer rows: entries (row 32p+s of matrix w, value v) by
+      // output (l = (c - r) mod 32, q = c / 32), code p << 7 | w << 5 | s
+      if (nb > (size_t(1) << 25)) throw Error(NENYA_E_SHAPE, "too many blocks for the layer-row entry code");
+      auto out_of = [&](uint64_t r, uint64_t c) { return (((c % 32) + 32 - (r % 32)) % 32) * nb + c / 32; };
+      std::vector<uint64_t> ptr(32 * nb + 1, 0);
+      size_t ne = 0;
+      for (int w = 0; w < 3; w++) {
+        ne += m3[w].nnz;
+        for (size_t k2 = 0; k2 < m3[w].nnz; k2++) ptr[out_of(m3[w].r[k2], m3[w].c[k2]) + 1]++;
+      }
+      for (size_t o = 0; o < 32 * nb; o++) ptr[o + 1] += ptr[o];
+      std::vector<uint64_t> pos(ptr.begin(), ptr.end() - 1);
+      std::vector<uint32_t> code(std::max<size_t>(ne, 1));
+      std::vector<L> ev(std::max<size_t>(ne, 1));
+      for (int w = 0; w < 3; w++) {
+        const L* vals = (const L*)m3[w].v;
+        for (size_t k2 = 0; k2 < m3[w].nnz; k2++) {
+          const uint64_t r = m3[w].r[k2], c = m3[w].c[k2];
+          const uint64_t o = pos[out_of(r, c)]++;
+          code[o] = (uint32_t)(((r / 32) << 7) | ((uint64_t)w << 5) | (r % 32));
+          ev[o] = vals[k2];
+        }
+      }
+      eptr = DBuf<uint64_t>(&pool, ptr.size());
+      ecode = DBuf<uint32_t>(&pool, code.size());
+      eval = DBuf<L>(&pool, ev.size());
+      eng->h2d(eptr.p, ptr.data(), ptr.size() * 8);
+      eng->h2d(ecode.p, code.data(), code.size() * 4);
+      eng->h2d(eval.p, ev.data(), ev.size() * sizeof(L));
+      eng->sync();
+    }
     eng->sync();
   }
   void upload_csr(const std::vector<uint64_t>& ptr, const std::vector<uint32_t>& dk, const std::vector<uint32_t>& sel,
@@ -366,6 +403,7 @@ template <class T> struct GeneralParams : ParamsBase {
     dhat.release();
     for (int w = 0; w < 3; w++) { lptr[w].release(); ldk[w].release(); lsel[w].release(); lidx[w].release(); }
     rptr.release(); rdk.release(); rsel.release(); ridx.release();
+    eptr.release(); ecode.release(); eval.release();
     if (st) (void)hipStreamDestroy(st);
   }
 
@@ -473,10 +511,23 @@ template <class T> struct GeneralParams : ParamsBase {
       e.scale(dg2.p, eq1.p, eqs.p + 2 * nb * E, nb);
     }
     DBuf<L> rows(&pool, 32 * nb * E);
-    double tc = (double)(ncell[0] + ncell[1] + ncell[2]);
-    e.launch("k_spmv_gr", tc * 2.0 * Engine<T>::EB, tc * 16 * Engine<T>::PM, (k_spmv_gr<T>), dim3(grid_for(32 * nb, GPB)),
-             dim3(GB), (const uint64_t*)rptr.p, (const uint32_t*)rdk.p, (const uint32_t*)rsel.p, (const uint32_t*)ridx.p,
-             (const L*)dhat.p, (const L*)eqs.p, (uint64_t)nb, rows.p, (uint64_t)(32 * nb));
+    if constexpr (kMx<T>) {
+      // G[w][p][i] = eqs[w][p] l_beta_i on the matrix cores, then the gather
+      DBuf<L> G(&pool, 3 * nb * 16 * E);
+      const uint64_t nx = 3 * nb, ntiles = (nx + 31) / 32;
+      e.launch("k_mulfix_mx", nx * 17.0 * Engine<T>::EB, nx * 16.0 * E * E, (k_mulfix_mx<T>),
+               dim3(grid_for(ntiles, 4, 32), 16), dim3(256), (const uint32_t*)e.d_lbfrag.p, (const L*)eqs.p, nx, G.p);
+      const double ne = (double)(mr[0].nnz + mr[1].nnz + mr[2].nnz);
+      e.launch("k_rows_gather", ne * (Engine<T>::EB + 8) + 32.0 * nb * Engine<T>::EB, ne * E, (k_rows_gather<T>),
+               dim3(grid_for(32 * nb, 8, 1u << 16)), dim3(256), (const uint64_t*)eptr.p, (const uint32_t*)ecode.p,
+               (const L*)eval.p, (const L*)G.p, (uint64_t)nb, rows.p, (uint64_t)(32 * nb));
+    } else {
+      double tc = (double)(ncell[0] + ncell[1] + ncell[2]);
+      e.launch("k_spmv_gr", tc * 2.0 * Engine<T>::EB, tc * 16 * Engine<T>::PM, (k_spmv_gr<T>),
+               dim3(grid_for(32 * nb, GPB)), dim3(GB), (const uint64_t*)rptr.p, (const uint32_t*)rdk.p,
+               (const uint32_t*)rsel.p, (const uint32_t*)ridx.p, (const L*)dhat.p, (const L*)eqs.p, (uint64_t)nb, rows.p,
+               (uint64_t)(32 * nb));
+    }
     eqs.release();
     stg.mark("layer_rows");
     // 10. linear check over the 32 (rho_l, row_l) pairs; rho is low-degree
