@@ -3,9 +3,15 @@
 2^20 R1CS constraints (log_blocks = 15) on one MI355X.
 
 A step is one full proof (prove_general on a witness already resident in
-HBM), returning the serialized proof bytes to the host.  Instance: the
+HBM) whose serialized bytes land in a pinned host buffer.  Instance: the
 random half-constrained R1CS of nenya_amd.instances (seed 1), PCS seeds
 0xDEAD / 0xCAFE, delta = 0.19, lambda = 100, rate 1/4, t = 1060.
+
+Roofline: one profiled proof (HIP events around every launch) names the
+dominant throughput kernel -- the one with the largest summed time among
+kernels moving at least 1 MiB per launch; the timed proofs then carry events
+around that kernel's launches only, and its algorithmic bytes (or int8 MFMA
+operations) per launch over its average duration give `achieved`.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -18,7 +24,9 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
+INT8_PEAK_TOPS = 5000.0    # MI355X dense int8 MFMA, TOP/s
+MX_KERNELS = ("k_fold_mx", "k_gram_lin", "k_mulfix_mx", "k_cgemm")  # int8 digit-plane GEMMs
 
 
 def log(*a):
@@ -26,7 +34,7 @@ def log(*a):
 
 
 def cpu_baseline(k, log_n, seed):
-    """Oracle (C restatement of the reference prover, single thread) timed on
+    """The oracle (C restatement of the reference prover, one thread) timed on
     a bounded sample: one proof at 2^log_n constraints, scaled linearly to
     2^20 (the prover is linear in n up to log factors)."""
     from oracle import load
@@ -37,20 +45,35 @@ def cpu_baseline(k, log_n, seed):
     o.general_prove(k, log_n - 5, A, B, C, w)
     dt = time.perf_counter() - t
     scale = 2 ** (20 - log_n)
-    return {"value": dt * scale, "unit": "s", "cores": 1, "kind": "port",
+    return {"value": round(dt * scale, 3), "unit": "s", "cores": 1, "kind": "port",
             "sample": f"oracle prove_general at 2^{log_n} u{k} constraints took {dt:.2f} s; scaled x{scale} to 2^20"}
+
+
+def traffic_for(kernel, k, log_n):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE scaled by a known-size copy), if one
+    exists for this kernel and workload."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        d = json.load(open(p))
+    except (OSError, ValueError):
+        return None
+    e = d.get(f"{kernel}@u{k}_2p{log_n}")
+    return e.get("bytes_per_launch") if e else None
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--log-n", type=int, default=20)
     ap.add_argument("--k", type=int, default=32)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-log-n", type=int, default=16)
+    ap.add_argument("--cpu-log-n", type=int, default=17)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a 256 MiB device copy (PMC calibration of FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -69,28 +92,22 @@ def main():
     t0 = time.perf_counter()
     params = na.GeneralSnarkParams.secure(log_n - 5, (A, B, C), 0xDEAD, 0xCAFE, k=k)
     log(f"setup (diagonal packing + PCS params) {time.perf_counter() - t0:.1f}s")
-    dt = {8: torch.uint8, 16: torch.int16, 32: torch.int32, 64: torch.int64}[k]
     npdt = {8: np.uint8, 16: np.int16, 32: np.int32, 64: np.int64}[k]
     wd = torch.from_numpy(w.astype({8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}[k]).view(npdt)).to("cuda:0")
     torch.cuda.synchronize()
     n = len(w)
 
+    if args.copy_calib:
+        src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 268435456 bytes device-to-device")
+
     proofs = []
     for i in range(args.warmup):
         proofs.append(na.prove_general_device(wd.data_ptr(), n, params))
         log(f"warmup {i}: proof {len(proofs[-1])} bytes")
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    for i in range(args.steps):
-        # the proof lands in a pinned host buffer owned by the params (no copy)
-        pv = na.prove_general_device(wd.data_ptr(), n, params, view=True)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t
-    ms = elapsed / args.steps * 1e3
-    p = bytes(pv)
-    log(f"timed {args.steps} proofs: {ms:.2f} ms/proof")
-    if proofs and p != proofs[0]:
-        raise SystemExit("non-deterministic proof bytes")
 
     # profiled proof: per-kernel HIP-event durations + stage breakdown
     params.set_profile(True)
@@ -98,21 +115,52 @@ def main():
     na.prove_general_device(wd.data_ptr(), n, params)
     stats = params.kernel_stats()
     stages = params.last_timing()
-    params.set_profile(False)
     params.set_timing(False)
     for name, s in sorted(stats.items(), key=lambda kv: -kv[1]["ms"]):
-        log(f"  {name:16s} {s['ms']:9.3f} ms  x{s['count']:<5d} {s['bytes'] / max(s['ms'], 1e-9) / 1e6:9.1f} GB/s "
-            f"{s['macs'] / max(s['ms'], 1e-9) / 1e9:9.1f} Gmac/s")
+        log(f"  {name:16s} {s['ms']:9.3f} ms  x{s['count']:<5d} {s['bytes'] / max(s['ms'], 1e-9) / 1e6:9.1f} GB/s")
     for name, v in stages.items():
         log(f"  stage {name:16s} {v:9.3f} ms")
-    dom_name, dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
+    top_name, top = max(stats.items(), key=lambda kv: kv[1]["ms"])
+    thr = {nm: s for nm, s in stats.items() if s["bytes"] / s["count"] >= (1 << 20)}
+    dom_name = max(thr.items(), key=lambda kv: kv[1]["ms"])[0]
+
+    # timed proofs; events only around the dominant kernel's launches
+    params.set_profile(True)
+    params.set_profile_only(dom_name)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for i in range(args.steps):
+        # the proof lands in a pinned host buffer owned by the params (no copy)
+        pv = na.prove_general_device(wd.data_ptr(), n, params, view=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t
+    dom = params.kernel_stats()[dom_name]
+    params.set_profile(False)
+    params.set_profile_only(None)
+    ms = elapsed / args.steps * 1e3
+    p = bytes(pv)
+    log(f"timed {args.steps} proofs: {ms:.2f} ms/proof")
+    if proofs and p != proofs[0]:
+        raise SystemExit("non-deterministic proof bytes")
+
     per_launch_ms = dom["ms"] / dom["count"]
     per_launch_bytes = dom["bytes"] / dom["count"]
-    achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom_name,
-                "avg_launch_ms": round(per_launch_ms, 4), "launches_per_proof": dom["count"],
-                "valu_gmacs": round(dom["macs"] / dom["ms"] / 1e6, 1)}
+    gbs = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
+    nd = k // 8
+    int8_ops = dom["macs"] / dom["count"] * (nd * (nd + 1) // 2) * 2 if dom_name in MX_KERNELS else 0.0
+    tops = int8_ops / (per_launch_ms * 1e-3) / 1e12
+    traffic = traffic_for(dom_name, k, log_n)
+    if int8_ops and tops / INT8_PEAK_TOPS > gbs / HBM_PEAK_GBS:
+        roofline = {"bound": "mfma", "achieved": round(tops, 2), "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
+                    "frac": round(tops / INT8_PEAK_TOPS, 4), "traffic": traffic}
+    else:
+        roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic}
+    roofline.update({"kernel": dom_name, "avg_launch_ms": round(per_launch_ms, 4),
+                     "launches_per_proof": round(dom["count"] / args.steps, 2),
+                     "algorithmic_bytes_per_launch": round(per_launch_bytes),
+                     "int8_mfma_tops": round(tops, 2) if int8_ops else None,
+                     "top_kernel_by_time": top_name, "top_kernel_us_per_launch": round(top["ms"] / top["count"] * 1e3, 1)})
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -122,7 +170,7 @@ def main():
 
     out = {
         "metric": "prover sec @2^20 u32 constraints; batched proofs/s + p99 vs plain GPU path",
-        "value": round(ms / 1e3, 5),
+        "value": round(ms / 1e3, 6),
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,

@@ -92,6 +92,9 @@ int nenya_last_timing(const nenya_params* p, double* ms, int cap, char* names, s
  * (ms), launch count, algorithmic bytes and limb multiply-accumulates over
  * the proofs made since profiling was (re)enabled. */
 int nenya_set_profile(nenya_params* p, int enabled);
+/* Restrict profiling to launches of one kernel name (NULL or "" = all), so a
+ * timed run can time its dominant kernel with negligible event overhead. */
+int nenya_set_profile_only(nenya_params* p, const char* kernel);
 int nenya_kernel_stats(const nenya_params* p, int cap, char* names, size_t names_cap, double* ms, double* count,
                        double* bytes, double* macs);
 

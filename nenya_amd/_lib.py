@@ -39,6 +39,7 @@ def _lib():
         L.nenya_set_timing.argtypes = [vp, C.c_int]
         L.nenya_last_timing.argtypes = [vp, C.POINTER(C.c_double), C.c_int, C.c_char_p, sz]
         L.nenya_set_profile.argtypes = [vp, C.c_int]
+        L.nenya_set_profile_only.argtypes = [vp, C.c_char_p]
         dp = C.POINTER(C.c_double)
         L.nenya_kernel_stats.argtypes = [vp, C.c_int, C.c_char_p, sz, dp, dp, dp, dp]
         L.nenya_params_free.argtypes = [vp]
@@ -98,6 +99,9 @@ class _Params:
 
     def set_profile(self, on=True):
         _check(_lib().nenya_set_profile(self._h, 1 if on else 0))
+
+    def set_profile_only(self, kernel=None):
+        _check(_lib().nenya_set_profile_only(self._h, (kernel or "").encode()))
 
     def kernel_stats(self):
         cap = 64

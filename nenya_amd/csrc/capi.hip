@@ -215,6 +215,11 @@ int nenya_set_profile(nenya_params* p, int enabled) {
   p->impl->prof.stats.clear();
   return NENYA_OK;
 }
+int nenya_set_profile_only(nenya_params* p, const char* kernel) {
+  if (!p || !p->impl) return NENYA_E_ARG;
+  p->impl->prof.only = kernel ? kernel : "";
+  return NENYA_OK;
+}
 int nenya_kernel_stats(const nenya_params* p, int cap, char* names, size_t names_cap, double* ms, double* count,
                        double* bytes, double* macs) {
   if (!p || !p->impl) return 0;

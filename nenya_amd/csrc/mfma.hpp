@@ -165,17 +165,19 @@ template <class T> struct FoldJob {
 // Element e of the job: lo pointer (hi = lo + m) and output pointer.
 template <class T>
 __device__ __forceinline__ bool fold_locate(const FoldJob<T>& J, uint64_t e, const LT<T>*& lo, LT<T>*& out) {
+  // selects rather than J.x[g]: a dynamic index into the argument struct
+  // would move it to scratch memory
   const uint64_t n0 = J.m * J.ntab[0];
-  const int g = e < n0 ? 0 : 1;
-  const uint64_t x = g ? e - n0 : e;
-  if (x >= J.m * J.ntab[g]) {
+  const bool g1 = e >= n0;
+  const uint64_t x = g1 ? e - n0 : e;
+  if (x >= J.m * (g1 ? J.ntab[1] : J.ntab[0])) {
     lo = J.src[0];
     out = nullptr;
     return false;
   }
   const uint64_t tab = x / J.m, i = x % J.m;
-  lo = J.src[g] + (tab * J.sstride[g] + i) * E;
-  out = J.dst[g] + (tab * J.dstride[g] + i) * E;
+  lo = (g1 ? J.src[1] : J.src[0]) + (tab * (g1 ? J.sstride[1] : J.sstride[0]) + i) * E;
+  out = (g1 ? J.dst[1] : J.dst[0]) + (tab * (g1 ? J.dstride[1] : J.dstride[0]) + i) * E;
   return true;
 }
 
@@ -246,11 +248,20 @@ __device__ __forceinline__ void fold_tile(const v4i* A, const FoldJob<T>& J, uin
 #pragma unroll
       for (int u = 0; u < NU; u++) {
         const int rb = rb0 + rp + u;
+        // lo at this row-block; rb0 may be a runtime value, so select the
+        // row statically (a dynamic register index would go to scratch)
+        uint32_t arow[16];
+#pragma unroll
+        for (int b = 0; b < 16; b++) arow[b] = a[0][b];
+#pragma unroll
+        for (int kb = 1; kb < 4; kb++)
+#pragma unroll
+          for (int b = 0; b < 16; b++) arow[b] = (rb == kb) ? a[kb][b] : arow[b];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           C x[4];
 #pragma unroll
-          for (int c = 0; c < 4; c++) x[c] = (C)(a[rb][4 * j + c] + (uint32_t)acc[u][4 * j + c]);
+          for (int c = 0; c < 4; c++) x[c] = (C)(arow[4 * j + c] + (uint32_t)acc[u][4 * j + c]);
           st4<T>(out + 32 * rb + 8 * j + 4 * h, x);
         }
       }
@@ -519,89 +530,119 @@ __device__ __forceinline__ void gram_contract_lds(const uint32_t* G, uint32_t (*
 }
 
 // Linear-check round (as k_lin_round): gram 0 over (f_lo, g_lo), gram 1
-// over (f_hi - f_lo, g_hi - g_lo).  Block b (16 waves) handles gram b&1 over
-// chunks of 32 elements: its threads convert the two 128 x 32 limb matrices
-// to digit-plane fragments in LDS (thread (j, eg): limb j of elements
-// 4eg..4eg+3, which are k = kappa(eg&1, 4(eg>>1) + c)), then wave w adds
-// Gram block (w/4, w%4) with one accumulator per digit class.  x rows at
-// limbs >= xrows are zero (low-degree f) and skipped.  The block contracts
-// its Gram matrix and writes the sum as partials[blockIdx.x] (one element;
-// partials laid out [block/2][gram]).
+// over (f_hi - f_lo, g_hi - g_lo), both from one read of the tables.  A
+// block (16 waves) walks chunks of 32 elements: its threads load the four
+// 128 x 32 limb matrices (thread (j, eg): limb j of elements 4eg..4eg+3,
+// which are k = kappa(eg&1, 4(eg>>1) + c)) and store their digit planes in
+// LDS; then each wave takes one or two (gram, 32x32 block) pairs.  Per
+// chunk a Horner pass over the digit classes fills a temporary accumulator
+// that is added to the wave's running total (i32 wrap = mod 2^32).  x rows
+// at limbs >= xrows are zero (low-degree f) and skipped.  The block
+// contracts its two Gram matrices and writes the sums as partials
+// [block][gram].
 template <class T>
 __global__ void __launch_bounds__(1024) k_gram_lin(const LT<T>* f, uint64_t fstride, const LT<T>* gt, uint64_t gstride,
                                                    uint64_t ntab, uint64_t m, int xrows, LT<T>* partials) {
   using L = LT<T>;
   constexpr int ND = kND<T>;
-  // fragments [x|y][plane][rb][lane][word] during the loop, the Gram matrix after
+  // digit planes [x0|y0|x1|y1][plane][rb][lane][word] in the loop, a Gram matrix after
   __shared__ __attribute__((aligned(16))) uint32_t lds[16384];
   __shared__ uint32_t raw[63][7], q[63][4];
   uint32_t (*sF)[ND][4][64][4] = reinterpret_cast<uint32_t (*)[ND][4][64][4]>(lds);
-  const int gram = blockIdx.x & 1;
-  const uint64_t wgs = gridDim.x >> 1, wg = blockIdx.x >> 1;
   const int tid = threadIdx.x, j = tid & 127, eg = tid >> 7;
-  const int w = tid >> 6, l = tid & 63, bx = w >> 2, by = w & 3;
-  const bool active = 32 * bx < xrows;
-  const uint64_t total = m * ntab, nchunks = (total + 31) / 32;
-  v16i acc[ND];
+  const int w = tid >> 6, l = tid & 63;
+  // (gram, bx, by) pairs: xb = xrows/32 row-blocks of x per gram
+  const int xb = xrows / 32, npairs = 2 * xb * 4;
+  int pg[2], pbx[2], pby[2];
+  bool pon[2];
 #pragma unroll
-  for (int d = 0; d < ND; d++) acc[d] = (v16i)0;
+  for (int u = 0; u < 2; u++) {
+    const int pr = w + 16 * u;
+    pon[u] = pr < npairs;
+    pg[u] = pr / (xb * 4);
+    pbx[u] = (pr % (xb * 4)) >> 2;
+    pby[u] = pr & 3;
+  }
+  const uint64_t total = m * ntab, nchunks = (total + 31) / 32;
+  v16i tot[2] = {(v16i)0, (v16i)0};
   const int lane = (j & 31) + 32 * (eg & 1), rb = j >> 5, wd = eg >> 1;
-  for (uint64_t ch = wg; ch < nchunks; ch += wgs) {
-    uint32_t zx[4], zy[4];
+  // raw limbs of the chunk: loads for chunk ch+stride are issued before the
+  // MFMA work of chunk ch, so HBM latency hides behind the matrix cores
+  uint32_t rf[4][2], rg[4][2];  // [element][lo, hi]
+  auto load = [&](uint64_t ch) {
 #pragma unroll
     for (int c = 0; c < 4; c++) {
       const uint64_t e = ch * 32 + 4 * eg + c;
-      uint32_t x = 0, y = 0;
-      if (e < total) {
+      rf[c][0] = rf[c][1] = rg[c][0] = rg[c][1] = 0;
+      if (ch < nchunks && e < total) {
         const uint64_t tab = e / m, i = e % m;
         const L* F = f + (tab * fstride + i) * E;
         const L* G = gt + (tab * gstride + i) * E;
-        x = (uint32_t)F[j];
-        y = (uint32_t)G[j];
-        if (gram) {
-          x = (uint32_t)F[m * E + j] - x;
-          y = (uint32_t)G[m * E + j] - y;
-        }
+        rf[c][0] = (uint32_t)F[j];
+        rf[c][1] = (uint32_t)F[m * E + j];
+        rg[c][0] = (uint32_t)G[j];
+        rg[c][1] = (uint32_t)G[m * E + j];
       }
-      zx[c] = bal_digits<ND>(x);
-      zy[c] = bal_digits<ND>(y);
+    }
+  };
+  load(blockIdx.x);
+  for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    uint32_t z[4][4];  // [x0, y0, x1, y1][element]
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      z[0][c] = bal_digits<ND>(rf[c][0]);
+      z[1][c] = bal_digits<ND>(rg[c][0]);
+      z[2][c] = bal_digits<ND>(rf[c][1] - rf[c][0]);
+      z[3][c] = bal_digits<ND>(rg[c][1] - rg[c][0]);
     }
 #pragma unroll
-    for (int p = 0; p < ND; p++) {
-      uint32_t wx = 0, wy = 0;
+    for (int mt = 0; mt < 4; mt++)
 #pragma unroll
-      for (int c = 0; c < 4; c++) {
-        wx |= ((zx[c] >> (8 * p)) & 0xffu) << (8 * c);
-        wy |= ((zy[c] >> (8 * p)) & 0xffu) << (8 * c);
+      for (int p = 0; p < ND; p++) {
+        uint32_t wv = 0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) wv |= ((z[mt][c] >> (8 * p)) & 0xffu) << (8 * c);
+        sF[mt][p][rb][lane][wd] = wv;
       }
-      sF[0][p][rb][lane][wd] = wx;
-      sF[1][p][rb][lane][wd] = wy;
-    }
     __syncthreads();
-    if (active) {
+    load(ch + gridDim.x);
 #pragma unroll
-      for (int pd = 0; pd < ND; pd++) {
-        const v4i a = *reinterpret_cast<const v4i*>(sF[0][pd][bx][l]);
+    for (int u = 0; u < 2; u++) {
+      if (!pon[u]) continue;
+      const int xm = 2 * pg[u], ym = 2 * pg[u] + 1;
+      v16i tmp = (v16i)0;
 #pragma unroll
-        for (int q2 = 0; pd + q2 < ND; q2++) {
-          const v4i b = *reinterpret_cast<const v4i*>(sF[1][q2][by][l]);
-          acc[pd + q2] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, acc[pd + q2], 0, 0, 0);
+      for (int sd = ND - 1; sd >= 0; sd--) {
+        if (sd != ND - 1) tmp <<= 8;
+#pragma unroll
+        for (int pd = 0; pd <= sd; pd++) {
+          const v4i a = *reinterpret_cast<const v4i*>(sF[xm][pd][pbx[u]][l]);
+          const v4i b = *reinterpret_cast<const v4i*>(sF[ym][sd - pd][pby[u]][l]);
+          tmp = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, tmp, 0, 0, 0);
         }
       }
+      tot[u] += tmp;
     }
     __syncthreads();
   }
+  for (int g = 0; g < 2; g++) {
+    // Gram matrix g into LDS (rows >= xrows stay zero), then contract
+    for (int x = tid; x < 16384; x += 1024) lds[x] = 0;
+    __syncthreads();
 #pragma unroll
-  for (int i = 0; i < 16; i++) {
-    uint32_t y = 0;
+    for (int u = 0; u < 2; u++) {
+      if (!pon[u] || pg[u] != g) continue;
 #pragma unroll
-    for (int d = 0; d < ND; d++) y += (uint32_t)acc[d][i] << (8 * d);
-    const int row = 32 * bx + 8 * (i >> 2) + 4 * (l >> 5) + (i & 3), col = 32 * by + (l & 31);
-    lds[row * 128 + col] = active ? y : 0u;
+      for (int i = 0; i < 16; i++) {
+        const int row = 32 * pbx[u] + 8 * (i >> 2) + 4 * (l >> 5) + (i & 3), col = 32 * pby[u] + (l & 31);
+        lds[row * 128 + col] = (uint32_t)tot[u][i];
+      }
+    }
+    __syncthreads();
+    gram_contract_lds(lds, raw, q);
+    if (tid < E) partials[((size_t)blockIdx.x * 2 + g) * E + tid] = (L)q[tid >> 2][tid & 3];
+    __syncthreads();
   }
-  __syncthreads();
-  gram_contract_lds(lds, raw, q);
-  if (tid < E) partials[(size_t)blockIdx.x * E + tid] = (L)q[tid >> 2][tid & 3];
 }
 
 }  // namespace nenya

@@ -47,6 +47,7 @@ struct Stages {
 // each launch (for roofline figures); off unless enabled on the params.
 struct KProf {
   bool on = false;
+  std::string only;  // when non-empty, only launches of this kernel are timed
   struct Rec { std::string name; hipEvent_t a, b; double bytes, macs; };
   struct Stat { double ms = 0, bytes = 0, macs = 0; long count = 0; };
   std::vector<Rec> recs;
@@ -170,14 +171,15 @@ template <class T> struct Engine {
   template <class Kern, class... Args>
   void launch(const char* name, double bytes, double macs, Kern kern, dim3 grid, dim3 block, Args... args) {
     hipEvent_t a = nullptr, b = nullptr;
-    if (prof && prof->on) {
+    const bool timed = prof && prof->on && (prof->only.empty() || prof->only == name);
+    if (timed) {
       HIP_CHECK(hipEventCreate(&a));
       HIP_CHECK(hipEventCreate(&b));
       HIP_CHECK(hipEventRecord(a, st));
     }
     hipLaunchKernelGGL(kern, grid, block, 0, st, args...);
     HIP_CHECK(hipGetLastError());
-    if (prof && prof->on) {
+    if (timed) {
       HIP_CHECK(hipEventRecord(b, st));
       prof->recs.push_back({name, a, b, bytes, macs});
     }
@@ -331,10 +333,10 @@ template <class T> struct Engine {
   // on the matrix cores; x rows past `xrows` limbs are zero.
   DBuf<L> gram_lin(const L* f, uint64_t fs, const L* g, uint64_t gs, uint64_t ntab, uint64_t m, int xrows) {
     const uint64_t nch = (m * ntab + 31) / 32;
-    const unsigned nwg = (unsigned)std::min<uint64_t>(nch, 128);
+    const unsigned nwg = (unsigned)std::min<uint64_t>(nch, 256);
     DBuf<L> part(pool, (size_t)2 * nwg * E);
-    launch("k_gram_lin", m * ntab * 4.0 * EB, m * ntab * 2.0 * E * E, (k_gram_lin<T>), dim3(2 * nwg), dim3(1024), f, fs,
-           g, gs, ntab, m, xrows, part.p);
+    launch("k_gram_lin", m * ntab * 4.0 * EB, m * ntab * 2.0 * E * E, (k_gram_lin<T>), dim3(nwg), dim3(1024), f, fs, g,
+           gs, ntab, m, xrows, part.p);
     return reduce(part, nwg, 2);
   }
   void marg(const L* s, L* eg, uint64_t m) {
