@@ -27,6 +27,17 @@ def test_simd_config0_bytes_equal_oracle(orc, k):
     assert orc.simd_verify(k, 5, 5, A, B, Cm, got) == "Ok"
 
 
+@pytest.mark.parametrize("k,log", [(32, 9), (8, 10), (16, 11), (64, 9), (32, 12)])
+def test_simd_larger_bytes_equal_oracle(orc, k, log):
+    print(f"simd k={k} log={log} seed=11", flush=True)
+    A, B, Cm, wit = simd_instance(log, k, 11)
+    params = na.SimdSnarkParams.secure(log, log, 0xDEAD, 0xCAFE, k=k)
+    got = na.prove_simd(na.SimdR1csInstance(A, B, Cm, wit), params)
+    want = orc.simd_prove(k, log, log, A, B, Cm, wit)
+    assert len(got) == len(want) and got == want, f"first differing byte {_first_diff(got, want)}"
+    assert orc.simd_verify(k, log, log, A, B, Cm, got) == "Ok"
+
+
 @pytest.mark.parametrize("k,log_n", [(32, 10), (8, 10), (16, 11), (64, 10), (32, 12)])
 def test_general_bytes_equal_oracle(orc, k, log_n):
     print(f"general k={k} log_n={log_n} seed=3", flush=True)
