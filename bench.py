@@ -49,6 +49,71 @@ def cpu_baseline(k, log_n, seed):
             "sample": f"oracle prove_general at 2^{log_n} u{k} constraints took {dt:.2f} s; scaled x{scale} to 2^20"}
 
 
+def mixed_jobs(na, njobs, seed):
+    """The mixed batch: logN ~ U{12..20}, k ~ U{8,16,32,64}, model ~ U{SIMD,
+    General} (SIMD per-instance log = logN - 5, General log_blocks = logN - 5),
+    instance seed = seed + i."""
+    import numpy as np
+    from nenya_amd.instances import general_instance, simd_instance
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(njobs):
+        log_n, k, model = int(rng.integers(12, 21)), int(rng.choice([8, 16, 32, 64])), int(rng.integers(0, 2))
+        if model:
+            A, B, C, w = general_instance(log_n, k, seed + i)
+            p = na.GeneralSnarkParams.secure(log_n - 5, (A, B, C), 0xDEAD, 0xCAFE, k=k)
+            jobs.append(("general", p, w))
+        else:
+            A, B, C, wit = simd_instance(log_n - 5, k, seed + i)
+            p = na.SimdSnarkParams.secure(log_n - 5, log_n - 5, 0xDEAD, 0xCAFE, k=k)
+            jobs.append(("simd", p, na.SimdR1csInstance(A, B, C, wit)))
+    return jobs
+
+
+def run_queue(na, jobs, workers, arrivals=None):
+    """Submit `jobs` to a server with `workers` streams (all at t=0, or at
+    the given arrival offsets in s); returns (wall s, latencies ms)."""
+    import numpy as np
+    srv = na.Server(workers=workers)
+    t0 = time.perf_counter()
+    ids, sub = [], []
+    for i, (model, p, x) in enumerate(jobs):
+        if arrivals is not None:
+            while time.perf_counter() - t0 < arrivals[i]:
+                time.sleep(0.0002)
+        sub.append(time.perf_counter())
+        ids.append(srv.submit_general(p, x) if model == "general" else srv.submit_simd(x, p))
+    lat = []
+    for jid in ids:
+        _, q, r = srv.wait(jid)
+        lat.append(q + r)
+    wall = time.perf_counter() - t0
+    srv.close()
+    return wall, np.array(lat)
+
+
+def queue_stats(na, jobs, workers, seed):
+    """Batch (all jobs at t=0) and serving (Poisson arrivals at 70% of the
+    batch throughput) figures for `workers` streams and for the plain
+    one-at-a-time path (one worker)."""
+    import numpy as np
+    run_queue(na, jobs[:4], workers)  # warm the worker contexts
+    out = {}
+    for tag, wk in (("plain", 1), ("batched", workers)):
+        wall, lat = run_queue(na, jobs, wk)
+        out[tag] = {"proofs_per_s": round(len(jobs) / wall, 2), "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                    "p99_ms": round(float(np.percentile(lat, 99)), 2)}
+        log(f"  batch {tag:8s} workers={wk}: {out[tag]}")
+    lam = 0.7 * out["batched"]["proofs_per_s"]
+    arr = np.cumsum(np.random.default_rng(seed).exponential(1.0 / lam, len(jobs)))
+    serve = {"lambda_per_s": round(lam, 2)}
+    for tag, wk in (("plain", 1), ("batched", workers)):
+        wall, lat = run_queue(na, jobs, wk, arrivals=arr)
+        serve[tag] = {"p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2)}
+        log(f"  serve {tag:8s} workers={wk} lambda={lam:.1f}/s: {serve[tag]}")
+    return out, serve
+
+
 def traffic_for(kernel, k, log_n):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE scaled by a known-size copy), if one
@@ -72,6 +137,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-log-n", type=int, default=17)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-jobs", type=int, default=64, help="mixed batch size (0 skips batch/serve figures)")
+    ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a 256 MiB device copy (PMC calibration of FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -162,6 +229,20 @@ def main():
                      "int8_mfma_tops": round(tops, 2) if int8_ops else None,
                      "top_kernel_by_time": top_name, "top_kernel_us_per_launch": round(top["ms"] / top["count"] * 1e3, 1)})
 
+    batch = serve = None
+    if args.batch_jobs > 0:
+        t0 = time.perf_counter()
+        jobs = mixed_jobs(na, args.batch_jobs, 1000 + args.seed)
+        log(f"mixed batch: {len(jobs)} jobs set up in {time.perf_counter() - t0:.1f}s")
+        q, srv = queue_stats(na, jobs, args.workers, 77 + args.seed)
+        batch = {"jobs": len(jobs), "workers": args.workers, "proofs_per_s": q["batched"]["proofs_per_s"],
+                 "p50_ms": q["batched"]["p50_ms"], "p99_ms": q["batched"]["p99_ms"], "plain": q["plain"],
+                 "mix": "logN~U{12..20}, k~U{8,16,32,64}, model~U{SIMD,General}, host inputs, seed_i=base+i"}
+        serve = {"lambda_per_s": srv["lambda_per_s"], "jobs": len(jobs), "workers": args.workers,
+                 "p50_ms": srv["batched"]["p50_ms"], "p99_ms": srv["batched"]["p99_ms"], "plain": srv["plain"],
+                 "arrivals": "Poisson at 70% of the batched throughput, same job mix"}
+        del jobs
+
     cpu = None
     if not args.no_cpu_baseline:
         log(f"cpu baseline: oracle at 2^{args.cpu_log_n} ...")
@@ -188,6 +269,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "stages_ms": {k2: round(v, 3) for k2, v in stages.items()},
+        "batch": batch,
+        "serve": serve,
     }
     print(json.dumps(out), flush=True)
 

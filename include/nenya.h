@@ -98,6 +98,28 @@ int nenya_set_profile_only(nenya_params* p, const char* kernel);
 int nenya_kernel_stats(const nenya_params* p, int cap, char* names, size_t names_cap, double* ms, double* count,
                        double* bytes, double* macs);
 
+/* ---- Batched and served proofs (no reference counterpart: the reference
+ * proves one instance per call, general.rs:367-372, simd.rs:212-217).
+ * A server owns `workers` host threads, each with its own HIP stream, device
+ * pool and pinned proof buffers, so independent proofs (any params, any
+ * width) overlap on one GPU.  Jobs start in submission order, each once its
+ * device footprint fits `mem_budget` bytes (0 = 80% of the free HBM).  Input
+ * buffers are borrowed until the job is done; params must outlive their
+ * jobs.  nenya_wait returns the job's status and a view of its proof bytes,
+ * valid until nenya_job_release; queued/run times are in milliseconds. */
+typedef struct nenya_server nenya_server;
+int nenya_server_create(int workers, size_t mem_budget, nenya_server** out);
+int nenya_submit_general(nenya_server* s, const nenya_params* p, const void* w, size_t n, uint64_t* job);
+int nenya_submit_simd(nenya_server* s, const nenya_params* p, const uint32_t* ar, const uint32_t* ac, const void* av,
+                      size_t an, const uint32_t* br, const uint32_t* bc, const void* bv, size_t bn, const uint32_t* cr,
+                      const uint32_t* cc, const void* cv, size_t cn, const void* witnesses, uint64_t* job);
+/* 1 = done, 0 = queued or running, < 0 = -(error code) of the call itself */
+int nenya_poll(nenya_server* s, uint64_t job);
+int nenya_wait(nenya_server* s, uint64_t job, const uint8_t** proof, size_t* proof_len, double* queued_ms,
+               double* run_ms);
+int nenya_job_release(nenya_server* s, uint64_t job);
+void nenya_server_free(nenya_server* s);
+
 void nenya_params_free(nenya_params* p);
 void nenya_free(void* ptr);
 const char* nenya_last_error(void);

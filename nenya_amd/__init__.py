@@ -9,12 +9,18 @@ crates/nenya/snark-ring/src/lib.rs:23-36):
     params = GeneralSnarkParams.secure(log_blocks, (a, b, c), seed_w, seed_e, k=32)
     proof  = prove_general(w, params)
 
+Batched / served proofs (several proofs overlapping on one GPU):
+
+    srv = Server(workers=4)
+    job = srv.submit_general(params, w)       # or srv.submit_simd(inst, params)
+    proof, queued_ms, run_ms = srv.wait(job)
+
 Matrices are COO triples (rows u32, cols u32, vals) with vals limbs of Z/2^k;
 witnesses are limb arrays.  Proofs are the reference's serialized bytes.
 Every call runs the HIP library nenya_amd/libnenya_amd.so; there is no CPU
 fallback: without the library or a GPU the calls raise.
 """
 from ._lib import (  # noqa: F401
-    NenyaError, GeneralSnarkParams, SimdR1csInstance, SimdSnarkParams, device_count, lib_path,
+    NenyaError, GeneralSnarkParams, Server, SimdR1csInstance, SimdSnarkParams, device_count, lib_path,
     prove_general, prove_general_device, prove_simd,
 )

@@ -42,6 +42,13 @@ def _lib():
         L.nenya_set_profile_only.argtypes = [vp, C.c_char_p]
         dp = C.POINTER(C.c_double)
         L.nenya_kernel_stats.argtypes = [vp, C.c_int, C.c_char_p, sz, dp, dp, dp, dp]
+        L.nenya_server_create.argtypes = [C.c_int, sz, C.POINTER(vp)]
+        L.nenya_submit_general.argtypes = [vp, vp, vp, sz, C.POINTER(u64)]
+        L.nenya_submit_simd.argtypes = [vp, vp] + coo * 3 + [vp, C.POINTER(u64)]
+        L.nenya_poll.argtypes = [vp, u64]
+        L.nenya_wait.argtypes = [vp, u64, C.POINTER(vp), C.POINTER(sz), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.nenya_job_release.argtypes = [vp, u64]
+        L.nenya_server_free.argtypes = [vp]
         L.nenya_params_free.argtypes = [vp]
         L.nenya_free.argtypes = [vp]
         L.nenya_last_error.restype = C.c_char_p
@@ -193,3 +200,63 @@ def prove_general_device(dev_ptr, n_limbs, params, view=False):
         return memoryview((C.c_ubyte * n.value).from_address(out.value)).toreadonly()
     _check(_lib().nenya_prove_general_device(params._h, C.c_void_p(dev_ptr), n_limbs, C.byref(out), C.byref(n)))
     return _take_proof(out, n)
+
+
+class Server:
+    """Batched / served proofs on one GPU (nenya_server_* in include/nenya.h):
+    `workers` host threads with their own streams drain a FIFO job queue,
+    admitting jobs by device-memory footprint.  Inputs are kept alive by the
+    Server until the job is collected."""
+
+    def __init__(self, workers=4, mem_budget=0):
+        h = C.c_void_p()
+        _check(_lib().nenya_server_create(workers, mem_budget, C.byref(h)))
+        self._h = h
+        self._keep = {}
+
+    def submit_general(self, params, w):
+        w = np.ascontiguousarray(np.asarray(w).astype(_DT[params.k]))
+        j = C.c_uint64()
+        _check(_lib().nenya_submit_general(self._h, params._h, w.ctypes.data, len(w), C.byref(j)))
+        self._keep[j.value] = (params, w)
+        return j.value
+
+    def submit_simd(self, inst, params):
+        k = params.k
+        mats = [_coo(m, k) for m in (inst.a, inst.b, inst.c)]
+        wit = np.ascontiguousarray(np.asarray(inst.witnesses).astype(_DT[k]))
+        if wit.shape != (32, 1 << params.log_m):
+            raise NenyaError(1, f"witnesses must have shape (32, {1 << params.log_m})")
+        args = [self._h, params._h]
+        for m in mats:
+            args += _coo_args(m)
+        j = C.c_uint64()
+        _check(_lib().nenya_submit_simd(*args, wit.ctypes.data, C.byref(j)))
+        self._keep[j.value] = (params, mats, wit)
+        return j.value
+
+    def poll(self, job):
+        r = _lib().nenya_poll(self._h, job)
+        if r < 0:
+            _check(-r)
+        return r == 1
+
+    def wait(self, job):
+        """-> (proof bytes, queued_ms, run_ms); releases the job."""
+        buf, n = C.c_void_p(), C.c_size_t()
+        q, r = C.c_double(), C.c_double()
+        try:
+            _check(_lib().nenya_wait(self._h, job, C.byref(buf), C.byref(n), C.byref(q), C.byref(r)))
+            return C.string_at(buf, n.value), q.value, r.value
+        finally:
+            _lib().nenya_job_release(self._h, job)
+            self._keep.pop(job, None)
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h and _LIB is not None:
+            _LIB.nenya_server_free(h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

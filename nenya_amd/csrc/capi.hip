@@ -3,7 +3,7 @@
 #include <mutex>
 
 #include "../../include/nenya.h"
-#include "snark.hpp"
+#include "server.hpp"
 
 using namespace nenya;
 
@@ -52,16 +52,16 @@ static uint8_t* emit(const ProofView& v, uint8_t** proof, size_t* len) {
 template <class T>
 static ProofView simd_prove_t(ParamsBase* b, const HostCoo* m3, const void* wit) {
   auto* sp = static_cast<SimdParams<T>*>(b);
-  return sp->prove(m3[0], m3[1], m3[2], (const typename T::L*)wit);
+  return sp->prove(sp->cx0, m3[0], m3[1], m3[2], (const typename T::L*)wit);
 }
 
 template <class T> static ProofView general_t(ParamsBase* b, const void* w, size_t n, bool dev) {
   auto* gp = static_cast<GeneralParams<T>*>(b);
   if (dev) {
     if (n != gp->N) throw Error(NENYA_E_SHAPE, "witness length must be 32 * 2^log_blocks");
-    return gp->prove_device((const typename T::L*)w);
+    return gp->prove_device(gp->cx0, (const typename T::L*)w);
   }
-  return gp->prove_host((const typename T::L*)w, n);
+  return gp->prove_host(gp->cx0, (const typename T::L*)w, n);
 }
 
 static int general_common(const nenya_params* p, const void* w, size_t n, uint8_t** proof, size_t* len, bool dev,
@@ -206,18 +206,18 @@ int nenya_prove_general_device_view(const nenya_params* p, const void* d_w, size
 
 int nenya_set_timing(nenya_params* p, int enabled) {
   if (!p || !p->impl) return NENYA_E_ARG;
-  p->impl->timing = enabled != 0;
+  p->impl->ctx0().timing = enabled != 0;
   return NENYA_OK;
 }
 int nenya_set_profile(nenya_params* p, int enabled) {
   if (!p || !p->impl) return NENYA_E_ARG;
-  p->impl->prof.on = enabled != 0;
-  p->impl->prof.stats.clear();
+  p->impl->ctx0().prof.on = enabled != 0;
+  p->impl->ctx0().prof.stats.clear();
   return NENYA_OK;
 }
 int nenya_set_profile_only(nenya_params* p, const char* kernel) {
   if (!p || !p->impl) return NENYA_E_ARG;
-  p->impl->prof.only = kernel ? kernel : "";
+  p->impl->ctx0().prof.only = kernel ? kernel : "";
   return NENYA_OK;
 }
 int nenya_kernel_stats(const nenya_params* p, int cap, char* names, size_t names_cap, double* ms, double* count,
@@ -225,7 +225,7 @@ int nenya_kernel_stats(const nenya_params* p, int cap, char* names, size_t names
   if (!p || !p->impl) return 0;
   std::string all;
   int i = 0;
-  for (const auto& kv : p->impl->prof.stats) {
+  for (const auto& kv : p->impl->ctx0().prof.stats) {
     if (i >= cap) break;
     if (ms) ms[i] = kv.second.ms;
     if (count) count[i] = (double)kv.second.count;
@@ -244,7 +244,7 @@ int nenya_kernel_stats(const nenya_params* p, int cap, char* names, size_t names
 }
 int nenya_last_timing(const nenya_params* p, double* ms, int cap, char* names, size_t names_cap) {
   if (!p || !p->impl) return 0;
-  const auto& t = p->impl->last_timing;
+  const auto& t = p->impl->ctx0().last_timing;
   int n = (int)std::min<size_t>(t.size(), cap > 0 ? (size_t)cap : 0);
   std::string all;
   for (int i = 0; i < n; i++) {
@@ -258,6 +258,107 @@ int nenya_last_timing(const nenya_params* p, double* ms, int cap, char* names, s
     names[c] = 0;
   }
   return n;
+}
+
+// ---------------- batched / served proofs ----------------
+struct nenya_server {
+  Server* impl = nullptr;
+};
+
+int nenya_server_create(int workers, size_t mem_budget, nenya_server** out) {
+  return guard([&] {
+    if (!out || workers < 1 || workers > 64) throw Error(NENYA_E_ARG, "workers must be 1..64");
+    need_gpu();
+    auto* s = new nenya_server;
+    try {
+      s->impl = new Server(workers, mem_budget);
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+    return NENYA_OK;
+  });
+}
+
+int nenya_submit_general(nenya_server* s, const nenya_params* p, const void* w, size_t n, uint64_t* job) {
+  return guard([&] {
+    if (!s || !s->impl || !p || !p->impl || p->impl->kind != 1 || !w || !job) throw Error(NENYA_E_ARG, "bad argument");
+    std::unique_ptr<Job> j(new Job);
+    j->params = p->impl;
+    j->kind = 1;
+    j->w = w;
+    j->n = n;
+    *job = s->impl->submit(std::move(j));
+    return NENYA_OK;
+  });
+}
+
+int nenya_submit_simd(nenya_server* s, const nenya_params* p, const uint32_t* ar, const uint32_t* ac, const void* av,
+                      size_t an, const uint32_t* br, const uint32_t* bc, const void* bv, size_t bn, const uint32_t* cr,
+                      const uint32_t* cc, const void* cv, size_t cn, const void* witnesses, uint64_t* job) {
+  return guard([&] {
+    if (!s || !s->impl || !p || !p->impl || p->impl->kind != 0 || !witnesses || !job)
+      throw Error(NENYA_E_ARG, "bad argument");
+    if ((an && (!ar || !ac || !av)) || (bn && (!br || !bc || !bv)) || (cn && (!cr || !cc || !cv)))
+      throw Error(NENYA_E_ARG, "null matrix array");
+    ParamsBase* b = p->impl;
+    int ln = 0, lm = 0;
+    switch (b->k) {
+      case 8: ln = static_cast<SimdParams<W<8>>*>(b)->log_n; lm = static_cast<SimdParams<W<8>>*>(b)->log_m; break;
+      case 16: ln = static_cast<SimdParams<W<16>>*>(b)->log_n; lm = static_cast<SimdParams<W<16>>*>(b)->log_m; break;
+      case 32: ln = static_cast<SimdParams<W<32>>*>(b)->log_n; lm = static_cast<SimdParams<W<32>>*>(b)->log_m; break;
+      default: ln = static_cast<SimdParams<W<64>>*>(b)->log_n; lm = static_cast<SimdParams<W<64>>*>(b)->log_m; break;
+    }
+    size_t n = size_t(1) << ln, m = size_t(1) << lm;
+    std::unique_ptr<Job> j(new Job);
+    j->params = b;
+    j->kind = 0;
+    j->w = witnesses;
+    j->m3[0] = {n, m, an, ar, ac, av};
+    j->m3[1] = {n, m, bn, br, bc, bv};
+    j->m3[2] = {n, m, cn, cr, cc, cv};
+    *job = s->impl->submit(std::move(j));
+    return NENYA_OK;
+  });
+}
+
+int nenya_poll(nenya_server* s, uint64_t job) {
+  int done = 0;
+  int rc = guard([&] {
+    if (!s || !s->impl) throw Error(NENYA_E_ARG, "bad argument");
+    done = s->impl->poll(job) ? 1 : 0;
+    return NENYA_OK;
+  });
+  return rc == NENYA_OK ? done : -rc;
+}
+
+int nenya_wait(nenya_server* s, uint64_t job, const uint8_t** proof, size_t* proof_len, double* queued_ms,
+               double* run_ms) {
+  return guard([&] {
+    if (!s || !s->impl || !proof || !proof_len) throw Error(NENYA_E_ARG, "bad argument");
+    Job* j = s->impl->wait(job);
+    if (queued_ms) *queued_ms = j->t_start - j->t_submit;
+    if (run_ms) *run_ms = j->t_end - j->t_start;
+    if (j->rc != NENYA_OK) throw Error(j->rc, j->err);
+    *proof = j->proof.p;
+    *proof_len = j->len;
+    return NENYA_OK;
+  });
+}
+
+int nenya_job_release(nenya_server* s, uint64_t job) {
+  return guard([&] {
+    if (!s || !s->impl) throw Error(NENYA_E_ARG, "bad argument");
+    s->impl->release(job);
+    return NENYA_OK;
+  });
+}
+
+void nenya_server_free(nenya_server* s) {
+  if (!s) return;
+  delete s->impl;
+  delete s;
 }
 
 }  // extern "C"

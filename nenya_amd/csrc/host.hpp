@@ -2,6 +2,8 @@
 // PCS parameters, Fiat-Shamir transcript.
 #pragma once
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -16,6 +18,30 @@ namespace nenya {
 // Caching device allocator: blocks are returned to per-size free lists and
 // reused by later proofs with the same parameters (no hipMalloc in steady state).
 // ---------------------------------------------------------------------------
+// Debug switch: NENYA_POISON=1 fills every block the pool hands out with
+// 0xA5 bytes, so a kernel that reads memory nobody wrote shows up as a
+// wrong proof instead of depending on what the block held before.
+inline bool pool_poison() {
+  static const bool on = [] {
+    const char* e = std::getenv("NENYA_POISON");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
+
+// Debug switch: NENYA_GUARD=1 surrounds every pool block with 64 KiB guard
+// zones of 0x5A bytes, checked when the block is returned (an out-of-bounds
+// write is reported with the block size and offset; an out-of-bounds read
+// sees 0x5A instead of whatever neighbours the block).
+inline bool pool_guard() {
+  static const bool on = [] {
+    const char* e = std::getenv("NENYA_GUARD");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
+constexpr size_t GUARD = 1 << 16;
+
 struct Pool {
   std::multimap<size_t, void*> free_;
   std::vector<std::pair<void*, size_t>> all_;
@@ -31,9 +57,21 @@ struct Pool {
       free_.erase(it);
       live += bytes;
       peak = std::max(peak, live);
+      if (pool_poison()) poison(p, bytes);
+      if (pool_guard()) fill_guards(p, bytes);
       return p;
     }
     void* p = nullptr;
+    if (pool_guard()) {
+      void* base = nullptr;
+      if (hipMalloc(&base, bytes + 2 * GUARD) != hipSuccess) throw Error(NENYA_E_OOM, "guarded allocation failed");
+      p = (uint8_t*)base + GUARD;
+      all_.push_back({base, bytes});
+      live += bytes;
+      peak = std::max(peak, live);
+      fill_guards(p, bytes);
+      return p;
+    }
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
       // release cached blocks and retry once
@@ -50,12 +88,47 @@ struct Pool {
     all_.push_back({p, bytes});
     live += bytes;
     peak = std::max(peak, live);
+    if (pool_poison()) poison(p, bytes);
     return p;
+  }
+  static void fill_guards(void* p, size_t bytes) {
+    if (hipDeviceSynchronize() != hipSuccess || hipMemset((uint8_t*)p - GUARD, 0x5A, GUARD) != hipSuccess ||
+        hipMemset((uint8_t*)p + bytes, 0x5A, GUARD) != hipSuccess || hipMemset(p, 0xA5, bytes) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess)
+      throw Error(NENYA_E_DEVICE, "guard fill failed");
+  }
+  static void check_guards(void* p, size_t bytes) {
+    std::vector<uint8_t> g(GUARD);
+    if (hipDeviceSynchronize() != hipSuccess) throw Error(NENYA_E_DEVICE, "guard check sync failed");
+    for (int side = 0; side < 2; side++) {
+      const uint8_t* at = side ? (const uint8_t*)p + bytes : (const uint8_t*)p - GUARD;
+      if (hipMemcpy(g.data(), at, GUARD, hipMemcpyDeviceToHost) != hipSuccess) throw Error(NENYA_E_DEVICE, "guard read failed");
+      for (size_t i = 0; i < GUARD; i++)
+        if (g[i] != 0x5A) {
+          std::fprintf(stderr, "NENYA_GUARD: %s overrun of a %zu-byte block at guard offset %zu (byte 0x%02x)\n",
+                       side ? "upper" : "lower", bytes, i, g[i]);
+          std::fflush(stderr);
+          break;
+        }
+    }
+  }
+  static void poison(void* p, size_t bytes) {
+    // device-wide: orders after every stream's pending work on the block
+    if (hipDeviceSynchronize() != hipSuccess || hipMemset(p, 0xA5, bytes) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess)
+      throw Error(NENYA_E_DEVICE, "poison fill failed");
+  }
+  // restart peak tracking; returns the bytes live now
+  size_t mark() {
+    std::lock_guard<std::mutex> g(mu);
+    peak = live;
+    return live;
   }
   void put(void* p, size_t bytes) {
     if (!p) return;
     bytes = (bytes + 255) & ~size_t(255);
     if (bytes == 0) bytes = 256;
+    if (pool_guard()) check_guards(p, bytes);
     std::lock_guard<std::mutex> g(mu);
     free_.insert({bytes, p});
     live -= bytes;

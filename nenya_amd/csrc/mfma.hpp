@@ -183,10 +183,12 @@ __device__ __forceinline__ bool fold_locate(const FoldJob<T>& J, uint64_t e, con
 
 // One 32-element fold tile: loads lo/hi, forms the digit planes of hi - lo,
 // then row-blocks rb0 .. rb0+NRB-1 of lo + M_r (hi - lo).  A points at the
-// fragments (LDS or global); with PRE the wave first loads every fragment it
-// needs into registers (global source: one round of latency, not one per MFMA).
-template <class T, int NRB, bool PRE>
+// fragments (LDS or global).  SPLIT: the block's four waves share the tile,
+// one row-block each; every wave first loads the fragments it needs into
+// registers (global source: one round of latency, not one per MFMA).
+template <class T, int NRB, bool SPLIT>
 __device__ __forceinline__ void fold_tile(const v4i* A, const FoldJob<T>& J, uint64_t tile, int rb0) {
+  constexpr bool PRE = SPLIT;
   using C = CT<T>;
   using L = LT<T>;
   constexpr int ND = kND<T>;
@@ -221,6 +223,13 @@ __device__ __forceinline__ void fold_tile(const v4i* A, const FoldJob<T>& J, uin
 #pragma unroll
     for (int b = 0; b < 16; b++) d[kb][b] -= a[kb][b];
     planes16<ND>(d[kb], B[kb]);
+  }
+  if constexpr (SPLIT) {
+    // Each wave has read all 128 limbs of lo and hi but writes only its own
+    // row-block of lo.  The fold may be in place (dst == src), so no wave
+    // stores before every wave of the block holds its operands.
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
   }
   constexpr int NU = NRB >= 2 ? 2 : 1;  // row-blocks per accumulator set
 #pragma unroll
@@ -274,7 +283,9 @@ __device__ __forceinline__ void fold_tile(const v4i* A, const FoldJob<T>& J, uin
 // Bulk mode: persistent blocks of 4 waves, a wave folds 32 elements per
 // tile, the M_r fragments (ND*16 KB) sit in LDS for the block's lifetime.
 // Split mode (few tiles): block b takes tile b and wave w its row-block w,
-// with its fragments loaded from global memory (L2-resident) up front.
+// with its fragments loaded from global memory (L2-resident) up front; a
+// barrier between the loads and the stores keeps in-place folds exact.
+// Bulk mode needs none: a wave loads and stores only its own tile.
 template <class T>
 __global__ void __launch_bounds__(256) k_fold_mx(const uint32_t* frag, FoldJob<T> J, int split) {
   constexpr int ND = kND<T>;

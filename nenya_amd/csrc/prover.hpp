@@ -73,12 +73,26 @@ struct KProf {
 struct Pinned {
   uint8_t* p = nullptr;
   size_t cap = 0;
+  unsigned flags = hipHostMallocDefault;  // hipHostMallocCoherent for kernel-shared areas
+  Pinned() = default;
+  Pinned(const Pinned&) = delete;
+  Pinned& operator=(const Pinned&) = delete;
+  Pinned(Pinned&& o) noexcept : p(o.p), cap(o.cap), flags(o.flags) {
+    o.p = nullptr;
+    o.cap = 0;
+  }
+  Pinned& operator=(Pinned&& o) noexcept {
+    std::swap(p, o.p);
+    std::swap(cap, o.cap);
+    std::swap(flags, o.flags);
+    return *this;
+  }
   uint8_t* get(size_t n) {
     if (n > cap) {
       if (p) (void)hipHostFree(p);
       p = nullptr;
       cap = 0;
-      HIP_CHECK(hipHostMalloc((void**)&p, n ? n : 1, hipHostMallocDefault));
+      HIP_CHECK(hipHostMalloc((void**)&p, n ? n : 1, flags));
       cap = n;
     }
     return p;
@@ -125,18 +139,18 @@ template <class T> struct PcsDev {
   PcsParams P;
   DBuf<uint32_t> perm1, perm2;
   DBuf<L> d1, d2;
-  void upload(Pool* pool, hipStream_t st) {
-    perm1 = DBuf<uint32_t>(pool, P.n);
-    perm2 = DBuf<uint32_t>(pool, P.n);
-    d1 = DBuf<L>(pool, P.n);
-    d2 = DBuf<L>(pool, P.n);
+  template <class Eng> void upload(Eng& e) {
+    perm1 = DBuf<uint32_t>(e.pool, P.n);
+    perm2 = DBuf<uint32_t>(e.pool, P.n);
+    d1 = DBuf<L>(e.pool, P.n);
+    d2 = DBuf<L>(e.pool, P.n);
     std::vector<L> a(P.n), b(P.n);
     for (size_t i = 0; i < P.n; i++) { a[i] = (L)P.d1[i]; b[i] = (L)P.d2[i]; }
-    HIP_CHECK(hipMemcpyAsync(perm1.p, P.perm1.data(), P.n * 4, hipMemcpyHostToDevice, st));
-    HIP_CHECK(hipMemcpyAsync(perm2.p, P.perm2.data(), P.n * 4, hipMemcpyHostToDevice, st));
-    HIP_CHECK(hipMemcpyAsync(d1.p, a.data(), P.n * sizeof(L), hipMemcpyHostToDevice, st));
-    HIP_CHECK(hipMemcpyAsync(d2.p, b.data(), P.n * sizeof(L), hipMemcpyHostToDevice, st));
-    HIP_CHECK(hipStreamSynchronize(st));
+    e.h2d(perm1.p, P.perm1.data(), P.n * 4);
+    e.h2d(perm2.p, P.perm2.data(), P.n * 4);
+    e.h2d(d1.p, a.data(), P.n * sizeof(L));
+    e.h2d(d2.p, b.data(), P.n * sizeof(L));
+    e.sync();
   }
 };
 
@@ -195,43 +209,50 @@ template <class T> struct Engine {
     std::vector<L> b(32 * E);
     for (int i = 0; i < 32; i++) std::memcpy(&b[i * E], K.basis[i].v, E * sizeof(L));
     G one = G::one();
-    HIP_CHECK(hipMemcpyAsync(d_basis.p, b.data(), b.size() * sizeof(L), hipMemcpyHostToDevice, st));
-    HIP_CHECK(hipMemcpyAsync(d_pw.p, K.psi_pw.data(), K.psi_pw.size() * sizeof(L), hipMemcpyHostToDevice, st));
-    HIP_CHECK(hipMemcpyAsync(d_one.p, one.v, E * sizeof(L), hipMemcpyHostToDevice, st));
-    HIP_CHECK(hipMemcpyAsync(d_frob.p, K.frob_t.data(), E * E * sizeof(L), hipMemcpyHostToDevice, st));
+    upbuf.flags = resbuf.flags = hipHostMallocCoherent;
+    h2d(d_basis.p, b.data(), b.size() * sizeof(L));
+    h2d(d_pw.p, K.psi_pw.data(), K.psi_pw.size() * sizeof(L));
+    h2d(d_one.p, one.v, E * sizeof(L));
+    h2d(d_frob.p, K.frob_t.data(), E * E * sizeof(L));
     if (!K.b2frag.empty()) {
       d_b2frag = DBuf<uint32_t>(pool, K.b2frag.size());
-      HIP_CHECK(hipMemcpyAsync(d_b2frag.p, K.b2frag.data(), K.b2frag.size() * 4, hipMemcpyHostToDevice, st));
+      h2d(d_b2frag.p, K.b2frag.data(), K.b2frag.size() * 4);
       d_lbfrag = DBuf<uint32_t>(pool, K.lbfrag.size());
-      HIP_CHECK(hipMemcpyAsync(d_lbfrag.p, K.lbfrag.data(), K.lbfrag.size() * 4, hipMemcpyHostToDevice, st));
+      h2d(d_lbfrag.p, K.lbfrag.data(), K.lbfrag.size() * 4);
     }
-    HIP_CHECK(hipStreamSynchronize(st));
+    sync();
   }
 
   void check() { HIP_CHECK(hipGetLastError()); }
   void sync() {
-    // spin on the stream instead of sleeping on its completion signal: the
-    // sumcheck rounds wait for a few hundred bytes about 30 times per proof
-    hipError_t e;
-    while ((e = hipStreamQuery(st)) == hipErrorNotReady) {
-    }
-    HIP_CHECK(e);
-    up_off = 0;
+    // the sumcheck rounds wait here for a few hundred bytes about 30 times
+    // per proof; other contexts keep the device busy meanwhile
+    HIP_CHECK(hipStreamSynchronize(st));
   }
 
-  // Host-to-device copies through pinned staging: a slot stays untouched
-  // until the next stream synchronization, which recycles the ring.
+  // Host-device transfers never touch pageable memory (pageable copies stage
+  // through buffers shared by the whole process): small uploads go through a
+  // coherent pinned ring, larger ones and all downloads through a pinned
+  // bounce buffer.  A ring slot is recycled only after a synchronization
+  // that follows its use (the ring rotates; it is not reset per round).
   static constexpr size_t UPCAP = 1 << 16;
-  Pinned upbuf;
+  Pinned upbuf, xfer;
   size_t up_off = 0;
+  uint8_t* ring_slot(size_t bytes) {
+    if (up_off + bytes > UPCAP) {
+      sync();
+      up_off = 0;
+    }
+    uint8_t* slot = upbuf.get(UPCAP) + up_off;
+    up_off += (bytes + 255) & ~size_t(255);
+    return slot;
+  }
   // Copies src into the pinned ring and returns its device-side address, for
-  // kernels that read it directly (valid until the next synchronization).
+  // kernels that read it directly.
   const void* stage_host(const void* src, size_t bytes) {
     if (bytes > UPCAP) throw Error(NENYA_E_DEVICE, "staging request too large");
-    if (up_off + bytes > UPCAP) sync();
-    uint8_t* slot = upbuf.get(UPCAP) + up_off;
+    uint8_t* slot = ring_slot(bytes);
     std::memcpy(slot, src, bytes);
-    up_off += (bytes + 255) & ~size_t(255);
     void* dptr = nullptr;
     HIP_CHECK(hipHostGetDevicePointer(&dptr, slot, 0));
     return dptr;
@@ -245,23 +266,33 @@ template <class T> struct Engine {
     HIP_CHECK(hipHostGetDevicePointer(&dptr, res_host(), 0));
     return (L*)dptr;
   }
-  void put_async(L* dst, const void* src, size_t bytes) {
-    if (bytes > UPCAP) {
-      h2d(dst, src, bytes);
-      sync();
+  void put_async(void* dst, const void* src, size_t bytes) { h2d(dst, src, bytes); }
+
+  // Upload (stream-ordered; the host buffer may be reused on return).
+  void h2d(void* d, const void* h, size_t bytes) {
+    if (!bytes) return;
+    if (bytes <= UPCAP / 4) {
+      uint8_t* slot = ring_slot(bytes);
+      std::memcpy(slot, h, bytes);
+      HIP_CHECK(hipMemcpyAsync(d, slot, bytes, hipMemcpyHostToDevice, st));
       return;
     }
-    if (up_off + bytes > UPCAP) sync();
-    uint8_t* slot = upbuf.get(UPCAP) + up_off;
-    std::memcpy(slot, src, bytes);
-    up_off += (bytes + 255) & ~size_t(255);
-    HIP_CHECK(hipMemcpyAsync(dst, slot, bytes, hipMemcpyHostToDevice, st));
-  }
-
-  void h2d(void* d, const void* h, size_t bytes) { HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st)); }
-  void d2h(void* h, const void* d, size_t bytes) {
-    HIP_CHECK(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st));
+    const size_t CH = size_t(8) << 20;  // bounce in chunks, waiting before reuse
+    uint8_t* b = xfer.get(std::min(bytes, CH));
+    for (size_t off = 0; off < bytes; off += CH) {
+      const size_t c = std::min(CH, bytes - off);
+      if (off) sync();
+      std::memcpy(b, (const uint8_t*)h + off, c);
+      HIP_CHECK(hipMemcpyAsync((uint8_t*)d + off, b, c, hipMemcpyHostToDevice, st));
+    }
     sync();
+  }
+  void d2h(void* h, const void* d, size_t bytes) {
+    if (!bytes) return;
+    uint8_t* b = xfer.get(bytes);
+    HIP_CHECK(hipMemcpyAsync(b, d, bytes, hipMemcpyDeviceToHost, st));
+    sync();
+    std::memcpy(h, b, bytes);
   }
   void d2d(void* d, const void* s, size_t bytes) { HIP_CHECK(hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, st)); }
   G get(const L* d) {

@@ -1,0 +1,189 @@
+// Batched and served proofs on one GPU.  The reference proves one instance
+// per call (snark-ring/src/general.rs:367-372, simd.rs:212-217); here
+// independent proofs are queued and drained by worker threads, each with its
+// own execution contexts (HIP stream, stream-ordered pool, pinned proof
+// buffer) per limb width, so the latency-bound phases of one proof (sumcheck
+// round trips, challenge lifts) overlap the throughput phases of others.
+// Admission: the queue is FIFO and its head starts only when its device
+// footprint fits the remaining budget (or nothing else is running).
+#pragma once
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <thread>
+#include <unordered_map>
+
+#include "snark.hpp"
+
+namespace nenya {
+
+inline double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Job {
+  uint64_t id = 0;
+  ParamsBase* params = nullptr;
+  int kind = 1;              // 0 simd, 1 general
+  const void* w = nullptr;   // general: witness (host); simd: 32 x m witnesses
+  size_t n = 0;
+  HostCoo m3[3];             // simd matrices (borrowed until the job ends)
+  Pinned proof;
+  size_t len = 0;
+  size_t fp = 0;
+  int rc = NENYA_OK;
+  std::string err;
+  double t_submit = 0, t_start = 0, t_end = 0;
+  bool done = false;
+};
+
+struct Worker {
+  std::unique_ptr<CtxBase> ctx[4];  // per width: 8, 16, 32, 64
+  template <class T> Ctx<T>& get() {
+    const int i = T::K == 8 ? 0 : T::K == 16 ? 1 : T::K == 32 ? 2 : 3;
+    if (!ctx[i]) ctx[i].reset(new Ctx<T>());
+    return static_cast<Ctx<T>&>(*ctx[i]);
+  }
+};
+
+struct Server {
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Job*> queue;
+  std::unordered_map<uint64_t, std::unique_ptr<Job>> jobs;
+  std::vector<Pinned> spare;  // proof buffers returned by released jobs
+  std::vector<std::unique_ptr<Worker>> workers;
+  std::vector<std::thread> threads;
+  size_t budget = 0, in_use = 0;
+  int running = 0;
+  uint64_t next_id = 1;
+  bool stop = false;
+  int device = 0;
+
+  Server(int nworkers, size_t budget_bytes) {
+    HIP_CHECK(hipGetDevice(&device));
+    if (budget_bytes == 0) {
+      size_t fr = 0, tot = 0;
+      HIP_CHECK(hipMemGetInfo(&fr, &tot));
+      budget_bytes = fr / 10 * 8;
+    }
+    budget = budget_bytes;
+    for (int i = 0; i < nworkers; i++) workers.emplace_back(new Worker());
+    for (int i = 0; i < nworkers; i++) threads.emplace_back([this, i] { loop(i); });
+  }
+  ~Server() {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      stop = true;
+    }
+    cv_work.notify_all();
+    for (auto& t : threads) t.join();
+    workers.clear();
+  }
+
+  uint64_t submit(std::unique_ptr<Job> j) {
+    std::lock_guard<std::mutex> g(mu);
+    j->id = next_id++;
+    j->fp = j->params->footprint;
+    j->t_submit = now_ms();
+    Job* raw = j.get();
+    jobs[raw->id] = std::move(j);
+    queue.push_back(raw);
+    cv_work.notify_all();
+    return raw->id;
+  }
+  Job* find(uint64_t id) {
+    auto it = jobs.find(id);
+    if (it == jobs.end()) throw Error(NENYA_E_ARG, "unknown job");
+    return it->second.get();
+  }
+  Job* wait(uint64_t id) {
+    std::unique_lock<std::mutex> g(mu);
+    Job* j = find(id);
+    cv_done.wait(g, [&] { return j->done; });
+    return j;
+  }
+  bool poll(uint64_t id) {
+    std::lock_guard<std::mutex> g(mu);
+    return find(id)->done;
+  }
+  void release(uint64_t id) {
+    std::lock_guard<std::mutex> g(mu);
+    Job* j = find(id);
+    if (!j->done) throw Error(NENYA_E_ARG, "job still running");
+    if (j->proof.p) spare.push_back(std::move(j->proof));
+    jobs.erase(id);
+  }
+
+  template <class T> void run_t(Worker& wk, Job& j) {
+    Ctx<T>& cx = wk.get<T>();
+    ProofView v;
+    if (j.kind == 1) {
+      auto* gp = static_cast<GeneralParams<T>*>(j.params);
+      v = gp->prove_host(cx, (const typename T::L*)j.w, j.n);
+    } else {
+      auto* sp = static_cast<SimdParams<T>*>(j.params);
+      v = sp->prove(cx, j.m3[0], j.m3[1], j.m3[2], (const typename T::L*)j.w);
+    }
+    j.len = v.second;
+    // the proof was assembled in cx.proofbuf: hand that buffer to the job
+    std::swap(j.proof, cx.proofbuf);
+  }
+
+  void loop(int wi) {
+    (void)hipSetDevice(device);
+    Worker& wk = *workers[wi];
+    for (;;) {
+      Job* j = nullptr;
+      {
+        std::unique_lock<std::mutex> g(mu);
+        cv_work.wait(g, [&] {
+          return stop || (!queue.empty() && (running == 0 || in_use + queue.front()->fp <= budget));
+        });
+        if (stop) return;
+        j = queue.front();
+        queue.pop_front();
+        in_use += j->fp;
+        running++;
+        j->t_start = now_ms();
+        // a recycled pinned buffer becomes the context's proof buffer
+        if (!spare.empty()) {
+          j->proof = std::move(spare.back());
+          spare.pop_back();
+        }
+      }
+      int rc = NENYA_OK;
+      std::string err;
+      try {
+        switch (j->params->k) {
+          case 8: swap_in<W<8>>(wk, *j); run_t<W<8>>(wk, *j); break;
+          case 16: swap_in<W<16>>(wk, *j); run_t<W<16>>(wk, *j); break;
+          case 32: swap_in<W<32>>(wk, *j); run_t<W<32>>(wk, *j); break;
+          default: swap_in<W<64>>(wk, *j); run_t<W<64>>(wk, *j); break;
+        }
+      } catch (const Error& e) {
+        rc = e.code;
+        err = e.what();
+      } catch (const std::exception& e) {
+        rc = NENYA_E_DEVICE;
+        err = e.what();
+      }
+      {
+        std::lock_guard<std::mutex> g(mu);
+        in_use -= j->fp;
+        running--;
+        j->rc = rc;
+        j->err = err;
+        j->t_end = now_ms();
+        j->done = true;
+      }
+      cv_done.notify_all();
+      cv_work.notify_all();
+    }
+  }
+  // the job's (recycled) buffer becomes the context's proof buffer
+  template <class T> void swap_in(Worker& wk, Job& j) { std::swap(j.proof, wk.get<T>().proofbuf); }
+};
+
+}  // namespace nenya

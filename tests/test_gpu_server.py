@@ -1,0 +1,65 @@
+"""GPU: proofs made through the batch/serve entry points (several worker
+streams, mixed widths and models in one queue) are byte-identical to the
+direct path's, and the oracle verifier accepts them."""
+import numpy as np
+import pytest
+
+import nenya_amd as na
+from nenya_amd.instances import general_instance, simd_instance
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mixed_queue_matches_direct_path(orc):
+    print("server: 3 workers, 8 mixed jobs", flush=True)
+    jobs = []
+    specs = [("g", 32, 10), ("s", 8, 6), ("g", 16, 10), ("s", 64, 5), ("g", 8, 11), ("s", 32, 7), ("g", 64, 10),
+             ("g", 32, 11)]
+    for i, (model, k, log) in enumerate(specs):
+        if model == "g":
+            A, B, Cm, w = general_instance(log, k, 40 + i)
+            p = na.GeneralSnarkParams.secure(log - 5, (A, B, Cm), 0xDEAD, 0xCAFE, k=k)
+            jobs.append(("g", k, log, p, (A, B, Cm), w))
+        else:
+            A, B, Cm, wit = simd_instance(log, k, 40 + i)
+            p = na.SimdSnarkParams.secure(log, log, 0xDEAD, 0xCAFE, k=k)
+            jobs.append(("s", k, log, p, (A, B, Cm), wit))
+    srv = na.Server(workers=3)
+    ids = []
+    for model, k, log, p, m3, w in jobs:
+        if model == "g":
+            ids.append(srv.submit_general(p, w))
+        else:
+            ids.append(srv.submit_simd(na.SimdR1csInstance(*m3, w), p))
+    for jid, (model, k, log, p, m3, w) in zip(ids, jobs):
+        got, queued, run = srv.wait(jid)
+        assert run > 0 and queued >= 0
+        if model == "g":
+            direct = na.prove_general(w, p)
+            assert got == direct, f"general k={k} log={log}"
+            assert orc.general_verify(k, log - 5, *m3, got) == "Ok"
+        else:
+            direct = na.prove_simd(na.SimdR1csInstance(*m3, w), p)
+            assert got == direct, f"simd k={k} log={log}"
+            assert orc.simd_verify(k, log, log, *m3, got) == "Ok"
+    srv.close()
+
+
+def test_concurrent_small_folds_match_direct_path():
+    """Twelve proofs over two shared params objects on four workers.  The
+    last sumcheck rounds fold a few tiles in place with four waves per tile;
+    kernels of the other workers share the compute units meanwhile, so the
+    waves of a block run out of step (an in-place fold must not store before
+    every wave of its block holds its operands)."""
+    print("server: 4 workers, 12 jobs on two shared params", flush=True)
+    cases = []
+    for i, (k, log) in enumerate([(8, 12), (32, 11)]):
+        A, B, Cm, w = general_instance(log, k, 70 + i)
+        p = na.GeneralSnarkParams.secure(log - 5, (A, B, Cm), 0xDEAD, 0xCAFE, k=k)
+        cases.append((k, log, p, w, na.prove_general(w, p)))
+    srv = na.Server(workers=4)
+    ids = [(c, srv.submit_general(c[2], c[3])) for _ in range(6) for c in cases]
+    for (k, log, p, w, direct), jid in ids:
+        got, _, _ = srv.wait(jid)
+        assert got == direct, f"general k={k} log={log}"
+    srv.close()
