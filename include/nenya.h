@@ -52,6 +52,23 @@ int nenya_prove_simd(const nenya_params* p,
                      const uint32_t* c_rows, const uint32_t* c_cols, const void* c_vals, size_t c_nnz,
                      const void* witnesses, uint8_t** proof, size_t* proof_len);
 
+/* MimdSnarkParams::secure(log_n, log_m, pack_mimd_matrices(a, b, c), seed_w,
+ * seed_e) (snark-ring/src/mimd.rs:84-107, reduction/src/reduce.rs:249-273):
+ * 32 independent n x m instances (n = 2^log_n, m = 2^log_m).  Each matrix
+ * set is the 32 instance matrices in COO concatenated, instance s holding
+ * entries off[s] .. off[s+1]-1 (33 offsets); the union-support packing is
+ * done here, once, on setup. */
+int nenya_params_mimd_secure(int k, int log_n, int log_m,
+                             const size_t* a_off, const uint32_t* a_rows, const uint32_t* a_cols, const void* a_vals,
+                             const size_t* b_off, const uint32_t* b_rows, const uint32_t* b_cols, const void* b_vals,
+                             const size_t* c_off, const uint32_t* c_rows, const uint32_t* c_cols, const void* c_vals,
+                             uint64_t seed_w, uint64_t seed_e, nenya_params** out);
+
+/* prove_mimd(&witnesses, &params) (snark-ring/src/mimd.rs:235-240):
+ * `witnesses` holds the 32 witnesses back to back (32 * m limbs).  Returns
+ * MimdSnarkProof::serialize. */
+int nenya_prove_mimd(const nenya_params* p, const void* witnesses, uint8_t** proof, size_t* proof_len);
+
 /* GeneralSnarkParams::secure(log_blocks, pack_general_matrices(a, b, c),
  * seed_w, seed_e) (snark-ring/src/general.rs:69-95,
  * reduction/src/reduce.rs:618-651): the cyclic-diagonal packing of the
@@ -99,7 +116,8 @@ int nenya_kernel_stats(const nenya_params* p, int cap, char* names, size_t names
                        double* bytes, double* macs);
 
 /* ---- Batched and served proofs (no reference counterpart: the reference
- * proves one instance per call, general.rs:367-372, simd.rs:212-217).
+ * proves one instance per call, general.rs:367-372, simd.rs:212-217,
+ * mimd.rs:235-240).
  * A server owns `workers` host threads, each with its own HIP stream, device
  * pool and pinned proof buffers, so independent proofs (any params, any
  * width) overlap on one GPU.  Jobs start in submission order, each once its
@@ -113,6 +131,7 @@ int nenya_submit_general(nenya_server* s, const nenya_params* p, const void* w, 
 int nenya_submit_simd(nenya_server* s, const nenya_params* p, const uint32_t* ar, const uint32_t* ac, const void* av,
                       size_t an, const uint32_t* br, const uint32_t* bc, const void* bv, size_t bn, const uint32_t* cr,
                       const uint32_t* cc, const void* cv, size_t cn, const void* witnesses, uint64_t* job);
+int nenya_submit_mimd(nenya_server* s, const nenya_params* p, const void* witnesses, uint64_t* job);
 /* 1 = done, 0 = queued or running, < 0 = -(error code) of the call itself */
 int nenya_poll(nenya_server* s, uint64_t job);
 int nenya_wait(nenya_server* s, uint64_t job, const uint8_t** proof, size_t* proof_len, double* queued_ms,

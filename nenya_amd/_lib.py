@@ -33,6 +33,10 @@ def _lib():
         L.nenya_params_simd_secure.argtypes = [C.c_int, C.c_int, C.c_int, u64, u64, C.POINTER(vp)]
         L.nenya_prove_simd.argtypes = [vp] + coo * 3 + [vp, C.POINTER(vp), C.POINTER(sz)]
         L.nenya_params_general_secure.argtypes = [C.c_int, C.c_int] + coo * 3 + [u64, u64, C.POINTER(vp)]
+        mset = [vp, vp, vp, vp]
+        L.nenya_params_mimd_secure.argtypes = [C.c_int, C.c_int, C.c_int] + mset * 3 + [u64, u64, C.POINTER(vp)]
+        L.nenya_prove_mimd.argtypes = [vp, vp, C.POINTER(vp), C.POINTER(sz)]
+        L.nenya_submit_mimd.argtypes = [vp, vp, vp, C.POINTER(u64)]
         L.nenya_prove_general.argtypes = [vp, vp, sz, C.POINTER(vp), C.POINTER(sz)]
         L.nenya_prove_general_device.argtypes = [vp, vp, sz, C.POINTER(vp), C.POINTER(sz)]
         L.nenya_prove_general_device_view.argtypes = [vp, vp, sz, C.POINTER(vp), C.POINTER(sz)]
@@ -163,6 +167,51 @@ def prove_simd(inst, params):
     return _take_proof(out, n)
 
 
+def _mimd_set(ms, k):
+    """32 COO matrices -> (offsets[33] as size_t, rows, cols, vals) arrays."""
+    if len(ms) != 32:
+        raise NenyaError(4, "a MIMD matrix set holds 32 instance matrices")
+    mats = [_coo(m, k) for m in ms]
+    off = np.zeros(33, np.uintp)
+    off[1:] = np.cumsum([len(m[0]) for m in mats])
+    return (off, np.ascontiguousarray(np.concatenate([m[0] for m in mats]), np.uint32),
+            np.ascontiguousarray(np.concatenate([m[1] for m in mats]), np.uint32),
+            np.ascontiguousarray(np.concatenate([m[2] for m in mats]).astype(_DT[k])))
+
+
+class MimdSnarkParams(_Params):
+    """MimdSnarkParams::secure(log_n, log_m, pack_mimd_matrices(a, b, c), seed_w, seed_e)
+    (snark-ring/src/mimd.rs:84-107, reduction/src/reduce.rs:249-273).
+    `matrices` = (As, Bs, Cs), each a list of the 32 instances' COO triples."""
+
+    @classmethod
+    def secure(cls, log_n, log_m, matrices, seed_w=0xDEAD, seed_e=0xCAFE, k=32):
+        sets = [_mimd_set(ms, k) for ms in matrices]
+        args = []
+        for st in sets:
+            args += [a.ctypes.data for a in st]
+        h = C.c_void_p()
+        _check(_lib().nenya_params_mimd_secure(k, log_n, log_m, *args, seed_w, seed_e, C.byref(h)))
+        p = cls(h, k)
+        p.log_n, p.log_m = log_n, log_m
+        return p
+
+
+def _mimd_witnesses(witnesses, params):
+    wit = np.ascontiguousarray(np.asarray(witnesses).astype(_DT[params.k]))
+    if wit.shape != (32, 1 << params.log_m):
+        raise NenyaError(1, f"witnesses must have shape (32, {1 << params.log_m})")
+    return wit
+
+
+def prove_mimd(witnesses, params):
+    """prove_mimd(&[Vec<Gr1>; 32], &MimdSnarkParams) -> proof bytes."""
+    wit = _mimd_witnesses(witnesses, params)
+    out, n = C.c_void_p(), C.c_size_t()
+    _check(_lib().nenya_prove_mimd(params._h, wit.ctypes.data, C.byref(out), C.byref(n)))
+    return _take_proof(out, n)
+
+
 class GeneralSnarkParams(_Params):
     """GeneralSnarkParams::secure(log_blocks, pack_general_matrices(a,b,c), seed_w, seed_e)
     (snark-ring/src/general.rs:69-95)."""
@@ -233,6 +282,13 @@ class Server:
         j = C.c_uint64()
         _check(_lib().nenya_submit_simd(*args, wit.ctypes.data, C.byref(j)))
         self._keep[j.value] = (params, mats, wit)
+        return j.value
+
+    def submit_mimd(self, params, witnesses):
+        wit = _mimd_witnesses(witnesses, params)
+        j = C.c_uint64()
+        _check(_lib().nenya_submit_mimd(self._h, params._h, wit.ctypes.data, C.byref(j)))
+        self._keep[j.value] = (params, wit)
         return j.value
 
     def poll(self, job):

@@ -55,6 +55,11 @@ static ProofView simd_prove_t(ParamsBase* b, const HostCoo* m3, const void* wit)
   return sp->prove(sp->cx0, m3[0], m3[1], m3[2], (const typename T::L*)wit);
 }
 
+template <class T> static ProofView mimd_t(ParamsBase* b, const void* wit) {
+  auto* mp = static_cast<MimdParams<T>*>(b);
+  return mp->prove(mp->cx0, (const typename T::L*)wit);
+}
+
 template <class T> static ProofView general_t(ParamsBase* b, const void* w, size_t n, bool dev) {
   auto* gp = static_cast<GeneralParams<T>*>(b);
   if (dev) {
@@ -159,6 +164,59 @@ int nenya_prove_simd(const nenya_params* p, const uint32_t* ar, const uint32_t* 
       default: v = simd_prove_t<W<64>>(b, m3, witnesses); break;
     }
     emit(v, proof, proof_len);
+    return NENYA_OK;
+  });
+}
+
+int nenya_params_mimd_secure(int k, int log_n, int log_m, const size_t* a_off, const uint32_t* ar, const uint32_t* ac,
+                             const void* av, const size_t* b_off, const uint32_t* br, const uint32_t* bc, const void* bv,
+                             const size_t* c_off, const uint32_t* cr, const uint32_t* cc, const void* cv,
+                             uint64_t seed_w, uint64_t seed_e, nenya_params** out) {
+  return guard([&] {
+    if (!out || !a_off || !b_off || !c_off) throw Error(NENYA_E_ARG, "null argument");
+    need_width(k);
+    if (log_n < 1 || log_n > 26 || log_m < 1 || log_m > 26) throw Error(NENYA_E_SHAPE, "log_n / log_m out of range");
+    const size_t* offs[3] = {a_off, b_off, c_off};
+    const uint32_t* rs[3] = {ar, br, cr};
+    const uint32_t* cs[3] = {ac, bc, cc};
+    const void* vs[3] = {av, bv, cv};
+    MimdSet sets[3];
+    for (int w = 0; w < 3; w++) {
+      if (offs[w][0] != 0) throw Error(NENYA_E_ARG, "instance offsets must start at 0");
+      if (offs[w][32] && (!rs[w] || !cs[w] || !vs[w])) throw Error(NENYA_E_ARG, "null matrix array");
+      sets[w] = {offs[w], rs[w], cs[w], vs[w]};
+    }
+    need_gpu();
+    auto* p = new nenya_params;
+    try {
+      switch (k) {
+        case 8: p->impl = new MimdParams<W<8>>(log_n, log_m, sets, seed_w, seed_e); break;
+        case 16: p->impl = new MimdParams<W<16>>(log_n, log_m, sets, seed_w, seed_e); break;
+        case 32: p->impl = new MimdParams<W<32>>(log_n, log_m, sets, seed_w, seed_e); break;
+        default: p->impl = new MimdParams<W<64>>(log_n, log_m, sets, seed_w, seed_e); break;
+      }
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *out = p;
+    return NENYA_OK;
+  });
+}
+
+int nenya_prove_mimd(const nenya_params* p, const void* witnesses, uint8_t** proof, size_t* len) {
+  return guard([&] {
+    if (!p || !p->impl || p->impl->kind != 2 || !witnesses || !proof || !len) throw Error(NENYA_E_ARG, "bad argument");
+    auto* np = const_cast<nenya_params*>(p);
+    std::lock_guard<std::mutex> g(np->mu);
+    ProofView v;
+    switch (p->impl->k) {
+      case 8: v = mimd_t<W<8>>(p->impl, witnesses); break;
+      case 16: v = mimd_t<W<16>>(p->impl, witnesses); break;
+      case 32: v = mimd_t<W<32>>(p->impl, witnesses); break;
+      default: v = mimd_t<W<64>>(p->impl, witnesses); break;
+    }
+    emit(v, proof, len);
     return NENYA_OK;
   });
 }
@@ -331,6 +389,19 @@ int nenya_poll(nenya_server* s, uint64_t job) {
     return NENYA_OK;
   });
   return rc == NENYA_OK ? done : -rc;
+}
+
+int nenya_submit_mimd(nenya_server* s, const nenya_params* p, const void* witnesses, uint64_t* job) {
+  return guard([&] {
+    if (!s || !s->impl || !p || !p->impl || p->impl->kind != 2 || !witnesses || !job)
+      throw Error(NENYA_E_ARG, "bad argument");
+    std::unique_ptr<Job> j(new Job);
+    j->params = p->impl;
+    j->kind = 2;
+    j->w = witnesses;
+    *job = s->impl->submit(std::move(j));
+    return NENYA_OK;
+  });
 }
 
 int nenya_wait(nenya_server* s, uint64_t job, const uint8_t** proof, size_t* proof_len, double* queued_ms,

@@ -774,6 +774,21 @@ __global__ void k_r0_matvec(const uint32_t* ptr, const uint32_t* col, const LT<T
   out[r * ostride + v * ovstride] = (LT<T>)acc;
 }
 
+// The 32 per-instance R0 matvecs of a MIMD matrix set over its union
+// support (reduce.rs:194-201): out[r][s] = sum_{cells g of row r}
+// val[g][s] * w[col[g]][s], i.e. (A_s w_s)[r] in slot layout [row][32].
+template <class T>
+__global__ void k_mimd_slots(const uint64_t* ptr, const uint32_t* col, const LT<T>* val, const LT<T>* w, LT<T>* out,
+                             uint64_t nrows) {
+  for (uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; id < nrows * 32;
+       id += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = id >> 5, s = id & 31;
+    CT<T> acc = 0;
+    for (uint64_t g = ptr[r]; g < ptr[r + 1]; g++) acc += (CT<T>)val[g * 32 + s] * (CT<T>)w[(uint64_t)col[g] * 32 + s];
+    out[id] = (LT<T>)acc;
+  }
+}
+
 // out[o] (+)= sum_{k in ptr[o]..ptr[o+1]} val[k] * X[idx[k]]   (R0 scalar x R2,
 // the mul_by_gr1 matvecs of matrix.rs:93-105 and simd.rs:570-580)
 template <class T>

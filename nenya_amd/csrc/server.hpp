@@ -1,5 +1,6 @@
 // Batched and served proofs on one GPU.  The reference proves one instance
-// per call (snark-ring/src/general.rs:367-372, simd.rs:212-217); here
+// per call (snark-ring/src/general.rs:367-372, simd.rs:212-217,
+// mimd.rs:235-240); here
 // independent proofs are queued and drained by worker threads, each with its
 // own execution contexts (HIP stream, stream-ordered pool, pinned proof
 // buffer) per limb width, so the latency-bound phases of one proof (sumcheck
@@ -25,8 +26,8 @@ inline double now_ms() {
 struct Job {
   uint64_t id = 0;
   ParamsBase* params = nullptr;
-  int kind = 1;              // 0 simd, 1 general
-  const void* w = nullptr;   // general: witness (host); simd: 32 x m witnesses
+  int kind = 1;              // 0 simd, 1 general, 2 mimd
+  const void* w = nullptr;   // general: witness (host); simd, mimd: 32 x m witnesses
   size_t n = 0;
   HostCoo m3[3];             // simd matrices (borrowed until the job ends)
   Pinned proof;
@@ -122,6 +123,9 @@ struct Server {
     if (j.kind == 1) {
       auto* gp = static_cast<GeneralParams<T>*>(j.params);
       v = gp->prove_host(cx, (const typename T::L*)j.w, j.n);
+    } else if (j.kind == 2) {
+      auto* mp = static_cast<MimdParams<T>*>(j.params);
+      v = mp->prove(cx, (const typename T::L*)j.w);
     } else {
       auto* sp = static_cast<SimdParams<T>*>(j.params);
       v = sp->prove(cx, j.m3[0], j.m3[1], j.m3[2], (const typename T::L*)j.w);

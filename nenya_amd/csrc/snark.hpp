@@ -48,7 +48,7 @@ inline bool skip_selfcheck() {
 
 struct ParamsBase {
   int k = 32;
-  int kind = 0;  // 0 simd, 1 general
+  int kind = 0;  // 0 simd, 1 general, 2 mimd
   // device bytes of one proof: an estimate until a proof has run, then the
   // largest measured pool growth of a proof
   std::atomic<size_t> footprint{0};
@@ -102,6 +102,25 @@ template <class T> struct Csr {
     e.sync();
   }
 };
+
+// Uploads a CSR of (dk, sel, idx) entries (the k_spmv_gr layout).
+template <class T>
+void upload_csr(Engine<T>& eng, Pool& pool, const std::vector<uint64_t>& ptr, const std::vector<uint32_t>& dk,
+                const std::vector<uint32_t>& sel, const std::vector<uint32_t>& idx, DBuf<uint64_t>& dp,
+                DBuf<uint32_t>& ddk, DBuf<uint32_t>& dsel, DBuf<uint32_t>& didx) {
+  dp = DBuf<uint64_t>(&pool, ptr.size());
+  size_t nn = std::max<size_t>(dk.size(), 1);
+  ddk = DBuf<uint32_t>(&pool, nn);
+  dsel = DBuf<uint32_t>(&pool, nn);
+  didx = DBuf<uint32_t>(&pool, nn);
+  eng.h2d(dp.p, ptr.data(), ptr.size() * 8);
+  if (!dk.empty()) {
+    eng.h2d(ddk.p, dk.data(), dk.size() * 4);
+    eng.h2d(dsel.p, sel.data(), sel.size() * 4);
+    eng.h2d(didx.p, idx.data(), idx.size() * 4);
+  }
+  eng.sync();
+}
 
 inline void check_coo(const HostCoo& m, size_t rows, size_t cols) {
   if (m.rows != rows || m.cols != cols) throw Error(NENYA_E_SHAPE, "matrix shape mismatch");
@@ -279,6 +298,273 @@ template <class T> struct SimdParams : ParamsBase {
 };
 
 // ============================================================================
+// MIMD (mimd.rs:242-397)
+// ============================================================================
+// One MIMD matrix set: the 32 instance matrices concatenated, instance s at
+// entries off[s] .. off[s+1]-1 (values: limbs of the native width).
+struct MimdSet {
+  const size_t* off;
+  const uint32_t* r;
+  const uint32_t* c;
+  const void* v;
+};
+
+template <class T> struct MimdParams : ParamsBase {
+  using L = typename T::L;
+  using G = Gr<T>;
+  int log_n = 0, log_m = 0;
+  size_t n = 0, m = 0;
+  Ctx<T> cx0;
+  PcsDev<T> pw, pabc, pe;
+  // union-support cells of A, B, C (pack_matrix_set, reduce.rs:119-141),
+  // concatenated: 32 instance values and the packed entry per cell
+  size_t ncell[3] = {0, 0, 0};
+  DBuf<uint64_t> cptr;  // 3 x (n+1) row pointers (global cell indices)
+  DBuf<uint32_t> ccol;
+  DBuf<L> cval, dhat;
+  // matvec entries by row (k_spmv_gr layout) and M_gamma-row entries by column
+  DBuf<uint64_t> mvptr, rwptr;
+  DBuf<uint32_t> mvdk, mvsel, mvidx, rwdk, rwsel, rwidx;
+  CtxBase& ctx0() override { return cx0; }
+
+  // MimdSnarkParams::secure (mimd.rs:84-107) with pack_mimd_matrices
+  // (reduce.rs:249-273) done once here (setup, not prover time).
+  MimdParams(int ln, int lm, const MimdSet* sets, uint64_t seed_w, uint64_t seed_e) {
+    k = T::K;
+    kind = 2;
+    log_n = ln;
+    log_m = lm;
+    n = size_t(1) << ln;
+    m = size_t(1) << lm;
+    Pool& pool = cx0.pool;
+    Engine<T>* eng = cx0.eng.get();
+    pw.P = PcsParams::setup(lm, seed_w, T::K);
+    pabc.P = PcsParams::setup_stacked(ln, seed_w ^ 0xABC, 3, T::K);
+    pe.P = PcsParams::setup_r2(ln, seed_e, T::K);
+    pw.upload(*eng);
+    pabc.upload(*eng);
+    pe.upload(*eng);
+    std::vector<uint64_t> rowptr(3 * (n + 1), 0);
+    std::vector<uint32_t> col, cellrow;
+    std::vector<L> vals;
+    size_t base = 0;
+    for (int w = 0; w < 3; w++) {
+      const MimdSet& S = sets[w];
+      const L* sv = (const L*)S.v;
+      const size_t tot = S.off[32];
+      struct Item { uint64_t key; uint32_t s; L v; };
+      std::vector<Item> it;
+      it.reserve(tot);
+      for (int inst = 0; inst < 32; inst++) {
+        if (S.off[inst + 1] < S.off[inst] || S.off[inst + 1] > tot) throw Error(NENYA_E_ARG, "bad instance offsets");
+        for (size_t j = S.off[inst]; j < S.off[inst + 1]; j++) {
+          if (S.r[j] >= n || S.c[j] >= m) throw Error(NENYA_E_SHAPE, "matrix entry out of range");
+          it.push_back({(uint64_t)S.r[j] * m + S.c[j], (uint32_t)inst, sv[j]});
+        }
+      }
+      std::stable_sort(it.begin(), it.end(), [](const Item& a, const Item& b) { return a.key < b.key; });
+      uint64_t* rp = &rowptr[w * (n + 1)];
+      for (size_t i = 0; i < it.size(); i++) {
+        if (i == 0 || it[i].key != it[i - 1].key) {
+          const uint64_t r = it[i].key / m;
+          col.push_back((uint32_t)(it[i].key % m));
+          cellrow.push_back((uint32_t)r);
+          vals.resize(vals.size() + 32, (L)0);
+          rp[r + 1]++;
+        }
+        L& x = vals[vals.size() - 32 + it[i].s];
+        x = (L)(x + it[i].v);
+      }
+      ncell[w] = col.size() - base;
+      rp[0] = base;
+      for (size_t r = 0; r < n; r++) rp[r + 1] += rp[r];
+      base = col.size();
+    }
+    const size_t total = base;
+    if (total >= (size_t(1) << 32)) throw Error(NENYA_E_SHAPE, "too many matrix cells");
+    cptr = DBuf<uint64_t>(&pool, rowptr.size());
+    ccol = DBuf<uint32_t>(&pool, std::max<size_t>(total, 1));
+    cval = DBuf<L>(&pool, std::max<size_t>(total, 1) * 32);
+    dhat = DBuf<L>(&pool, std::max<size_t>(total, 1) * E);
+    eng->h2d(cptr.p, rowptr.data(), rowptr.size() * 8);
+    if (total) {
+      eng->h2d(ccol.p, col.data(), total * 4);
+      eng->h2d(cval.p, vals.data(), vals.size() * sizeof(L));
+      eng->pack(cval.p, dhat.p, total, 1);
+    }
+    // matvec entries: row r of matrix w sums dhat[g] * w_phi[col[g]]
+    {
+      std::vector<uint64_t> ptr(rowptr);
+      std::vector<uint32_t> dk(total), sel(total, 0), idx(total);
+      for (size_t g = 0; g < total; g++) {
+        dk[g] = (uint32_t)g;
+        idx[g] = col[g];
+      }
+      upload_csr(*eng, pool, ptr, dk, sel, idx, mvptr, mvdk, mvsel, mvidx);
+    }
+    // M_gamma row entries by column: dhat[g] * eqs[w][row(g)] (mimd.rs:196-219)
+    {
+      std::vector<uint64_t> ptr(m + 1, 0);
+      for (size_t g = 0; g < total; g++) ptr[col[g] + 1]++;
+      for (size_t h = 0; h < m; h++) ptr[h + 1] += ptr[h];
+      std::vector<uint64_t> pos(ptr.begin(), ptr.end() - 1);
+      std::vector<uint32_t> dk(total), sel(total), idx(total);
+      size_t g = 0;
+      for (int w = 0; w < 3; w++)
+        for (size_t i = 0; i < ncell[w]; i++, g++) {
+          const uint64_t o = pos[col[g]]++;
+          dk[o] = (uint32_t)g;
+          sel[o] = (uint32_t)w;
+          idx[o] = cellrow[g];
+        }
+      upload_csr(*eng, pool, ptr, dk, sel, idx, rwptr, rwdk, rwsel, rwidx);
+    }
+    // setup data is read by other streams (server workers): finish it here
+    eng->sync();
+    footprint = (size_t)((24.0 * E + 3 * 32) * sizeof(L) * n + (8.0 * E + 64) * sizeof(L) * m);
+  }
+  ~MimdParams() override {
+    pw = PcsDev<T>();
+    pabc = PcsDev<T>();
+    pe = PcsDev<T>();
+    cptr.release(); ccol.release(); cval.release(); dhat.release();
+    mvptr.release(); mvdk.release(); mvsel.release(); mvidx.release();
+    rwptr.release(); rwdk.release(); rwsel.release(); rwidx.release();
+  }
+
+  ProofView prove(Ctx<T>& cx, const L* wit /* 32 x m */) {
+    Engine<T>& e = *cx.eng;
+    Pool& pool = cx.pool;
+    const size_t pool_base = pool.mark();
+    KProf& prof = cx.prof;
+    e.prof = &prof;
+    Stages stg;
+    stg.on = cx.timing;
+    stg.st = cx.st;
+    stg.start();
+    // witness slots [m][32] (the slot-form commitment message)
+    DBuf<L> slots(&pool, m * 32);
+    {
+      std::vector<L> tr(m * 32);
+      for (size_t j = 0; j < m; j++)
+        for (int s = 0; s < 32; s++) tr[j * 32 + s] = wit[s * m + j];
+      e.h2d(slots.p, tr.data(), tr.size() * sizeof(L));
+    }
+    stg.mark("upload");
+    // 1. w_phi = pack_r2(witness column j) (reduce.rs:294-312)
+    DBuf<L> wphi(&pool, m * E);
+    e.pack(slots.p, wphi.p, m, 1);
+    // 2. a_hat, b_hat, c_hat.  With Ahat w_phi = phi(A_s w_s) + q1 v_alpha +
+    //    q2 v_beta (the quotient identity summed over a row, reduce.rs:172-192),
+    //    a_hat = Ahat w_phi - e_A = pack_r2((A_s w_s)_s) exactly, and for C,
+    //    pack_r2(psi(Chat w_phi)) = pack_r2((C_s w_s)_s) (mimd.rs:276-287).
+    DBuf<L> abc_slots(&pool, 3 * n * 32);
+    for (int w = 0; w < 3; w++)
+      e.launch("k_mimd_slots", 0, 0, (k_mimd_slots<T>), dim3(grid_for(n * 32, 256, 1u << 16)), dim3(256),
+               (const uint64_t*)(cptr.p + w * (n + 1)), (const uint32_t*)ccol.p, (const L*)cval.p,
+               (const L*)slots.p, abc_slots.p + w * n * 32, (uint64_t)n);
+    DBuf<L> hats(&pool, 4 * n * E);  // a_hat, b_hat, c_hat, e_mult
+    e.pack(abc_slots.p, hats.p, 3 * n, 1);
+    L* emult = hats.p + 3 * n * E;
+    e.template mul_sub<16>(hats.p, hats.p + n * E, hats.p + 2 * n * E, emult, n);
+    // 3. Ahat w_phi, Bhat w_phi, Chat w_phi (matvec_gr128_dense_csr) and the
+    //    errors e_X = Xhat w_phi - x_hat
+    DBuf<L> estack(&pool, 3 * n * E);
+    for (int w = 0; w < 3; w++)
+      e.launch("k_spmv_gr", (double)ncell[w] * 2 * Engine<T>::EB, (double)ncell[w] * 16 * Engine<T>::PM,
+               (k_spmv_gr<T>), dim3(grid_for(n, GPB)), dim3(GB), (const uint64_t*)(mvptr.p + w * (n + 1)),
+               (const uint32_t*)mvdk.p, (const uint32_t*)mvsel.p, (const uint32_t*)mvidx.p, (const L*)dhat.p,
+               (const L*)wphi.p, (uint64_t)0, estack.p + w * n * E, (uint64_t)n);
+    e.sub(estack.p, hats.p, estack.p, 3 * n * E);
+    stg.mark("reduce_matvec");
+    // 4. commits (mimd.rs:302-316)
+    CommitDev<T> cw = e.commit(pw, slots.p, pw.P.k1, wphi.p);
+    CommitDev<T> cabc = e.commit(pabc, abc_slots.p, 3 * pabc.P.k1, hats.p);
+    CommitDev<T> cem = e.commit(pe, emult, pe.P.k1, emult);
+    CommitDev<T> ces = e.commit(pe, estack.p, 3 * pe.P.k1, estack.p);
+    stg.mark("commit");
+    Transcript tr("mimd");
+    tr.root(cw.root);
+    tr.root(cabc.root);
+    tr.root(cem.root);
+    tr.root(ces.root);
+    StdRng rs = tr.sampler("mult_check_target");
+    std::vector<G> rt = e.sample(rs, log_n);
+    // 5. mult check, sub form, on (a_hat, b_hat, c_hat, e_mult)
+    tr.label("mult_check_sumcheck");
+    const L* mt[4] = {hats.p, hats.p + n * E, hats.p + 2 * n * E, emult};
+    auto mp = e.mult_check(mt, 4, log_n, rt, tr);
+    tr.elems("mult/finals", mp.finals.data(), 4);
+    G gam = e.challenge(tr, "linear_check_gamma");
+    G gam2 = gam * gam;
+    stg.mark("mult_sumcheck");
+    // proof layout (mimd.rs:128-152): roots, mult, lin, open_w, open_emult,
+    // open_abc, open_estack -- sections sized up front
+    const size_t EBs = E * sizeof(L);
+    size_t s_mult = 8 + 3 * (size_t)log_n * EBs + 4 * EBs, s_lin = EBs + 8 + 3 * (size_t)log_m * EBs + 2 * EBs;
+    size_t s_w = opening_bytes(pw.P, pw.P.k1, sizeof(L), 0), s_em = opening_bytes(pe.P, pe.P.k1, sizeof(L), 0);
+    size_t s_abc = opening_bytes(pabc.P, 3 * pabc.P.k1, sizeof(L), 0), s_es = opening_bytes(pe.P, 3 * pe.P.k1, sizeof(L), 0);
+    size_t total = 128 + s_mult + s_lin + s_w + s_em + s_abc + s_es;
+    uint8_t* buf = cx.proofbuf.get(total);
+    uint8_t* o_w = buf + 128 + s_mult + s_lin;
+    uint8_t* o_em = o_w + s_w;
+    uint8_t* o_abc = o_em + s_em;
+    uint8_t* o_es = o_abc + s_abc;
+    // 6. openings at s1 (mimd.rs:340-352)
+    G wts[3] = {G::one(), gam, gam2};
+    Sink k_es(o_es, s_es), k_abc(o_abc, s_abc), k_em(o_em, s_em);
+    G estack_val = e.open(pe, ces, mp.chal.data(), wts, 3, k_es);
+    G abc_val = e.open(pabc, cabc, mp.chal.data(), wts, 3, k_abc);
+    G em_val = e.open(pe, cem, mp.chal.data(), nullptr, 1, k_em);
+    stg.mark("open_s1");
+    // 7. M_gamma row (mimd.rs:196-219, eq_s1 for all three matrices)
+    DBuf<L> eqs(&pool, 3 * n * E), mrow(&pool, m * E);
+    {
+      DBuf<L> eq1 = e.eq_table(mp.chal.data(), log_n);
+      e.d2d(eqs.p, eq1.p, n * E * sizeof(L));
+      DBuf<L> dg = e.put(&gam, 1), dg2 = e.put(&gam2, 1);
+      e.scale(dg.p, eq1.p, eqs.p + n * E, n);
+      e.scale(dg2.p, eq1.p, eqs.p + 2 * n * E, n);
+    }
+    const double ncl = (double)(ncell[0] + ncell[1] + ncell[2]);
+    e.launch("k_spmv_gr", ncl * 2 * Engine<T>::EB, ncl * 16 * Engine<T>::PM, (k_spmv_gr<T>), dim3(grid_for(m, GPB)),
+             dim3(GB), (const uint64_t*)rwptr.p, (const uint32_t*)rwdk.p, (const uint32_t*)rwsel.p,
+             (const uint32_t*)rwidx.p, (const L*)dhat.p, (const L*)eqs.p, (uint64_t)n, mrow.p, (uint64_t)m);
+    eqs.release();
+    stg.mark("linear_rows");
+    // 8. linear check on (M_gamma row, w_phi); w_phi is low-degree
+    G claim_lin = mp.finals[0] + gam * mp.finals[1] + gam2 * mp.finals[2] + estack_val;
+    tr.label("linear_check_sumcheck");
+    auto lp = e.template linear<16>(wphi.p, 0, mrow.p, 0, 1, log_m, claim_lin, tr);
+    G final_w = lp.finals[0], final_row = lp.finals[1];
+    stg.mark("linear_sumcheck");
+    // 9. open w at s2
+    Sink k_w(o_w, s_w);
+    G yw = e.open(pw, cw, lp.chal.data(), nullptr, 1, k_w);
+    stg.mark("open_w");
+    Sink hd(buf, 128 + s_mult + s_lin);
+    for (auto* r : {cw.root, cabc.root, cem.root, ces.root}) hd.bytes(r, 32);
+    hd.u64(mp.chal.size());
+    for (auto& g : mp.rounds) hd.gr(g);
+    for (int i = 0; i < 4; i++) hd.gr(mp.finals[i]);
+    hd.gr(claim_lin);
+    hd.u64(lp.chal.size());
+    for (auto& g : lp.rounds) hd.gr(g);
+    hd.gr(final_row);
+    hd.gr(final_w);
+    if (!hd.full() || !k_es.full() || !k_abc.full() || !k_em.full() || !k_w.full())
+      throw Error(NENYA_E_DEVICE, "proof size mismatch");
+    const bool ok = em_val == mp.finals[3] && abc_val == mp.finals[0] + gam * mp.finals[1] + gam2 * mp.finals[2] &&
+                    yw == final_w;
+    if (!ok && !skip_selfcheck()) throw Error(NENYA_E_DEVICE, "internal opening self-check failed");
+    cx.last_timing = stg.t;
+    if (prof.on) prof.collect();
+    note_footprint(pool.peak - pool_base);
+    return {buf, total};
+  }
+};
+
+// ============================================================================
 // General (general.rs:374-537)
 // ============================================================================
 template <class T> struct GeneralParams : ParamsBase {
@@ -377,7 +663,7 @@ template <class T> struct GeneralParams : ParamsBase {
           sel[o] = cl[w][i];
           idx[o] = cq[w][i];
         }
-        upload_csr(ptr, dk, sel, idx, lptr[w], ldk[w], lsel[w], lidx[w]);
+        upload_csr(*eng, pool, ptr, dk, sel, idx, lptr[w], ldk[w], lsel[w], lidx[w]);
       }
       // build_layer_rows CSR over (l, q) merging A, B, C (xsel = matrix id)
       {
@@ -394,7 +680,7 @@ template <class T> struct GeneralParams : ParamsBase {
             sel[o] = (uint32_t)w;
             idx[o] = cp[w][i];
           }
-        upload_csr(ptr, dk, sel, idx, rptr, rdk, rsel, ridx);
+        upload_csr(*eng, pool, ptr, dk, sel, idx, rptr, rdk, rsel, ridx);
       }
     }
     if (kMx<T>) {
@@ -428,23 +714,6 @@ template <class T> struct GeneralParams : ParamsBase {
       eng->h2d(ecode.p, code.data(), code.size() * 4);
       eng->h2d(eval.p, ev.data(), ev.size() * sizeof(L));
       eng->sync();
-    }
-    eng->sync();
-  }
-  void upload_csr(const std::vector<uint64_t>& ptr, const std::vector<uint32_t>& dk, const std::vector<uint32_t>& sel,
-                  const std::vector<uint32_t>& idx, DBuf<uint64_t>& dp, DBuf<uint32_t>& ddk, DBuf<uint32_t>& dsel,
-                  DBuf<uint32_t>& didx) {
-    Engine<T>* eng = cx0.eng.get();
-    dp = DBuf<uint64_t>(&cx0.pool, ptr.size());
-    size_t nn = std::max<size_t>(dk.size(), 1);
-    ddk = DBuf<uint32_t>(&cx0.pool, nn);
-    dsel = DBuf<uint32_t>(&cx0.pool, nn);
-    didx = DBuf<uint32_t>(&cx0.pool, nn);
-    eng->h2d(dp.p, ptr.data(), ptr.size() * 8);
-    if (!dk.empty()) {
-      eng->h2d(ddk.p, dk.data(), dk.size() * 4);
-      eng->h2d(dsel.p, sel.data(), sel.size() * 4);
-      eng->h2d(didx.p, idx.data(), idx.size() * 4);
     }
     eng->sync();
   }

@@ -104,3 +104,31 @@ def simd_instance(log, k, seed):
     A, B, Cm = random_matrices(n, k, seed)
     wit = np.stack([random_witness(n, k, A, B, seed, tag=1 + s) for s in range(32)])
     return A, B, Cm, wit
+
+
+def mimd_instance(log, k, seed, max_a=3):
+    """MIMD batch: 32 independent instances, each with its own n x n matrices
+    (n = 2^log) and satisfying witness.  Returns (As, Bs, Cs, wit[32, n])
+    with As[s] = (rows, cols, vals) of instance s."""
+    n = 1 << log
+    seeds = stream(seed, 300, 32)
+    As, Bs, Cs, ws = [], [], [], []
+    for s in range(32):
+        sd = int(seeds[s])
+        A, B, Cm = random_matrices(n, k, sd, max_a=max_a)
+        As.append(A)
+        Bs.append(B)
+        Cs.append(Cm)
+        ws.append(random_witness(n, k, A, B, sd))
+    return As, Bs, Cs, np.stack(ws)
+
+
+def set_arrays(ms):
+    """The 32 COO matrices of one MIMD set concatenated: (off[33], rows,
+    cols, vals) with instance s at entries off[s] .. off[s+1]-1."""
+    off = np.zeros(33, np.uint64)
+    off[1:] = np.cumsum([len(m[0]) for m in ms])
+    rows = np.ascontiguousarray(np.concatenate([m[0] for m in ms]), np.uint32)
+    cols = np.ascontiguousarray(np.concatenate([m[1] for m in ms]), np.uint32)
+    vals = np.ascontiguousarray(np.concatenate([m[2] for m in ms]), np.uint64)
+    return off, rows, cols, vals

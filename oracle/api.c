@@ -286,3 +286,57 @@ API int orc_general_verify(int k, int log_b,
   gdiag_free(&g);
   return rc;
 }
+
+/* MIMD: each matrix set is the 32 instance matrices concatenated, instance k
+ * holding entries off[k] .. off[k+1]-1. */
+static void mimd_sets(size_t n, size_t m, const size_t *off[3], const u32 *r[3], const u32 *c[3], const u64 *v[3],
+                      coo_t store[3][32], const coo_t *ptrs[3][32]) {
+  for (int w = 0; w < 3; w++)
+    for (int k = 0; k < 32; k++) {
+      store[w][k] = mkcoo(n, m, r[w] + off[w][k], c[w] + off[w][k], v[w] + off[w][k], off[w][k + 1] - off[w][k]);
+      ptrs[w][k] = &store[w][k];
+    }
+}
+API int orc_mimd_prove(int k, int log_n, int log_m,
+                       const size_t *aoff, const u32 *ar, const u32 *ac, const u64 *av,
+                       const size_t *boff, const u32 *br, const u32 *bc, const u64 *bv,
+                       const size_t *coff, const u32 *cr, const u32 *cc, const u64 *cv,
+                       const u64 *wit, u64 seed_w, u64 seed_e, u8 **out, size_t *len) {
+  or_set_width(k);
+  size_t n = (size_t)1 << log_n, m = (size_t)1 << log_m;
+  const size_t *off[3] = {aoff, boff, coff};
+  const u32 *r[3] = {ar, br, cr}, *c[3] = {ac, bc, cc};
+  const u64 *v[3] = {av, bv, cv};
+  coo_t store[3][32];
+  const coo_t *ptrs[3][32];
+  mimd_sets(n, m, off, r, c, v, store, ptrs);
+  const coo_t *const *sets[3] = {ptrs[0], ptrs[1], ptrs[2]};
+  mmats_t mm;
+  mimd_pack(&mm, n, m, sets);
+  buf_t b = {0};
+  int rc = mimd_prove(log_n, log_m, &mm, wit, seed_w, seed_e, &b);
+  mimd_free(&mm);
+  *out = b.p;
+  *len = b.n;
+  return rc;
+}
+API int orc_mimd_verify(int k, int log_n, int log_m,
+                        const size_t *aoff, const u32 *ar, const u32 *ac, const u64 *av,
+                        const size_t *boff, const u32 *br, const u32 *bc, const u64 *bv,
+                        const size_t *coff, const u32 *cr, const u32 *cc, const u64 *cv,
+                        u64 seed_w, u64 seed_e, const u8 *proof, size_t len) {
+  or_set_width(k);
+  size_t n = (size_t)1 << log_n, m = (size_t)1 << log_m;
+  const size_t *off[3] = {aoff, boff, coff};
+  const u32 *r[3] = {ar, br, cr}, *c[3] = {ac, bc, cc};
+  const u64 *v[3] = {av, bv, cv};
+  coo_t store[3][32];
+  const coo_t *ptrs[3][32];
+  mimd_sets(n, m, off, r, c, v, store, ptrs);
+  const coo_t *const *sets[3] = {ptrs[0], ptrs[1], ptrs[2]};
+  mmats_t mm;
+  mimd_pack(&mm, n, m, sets);
+  int rc = mimd_verify(log_n, log_m, &mm, seed_w, seed_e, proof, len);
+  mimd_free(&mm);
+  return rc;
+}

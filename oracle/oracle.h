@@ -256,6 +256,19 @@ void gdiag_free(gdiag_t *g);
 int general_prove(int log_b, const gdiag_t *diag, const u64 *w, u64 seed_w, u64 seed_e, buf_t *out);
 int general_verify(int log_b, const gdiag_t *diag, u64 seed_w, u64 seed_e, const u8 *proof, size_t len);
 
+typedef struct {
+  size_t rows, cols;
+  size_t ncell[3];
+  u32 *r[3], *c[3]; /* union-support cells, (r,c) sorted */
+  u64 *d[3];        /* 32 instance values per cell */
+  gr *dhat[3];      /* pack_r2(d) */
+} mmats_t;
+/* pack_mimd_matrices (reduce.rs:249-273): sets[w][k] = instance k of matrix w */
+void mimd_pack(mmats_t *mm, size_t rows, size_t cols, const coo_t *const *sets[3]);
+void mimd_free(mmats_t *mm);
+int mimd_prove(int log_n, int log_m, const mmats_t *mm, const u64 *wit /*32*m*/, u64 seed_w, u64 seed_e, buf_t *out);
+int mimd_verify(int log_n, int log_m, const mmats_t *mm, u64 seed_w, u64 seed_e, const u8 *proof, size_t len);
+
 /* ---------------- misc ---------------- */
 void *xmalloc(size_t n);
 void *xcalloc(size_t n, size_t s);

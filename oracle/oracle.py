@@ -78,6 +78,11 @@ class Oracle:
             _u64p, C.c_uint64, C.c_uint64, C.POINTER(C.c_void_p), C.POINTER(_sz)]
         L.orc_general_verify.argtypes = [C.c_int, C.c_int] + coo * 3 + [
             C.c_uint64, C.c_uint64, C.c_char_p, _sz]
+        mset = [_u64p, _u32p, _u32p, _u64p]
+        L.orc_mimd_prove.argtypes = [C.c_int, C.c_int, C.c_int] + mset * 3 + [
+            _u64p, C.c_uint64, C.c_uint64, C.POINTER(C.c_void_p), C.POINTER(_sz)]
+        L.orc_mimd_verify.argtypes = [C.c_int, C.c_int, C.c_int] + mset * 3 + [
+            C.c_uint64, C.c_uint64, C.c_char_p, _sz]
         L.orc_free.argtypes = [C.c_void_p]
 
     # ---- ring ----
@@ -258,6 +263,30 @@ class Oracle:
     def general_verify(self, k, log_b, A, B, Cm, proof, seed_w=0xDEAD, seed_e=0xCAFE):
         return VERIFY_ERRORS[self.L.orc_general_verify(k, log_b, *_coo_args(A), *_coo_args(B),
                                                        *_coo_args(Cm), seed_w, seed_e, proof, len(proof))]
+
+    # MIMD: each matrix set is a list of 32 COO triples (rows, cols, vals)
+    def mimd_prove(self, k, log_n, log_m, As, Bs, Cs, wit, seed_w=0xDEAD, seed_e=0xCAFE):
+        out, ln = C.c_void_p(), _sz()
+        rc = self.L.orc_mimd_prove(k, log_n, log_m, *_set_args(As), *_set_args(Bs), *_set_args(Cs),
+                                   np.ascontiguousarray(wit, np.uint64).reshape(-1), seed_w, seed_e,
+                                   C.byref(out), C.byref(ln))
+        if rc != 0:
+            raise RuntimeError("oracle mimd prover self-check failed")
+        return self._bytes(out, ln.value)
+
+    def mimd_verify(self, k, log_n, log_m, As, Bs, Cs, proof, seed_w=0xDEAD, seed_e=0xCAFE):
+        return VERIFY_ERRORS[self.L.orc_mimd_verify(k, log_n, log_m, *_set_args(As), *_set_args(Bs),
+                                                    *_set_args(Cs), seed_w, seed_e, proof, len(proof))]
+
+
+def _set_args(ms):
+    if len(ms) != 32:
+        raise ValueError("a MIMD matrix set holds 32 instance matrices")
+    off = np.zeros(33, np.uint64)
+    off[1:] = np.cumsum([len(m[0]) for m in ms])
+    return (off, np.ascontiguousarray(np.concatenate([m[0] for m in ms]), np.uint32),
+            np.ascontiguousarray(np.concatenate([m[1] for m in ms]), np.uint32),
+            np.ascontiguousarray(np.concatenate([m[2] for m in ms]), np.uint64))
 
 
 _ORACLE = None

@@ -1,11 +1,15 @@
 /*
- * Oracle: end-to-end SIMD and General provers/verifiers, proof
- * (de)serialization and the general-circuit diagonal packing.
+ * Oracle: end-to-end SIMD, MIMD and General provers/verifiers, proof
+ * (de)serialization, the MIMD union-support packing and the
+ * general-circuit diagonal packing.
  * TEST INFRASTRUCTURE ONLY.
  *
  * Follows: reduction/src/reduce.rs:33-104 (simd_to_packed),
  *          reduce.rs:372-438, 618-651 (diagonal layers), 670-744
  *          (general_reduce_witness), matrix.rs:73-154;
+ *          reduce.rs:119-141, 249-273, 287-345 (MIMD packing and
+ *          witness reduction);
+ *          snark-ring/src/mimd.rs:196-219, 242-397 (prove), 422-551 (verify);
  *          snark-ring/src/simd.rs:220-373 (prove), 391-522 (verify),
  *          528-580 (row builders), 742-748 (finals);
  *          snark-ring/src/general.rs:179-271 (helpers), 354-359,
@@ -603,6 +607,272 @@ done:
   for (int l = 0; l < 32; l++) free(rows[l]);
   free(rt); free(s1); free(s2); free(e1e); free(e1abc); free(eq_s1); free(eq_s2);
   sc_free(&mp); sc_free(&lp); open_free(&oem); open_free(&oabc); open_free(&oes); open_free(&orho);
+  pcs_free(&pw); pcs_free(&pabc); pcs_free(&pe);
+  return res;
+}
+
+/* ========================= MIMD ========================= */
+typedef struct { u32 r, c; int k; u64 v; } mitem_t;
+static int mitem_cmp(const void *x, const void *y) {
+  const mitem_t *a = x, *b = y;
+  if (a->r != b->r) return a->r < b->r ? -1 : 1;
+  if (a->c != b->c) return a->c < b->c ? -1 : 1;
+  return 0;
+}
+/* pack_matrix_set (reduce.rs:119-141): union support of the 32 instance
+ * matrices, entry (r,c) = pack_r2 of the 32 instance values (summed per
+ * instance when a triple repeats), cells in (r,c) order. */
+void mimd_pack(mmats_t *mm, size_t rows, size_t cols, const coo_t *const *sets[3]) {
+  memset(mm, 0, sizeof *mm);
+  mm->rows = rows;
+  mm->cols = cols;
+  for (int w = 0; w < 3; w++) {
+    size_t tot = 0;
+    for (int k = 0; k < 32; k++) tot += sets[w][k]->nnz;
+    mitem_t *it = xmalloc((tot + 1) * sizeof(mitem_t));
+    size_t o = 0;
+    for (int k = 0; k < 32; k++)
+      for (size_t j = 0; j < sets[w][k]->nnz; j++, o++) {
+        it[o].r = sets[w][k]->r[j]; it[o].c = sets[w][k]->c[j]; it[o].k = k; it[o].v = sets[w][k]->v[j];
+      }
+    qsort(it, tot, sizeof(mitem_t), mitem_cmp);
+    size_t nc = 0;
+    for (size_t j = 0; j < tot; j++)
+      if (j == 0 || mitem_cmp(&it[j], &it[j - 1]) != 0) nc++;
+    mm->ncell[w] = nc;
+    mm->r[w] = xmalloc((nc + 1) * sizeof(u32));
+    mm->c[w] = xmalloc((nc + 1) * sizeof(u32));
+    mm->d[w] = xcalloc((nc + 1) * 32, sizeof(u64));
+    mm->dhat[w] = xmalloc((nc + 1) * sizeof(gr));
+    size_t cur = (size_t)-1;
+    for (size_t j = 0; j < tot; j++) {
+      if (j == 0 || mitem_cmp(&it[j], &it[j - 1]) != 0) { cur++; mm->r[w][cur] = it[j].r; mm->c[w][cur] = it[j].c; }
+      mm->d[w][cur * 32 + it[j].k] = mk(mm->d[w][cur * 32 + it[j].k] + it[j].v);
+    }
+    for (size_t j = 0; j < nc; j++) pack_r2(&mm->dhat[w][j], &mm->d[w][j * 32]);
+    free(it);
+  }
+}
+void mimd_free(mmats_t *mm) {
+  for (int w = 0; w < 3; w++) { free(mm->r[w]); free(mm->c[w]); free(mm->d[w]); free(mm->dhat[w]); }
+  memset(mm, 0, sizeof *mm);
+}
+static void mimd_params(int log_n, int log_m, u64 seed_w, u64 seed_e, pcs_t *pw, pcs_t *pabc, pcs_t *pe) {
+  pcs_setup(pw, log_m, SECURE_Q, SECURE_DELTA, SECURE_LAMBDA, seed_w);
+  pcs_setup_stacked(pabc, log_n, SECURE_Q, SECURE_DELTA, SECURE_LAMBDA, seed_w ^ 0xABC, 3);
+  r2pcs_setup(pe, log_n, SECURE_Q, SECURE_DELTA, SECURE_LAMBDA, seed_e);
+}
+/* matvec_gr128_dense_csr over the union-support cells (matrix.rs) */
+static gr *mimd_matvec(const mmats_t *mm, int w, const gr *x) {
+  gr *o = xcalloc(mm->rows, sizeof(gr));
+  gr t;
+  for (size_t j = 0; j < mm->ncell[w]; j++) {
+    gr_mul(&t, &mm->dhat[w][j], &x[mm->c[w][j]]);
+    gr_add(&o[mm->r[w][j]], &o[mm->r[w][j]], &t);
+  }
+  return o;
+}
+/* build_combined_row (mimd.rs:196-219) with eq_s1 for all three matrices,
+ * as both the prover (mimd.rs:364) and the verifier (mimd.rs:492-499) call it */
+static gr *mimd_row(const mmats_t *mm, const gr *eq, const gr *gam, const gr *gam2) {
+  gr *acc[3], t;
+  for (int w = 0; w < 3; w++) {
+    acc[w] = xcalloc(mm->cols, sizeof(gr));
+    for (size_t j = 0; j < mm->ncell[w]; j++) {
+      gr_mul(&t, &eq[mm->r[w][j]], &mm->dhat[w][j]);
+      gr_add(&acc[w][mm->c[w][j]], &acc[w][mm->c[w][j]], &t);
+    }
+  }
+  for (size_t h = 0; h < mm->cols; h++) {
+    gr_mul(&t, gam, &acc[1][h]); gr_add(&acc[0][h], &acc[0][h], &t);
+    gr_mul(&t, gam2, &acc[2][h]); gr_add(&acc[0][h], &acc[0][h], &t);
+  }
+  free(acc[1]); free(acc[2]);
+  return acc[0];
+}
+
+/* prove_mimd_with_timing (mimd.rs:242-397), mimd_reduce_witness
+ * (reduce.rs:287-345) with the A,B quotients from mul_with_quotients_only */
+int mimd_prove(int log_n, int log_m, const mmats_t *mm, const u64 *wit, u64 seed_w, u64 seed_e, buf_t *out) {
+  size_t n = (size_t)1 << log_n, m = (size_t)1 << log_m;
+  const consts_t *K = consts();
+  pcs_t pw, pabc, pe;
+  mimd_params(log_n, log_m, seed_w, seed_e, &pw, &pabc, &pe);
+  /* 1. witness reduction */
+  gr *w_phi = xmalloc(m * sizeof(gr));
+  u64 *w_slots = xmalloc(m * 32 * sizeof(u64));
+  for (size_t j = 0; j < m; j++) {
+    for (int k = 0; k < 32; k++) w_slots[j * 32 + k] = wit[(size_t)k * m + j];
+    pack_r2(&w_phi[j], &w_slots[j * 32]);
+  }
+  gr va; gr_from_g4(&va, &K->v_alpha);
+  gr *e_[3], t, u;
+  for (int w = 0; w < 2; w++) {
+    gr *q1 = xcalloc(n, sizeof(gr)), *q2 = xcalloc(n, sizeof(gr));
+    for (size_t j = 0; j < mm->ncell[w]; j++) {
+      gr d1, d2;
+      mul_with_quotients_only(&d1, &d2, &mm->d[w][j * 32], &w_slots[(size_t)mm->c[w][j] * 32]);
+      gr_add(&q1[mm->r[w][j]], &q1[mm->r[w][j]], &d1);
+      gr_add(&q2[mm->r[w][j]], &q2[mm->r[w][j]], &d2);
+    }
+    e_[w] = xmalloc(n * sizeof(gr));
+    for (size_t i = 0; i < n; i++) {
+      gr_mul(&t, &q1[i], &va); gr_mul(&u, &q2[i], &K->v_beta); gr_add(&e_[w][i], &t, &u);
+    }
+    free(q1); free(q2);
+  }
+  /* 2. matvecs, hats and errors (mimd.rs:258-295) */
+  gr *asum = mimd_matvec(mm, 0, w_phi), *bsum = mimd_matvec(mm, 1, w_phi), *csum = mimd_matvec(mm, 2, w_phi);
+  gr *ah = xmalloc(n * sizeof(gr)), *bh = xmalloc(n * sizeof(gr)), *chh = xmalloc(n * sizeof(gr));
+  gr *em = xmalloc(n * sizeof(gr));
+  e_[2] = xmalloc(n * sizeof(gr));
+  u64 *as = xmalloc(n * 32 * sizeof(u64)), *bs = xmalloc(n * 32 * sizeof(u64)), *cs = xmalloc(n * 32 * sizeof(u64));
+  for (size_t i = 0; i < n; i++) {
+    gr_sub(&ah[i], &asum[i], &e_[0][i]);
+    gr_sub(&bh[i], &bsum[i], &e_[1][i]);
+    u64 sl[32];
+    psi_r2(sl, &csum[i]);
+    pack_r2(&chh[i], sl);
+    gr_sub(&e_[2][i], &csum[i], &chh[i]);
+    gr_mul(&t, &ah[i], &bh[i]);
+    gr_sub(&em[i], &t, &chh[i]);
+    psi_r2(&as[i * 32], &ah[i]);
+    psi_r2(&bs[i * 32], &bh[i]);
+    psi_r2(&cs[i * 32], &chh[i]);
+  }
+  free(asum); free(bsum); free(csum);
+  /* 3. commits + transcript */
+  comm_t cw, cabc, cem, ces;
+  pcs_commit(&pw, w_slots, &cw);
+  const u64 *abc[3] = {as, bs, cs};
+  pcs_commit_stacked(&pabc, abc, 3, &cabc);
+  r2pcs_commit(&pe, em, &cem);
+  const gr *est[3] = {e_[0], e_[1], e_[2]};
+  r2pcs_commit_stacked(&pe, est, 3, &ces);
+  tr_t tr;
+  tr_new(&tr, "mimd");
+  tr_root(&tr, cw.root); tr_root(&tr, cabc.root); tr_root(&tr, cem.root); tr_root(&tr, ces.root);
+  gr *rt = xmalloc(log_n * sizeof(gr));
+  tr_target(&tr, "mult_check_target", log_n, rt);
+  /* 4. mult check (sub form) */
+  tr_label(&tr, "mult_check_sumcheck");
+  ch_t ch = {0};
+  ch.tr = &tr;
+  sc_t mp, lp;
+  gr *s1 = xmalloc(log_n * sizeof(gr)), *s2 = xmalloc(log_m * sizeof(gr));
+  prove_mult_check_sub(ah, bh, chh, em, log_n, rt, &ch, &mp, s1);
+  tr_gr_slice(&tr, "mult/finals", mp.fin, 4);
+  gr gam, gam2;
+  tr_challenge(&tr, "linear_check_gamma", &gam);
+  gr_mul(&gam2, &gam, &gam);
+  /* 5. opens at s1 */
+  gr wts[3];
+  gr_one(&wts[0]); wts[1] = gam; wts[2] = gam2;
+  open_t oes, oabc, oem, ow;
+  gr estack_val, abc_val, em_val, yw;
+  pcs_open_weighted(&pe, &ces, s1, wts, &estack_val, &oes);
+  pcs_open_weighted(&pabc, &cabc, s1, wts, &abc_val, &oabc);
+  pcs_open_weighted(&pe, &cem, s1, NULL, &em_val, &oem);
+  /* 6. combined row + linear check (m_row first, w_phi second) */
+  gr *eq_s1 = xmalloc(n * sizeof(gr));
+  build_eq_table(s1, log_n, eq_s1);
+  gr *row = mimd_row(mm, eq_s1, &gam, &gam2);
+  gr claim_lin;
+  gr_mul(&t, &gam, &mp.fin[1]); gr_add(&claim_lin, &mp.fin[0], &t);
+  gr_mul(&t, &gam2, &mp.fin[2]); gr_add(&claim_lin, &claim_lin, &t);
+  gr abc_exp = claim_lin;
+  gr_add(&claim_lin, &claim_lin, &estack_val);
+  tr_label(&tr, "linear_check_sumcheck");
+  prove_linear_simd(row, w_phi, log_m, &claim_lin, &ch, &lp, s2);
+  /* 7. open w at s2 */
+  pcs_open_weighted(&pw, &cw, s2, NULL, &yw, &ow);
+  buf_put(out, cw.root, 32); buf_put(out, cabc.root, 32); buf_put(out, cem.root, 32); buf_put(out, ces.root, 32);
+  ser_mult(out, &mp, 0);
+  ser_mult(out, &lp, 1);
+  open_serialize(out, &ow, 0);
+  open_serialize(out, &oem, 0);
+  open_serialize(out, &oabc, 0);
+  open_serialize(out, &oes, 0);
+  int ok = gr_eq(&em_val, &mp.fin[3]) && gr_eq(&abc_val, &abc_exp) && gr_eq(&yw, &lp.fin[1]);
+  free(row); free(eq_s1); free(s1); free(s2); free(rt);
+  open_free(&oes); open_free(&oabc); open_free(&oem); open_free(&ow); sc_free(&mp); sc_free(&lp);
+  comm_free(&cw); comm_free(&cabc); comm_free(&cem); comm_free(&ces);
+  for (int w = 0; w < 3; w++) free(e_[w]);
+  free(ah); free(bh); free(chh); free(em); free(as); free(bs); free(cs); free(w_phi); free(w_slots);
+  pcs_free(&pw); pcs_free(&pabc); pcs_free(&pe);
+  return ok ? 0 : -1;
+}
+
+/* verify_mimd_with_timing (mimd.rs:422-551) */
+int mimd_verify(int log_n, int log_m, const mmats_t *mm, u64 seed_w, u64 seed_e, const u8 *proof, size_t len) {
+  size_t n = (size_t)1 << log_n, m = (size_t)1 << log_m;
+  rd_t r = {proof, len, 0, 0};
+  u8 roots[4][32];
+  for (int i = 0; i < 4; i++) { const u8 *p = rd_take(&r, 32); if (p) memcpy(roots[i], p, 32); }
+  sc_t mp, lp;
+  open_t ow, oem, oabc, oes;
+  rd_mult(&r, &mp, 0, 4);
+  rd_mult(&r, &lp, 1, 2);
+  rd_open(&r, &ow, 0, 32);
+  rd_open(&r, &oem, 0, E);
+  rd_open(&r, &oabc, 0, 32);
+  rd_open(&r, &oes, 0, E);
+  if (r.bad || r.o != r.n) {
+    sc_free(&mp); sc_free(&lp); open_free(&ow); open_free(&oem); open_free(&oabc); open_free(&oes);
+    return VE_PARSE;
+  }
+  pcs_t pw, pabc, pe;
+  mimd_params(log_n, log_m, seed_w, seed_e, &pw, &pabc, &pe);
+  int res = VE_OK;
+  gr *rt = xmalloc(log_n * sizeof(gr)), *s1 = xmalloc((log_n + 1) * sizeof(gr)), *s2 = xmalloc((lp.n + 1) * sizeof(gr));
+  gr *e1e = xmalloc(pe.k2 * sizeof(gr)), *e1abc = xmalloc(pabc.k2 * sizeof(gr));
+  gr *eq_s1 = NULL, *eq_s2 = NULL, *row = NULL;
+  tr_t tr;
+  tr_new(&tr, "mimd");
+  for (int i = 0; i < 4; i++) tr_root(&tr, roots[i]);
+  tr_target(&tr, "mult_check_target", log_n, rt);
+  tr_label(&tr, "mult_check_sumcheck");
+  ch_t ch = {0};
+  ch.tr = &tr;
+  if (mp.n != (size_t)log_n || verify_mult_check_sub(&mp, rt, log_n, &ch, s1) != 0) { res = VE_MULT; goto done; }
+  tr_gr_slice(&tr, "mult/finals", mp.fin, 4);
+  gr gam, gam2, t;
+  tr_challenge(&tr, "linear_check_gamma", &gam);
+  gr_mul(&gam2, &gam, &gam);
+  tr_label(&tr, "linear_check_sumcheck");
+  if (lp.n == 0 || verify_linear_simd(&lp, &ch, s2) != 0) { res = VE_LIN; goto done; }
+  if (lp.n != (size_t)log_m) { res = VE_LIN; goto done; }
+  size_t lk1e = (size_t)__builtin_ctzll(pe.k1), lk1a = (size_t)__builtin_ctzll(pabc.k1);
+  build_eq_table(s1 + lk1e, (size_t)log_n - lk1e, e1e);
+  build_eq_table(s1 + lk1a, (size_t)log_n - lk1a, e1abc);
+  if (oes.k2 != pe.k2 || oem.k2 != pe.k2 || oabc.k2 != pabc.k2) { res = VE_LINCLAIM; goto done; }
+  gr estack_val, em_s1, abc_val, abc_exp;
+  inner(oes.mbar, e1e, pe.k2, &estack_val);
+  inner(oem.mbar, e1e, pe.k2, &em_s1);
+  inner(oabc.mbar, e1abc, pabc.k2, &abc_val);
+  gr_mul(&t, &gam, &mp.fin[1]); gr_add(&abc_exp, &mp.fin[0], &t);
+  gr_mul(&t, &gam2, &mp.fin[2]); gr_add(&abc_exp, &abc_exp, &t);
+  if (!gr_eq(&abc_val, &abc_exp) || !gr_eq(&em_s1, &mp.fin[3])) { res = VE_LINCLAIM; goto done; }
+  gr exp_claim;
+  gr_add(&exp_claim, &abc_exp, &estack_val);
+  if (!gr_eq(&lp.claimed, &exp_claim)) { res = VE_LINCLAIM; goto done; }
+  eq_s1 = xmalloc(n * sizeof(gr)); eq_s2 = xmalloc(m * sizeof(gr));
+  build_eq_table(s1, log_n, eq_s1);
+  build_eq_table(s2, log_m, eq_s2);
+  row = mimd_row(mm, eq_s1, &gam, &gam2);
+  gr at_s2;
+  inner(eq_s2, row, m, &at_s2);
+  if (!gr_eq(&at_s2, &lp.fin[0])) { res = VE_AROW; goto done; }
+  gr wts[3];
+  gr_one(&wts[0]); wts[1] = gam; wts[2] = gam2;
+  if (!pcs_verify_weighted(&pw, roots[0], s2, NULL, 1, &lp.fin[1], &ow)) { res = VE_PCSW; goto done; }
+  if (!pcs_verify_weighted(&pe, roots[2], s1, NULL, 1, &em_s1, &oem)) { res = VE_PCSE; goto done; }
+  if (!pcs_verify_weighted(&pabc, roots[1], s1, wts, 3, &abc_val, &oabc)) { res = VE_PCSW; goto done; }
+  if (!pcs_verify_weighted(&pe, roots[3], s1, wts, 3, &estack_val, &oes)) { res = VE_PCSE; goto done; }
+  if (!columns_in_kernel(&oem) || !columns_in_kernel(&oes)) { res = VE_KERNEL; goto done; }
+done:
+  free(rt); free(s1); free(s2); free(e1e); free(e1abc); free(eq_s1); free(eq_s2); free(row);
+  sc_free(&mp); sc_free(&lp); open_free(&ow); open_free(&oem); open_free(&oabc); open_free(&oes);
   pcs_free(&pw); pcs_free(&pabc); pcs_free(&pe);
   return res;
 }

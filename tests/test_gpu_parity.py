@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import nenya_amd as na
-from nenya_amd.instances import general_instance, simd_instance
+from nenya_amd.instances import general_instance, mimd_instance, simd_instance
 
 pytestmark = pytest.mark.gpu
 
@@ -47,3 +47,14 @@ def test_general_bytes_equal_oracle(orc, k, log_n):
     want = orc.general_prove(k, log_n - 5, A, B, Cm, w)
     assert len(got) == len(want) and got == want, f"first differing byte {_first_diff(got, want)}"
     assert orc.general_verify(k, log_n - 5, A, B, Cm, got) == "Ok"
+
+
+@pytest.mark.parametrize("k,log", [(32, 5), (8, 5), (16, 5), (64, 5), (32, 8), (64, 9), (8, 10), (16, 9)])
+def test_mimd_bytes_equal_oracle(orc, k, log):
+    print(f"mimd k={k} log={log} seed=9", flush=True)
+    As, Bs, Cs, wit = mimd_instance(log, k, 9)
+    params = na.MimdSnarkParams.secure(log, log, (As, Bs, Cs), 0xDEAD, 0xCAFE, k=k)
+    got = na.prove_mimd(wit, params)
+    want = orc.mimd_prove(k, log, log, As, Bs, Cs, wit)
+    assert len(got) == len(want) and got == want, f"first differing byte {_first_diff(got, want)}"
+    assert orc.mimd_verify(k, log, log, As, Bs, Cs, got) == "Ok"

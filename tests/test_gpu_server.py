@@ -1,25 +1,29 @@
 """GPU: proofs made through the batch/serve entry points (several worker
-streams, mixed widths and models in one queue) are byte-identical to the
+streams, mixed widths and SIMD / MIMD / General models in one queue) are byte-identical to the
 direct path's, and the oracle verifier accepts them."""
 import numpy as np
 import pytest
 
 import nenya_amd as na
-from nenya_amd.instances import general_instance, simd_instance
+from nenya_amd.instances import general_instance, mimd_instance, simd_instance
 
 pytestmark = pytest.mark.gpu
 
 
 def test_mixed_queue_matches_direct_path(orc):
-    print("server: 3 workers, 8 mixed jobs", flush=True)
+    print("server: 3 workers, 11 mixed jobs", flush=True)
     jobs = []
-    specs = [("g", 32, 10), ("s", 8, 6), ("g", 16, 10), ("s", 64, 5), ("g", 8, 11), ("s", 32, 7), ("g", 64, 10),
-             ("g", 32, 11)]
+    specs = [("g", 32, 10), ("s", 8, 6), ("m", 64, 6), ("g", 16, 10), ("s", 64, 5), ("g", 8, 11), ("m", 8, 7),
+             ("s", 32, 7), ("g", 64, 10), ("m", 32, 5), ("g", 32, 11)]
     for i, (model, k, log) in enumerate(specs):
         if model == "g":
             A, B, Cm, w = general_instance(log, k, 40 + i)
             p = na.GeneralSnarkParams.secure(log - 5, (A, B, Cm), 0xDEAD, 0xCAFE, k=k)
             jobs.append(("g", k, log, p, (A, B, Cm), w))
+        elif model == "m":
+            As, Bs, Cs, wit = mimd_instance(log, k, 40 + i)
+            p = na.MimdSnarkParams.secure(log, log, (As, Bs, Cs), 0xDEAD, 0xCAFE, k=k)
+            jobs.append(("m", k, log, p, (As, Bs, Cs), wit))
         else:
             A, B, Cm, wit = simd_instance(log, k, 40 + i)
             p = na.SimdSnarkParams.secure(log, log, 0xDEAD, 0xCAFE, k=k)
@@ -29,6 +33,8 @@ def test_mixed_queue_matches_direct_path(orc):
     for model, k, log, p, m3, w in jobs:
         if model == "g":
             ids.append(srv.submit_general(p, w))
+        elif model == "m":
+            ids.append(srv.submit_mimd(p, w))
         else:
             ids.append(srv.submit_simd(na.SimdR1csInstance(*m3, w), p))
     for jid, (model, k, log, p, m3, w) in zip(ids, jobs):
@@ -38,6 +44,10 @@ def test_mixed_queue_matches_direct_path(orc):
             direct = na.prove_general(w, p)
             assert got == direct, f"general k={k} log={log}"
             assert orc.general_verify(k, log - 5, *m3, got) == "Ok"
+        elif model == "m":
+            direct = na.prove_mimd(w, p)
+            assert got == direct, f"mimd k={k} log={log}"
+            assert orc.mimd_verify(k, log, log, *m3, got) == "Ok"
         else:
             direct = na.prove_simd(na.SimdR1csInstance(*m3, w), p)
             assert got == direct, f"simd k={k} log={log}"
