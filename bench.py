@@ -51,23 +51,77 @@ def cpu_baseline(k, log_n, seed):
 
 def mixed_jobs(na, njobs, seed):
     """The mixed batch: logN ~ U{12..20}, k ~ U{8,16,32,64}, model ~ U{SIMD,
-    General} (SIMD per-instance log = logN - 5, General log_blocks = logN - 5),
-    instance seed = seed + i."""
+    MIMD, General} (SIMD/MIMD per-instance log = logN - 5, General
+    log_blocks = logN - 5), instance seed = seed + i."""
     import numpy as np
-    from nenya_amd.instances import general_instance, simd_instance
+    from nenya_amd.instances import general_instance, mimd_instance, simd_instance
     rng = np.random.default_rng(seed)
     jobs = []
     for i in range(njobs):
-        log_n, k, model = int(rng.integers(12, 21)), int(rng.choice([8, 16, 32, 64])), int(rng.integers(0, 2))
-        if model:
+        log_n, k, model = int(rng.integers(12, 21)), int(rng.choice([8, 16, 32, 64])), int(rng.integers(0, 3))
+        if model == 1:
             A, B, C, w = general_instance(log_n, k, seed + i)
             p = na.GeneralSnarkParams.secure(log_n - 5, (A, B, C), 0xDEAD, 0xCAFE, k=k)
             jobs.append(("general", p, w))
+        elif model == 2:
+            As, Bs, Cs, wit = mimd_instance(log_n - 5, k, seed + i)
+            p = na.MimdSnarkParams.secure(log_n - 5, log_n - 5, (As, Bs, Cs), 0xDEAD, 0xCAFE, k=k)
+            jobs.append(("mimd", p, wit))
         else:
             A, B, C, wit = simd_instance(log_n - 5, k, seed + i)
             p = na.SimdSnarkParams.secure(log_n - 5, log_n - 5, 0xDEAD, 0xCAFE, k=k)
             jobs.append(("simd", p, na.SimdR1csInstance(A, B, C, wit)))
     return jobs
+
+
+def mimd_serving_jobs(na, njobs, seed):
+    """The MIMD serving mix: k = 64, logN ~ U{10..22} (per-instance log =
+    logN - 5, A rows with 1..4 nonzeros); one circuit (32 instance matrix
+    sets) per size, set up before the run, and a fresh satisfying witness
+    batch per job (seed + i)."""
+    import numpy as np
+    from nenya_amd.instances import mimd_instance, mimd_witnesses
+    rng = np.random.default_rng(seed)
+    sizes = [int(rng.integers(10, 23)) for _ in range(njobs)]
+    circ = {}
+    jobs = []
+    for i, log_n in enumerate(sizes):
+        lg = log_n - 5
+        if lg not in circ:
+            As, Bs, Cs, _ = mimd_instance(lg, 64, seed + 1000 * lg, max_a=4)
+            circ[lg] = (na.MimdSnarkParams.secure(lg, lg, (As, Bs, Cs), 0xDEAD, 0xCAFE, k=64), As, Bs)
+        p, As, Bs = circ[lg]
+        jobs.append(("mimd", p, mimd_witnesses(As, Bs, lg, 64, seed + i)))
+    return jobs, sorted(set(sizes))
+
+
+def time_proofs(fn, reps):
+    fn()
+    t = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    return (time.perf_counter() - t) / reps * 1e3
+
+
+def width_sweep(na, seed, reps=3):
+    """SIMD at n = m = 2^17 per instance for k = 8, 16, 32, 64, and the PNG
+    filter batch (u8, 32 images x 2^15 bytes) -- ms per proof, host inputs."""
+    from nenya_amd.instances import png_instance, simd_instance
+    out = {}
+    for k in (8, 16, 32, 64):
+        A, B, C, wit = simd_instance(17, k, seed + k)
+        p = na.SimdSnarkParams.secure(17, 17, 0xDEAD, 0xCAFE, k=k)
+        inst = na.SimdR1csInstance(A, B, C, wit)
+        out[f"simd_u{k}_2p17_ms"] = round(time_proofs(lambda: na.prove_simd(inst, p), reps), 2)
+        log(f"  simd u{k} 2^17: {out[f'simd_u{k}_2p17_ms']} ms/proof")
+        del p, inst
+    ln, lm, A, B, C, wit = png_instance(20)
+    p = na.SimdSnarkParams.secure(ln, lm, 0xDEAD, 0xCAFE, k=8)
+    inst = na.SimdR1csInstance(A, B, C, wit)
+    out["png_u8_2p20_bytes_ms"] = round(time_proofs(lambda: na.prove_simd(inst, p), reps), 2)
+    out["png_shape"] = {"n": 2 ** ln, "m": 2 ** lm}
+    log(f"  png u8 log_bytes=20 (n=2^{ln}, m=2^{lm}): {out['png_u8_2p20_bytes_ms']} ms/proof")
+    return out
 
 
 def run_queue(na, jobs, workers, arrivals=None):
@@ -82,7 +136,12 @@ def run_queue(na, jobs, workers, arrivals=None):
             while time.perf_counter() - t0 < arrivals[i]:
                 time.sleep(0.0002)
         sub.append(time.perf_counter())
-        ids.append(srv.submit_general(p, x) if model == "general" else srv.submit_simd(x, p))
+        if model == "general":
+            ids.append(srv.submit_general(p, x))
+        elif model == "mimd":
+            ids.append(srv.submit_mimd(p, x))
+        else:
+            ids.append(srv.submit_simd(x, p))
     lat = []
     for jid in ids:
         _, q, r = srv.wait(jid)
@@ -139,6 +198,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=64, help="mixed batch size (0 skips batch/serve figures)")
     ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the width sweep, the MIMD serving mix and the 2^25 General proof")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a 256 MiB device copy (PMC calibration of FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -237,11 +298,35 @@ def main():
         q, srv = queue_stats(na, jobs, args.workers, 77 + args.seed)
         batch = {"jobs": len(jobs), "workers": args.workers, "proofs_per_s": q["batched"]["proofs_per_s"],
                  "p50_ms": q["batched"]["p50_ms"], "p99_ms": q["batched"]["p99_ms"], "plain": q["plain"],
-                 "mix": "logN~U{12..20}, k~U{8,16,32,64}, model~U{SIMD,General}, host inputs, seed_i=base+i"}
+                 "mix": "logN~U{12..20}, k~U{8,16,32,64}, model~U{SIMD,MIMD,General}, host inputs, seed_i=base+i"}
         serve = {"lambda_per_s": srv["lambda_per_s"], "jobs": len(jobs), "workers": args.workers,
                  "p50_ms": srv["batched"]["p50_ms"], "p99_ms": srv["batched"]["p99_ms"], "plain": srv["plain"],
                  "arrivals": "Poisson at 70% of the batched throughput, same job mix"}
         del jobs
+
+    widths = large = None
+    if not args.no_extras:
+        log("width sweep:")
+        widths = width_sweep(na, 300 + args.seed)
+        t0 = time.perf_counter()
+        mj, sizes = mimd_serving_jobs(na, 32, 2000 + args.seed)
+        log(f"mimd serving mix: {len(mj)} jobs over per-instance logs {sizes} set up in {time.perf_counter() - t0:.1f}s")
+        q, srv = queue_stats(na, mj, args.workers, 99 + args.seed)
+        large = {"mimd_u64_serve": {"jobs": len(mj), "workers": args.workers, "lambda_per_s": srv["lambda_per_s"],
+                                    "p50_ms": srv["batched"]["p50_ms"], "p99_ms": srv["batched"]["p99_ms"],
+                                    "plain": srv["plain"], "batch": q,
+                                    "mix": "k=64, logN~U{10..22}, one circuit per size, witness per job"}}
+        del mj
+        del params, wd
+        torch.cuda.empty_cache()
+        t0 = time.perf_counter()
+        A2, B2, C2, w2 = general_instance(25, 32, args.seed)
+        p2 = na.GeneralSnarkParams.secure(20, (A2, B2, C2), 0xDEAD, 0xCAFE, k=32)
+        log(f"general u32 2^25: set up in {time.perf_counter() - t0:.1f}s")
+        ms25 = time_proofs(lambda: na.prove_general(w2, p2), 2)
+        large["general_u32_2p25_ms"] = round(ms25, 1)
+        log(f"  general u32 2^25: {ms25:.1f} ms/proof (host witness)")
+        del p2, A2, B2, C2, w2
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -271,6 +356,8 @@ def main():
         "stages_ms": {k2: round(v, 3) for k2, v in stages.items()},
         "batch": batch,
         "serve": serve,
+        "widths": widths,
+        "large": large,
     }
     print(json.dumps(out), flush=True)
 

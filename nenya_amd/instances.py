@@ -132,3 +132,49 @@ def set_arrays(ms):
     cols = np.ascontiguousarray(np.concatenate([m[1] for m in ms]), np.uint32)
     vals = np.ascontiguousarray(np.concatenate([m[2] for m in ms]), np.uint64)
     return off, rows, cols, vals
+
+
+def png_instance(log_bytes):
+    """The PNG-filter batch of png_filter.rs (snark-ring/examples/png_filter.rs):
+    32 grayscale images of b = 2^(log_bytes-5) bytes, W = min(256, b) columns,
+    row r filtered with Sub (left neighbour) when r is even and Up (the row
+    above) when odd (lines 84-103); raw bytes from the example's own SplitMix64
+    byte generator (lines 67-73, seed (slot << 40) ^ i * 0x10000001).  The
+    ring-u8 circuit (build_ring, lines 126-163): row i says
+    Orig[i] * 1 = Filt[i] + Orig[pred(i)] over the witness
+    w = (1, Orig[0], Filt[0], Orig[1], Filt[1], ...), n = b rows and
+    m = 2^ceil_log2(1 + 2b) columns (lines 270-273).  Returns
+    (log_n, log_m, A, B, C, wit[32, m]) with k = 8."""
+    ls = log_bytes - 5
+    b = 1 << ls
+    log_m = (2 * b).bit_length()  # ceil_log2(1 + 2b) for b a power of two
+    m = 1 << log_m
+    w_cols = min(256, b)
+    i = np.arange(b, dtype=np.int64)
+    r, c = i // w_cols, i % w_cols
+    pred = np.where(r % 2 == 0, np.where(c > 0, i - 1, -1), np.where(r > 0, i - w_cols, -1))
+    has = pred >= 0
+    one = np.ones(b, np.uint64)
+    A = (i.astype(np.uint32), (1 + 2 * i).astype(np.uint32), one)
+    B = (i.astype(np.uint32), np.zeros(b, np.uint32), one)
+    C = (np.concatenate([i, i[has]]).astype(np.uint32),
+         np.concatenate([2 + 2 * i, 1 + 2 * pred[has]]).astype(np.uint32),
+         np.ones(b + int(has.sum()), np.uint64))
+    wit = np.zeros((32, m), np.uint64)
+    with np.errstate(over="ignore"):
+        for s in range(32):
+            seed = (np.uint64(s) << np.uint64(40)) ^ (i.astype(np.uint64) * np.uint64(0x10000001))
+            orig = _mix(seed + _G) & np.uint64(0xFF)
+            pv = np.where(has, orig[np.maximum(pred, 0)], np.uint64(0))
+            filt = (orig - pv) & np.uint64(0xFF)
+            wit[s, 0] = 1
+            wit[s, 1:1 + 2 * b:2] = orig
+            wit[s, 2:2 + 2 * b:2] = filt
+    return ls, log_m, A, B, C, wit
+
+
+def mimd_witnesses(As, Bs, log, k, seed):
+    """Fresh satisfying witnesses (inputs from `seed`) for the 32 instance
+    circuits (As, Bs) of mimd_instance(log, k, ...)."""
+    n = 1 << log
+    return np.stack([random_witness(n, k, As[s], Bs[s], seed, tag=500 + s) for s in range(32)])

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import nenya_amd as na
-from nenya_amd.instances import general_instance, mimd_instance, simd_instance
+from nenya_amd.instances import general_instance, mimd_instance, png_instance, simd_instance
 
 pytestmark = pytest.mark.gpu
 
@@ -58,3 +58,14 @@ def test_mimd_bytes_equal_oracle(orc, k, log):
     want = orc.mimd_prove(k, log, log, As, Bs, Cs, wit)
     assert len(got) == len(want) and got == want, f"first differing byte {_first_diff(got, want)}"
     assert orc.mimd_verify(k, log, log, As, Bs, Cs, got) == "Ok"
+
+
+@pytest.mark.parametrize("log_bytes", [14, 16])
+def test_png_bytes_equal_oracle(orc, log_bytes):
+    print(f"png u8 log_bytes={log_bytes}", flush=True)
+    ln, lm, A, B, Cm, wit = png_instance(log_bytes)
+    params = na.SimdSnarkParams.secure(ln, lm, 0xDEAD, 0xCAFE, k=8)
+    got = na.prove_simd(na.SimdR1csInstance(A, B, Cm, wit), params)
+    want = orc.simd_prove(8, ln, lm, A, B, Cm, wit)
+    assert len(got) == len(want) and got == want, f"first differing byte {_first_diff(got, want)}"
+    assert orc.simd_verify(8, ln, lm, A, B, Cm, got) == "Ok"

@@ -159,3 +159,31 @@ def test_mimd_packed_relation_identity(orc, k):
         rhs = (rhs + orc.gr_mul(k, q1[r], va) + orc.gr_mul(k, q2[r], vb)) & np.uint64(msk)
         assert np.array_equal(lhs[r], rhs), r
         assert np.array_equal(orc.psi(k, lhs[r].reshape(1, 128))[0], aw[r])
+
+
+def test_png_instance(orc):
+    """The PNG-filter batch (png_filter.rs): raw bytes equal the example's
+    own generator (lines 67-73, restated here as scalar code), every image
+    satisfies its Sub/Up recurrence circuit, and the oracle's SIMD proof of
+    the batch verifies."""
+    from nenya_amd.instances import is_satisfied, png_instance
+    M = (1 << 64) - 1
+
+    def rnd_byte(seed):
+        x = (seed + 0x9E3779B97F4A7C15) & M
+        x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & M
+        x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & M
+        return (x ^ (x >> 31)) & 0xFF
+
+    ln, lm, A, B, C, w = png_instance(14)
+    assert (ln, lm) == (9, 11)  # b = 512 bytes per image, m = 2^ceil_log2(1 + 2b)
+    b = 1 << ln
+    for s in (0, 7, 31):
+        for i in (0, 1, 255, 256, 300, b - 1):
+            assert int(w[s, 1 + 2 * i]) == rnd_byte(((s << 40) ^ (i * 0x10000001)) & M)
+        # row 1 (odd) uses Up: Filt = Orig - Orig[above]
+        i = 256 + 3
+        assert int(w[s, 2 + 2 * i]) == (int(w[s, 1 + 2 * i]) - int(w[s, 1 + 2 * (i - 256)])) % 256
+        assert is_satisfied(A, B, C, w[s], b, 8)
+    p = orc.simd_prove(8, ln, lm, A, B, C, w)
+    assert orc.simd_verify(8, ln, lm, A, B, C, p) == "Ok"
