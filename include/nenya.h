@@ -115,6 +115,23 @@ int nenya_set_profile_only(nenya_params* p, const char* kernel);
 int nenya_kernel_stats(const nenya_params* p, int cap, char* names, size_t names_cap, double* ms, double* count,
                        double* bytes, double* macs);
 
+/* ---- Verifiers: verify_simd / verify_mimd / verify_general
+ * (snark-ring/src/simd.rs:380-522, mimd.rs:403-551, general.rs:545-698) on
+ * serialized proofs.  Transcript, sumcheck and PCS checks run on the host;
+ * the linear-size row recompute runs on the GPU with the params' matrices.
+ * The call returns NENYA_OK or an error code; *verdict receives 0 (Ok) or the
+ * VerifyError variant in error.rs:5-16 order: 1 MultSumcheck, 2 LinSumcheck,
+ * 3 LinClaimMismatch, 4 LinARowMismatch, 5 PcsW, 6 PcsE, 7 KernelCheck; and 8
+ * when the bytes do not parse as a proof of these params (the reference
+ * verifies proof structs and has no such case). */
+int nenya_verify_simd(const nenya_params* p,
+                      const uint32_t* a_rows, const uint32_t* a_cols, const void* a_vals, size_t a_nnz,
+                      const uint32_t* b_rows, const uint32_t* b_cols, const void* b_vals, size_t b_nnz,
+                      const uint32_t* c_rows, const uint32_t* c_cols, const void* c_vals, size_t c_nnz,
+                      const uint8_t* proof, size_t proof_len, int* verdict);
+int nenya_verify_mimd(const nenya_params* p, const uint8_t* proof, size_t proof_len, int* verdict);
+int nenya_verify_general(const nenya_params* p, const uint8_t* proof, size_t proof_len, int* verdict);
+
 /* ---- Batched and served proofs (no reference counterpart: the reference
  * proves one instance per call, general.rs:367-372, simd.rs:212-217,
  * mimd.rs:235-240).

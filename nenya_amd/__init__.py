@@ -18,6 +18,11 @@ Batched / served proofs (several proofs overlapping on one GPU):
     job = srv.submit_general(params, w)       # or submit_simd / submit_mimd
     proof, queued_ms, run_ms = srv.wait(job)
 
+Verifiers (host checks, row recompute on the GPU) return "Ok" or the
+reference's VerifyError name:
+
+    verify_general(params, proof)   # verify_simd(inst, params, proof), verify_mimd(params, proof)
+
 Matrices are COO triples (rows u32, cols u32, vals) with vals limbs of Z/2^k;
 witnesses are limb arrays.  Proofs are the reference's serialized bytes.
 Every call runs the HIP library nenya_amd/libnenya_amd.so; there is no CPU
@@ -25,5 +30,5 @@ fallback: without the library or a GPU the calls raise.
 """
 from ._lib import (  # noqa: F401
     NenyaError, GeneralSnarkParams, MimdSnarkParams, Server, SimdR1csInstance, SimdSnarkParams, device_count,
-    lib_path, prove_general, prove_general_device, prove_mimd, prove_simd,
+    lib_path, prove_general, prove_general_device, prove_mimd, prove_simd, verify_general, verify_mimd, verify_simd,
 )

@@ -37,6 +37,9 @@ def _lib():
         L.nenya_params_mimd_secure.argtypes = [C.c_int, C.c_int, C.c_int] + mset * 3 + [u64, u64, C.POINTER(vp)]
         L.nenya_prove_mimd.argtypes = [vp, vp, C.POINTER(vp), C.POINTER(sz)]
         L.nenya_submit_mimd.argtypes = [vp, vp, vp, C.POINTER(u64)]
+        L.nenya_verify_simd.argtypes = [vp] + coo * 3 + [C.c_char_p, sz, C.POINTER(C.c_int)]
+        L.nenya_verify_mimd.argtypes = [vp, C.c_char_p, sz, C.POINTER(C.c_int)]
+        L.nenya_verify_general.argtypes = [vp, C.c_char_p, sz, C.POINTER(C.c_int)]
         L.nenya_prove_general.argtypes = [vp, vp, sz, C.POINTER(vp), C.POINTER(sz)]
         L.nenya_prove_general_device.argtypes = [vp, vp, sz, C.POINTER(vp), C.POINTER(sz)]
         L.nenya_prove_general_device_view.argtypes = [vp, vp, sz, C.POINTER(vp), C.POINTER(sz)]
@@ -251,6 +254,36 @@ def prove_general_device(dev_ptr, n_limbs, params, view=False):
     return _take_proof(out, n)
 
 
+VERIFY_ERRORS = {0: "Ok", 1: "MultSumcheck", 2: "LinSumcheck", 3: "LinClaimMismatch", 4: "LinARowMismatch",
+                 5: "PcsW", 6: "PcsE", 7: "KernelCheck", 8: "Parse"}
+
+
+def verify_simd(inst, params, proof):
+    """verify_simd(a, b, c, &params, &proof) (snark-ring/src/simd.rs:380-522)
+    -> "Ok" or the VerifyError variant name ("Parse" for malformed bytes)."""
+    k = params.k
+    args = [params._h]
+    for m in (inst.a, inst.b, inst.c):
+        args += _coo_args(_coo(m, k))
+    v = C.c_int()
+    _check(_lib().nenya_verify_simd(*args, bytes(proof), len(proof), C.byref(v)))
+    return VERIFY_ERRORS[v.value]
+
+
+def verify_mimd(params, proof):
+    """verify_mimd(&params, &proof) (snark-ring/src/mimd.rs:403-551)."""
+    v = C.c_int()
+    _check(_lib().nenya_verify_mimd(params._h, bytes(proof), len(proof), C.byref(v)))
+    return VERIFY_ERRORS[v.value]
+
+
+def verify_general(params, proof):
+    """verify_general(&params, &proof) (snark-ring/src/general.rs:545-698)."""
+    v = C.c_int()
+    _check(_lib().nenya_verify_general(params._h, bytes(proof), len(proof), C.byref(v)))
+    return VERIFY_ERRORS[v.value]
+
+
 class Server:
     """Batched / served proofs on one GPU (nenya_server_* in include/nenya.h):
     `workers` host threads with their own streams drain a FIFO job queue,
@@ -297,13 +330,15 @@ class Server:
             _check(-r)
         return r == 1
 
-    def wait(self, job):
-        """-> (proof bytes, queued_ms, run_ms); releases the job."""
+    def wait(self, job, copy=True):
+        """-> (proof bytes, queued_ms, run_ms); releases the job.  With
+        copy=False the proof is not copied out of its pinned buffer and its
+        length is returned in place of the bytes."""
         buf, n = C.c_void_p(), C.c_size_t()
         q, r = C.c_double(), C.c_double()
         try:
             _check(_lib().nenya_wait(self._h, job, C.byref(buf), C.byref(n), C.byref(q), C.byref(r)))
-            return C.string_at(buf, n.value), q.value, r.value
+            return (C.string_at(buf, n.value) if copy else n.value), q.value, r.value
         finally:
             _lib().nenya_job_release(self._h, job)
             self._keep.pop(job, None)

@@ -4,6 +4,7 @@
 
 #include "../../include/nenya.h"
 #include "server.hpp"
+#include "verify.hpp"
 
 using namespace nenya;
 
@@ -88,6 +89,16 @@ static int general_common(const nenya_params* p, const void* w, size_t n, uint8_
     } else {
       emit(v, proof, len);
     }
+    return NENYA_OK;
+  });
+}
+
+template <class F> static int verify_common(const nenya_params* p, int kind, int* verdict, F&& f) {
+  return guard([&] {
+    if (!p || !p->impl || p->impl->kind != kind || !verdict) throw Error(NENYA_E_ARG, "bad argument");
+    auto* np = const_cast<nenya_params*>(p);
+    std::lock_guard<std::mutex> g(np->mu);
+    *verdict = f(p->impl);
     return NENYA_OK;
   });
 }
@@ -260,6 +271,52 @@ int nenya_prove_general_device(const nenya_params* p, const void* d_w, size_t n,
 int nenya_prove_general_device_view(const nenya_params* p, const void* d_w, size_t n, const uint8_t** proof,
                                     size_t* len) {
   return general_common(p, d_w, n, nullptr, len, true, proof);
+}
+
+int nenya_verify_simd(const nenya_params* p, const uint32_t* ar, const uint32_t* ac, const void* av, size_t an,
+                      const uint32_t* br, const uint32_t* bc, const void* bv, size_t bn, const uint32_t* cr,
+                      const uint32_t* cc, const void* cv, size_t cn, const uint8_t* proof, size_t len, int* verdict) {
+  return verify_common(p, 0, verdict, [&](ParamsBase* b) -> int {
+    if (!proof && len) throw Error(NENYA_E_ARG, "null proof");
+    if ((an && (!ar || !ac || !av)) || (bn && (!br || !bc || !bv)) || (cn && (!cr || !cc || !cv)))
+      throw Error(NENYA_E_ARG, "null matrix array");
+    auto run = [&](auto* sp) {
+      size_t n = size_t(1) << sp->log_n, m = size_t(1) << sp->log_m;
+      HostCoo A{n, m, an, ar, ac, av}, B{n, m, bn, br, bc, bv}, Cm{n, m, cn, cr, cc, cv};
+      using T = typename std::remove_pointer_t<decltype(sp)>::Width;
+      return Verifier<T>::simd(*sp, A, B, Cm, proof, len);
+    };
+    switch (b->k) {
+      case 8: return run(static_cast<SimdParams<W<8>>*>(b));
+      case 16: return run(static_cast<SimdParams<W<16>>*>(b));
+      case 32: return run(static_cast<SimdParams<W<32>>*>(b));
+      default: return run(static_cast<SimdParams<W<64>>*>(b));
+    }
+  });
+}
+
+int nenya_verify_mimd(const nenya_params* p, const uint8_t* proof, size_t len, int* verdict) {
+  return verify_common(p, 2, verdict, [&](ParamsBase* b) -> int {
+    if (!proof && len) throw Error(NENYA_E_ARG, "null proof");
+    switch (b->k) {
+      case 8: return Verifier<W<8>>::mimd(*static_cast<MimdParams<W<8>>*>(b), proof, len);
+      case 16: return Verifier<W<16>>::mimd(*static_cast<MimdParams<W<16>>*>(b), proof, len);
+      case 32: return Verifier<W<32>>::mimd(*static_cast<MimdParams<W<32>>*>(b), proof, len);
+      default: return Verifier<W<64>>::mimd(*static_cast<MimdParams<W<64>>*>(b), proof, len);
+    }
+  });
+}
+
+int nenya_verify_general(const nenya_params* p, const uint8_t* proof, size_t len, int* verdict) {
+  return verify_common(p, 1, verdict, [&](ParamsBase* b) -> int {
+    if (!proof && len) throw Error(NENYA_E_ARG, "null proof");
+    switch (b->k) {
+      case 8: return Verifier<W<8>>::general(*static_cast<GeneralParams<W<8>>*>(b), proof, len);
+      case 16: return Verifier<W<16>>::general(*static_cast<GeneralParams<W<16>>*>(b), proof, len);
+      case 32: return Verifier<W<32>>::general(*static_cast<GeneralParams<W<32>>*>(b), proof, len);
+      default: return Verifier<W<64>>::general(*static_cast<GeneralParams<W<64>>*>(b), proof, len);
+    }
+  });
 }
 
 int nenya_set_timing(nenya_params* p, int enabled) {

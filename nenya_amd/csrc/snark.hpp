@@ -132,6 +132,7 @@ inline void check_coo(const HostCoo& m, size_t rows, size_t cols) {
 // SIMD (simd.rs:220-373)
 // ============================================================================
 template <class T> struct SimdParams : ParamsBase {
+  using Width = T;
   using L = typename T::L;
   using G = Gr<T>;
   int log_n = 0, log_m = 0;
@@ -728,6 +729,28 @@ template <class T> struct GeneralParams : ParamsBase {
     eptr.release(); ecode.release(); eval.release();
   }
 
+  // build_layer_rows (general.rs:233-271): rows[l][q] (32 x nb) from
+  // eqs = {eq(s1), gamma eq(s1), gamma^2 eq(s1)} (3 x nb), on e's stream
+  void layer_rows(Engine<T>& e, const L* eqs, L* rows) {
+    if constexpr (kMx<T>) {
+      // G[w][p][i] = eqs[w][p] l_beta_i on the matrix cores, then the gather
+      DBuf<L> G(e.pool, 3 * nb * 16 * E);
+      const uint64_t nx = 3 * nb, ntiles = (nx + 31) / 32;
+      e.launch("k_mulfix_mx", nx * 17.0 * Engine<T>::EB, nx * 16.0 * E * E, (k_mulfix_mx<T>),
+               dim3(grid_for(ntiles, 4, 32), 16), dim3(256), (const uint32_t*)e.d_lbfrag.p, eqs, nx, G.p);
+      const double ne = (double)(mr[0].nnz + mr[1].nnz + mr[2].nnz);
+      e.launch("k_rows_gather", ne * (Engine<T>::EB + 8) + 32.0 * nb * Engine<T>::EB, ne * E, (k_rows_gather<T>),
+               dim3(grid_for(32 * nb, 8, 1u << 16)), dim3(256), (const uint64_t*)eptr.p, (const uint32_t*)ecode.p,
+               (const L*)eval.p, (const L*)G.p, (uint64_t)nb, rows, (uint64_t)(32 * nb));
+    } else {
+      double tc = (double)(ncell[0] + ncell[1] + ncell[2]);
+      e.launch("k_spmv_gr", tc * 2.0 * Engine<T>::EB, tc * 16 * Engine<T>::PM, (k_spmv_gr<T>),
+               dim3(grid_for(32 * nb, GPB)), dim3(GB), (const uint64_t*)rptr.p, (const uint32_t*)rdk.p,
+               (const uint32_t*)rsel.p, (const uint32_t*)ridx.p, (const L*)dhat.p, eqs, (uint64_t)nb, rows,
+               (uint64_t)(32 * nb));
+    }
+  }
+
   ProofView prove_device(Ctx<T>& cx, const L* w /* device, N limbs */) {
     Engine<T>& e = *cx.eng;
     Pool& pool = cx.pool;
@@ -835,23 +858,7 @@ template <class T> struct GeneralParams : ParamsBase {
       e.scale(dg2.p, eq1.p, eqs.p + 2 * nb * E, nb);
     }
     DBuf<L> rows(&pool, 32 * nb * E);
-    if constexpr (kMx<T>) {
-      // G[w][p][i] = eqs[w][p] l_beta_i on the matrix cores, then the gather
-      DBuf<L> G(&pool, 3 * nb * 16 * E);
-      const uint64_t nx = 3 * nb, ntiles = (nx + 31) / 32;
-      e.launch("k_mulfix_mx", nx * 17.0 * Engine<T>::EB, nx * 16.0 * E * E, (k_mulfix_mx<T>),
-               dim3(grid_for(ntiles, 4, 32), 16), dim3(256), (const uint32_t*)e.d_lbfrag.p, (const L*)eqs.p, nx, G.p);
-      const double ne = (double)(mr[0].nnz + mr[1].nnz + mr[2].nnz);
-      e.launch("k_rows_gather", ne * (Engine<T>::EB + 8) + 32.0 * nb * Engine<T>::EB, ne * E, (k_rows_gather<T>),
-               dim3(grid_for(32 * nb, 8, 1u << 16)), dim3(256), (const uint64_t*)eptr.p, (const uint32_t*)ecode.p,
-               (const L*)eval.p, (const L*)G.p, (uint64_t)nb, rows.p, (uint64_t)(32 * nb));
-    } else {
-      double tc = (double)(ncell[0] + ncell[1] + ncell[2]);
-      e.launch("k_spmv_gr", tc * 2.0 * Engine<T>::EB, tc * 16 * Engine<T>::PM, (k_spmv_gr<T>),
-               dim3(grid_for(32 * nb, GPB)), dim3(GB), (const uint64_t*)rptr.p, (const uint32_t*)rdk.p,
-               (const uint32_t*)rsel.p, (const uint32_t*)ridx.p, (const L*)dhat.p, (const L*)eqs.p, (uint64_t)nb, rows.p,
-               (uint64_t)(32 * nb));
-    }
+    layer_rows(e, eqs.p, rows.p);
     eqs.release();
     stg.mark("layer_rows");
     // 10. linear check over the 32 (rho_l, row_l) pairs; rho is low-degree
