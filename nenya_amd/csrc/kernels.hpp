@@ -285,6 +285,41 @@ __global__ void __launch_bounds__(GB) k_eq_level(const LT<T>* r, const LT<T>* pr
   }
 }
 
+// The first nl levels of build_eq_table in one block of 32 groups (levels
+// of at most 32 products): from the entry 1, level l writes 2^(l+1) entries,
+// alternating a -> b -> a ... (level 0 reads nothing, so a needs no
+// initialisation); the table ends in b if nl is odd, else in a.
+constexpr int EQ_HEAD = 6;
+template <class T>
+__global__ void __launch_bounds__(1024) k_eq_head(const LT<T>* r, LT<T>* a, LT<T>* b, int nl) {
+  __shared__ __attribute__((aligned(16))) LT<T> sr[E];
+  __shared__ __attribute__((aligned(16))) LT<T> sh[32][E];
+  const int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  for (int l = 0; l < nl; l++) {
+    const LT<T>* prev = (l & 1) ? b : a;
+    LT<T>* next = (l & 1) ? a : b;
+    const bool ok = g < (1 << l);
+    if (threadIdx.x < 32) stage<T>(sr, r + (uint64_t)l * E, t);
+    CT<T> p[4] = {(CT<T>)(t == 0 ? 1 : 0), 0, 0, 0};
+    if (ok) {
+      if (l > 0) ld4<T>(prev + (uint64_t)g * E + 4 * t, p);
+      st4<T>(sh[g] + 4 * t, p);
+    }
+    __syncthreads();
+    Acc<T> acc;
+    acc.clear();
+    if (ok) acc_mul<T>(acc, sr, sh[g], t);
+    CT<T> o[4];
+    acc_final<T>(acc, t, o);
+    if (ok) {
+      CT<T> lo[4] = {p[0] - o[0], p[1] - o[1], p[2] - o[2], p[3] - o[3]};
+      st4<T>(next + (2 * (uint64_t)g + 1) * E + 4 * t, o);
+      st4<T>(next + (2 * (uint64_t)g) * E + 4 * t, lo);
+    }
+    __syncthreads();
+  }
+}
+
 // ============================================================================
 // Sumcheck round kernels: per-block raw partial sums -> reduced partials.
 // ============================================================================
@@ -979,21 +1014,85 @@ __global__ void __launch_bounds__(RAA_TB) k_raa_scan(LT<T>* buf, uint64_t n, con
 // ============================================================================
 // Merkle commitment over columns (merkle.rs:20-87, r2_pcs.rs:194-218)
 // ============================================================================
-// leaf[j] = BLAKE3(concat_i entry(i, j)); entry bytes es = W*sizeof(L); rows x n row-major.
-__global__ void k_hash_cols(const uint8_t* enc, uint64_t rows, uint64_t n, uint64_t es, uint32_t* leaves) {
-  uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  const uint8_t* base = enc + j * es;
-  uint64_t rowb = n * es;
-  auto word = [&](uint64_t w) -> uint32_t {
-    uint64_t o = 4 * w;
-    uint64_t r = o / es, off = o - r * es;
-    return *reinterpret_cast<const uint32_t*>(base + r * rowb + off);
-  };
-  uint32_t out[8];
-  b3::hash_words(rows * es, word, out);
-  for (int i = 0; i < 8; i++) leaves[j * 8 + i] = out[i];
+// Column leaves, chunk-parallel.  Entries are es = 2^esl bytes (a multiple of
+// 16), so a 64-byte block of a column is read as four 16-byte pieces with
+// shifts and masks; bytes at or past blen in the block read as 0.
+__device__ __forceinline__ void col_block(const uint8_t* base, uint64_t rowb, int esl, uint64_t o, uint32_t blen,
+                                          uint32_t m[16]) {
+  const uint64_t emask = (1ull << esl) - 1;
+#pragma unroll
+  for (int p = 0; p < 4; p++) {
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (16u * p < blen) {
+      const uint64_t q = o + 16 * p;
+      v = *reinterpret_cast<const uint4*>(base + (q >> esl) * rowb + (q & emask));
+    }
+    m[4 * p] = v.x; m[4 * p + 1] = v.y; m[4 * p + 2] = v.z; m[4 * p + 3] = v.w;
+  }
 }
+// Chaining value of chunk c of a len-byte column (or, with root, the hash of
+// a one-chunk column).
+__device__ __forceinline__ void col_chunk(const uint8_t* base, uint64_t rowb, int esl, uint64_t len, uint64_t c,
+                                          bool root, uint32_t out[8]) {
+  const uint64_t cbeg = c * 1024, clen = len - cbeg < 1024 ? len - cbeg : 1024;
+  const uint32_t nblk = (uint32_t)((clen + 63) / 64);
+  uint32_t cv[8], m[16], o[16];
+  b3::iv(cv);
+  for (uint32_t b = 0; b + 1 < nblk; b++) {
+    col_block(base, rowb, esl, cbeg + 64ull * b, 64, m);
+    b3::compress(cv, m, c, 64, b == 0 ? b3::CHUNK_START : 0, o);
+    for (int i = 0; i < 8; i++) cv[i] = o[i];
+  }
+  const uint32_t lb = nblk - 1, blen = (uint32_t)(clen - 64ull * lb);
+  col_block(base, rowb, esl, cbeg + 64ull * lb, blen, m);
+  const uint32_t fl = (lb == 0 ? b3::CHUNK_START : 0) | b3::CHUNK_END | (root ? b3::ROOT : 0);
+  b3::compress(cv, m, c, blen, fl, o);
+  for (int i = 0; i < 8; i++) out[i] = o[i];
+}
+// Thread per (column j, chunk c), columns fastest (n = 2^ln columns of
+// rows x es bytes, rows x n row-major).  One-chunk columns write their leaf;
+// longer ones write chunk chaining values cvs[j][c] for k_col_merge.
+__global__ void k_col_chunks(const uint8_t* enc, uint64_t rows, int ln, int esl, uint64_t nch, uint32_t* cvs,
+                             uint32_t* leaves) {
+  const uint64_t n = 1ull << ln, len = rows << esl, rowb = n << esl;
+  for (uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; id < n * nch;
+       id += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t j = id & (n - 1), c = id >> ln;
+    uint32_t out[8];
+    col_chunk(enc + (j << esl), rowb, esl, len, c, nch == 1, out);
+    uint32_t* dst = nch == 1 ? leaves + j * 8 : cvs + (j * nch + c) * 8;
+    for (int i = 0; i < 8; i++) dst[i] = out[i];
+  }
+}
+__device__ __forceinline__ void b3_parent(const uint32_t* l, const uint32_t* r, uint32_t flags, uint32_t out[8]) {
+  uint32_t key[8], m[16], o[16];
+  b3::iv(key);
+  for (int i = 0; i < 8; i++) { m[i] = l[i]; m[8 + i] = r[i]; }
+  b3::compress(key, m, 0, 64, b3::PARENT | flags, o);
+  for (int i = 0; i < 8; i++) out[i] = o[i];
+}
+// Leaf of each multi-chunk column from its chunk chaining values: BLAKE3's
+// left-complete tree is pairwise merging with an odd last node carried up;
+// the last parent is the root.  In place in the column's own CV slots.
+__global__ void k_col_merge(uint32_t* cvs, uint64_t n, uint64_t nch, uint32_t* leaves) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  uint32_t* cv = cvs + j * nch * 8;
+  uint64_t cnt = nch;
+  uint32_t o[8];
+  while (cnt > 2) {
+    for (uint64_t i = 0; i < cnt / 2; i++) {
+      b3_parent(cv + 16 * i, cv + 16 * i + 8, 0, o);
+      for (int k = 0; k < 8; k++) cv[8 * i + k] = o[k];
+    }
+    if (cnt & 1)
+      for (int k = 0; k < 8; k++) cv[8 * (cnt / 2) + k] = cv[8 * (cnt - 1) + k];
+    cnt = (cnt + 1) / 2;
+  }
+  b3_parent(cv, cv + 8, b3::ROOT, o);
+  for (int k = 0; k < 8; k++) leaves[j * 8 + k] = o[k];
+}
+
 // nodes: level-order, leaves first; next level = BLAKE3(left || right)
 __global__ void k_merkle_level(uint32_t* nodes, uint64_t start, uint64_t len) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1003,6 +1102,34 @@ __global__ void k_merkle_level(uint32_t* nodes, uint64_t start, uint64_t len) {
   b3::hash64(l, l + 8, out);
   uint32_t* o = nodes + (start + len + i) * 8;
   for (int k = 0; k < 8; k++) o[k] = out[k];
+}
+
+// The top levels of a Merkle tree in one block: level `len` <= MERKLE_TOP
+// nodes at `start` up to the root, staged through LDS, every level written
+// to `nodes`.
+constexpr int MERKLE_TOP = 1024;
+__global__ void __launch_bounds__(1024) k_merkle_top(uint32_t* nodes, uint64_t start, uint64_t len) {
+  __shared__ uint32_t a[MERKLE_TOP][8], b[MERKLE_TOP / 2][8];
+  for (uint64_t i = threadIdx.x; i < len; i += blockDim.x)
+    for (int k = 0; k < 8; k++) a[i][k] = nodes[(start + i) * 8 + k];
+  __syncthreads();
+  while (len > 1) {
+    const uint64_t half = len / 2;
+    for (uint64_t i = threadIdx.x; i < half; i += blockDim.x) {
+      uint32_t o[8];
+      b3::hash64(a[2 * i], a[2 * i + 1], o);
+      for (int k = 0; k < 8; k++) {
+        b[i][k] = o[k];
+        nodes[(start + len + i) * 8 + k] = o[k];
+      }
+    }
+    __syncthreads();
+    for (uint64_t i = threadIdx.x; i < half; i += blockDim.x)
+      for (int k = 0; k < 8; k++) a[i][k] = b[i][k];
+    __syncthreads();
+    start += len;
+    len = half;
+  }
 }
 
 // ---- one long message (Fiat-Shamir query derivation over m-bar) ----
@@ -1017,23 +1144,31 @@ __global__ void k_b3_chunks(const uint8_t* msg, uint64_t len, uint64_t nchunks, 
   uint32_t nblk = clen == 0 ? 1 : (uint32_t)((clen + 63) / 64);
   uint32_t cv[8], m[16], o[16];
   b3::iv(cv);
-  auto word = [&](uint64_t byte) -> uint32_t {
-    uint32_t w = 0;
-    for (int i = 0; i < 4; i++) {
-      uint64_t p = byte + i;
-      uint32_t v = p < len ? msg[p] : 0u;
-      w |= v << (8 * i);
+  // 16-byte pieces (the buffer is 16-byte aligned); the piece holding the
+  // message end is read byte by byte, with bytes past len as 0
+  auto block = [&](uint64_t byte) {
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+      const uint64_t q = byte + 16 * p;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (q + 16 <= len) {
+        v = *reinterpret_cast<const uint4*>(msg + q);
+      } else if (q < len) {
+        uint32_t w[4] = {0, 0, 0, 0};
+        for (int i = 0; i < 16 && q + i < len; i++) w[i / 4] |= (uint32_t)msg[q + i] << (8 * (i % 4));
+        v = make_uint4(w[0], w[1], w[2], w[3]);
+      }
+      m[4 * p] = v.x; m[4 * p + 1] = v.y; m[4 * p + 2] = v.z; m[4 * p + 3] = v.w;
     }
-    return w;
   };
   for (uint32_t b = 0; b + 1 < nblk; b++) {
-    for (int i = 0; i < 16; i++) m[i] = word(cbeg + 64 * (uint64_t)b + 4 * i);
+    block(cbeg + 64 * (uint64_t)b);
     b3::compress(cv, m, c, 64, b == 0 ? b3::CHUNK_START : 0, o);
     for (int i = 0; i < 8; i++) cv[i] = o[i];
   }
   uint32_t lb = nblk - 1;
   uint32_t blen = (uint32_t)(clen - 64 * (uint64_t)lb);
-  for (int i = 0; i < 16; i++) m[i] = word(cbeg + 64 * (uint64_t)lb + 4 * i);
+  block(cbeg + 64 * (uint64_t)lb);
   uint32_t fl = (lb == 0 ? b3::CHUNK_START : 0) | b3::CHUNK_END;
   if (nchunks == 1) {
     for (int i = 0; i < 8; i++) rootnode[i] = cv[i];
@@ -1045,6 +1180,27 @@ __global__ void k_b3_chunks(const uint8_t* msg, uint64_t len, uint64_t nchunks, 
   b3::compress(cv, m, c, blen, fl, o);
   for (int i = 0; i < 8; i++) cvs[c * 8 + i] = o[i];
 }
+// The last merge levels of a long message in one block: cnt <= MERKLE_TOP
+// chaining values down to the two children of the root node (kids).
+__global__ void __launch_bounds__(1024) k_b3_merge_top(const uint32_t* in, uint64_t cnt, uint32_t* kids) {
+  __shared__ uint32_t a[MERKLE_TOP][8], b[MERKLE_TOP / 2 + 1][8];
+  for (uint64_t i = threadIdx.x; i < cnt; i += blockDim.x)
+    for (int k = 0; k < 8; k++) a[i][k] = in[i * 8 + k];
+  __syncthreads();
+  while (cnt > 2) {
+    const uint64_t half = cnt / 2;
+    for (uint64_t i = threadIdx.x; i < half; i += blockDim.x) b3_parent(a[2 * i], a[2 * i + 1], 0, b[i]);
+    if (threadIdx.x == 0 && (cnt & 1))
+      for (int k = 0; k < 8; k++) b[half][k] = a[cnt - 1][k];
+    __syncthreads();
+    cnt = (cnt + 1) / 2;
+    for (uint64_t i = threadIdx.x; i < cnt; i += blockDim.x)
+      for (int k = 0; k < 8; k++) a[i][k] = b[i][k];
+    __syncthreads();
+  }
+  if (threadIdx.x < 16) kids[threadIdx.x] = a[threadIdx.x / 8][threadIdx.x % 8];
+}
+
 // One merge level of the left-complete BLAKE3 tree: pairs become parents, an
 // odd last node is carried up unchanged.
 __global__ void k_b3_merge(const uint32_t* in, uint64_t cnt, uint32_t* out) {

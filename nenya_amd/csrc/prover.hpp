@@ -409,13 +409,18 @@ template <class T> struct Engine {
   DBuf<L> eq_table(const G* r, size_t n) {
     size_t len = size_t(1) << n;
     DBuf<L> a(pool, len * E), b(pool, len * E);
-    d2d(a.p, d_one.p, E * sizeof(L));
-    if (n == 0) return a;
+    if (n == 0) {
+      d2d(a.p, d_one.p, E * sizeof(L));
+      return a;
+    }
     DBuf<L> rr = put(r, n);
-    L* prev = a.p;
-    L* next = b.p;
-    size_t cur = 1;
-    for (size_t l = 0; l < n; l++) {
+    // the narrow first levels in one block, then a launch per level
+    const int nh = (int)std::min<size_t>(n, EQ_HEAD);
+    launch("k_eq_head", 0, 0, (k_eq_head<T>), dim3(1), dim3(1024), (const L*)rr.p, a.p, b.p, nh);
+    L* prev = (nh & 1) ? b.p : a.p;
+    L* next = (nh & 1) ? a.p : b.p;
+    size_t cur = size_t(1) << nh;
+    for (size_t l = nh; l < n; l++) {
       launch("k_eq_level", cur * 3 * EB, cur * 32 * PM, (k_eq_level<T>), dim3(grid_for(cur, GPB)), dim3(GB), rr.p + l * E,
              prev, next, (uint64_t)cur);
       std::swap(prev, next);
@@ -498,15 +503,24 @@ template <class T> struct Engine {
     else raa<E>(pc, rowsbuf, rows, tmp.p, c.enc.p);
     size_t n = P.n;  // a power of two
     c.nodes = DBuf<uint32_t>(pool, (2 * n - 1) * 8);
-    launch("k_hash_cols", (double)rows * n * W * sizeof(L), 0, k_hash_cols, dim3(grid_for(n, 64)), dim3(64),
-           (const uint8_t*)c.enc.p, (uint64_t)rows, (uint64_t)n, (uint64_t)W * sizeof(L), c.nodes.p);
+    // leaves: thread per (column, 1 KiB chunk), then the per-column merge
+    const int esl = __builtin_ctzll((uint64_t)W * sizeof(L)), ln = __builtin_ctzll((uint64_t)n);
+    const uint64_t colb = (uint64_t)rows * W * sizeof(L), nch = (colb + 1023) / 1024;
+    DBuf<uint32_t> cvs(pool, nch > 1 ? n * nch * 8 : 8);
+    launch("k_col_chunks", (double)rows * n * W * sizeof(L), 0, k_col_chunks, dim3(grid_for(n * nch, 64, 1u << 16)),
+           dim3(64), (const uint8_t*)c.enc.p, (uint64_t)rows, ln, esl, nch, cvs.p, c.nodes.p);
+    if (nch > 1)
+      launch("k_col_merge", (double)n * nch * 32, 0, k_col_merge, dim3(grid_for(n, 64)), dim3(64), cvs.p, (uint64_t)n,
+             nch, c.nodes.p);
     size_t start = 0, len = n;
-    while (len > 1) {
+    while (len > (size_t)MERKLE_TOP) {
       launch("k_merkle_level", len * 32.0 + len * 16.0, 0, k_merkle_level, dim3(grid_for(len / 2, 128)), dim3(128),
              c.nodes.p, (uint64_t)start, (uint64_t)len);
       start += len;
       len /= 2;
     }
+    if (len > 1)
+      launch("k_merkle_top", len * 64.0, 0, k_merkle_top, dim3(1), dim3(1024), c.nodes.p, (uint64_t)start, (uint64_t)len);
     d2h(c.root, c.nodes.p + (2 * n - 2) * 8, 32);
     return c;
   }
@@ -548,11 +562,15 @@ template <class T> struct Engine {
     uint32_t* a = cv.p;
     uint32_t* b = cv2.p;
     uint64_t cnt = nch;
-    while (cnt > 2) {
+    while (cnt > (uint64_t)MERKLE_TOP) {
       launch("k_b3_merge", cnt * 32.0, 0, k_b3_merge, dim3(grid_for(cnt / 2 + 1, 64, 1u << 20)), dim3(64),
              (const uint32_t*)a, cnt, b);
       std::swap(a, b);
       cnt = (cnt + 1) / 2;
+    }
+    if (cnt > 2) {
+      launch("k_b3_merge_top", cnt * 32.0, 0, k_b3_merge_top, dim3(1), dim3(1024), (const uint32_t*)a, cnt, b);
+      std::swap(a, b);
     }
     uint32_t kids[16];
     d2h(kids, a, 64);
