@@ -685,21 +685,17 @@ __device__ __forceinline__ void teich_lift_block(const LT<T>* src, LT<T>* el, co
     sred[dl >> 2][t][dl & 3] = sum;
     __syncthreads();
     if (tid < E) {
+      // branch-free: every term is read (clamped index) and masked
       const int u = tid >> 2, c = tid & 3;
       auto hc = [&](int s, int cc) -> C {  // hi of position 32+s after the 61/62 cascade
-        C r = sred[1][s][cc];
-        if (s < 2) r -= sred[1][s + 29][cc];
-        return r;
+        const C w = s < 2 ? ~(C)0 : (C)0;
+        return sred[1][s][cc] - (sred[1][s < 2 ? s + 29 : 0][cc] & w);
       };
-      C lo = sred[0][u][c] - sred[1][u][c];
-      if (u >= 1) lo -= hc(u - 1, c);
-      if (u >= 3) lo -= hc(u - 3, c);
-      C out;
-      if (c == 0) out = lo + hc(u, 3);
-      else if (c == 1) out = lo - hc(u, 0) + hc(u, 3);
-      else if (c == 2) out = lo - hc(u, 1);
-      else out = lo - hc(u, 2);
-      sx[tid] = out;
+      const C m1 = u >= 1 ? ~(C)0 : (C)0, m3 = u >= 3 ? ~(C)0 : (C)0;
+      const C lo = sred[0][u][c] - sred[1][u][c] - (hc(u >= 1 ? u - 1 : 0, c) & m1) - (hc(u >= 3 ? u - 3 : 0, c) & m3);
+      // -alpha * hi(u): c0 += h3, c1 += h3 - h0, c2 -= h1, c3 -= h2
+      const C ma = c <= 1 ? ~(C)0 : (C)0, mb = c >= 1 ? ~(C)0 : (C)0;
+      sx[tid] = lo + (hc(u, 3) & ma) - (hc(u, (c + 3) & 3) & mb);
     }
     __syncthreads();
   }
