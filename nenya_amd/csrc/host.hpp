@@ -250,6 +250,7 @@ template <class T> struct Consts {
   std::vector<L> frob_t;  // [j][o]: matrix of sigma^-(k-1), transposed
   std::vector<uint32_t> b2frag;  // MFMA fragments of [b_s b_s'] (128 x 1024), k <= 32
   std::vector<uint32_t> lbfrag;  // MFMA fragments of x -> l_beta_i x for i < 16, k <= 32
+  std::vector<uint32_t> packfrag;  // MFMA fragments of pack_r2 o rot_L (128 x 32) for L < 32, k <= 32
   L inv15;
 
   Consts() {
@@ -323,6 +324,11 @@ template <class T> struct Consts {
       for (int a = 0; a < 32; a++)
         for (int b = a; b < 32; b++) b2[32 * a + b] = b2[32 * b + a] = basis[a] * basis[b];
       b2frag = mx_frags(K / 8, 32, [&](int row, int k) { return (uint32_t)b2[k].v[row]; });
+      // pack_r2(rot_L(x)) = sum_s x[(s + L) % 32] b_s = P_L x, P_L[row][k] = b_{(k - L) % 32}[row]
+      for (int L = 0; L < 32; L++) {
+        std::vector<uint32_t> f = mx_frags(K / 8, 1, [&](int row, int k) { return (uint32_t)basis[(k - L) & 31].v[row]; });
+        packfrag.insert(packfrag.end(), f.begin(), f.end());
+      }
       {  // self-check of mulmat_host against the product on a fixed element
         Gr<T> x;
         uint64_t sd = 0xD1B54A32D192ED03ull;

@@ -933,31 +933,34 @@ __global__ void __launch_bounds__(GB) k_rot_mbar(const LT<T>* e0, const LT<T>* r
 // RAA encoder (raa.rs:67-153), all rows of a commitment at once, W limbs per
 // entry.  Each pass is gather-and-scale then an inclusive prefix sum over the
 // codeword index, done as a segmented scan across many blocks:
-//   pass 1  v[i] = src[perm[i] / q] * d[i]  (+ per-segment sums)
-//   scan    exclusive scan of the segment sums (per row, per limb)
-//   pass 3  local inclusive scan of each segment from its offset
+//   gather  v[i] = src[perm[i] >> qs] * d[i]  (q = 2^qs) and per-segment sums
+//   segscan exclusive scan of the segment sums, one block per row
+//   scan    each block scans its segment (read once into registers) from its
+//           segment's offset
 // Encode = pass(msg -> t, perm1, d1, q) then pass(t -> z, perm2, d2, 1).
+// A segment is 32 entries per thread: RAA_TB / W threads share a limb.
 // ============================================================================
-constexpr int RAA_SEG = 256;  // codeword entries per segment
-constexpr int RAA_TB = 256;   // threads per block
+constexpr int RAA_TB = 256;  // threads per block
+template <int W> constexpr int raa_seg() { return 32 * (RAA_TB / W); }
 
 template <class T, int W>
 __global__ void __launch_bounds__(RAA_TB) k_raa_gather(const LT<T>* src, uint64_t src_row, const uint32_t* perm,
-                                                       const LT<T>* d, uint64_t q, uint64_t n, LT<T>* dst,
+                                                       const LT<T>* d, int qs, uint64_t n, LT<T>* dst,
                                                        CT<T>* segsum, uint64_t nseg) {
   constexpr int SUB = RAA_TB / W;
-  constexpr int PER = RAA_SEG / SUB;
+  constexpr int PER = 32;
   __shared__ CT<T> red[SUB][W];
   uint64_t row = blockIdx.y, seg = blockIdx.x;
   int s = threadIdx.x % W, sub = threadIdx.x / W;
   const LT<T>* sr = src + row * src_row * W;
   LT<T>* dr = dst + row * n * W;
-  uint64_t b = seg * RAA_SEG + (uint64_t)sub * PER;
+  uint64_t b = seg * raa_seg<W>() + (uint64_t)sub * PER;
   CT<T> acc = 0;
+#pragma unroll 8
   for (int k = 0; k < PER; k++) {
     uint64_t i = b + k;
     if (i < n) {
-      CT<T> v = (CT<T>)sr[(uint64_t)(perm[i] / q) * W + s] * (CT<T>)d[i];
+      CT<T> v = (CT<T>)sr[(uint64_t)(perm[i] >> qs) * W + s] * (CT<T>)d[i];
       dr[i * W + s] = (LT<T>)v;
       acc += v;
     }
@@ -969,41 +972,52 @@ __global__ void __launch_bounds__(RAA_TB) k_raa_gather(const LT<T>* src, uint64_
     segsum[(row * nseg + seg) * W + s] = acc;
   }
 }
+// Exclusive scan of one row's segment sums (grid: rows): each of the
+// 1024 / W threads of a limb takes a contiguous range of segments.
 template <class T, int W>
-__global__ void k_raa_segscan(CT<T>* segsum, uint64_t rows, uint64_t nseg) {
-  uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= rows * W) return;
-  uint64_t row = id / W, s = id % W;
+__global__ void __launch_bounds__(1024) k_raa_segscan(CT<T>* segsum, uint64_t nseg) {
+  constexpr int SUBS = 1024 / W;
+  __shared__ CT<T> part[SUBS][W];
+  const int s = threadIdx.x % W, sub = threadIdx.x / W;
+  const uint64_t per = (nseg + SUBS - 1) / SUBS, g0 = (uint64_t)sub * per;
+  const uint64_t g1 = g0 + per < nseg ? g0 + per : nseg;
+  CT<T>* base = segsum + (uint64_t)blockIdx.x * nseg * W;
+  CT<T> tot = 0;
+  for (uint64_t g = g0; g < g1; g++) tot += base[g * W + s];
+  part[sub][s] = tot;
+  __syncthreads();
   CT<T> run = 0;
-  for (uint64_t g = 0; g < nseg; g++) {
-    CT<T>& x = segsum[(row * nseg + g) * W + s];
-    CT<T> v = x;
-    x = run;
+  for (int k = 0; k < sub; k++) run += part[k][s];
+  for (uint64_t g = g0; g < g1; g++) {
+    const CT<T> v = base[g * W + s];
+    base[g * W + s] = run;
     run += v;
   }
 }
 template <class T, int W>
 __global__ void __launch_bounds__(RAA_TB) k_raa_scan(LT<T>* buf, uint64_t n, const CT<T>* segsum, uint64_t nseg) {
   constexpr int SUB = RAA_TB / W;
-  constexpr int PER = RAA_SEG / SUB;
+  constexpr int PER = 32;
   __shared__ CT<T> red[SUB][W];
   uint64_t row = blockIdx.y, seg = blockIdx.x;
   int s = threadIdx.x % W, sub = threadIdx.x / W;
   LT<T>* br = buf + row * n * W;
-  uint64_t b = seg * RAA_SEG + (uint64_t)sub * PER;
+  uint64_t b = seg * raa_seg<W>() + (uint64_t)sub * PER;
+  CT<T> v[PER];
   CT<T> tot = 0;
-  for (int k = 0; k < PER; k++)
-    if (b + k < n) tot += br[(b + k) * W + s];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    v[k] = b + k < n ? (CT<T>)br[(b + k) * W + s] : (CT<T>)0;
+    tot += v[k];
+  }
   red[sub][s] = tot;
   __syncthreads();
   CT<T> run = segsum[(row * nseg + seg) * W + s];
   for (int k = 0; k < sub; k++) run += red[k][s];
+#pragma unroll
   for (int k = 0; k < PER; k++) {
-    uint64_t i = b + k;
-    if (i < n) {
-      run += br[i * W + s];
-      br[i * W + s] = (LT<T>)run;
-    }
+    run += v[k];
+    if (b + k < n) br[(b + k) * W + s] = (LT<T>)run;
   }
 }
 

@@ -117,11 +117,14 @@ template <class T> __device__ __forceinline__ void mulmat_frag(const CT<T> (*v)[
   }
 }
 
-// Standalone M_r fragments for an r already in device memory (one block).
+// M_r fragments for r already in device memory; block b handles r + b
+// (fragments at frag + b * ND * 4096 words).
 template <class T>
 __global__ void __launch_bounds__(256) k_mulmat(const LT<T>* r, uint32_t* frag) {
   using C = CT<T>;
   __shared__ C v[32][E];
+  r += (uint64_t)blockIdx.x * E;
+  frag += (uint64_t)blockIdx.x * kND<T> * 4096;
   if (threadIdx.x < E) v[0][threadIdx.x] = (C)r[threadIdx.x];
   __syncthreads();
   mulmat_chain<C>(v);
@@ -487,6 +490,134 @@ __global__ void __launch_bounds__(256) k_cgemm(const uint32_t* frag, const LT<T>
         st4<T>(out + p * E + 32 * rb + 8 * j + 4 * h, x);
       }
     }
+  }
+}
+
+// m-bars on the matrix cores (pcs.rs:554-561, 984-1001):
+//   out[l][j] = sum_{i < rows} e_i X[l * lstride + i * k2 + j]
+// with the M_{e_i} fragments (frag + i * ND * 4096 words, from k_mulmat).
+// A block takes 32 consecutive outputs of one l; wave w computes output rows
+// 32w..32w+31 and keeps one accumulator per digit class over all i (the
+// i32 accumulators wrap mod 2^32, so the sum over i is exact).
+// blockIdx.y = s of nsplit takes rows [s*rows/nsplit, (s+1)*rows/nsplit) and
+// writes its partial sums to out + s * nl * k2 elements (summed afterwards by
+// k_sum_split when nsplit > 1).
+template <class T>
+__global__ void __launch_bounds__(256) k_mbar_mx(const uint32_t* frag, const LT<T>* X, uint64_t rows, uint64_t k2,
+                                                 uint64_t lstride, uint64_t nl, LT<T>* out) {
+  using C = CT<T>;
+  constexpr int ND = kND<T>;
+  const int l64 = threadIdx.x & 63, col = l64 & 31, h = l64 >> 5, rb = threadIdx.x >> 6;
+  const uint64_t tpl = (k2 + 31) / 32, ntiles = tpl * nl;
+  const uint64_t i0 = rows * blockIdx.y / gridDim.y, i1 = rows * (blockIdx.y + 1) / gridDim.y;
+  out += (uint64_t)blockIdx.y * nl * k2 * E;
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint64_t l = tile / tpl, j = (tile % tpl) * 32 + col;
+    const bool ok = j < k2;
+    const LT<T>* xl = X + (l * lstride + (ok ? j : 0)) * E;
+    v16i acc[ND];
+#pragma unroll
+    for (int d = 0; d < ND; d++) acc[d] = (v16i)0;
+    for (uint64_t i = i0; i < i1; i++) {
+      const v4i* A = reinterpret_cast<const v4i*>(frag + i * ND * 4096);
+      const LT<T>* x = xl + i * k2 * E;
+#pragma unroll
+      for (int kb = 0; kb < 4; kb++) {
+        uint32_t v[16];
+        if (ok) {
+          ld16k<T>(x, kb, h, v);
+        } else {
+#pragma unroll
+          for (int b = 0; b < 16; b++) v[b] = 0;
+        }
+        v4i B[ND];
+        planes16<ND>(v, B);
+#pragma unroll
+        for (int pd = 0; pd < ND; pd++) {
+          const v4i a = A[frag_at(pd, kb, rb) * 64 + l64];
+#pragma unroll
+          for (int q = 0; pd + q < ND; q++) acc[pd + q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, B[q], acc[pd + q], 0, 0, 0);
+        }
+      }
+    }
+    if (ok) {
+#pragma unroll
+      for (int jj = 0; jj < 4; jj++) {
+        C y4[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          uint32_t y = 0;
+#pragma unroll
+          for (int d = 0; d < ND; d++) y += (uint32_t)acc[d][4 * jj + c] << (8 * d);
+          y4[c] = (C)y;
+        }
+        st4<T>(out + (l * k2 + j) * E + 32 * rb + 8 * jj + 4 * h, y4);
+      }
+    }
+  }
+}
+
+// pack_r2 on the matrix cores: out[r * n + e] = pack_r2(rot_{r + rot0}(slots[e]))
+// = P_L slots[e] with L = (r + rot0) mod 32 (frag: the 32 P_L, ND * 1024
+// words each).  Block: 32 elements, wave w output rows 32w..32w+31; the
+// slots are read once for all nrot rotations.
+template <class T>
+__global__ void __launch_bounds__(256) k_pack_mx(const uint32_t* frag, const LT<T>* slots, uint64_t n, int nrot,
+                                                 int rot0, LT<T>* out) {
+  using C = CT<T>;
+  constexpr int ND = kND<T>;
+  const int l64 = threadIdx.x & 63, col = l64 & 31, h = l64 >> 5, rb = threadIdx.x >> 6;
+  const v4i* A = reinterpret_cast<const v4i*>(frag);
+  const uint64_t ntiles = (n + 31) / 32;
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint64_t e = tile * 32 + col;
+    const bool ok = e < n;
+    uint32_t v[16];
+    if (ok) {
+      ld16k<T>(slots + e * 32, 0, h, v);
+    } else {
+#pragma unroll
+      for (int b = 0; b < 16; b++) v[b] = 0;
+    }
+    v4i B[ND];
+    planes16<ND>(v, B);
+    for (int r = 0; r < nrot; r++) {
+      const int L = (r + rot0) & 31;
+      v16i acc[ND];
+#pragma unroll
+      for (int d = 0; d < ND; d++) acc[d] = (v16i)0;
+#pragma unroll
+      for (int pd = 0; pd < ND; pd++) {
+        const v4i a = A[((size_t)L * ND * 4 + pd * 4 + rb) * 64 + l64];
+#pragma unroll
+        for (int q = 0; pd + q < ND; q++) acc[pd + q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, B[q], acc[pd + q], 0, 0, 0);
+      }
+      if (ok) {
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+          C y4[4];
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            uint32_t y = 0;
+#pragma unroll
+            for (int d = 0; d < ND; d++) y += (uint32_t)acc[d][4 * jj + c] << (8 * d);
+            y4[c] = (C)y;
+          }
+          st4<T>(out + ((uint64_t)r * n + e) * E + 32 * rb + 8 * jj + 4 * h, y4);
+        }
+      }
+    }
+  }
+}
+
+// out[e] = sum_{s < nsplit} part[s * cnt + e] (limbwise, cnt elements)
+template <class T>
+__global__ void k_sum_split(const LT<T>* part, uint64_t nsplit, uint64_t cnt, LT<T>* out) {
+  const uint64_t nl = cnt * E;
+  for (uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; id < nl; id += (uint64_t)gridDim.x * blockDim.x) {
+    CT<T> a = 0;
+    for (uint64_t s = 0; s < nsplit; s++) a += (CT<T>)part[s * nl + id];
+    out[id] = (LT<T>)a;
   }
 }
 

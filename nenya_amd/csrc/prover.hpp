@@ -179,6 +179,7 @@ template <class T> struct Engine {
   DBuf<L> d_frob;  // sigma^-(k-1), transposed [j][o]
   DBuf<uint32_t> d_b2frag;  // MFMA fragments of the packing-basis products
   DBuf<uint32_t> d_lbfrag;  // MFMA fragments of x -> l_beta_i x
+  DBuf<uint32_t> d_packfrag;  // MFMA fragments of pack_r2 o rot_L
   KProf* prof = nullptr;
 
   // Launch on the engine's stream, bracketed by events when profiling.
@@ -217,6 +218,8 @@ template <class T> struct Engine {
     if (!K.b2frag.empty()) {
       d_b2frag = DBuf<uint32_t>(pool, K.b2frag.size());
       h2d(d_b2frag.p, K.b2frag.data(), K.b2frag.size() * 4);
+      d_packfrag = DBuf<uint32_t>(pool, K.packfrag.size());
+      h2d(d_packfrag.p, K.packfrag.data(), K.packfrag.size() * 4);
       d_lbfrag = DBuf<uint32_t>(pool, K.lbfrag.size());
       h2d(d_lbfrag.p, K.lbfrag.data(), K.lbfrag.size() * 4);
     }
@@ -308,8 +311,15 @@ template <class T> struct Engine {
 
   // ---------------- element kernels ----------------
   void pack(const L* slots, L* out, uint64_t n, int nrot, int rot0 = 0) {
-    launch("k_pack", n * nrot * (32.0 * sizeof(L) + EB), n * nrot * 4096.0, (k_pack<T>), dim3(grid_for(n * nrot, GPB)),
-           dim3(GB), slots, d_basis.p, out, n, nrot, rot0);
+    if constexpr (kMx<T>) {
+      const uint64_t ntiles = (n + 31) / 32;
+      launch("k_pack_mx", n * (32.0 * sizeof(L) + nrot * EB), n * nrot * 4096.0, (k_pack_mx<T>),
+             dim3((unsigned)std::min<uint64_t>(ntiles, 8192)), dim3(256), (const uint32_t*)d_packfrag.p, slots, n, nrot,
+             rot0, out);
+    } else {
+      launch("k_pack", n * nrot * (32.0 * sizeof(L) + EB), n * nrot * 4096.0, (k_pack<T>), dim3(grid_for(n * nrot, GPB)),
+             dim3(GB), slots, d_basis.p, out, n, nrot, rot0);
+    }
   }
   void psi(const L* x, L* slots, uint64_t n) {
     launch("k_psi", n * (EB + 32.0 * sizeof(L)), n * 16 * 32 * 16.0, (k_psi<T>), dim3(grid_for(n, GPB)), dim3(GB), x,
@@ -528,7 +538,9 @@ template <class T> struct Engine {
   // RAA encode of `rows` rows (raa.rs:106-153 / 67-98), W limbs per entry.
   template <int W> void raa(const PcsDev<T>& pc, const L* msg, size_t rows, L* tmp, L* out) {
     const PcsParams& P = pc.P;
-    uint64_t nseg = (P.n + RAA_SEG - 1) / RAA_SEG;
+    if (P.q & (P.q - 1)) throw Error(NENYA_E_SHAPE, "RAA repetition must be a power of two");
+    const int qs = __builtin_ctzll((uint64_t)P.q);
+    uint64_t nseg = (P.n + raa_seg<W>() - 1) / raa_seg<W>();
     DBuf<C> seg(pool, rows * nseg * W);
     dim3 grid((unsigned)nseg, (unsigned)rows);
     double eb = W * sizeof(L), rowb = (double)rows * P.n * eb;
@@ -537,9 +549,8 @@ template <class T> struct Engine {
       L* dst = pass == 0 ? tmp : out;
       launch("k_raa_gather", rowb * 2.0, 0, (k_raa_gather<T, W>), grid, dim3(RAA_TB), src,
              (uint64_t)(pass == 0 ? P.k2 : P.n), (const uint32_t*)(pass == 0 ? pc.perm1.p : pc.perm2.p),
-             (const L*)(pass == 0 ? pc.d1.p : pc.d2.p), (uint64_t)(pass == 0 ? P.q : 1), (uint64_t)P.n, dst, seg.p, nseg);
-      launch("k_raa_segscan", 0, 0, (k_raa_segscan<T, W>), dim3(grid_for(rows * W, 256)), dim3(256), seg.p, (uint64_t)rows,
-             nseg);
+             (const L*)(pass == 0 ? pc.d1.p : pc.d2.p), pass == 0 ? qs : 0, (uint64_t)P.n, dst, seg.p, nseg);
+      launch("k_raa_segscan", 0, 0, (k_raa_segscan<T, W>), dim3((unsigned)rows), dim3(1024), seg.p, nseg);
       launch("k_raa_scan", rowb * 2.0, 0, (k_raa_scan<T, W>), grid, dim3(RAA_TB), dst, (uint64_t)P.n, (const C*)seg.p, nseg);
     }
   }
@@ -622,6 +633,23 @@ template <class T> struct Engine {
 
   // Pcs::open / open_stacked (pcs.rs:547-591, 734-779) and R2Pcs::open /
   // open_stacked (r2_pcs.rs:297-338, 512-559).  weights: s elements or null (s=1).
+  // out[l][j] = sum_{i < rows} e[i] X[l * lstride + i * k2 + j] on the matrix
+  // cores: fragments of every e[i], then one pass over X.
+  void mbar_mx(const L* e, uint64_t rows, const L* X, uint64_t k2, uint64_t lstride, uint64_t nl, L* out) {
+    DBuf<uint32_t> fr(pool, (size_t)rows * kND<T> * 4096);
+    launch("k_mulmat", rows * (EB + kND<T> * 16384.0), 0, (k_mulmat<T>), dim3((unsigned)rows), dim3(256), e, fr.p);
+    const uint64_t tiles = nl * ((k2 + 31) / 32);
+    // split the row range over blocks until there are ~2048 blocks
+    const uint64_t nsplit = std::max<uint64_t>(1, std::min<uint64_t>(rows, (2048 + tiles - 1) / tiles));
+    DBuf<L> part(pool, nsplit > 1 ? nsplit * nl * k2 * E : 1);
+    launch("k_mbar_mx", (double)nl * rows * k2 * EB + nsplit * nl * k2 * EB, (double)nl * rows * k2 * E * E,
+           (k_mbar_mx<T>), dim3((unsigned)std::min<uint64_t>(tiles, 65535), (unsigned)nsplit), dim3(256),
+           (const uint32_t*)fr.p, X, rows, k2, lstride, nl, nsplit > 1 ? part.p : out);
+    if (nsplit > 1)
+      launch("k_sum_split", (nsplit + 1.0) * nl * k2 * EB, 0, (k_sum_split<T>), dim3(grid_for(nl * k2 * E, 256, 4096)),
+             dim3(256), (const L*)part.p, nsplit, nl * k2, out);
+  }
+
   G open(const PcsDev<T>& pc, const CommitDev<T>& c, const G* z, const G* weights, size_t s, Sink& out) {
     const PcsParams& P = pc.P;
     size_t lk1 = (size_t)__builtin_ctzll(P.k1), lk2 = (size_t)__builtin_ctzll(P.k2);
@@ -637,7 +665,9 @@ template <class T> struct Engine {
     DBuf<L> mbar(pool, P.k2 * E);
     uint64_t rows = s * P.k1;
     double mbb = (rows * P.k2 + P.k2) * EB;
-    if (P.packed)
+    if constexpr (kMx<T>) {
+      mbar_mx(e0s.p, rows, c.msg, P.k2, 0, 1, mbar.p);
+    } else if (P.packed)
       launch("k_mbar", mbb, rows * P.k2 * 16 * PM, (k_mbar<T, 16>), dim3(grid_for(P.k2, GPB)), dim3(GB), (const L*)e0s.p,
              c.msg, rows, (uint64_t)P.k2, mbar.p, (uint64_t)0);
     else
@@ -658,9 +688,12 @@ template <class T> struct Engine {
     size_t lk1 = (size_t)__builtin_ctzll(P.k1), lk2 = (size_t)__builtin_ctzll(P.k2);
     DBuf<L> e0 = eq_table(z, lk1), e1 = eq_table(z + lk1, lk2);
     DBuf<L> mbar(pool, (size_t)nl * P.k2 * E);
-    launch("k_rot_mbar", (double)nl * P.k1 * P.k2 * EB, (double)nl * P.k1 * P.k2 * 16 * PM, (k_rot_mbar<T>),
-           dim3(grid_for((uint64_t)nl * P.k2, GPB)), dim3(GB), (const L*)e0.p, rho, nb, (uint64_t)P.k1, (uint64_t)P.k2, nl,
-           mbar.p);
+    if constexpr (kMx<T>)
+      mbar_mx(e0.p, P.k1, rho, P.k2, nb, nl, mbar.p);
+    else
+      launch("k_rot_mbar", (double)nl * P.k1 * P.k2 * EB, (double)nl * P.k1 * P.k2 * 16 * PM, (k_rot_mbar<T>),
+             dim3(grid_for((uint64_t)nl * P.k2, GPB)), dim3(GB), (const L*)e0.p, rho, nb, (uint64_t)P.k1, (uint64_t)P.k2,
+             nl, mbar.p);
     std::vector<G> ys = dots<32>(mbar.p, P.k2, nl, e1.p, P.k2);
     out.u64(nl);
     size_t per = P.k2 * E * sizeof(L);
