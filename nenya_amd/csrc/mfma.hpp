@@ -442,6 +442,31 @@ __global__ void __launch_bounds__(256) k_cmat(const uint32_t* ptr, const uint32_
   }
 }
 
+// MIMD rows: C_r[s][s'] = sum_{cells g of row r} val[g][s] * w[col[g]][s']
+// (cells of the union support with their 32 instance values, w the witness
+// slots [m][32]); one 32-lane group per row, lane s'.  With
+// Ahat[r,c] = sum_s a_s b_s and w_phi[c] = sum_s' w_s' b_s', the row's
+// sum_c Ahat[r,c] w_phi[c] is sum_{s,s'} C_r[s][s'] b_s b_s' (k_cgemm).
+template <class T>
+__global__ void __launch_bounds__(256) k_cmat_mimd(const uint64_t* ptr, const uint32_t* col, const LT<T>* val,
+                                                   const LT<T>* w, LT<T>* cmat, uint64_t n) {
+  using C = CT<T>;
+  const uint32_t lane = threadIdx.x & 31;
+  for (uint64_t r = (uint64_t)blockIdx.x * 8 + (threadIdx.x >> 5); r < n; r += (uint64_t)gridDim.x * 8) {
+    C acc[32];
+#pragma unroll
+    for (int s = 0; s < 32; s++) acc[s] = 0;
+    for (uint64_t g = ptr[r]; g < ptr[r + 1]; g++) {
+      const C wv = (C)w[(uint64_t)col[g] * 32 + lane];
+      const C vl = (C)val[g * 32 + lane];
+#pragma unroll
+      for (int s = 0; s < 32; s++) acc[s] += __shfl(vl, s, 32) * wv;
+    }
+#pragma unroll
+    for (int s = 0; s < 32; s++) cmat[r * 1024 + 32 * s + lane] = (LT<T>)acc[s];
+  }
+}
+
 // out[p] = sum_k B2[.][k] C_p[k] over k = 32s + s' (1024).  Block: 4 waves on
 // the same 32 block rows, wave w computing output rows 32w..32w+31; one
 // accumulator per digit class (K streams, so no Horner), combined at the end.

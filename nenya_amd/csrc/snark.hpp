@@ -323,9 +323,15 @@ template <class T> struct MimdParams : ParamsBase {
   DBuf<uint64_t> cptr;  // 3 x (n+1) row pointers (global cell indices)
   DBuf<uint32_t> ccol;
   DBuf<L> cval, dhat;
-  // matvec entries by row (k_spmv_gr layout) and M_gamma-row entries by column
+  // vector-ALU path (k = 64): matvec entries by row (k_spmv_gr layout) and
+  // M_gamma-row entries by column, over the packed cells dhat
   DBuf<uint64_t> mvptr, rwptr;
   DBuf<uint32_t> mvdk, mvsel, mvidx, rwdk, rwsel, rwidx;
+  // matrix-core path (k <= 32): M_gamma-row entries by column, one per
+  // nonzero instance value, code = y << 7 | w << 5 | s (k_rows_gather)
+  DBuf<uint64_t> eptr;
+  DBuf<uint32_t> ecode;
+  DBuf<L> eval;
   CtxBase& ctx0() override { return cx0; }
 
   // MimdSnarkParams::secure (mimd.rs:84-107) with pack_mimd_matrices
@@ -386,13 +392,46 @@ template <class T> struct MimdParams : ParamsBase {
     cptr = DBuf<uint64_t>(&pool, rowptr.size());
     ccol = DBuf<uint32_t>(&pool, std::max<size_t>(total, 1));
     cval = DBuf<L>(&pool, std::max<size_t>(total, 1) * 32);
-    dhat = DBuf<L>(&pool, std::max<size_t>(total, 1) * E);
     eng->h2d(cptr.p, rowptr.data(), rowptr.size() * 8);
     if (total) {
       eng->h2d(ccol.p, col.data(), total * 4);
       eng->h2d(cval.p, vals.data(), vals.size() * sizeof(L));
-      eng->pack(cval.p, dhat.p, total, 1);
     }
+    if constexpr (kMx<T>) {
+      if (n > (size_t(1) << 25)) throw Error(NENYA_E_SHAPE, "too many rows for the row-entry code");
+      std::vector<uint64_t> ptr(m + 1, 0);
+      size_t g = 0;
+      for (int w = 0; w < 3; w++)
+        for (size_t i = 0; i < ncell[w]; i++, g++)
+          for (int sl = 0; sl < 32; sl++)
+            if (vals[g * 32 + sl]) ptr[col[g] + 1]++;
+      for (size_t h = 0; h < m; h++) ptr[h + 1] += ptr[h];
+      std::vector<uint64_t> pos(ptr.begin(), ptr.end() - 1);
+      std::vector<uint32_t> code(std::max<uint64_t>(ptr[m], 1));
+      std::vector<L> ev(std::max<uint64_t>(ptr[m], 1));
+      g = 0;
+      for (int w = 0; w < 3; w++)
+        for (size_t i = 0; i < ncell[w]; i++, g++)
+          for (int sl = 0; sl < 32; sl++) {
+            const L v = vals[g * 32 + sl];
+            if (!v) continue;
+            const uint64_t o = pos[col[g]]++;
+            code[o] = (uint32_t)(((uint64_t)cellrow[g] << 7) | ((uint64_t)w << 5) | (uint64_t)sl);
+            ev[o] = v;
+          }
+      eptr = DBuf<uint64_t>(&pool, ptr.size());
+      ecode = DBuf<uint32_t>(&pool, code.size());
+      eval = DBuf<L>(&pool, ev.size());
+      eng->h2d(eptr.p, ptr.data(), ptr.size() * 8);
+      eng->h2d(ecode.p, code.data(), code.size() * 4);
+      eng->h2d(eval.p, ev.data(), ev.size() * sizeof(L));
+      eng->sync();
+      footprint = (size_t)((24.0 * E + 3 * 32) * sizeof(L) * n + (8.0 * E + 64) * sizeof(L) * m +
+                           (48.0 * E + 1024) * sizeof(L) * n);
+      return;
+    }
+    dhat = DBuf<L>(&pool, std::max<size_t>(total, 1) * E);
+    if (total) eng->pack(cval.p, dhat.p, total, 1);
     // matvec entries: row r of matrix w sums dhat[g] * w_phi[col[g]]
     {
       std::vector<uint64_t> ptr(rowptr);
@@ -431,6 +470,48 @@ template <class T> struct MimdParams : ParamsBase {
     cptr.release(); ccol.release(); cval.release(); dhat.release();
     mvptr.release(); mvdk.release(); mvsel.release(); mvidx.release();
     rwptr.release(); rwdk.release(); rwsel.release(); rwidx.release();
+    eptr.release(); ecode.release(); eval.release();
+  }
+
+  // sum_c Xhat[r, c] w_phi[c] for the three matrices into out (3 x n)
+  void matvecs(Engine<T>& e, const L* slots, const L* wphi, L* out) {
+    if constexpr (kMx<T>) {
+      DBuf<L> cm(e.pool, n * 1024);
+      for (int w = 0; w < 3; w++) {
+        e.launch("k_cmat_mimd", (double)n * 1024 * sizeof(L), (double)ncell[w] * 1024, (k_cmat_mimd<T>),
+                 dim3(grid_for(n, 8, 4096)), dim3(256), (const uint64_t*)(cptr.p + w * (n + 1)), (const uint32_t*)ccol.p,
+                 (const L*)cval.p, slots, cm.p, (uint64_t)n);
+        e.launch("k_cgemm", (double)n * (1024 * sizeof(L) + Engine<T>::EB), (double)n * 1024 * E, (k_cgemm<T>),
+                 dim3(grid_for(n, 32, 1024)), dim3(256), (const uint32_t*)e.d_b2frag.p, (const L*)cm.p,
+                 out + w * n * E, (uint64_t)n);
+      }
+    } else {
+      (void)slots;
+      for (int w = 0; w < 3; w++)
+        e.launch("k_spmv_gr", (double)ncell[w] * 2 * Engine<T>::EB, (double)ncell[w] * 16 * Engine<T>::PM,
+                 (k_spmv_gr<T>), dim3(grid_for(n, GPB)), dim3(GB), (const uint64_t*)(mvptr.p + w * (n + 1)),
+                 (const uint32_t*)mvdk.p, (const uint32_t*)mvsel.p, (const uint32_t*)mvidx.p, (const L*)dhat.p, wphi,
+                 (uint64_t)0, out + w * n * E, (uint64_t)n);
+    }
+  }
+  // build_combined_row (mimd.rs:196-219): M[col] = sum over the three
+  // matrices of eqs[w][y] Xhat[y, col], eqs = {eq, gamma eq, gamma^2 eq} (3 x n)
+  void m_row(Engine<T>& e, const L* eqs, L* mrow) {
+    if constexpr (kMx<T>) {
+      // G[w][y][i] = eqs[w][y] l_beta_i, then entries v * G[w][y][s/2] u_{s%2}
+      DBuf<L> G(e.pool, 3 * n * 16 * E);
+      const uint64_t nx = 3 * n, ntiles = (nx + 31) / 32;
+      e.launch("k_mulfix_mx", nx * 17.0 * Engine<T>::EB, nx * 16.0 * E * E, (k_mulfix_mx<T>),
+               dim3(grid_for(ntiles, 4, 32), 16), dim3(256), (const uint32_t*)e.d_lbfrag.p, eqs, nx, G.p);
+      e.launch("k_rows_gather", 0, 0, (k_rows_gather<T>), dim3(grid_for(m, 8, 1u << 16)), dim3(256),
+               (const uint64_t*)eptr.p, (const uint32_t*)ecode.p, (const L*)eval.p, (const L*)G.p, (uint64_t)n, mrow,
+               (uint64_t)m);
+    } else {
+      const double ncl = (double)(ncell[0] + ncell[1] + ncell[2]);
+      e.launch("k_spmv_gr", ncl * 2 * Engine<T>::EB, ncl * 16 * Engine<T>::PM, (k_spmv_gr<T>), dim3(grid_for(m, GPB)),
+               dim3(GB), (const uint64_t*)rwptr.p, (const uint32_t*)rwdk.p, (const uint32_t*)rwsel.p,
+               (const uint32_t*)rwidx.p, (const L*)dhat.p, eqs, (uint64_t)n, mrow, (uint64_t)m);
+    }
   }
 
   ProofView prove(Ctx<T>& cx, const L* wit /* 32 x m */) {
@@ -471,11 +552,7 @@ template <class T> struct MimdParams : ParamsBase {
     // 3. Ahat w_phi, Bhat w_phi, Chat w_phi (matvec_gr128_dense_csr) and the
     //    errors e_X = Xhat w_phi - x_hat
     DBuf<L> estack(&pool, 3 * n * E);
-    for (int w = 0; w < 3; w++)
-      e.launch("k_spmv_gr", (double)ncell[w] * 2 * Engine<T>::EB, (double)ncell[w] * 16 * Engine<T>::PM,
-               (k_spmv_gr<T>), dim3(grid_for(n, GPB)), dim3(GB), (const uint64_t*)(mvptr.p + w * (n + 1)),
-               (const uint32_t*)mvdk.p, (const uint32_t*)mvsel.p, (const uint32_t*)mvidx.p, (const L*)dhat.p,
-               (const L*)wphi.p, (uint64_t)0, estack.p + w * n * E, (uint64_t)n);
+    matvecs(e, slots.p, wphi.p, estack.p);
     e.sub(estack.p, hats.p, estack.p, 3 * n * E);
     stg.mark("reduce_matvec");
     // 4. commits (mimd.rs:302-316)
@@ -527,10 +604,7 @@ template <class T> struct MimdParams : ParamsBase {
       e.scale(dg.p, eq1.p, eqs.p + n * E, n);
       e.scale(dg2.p, eq1.p, eqs.p + 2 * n * E, n);
     }
-    const double ncl = (double)(ncell[0] + ncell[1] + ncell[2]);
-    e.launch("k_spmv_gr", ncl * 2 * Engine<T>::EB, ncl * 16 * Engine<T>::PM, (k_spmv_gr<T>), dim3(grid_for(m, GPB)),
-             dim3(GB), (const uint64_t*)rwptr.p, (const uint32_t*)rwdk.p, (const uint32_t*)rwsel.p,
-             (const uint32_t*)rwidx.p, (const L*)dhat.p, (const L*)eqs.p, (uint64_t)n, mrow.p, (uint64_t)m);
+    m_row(e, eqs.p, mrow.p);
     eqs.release();
     stg.mark("linear_rows");
     // 8. linear check on (M_gamma row, w_phi); w_phi is low-degree

@@ -623,9 +623,7 @@ template <class T> struct Verifier {
         e.scale(dg.p, eq1.p, eqs.p + n * E, n);
         e.scale(dg2.p, eq1.p, eqs.p + 2 * n * E, n);
       }
-      e.launch("k_spmv_gr", 0, 0, (k_spmv_gr<T>), dim3(grid_for(m, GPB)), dim3(GB), (const uint64_t*)mp_.rwptr.p,
-               (const uint32_t*)mp_.rwdk.p, (const uint32_t*)mp_.rwsel.p, (const uint32_t*)mp_.rwidx.p,
-               (const L*)mp_.dhat.p, (const L*)eqs.p, (uint64_t)n, mrow.p, (uint64_t)m);
+      mp_.m_row(e, eqs.p, mrow.p);
       DBuf<L> eq_s2 = e.eq_table(s2.data(), mp_.log_m);
       at_s2 = e.template dot<32>(eq_s2.p, mrow.p, m);
     }
