@@ -152,6 +152,32 @@ def run_queue(na, jobs, workers, arrivals=None):
     return wall, np.array(lat)
 
 
+def verify_queue(na, jobs, workers):
+    """One more (untimed) pass of the queue keeping the proofs, each checked
+    with the library's verifier; returns the number accepted."""
+    srv = na.Server(workers=workers)
+    ids = []
+    for model, p, x in jobs:
+        if model == "general":
+            ids.append(srv.submit_general(p, x))
+        elif model == "mimd":
+            ids.append(srv.submit_mimd(p, x))
+        else:
+            ids.append(srv.submit_simd(x, p))
+    ok = 0
+    for (model, p, x), jid in zip(jobs, ids):
+        proof, _, _ = srv.wait(jid)
+        if model == "general":
+            v = na.verify_general(p, proof)
+        elif model == "mimd":
+            v = na.verify_mimd(p, proof)
+        else:
+            v = na.verify_simd(x, p, proof)
+        ok += v == "Ok"
+    srv.close()
+    return ok
+
+
 def queue_stats(na, jobs, workers, seed):
     """Batch (all jobs at t=0) and serving (Poisson arrivals at 70% of the
     batch throughput) figures for `workers` streams and for the plain
@@ -297,7 +323,10 @@ def main():
         jobs = mixed_jobs(na, args.batch_jobs, 1000 + args.seed)
         log(f"mixed batch: {len(jobs)} jobs set up in {time.perf_counter() - t0:.1f}s")
         q, srv = queue_stats(na, jobs, args.workers, 77 + args.seed)
+        nver = verify_queue(na, jobs, args.workers)
+        log(f"  batch proofs accepted by the verifier: {nver}/{len(jobs)}")
         batch = {"jobs": len(jobs), "workers": args.workers, "proofs_per_s": q["batched"]["proofs_per_s"],
+                 "verified": f"{nver}/{len(jobs)}",
                  "p50_ms": q["batched"]["p50_ms"], "p99_ms": q["batched"]["p99_ms"], "plain": q["plain"],
                  "mix": "logN~U{12..20}, k~U{8,16,32,64}, model~U{SIMD,MIMD,General}, host inputs, seed_i=base+i"}
         serve = {"lambda_per_s": srv["lambda_per_s"], "jobs": len(jobs), "workers": args.workers,
@@ -313,7 +342,10 @@ def main():
         mj, sizes = mimd_serving_jobs(na, 32, 2000 + args.seed)
         log(f"mimd serving mix: {len(mj)} jobs over per-instance logs {sizes} set up in {time.perf_counter() - t0:.1f}s")
         q, srv = queue_stats(na, mj, args.workers, 99 + args.seed)
+        nver = verify_queue(na, mj, args.workers)
+        log(f"  mimd serving proofs accepted by the verifier: {nver}/{len(mj)}")
         large = {"mimd_u64_serve": {"jobs": len(mj), "workers": args.workers, "lambda_per_s": srv["lambda_per_s"],
+                                    "verified": f"{nver}/{len(mj)}",
                                     "p50_ms": srv["batched"]["p50_ms"], "p99_ms": srv["batched"]["p99_ms"],
                                     "plain": srv["plain"], "batch": q,
                                     "mix": "k=64, logN~U{10..22}, one circuit per size, witness per job"}}

@@ -73,7 +73,14 @@ def _check(rc):
         raise NenyaError(rc, _lib().nenya_last_error().decode(errors="replace"))
 
 
+def _width(k):
+    if k not in _DT:
+        raise NenyaError(4, "k must be 8, 16, 32 or 64")
+    return _DT[k]
+
+
 def _coo(m, k):
+    _width(k)
     r, c, v = m
     r = np.ascontiguousarray(r, np.uint32)
     c = np.ascontiguousarray(c, np.uint32)
@@ -172,6 +179,7 @@ def prove_simd(inst, params):
 
 def _mimd_set(ms, k):
     """32 COO matrices -> (offsets[33] as size_t, rows, cols, vals) arrays."""
+    _width(k)
     if len(ms) != 32:
         raise NenyaError(4, "a MIMD matrix set holds 32 instance matrices")
     mats = [_coo(m, k) for m in ms]
@@ -262,9 +270,10 @@ def verify_simd(inst, params, proof):
     """verify_simd(a, b, c, &params, &proof) (snark-ring/src/simd.rs:380-522)
     -> "Ok" or the VerifyError variant name ("Parse" for malformed bytes)."""
     k = params.k
+    mats = [_coo(m, k) for m in (inst.a, inst.b, inst.c)]  # referenced until the call returns
     args = [params._h]
-    for m in (inst.a, inst.b, inst.c):
-        args += _coo_args(_coo(m, k))
+    for m in mats:
+        args += _coo_args(m)
     v = C.c_int()
     _check(_lib().nenya_verify_simd(*args, bytes(proof), len(proof), C.byref(v)))
     return VERIFY_ERRORS[v.value]

@@ -43,3 +43,22 @@ def test_argument_errors_are_codes_not_crashes():
     with pytest.raises(na.NenyaError) as e:
         na.SimdSnarkParams.secure(5, 5, k=12)
     assert e.value.code == 4
+
+
+def test_mimd_and_verify_argument_errors():
+    """Bad widths, a MIMD set without 32 instances and null handles come back
+    as NENYA_E_ARG (4), before any device work."""
+    one = ([0], [0], [1])
+    with pytest.raises(na.NenyaError) as e:
+        na.MimdSnarkParams.secure(5, 5, ([one] * 31, [one] * 32, [one] * 32))
+    assert e.value.code == 4
+    with pytest.raises(na.NenyaError) as e:
+        na.MimdSnarkParams.secure(5, 5, ([one] * 32,) * 3, k=12)
+    assert e.value.code == 4
+    lib = na._lib._lib()
+    v = ctypes.c_int(-1)
+    assert lib.nenya_verify_general(None, b"", 0, ctypes.byref(v)) == 4
+    assert lib.nenya_verify_mimd(None, b"", 0, ctypes.byref(v)) == 4
+    assert v.value == -1
+    buf, n = ctypes.c_void_p(), ctypes.c_size_t()
+    assert lib.nenya_prove_mimd(None, None, ctypes.byref(buf), ctypes.byref(n)) == 4

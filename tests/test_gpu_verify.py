@@ -76,3 +76,18 @@ def test_verify_general_u32_2p15_blocks_accepts():
     params = na.GeneralSnarkParams.secure(15, (A, B, Cm), 0xDEAD, 0xCAFE, k=32)
     p = na.prove_general(w, params)
     assert na.verify_general(params, p) == "Ok"
+
+
+@pytest.mark.parametrize("k,log", [(32, 9), (8, 10), (64, 8)])
+def test_verify_simd_larger_accepts(orc, k, log):
+    """Honest SIMD proofs at sizes where the matrices span many pages (the
+    wrapper keeps its converted COO arrays alive through the call)."""
+    print(f"verify simd k={k} log={log}", flush=True)
+    A, B, Cm, wit = simd_instance(log, k, 34)
+    params = na.SimdSnarkParams.secure(log, log, 0xDEAD, 0xCAFE, k=k)
+    inst = na.SimdR1csInstance(A, B, Cm, wit)
+    p = na.prove_simd(inst, params)
+    assert na.verify_simd(inst, params, p) == "Ok"
+    bad = bytearray(p)
+    bad[len(p) // 2] ^= 1
+    assert na.verify_simd(inst, params, bytes(bad)) == orc.simd_verify(k, log, log, A, B, Cm, bytes(bad))
