@@ -69,3 +69,28 @@ def test_png_bytes_equal_oracle(orc, log_bytes):
     want = orc.simd_prove(8, ln, lm, A, B, Cm, wit)
     assert len(got) == len(want) and got == want, f"first differing byte {_first_diff(got, want)}"
     assert orc.simd_verify(8, ln, lm, A, B, Cm, got) == "Ok"
+
+
+def test_proofs_match_committed_golden_digests():
+    """The GPU proofs of the committed fixtures hash to the digests stored in
+    tests/golden/oracle_c0.json (no oracle call at run time)."""
+    import hashlib
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_c0.json")))
+
+    def dg(b):
+        return hashlib.blake2b(b, digest_size=16).hexdigest()
+
+    print("golden simd k=32 log=5 seed=7", flush=True)
+    A, B, Cm, wit = simd_instance(5, 32, 7)
+    p = na.prove_simd(na.SimdR1csInstance(A, B, Cm, wit), na.SimdSnarkParams.secure(5, 5, 0xDEAD, 0xCAFE, k=32))
+    assert len(p) == g["simd_u32_log5_seed7"]["len"] and dg(p) == g["simd_u32_log5_seed7"]["digest"]
+    print("golden general k=32 log_n=10 seed=3", flush=True)
+    A, B, Cm, w = general_instance(10, 32, 3)
+    p = na.prove_general(w, na.GeneralSnarkParams.secure(5, (A, B, Cm), 0xDEAD, 0xCAFE, k=32))
+    assert dg(p) == g["general_u32_logn10_seed3"]["digest"]
+    print("golden mimd k=64 log=5 seed=21", flush=True)
+    As, Bs, Cs, wit = mimd_instance(5, 64, 21)
+    p = na.prove_mimd(wit, na.MimdSnarkParams.secure(5, 5, (As, Bs, Cs), 0xDEAD, 0xCAFE, k=64))
+    assert len(p) == g["mimd_u64_log5_seed21"]["len"] and dg(p) == g["mimd_u64_log5_seed21"]["digest"]

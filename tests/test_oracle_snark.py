@@ -55,6 +55,10 @@ def test_golden_digests(orc):
     p = orc.general_prove(32, 5, A, B, C, w)
     assert digest(p) == g["general_u32_logn10_seed3"]["digest"]
     assert [int(x) for x in orc.rng_stream(32, 0, 0, 8)] == g["rng_u32_seed0"]
+    from nenya_amd.instances import mimd_instance
+    As, Bs, Cs, wit = mimd_instance(5, 64, 21)
+    p = orc.mimd_prove(64, 5, 5, As, Bs, Cs, wit)
+    assert len(p) == g["mimd_u64_log5_seed21"]["len"] and digest(p) == g["mimd_u64_log5_seed21"]["digest"]
 
 
 @pytest.mark.parametrize("k", (8, 16, 32, 64))
