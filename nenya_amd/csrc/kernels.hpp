@@ -350,26 +350,36 @@ __device__ __forceinline__ void block_emit(Acc<T> (&acc)[NS], LT<T>* partials, i
 // Gruen mult-check round (mult_check.rs:250-296, sub form 426-461):
 //   s0 += eg*(a_lo b_lo - c_lo - e_lo), s1 += eg*(a_hi b_hi - c_hi - e_hi),
 //   sinf += eg*(da db).  c, e may be null (plain form).
-template <class T>
+// SPLIT (small m, latency-bound): block b computes only sum s = b % 3 over
+// element groups b / 3, so an element's three product chains run on three
+// blocks at once; the other two partial sums of the block are zero.
+template <class T, bool SPLIT>
 __global__ void __launch_bounds__(GB) k_mult_round(const LT<T>* eg, const LT<T>* a, const LT<T>* b, const LT<T>* cc,
                                                    const LT<T>* ee, uint64_t m, LT<T>* partials) {
   __shared__ __attribute__((aligned(16))) LT<T> sh[GPB][3][E];
   int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  const int sblk = SPLIT ? (int)(blockIdx.x % 3) : 0;
+  const uint64_t bid = SPLIT ? blockIdx.x / 3 : blockIdx.x, nbid = SPLIT ? gridDim.x / 3 : gridDim.x;
   Acc<T> acc[3];
   for (int s = 0; s < 3; s++) acc[s].clear();
-  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < m; base += (uint64_t)gridDim.x * GPB) {
+  for (uint64_t base = bid * GPB; base < m; base += nbid * GPB) {
     uint64_t i = base + g;
     bool ok = i < m;
     CT<T> alo[4] = {0, 0, 0, 0}, ahi[4] = {0, 0, 0, 0}, blo[4] = {0, 0, 0, 0}, bhi[4] = {0, 0, 0, 0};
     if (ok) {
-      ld4<T>(a + i * E + 4 * t, alo);
-      ld4<T>(a + (i + m) * E + 4 * t, ahi);
-      ld4<T>(b + i * E + 4 * t, blo);
-      ld4<T>(b + (i + m) * E + 4 * t, bhi);
+      if (!SPLIT || sblk != 1) {
+        ld4<T>(a + i * E + 4 * t, alo);
+        ld4<T>(b + i * E + 4 * t, blo);
+      }
+      if (!SPLIT || sblk != 0) {
+        ld4<T>(a + (i + m) * E + 4 * t, ahi);
+        ld4<T>(b + (i + m) * E + 4 * t, bhi);
+      }
       stage<T>(sh[g][2], eg + i * E, t);
     }
     // three products of the varying tables, each then weighted by eg
     for (int s = 0; s < 3; s++) {
+      if (SPLIT && s != sblk) continue;  // uniform over the block
       CT<T> x[4], y[4];
       for (int c = 0; c < 4; c++) {
         x[c] = s == 0 ? alo[c] : s == 1 ? ahi[c] : ahi[c] - alo[c];
@@ -1242,6 +1252,16 @@ __global__ void k_merkle_paths(const uint32_t* nodes, uint64_t n, const uint32_t
   uint64_t node = start + ((idx[q] >> lvl) ^ 1);
   uint32_t* dst = reinterpret_cast<uint32_t*>(rec + 8 + (uint64_t)lvl * 32);
   for (int k = 0; k < 8; k++) dst[k] = nodes[node * 8 + k];
+}
+
+// Serialized rotated m-bars (pcs.rs:426-471): per rotation a u64 length
+// `cnt`, then the rotation's `per` 8-byte words of src.
+__global__ void k_len_prefixed(const uint64_t* src, uint64_t nrec, uint64_t per, uint64_t cnt, uint64_t* dst) {
+  const uint64_t rec = per + 1, n = nrec * rec;
+  for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t l = x / rec, o = x % rec;
+    dst[x] = o == 0 ? cnt : src[l * per + o - 1];
+  }
 }
 
 // Serialized queried columns (pcs.rs:379-388): per query a u64 entry count,

@@ -654,48 +654,6 @@ __global__ void k_sum_split(const LT<T>* part, uint64_t nsplit, uint64_t cnt, LT
 // meet at beta^(i+i') alpha^(c+c').
 // ---------------------------------------------------------------------------
 
-// Contract a 128 x 128 Gram matrix held in LDS (G[j][j'], row-major) into
-// the 128 limbs of sum_e x_e y_e: raw[i+i'][c+c'] += G[4i+c][4i'+c'], the
-// alpha reduction (X^4 = -X - 1), then Y^32 = -Y^3 - Y - alpha from the top.
-// All threads of the block call it; the result is left in q (LDS).
-__device__ __forceinline__ void gram_contract_lds(const uint32_t* G, uint32_t (*raw)[7], uint32_t (*q)[4]) {
-  const int tid = threadIdx.x;
-  for (int o = tid; o < 63 * 7; o += blockDim.x) {
-    const int mpos = o / 7, d = o % 7;
-    uint32_t s = 0;
-    const int i0 = mpos > 31 ? mpos - 31 : 0, i1 = mpos < 31 ? mpos : 31;
-    const int c0 = d > 3 ? d - 3 : 0, c1 = d < 3 ? d : 3;
-    for (int i = i0; i <= i1; i++)
-      for (int c = c0; c <= c1; c++) s += G[(4 * i + c) * 128 + 4 * (mpos - i) + (d - c)];
-    raw[mpos][d] = s;
-  }
-  __syncthreads();
-  if (tid < 63) {
-    const uint32_t* p = raw[tid];
-    q[tid][0] = p[0] - p[4];
-    q[tid][1] = p[1] - p[4] - p[5];
-    q[tid][2] = p[2] - p[5] - p[6];
-    q[tid][3] = p[3] - p[6];
-  }
-  __syncthreads();
-  if (tid == 0) {
-    for (int sp = 62; sp >= 32; sp--) {
-      const uint32_t h0 = q[sp][0], h1 = q[sp][1], h2 = q[sp][2], h3 = q[sp][3];
-      const int i = sp - 32;
-      for (int c = 0; c < 4; c++) {
-        const uint32_t hc = q[sp][c];
-        q[i + 3][c] -= hc;
-        q[i + 1][c] -= hc;
-      }
-      q[i][0] += h3;
-      q[i][1] -= h0 - h3;
-      q[i][2] -= h1;
-      q[i][3] -= h2;
-    }
-  }
-  __syncthreads();
-}
-
 // Linear-check round (as k_lin_round): gram 0 over (f_lo, g_lo), gram 1
 // over (f_hi - f_lo, g_hi - g_lo), both from one read of the tables.  A
 // block (16 waves) walks chunks of 32 elements: its threads load the four
@@ -704,9 +662,12 @@ __device__ __forceinline__ void gram_contract_lds(const uint32_t* G, uint32_t (*
 // LDS; then each wave takes one or two (gram, 32x32 block) pairs.  Per
 // chunk a Horner pass over the digit classes fills a temporary accumulator
 // that is added to the wave's running total (i32 wrap = mod 2^32).  x rows
-// at limbs >= xrows are zero (low-degree f) and skipped.  The block
-// contracts its two Gram matrices and writes the sums as partials
-// [block][gram].
+// at limbs >= xrows are zero (low-degree f) and skipped.
+// Contraction: limb j = 4i+c of x and j' = 4i'+c' of y meet at
+// beta^(i+i') alpha^(c+c'), so each wave adds its accumulator entries into
+// the 63 x 7 bins raw[i+i'][c+c'] with LDS atomics (both Gram matrices at
+// once); then alpha^4 = -alpha - 1 and Y^32 = -Y^3 - Y - alpha reduce the
+// bins to 128 limbs per matrix, written as partials [block][gram].
 template <class T>
 __global__ void __launch_bounds__(1024) k_gram_lin(const LT<T>* f, uint64_t fstride, const LT<T>* gt, uint64_t gstride,
                                                    uint64_t ntab, uint64_t m, int xrows, LT<T>* partials) {
@@ -714,7 +675,7 @@ __global__ void __launch_bounds__(1024) k_gram_lin(const LT<T>* f, uint64_t fstr
   constexpr int ND = kND<T>;
   // digit planes [x0|y0|x1|y1][plane][rb][lane][word] in the loop, a Gram matrix after
   __shared__ __attribute__((aligned(16))) uint32_t lds[16384];
-  __shared__ uint32_t raw[63][7], q[63][4];
+  __shared__ uint32_t raw[2][63][8], q[2][63][4];
   uint32_t (*sF)[ND][4][64][4] = reinterpret_cast<uint32_t (*)[ND][4][64][4]>(lds);
   const int tid = threadIdx.x, j = tid & 127, eg = tid >> 7;
   const int w = tid >> 6, l = tid & 63;
@@ -731,6 +692,7 @@ __global__ void __launch_bounds__(1024) k_gram_lin(const LT<T>* f, uint64_t fstr
     pby[u] = pr & 3;
   }
   const uint64_t total = m * ntab, nchunks = (total + 31) / 32;
+  if (tid < 2 * 63 * 8) (&raw[0][0][0])[tid] = 0;
   v16i tot[2] = {(v16i)0, (v16i)0};
   const int lane = (j & 31) + 32 * (eg & 1), rb = j >> 5, wd = eg >> 1;
   // raw limbs of the chunk: loads for chunk ch+stride are issued before the
@@ -792,23 +754,39 @@ __global__ void __launch_bounds__(1024) k_gram_lin(const LT<T>* f, uint64_t fstr
     }
     __syncthreads();
   }
-  for (int g = 0; g < 2; g++) {
-    // Gram matrix g into LDS (rows >= xrows stay zero), then contract
-    for (int x = tid; x < 16384; x += 1024) lds[x] = 0;
-    __syncthreads();
+  __syncthreads();
+  // accumulator entry i of lane l in block (bx, by): row 32bx + 8(i>>2) +
+  // 4(l>>5) + (i&3), column 32by + (l&31)
 #pragma unroll
-    for (int u = 0; u < 2; u++) {
-      if (!pon[u] || pg[u] != g) continue;
+  for (int u = 0; u < 2; u++) {
+    if (!pon[u]) continue;
+    const int mp0 = 8 * (pbx[u] + pby[u]) + (l >> 5) + ((l & 31) >> 2), cp = l & 3;
 #pragma unroll
-      for (int i = 0; i < 16; i++) {
-        const int row = 32 * pbx[u] + 8 * (i >> 2) + 4 * (l >> 5) + (i & 3), col = 32 * pby[u] + (l & 31);
-        lds[row * 128 + col] = (uint32_t)tot[u][i];
-      }
+    for (int i = 0; i < 16; i++) atomicAdd(&raw[pg[u]][mp0 + 2 * (i >> 2)][(i & 3) + cp], (uint32_t)tot[u][i]);
+  }
+  __syncthreads();
+  if (tid < 2 * 63) {
+    const int g = tid / 63, p = tid % 63;
+    const uint32_t* r = raw[g][p];
+    q[g][p][0] = r[0] - r[4];
+    q[g][p][1] = r[1] - r[4] - r[5];
+    q[g][p][2] = r[2] - r[5] - r[6];
+    q[g][p][3] = r[3] - r[6];
+  }
+  __syncthreads();
+  if (tid < 2 * E) {
+    const int g = tid >> 7, u = (tid & 127) >> 2, c = tid & 3;
+    // position 32 + s after positions 61 and 62 have folded into 32 and 33
+    auto hc = [&](int s, int cc) -> uint32_t { return q[g][32 + s][cc] - (s < 2 ? q[g][61 + s][cc] : 0u); };
+    uint32_t v = q[g][u][c];
+    if (u >= 1) v -= hc(u - 1, c);  // -Y from position u + 31
+    if (u >= 3) v -= hc(u - 3, c);  // -Y^3 from position u + 29
+    if (u <= 30) {                  // -alpha from position u + 32
+      if (c == 0) v += hc(u, 3);
+      else if (c == 1) v += hc(u, 3) - hc(u, 0);
+      else v -= hc(u, c - 1);
     }
-    __syncthreads();
-    gram_contract_lds(lds, raw, q);
-    if (tid < E) partials[((size_t)blockIdx.x * 2 + g) * E + tid] = (L)q[tid >> 2][tid & 3];
-    __syncthreads();
+    partials[((size_t)blockIdx.x * 2 + g) * E + (tid & 127)] = (L)v;
   }
 }
 

@@ -696,11 +696,12 @@ template <class T> struct Engine {
              nl, mbar.p);
     std::vector<G> ys = dots<32>(mbar.p, P.k2, nl, e1.p, P.k2);
     out.u64(nl);
-    size_t per = P.k2 * E * sizeof(L);
-    for (int l = 0; l < nl; l++) {
-      out.u64(P.k2);
-      HIP_CHECK(hipMemcpyAsync(out.take(per), mbar.p + (size_t)l * P.k2 * E, per, hipMemcpyDeviceToHost, st));
-    }
+    // the nl length-prefixed m-bars, laid out on the device, one download
+    const uint64_t per = P.k2 * E * sizeof(L) / 8, recb = (size_t)nl * (per + 1) * 8;
+    DBuf<uint64_t> rec(pool, (size_t)nl * (per + 1));
+    launch("k_len_prefixed", 2.0 * recb, 0, k_len_prefixed, dim3(grid_for((uint64_t)nl * (per + 1), 256, 4096)),
+           dim3(256), (const uint64_t*)mbar.p, (uint64_t)nl, per, (uint64_t)P.k2, rec.p);
+    HIP_CHECK(hipMemcpyAsync(out.take(recb), rec.p, recb, hipMemcpyDeviceToHost, st));
     open_tail(pc, c, mbar.p, (size_t)nl * P.k2, out);
     return ys;
   }
@@ -729,11 +730,18 @@ template <class T> struct Engine {
     for (int t = 0; t < ntab; t++) cur[t] = tabs[t];
     for (size_t r = 0; r < nv; r++) {
       size_t m = len / 2;
-      unsigned nbk = grid_for(m, GPB, 2048);
+      // small rounds are latency-bound: spread each element's three sums over three blocks
+      const bool split = m <= 4096;
+      unsigned nbk = split ? 3 * grid_for(m, GPB, 682) : grid_for(m, GPB, 2048);
       DBuf<L> part(pool, (size_t)nbk * 3 * E);
-      launch("k_mult_round", m * (2.0 * ntab + 1) * EB, m * 6 * 32 * PM, (k_mult_round<T>), dim3(nbk), dim3(GB),
-             (const L*)eg.p, cur[0], cur[1], ntab > 2 ? cur[2] : (const L*)nullptr, ntab > 2 ? cur[3] : (const L*)nullptr,
-             (uint64_t)m, part.p);
+      const L* c3 = ntab > 2 ? cur[2] : (const L*)nullptr;
+      const L* e3 = ntab > 2 ? cur[3] : (const L*)nullptr;
+      if (split)
+        launch("k_mult_round", m * (2.0 * ntab + 1) * EB, m * 6 * 32 * PM, (k_mult_round<T, true>), dim3(nbk),
+               dim3(GB), (const L*)eg.p, cur[0], cur[1], c3, e3, (uint64_t)m, part.p);
+      else
+        launch("k_mult_round", m * (2.0 * ntab + 1) * EB, m * 6 * 32 * PM, (k_mult_round<T, false>), dim3(nbk),
+               dim3(GB), (const L*)eg.p, cur[0], cur[1], c3, e3, (uint64_t)m, part.p);
       reduce_into(part, nbk, 3, res_dev());
       sync();
       G sv[3];
