@@ -1,6 +1,7 @@
 // Host-side building blocks: device memory pool, per-width constants,
 // PCS parameters, Fiat-Shamir transcript.
 #pragma once
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -41,6 +42,12 @@ inline bool pool_guard() {
   return on;
 }
 constexpr size_t GUARD = 1 << 16;
+
+// Number of live batched-proof servers in the process.
+inline std::atomic<int>& live_servers() {
+  static std::atomic<int> n{0};
+  return n;
+}
 
 struct Pool {
   std::multimap<size_t, void*> free_;

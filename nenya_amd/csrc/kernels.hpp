@@ -516,6 +516,115 @@ __global__ void __launch_bounds__(1024) k_sum_partials(const LT<T>* partials, ui
 }
 
 // ============================================================================
+// Host-device mailbox (coherent pinned memory, no caching on the device)
+// ============================================================================
+
+// A kernel that needs a value the host has not posted yet waits on a flag
+// word.  Every wait ends: on the posted value, on the abort word (abort[0]),
+// or after `ticks` of the constant-rate wall clock, which it reports in
+// abort[1] so the host fails the proof.
+struct MbWait {
+  const uint32_t* flag;  // null: the value is already in place
+  uint32_t val;
+  uint32_t* abort;
+  uint64_t ticks;
+};
+
+__device__ __forceinline__ void mb_wait(const MbWait& w) {
+  if (threadIdx.x == 0) {
+    const uint64_t t0 = (uint64_t)wall_clock64();
+    while (__hip_atomic_load(w.flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) != w.val) {
+      if (__hip_atomic_load(w.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u) break;
+      if ((uint64_t)wall_clock64() - t0 > w.ticks) {
+        __hip_atomic_store(w.abort + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  __syncthreads();
+}
+
+// After mb_wait: the `words` posted 32-bit words into LDS through
+// system-scope loads.
+__device__ __forceinline__ void mb_fetch(const uint32_t* src, int words, uint32_t* dst) {
+  for (int i = threadIdx.x; i < words; i += blockDim.x)
+    dst[i] = __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __syncthreads();
+}
+
+// Sumcheck round values posted to the host.  The partial sums are reduced as
+// in k_sum_partials: one block per (sum, 32 limbs) piece into `sums`
+// (device), the last block to finish taking over (cross-block handoff
+// through `count`, which the last block resets), or with a single block
+// (gridDim.x == 1) all pieces in turn, with no handoff.  The finishing
+// block forms the round polynomial q, writes it to the mailbox slot `qh`
+// (and to `qd` on the device when given), then raises `flag` to `val`.
+//   mode 0 (mult check, ns = 3):   q = (s0, s1 - s0 - s2, s2)
+//                                   (mult_check.rs:250-296)
+//   mode 1 (linear check, ns = 2): q = (s0, c - 2 s0 - s1, s1)
+//                                   (linear_check_general.rs:289-334)
+template <class T>
+__global__ void __launch_bounds__(1024) k_sum_post(const LT<T>* partials, uint64_t nblocks, int ns, int mode,
+                                                  const LT<T>* c, LT<T>* sums, LT<T>* qd, LT<T>* qh, uint32_t* count,
+                                                  uint32_t* flag, uint32_t val) {
+  using C = CT<T>;
+  __shared__ C red[32][33];
+  __shared__ C ssum[3][E];
+  __shared__ bool last;
+  const bool single = gridDim.x == 1;
+  const int l = threadIdx.x & 31, sl = threadIdx.x >> 5;
+  for (int piece = single ? 0 : (int)blockIdx.x; piece < (single ? ns * 4 : (int)blockIdx.x + 1); piece++) {
+    const int s = piece / 4, l0 = (piece % 4) * 32;
+    C a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t b = sl;
+    for (; b + 7 * 32 < nblocks; b += 8 * 32)
+#pragma unroll
+      for (int u = 0; u < 8; u++) a[u] += partials[((b + 32 * u) * ns + s) * E + l0 + l];
+    for (; b < nblocks; b += 32) a[0] += partials[(b * ns + s) * E + l0 + l];
+    C acc = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    red[sl][l] = acc;
+    __syncthreads();
+    if (sl == 0) {
+      for (int k = 1; k < 32; k++) acc += red[k][l];
+      if (single) ssum[s][l0 + l] = acc;
+      else {
+        sums[s * E + l0 + l] = (LT<T>)acc;
+        __threadfence();  // the writing wave publishes its piece
+      }
+    }
+    __syncthreads();
+  }
+  if (!single) {
+    if (threadIdx.x == 0) last = atomicAdd(count, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    if (threadIdx.x == 0) __threadfence();  // acquire the other blocks' pieces (same CU: one wave suffices)
+    __syncthreads();
+    for (int i = threadIdx.x; i < ns * E; i += blockDim.x) ssum[i / E][i % E] = (C)sums[i];
+    __syncthreads();
+  }
+  const int i = threadIdx.x;
+  if (i < 3 * E) {
+    const int k = i / E, li = i % E;
+    C v;
+    if (mode == 0) v = k == 1 ? (C)(ssum[1][li] - ssum[0][li] - ssum[2][li]) : ssum[k][li];
+    else v = k == 0 ? ssum[0][li] : k == 1 ? (C)((C)c[li] - ssum[0][li] - ssum[0][li] - ssum[1][li]) : ssum[1][li];
+    if (qd) qd[i] = (LT<T>)v;
+    qh[i] = (LT<T>)v;
+  }
+  // each writer's stores have completed before one system-scope release
+  // orders them ahead of the flag
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (!single) *count = 0;
+    __threadfence_system();
+    __hip_atomic_store(flag, val, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// ============================================================================
 // Sumcheck scalar updates kept on the device (one 32-lane group)
 // ============================================================================
 
@@ -540,21 +649,6 @@ template <class T> __global__ void __launch_bounds__(64) k_eq_one(const LT<T>* t
     if (l == 0) o[0] += 1;
     st4<T>(out + 4 * l, o);
   }
-}
-
-// Linear-check round message from the sums s = (s0, s_inf) and claim c:
-// q = (s0, c - 2 s0 - s_inf, s_inf)   (linear_check_general.rs:289-334).
-// q is written to device memory and to qh (host-pinned) for the transcript.
-template <class T>
-__global__ void __launch_bounds__(128) k_lin_q(const LT<T>* s, const LT<T>* c, LT<T>* q, LT<T>* qh) {
-  const int i = threadIdx.x;
-  const LT<T> s0 = s[i], s1 = s[E + i], q1 = (LT<T>)(c[i] - s0 - s0 - s1);
-  q[i] = s0;
-  q[E + i] = q1;
-  q[2 * E + i] = s1;
-  qh[i] = s0;
-  qh[E + i] = q1;
-  qh[2 * E + i] = s1;
 }
 
 // Next claim c = q0 + r (q1 + r q2).
@@ -713,8 +807,15 @@ __device__ __forceinline__ void teich_lift_block(const LT<T>* src, LT<T>* el, co
 }
 
 template <class T>
-__global__ void __launch_bounds__(256) k_lift_from(const LT<T>* src, LT<T>* x, const LT<T>* frob_t, int squarings) {
+__global__ void __launch_bounds__(256) k_lift_from(const LT<T>* src, LT<T>* x, const LT<T>* frob_t, int squarings,
+                                                   MbWait w) {
   __shared__ __attribute__((aligned(16))) CT<T> sx[E];
+  __shared__ uint32_t posted[E * sizeof(LT<T>) / 4];
+  if (w.flag) {
+    mb_wait(w);
+    mb_fetch(reinterpret_cast<const uint32_t*>(src), E * sizeof(LT<T>) / 4, posted);
+    src = reinterpret_cast<const LT<T>*>(posted);
+  }
   teich_lift_block<T>(src, x, frob_t, squarings, sx);
 }
 

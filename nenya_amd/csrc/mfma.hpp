@@ -131,13 +131,20 @@ __global__ void __launch_bounds__(256) k_mulmat(const LT<T>* r, uint32_t* frag) 
   mulmat_frag<T>(v, frag);
 }
 
-// Lift one challenge (natural sample src, may be host-pinned) into x and
-// emit the M_r fragments of the result.
+// Lift one challenge (natural sample src, may be host-pinned; with w.flag
+// set, posted by the host into the mailbox) into x and emit the M_r
+// fragments of the result.
 template <class T>
 __global__ void __launch_bounds__(256) k_lift_mulmat(const LT<T>* src, LT<T>* x, const LT<T>* frob_t, int squarings,
-                                                     uint32_t* frag) {
+                                                     uint32_t* frag, MbWait w) {
   using C = CT<T>;
   __shared__ __attribute__((aligned(16))) C v[32][E];
+  __shared__ uint32_t posted[E * sizeof(LT<T>) / 4];
+  if (w.flag) {
+    mb_wait(w);
+    mb_fetch(reinterpret_cast<const uint32_t*>(src), E * sizeof(LT<T>) / 4, posted);
+    src = reinterpret_cast<const LT<T>*>(posted);
+  }
   teich_lift_block<T>(src, x, frob_t, squarings, v[0]);
   __syncthreads();
   mulmat_chain<C>(v);

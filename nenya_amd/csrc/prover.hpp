@@ -7,7 +7,9 @@
 // the reference's bytes (see the identities cited at each step).
 #pragma once
 #include <chrono>
+#include <exception>
 #include <map>
+#include <thread>
 
 #include "host.hpp"
 #include "kernels.hpp"
@@ -182,6 +184,30 @@ template <class T> struct Engine {
   DBuf<uint32_t> d_packfrag;  // MFMA fragments of pack_r2 o rot_L
   KProf* prof = nullptr;
 
+  // Host-device mailbox for the sumcheck rounds (coherent pinned memory):
+  // word 0 aborts waiting kernels, word 1 is set by a wait that ran out of
+  // time (the host then fails the proof), then a flag word per slot and direction,
+  // then per slot the round polynomial (3 elements, device -> host) and the
+  // natural challenge sample (1 element, host -> device).  Flags take
+  // strictly increasing sequence values, so slots need no reset.
+  static constexpr int MB_SLOTS = 64;
+  Pinned mbox;
+  uint8_t* mb_h = nullptr;  // host view
+  uint8_t* mb_d = nullptr;  // device view
+  uint32_t mseq = 0;
+  uint64_t spin_ticks = 0;
+  // Pipelined rounds (a kernel waits for the host's challenge) only where
+  // one thread drives the device: allocation and free calls of other threads
+  // can wait for the device while holding runtime locks this thread's
+  // launches need.  Contexts of a multi-worker server (and every context
+  // while a server exists, or with the pool's synchronizing debug modes)
+  // post each challenge before the lift that reads it is enqueued, so no
+  // kernel waits on a host thread.
+  bool pipelined = true;
+  bool pipeline_rounds() const { return pipelined && live_servers().load() == 0 && !pool_poison() && !pool_guard(); }
+  DBuf<uint32_t> d_count;  // last-block counter of k_sum_post
+  DBuf<L> d_sums;          // reduced round sums (3 elements)
+
   // Launch on the engine's stream, bracketed by events when profiling.
   template <class Kern, class... Args>
   void launch(const char* name, double bytes, double macs, Kern kern, dim3 grid, dim3 block, Args... args) {
@@ -210,7 +236,19 @@ template <class T> struct Engine {
     std::vector<L> b(32 * E);
     for (int i = 0; i < 32; i++) std::memcpy(&b[i * E], K.basis[i].v, E * sizeof(L));
     G one = G::one();
-    upbuf.flags = resbuf.flags = hipHostMallocCoherent;
+    upbuf.flags = resbuf.flags = mbox.flags = hipHostMallocCoherent;
+    mb_h = mbox.get(mb_bytes());
+    std::memset(mb_h, 0, mb_bytes());
+    HIP_CHECK(hipHostGetDevicePointer((void**)&mb_d, mb_h, 0));
+    {
+      int dev = 0, khz = 0;
+      HIP_CHECK(hipGetDevice(&dev));
+      HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+      spin_ticks = (uint64_t)(khz > 0 ? khz : 100000) * 1000 * 10;  // 10 s
+    }
+    d_count = DBuf<uint32_t>(pool, 1);
+    d_sums = DBuf<L>(pool, 3 * E);
+    HIP_CHECK(hipMemsetAsync(d_count.p, 0, 4, st));
     h2d(d_basis.p, b.data(), b.size() * sizeof(L));
     h2d(d_pw.p, K.psi_pw.data(), K.psi_pw.size() * sizeof(L));
     h2d(d_one.p, one.v, E * sizeof(L));
@@ -341,7 +379,8 @@ template <class T> struct Engine {
   // x -> r*x as matrix-core fragments (k <= 32); folds by the same
   // challenge share one.
   struct MulMat {
-    DBuf<uint32_t> frag;
+    DBuf<uint32_t> frag;         // owned fragments, or
+    const uint32_t* fp = nullptr;  // the fragments the fold reads
     const L* r = nullptr;
   };
   MulMat mulmat(const L* r) {
@@ -349,6 +388,7 @@ template <class T> struct Engine {
     mm.r = r;
     if constexpr (kMx<T>) {
       mm.frag = DBuf<uint32_t>(pool, (size_t)kND<T> * 4096);
+      mm.fp = mm.frag.p;
       launch("k_mulmat", EB + kND<T> * 16384.0, 0, (k_mulmat<T>), dim3(1), dim3(256), r, mm.frag.p);
     }
     return mm;
@@ -364,7 +404,7 @@ template <class T> struct Engine {
       const uint64_t ntiles = (m * (J.ntab[0] + J.ntab[1]) + 31) / 32;
       const int split = ntiles <= 256;  // few tiles: 4 waves per tile, fragments from L2
       launch("k_fold_mx", m * (J.ntab[0] + J.ntab[1]) * 3 * EB, m * (J.ntab[0] + J.ntab[1]) * 32 * PM, (k_fold_mx<T>),
-             dim3(split ? (unsigned)ntiles : grid_for(ntiles, 4, 512)), dim3(256), (const uint32_t*)mm.frag.p, J, split);
+             dim3(split ? (unsigned)ntiles : grid_for(ntiles, 4, 512)), dim3(256), mm.fp, J, split);
     } else {
       fold(mm.r, s0, d0, m, n0, ss0, ds0);
       if (s1) fold(mm.r, s1, d1, m, n1, ss1, ds1);
@@ -372,21 +412,18 @@ template <class T> struct Engine {
   }
   // Linear-check round sums {sum f_lo g_lo, sum df dg} as two Gram matrices
   // on the matrix cores; x rows past `xrows` limbs are zero.
-  DBuf<L> gram_lin(const L* f, uint64_t fs, const L* g, uint64_t gs, uint64_t ntab, uint64_t m, int xrows) {
+  // Partials [block][2] of the two sums into `part` (room for 2 * 256
+  // elements); returns the block count.
+  static constexpr unsigned GRAM_BLOCKS = 256;
+  unsigned gram_lin(const L* f, uint64_t fs, const L* g, uint64_t gs, uint64_t ntab, uint64_t m, int xrows, L* part) {
     const uint64_t nch = (m * ntab + 31) / 32;
-    const unsigned nwg = (unsigned)std::min<uint64_t>(nch, 256);
-    DBuf<L> part(pool, (size_t)2 * nwg * E);
+    const unsigned nwg = (unsigned)std::min<uint64_t>(nch, GRAM_BLOCKS);
     launch("k_gram_lin", m * ntab * 4.0 * EB, m * ntab * 2.0 * E * E, (k_gram_lin<T>), dim3(nwg), dim3(1024), f, fs, g,
-           gs, ntab, m, xrows, part.p);
-    return reduce(part, nwg, 2);
+           gs, ntab, m, xrows, part);
+    return nwg;
   }
   void marg(const L* s, L* eg, uint64_t m) {
     launch("k_marg", m * 3 * EB, m * 32 * PM, (k_marg<T>), dim3(grid_for(m, GPB)), dim3(GB), s, eg, m);
-  }
-  // sum of partial sums -> ns elements on device
-  void reduce_into(const DBuf<L>& partials, unsigned nblocks, int ns, L* out) {
-    launch("k_sum_partials", (double)nblocks * ns * EB, 0, (k_sum_partials<T>), dim3(ns * 4), dim3(1024), partials.p,
-           (uint64_t)nblocks, ns, out);
   }
   DBuf<L> reduce(const DBuf<L>& partials, unsigned nblocks, int ns) {
     DBuf<L> out(pool, ns * E);
@@ -440,6 +477,82 @@ template <class T> struct Engine {
     return b;
   }
 
+  // ---------------- mailbox ----------------
+  static constexpr size_t mb_bytes() { return 1024 + (size_t)MB_SLOTS * 4 * E * sizeof(L); }
+  static constexpr size_t mb_down_flag(int s) { return 64 + 4 * (size_t)s; }
+  static constexpr size_t mb_up_flag(int s) { return 64 + 4 * ((size_t)MB_SLOTS + s); }
+  static constexpr size_t mb_down(int s) { return 1024 + (size_t)s * 3 * E * sizeof(L); }
+  static constexpr size_t mb_up(int s) { return 1024 + ((size_t)MB_SLOTS * 3 + s) * E * sizeof(L); }
+  template <class X> X* mbh(size_t off) { return reinterpret_cast<X*>(mb_h + off); }
+  template <class X> X* mbd(size_t off) { return reinterpret_cast<X*>(mb_d + off); }
+
+  // Enqueue: reduce a round's partial sums and post q to slot s (flag value
+  // returned).  mode 0 mult check, 1 linear check (claim c, q also to qd).
+  uint32_t post_round(const L* part, uint64_t nparts, int ns, int mode, const L* c, L* qd, int s) {
+    const uint32_t v = ++mseq;
+    // few partials: one block, no cross-block handoff
+    const unsigned grid = nparts <= 256 ? 1 : ns * 4;
+    launch("k_sum_post", (double)nparts * ns * EB, 0, (k_sum_post<T>), dim3(grid), dim3(1024), part, nparts, ns, mode, c,
+           d_sums.p, qd, mbd<L>(mb_down(s)), d_count.p, mbd<uint32_t>(mb_down_flag(s)), v);
+    return v;
+  }
+  // Host: wait for slot s's round polynomial (bounded; device errors raise).
+  void wait_round(int s, uint32_t v, G q[3]) {
+    const uint32_t* f = mbh<uint32_t>(mb_down_flag(s));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t it = 1; __atomic_load_n(f, __ATOMIC_ACQUIRE) != v; it++) {
+      __builtin_ia32_pause();
+      if ((it & 4095) != 0) continue;
+      if (__atomic_load_n(mbh<uint32_t>(4), __ATOMIC_ACQUIRE) != 0u)
+        throw Error(NENYA_E_DEVICE, "a device wait for a posted challenge ran out of time");
+      const hipError_t e = hipStreamQuery(st);
+      if (e != hipSuccess && e != hipErrorNotReady) throw Error(NENYA_E_DEVICE, hipGetErrorString(e));
+      if (e == hipSuccess && __atomic_load_n(f, __ATOMIC_ACQUIRE) != v)
+        throw Error(NENYA_E_DEVICE, "sumcheck round values were not posted");
+      if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(60))
+        throw Error(NENYA_E_DEVICE, "device did not post sumcheck round values");
+      std::this_thread::yield();
+    }
+    std::memcpy(q, mbh<L>(mb_down(s)), 3 * sizeof(G));
+  }
+  // Enqueue: lift the challenge the host will post to slot s (flag value v)
+  // into d; with k <= 32 its multiplication fragments go to `frag`
+  // (kND * 4096 words, allocated by the caller) and mm points at them.
+  void lift_posted(int s, uint32_t v, L* d, uint32_t* frag, MulMat* mm) {
+    const MbWait w{mbd<uint32_t>(mb_up_flag(s)), v, mbd<uint32_t>(0), spin_ticks};
+    const L* src = mbd<L>(mb_up(s));
+    const double lm = E * E + (T::K - 1) * 16.5 * PM;
+    mm->r = d;
+    if constexpr (kMx<T>) {
+      mm->fp = frag;
+      launch("k_lift_mulmat", 2 * EB, lm, (k_lift_mulmat<T>), dim3(1), dim3(256), src, d, d_frob.p, T::K - 1, frag,
+             w);
+    } else {
+      launch("k_lift", 2 * EB, lm, (k_lift_from<T>), dim3(1), dim3(256), src, d, d_frob.p, T::K - 1, w);
+    }
+  }
+  // Host: sample the challenge (the transcript's sampler) and post it.
+  void post_challenge(Transcript& tr, const char* label, int s, uint32_t v) {
+    StdRng rng = tr.sampler(label);
+    L* up = mbh<L>(mb_up(s));
+    for (int l = 0; l < E; l++) up[l] = (L)rng.limb(T::K);
+    __atomic_store_n(mbh<uint32_t>(mb_up_flag(s)), v, __ATOMIC_RELEASE);
+  }
+  // Kernels left waiting when the host leaves a round loop by an exception
+  // are released through the abort word before the stack unwinds further.
+  struct MbGuard {
+    Engine& e;
+    int n0 = std::uncaught_exceptions();
+    explicit MbGuard(Engine& en) : e(en) {}
+    ~MbGuard() {
+      if (std::uncaught_exceptions() <= n0) return;
+      __atomic_store_n(e.mbh<uint32_t>(0), 1u, __ATOMIC_RELEASE);
+      (void)hipStreamSynchronize(e.st);
+      __atomic_store_n(e.mbh<uint32_t>(0), 0u, __ATOMIC_RELEASE);
+      __atomic_store_n(e.mbh<uint32_t>(4), 0u, __ATOMIC_RELEASE);
+    }
+  };
+
   // ---------------- Fiat-Shamir challenges ----------------
   // R2Sampler::sample (sampling.rs:57-68): 128 L::random draws then the
   // Teichmuller lift; the lift runs on the device.
@@ -453,8 +566,9 @@ template <class T> struct Engine {
     if constexpr (kMx<T>) {
       if (mm && cnt == 1) {
         mm->frag = DBuf<uint32_t>(pool, (size_t)kND<T> * 4096);
+        mm->fp = mm->frag.p;
         launch("k_lift_mulmat", 2 * EB, lm, (k_lift_mulmat<T>), dim3(1), dim3(256), (const L*)d.p, d.p, d_frob.p,
-               T::K - 1, mm->frag.p);
+               T::K - 1, mm->frag.p, MbWait{});
         fused = true;
       }
     }
@@ -481,12 +595,14 @@ template <class T> struct Engine {
     if constexpr (kMx<T>) {
       if (mm) {
         mm->frag = DBuf<uint32_t>(pool, (size_t)kND<T> * 4096);
+        mm->fp = mm->frag.p;
         launch("k_lift_mulmat", 2 * EB, lm, (k_lift_mulmat<T>), dim3(1), dim3(256), src, d, d_frob.p, T::K - 1,
-               mm->frag.p);
+               mm->frag.p, MbWait{});
         fused = true;
       }
     }
-    if (!fused) launch("k_lift", 2 * EB, lm, (k_lift_from<T>), dim3(1), dim3(256), src, d, d_frob.p, T::K - 1);
+    if (!fused)
+      launch("k_lift", 2 * EB, lm, (k_lift_from<T>), dim3(1), dim3(256), src, d, d_frob.p, T::K - 1, MbWait{});
     if (mm) {
       mm->r = d;
       if (!fused) *mm = mulmat(d);
@@ -717,23 +833,40 @@ template <class T> struct Engine {
     std::vector<G> chal;
     std::vector<G> finals;
   };
+  // Rounds go through the mailbox (slot r carries round r).  Pipelined:
+  // before the host waits for round r's polynomial, the stream already holds
+  // round r's lift (waiting for the challenge the host posts), its folds and
+  // round r+1's sums, so no stream synchronization or launch sits between a
+  // round's values and the next round's work.  Otherwise the host posts the
+  // challenge first and then enqueues the round's work.
   SumcheckOut mult_check(const L* const* tabs, int ntab, size_t nv, const std::vector<G>& rt, Transcript& tr) {
     SumcheckOut o;
+    if (nv > (size_t)MB_SLOTS) throw Error(NENYA_E_ARG, "too many sumcheck rounds");
     size_t N = size_t(1) << nv;
     G one = G::one();
     DBuf<L> eg = nv > 1 ? eq_table(rt.data() + 1, nv - 1) : put(&one, 1);
     DBuf<L> work(pool, (size_t)ntab * (N / 2 > 0 ? N / 2 : 1) * E);
-    // challenges and the point rt stay on the device; one sync per round
+    // challenges and the point rt stay on the device
     DBuf<L> drt = put(rt.data(), nv), dch(pool, nv * E), dsc(pool, E);
     size_t len = N;
     const L* cur[4];
     for (int t = 0; t < ntab; t++) cur[t] = tabs[t];
-    for (size_t r = 0; r < nv; r++) {
-      size_t m = len / 2;
+    // everything the pipelined rounds use is allocated before the first
+    // waiting kernel is enqueued (no allocator call can then wait on them)
+    auto blocks = [](size_t m) -> unsigned {
       // small rounds are latency-bound: spread each element's three sums over three blocks
+      return m <= 4096 ? 3 * grid_for(m, GPB, 682) : grid_for(m, GPB, 2048);
+    };
+    unsigned nbk_max = 1;
+    for (size_t m = N / 2; m >= 1; m /= 2) nbk_max = std::max(nbk_max, blocks(m));
+    DBuf<L> part(pool, (size_t)nbk_max * 3 * E);
+    DBuf<uint32_t> frags(pool, kMx<T> ? nv * kND<T> * 4096 : 1);
+    MbGuard guard(*this);
+    const bool pipe = pipeline_rounds();
+    // enqueue the sums of round r over tables of 2m entries; returns the flag value
+    auto round_sums = [&](size_t r, size_t m) -> uint32_t {
       const bool split = m <= 4096;
-      unsigned nbk = split ? 3 * grid_for(m, GPB, 682) : grid_for(m, GPB, 2048);
-      DBuf<L> part(pool, (size_t)nbk * 3 * E);
+      const unsigned nbk = blocks(m);
       const L* c3 = ntab > 2 ? cur[2] : (const L*)nullptr;
       const L* e3 = ntab > 2 ? cur[3] : (const L*)nullptr;
       if (split)
@@ -742,16 +875,24 @@ template <class T> struct Engine {
       else
         launch("k_mult_round", m * (2.0 * ntab + 1) * EB, m * 6 * 32 * PM, (k_mult_round<T, false>), dim3(nbk),
                dim3(GB), (const L*)eg.p, cur[0], cur[1], c3, e3, (uint64_t)m, part.p);
-      reduce_into(part, nbk, 3, res_dev());
-      sync();
-      G sv[3];
-      std::memcpy(sv, res_host(), sizeof sv);
-      G q[3] = {sv[0], sv[1] - sv[0] - sv[2], sv[2]};
-      tr.elems("sc/round", q, 3);
-      for (int i = 0; i < 3; i++) o.rounds.push_back(q[i]);
+      return post_round(part.p, nbk, 3, 0, nullptr, nullptr, (int)r);
+    };
+    uint32_t vdown = round_sums(0, len / 2);
+    for (size_t r = 0; r < nv; r++) {
+      size_t m = len / 2;
       L* rc = dch.p + r * E;
       MulMat mr;
-      challenge_into(tr, "sc/challenge", rc, &mr);
+      const uint32_t vup = ++mseq;
+      // host side of round r: its polynomial, the transcript, the challenge
+      auto host_round = [&] {
+        G q[3];
+        wait_round((int)r, vdown, q);
+        tr.elems("sc/round", q, 3);
+        for (int i = 0; i < 3; i++) o.rounds.push_back(q[i]);
+        post_challenge(tr, "sc/challenge", (int)r, vup);
+      };
+      if (!pipe) host_round();
+      lift_posted((int)r, vup, rc, frags.p + (kMx<T> ? r * kND<T> * 4096 : 0), &mr);
       // tables at a common stride fold in one launch
       bool uniform = true;
       uniform = ntab > 1 && cur[1] > cur[0] && (cur[1] - cur[0]) % E == 0;
@@ -763,11 +904,15 @@ template <class T> struct Engine {
       }
       for (int t = 0; t < ntab; t++) cur[t] = work.p + (size_t)t * (N / 2) * E;
       len = m;
+      uint32_t vnext = 0;
       if (r + 1 < nv) {
         launch("k_eq_one", 3 * EB, 32 * PM, (k_eq_one<T>), dim3(1), dim3(64), (const L*)(drt.p + r * E), (const L*)rc,
                dsc.p);
         marg(dsc.p, eg.p, m / 2);
+        vnext = round_sums(r + 1, m / 2);
       }
+      if (pipe) host_round();
+      vdown = vnext;
     }
     o.chal.resize(nv);
     d2h(o.chal.data(), dch.p, nv * sizeof(G));
@@ -783,23 +928,29 @@ template <class T> struct Engine {
   SumcheckOut linear(const L* f0, uint64_t fstride0, const L* g0, uint64_t gstride0, int ntab, size_t nv, G claim,
                      Transcript& tr) {
     SumcheckOut o;
+    if (nv > (size_t)MB_SLOTS) throw Error(NENYA_E_ARG, "too many sumcheck rounds");
     size_t N = size_t(1) << nv;
     size_t H = N / 2 > 0 ? N / 2 : 1;
     DBuf<L> fw(pool, (size_t)ntab * H * E), gw(pool, (size_t)ntab * H * E);
     const L* f = f0;
     const L* g = g0;
     uint64_t fs = fstride0, gs = gstride0;
-    // the running claim and the challenges stay on the device; one sync per round
+    // the running claim and the challenges stay on the device (pipelined
+    // through the mailbox as in mult_check)
     DBuf<L> dc = put(&claim, 1), dq(pool, 3 * E), dch(pool, nv * E);
     size_t len = N;
-    for (size_t r = 0; r < nv; r++) {
-      size_t m = len / 2;
-      DBuf<L> s;
+    // allocated before the first waiting kernel (see mult_check)
+    const size_t nparts = kMx<T> ? GRAM_BLOCKS : grid_for((uint64_t)(N / 2 > 0 ? N / 2 : 1) * ntab, GPB, 2048);
+    DBuf<L> part(pool, nparts * 2 * E);
+    DBuf<uint32_t> frags(pool, kMx<T> ? nv * kND<T> * 4096 : 1);
+    MbGuard guard(*this);
+    const bool pipe = pipeline_rounds();
+    auto round_sums = [&](size_t r, size_t m) -> uint32_t {
       if constexpr (kMx<T>) {
-        s = gram_lin(f, fs, g, gs, (uint64_t)ntab, (uint64_t)m, (r == 0 && NAF0 == 16) ? 64 : 128);
+        const unsigned nwg = gram_lin(f, fs, g, gs, (uint64_t)ntab, (uint64_t)m, (r == 0 && NAF0 == 16) ? 64 : 128, part.p);
+        return post_round(part.p, nwg, 2, 1, dc.p, dq.p, (int)r);
       } else {
-        unsigned nbk = grid_for(m * ntab, GPB, 2048);
-        DBuf<L> part(pool, (size_t)nbk * 2 * E);
+        const unsigned nbk = grid_for(m * ntab, GPB, 2048);
         double lb = m * ntab * 4.0 * EB;
         if (r == 0 && NAF0 == 16)
           launch("k_lin_round", lb, m * ntab * 2 * 16 * PM, (k_lin_round<T, 16>), dim3(nbk), dim3(GB), f, fs, g, gs,
@@ -807,23 +958,33 @@ template <class T> struct Engine {
         else
           launch("k_lin_round", lb, m * ntab * 2 * 32 * PM, (k_lin_round<T, 32>), dim3(nbk), dim3(GB), f, fs, g, gs,
                  (uint64_t)ntab, (uint64_t)m, part.p);
-        s = reduce(part, nbk, 2);
+        return post_round(part.p, nbk, 2, 1, dc.p, dq.p, (int)r);
       }
-      launch("k_lin_q", 5 * EB, 0, (k_lin_q<T>), dim3(1), dim3(128), (const L*)s.p, (const L*)dc.p, dq.p, res_dev());
-      sync();
-      G q[3];
-      std::memcpy(q, res_host(), sizeof q);
-      tr.elems("sc/round", q, 3);
-      for (int i = 0; i < 3; i++) o.rounds.push_back(q[i]);
+    };
+    uint32_t vdown = round_sums(0, len / 2);
+    for (size_t r = 0; r < nv; r++) {
+      size_t m = len / 2;
       L* rc = dch.p + r * E;
       MulMat mr;
-      challenge_into(tr, "sc/challenge", rc, &mr);
+      const uint32_t vup = ++mseq;
+      auto host_round = [&] {
+        G q[3];
+        wait_round((int)r, vdown, q);
+        tr.elems("sc/round", q, 3);
+        for (int i = 0; i < 3; i++) o.rounds.push_back(q[i]);
+        post_challenge(tr, "sc/challenge", (int)r, vup);
+      };
+      if (!pipe) host_round();
+      lift_posted((int)r, vup, rc, frags.p + (kMx<T> ? r * kND<T> * 4096 : 0), &mr);
       launch("k_lin_claim", 5 * EB, 64 * PM, (k_lin_claim<T>), dim3(1), dim3(64), (const L*)dq.p, (const L*)rc, dc.p);
       fold2(mr, m, f, fw.p, ntab, fs, H, g, gw.p, ntab, gs, H);
       f = fw.p;
       g = gw.p;
       fs = gs = H;
       len = m;
+      const uint32_t vnext = r + 1 < nv ? round_sums(r + 1, m / 2) : 0;
+      if (pipe) host_round();
+      vdown = vnext;
     }
     o.chal.resize(nv);
     d2h(o.chal.data(), dch.p, nv * sizeof(G));

@@ -43,7 +43,11 @@ struct Worker {
   std::unique_ptr<CtxBase> ctx[4];  // per width: 8, 16, 32, 64
   template <class T> Ctx<T>& get() {
     const int i = T::K == 8 ? 0 : T::K == 16 ? 1 : T::K == 32 ? 2 : 3;
-    if (!ctx[i]) ctx[i].reset(new Ctx<T>());
+    if (!ctx[i]) {
+      ctx[i].reset(new Ctx<T>());
+      // several workers drive the device: no kernel waits on a host thread
+      static_cast<Ctx<T>&>(*ctx[i]).eng->pipelined = false;
+    }
     return static_cast<Ctx<T>&>(*ctx[i]);
   }
 };
@@ -71,6 +75,7 @@ struct Server {
     }
     budget = budget_bytes;
     for (int i = 0; i < nworkers; i++) workers.emplace_back(new Worker());
+    live_servers()++;  // before any job can be submitted
     for (int i = 0; i < nworkers; i++) threads.emplace_back([this, i] { loop(i); });
   }
   ~Server() {
@@ -81,6 +86,7 @@ struct Server {
     cv_work.notify_all();
     for (auto& t : threads) t.join();
     workers.clear();
+    live_servers()--;
   }
 
   uint64_t submit(std::unique_ptr<Job> j) {
