@@ -102,6 +102,24 @@ __device__ __forceinline__ void acc_mul(Acc<T>& acc, const LT<T>* A, const LT<T>
   }
 }
 
+// acc += the pair terms i0 <= i < i1 of A * B (lane t of a group).
+template <class T>
+__device__ __forceinline__ void acc_mul_range(Acc<T>& acc, const LT<T>* A, const LT<T>* B, int t, int i0, int i1) {
+  using C = CT<T>;
+  for (int i = i0; i < i1; i++) {
+    C a[4], b[4], p[7];
+    ld4<T>(A + 4 * i, a);
+    ld4<T>(B + 4 * ((t - i) & 31), b);
+    d_raw(a, b, p);
+    const C m = (i > t) ? (C)~(C)0 : (C)0;
+#pragma unroll
+    for (int c = 0; c < 7; c++) {
+      acc.c[c] += p[c];
+      acc.h[c] += p[c] & m;
+    }
+  }
+}
+
 // Reduce the group's accumulated raw sums; lane t gets coefficient t.
 template <class T> __device__ __forceinline__ void acc_final(const Acc<T>& acc, int t, CT<T> out[4]) {
   using C = CT<T>;
@@ -629,7 +647,10 @@ __global__ void __launch_bounds__(1024) k_sum_post(const LT<T>* partials, uint64
 // ============================================================================
 
 // out = t r + (1 - t)(1 - r) = 2 t r - t - r + 1   (eq in one variable)
-template <class T> __global__ void __launch_bounds__(64) k_eq_one(const LT<T>* t, const LT<T>* r, LT<T>* out) {
+// The two scalar updates below run on threads 0..63 of any block of at least
+// 64 threads (the others only meet the barriers): as kernels of their own,
+// or as the side task of a fold launch (one extra block).
+template <class T> __device__ __forceinline__ void eq_one_block(const LT<T>* t, const LT<T>* r, LT<T>* out) {
   __shared__ __attribute__((aligned(16))) LT<T> st[E], sr[E];
   const int g = threadIdx.x >> 5, l = threadIdx.x & 31;
   if (g == 0) {
@@ -650,9 +671,12 @@ template <class T> __global__ void __launch_bounds__(64) k_eq_one(const LT<T>* t
     st4<T>(out + 4 * l, o);
   }
 }
+template <class T> __global__ void __launch_bounds__(64) k_eq_one(const LT<T>* t, const LT<T>* r, LT<T>* out) {
+  eq_one_block<T>(t, r, out);
+}
 
 // Next claim c = q0 + r (q1 + r q2).
-template <class T> __global__ void __launch_bounds__(64) k_lin_claim(const LT<T>* q, const LT<T>* r, LT<T>* c) {
+template <class T> __device__ __forceinline__ void lin_claim_block(const LT<T>* q, const LT<T>* r, LT<T>* c) {
   __shared__ __attribute__((aligned(16))) LT<T> sr[E], sx[E];
   const int g = threadIdx.x >> 5, l = threadIdx.x & 31;
   if (g == 0) {
@@ -672,6 +696,78 @@ template <class T> __global__ void __launch_bounds__(64) k_lin_claim(const LT<T>
     }
     __syncthreads();
     if (g == 0) st4<T>(step ? sx + 4 * l : c + 4 * l, o);
+  }
+}
+template <class T> __global__ void __launch_bounds__(64) k_lin_claim(const LT<T>* q, const LT<T>* r, LT<T>* c) {
+  lin_claim_block<T>(q, r, c);
+}
+
+// One product A * B (elements in LDS) by a 256-thread block: group p sums
+// the pair terms 4p..4p+3 of every coefficient, the eight raw sums meet in
+// LDS, lanes 0..31 reduce; lane t < 32 returns coefficient t in out.  All
+// threads call it.
+template <class T> __device__ __forceinline__ void block_mul256(const LT<T>* A, const LT<T>* B, CT<T> out[4]) {
+  using C = CT<T>;
+  __shared__ C red[8][14][32];
+  const int t = threadIdx.x & 31, p = threadIdx.x >> 5;
+  Acc<T> acc;
+  acc.clear();
+  acc_mul_range<T>(acc, A, B, t, 4 * p, 4 * p + 4);
+#pragma unroll
+  for (int c = 0; c < 7; c++) {
+    red[p][c][t] = acc.c[c];
+    red[p][7 + c][t] = acc.h[c];
+  }
+  __syncthreads();
+  if (p == 0) {
+#pragma unroll
+    for (int q = 1; q < 8; q++)
+#pragma unroll
+      for (int c = 0; c < 7; c++) {
+        acc.c[c] += red[q][c][t];
+        acc.h[c] += red[q][7 + c][t];
+      }
+    acc_final<T>(acc, t, out);
+  }
+  __syncthreads();
+}
+
+// eq_one_block / lin_claim_block for 256-thread blocks (fold side tasks).
+template <class T> __device__ __forceinline__ void eq_one_256(const LT<T>* t, const LT<T>* r, LT<T>* out) {
+  __shared__ __attribute__((aligned(16))) LT<T> st[E], sr[E];
+  const int l = threadIdx.x;
+  if (l < 32) {
+    stage<T>(st, t, l);
+    stage<T>(sr, r, l);
+  }
+  __syncthreads();
+  CT<T> o[4], a[4], b[4];
+  block_mul256<T>(st, sr, o);
+  if (l < 32) {
+    ld4<T>(st + 4 * l, a);
+    ld4<T>(sr + 4 * l, b);
+    for (int c = 0; c < 4; c++) o[c] = o[c] + o[c] - a[c] - b[c];
+    if (l == 0) o[0] += 1;
+    st4<T>(out + 4 * l, o);
+  }
+}
+template <class T> __device__ __forceinline__ void lin_claim_256(const LT<T>* q, const LT<T>* r, LT<T>* c) {
+  __shared__ __attribute__((aligned(16))) LT<T> sr[E], sx[E];
+  const int l = threadIdx.x;
+  if (l < 32) {
+    stage<T>(sr, r, l);
+    stage<T>(sx, q + 2 * E, l);
+  }
+  __syncthreads();
+  for (int step = 1; step >= 0; step--) {
+    CT<T> o[4], a[4];
+    block_mul256<T>(sr, sx, o);
+    if (l < 32) {
+      ld4<T>(q + step * E + 4 * l, a);
+      for (int k = 0; k < 4; k++) o[k] += a[k];
+      st4<T>(step ? sx + 4 * l : c + 4 * l, o);
+    }
+    __syncthreads();
   }
 }
 

@@ -165,11 +165,17 @@ template <class T> __device__ __forceinline__ void ld16k(const LT<T>* el, int kb
 // Up to two groups of tables folded by one launch (same challenge, same m):
 // group g has ntab[g] tables, table t at src[g] + t*sstride[g] elements,
 // output at dst[g] + t*dstride[g].
+// side: a scalar update run by one extra block of the launch, next to the
+// folds (1: so = eq_one(sa, sb), 2: so = lin_claim(q = sa, r = sb)).
 template <class T> struct FoldJob {
   const LT<T>* src[2];
   LT<T>* dst[2];
   uint64_t ntab[2], sstride[2], dstride[2];
   uint64_t m;
+  int side = 0;
+  const LT<T>* sa = nullptr;
+  const LT<T>* sb = nullptr;
+  LT<T>* so = nullptr;
 };
 
 // Element e of the job: lo pointer (hi = lo + m) and output pointer.
@@ -296,19 +302,26 @@ __device__ __forceinline__ void fold_tile(const v4i* A, const FoldJob<T>& J, uin
 // with its fragments loaded from global memory (L2-resident) up front; a
 // barrier between the loads and the stores keeps in-place folds exact.
 // Bulk mode needs none: a wave loads and stores only its own tile.
+// With a side task the last block runs it instead of folding.
 template <class T>
 __global__ void __launch_bounds__(256) k_fold_mx(const uint32_t* frag, FoldJob<T> J, int split) {
   constexpr int ND = kND<T>;
   __shared__ v4i sA[ND * 16 * 64];
   const v4i* gA = reinterpret_cast<const v4i*>(frag);
   const uint64_t total = J.m * (J.ntab[0] + J.ntab[1]), ntiles = (total + 31) / 32;
+  const unsigned nfold = gridDim.x - (J.side ? 1 : 0);
+  if (J.side && blockIdx.x == nfold) {
+    if (J.side == 1) eq_one_256<T>(J.sa, J.sb, J.so);
+    else lin_claim_256<T>(J.sa, J.sb, J.so);
+    return;
+  }
   if (split) {
     fold_tile<T, 1, true>(gA, J, blockIdx.x, threadIdx.x >> 6);
     return;
   }
   for (int x = threadIdx.x; x < ND * 16 * 64; x += 256) sA[x] = gA[x];
   __syncthreads();
-  for (uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntiles; tile += (uint64_t)gridDim.x * 4) {
+  for (uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntiles; tile += (uint64_t)nfold * 4) {
     // keeps the A-fragment reads inside the loop (hoisted, they would pin
     // ND*64 registers)
     asm volatile("" ::: "memory");

@@ -195,7 +195,16 @@ template <class T> struct Engine {
   uint8_t* mb_h = nullptr;  // host view
   uint8_t* mb_d = nullptr;  // device view
   uint32_t mseq = 0;
-  uint64_t spin_ticks = 0;
+  uint64_t clock_khz = 0;  // device wall clock
+  // Device-side limit of a wait for a posted challenge: 10 s, or
+  // NENYA_WAIT_LIMIT_US (a debug switch that lets a test force the time-out
+  // path from the host).
+  uint64_t spin_ticks() const {
+    uint64_t us = 10000000;
+    if (const char* e = std::getenv("NENYA_WAIT_LIMIT_US")) us = std::strtoull(e, nullptr, 10);
+    return us * clock_khz / 1000;
+  }
+  uint64_t wait_ticks = 0;  // spin_ticks() for the running sumcheck
   // Pipelined rounds (a kernel waits for the host's challenge) only where
   // one thread drives the device: allocation and free calls of other threads
   // can wait for the device while holding runtime locks this thread's
@@ -244,7 +253,7 @@ template <class T> struct Engine {
       int dev = 0, khz = 0;
       HIP_CHECK(hipGetDevice(&dev));
       HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-      spin_ticks = (uint64_t)(khz > 0 ? khz : 100000) * 1000 * 10;  // 10 s
+      clock_khz = khz > 0 ? (uint64_t)khz : 100000;
     }
     d_count = DBuf<uint32_t>(pool, 1);
     d_sums = DBuf<L>(pool, 3 * E);
@@ -393,21 +402,40 @@ template <class T> struct Engine {
     }
     return mm;
   }
-  void fold(const MulMat& mm, const L* src, L* dst, uint64_t m, uint64_t ntab, uint64_t sstride, uint64_t dstride) {
-    fold2(mm, m, src, dst, ntab, sstride, dstride, nullptr, nullptr, 0, 0, 0);
+  // A scalar update that needs nothing from the fold it accompanies: on the
+  // matrix-core path it runs in an extra block of the fold launch, otherwise
+  // as a kernel of its own after the folds.
+  struct Side {
+    int kind = 0;  // 0 none, 1 so = eq_one(sa, sb), 2 so = lin_claim(q = sa, r = sb)
+    const L* sa = nullptr;
+    const L* sb = nullptr;
+    L* so = nullptr;
+  };
+  void side_kernel(const Side& sd) {
+    if (sd.kind == 1)
+      launch("k_eq_one", 3 * EB, 32 * PM, (k_eq_one<T>), dim3(1), dim3(64), sd.sa, sd.sb, sd.so);
+    else if (sd.kind == 2)
+      launch("k_lin_claim", 5 * EB, 64 * PM, (k_lin_claim<T>), dim3(1), dim3(64), sd.sa, sd.sb, sd.so);
+  }
+  void fold(const MulMat& mm, const L* src, L* dst, uint64_t m, uint64_t ntab, uint64_t sstride, uint64_t dstride,
+            const Side& sd = Side{}) {
+    fold2(mm, m, src, dst, ntab, sstride, dstride, nullptr, nullptr, 0, 0, 0, sd);
   }
   // Two groups of tables folded by the same challenge in one launch.
   void fold2(const MulMat& mm, uint64_t m, const L* s0, L* d0, uint64_t n0, uint64_t ss0, uint64_t ds0, const L* s1,
-             L* d1, uint64_t n1, uint64_t ss1, uint64_t ds1) {
+             L* d1, uint64_t n1, uint64_t ss1, uint64_t ds1, const Side& sd = Side{}) {
     if constexpr (kMx<T>) {
-      FoldJob<T> J{{s0, s1 ? s1 : s0}, {d0, d1 ? d1 : d0}, {n0, s1 ? n1 : 0}, {ss0, ss1}, {ds0, ds1}, m};
+      FoldJob<T> J{{s0, s1 ? s1 : s0}, {d0, d1 ? d1 : d0}, {n0, s1 ? n1 : 0}, {ss0, ss1}, {ds0, ds1}, m,
+                   sd.kind, sd.sa, sd.sb, sd.so};
       const uint64_t ntiles = (m * (J.ntab[0] + J.ntab[1]) + 31) / 32;
       const int split = ntiles <= 256;  // few tiles: 4 waves per tile, fragments from L2
+      const unsigned nb = split ? (unsigned)ntiles : grid_for(ntiles, 4, 512);
       launch("k_fold_mx", m * (J.ntab[0] + J.ntab[1]) * 3 * EB, m * (J.ntab[0] + J.ntab[1]) * 32 * PM, (k_fold_mx<T>),
-             dim3(split ? (unsigned)ntiles : grid_for(ntiles, 4, 512)), dim3(256), mm.fp, J, split);
+             dim3(nb + (sd.kind ? 1 : 0)), dim3(256), mm.fp, J, split);
     } else {
       fold(mm.r, s0, d0, m, n0, ss0, ds0);
       if (s1) fold(mm.r, s1, d1, m, n1, ss1, ds1);
+      side_kernel(sd);
     }
   }
   // Linear-check round sums {sum f_lo g_lo, sum df dg} as two Gram matrices
@@ -503,8 +531,7 @@ template <class T> struct Engine {
     for (uint32_t it = 1; __atomic_load_n(f, __ATOMIC_ACQUIRE) != v; it++) {
       __builtin_ia32_pause();
       if ((it & 4095) != 0) continue;
-      if (__atomic_load_n(mbh<uint32_t>(4), __ATOMIC_ACQUIRE) != 0u)
-        throw Error(NENYA_E_DEVICE, "a device wait for a posted challenge ran out of time");
+      check_waits();
       const hipError_t e = hipStreamQuery(st);
       if (e != hipSuccess && e != hipErrorNotReady) throw Error(NENYA_E_DEVICE, hipGetErrorString(e));
       if (e == hipSuccess && __atomic_load_n(f, __ATOMIC_ACQUIRE) != v)
@@ -514,12 +541,20 @@ template <class T> struct Engine {
       std::this_thread::yield();
     }
     std::memcpy(q, mbh<L>(mb_down(s)), 3 * sizeof(G));
+    check_waits();
+  }
+  // A device wait that ran out of time went on with a stale sample: the
+  // proof fails (the caller's guard releases the other waits).  Also called
+  // once the round loop's work has finished.
+  void check_waits() {
+    if (__atomic_load_n(mbh<uint32_t>(4), __ATOMIC_ACQUIRE) != 0u)
+      throw Error(NENYA_E_DEVICE, "a device wait for a posted challenge ran out of time");
   }
   // Enqueue: lift the challenge the host will post to slot s (flag value v)
   // into d; with k <= 32 its multiplication fragments go to `frag`
   // (kND * 4096 words, allocated by the caller) and mm points at them.
   void lift_posted(int s, uint32_t v, L* d, uint32_t* frag, MulMat* mm) {
-    const MbWait w{mbd<uint32_t>(mb_up_flag(s)), v, mbd<uint32_t>(0), spin_ticks};
+    const MbWait w{mbd<uint32_t>(mb_up_flag(s)), v, mbd<uint32_t>(0), wait_ticks};
     const L* src = mbd<L>(mb_up(s));
     const double lm = E * E + (T::K - 1) * 16.5 * PM;
     mm->r = d;
@@ -863,6 +898,7 @@ template <class T> struct Engine {
     DBuf<uint32_t> frags(pool, kMx<T> ? nv * kND<T> * 4096 : 1);
     MbGuard guard(*this);
     const bool pipe = pipeline_rounds();
+    wait_ticks = spin_ticks();
     // enqueue the sums of round r over tables of 2m entries; returns the flag value
     auto round_sums = [&](size_t r, size_t m) -> uint32_t {
       const bool split = m <= 4096;
@@ -893,21 +929,22 @@ template <class T> struct Engine {
       };
       if (!pipe) host_round();
       lift_posted((int)r, vup, rc, frags.p + (kMx<T> ? r * kND<T> * 4096 : 0), &mr);
+      // the next round's eq factor eq_one(rt[r], r_r) rides along with the fold
+      Side sd;
+      if (r + 1 < nv) sd = Side{1, drt.p + r * E, rc, dsc.p};
       // tables at a common stride fold in one launch
       bool uniform = true;
       uniform = ntab > 1 && cur[1] > cur[0] && (cur[1] - cur[0]) % E == 0;
       for (int t = 2; t < ntab; t++) uniform = uniform && cur[t] == cur[0] + (cur[1] - cur[0]) * t;
       if (uniform && ntab > 1) {
-        fold(mr, cur[0], work.p, m, ntab, (uint64_t)(cur[1] - cur[0]) / E, N / 2);
+        fold(mr, cur[0], work.p, m, ntab, (uint64_t)(cur[1] - cur[0]) / E, N / 2, sd);
       } else {
-        for (int t = 0; t < ntab; t++) fold(mr, cur[t], work.p + (size_t)t * (N / 2) * E, m, 1, 0, 0);
+        for (int t = 0; t < ntab; t++) fold(mr, cur[t], work.p + (size_t)t * (N / 2) * E, m, 1, 0, 0, t ? Side{} : sd);
       }
       for (int t = 0; t < ntab; t++) cur[t] = work.p + (size_t)t * (N / 2) * E;
       len = m;
       uint32_t vnext = 0;
       if (r + 1 < nv) {
-        launch("k_eq_one", 3 * EB, 32 * PM, (k_eq_one<T>), dim3(1), dim3(64), (const L*)(drt.p + r * E), (const L*)rc,
-               dsc.p);
         marg(dsc.p, eg.p, m / 2);
         vnext = round_sums(r + 1, m / 2);
       }
@@ -916,6 +953,7 @@ template <class T> struct Engine {
     }
     o.chal.resize(nv);
     d2h(o.chal.data(), dch.p, nv * sizeof(G));
+    check_waits();
     o.finals = pick(work.p, N / 2, ntab);
     return o;
   }
@@ -945,6 +983,7 @@ template <class T> struct Engine {
     DBuf<uint32_t> frags(pool, kMx<T> ? nv * kND<T> * 4096 : 1);
     MbGuard guard(*this);
     const bool pipe = pipeline_rounds();
+    wait_ticks = spin_ticks();
     auto round_sums = [&](size_t r, size_t m) -> uint32_t {
       if constexpr (kMx<T>) {
         const unsigned nwg = gram_lin(f, fs, g, gs, (uint64_t)ntab, (uint64_t)m, (r == 0 && NAF0 == 16) ? 64 : 128, part.p);
@@ -976,8 +1015,8 @@ template <class T> struct Engine {
       };
       if (!pipe) host_round();
       lift_posted((int)r, vup, rc, frags.p + (kMx<T> ? r * kND<T> * 4096 : 0), &mr);
-      launch("k_lin_claim", 5 * EB, 64 * PM, (k_lin_claim<T>), dim3(1), dim3(64), (const L*)dq.p, (const L*)rc, dc.p);
-      fold2(mr, m, f, fw.p, ntab, fs, H, g, gw.p, ntab, gs, H);
+      // the next claim q_r(r_r) rides along with the folds
+      fold2(mr, m, f, fw.p, ntab, fs, H, g, gw.p, ntab, gs, H, Side{2, dq.p, rc, dc.p});
       f = fw.p;
       g = gw.p;
       fs = gs = H;
@@ -988,6 +1027,7 @@ template <class T> struct Engine {
     }
     o.chal.resize(nv);
     d2h(o.chal.data(), dch.p, nv * sizeof(G));
+    check_waits();
     o.finals = pick(f, fs, ntab);
     std::vector<G> gf = pick(g, gs, ntab);
     o.finals.insert(o.finals.end(), gf.begin(), gf.end());

@@ -37,3 +37,21 @@ def test_shape_errors_are_codes():
     with pytest.raises(na.NenyaError) as e:
         na.MimdSnarkParams.secure(5, 5, (As, Bs, Cs), k=16)
     assert e.value.code == 1
+
+
+def test_challenge_wait_time_limit_fails_the_proof(monkeypatch):
+    """A device-side wait for a host-posted sumcheck challenge that runs out
+    of time fails the proof with a device error instead of returning a proof
+    built from a stale challenge.  The condition is forced from the host by a
+    zero wait limit (every kernel still exits normally); later proofs are
+    unaffected.  Exercises the single-caller pipelined round loop."""
+    print("challenge wait limit", flush=True)
+    A, B, Cm, w = general_instance(10, 32, 5)
+    gp = na.GeneralSnarkParams.secure(5, (A, B, Cm), k=32)
+    good = na.prove_general(w, gp)
+    monkeypatch.setenv("NENYA_WAIT_LIMIT_US", "0")
+    with pytest.raises(na.NenyaError) as e:
+        na.prove_general(w, gp)
+    assert e.value.code == 3
+    monkeypatch.delenv("NENYA_WAIT_LIMIT_US")
+    assert na.prove_general(w, gp) == good
