@@ -1451,6 +1451,26 @@ __global__ void k_merkle_paths(const uint32_t* nodes, uint64_t n, const uint32_t
   for (int k = 0; k < 8; k++) dst[k] = nodes[node * 8 + k];
 }
 
+// Proof-section download: `bytes` (a multiple of 4) from device memory to
+// the pinned proof buffer through its device address; 16-byte stores when
+// the destination allows them.  Runs on the download stream with a small
+// grid beside the prover's kernels.
+__global__ void __launch_bounds__(256) k_copy_out(const uint8_t* src, uint8_t* dst, uint64_t bytes) {
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nth = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t done = 0;
+  if (((uintptr_t)dst & 15) == 0) {
+    const uint64_t n16 = bytes / 16;
+    const uint4* s4 = reinterpret_cast<const uint4*>(src);
+    uint4* d4 = reinterpret_cast<uint4*>(dst);
+    for (uint64_t i = tid; i < n16; i += nth) d4[i] = s4[i];
+    done = n16 * 16;
+  }
+  const uint64_t nw = (bytes - done) / 4;
+  const uint32_t* s1 = reinterpret_cast<const uint32_t*>(src + done);
+  uint32_t* d1 = reinterpret_cast<uint32_t*>(dst + done);
+  for (uint64_t i = tid; i < nw; i += nth) d1[i] = s1[i];
+}
+
 // Serialized rotated m-bars (pcs.rs:426-471): per rotation a u64 length
 // `cnt`, then the rotation's `per` 8-byte words of src.
 __global__ void k_len_prefixed(const uint64_t* src, uint64_t nrec, uint64_t per, uint64_t cnt, uint64_t* dst) {

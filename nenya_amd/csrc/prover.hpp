@@ -217,6 +217,14 @@ template <class T> struct Engine {
   DBuf<uint32_t> d_count;  // last-block counter of k_sum_post
   DBuf<L> d_sums;          // reduced round sums (3 elements)
 
+  // Proof sections leave the device on a second stream (default priority,
+  // created on first use) while the prover's later stages run; their device
+  // buffers are held until finish_downloads(), which every proof calls
+  // before it returns and again before it starts (after a failed proof).
+  hipStream_t cst = nullptr;
+  hipEvent_t cev = nullptr;
+  std::vector<std::pair<void*, size_t>> held;
+
   // Launch on the engine's stream, bracketed by events when profiling.
   template <class Kern, class... Args>
   void launch(const char* name, double bytes, double macs, Kern kern, dim3 grid, dim3 block, Args... args) {
@@ -505,6 +513,48 @@ template <class T> struct Engine {
     return b;
   }
 
+  // ---------------- proof downloads ----------------
+  // Enqueue the copy of a finished device buffer into the pinned proof
+  // buffer at host address `dst` (after the prover stream's work so far);
+  // the buffer is held until finish_downloads().
+  template <class X> void download(uint8_t* dst, DBuf<X>& src, size_t bytes) {
+    if (!bytes) return;
+    if (!pipelined) {
+      // contexts of a multi-worker server: in order on the prover stream
+      // (their streams already share the hardware queues)
+      HIP_CHECK(hipMemcpyAsync(dst, src.p, bytes, hipMemcpyDeviceToHost, st));
+      return;
+    }
+    if (!cst) {
+      HIP_CHECK(hipStreamCreate(&cst));
+      HIP_CHECK(hipEventCreateWithFlags(&cev, hipEventDisableTiming));
+    }
+    uint8_t* ddst = nullptr;
+    HIP_CHECK(hipHostGetDevicePointer((void**)&ddst, dst, 0));
+    HIP_CHECK(hipEventRecord(cev, st));
+    HIP_CHECK(hipStreamWaitEvent(cst, cev, 0));
+    hipLaunchKernelGGL(k_copy_out, dim3((unsigned)std::min<uint64_t>((bytes + 4095) / 4096, 64)), dim3(256), 0, cst,
+                       (const uint8_t*)src.p, ddst, (uint64_t)bytes);
+    HIP_CHECK(hipGetLastError());
+    held.push_back({src.p, src.n * sizeof(X)});
+    src.p = nullptr;
+    src.n = 0;
+  }
+  void finish_downloads() {
+    if (!cst) return;
+    HIP_CHECK(hipStreamSynchronize(cst));
+    for (auto& h : held) pool->put(h.first, h.second);
+    held.clear();
+  }
+  ~Engine() {
+    if (cst) {
+      (void)hipStreamSynchronize(cst);
+      for (auto& h : held) pool->put(h.first, h.second);
+      (void)hipStreamDestroy(cst);
+    }
+    if (cev) (void)hipEventDestroy(cev);
+  }
+
   // ---------------- mailbox ----------------
   static constexpr size_t mb_bytes() { return 1024 + (size_t)MB_SLOTS * 4 * E * sizeof(L); }
   static constexpr size_t mb_down_flag(int s) { return 64 + 4 * (size_t)s; }
@@ -772,14 +822,13 @@ template <class T> struct Engine {
       launch("k_merkle_paths", 0, 0, k_merkle_paths, dim3(grid_for(nq * plen, 64, 1u << 20)), dim3(64),
              (const uint32_t*)c.nodes.p, (uint64_t)n, (const uint32_t*)didx.p, (uint64_t)nq, plen, paths.p);
     out.u64(nq);
-    HIP_CHECK(hipMemcpyAsync(out.take(colb), cols.p, colb, hipMemcpyDeviceToHost, st));
+    download(out.take(colb), cols, colb);
     out.u64(nq);
     if (plen > 0) {
-      HIP_CHECK(hipMemcpyAsync(out.take(pathb), paths.p, pathb, hipMemcpyDeviceToHost, st));
+      download(out.take(pathb), paths, pathb);
     } else {
       for (size_t q = 0; q < nq; q++) out.u64(0);
     }
-    sync();
   }
 
   // Pcs::open / open_stacked (pcs.rs:547-591, 734-779) and R2Pcs::open /
@@ -826,8 +875,9 @@ template <class T> struct Engine {
              c.msg, rows, (uint64_t)P.k2, mbar.p, (uint64_t)0);
     G y = dot<32>(mbar.p, e1.p, P.k2);
     out.u64(P.k2);
-    HIP_CHECK(hipMemcpyAsync(out.take(P.k2 * E * sizeof(L)), mbar.p, P.k2 * E * sizeof(L), hipMemcpyDeviceToHost, st));
+    uint8_t* mdst = out.take(P.k2 * E * sizeof(L));
     open_tail(pc, c, mbar.p, P.k2, out);
+    download(mdst, mbar, P.k2 * E * sizeof(L));
     return y;
   }
 
@@ -852,7 +902,7 @@ template <class T> struct Engine {
     DBuf<uint64_t> rec(pool, (size_t)nl * (per + 1));
     launch("k_len_prefixed", 2.0 * recb, 0, k_len_prefixed, dim3(grid_for((uint64_t)nl * (per + 1), 256, 4096)),
            dim3(256), (const uint64_t*)mbar.p, (uint64_t)nl, per, (uint64_t)P.k2, rec.p);
-    HIP_CHECK(hipMemcpyAsync(out.take(recb), rec.p, recb, hipMemcpyDeviceToHost, st));
+    download(out.take(recb), rec, recb);
     open_tail(pc, c, mbar.p, (size_t)nl * P.k2, out);
     return ys;
   }

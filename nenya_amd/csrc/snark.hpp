@@ -160,6 +160,7 @@ template <class T> struct SimdParams : ParamsBase {
 
   ProofView prove(Ctx<T>& cx, const HostCoo& A, const HostCoo& B, const HostCoo& Cm, const L* wit /*32 x m*/) {
     Engine<T>& e = *cx.eng;
+    e.finish_downloads();  // a failed earlier proof may have left copies in flight
     Pool& pool = cx.pool;
     const size_t pool_base = pool.mark();
     hipStream_t st = cx.st;
@@ -291,6 +292,7 @@ template <class T> struct SimdParams : ParamsBase {
     if ((!(yw == final_w) || !(ye == e_at_r)) && !skip_selfcheck())
       throw Error(NENYA_E_DEVICE, "internal opening self-check failed");
     if (!out.full()) throw Error(NENYA_E_DEVICE, "proof size mismatch");
+    e.finish_downloads();
     cx.last_timing = stg.t;
     if (prof.on) prof.collect();
     note_footprint(pool.peak - pool_base);
@@ -516,6 +518,7 @@ template <class T> struct MimdParams : ParamsBase {
 
   ProofView prove(Ctx<T>& cx, const L* wit /* 32 x m */) {
     Engine<T>& e = *cx.eng;
+    e.finish_downloads();  // a failed earlier proof may have left copies in flight
     Pool& pool = cx.pool;
     const size_t pool_base = pool.mark();
     KProf& prof = cx.prof;
@@ -632,6 +635,7 @@ template <class T> struct MimdParams : ParamsBase {
     const bool ok = em_val == mp.finals[3] && abc_val == mp.finals[0] + gam * mp.finals[1] + gam2 * mp.finals[2] &&
                     yw == final_w;
     if (!ok && !skip_selfcheck()) throw Error(NENYA_E_DEVICE, "internal opening self-check failed");
+    e.finish_downloads();
     cx.last_timing = stg.t;
     if (prof.on) prof.collect();
     note_footprint(pool.peak - pool_base);
@@ -827,6 +831,7 @@ template <class T> struct GeneralParams : ParamsBase {
 
   ProofView prove_device(Ctx<T>& cx, const L* w /* device, N limbs */) {
     Engine<T>& e = *cx.eng;
+    e.finish_downloads();  // a failed earlier proof may have left copies in flight
     Pool& pool = cx.pool;
     const size_t pool_base = pool.mark();
     KProf& prof = cx.prof;
@@ -961,6 +966,7 @@ template <class T> struct GeneralParams : ParamsBase {
               abc_val == mp.finals[0] + gam * mp.finals[1] + gam2 * mp.finals[2];
     for (int l = 0; l < 32; l++) ok = ok && ys[l] == lp.finals[l];
     if (!ok && !skip_selfcheck()) throw Error(NENYA_E_DEVICE, "internal opening self-check failed");
+    e.finish_downloads();
     cx.last_timing = stg.t;
     if (prof.on) prof.collect();
     note_footprint(pool.peak - pool_base);
