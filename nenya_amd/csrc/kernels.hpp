@@ -106,6 +106,7 @@ __device__ __forceinline__ void acc_mul(Acc<T>& acc, const LT<T>* A, const LT<T>
 template <class T>
 __device__ __forceinline__ void acc_mul_range(Acc<T>& acc, const LT<T>* A, const LT<T>* B, int t, int i0, int i1) {
   using C = CT<T>;
+#pragma unroll 4
   for (int i = i0; i < i1; i++) {
     C a[4], b[4], p[7];
     ld4<T>(A + 4 * i, a);
@@ -303,38 +304,65 @@ __global__ void __launch_bounds__(GB) k_eq_level(const LT<T>* r, const LT<T>* pr
   }
 }
 
-// The first nl levels of build_eq_table in one block of 32 groups (levels
-// of at most 32 products): from the entry 1, level l writes 2^(l+1) entries,
-// alternating a -> b -> a ... (level 0 reads nothing, so a needs no
-// initialisation); the table ends in b if nl is odd, else in a.
+// The first nl levels of build_eq_table in one block (levels of at most 32
+// products): from the entry 1, level l writes 2^(l+1) entries, alternating
+// a -> b -> a ... (level 0 reads nothing, so a needs no initialisation).
 constexpr int EQ_HEAD = 6;
+
+// Level l has 2^l products prev[p] * r_l (as k_eq_level); a product takes
+// S = min(8, 32 / 2^l) 32-lane groups, group q of them summing the pair
+// terms [32q/S, 32(q+1)/S), so the early levels run at a fraction of one
+// product's latency.  The entries stay in LDS between levels (each level
+// reads all of them before it writes any) and only the last level goes to
+// memory: to b if nl is odd, else to a.
 template <class T>
 __global__ void __launch_bounds__(1024) k_eq_head(const LT<T>* r, LT<T>* a, LT<T>* b, int nl) {
-  __shared__ __attribute__((aligned(16))) LT<T> sr[E];
+  using C = CT<T>;
+  __shared__ __attribute__((aligned(16))) LT<T> sr[EQ_HEAD][E];
   __shared__ __attribute__((aligned(16))) LT<T> sh[32][E];
+  __shared__ C red[32][14][32];
   const int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  LT<T>* out = (nl & 1) ? b : a;
+  if (g < nl) stage<T>(sr[g], r + (uint64_t)g * E, t);
+  if (g == 0) {
+    const C one[4] = {(C)(t == 0 ? 1 : 0), 0, 0, 0};
+    st4<T>(sh[0] + 4 * t, one);
+  }
   for (int l = 0; l < nl; l++) {
-    const LT<T>* prev = (l & 1) ? b : a;
-    LT<T>* next = (l & 1) ? a : b;
-    const bool ok = g < (1 << l);
-    if (threadIdx.x < 32) stage<T>(sr, r + (uint64_t)l * E, t);
-    CT<T> p[4] = {(CT<T>)(t == 0 ? 1 : 0), 0, 0, 0};
-    if (ok) {
-      if (l > 0) ld4<T>(prev + (uint64_t)g * E + 4 * t, p);
-      st4<T>(sh[g] + 4 * t, p);
-    }
-    __syncthreads();
+    const int P = 1 << l, S = P >= 32 ? 1 : (32 / P < 8 ? 32 / P : 8), span = 32 / S;
+    const int p = g / S, q = g % S;
+    const bool active = p < P;
+    __syncthreads();  // the previous level's entries (or the start values) are in place
+    C pv[4];
+    if (active && q == 0) ld4<T>(sh[p] + 4 * t, pv);
     Acc<T> acc;
     acc.clear();
-    if (ok) acc_mul<T>(acc, sr, sh[g], t);
-    CT<T> o[4];
-    acc_final<T>(acc, t, o);
-    if (ok) {
-      CT<T> lo[4] = {p[0] - o[0], p[1] - o[1], p[2] - o[2], p[3] - o[3]};
-      st4<T>(next + (2 * (uint64_t)g + 1) * E + 4 * t, o);
-      st4<T>(next + (2 * (uint64_t)g) * E + 4 * t, lo);
+    if (active) acc_mul_range<T>(acc, sr[l], sh[p], t, q * span, (q + 1) * span);
+    if (active && q > 0)
+#pragma unroll
+      for (int c = 0; c < 7; c++) {
+        red[g][c][t] = acc.c[c];
+        red[g][7 + c][t] = acc.h[c];
+      }
+    __syncthreads();  // every read of this level's entries is done
+    if (active && q == 0) {
+      for (int k = 1; k < S; k++)
+#pragma unroll
+        for (int c = 0; c < 7; c++) {
+          acc.c[c] += red[g + k][c][t];
+          acc.h[c] += red[g + k][7 + c][t];
+        }
+      C o[4];
+      acc_final<T>(acc, t, o);
+      C lo[4] = {pv[0] - o[0], pv[1] - o[1], pv[2] - o[2], pv[3] - o[3]};
+      if (l == nl - 1) {
+        st4<T>(out + (2 * (uint64_t)p + 1) * E + 4 * t, o);
+        st4<T>(out + (2 * (uint64_t)p) * E + 4 * t, lo);
+      } else {
+        st4<T>(sh[2 * p + 1] + 4 * t, o);
+        st4<T>(sh[2 * p] + 4 * t, lo);
+      }
     }
-    __syncthreads();
   }
 }
 
@@ -592,7 +620,29 @@ __global__ void __launch_bounds__(1024) k_sum_post(const LT<T>* partials, uint64
   __shared__ bool last;
   const bool single = gridDim.x == 1;
   const int l = threadIdx.x & 31, sl = threadIdx.x >> 5;
-  for (int piece = single ? 0 : (int)blockIdx.x; piece < (single ? ns * 4 : (int)blockIdx.x + 1); piece++) {
+  if (single) {
+    // every (sum, limb) column at once: S slices of the blocks per column
+    const int cols = ns * E, S = 1024 / cols, o = threadIdx.x % cols, sv = threadIdx.x / cols;
+    C a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (sv < S) {
+      const int s = o / E, li = o % E;
+      uint64_t bb = sv;
+      for (; bb + 7 * (uint64_t)S < nblocks; bb += 8 * (uint64_t)S)
+#pragma unroll
+        for (int u = 0; u < 8; u++) a[u] += partials[((bb + (uint64_t)S * u) * ns + s) * E + li];
+      for (; bb < nblocks; bb += S) a[0] += partials[(bb * ns + s) * E + li];
+    }
+    C acc = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    C* part = &red[0][0];  // 33 * 32 >= 1024 slots
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < cols) {
+      for (int k = 1; k < S; k++) acc += part[threadIdx.x + k * cols];
+      ssum[threadIdx.x / E][threadIdx.x % E] = acc;
+    }
+    __syncthreads();
+  }
+  for (int piece = (int)blockIdx.x; !single && piece < (int)blockIdx.x + 1; piece++) {
     const int s = piece / 4, l0 = (piece % 4) * 32;
     C a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     uint64_t b = sl;
@@ -605,11 +655,8 @@ __global__ void __launch_bounds__(1024) k_sum_post(const LT<T>* partials, uint64
     __syncthreads();
     if (sl == 0) {
       for (int k = 1; k < 32; k++) acc += red[k][l];
-      if (single) ssum[s][l0 + l] = acc;
-      else {
-        sums[s * E + l0 + l] = (LT<T>)acc;
-        __threadfence();  // the writing wave publishes its piece
-      }
+      sums[s * E + l0 + l] = (LT<T>)acc;
+      __threadfence();  // the writing wave publishes its piece
     }
     __syncthreads();
   }

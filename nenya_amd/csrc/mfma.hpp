@@ -541,63 +541,89 @@ __global__ void __launch_bounds__(256) k_cgemm(const uint32_t* frag, const LT<T>
 // m-bars on the matrix cores (pcs.rs:554-561, 984-1001):
 //   out[l][j] = sum_{i < rows} e_i X[l * lstride + i * k2 + j]
 // with the M_{e_i} fragments (frag + i * ND * 4096 words, from k_mulmat).
-// A block takes 32 consecutive outputs of one l; wave w computes output rows
-// 32w..32w+31 and keeps one accumulator per digit class over all i (the
-// i32 accumulators wrap mod 2^32, so the sum over i is exact).
-// blockIdx.y = s of nsplit takes rows [s*rows/nsplit, (s+1)*rows/nsplit) and
-// writes its partial sums to out + s * nl * k2 elements (summed afterwards by
-// k_sum_split when nsplit > 1).
+// A block takes four tiles of 32 consecutive outputs (tile = l * tiles per
+// layer + j / 32), wave w tile 4b + w, all four output row-blocks; for each
+// row i the block stages M_{e_i} (ND * 16 KB) in LDS once for its four
+// tiles, each wave forms the digit planes of its 32 X elements and runs a
+// Horner pass over the digit classes (as in the fold), adding the result to
+// its running sums (i32 wrap = mod 2^32, so the sum over i is exact).
+// Block b takes split s = b % nsplit, rows [s*rows/nsplit, (s+1)*rows/nsplit),
+// and quads b / nsplit, b / nsplit + gridDim.x / nsplit, ...; it writes its
+// partial sums to out + s * nl * k2 elements (summed afterwards by
+// k_sum_split when nsplit > 1).  Workgroups are dealt to the eight XCDs
+// round-robin, so with nsplit a multiple of 8 an XCD's blocks share one
+// eighth of the rows and their fragments stay in that XCD's L2.
 template <class T>
 __global__ void __launch_bounds__(256) k_mbar_mx(const uint32_t* frag, const LT<T>* X, uint64_t rows, uint64_t k2,
-                                                 uint64_t lstride, uint64_t nl, LT<T>* out) {
+                                                 uint64_t lstride, uint64_t nl, uint64_t nsplit, LT<T>* out) {
   using C = CT<T>;
   constexpr int ND = kND<T>;
-  const int l64 = threadIdx.x & 63, col = l64 & 31, h = l64 >> 5, rb = threadIdx.x >> 6;
+  __shared__ v4i sA[ND * 16 * 64];
+  const int l64 = threadIdx.x & 63, col = l64 & 31, h = l64 >> 5, w = threadIdx.x >> 6;
   const uint64_t tpl = (k2 + 31) / 32, ntiles = tpl * nl;
-  const uint64_t i0 = rows * blockIdx.y / gridDim.y, i1 = rows * (blockIdx.y + 1) / gridDim.y;
-  out += (uint64_t)blockIdx.y * nl * k2 * E;
-  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const uint64_t l = tile / tpl, j = (tile % tpl) * 32 + col;
-    const bool ok = j < k2;
+  const uint64_t sp = blockIdx.x % nsplit, nq = gridDim.x / nsplit;
+  const uint64_t i0 = rows * sp / nsplit, i1 = rows * (sp + 1) / nsplit;
+  out += sp * nl * k2 * E;
+  for (uint64_t tb = (uint64_t)(blockIdx.x / nsplit) * 4; tb < ntiles; tb += nq * 4) {
+    const uint64_t tile = tb + w;
+    const uint64_t l = tile < ntiles ? tile / tpl : 0, j = (tile < ntiles ? (tile % tpl) * 32 : k2) + col;
+    const bool ok = tile < ntiles && j < k2;
     const LT<T>* xl = X + (l * lstride + (ok ? j : 0)) * E;
-    v16i acc[ND];
+    v16i tot[4];
 #pragma unroll
-    for (int d = 0; d < ND; d++) acc[d] = (v16i)0;
+    for (int rb = 0; rb < 4; rb++) tot[rb] = (v16i)0;
     for (uint64_t i = i0; i < i1; i++) {
-      const v4i* A = reinterpret_cast<const v4i*>(frag + i * ND * 4096);
-      const LT<T>* x = xl + i * k2 * E;
+      __syncthreads();  // the previous row's fragments are no longer read
+      const v4i* gA = reinterpret_cast<const v4i*>(frag + i * ND * 4096);
+      for (int x = threadIdx.x; x < ND * 16 * 64; x += 256) sA[x] = gA[x];
+      v4i B[4][ND];
+      const LT<T>* xi = xl + i * k2 * E;
 #pragma unroll
       for (int kb = 0; kb < 4; kb++) {
         uint32_t v[16];
         if (ok) {
-          ld16k<T>(x, kb, h, v);
+          ld16k<T>(xi, kb, h, v);
         } else {
 #pragma unroll
           for (int b = 0; b < 16; b++) v[b] = 0;
         }
-        v4i B[ND];
-        planes16<ND>(v, B);
+        planes16<ND>(v, B[kb]);
+      }
+      __syncthreads();
 #pragma unroll
-        for (int pd = 0; pd < ND; pd++) {
-          const v4i a = A[frag_at(pd, kb, rb) * 64 + l64];
+      for (int rp = 0; rp < 4; rp += 2) {
+        // keeps the fragment reads next to their matrix ops
+        asm volatile("" ::: "memory");
+        v16i acc[2] = {(v16i)0, (v16i)0};
 #pragma unroll
-          for (int q = 0; pd + q < ND; q++) acc[pd + q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, B[q], acc[pd + q], 0, 0, 0);
+        for (int sd = ND - 1; sd >= 0; sd--) {
+          if (sd != ND - 1) {
+            acc[0] <<= 8;
+            acc[1] <<= 8;
+          }
+#pragma unroll
+          for (int p = 0; p <= sd; p++)
+#pragma unroll
+            for (int kb = 0; kb < 4; kb++)
+#pragma unroll
+              for (int u = 0; u < 2; u++)
+                acc[u] = __builtin_amdgcn_mfma_i32_32x32x32_i8(sA[frag_at(p, kb, rp + u) * 64 + l64], B[kb][sd - p],
+                                                               acc[u], 0, 0, 0);
         }
+        tot[rp] += acc[0];
+        tot[rp + 1] += acc[1];
       }
     }
     if (ok) {
 #pragma unroll
-      for (int jj = 0; jj < 4; jj++) {
-        C y4[4];
+      for (int rb = 0; rb < 4; rb++)
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-          uint32_t y = 0;
+        for (int jj = 0; jj < 4; jj++) {
+          C y4[4];
 #pragma unroll
-          for (int d = 0; d < ND; d++) y += (uint32_t)acc[d][4 * jj + c] << (8 * d);
-          y4[c] = (C)y;
+          for (int c = 0; c < 4; c++) y4[c] = (C)(uint32_t)tot[rb][4 * jj + c];
+          st4<T>(out + (l * k2 + j) * E + 32 * rb + 8 * jj + 4 * h, y4);
         }
-        st4<T>(out + (l * k2 + j) * E + 32 * rb + 8 * jj + 4 * h, y4);
-      }
     }
   }
 }

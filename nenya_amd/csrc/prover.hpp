@@ -569,7 +569,7 @@ template <class T> struct Engine {
   uint32_t post_round(const L* part, uint64_t nparts, int ns, int mode, const L* c, L* qd, int s) {
     const uint32_t v = ++mseq;
     // few partials: one block, no cross-block handoff
-    const unsigned grid = nparts <= 256 ? 1 : ns * 4;
+    const unsigned grid = nparts <= 64 ? 1 : ns * 4;
     launch("k_sum_post", (double)nparts * ns * EB, 0, (k_sum_post<T>), dim3(grid), dim3(1024), part, nparts, ns, mode, c,
            d_sums.p, qd, mbd<L>(mb_down(s)), d_count.p, mbd<uint32_t>(mb_down_flag(s)), v);
     return v;
@@ -838,13 +838,16 @@ template <class T> struct Engine {
   void mbar_mx(const L* e, uint64_t rows, const L* X, uint64_t k2, uint64_t lstride, uint64_t nl, L* out) {
     DBuf<uint32_t> fr(pool, (size_t)rows * kND<T> * 4096);
     launch("k_mulmat", rows * (EB + kND<T> * 16384.0), 0, (k_mulmat<T>), dim3((unsigned)rows), dim3(256), e, fr.p);
-    const uint64_t tiles = nl * ((k2 + 31) / 32);
-    // split the row range over blocks until there are ~2048 blocks
-    const uint64_t nsplit = std::max<uint64_t>(1, std::min<uint64_t>(rows, (2048 + tiles - 1) / tiles));
+    const uint64_t tiles = nl * ((k2 + 31) / 32), quads = (tiles + 3) / 4;
+    // split the row range eight ways (one share per XCD), and further while
+    // there are fewer than ~512 blocks
+    uint64_t nsplit = std::min<uint64_t>(rows, 8);
+    while (nsplit * 2 <= rows && quads * nsplit < 512) nsplit *= 2;
+    const uint64_t nqb = std::min<uint64_t>(quads, 65535 / nsplit);
     DBuf<L> part(pool, nsplit > 1 ? nsplit * nl * k2 * E : 1);
     launch("k_mbar_mx", (double)nl * rows * k2 * EB + nsplit * nl * k2 * EB, (double)nl * rows * k2 * E * E,
-           (k_mbar_mx<T>), dim3((unsigned)std::min<uint64_t>(tiles, 65535), (unsigned)nsplit), dim3(256),
-           (const uint32_t*)fr.p, X, rows, k2, lstride, nl, nsplit > 1 ? part.p : out);
+           (k_mbar_mx<T>), dim3((unsigned)(nqb * nsplit)), dim3(256), (const uint32_t*)fr.p, X, rows, k2, lstride, nl,
+           nsplit, nsplit > 1 ? part.p : out);
     if (nsplit > 1)
       launch("k_sum_split", (nsplit + 1.0) * nl * k2 * EB, 0, (k_sum_split<T>), dim3(grid_for(nl * k2 * E, 256, 4096)),
              dim3(256), (const L*)part.p, nsplit, nl * k2, out);
