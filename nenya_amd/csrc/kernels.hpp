@@ -273,57 +273,33 @@ __global__ void __launch_bounds__(GB) k_marg(const LT<T>* scale, LT<T>* eg, uint
   }
 }
 
-// One doubling level of build_eq_table (eq.rs:35-58): from prev (len) to
-// next (2 len): next[2k+1] = prev[k]*r, next[2k] = prev[k] - next[2k+1].
-template <class T>
-__global__ void __launch_bounds__(GB) k_eq_level(const LT<T>* r, const LT<T>* prev, LT<T>* next, uint64_t len) {
-  __shared__ __attribute__((aligned(16))) LT<T> sr[E];
-  __shared__ __attribute__((aligned(16))) LT<T> sh[GPB][E];
-  int g = threadIdx.x >> 5, t = threadIdx.x & 31;
-  if (threadIdx.x < 32) stage<T>(sr, r, t);
-  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < len; base += (uint64_t)gridDim.x * GPB) {
-    uint64_t k = base + g;
-    bool ok = k < len;
-    CT<T> p[4];
-    if (ok) {
-      ld4<T>(prev + k * E + 4 * t, p);
-      st4<T>(sh[g] + 4 * t, p);
-    }
-    __syncthreads();
-    Acc<T> acc;
-    acc.clear();
-    if (ok) acc_mul<T>(acc, sr, sh[g], t);
-    CT<T> o[4];
-    acc_final<T>(acc, t, o);
-    if (ok) {
-      CT<T> lo[4] = {p[0] - o[0], p[1] - o[1], p[2] - o[2], p[3] - o[3]};
-      st4<T>(next + (2 * k + 1) * E + 4 * t, o);
-      st4<T>(next + (2 * k) * E + 4 * t, lo);
-    }
-    __syncthreads();
-  }
-}
-
-// The first nl levels of build_eq_table in one block (levels of at most 32
-// products): from the entry 1, level l writes 2^(l+1) entries, alternating
-// a -> b -> a ... (level 0 reads nothing, so a needs no initialisation).
+// build_eq_table (eq.rs:35-58) by chunks: the table over variables
+// r_0..r_{n-1} (r_0 the most significant index bit) is the outer product of
+// the tables over consecutive chunks of at most EQ_HEAD variables.
 constexpr int EQ_HEAD = 6;
 
-// Level l has 2^l products prev[p] * r_l (as k_eq_level); a product takes
-// S = min(8, 32 / 2^l) 32-lane groups, group q of them summing the pair
-// terms [32q/S, 32(q+1)/S), so the early levels run at a fraction of one
-// product's latency.  The entries stay in LDS between levels (each level
-// reads all of them before it writes any) and only the last level goes to
-// memory: to b if nl is odd, else to a.
+// The chunk tables, one block each: chunk j of k (n/k variables, one more
+// for the first n % k chunks) is written to out + off_j * E, off_j the
+// entries of the chunks before it.  Within a chunk, level l (as one doubling
+// level of build_eq_table: next[2p+1] = prev[p] r_l, next[2p] = prev[p] -
+// next[2p+1]) has 2^l products; a product takes S = min(8, 32 / 2^l)
+// 32-lane groups, group q of them summing the pair terms
+// [32q/S, 32(q+1)/S), so the early levels run at a fraction of one product's
+// latency.  The entries stay in LDS between levels (each level reads all of
+// them before it writes any); only the last level goes to memory.
 template <class T>
-__global__ void __launch_bounds__(1024) k_eq_head(const LT<T>* r, LT<T>* a, LT<T>* b, int nl) {
+__global__ void __launch_bounds__(1024) k_eq_chunks(const LT<T>* r, int n, int k, LT<T>* out) {
   using C = CT<T>;
   __shared__ __attribute__((aligned(16))) LT<T> sr[EQ_HEAD][E];
   __shared__ __attribute__((aligned(16))) LT<T> sh[32][E];
   __shared__ C red[32][14][32];
-  const int g = threadIdx.x >> 5, t = threadIdx.x & 31;
-  LT<T>* out = (nl & 1) ? b : a;
-  if (g < nl) stage<T>(sr[g], r + (uint64_t)g * E, t);
+  const int g = threadIdx.x >> 5, t = threadIdx.x & 31, j = blockIdx.x;
+  const int q0 = n / k, rem = n % k;
+  const int nl = q0 + (j < rem ? 1 : 0), start = j * q0 + (j < rem ? j : rem);
+  uint64_t off = 0;
+  for (int i = 0; i < j; i++) off += (uint64_t)1 << (q0 + (i < rem ? 1 : 0));
+  out += off * E;
+  if (g < nl) stage<T>(sr[g], r + (uint64_t)(start + g) * E, t);
   if (g == 0) {
     const C one[4] = {(C)(t == 0 ? 1 : 0), 0, 0, 0};
     st4<T>(sh[0] + 4 * t, one);
@@ -346,11 +322,11 @@ __global__ void __launch_bounds__(1024) k_eq_head(const LT<T>* r, LT<T>* a, LT<T
       }
     __syncthreads();  // every read of this level's entries is done
     if (active && q == 0) {
-      for (int k = 1; k < S; k++)
+      for (int kk = 1; kk < S; kk++)
 #pragma unroll
         for (int c = 0; c < 7; c++) {
-          acc.c[c] += red[g + k][c][t];
-          acc.h[c] += red[g + k][7 + c][t];
+          acc.c[c] += red[g + kk][c][t];
+          acc.h[c] += red[g + kk][7 + c][t];
         }
       C o[4];
       acc_final<T>(acc, t, o);
@@ -363,6 +339,31 @@ __global__ void __launch_bounds__(1024) k_eq_head(const LT<T>* r, LT<T>* a, LT<T
         st4<T>(sh[2 * p] + 4 * t, lo);
       }
     }
+  }
+}
+
+// out[a * nb + b] = A[a] * B[b]: the table over (A's variables, B's
+// variables) from the two tables.
+template <class T>
+__global__ void __launch_bounds__(GB) k_eq_outer(const LT<T>* A, uint64_t na, const LT<T>* B, uint64_t nb, LT<T>* out) {
+  __shared__ __attribute__((aligned(16))) LT<T> sa[GPB][E], sb[GPB][E];
+  const int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  const uint64_t total = na * nb;
+  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < total; base += (uint64_t)gridDim.x * GPB) {
+    const uint64_t x = base + g;
+    const bool ok = x < total;
+    if (ok) {
+      stage<T>(sa[g], A + (x / nb) * E, t);
+      stage<T>(sb[g], B + (x % nb) * E, t);
+    }
+    __syncthreads();
+    Acc<T> acc;
+    acc.clear();
+    if (ok) acc_mul<T>(acc, sa[g], sb[g], t);
+    CT<T> o[4];
+    acc_final<T>(acc, t, o);
+    if (ok) st4<T>(out + x * E + 4 * t, o);
+    __syncthreads();
   }
 }
 

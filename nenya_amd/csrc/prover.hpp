@@ -490,28 +490,39 @@ template <class T> struct Engine {
 
   // build_eq_table (eq.rs:35-58) for host points r[0..n): 2^n entries
   DBuf<L> eq_table(const G* r, size_t n) {
-    size_t len = size_t(1) << n;
-    DBuf<L> a(pool, len * E), b(pool, len * E);
     if (n == 0) {
+      DBuf<L> a(pool, E);
       d2d(a.p, d_one.p, E * sizeof(L));
       return a;
     }
     DBuf<L> rr = put(r, n);
-    // the narrow first levels in one block, then a launch per level
-    const int nh = (int)std::min<size_t>(n, EQ_HEAD);
-    launch("k_eq_head", 0, 0, (k_eq_head<T>), dim3(1), dim3(1024), (const L*)rr.p, a.p, b.p, nh);
-    L* prev = (nh & 1) ? b.p : a.p;
-    L* next = (nh & 1) ? a.p : b.p;
-    size_t cur = size_t(1) << nh;
-    for (size_t l = nh; l < n; l++) {
-      launch("k_eq_level", cur * 3 * EB, cur * 32 * PM, (k_eq_level<T>), dim3(grid_for(cur, GPB)), dim3(GB), rr.p + l * E,
-             prev, next, (uint64_t)cur);
-      std::swap(prev, next);
-      cur *= 2;
+    // chunk tables in one launch, then outer products from the last chunk leftwards
+    const int k = (int)((n + EQ_HEAD - 1) / EQ_HEAD);
+    std::vector<size_t> sz(k), off(k);
+    size_t tot = 0;
+    for (int j = 0; j < k; j++) {
+      sz[j] = n / k + ((size_t)j < n % k ? 1 : 0);
+      off[j] = tot;
+      tot += size_t(1) << sz[j];
     }
-    if (prev == a.p) return a;
-    return b;
+    DBuf<L> heads(pool, tot * E);
+    launch("k_eq_chunks", 0, 0, (k_eq_chunks<T>), dim3(k), dim3(1024), (const L*)rr.p, (int)n, k, heads.p);
+    if (k == 1) return heads;
+    DBuf<L> cur;
+    const L* tail = heads.p + off[k - 1] * E;
+    size_t len = size_t(1) << sz[k - 1];
+    for (int j = k - 2; j >= 0; j--) {
+      const size_t na = size_t(1) << sz[j];
+      DBuf<L> nx(pool, na * len * E);
+      launch("k_eq_outer", na * len * EB, na * len * 32 * PM, (k_eq_outer<T>), dim3(grid_for(na * len, GPB)), dim3(GB),
+             (const L*)(heads.p + off[j] * E), (uint64_t)na, tail, (uint64_t)len, nx.p);
+      cur = std::move(nx);
+      tail = cur.p;
+      len *= na;
+    }
+    return cur;
   }
+
 
   // ---------------- proof downloads ----------------
   // Enqueue the copy of a finished device buffer into the pinned proof
