@@ -54,6 +54,28 @@ template <int ND> __device__ __forceinline__ void planes16(const uint32_t v[16],
   }
 }
 
+// Workgroup barrier that orders LDS only: waits for this wave's LDS
+// operations (lgkmcnt 0; vmcnt and expcnt left at their maxima), so global
+// loads issued before it stay in flight.  __syncthreads() would also wait
+// for them (its release fence drains vmcnt).
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xC07F);
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// Copies one matrix's fragments (ND * 16 KB) into LDS with a 256-thread
+// block: every load is issued before the first LDS write, so the copy costs
+// one memory latency, not ND * 4 of them.
+template <int ND> __device__ __forceinline__ void stage_frags(const v4i* gA, v4i* sA) {
+  v4i t[ND * 4];
+#pragma unroll
+  for (int u = 0; u < ND * 4; u++) t[u] = gA[threadIdx.x + 256 * u];
+#pragma unroll
+  for (int u = 0; u < ND * 4; u++) sA[threadIdx.x + 256 * u] = t[u];
+}
+
 // Fragment index of M_r digit plane p, k-block kb, row-block rb (16 bytes per lane).
 __device__ __forceinline__ int frag_at(int p, int kb, int rb) { return (p * 4 + kb) * 4 + rb; }
 
@@ -319,7 +341,7 @@ __global__ void __launch_bounds__(256) k_fold_mx(const uint32_t* frag, FoldJob<T
     fold_tile<T, 1, true>(gA, J, blockIdx.x, threadIdx.x >> 6);
     return;
   }
-  for (int x = threadIdx.x; x < ND * 16 * 64; x += 256) sA[x] = gA[x];
+  stage_frags<ND>(gA, sA);
   __syncthreads();
   for (uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntiles; tile += (uint64_t)nfold * 4) {
     // keeps the A-fragment reads inside the loop (hoisted, they would pin
@@ -391,7 +413,7 @@ __global__ void __launch_bounds__(256) k_mulfix_mx(const uint32_t* frag, const L
   __shared__ v4i sA[ND * 16 * 64];
   const int i = blockIdx.y, nmat = gridDim.y;
   const v4i* gA = reinterpret_cast<const v4i*>(frag) + (size_t)i * ND * 16 * 64;
-  for (int q = threadIdx.x; q < ND * 16 * 64; q += 256) sA[q] = gA[q];
+  stage_frags<ND>(gA, sA);
   __syncthreads();
   const uint64_t ntiles = (n + 31) / 32;
   for (uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntiles; tile += (uint64_t)gridDim.x * 4) {
@@ -544,9 +566,10 @@ __global__ void __launch_bounds__(256) k_cgemm(const uint32_t* frag, const LT<T>
 // A block takes four tiles of 32 consecutive outputs (tile = l * tiles per
 // layer + j / 32), wave w tile 4b + w, all four output row-blocks; for each
 // row i the block stages M_{e_i} (ND * 16 KB) in LDS once for its four
-// tiles, each wave forms the digit planes of its 32 X elements and runs a
-// Horner pass over the digit classes (as in the fold), adding the result to
-// its running sums (i32 wrap = mod 2^32, so the sum over i is exact).
+// tiles, each wave forms the digit planes of its 32 X elements (loaded
+// during the previous row's matrix work) and accumulates one digit class per
+// accumulator, adding sum_d acc_d << 8d to its running sums (i32 wrap =
+// mod 2^32, so the sum over i is exact).
 // Block b takes split s = b % nsplit, rows [s*rows/nsplit, (s+1)*rows/nsplit),
 // and quads b / nsplit, b / nsplit + gridDim.x / nsplit, ...; it writes its
 // partial sums to out + s * nl * k2 elements (summed afterwards by
@@ -554,7 +577,7 @@ __global__ void __launch_bounds__(256) k_cgemm(const uint32_t* frag, const LT<T>
 // round-robin, so with nsplit a multiple of 8 an XCD's blocks share one
 // eighth of the rows and their fragments stay in that XCD's L2.
 template <class T>
-__global__ void __launch_bounds__(256) k_mbar_mx(const uint32_t* frag, const LT<T>* X, uint64_t rows, uint64_t k2,
+__global__ void __launch_bounds__(256, 2) k_mbar_mx(const uint32_t* frag, const LT<T>* X, uint64_t rows, uint64_t k2,
                                                  uint64_t lstride, uint64_t nl, uint64_t nsplit, LT<T>* out) {
   using C = CT<T>;
   constexpr int ND = kND<T>;
@@ -573,45 +596,42 @@ __global__ void __launch_bounds__(256) k_mbar_mx(const uint32_t* frag, const LT<
 #pragma unroll
     for (int rb = 0; rb < 4; rb++) tot[rb] = (v16i)0;
     for (uint64_t i = i0; i < i1; i++) {
-      __syncthreads();  // the previous row's fragments are no longer read
+      lds_barrier();  // the previous row's fragments are no longer read
       const v4i* gA = reinterpret_cast<const v4i*>(frag + i * ND * 4096);
-      for (int x = threadIdx.x; x < ND * 16 * 64; x += 256) sA[x] = gA[x];
+      stage_frags<ND>(gA, sA);
       v4i B[4][ND];
-      const LT<T>* xi = xl + i * k2 * E;
 #pragma unroll
       for (int kb = 0; kb < 4; kb++) {
         uint32_t v[16];
         if (ok) {
-          ld16k<T>(xi, kb, h, v);
+          ld16k<T>(xl + i * k2 * E, kb, h, v);
         } else {
 #pragma unroll
           for (int b = 0; b < 16; b++) v[b] = 0;
         }
         planes16<ND>(v, B[kb]);
       }
-      __syncthreads();
+      lds_barrier();  // the fragments are in LDS
 #pragma unroll
-      for (int rp = 0; rp < 4; rp += 2) {
+      for (int rb = 0; rb < 4; rb++) {
         // keeps the fragment reads next to their matrix ops
         asm volatile("" ::: "memory");
-        v16i acc[2] = {(v16i)0, (v16i)0};
+        // one accumulator per digit class: each fragment is read from LDS
+        // once and used for every class it feeds
+        v16i acc[ND];
 #pragma unroll
-        for (int sd = ND - 1; sd >= 0; sd--) {
-          if (sd != ND - 1) {
-            acc[0] <<= 8;
-            acc[1] <<= 8;
+        for (int d = 0; d < ND; d++) acc[d] = (v16i)0;
+#pragma unroll
+        for (int p = 0; p < ND; p++)
+#pragma unroll
+          for (int kb = 0; kb < 4; kb++) {
+            const v4i af = sA[frag_at(p, kb, rb) * 64 + l64];
+#pragma unroll
+            for (int q = 0; p + q < ND; q++)
+              acc[p + q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af, B[kb][q], acc[p + q], 0, 0, 0);
           }
 #pragma unroll
-          for (int p = 0; p <= sd; p++)
-#pragma unroll
-            for (int kb = 0; kb < 4; kb++)
-#pragma unroll
-              for (int u = 0; u < 2; u++)
-                acc[u] = __builtin_amdgcn_mfma_i32_32x32x32_i8(sA[frag_at(p, kb, rp + u) * 64 + l64], B[kb][sd - p],
-                                                               acc[u], 0, 0, 0);
-        }
-        tot[rp] += acc[0];
-        tot[rp + 1] += acc[1];
+        for (int d = 0; d < ND; d++) tot[rb] += acc[d] << (8 * d);
       }
     }
     if (ok) {
