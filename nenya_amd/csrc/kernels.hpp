@@ -878,9 +878,13 @@ __device__ __forceinline__ void teich_lift_block(const LT<T>* src, LT<T>* el, co
   __syncthreads();
   {
     const int o = tid & (E - 1), h = tid >> 7;
+    // all 64 matrix entries in flight at once (one memory latency)
+    C f[64];
+#pragma unroll
+    for (int j = 0; j < 64; j++) f[j] = (C)frob_t[(size_t)(64 * h + j) * E + o];
     C s = 0;
-#pragma unroll 8
-    for (int j = 64 * h; j < 64 * h + 64; j++) s += (C)frob_t[(size_t)j * E + o] * sx[j];
+#pragma unroll
+    for (int j = 0; j < 64; j++) s += f[j] * sx[64 * h + j];
     if (h) part[o] = s;
     __syncthreads();
     if (!h) sx[o] = s + part[o];
@@ -982,10 +986,14 @@ __global__ void __launch_bounds__(GB) k_pack(const LT<T>* slots, const LT<T>* ba
   __shared__ __attribute__((aligned(16))) LT<T> sb[32 * E];
   __shared__ CT<T> ssl[GPB][32];
   int g = threadIdx.x >> 5, t = threadIdx.x & 31;
-  for (int i = threadIdx.x; i < 32 * E / 4; i += GB) {
-    CT<T> v[4];
-    ld4<T>(basis + 4 * i, v);
-    st4<T>(sb + 4 * i, v);
+  // 16 groups of 4 limbs per thread, loads issued in batches of 8
+#pragma unroll
+  for (int u0 = 0; u0 < 32 * E / 4 / GB; u0 += 8) {
+    CT<T> v[8][4];
+#pragma unroll
+    for (int u = 0; u < 8; u++) ld4<T>(basis + 4 * (threadIdx.x + (u0 + u) * GB), v[u]);
+#pragma unroll
+    for (int u = 0; u < 8; u++) st4<T>(sb + 4 * (threadIdx.x + (u0 + u) * GB), v[u]);
   }
   uint64_t total = n * (uint64_t)nrot;
   for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < total; base += (uint64_t)gridDim.x * GPB) {
@@ -1016,10 +1024,12 @@ __global__ void __launch_bounds__(GB) k_psi(const LT<T>* x, const LT<T>* pw, LT<
   __shared__ __attribute__((aligned(16))) LT<T> sp[16 * 32 * 4];
   __shared__ __attribute__((aligned(16))) LT<T> sx[GPB][E];
   int g = threadIdx.x >> 5, t = threadIdx.x & 31;
-  for (int i = threadIdx.x; i < 16 * 32; i += GB) {
-    CT<T> v[4];
-    ld4<T>(pw + 4 * i, v);
-    st4<T>(sp + 4 * i, v);
+  {
+    CT<T> v[16 * 32 / GB][4];  // all loads in flight before the first store
+#pragma unroll
+    for (int u = 0; u < 16 * 32 / GB; u++) ld4<T>(pw + 4 * (threadIdx.x + u * GB), v[u]);
+#pragma unroll
+    for (int u = 0; u < 16 * 32 / GB; u++) st4<T>(sp + 4 * (threadIdx.x + u * GB), v[u]);
   }
   for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < n; base += (uint64_t)gridDim.x * GPB) {
     uint64_t e = base + g;
