@@ -1509,6 +1509,33 @@ __global__ void k_merkle_paths(const uint32_t* nodes, uint64_t n, const uint32_t
   for (int k = 0; k < 8; k++) dst[k] = nodes[node * 8 + k];
 }
 
+// Query indices (pcs.rs:1130-1156; r2_pcs.rs:220-246) when the codeword
+// length is a power of two (mask + 1): every 4-byte XOF word is kept, so
+// index q is word q % 16 of the root node's output block q / 16, masked.
+// The root node is the last parent (parent != 0: its 16 message words at
+// src) or the single chunk's final compression (src: cv, block, blen,
+// flags).  Written to idx (device) and to the proof buffer's index section.
+__global__ void k_queries(const uint32_t* src, int parent, uint64_t t, uint32_t mask, uint32_t* idx, uint32_t* pidx) {
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= t) return;
+  uint32_t cv[8], m[16], o[16], blen, flags;
+  if (parent) {
+    b3::iv(cv);
+    for (int i = 0; i < 16; i++) m[i] = src[i];
+    blen = 64;
+    flags = b3::PARENT;
+  } else {
+    for (int i = 0; i < 8; i++) cv[i] = src[i];
+    for (int i = 0; i < 16; i++) m[i] = src[8 + i];
+    blen = src[24];
+    flags = src[25];
+  }
+  b3::compress(cv, m, q / 16, blen, flags | b3::ROOT, o);
+  const uint32_t v = o[q % 16] & mask;
+  idx[q] = v;
+  pidx[q] = v;
+}
+
 // Proof-section download: `bytes` (a multiple of 4) from device memory to
 // the pinned proof buffer through its device address; 16-byte stores when
 // the destination allows them.  Runs on the download stream with a small

@@ -466,21 +466,51 @@ __global__ void __launch_bounds__(256) k_rows_gather(const uint64_t* ptr, const 
 // k_cmat forms C_p from the row CSR of the matrix; k_cgemm multiplies by
 // the fixed 128 x 1024 matrix of the b_s b_s' (fragments built on the host).
 // ---------------------------------------------------------------------------
+// A 32-lane group per block row p: the entries of its 32 rows are contiguous
+// in the CSR, so the group reads them 32 at a time (lane i: entry k0 + i,
+// its column, value and row s found among the 33 row pointers), then every
+// lane s' adds each entry's term into C_p[s][s'] (LDS, column s' owned by
+// lane s'); the w reads of a chunk are independent, so they overlap.
 template <class T>
 __global__ void __launch_bounds__(256) k_cmat(const uint32_t* ptr, const uint32_t* col, const LT<T>* val,
                                               const LT<T>* w, LT<T>* cmat, uint64_t nb) {
   using C = CT<T>;
-  const uint32_t lane = threadIdx.x & 31;
-  for (uint64_t p = (uint64_t)blockIdx.x * 8 + (threadIdx.x >> 5); p < nb; p += (uint64_t)gridDim.x * 8) {
-    for (uint32_t s = 0; s < 32; s++) {
-      const uint64_t r = 32 * p + s;
-      C acc = 0;
-      for (uint32_t k = ptr[r]; k < ptr[r + 1]; k++) {
-        const uint32_t c = col[k];
-        acc += (C)val[k] * (C)w[(c & ~31u) + ((lane + c - s) & 31u)];
+  __shared__ C cm[8][32][32];
+  const uint32_t lane = threadIdx.x & 31, g = threadIdx.x >> 5;
+  for (uint64_t pb = (uint64_t)blockIdx.x * 8; pb < nb; pb += (uint64_t)gridDim.x * 8) {
+    const uint64_t p = pb + g;
+    const bool ok = p < nb;
+    // row pointers 32p .. 32p+32 (lane i holds ptr[32p + i]; lane 0 also the end)
+    const uint32_t rp = ok ? ptr[32 * p + lane] : 0, rend = ok ? ptr[32 * p + 32] : 0;
+    const uint32_t k0 = __shfl(rp, 0, 32);
+#pragma unroll
+    for (int s = 0; s < 32; s++) cm[g][s][lane] = 0;
+    for (uint32_t kb = k0; kb < rend; kb += 32) {
+      const uint32_t k = kb + lane;
+      const bool ek = k < rend;
+      const uint32_t c = ek ? col[k] : 0;
+      const C v = ek ? (C)val[k] : (C)0;
+      // row of entry k: the number of row starts rp[1..31] at or below k
+      uint32_t s = 0;
+#pragma unroll
+      for (int i = 1; i < 32; i++) s += (__shfl(rp, i, 32) <= k) ? 1u : 0u;
+      const uint32_t ne = rend - kb < 32 ? rend - kb : 32;
+      C term[32];
+#pragma unroll
+      for (int j = 0; j < 32; j++) {
+        const uint32_t cj = __shfl(c, j, 32), sj = __shfl(s, j, 32);
+        term[j] = (uint32_t)j < ne ? (C)w[(cj & ~31u) + ((lane + cj - sj) & 31u)] : (C)0;
       }
-      cmat[p * 1024 + 32 * s + lane] = (LT<T>)acc;
+#pragma unroll
+      for (int j = 0; j < 32; j++) {
+        const C vj = __shfl(v, j, 32);
+        const uint32_t sj = __shfl(s, j, 32);
+        if ((uint32_t)j < ne) cm[g][sj][lane] += vj * term[j];
+      }
     }
+    if (ok)
+#pragma unroll
+      for (int s = 0; s < 32; s++) cmat[p * 1024 + 32 * s + lane] = (LT<T>)cm[g][s][lane];
   }
 }
 
@@ -526,22 +556,31 @@ __global__ void __launch_bounds__(256) k_cgemm(const uint32_t* frag, const LT<T>
     v16i acc[ND];
 #pragma unroll
     for (int d = 0; d < ND; d++) acc[d] = (v16i)0;
-    for (int kb = 0; kb < 32; kb++) {
-      uint32_t v[16];
-      if (ok) {
+    // k-block kb+1's operands are loaded during kb's matrix work
+    uint32_t v[16];
+    v4i a[ND];
+    auto load_kb = [&](int kb) {
+      if (ok && kb < 32) {
         ld16k<T>(cp, kb, h, v);
       } else {
 #pragma unroll
         for (int b = 0; b < 16; b++) v[b] = 0;
       }
-      v4i B[ND];
+#pragma unroll
+      for (int pd = 0; pd < ND; pd++) a[pd] = A[((pd * 32 + (kb & 31)) * 4 + rb) * 64 + l];
+    };
+    load_kb(0);
+    for (int kb = 0; kb < 32; kb++) {
+      v4i B[ND], ac[ND];
       planes16<ND>(v, B);
 #pragma unroll
-      for (int pd = 0; pd < ND; pd++) {
-        const v4i a = A[((pd * 32 + kb) * 4 + rb) * 64 + l];
+      for (int pd = 0; pd < ND; pd++) ac[pd] = a[pd];
+      load_kb(kb + 1);
 #pragma unroll
-        for (int q = 0; pd + q < ND; q++) acc[pd + q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, B[q], acc[pd + q], 0, 0, 0);
-      }
+      for (int pd = 0; pd < ND; pd++)
+#pragma unroll
+        for (int q = 0; pd + q < ND; q++)
+          acc[pd + q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ac[pd], B[q], acc[pd + q], 0, 0, 0);
     }
     if (ok) {
 #pragma unroll

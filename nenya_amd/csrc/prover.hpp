@@ -769,7 +769,18 @@ template <class T> struct Engine {
 
   // BLAKE3 root node of prefix || device bytes, hashed on the device
   // (chunk chaining values in parallel, then the left-complete merge levels).
-  void hash_root_node(const uint8_t* prefix, size_t plen, const void* dbytes, size_t dlen, uint32_t node[26]) {
+  // With didx (device, t entries) and pidx (host address of the proof's
+  // index section, t words) given and a power-of-two codeword length n, the
+  // indices are expanded on the device (k_queries) and node is not filled:
+  // no host round trip.
+  void hash_root_node(const uint8_t* prefix, size_t plen, const void* dbytes, size_t dlen, uint32_t node[26],
+                      uint32_t* didx = nullptr, uint8_t* pidx = nullptr, uint64_t t = 0, uint64_t n = 0) {
+    auto expand = [&](const uint32_t* src, int parent) {
+      uint32_t* dp = nullptr;
+      HIP_CHECK(hipHostGetDevicePointer((void**)&dp, pidx, 0));
+      launch("k_queries", 0, 0, k_queries, dim3((unsigned)((t + 255) / 256)), dim3(256), src, parent, t,
+             (uint32_t)(n - 1), didx, dp);
+    };
     size_t len = plen + dlen;
     DBuf<uint8_t> msg(pool, len + 64);
     h2d(msg.p, prefix, plen);
@@ -779,7 +790,8 @@ template <class T> struct Engine {
     launch("k_b3_chunks", (double)len, 0, k_b3_chunks, dim3(grid_for(nch, 64, 1u << 20)), dim3(64), (const uint8_t*)msg.p,
            (uint64_t)len, nch, cv.p, rn.p);
     if (nch == 1) {
-      d2h(node, rn.p, 26 * 4);
+      if (didx) expand(rn.p, 0);
+      else d2h(node, rn.p, 26 * 4);
       return;
     }
     uint32_t* a = cv.p;
@@ -794,6 +806,10 @@ template <class T> struct Engine {
     if (cnt > 2) {
       launch("k_b3_merge_top", cnt * 32.0, 0, k_b3_merge_top, dim3(1), dim3(1024), (const uint32_t*)a, cnt, b);
       std::swap(a, b);
+    }
+    if (didx) {
+      expand(a, 1);
+      return;
     }
     uint32_t kids[16];
     d2h(kids, a, 64);
@@ -812,14 +828,21 @@ template <class T> struct Engine {
     const char* dom = P.packed ? "pcs-ring/fs/queries" : "snark-ring/r2-pcs/fs/queries";
     std::vector<uint8_t> prefix(dom, dom + std::strlen(dom));
     prefix.insert(prefix.end(), c.root, c.root + 32);
-    uint32_t node[26];
-    hash_root_node(prefix.data(), prefix.size(), d_mbar, nmbar * E * sizeof(L), node);
-    std::vector<uint32_t> idx = queries_from_root(node, P.n, P.ell);
-    out.u64(idx.size());
-    for (uint32_t j : idx) out.u32(j);
-    DBuf<uint32_t> didx(pool, idx.size());
-    h2d(didx.p, idx.data(), idx.size() * 4);
-    size_t per = c.rows * W, nq = idx.size();
+    size_t nq = P.ell;
+    DBuf<uint32_t> didx(pool, nq);
+    out.u64(nq);
+    if ((P.n & (P.n - 1)) == 0) {
+      // indices expanded on the device, straight into the proof
+      hash_root_node(prefix.data(), prefix.size(), d_mbar, nmbar * E * sizeof(L), nullptr, didx.p, out.take(nq * 4),
+                     nq, P.n);
+    } else {
+      uint32_t node[26];
+      hash_root_node(prefix.data(), prefix.size(), d_mbar, nmbar * E * sizeof(L), node);
+      std::vector<uint32_t> idx = queries_from_root(node, P.n, P.ell);
+      for (uint32_t j : idx) out.u32(j);
+      h2d(didx.p, idx.data(), idx.size() * 4);
+    }
+    size_t per = c.rows * W;
     size_t colb = nq * (8 + per * sizeof(L));
     DBuf<uint8_t> cols(pool, colb);
     launch("k_gather_cols", nq * per * 2.0 * sizeof(L), 0, (k_gather_cols<T>), dim3(grid_for(nq * per, 256, 1u << 20)),
