@@ -1538,8 +1538,8 @@ __global__ void k_queries(const uint32_t* src, int parent, uint64_t t, uint32_t 
 
 // Proof-section download: `bytes` (a multiple of 4) from device memory to
 // the pinned proof buffer through its device address; 16-byte stores when
-// the destination allows them.  Runs on the download stream with a small
-// grid beside the prover's kernels.
+// the destination allows them.  Runs on the download stream (at most 256
+// blocks) beside the prover's kernels.
 __global__ void __launch_bounds__(256) k_copy_out(const uint8_t* src, uint8_t* dst, uint64_t bytes) {
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nth = (uint64_t)gridDim.x * blockDim.x;
   uint64_t done = 0;

@@ -544,7 +544,7 @@ template <class T> struct Engine {
     HIP_CHECK(hipHostGetDevicePointer((void**)&ddst, dst, 0));
     HIP_CHECK(hipEventRecord(cev, st));
     HIP_CHECK(hipStreamWaitEvent(cst, cev, 0));
-    hipLaunchKernelGGL(k_copy_out, dim3((unsigned)std::min<uint64_t>((bytes + 4095) / 4096, 64)), dim3(256), 0, cst,
+    hipLaunchKernelGGL(k_copy_out, dim3((unsigned)std::min<uint64_t>((bytes + 4095) / 4096, 256)), dim3(256), 0, cst,
                        (const uint8_t*)src.p, ddst, (uint64_t)bytes);
     HIP_CHECK(hipGetLastError());
     held.push_back({src.p, src.n * sizeof(X)});
