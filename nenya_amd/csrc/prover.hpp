@@ -8,6 +8,7 @@
 #pragma once
 #include <chrono>
 #include <exception>
+#include <initializer_list>
 #include <map>
 #include <thread>
 
@@ -159,6 +160,7 @@ template <class T> struct PcsDev {
 template <class T> struct CommitDev {
   using L = typename T::L;
   uint8_t root[32];
+  const uint8_t* root_stage = nullptr;  // pinned copy in flight (Engine::resolve_roots fills root)
   size_t rows = 0;
   DBuf<L> enc;          // rows x n x W
   DBuf<uint32_t> nodes;  // (2n-1) x 8 words, level order
@@ -315,6 +317,10 @@ template <class T> struct Engine {
     HIP_CHECK(hipHostGetDevicePointer(&dptr, slot, 0));
     return dptr;
   }
+  // Commitment roots in flight to the host (see commit / resolve_roots).
+  static constexpr int ROOT_SLOTS = 16;
+  Pinned rootbuf;
+  int root_used = 0;
   // Pinned area (up to 64 elements) that small result kernels write directly;
   // the host reads it after sync().
   Pinned resbuf;
@@ -552,6 +558,10 @@ template <class T> struct Engine {
     src.n = 0;
   }
   void finish_downloads() {
+    if (root_used) {  // a failed proof left commitment roots unresolved
+      sync();
+      root_used = 0;
+    }
     if (!cst) return;
     HIP_CHECK(hipStreamSynchronize(cst));
     for (auto& h : held) pool->put(h.first, h.second);
@@ -743,8 +753,23 @@ template <class T> struct Engine {
     }
     if (len > 1)
       launch("k_merkle_top", len * 64.0, 0, k_merkle_top, dim3(1), dim3(1024), c.nodes.p, (uint64_t)start, (uint64_t)len);
-    d2h(c.root, c.nodes.p + (2 * n - 2) * 8, 32);
+    // the root lands in a pinned slot without a synchronization; the
+    // caller's resolve_roots() waits once for a group of commitments
+    if (root_used >= ROOT_SLOTS) throw Error(NENYA_E_DEVICE, "too many unresolved commitment roots");
+    uint8_t* slot = rootbuf.get(ROOT_SLOTS * 32) + 32 * root_used++;
+    HIP_CHECK(hipMemcpyAsync(slot, c.nodes.p + (2 * n - 2) * 8, 32, hipMemcpyDeviceToHost, st));
+    c.root_stage = slot;
     return c;
+  }
+  // One synchronization for the roots of the given commitments.
+  void resolve_roots(std::initializer_list<CommitDev<T>*> cs) {
+    sync();
+    for (CommitDev<T>* c : cs) {
+      if (!c->root_stage) continue;
+      std::memcpy(c->root, c->root_stage, 32);
+      c->root_stage = nullptr;
+    }
+    root_used = 0;
   }
 
   // RAA encode of `rows` rows (raa.rs:106-153 / 67-98), W limbs per entry.

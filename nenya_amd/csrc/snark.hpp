@@ -215,6 +215,7 @@ template <class T> struct SimdParams : ParamsBase {
     // 2. commits
     CommitDev<T> cw = e.commit(pw, slots.p, pw.P.k1, wphi.p);
     CommitDev<T> ce = e.commit(pq, ev.p, pq.P.k1, ev.p);
+    e.resolve_roots({&cw, &ce});
     stg.mark("commit");
     // 3. transcript
     Transcript tr("simd");
@@ -563,6 +564,7 @@ template <class T> struct MimdParams : ParamsBase {
     CommitDev<T> cabc = e.commit(pabc, abc_slots.p, 3 * pabc.P.k1, hats.p);
     CommitDev<T> cem = e.commit(pe, emult, pe.P.k1, emult);
     CommitDev<T> ces = e.commit(pe, estack.p, 3 * pe.P.k1, estack.p);
+    e.resolve_roots({&cw, &cabc, &cem, &ces});
     stg.mark("commit");
     Transcript tr("mimd");
     tr.root(cw.root);
@@ -889,6 +891,7 @@ template <class T> struct GeneralParams : ParamsBase {
     CommitDev<T> cabc = e.commit(pabc, abc_slots.p, 3 * pabc.P.k1, hats.p);
     CommitDev<T> cem = e.commit(pe, emult, pe.P.k1, emult);
     CommitDev<T> ces = e.commit(pe, estack.p, 3 * pe.P.k1, estack.p);
+    e.resolve_roots({&cw, &cabc, &cem, &ces});
     stg.mark("commit");
     Transcript tr("general");
     tr.root(cw.root);
