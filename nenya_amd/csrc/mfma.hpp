@@ -463,8 +463,9 @@ __global__ void __launch_bounds__(256) k_rows_gather(const uint64_t* ptr, const 
 // entry, D[s] = A[32p+s][c]):
 //   sum_cells D_hat rho = sum_{s,s'} C_p[s][s'] (b_s b_s'),
 //   C_p[s][s'] = sum_{entries (32p+s, c, v)} v * w[32(c/32) + (s'+c-s) mod 32].
-// k_cmat forms C_p from the row CSR of the matrix; k_cgemm multiplies by
-// the fixed 128 x 1024 matrix of the b_s b_s' (fragments built on the host).
+// k_cmat forms C_p from the row CSR of the matrix (stored k-block-major:
+// row s of C_p at cmat + (s * nb + p) * 32); k_cgemm multiplies by the fixed
+// 128 x 1024 matrix of the b_s b_s' (fragments built on the host).
 // ---------------------------------------------------------------------------
 // A 32-lane group per block row p: the entries of its 32 rows are contiguous
 // in the CSR, so the group reads them 32 at a time (lane i: entry k0 + i,
@@ -510,7 +511,7 @@ __global__ void __launch_bounds__(256) k_cmat(const uint32_t* ptr, const uint32_
     }
     if (ok)
 #pragma unroll
-      for (int s = 0; s < 32; s++) cmat[p * 1024 + 32 * s + lane] = (LT<T>)cm[g][s][lane];
+      for (int s = 0; s < 32; s++) cmat[((uint64_t)s * nb + p) * 32 + lane] = (LT<T>)cm[g][s][lane];
   }
 }
 
@@ -518,7 +519,8 @@ __global__ void __launch_bounds__(256) k_cmat(const uint32_t* ptr, const uint32_
 // (cells of the union support with their 32 instance values, w the witness
 // slots [m][32]); one 32-lane group per row, lane s'.  With
 // Ahat[r,c] = sum_s a_s b_s and w_phi[c] = sum_s' w_s' b_s', the row's
-// sum_c Ahat[r,c] w_phi[c] is sum_{s,s'} C_r[s][s'] b_s b_s' (k_cgemm).
+// sum_c Ahat[r,c] w_phi[c] is sum_{s,s'} C_r[s][s'] b_s b_s' (k_cgemm; C_r
+// stored k-block-major as in k_cmat).
 template <class T>
 __global__ void __launch_bounds__(256) k_cmat_mimd(const uint64_t* ptr, const uint32_t* col, const LT<T>* val,
                                                    const LT<T>* w, LT<T>* cmat, uint64_t n) {
@@ -535,7 +537,7 @@ __global__ void __launch_bounds__(256) k_cmat_mimd(const uint64_t* ptr, const ui
       for (int s = 0; s < 32; s++) acc[s] += __shfl(vl, s, 32) * wv;
     }
 #pragma unroll
-    for (int s = 0; s < 32; s++) cmat[r * 1024 + 32 * s + lane] = (LT<T>)acc[s];
+    for (int s = 0; s < 32; s++) cmat[((uint64_t)s * n + r) * 32 + lane] = (LT<T>)acc[s];
   }
 }
 
@@ -552,7 +554,8 @@ __global__ void __launch_bounds__(256) k_cgemm(const uint32_t* frag, const LT<T>
   for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const uint64_t p = tile * 32 + col;
     const bool ok = p < nb;
-    const LT<T>* cp = cmat + (ok ? p : 0) * 1024;
+    // k-block kb of C_p at cmat + (kb * nb + p) * 32: a wave's loads are contiguous
+    const LT<T>* cp = cmat + (ok ? p : 0) * 32;
     v16i acc[ND];
 #pragma unroll
     for (int d = 0; d < ND; d++) acc[d] = (v16i)0;
@@ -561,7 +564,7 @@ __global__ void __launch_bounds__(256) k_cgemm(const uint32_t* frag, const LT<T>
     v4i a[ND];
     auto load_kb = [&](int kb) {
       if (ok && kb < 32) {
-        ld16k<T>(cp, kb, h, v);
+        ld16k<T>(cp + (uint64_t)kb * nb * 32, 0, h, v);
       } else {
 #pragma unroll
         for (int b = 0; b < 16; b++) v[b] = 0;
