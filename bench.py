@@ -231,8 +231,8 @@ def main():
                     help="also run a 256 MiB device copy (PMC calibration of FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "independent proofs shard trivially across GPUs, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); independent proofs shard "
+              "trivially across GPUs (one process per GPU, jobs placed by device footprint)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
