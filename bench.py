@@ -96,14 +96,21 @@ def mimd_serving_jobs(na, njobs, seed):
 
 
 def time_proofs(fn, reps):
+    """Median ms per proof over `reps` proofs after one warm-up (garbage of
+    the previous section collected first, so a host pause cannot skew it)."""
+    import gc
+    gc.collect()
     fn()
-    t = time.perf_counter()
+    ts = []
     for _ in range(reps):
+        t = time.perf_counter()
         fn()
-    return (time.perf_counter() - t) / reps * 1e3
+        ts.append(time.perf_counter() - t)
+    ts.sort()
+    return ts[len(ts) // 2] * 1e3
 
 
-def width_sweep(na, seed, reps=3):
+def width_sweep(na, seed, reps=5):
     """SIMD at n = m = 2^17 per instance for k = 8, 16, 32, 64, and the PNG
     filter batch (u8, 32 images x 2^15 bytes) -- ms per proof, host inputs."""
     from nenya_amd.instances import png_instance, simd_instance

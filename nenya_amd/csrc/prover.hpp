@@ -222,7 +222,9 @@ template <class T> struct Engine {
   // Proof sections leave the device on a second stream (default priority,
   // created on first use) while the prover's later stages run; their device
   // buffers are held until finish_downloads(), which every proof calls
-  // before it returns and again before it starts (after a failed proof).
+  // before it returns (it also waits for the prover stream, whose kernels and
+  // copies write parts of the proof too) and again before it starts (after a
+  // failed proof).
   hipStream_t cst = nullptr;
   hipEvent_t cev = nullptr;
   std::vector<std::pair<void*, size_t>> held;
@@ -558,10 +560,11 @@ template <class T> struct Engine {
     src.n = 0;
   }
   void finish_downloads() {
-    if (root_used) {  // a failed proof left commitment roots unresolved
-      sync();
-      root_used = 0;
-    }
+    // the prover stream's own writes into the proof buffer (server contexts
+    // copy their sections on it; k_queries writes indices directly) and any
+    // commitment roots a failed proof left unresolved
+    sync();
+    root_used = 0;
     if (!cst) return;
     HIP_CHECK(hipStreamSynchronize(cst));
     for (auto& h : held) pool->put(h.first, h.second);

@@ -73,3 +73,23 @@ def test_concurrent_small_folds_match_direct_path():
         got, _, _ = srv.wait(jid)
         assert got == direct, f"general k={k} log={log}"
     srv.close()
+
+
+def test_single_proof_beside_a_live_server(orc):
+    """While a server exists, single-proof calls post each sumcheck challenge
+    before enqueueing the lift that reads it (no kernel waits on a host
+    thread); the bytes are the oracle's either way, and a proof made before
+    the server and one made while it serves other jobs are identical."""
+    print("single proof beside a live server", flush=True)
+    A, B, Cm, w = general_instance(10, 32, 81)
+    p = na.GeneralSnarkParams.secure(5, (A, B, Cm), 0xDEAD, 0xCAFE, k=32)
+    before = na.prove_general(w, p)
+    srv = na.Server(workers=2)
+    ids = [srv.submit_general(p, w) for _ in range(4)]
+    during = na.prove_general(w, p)
+    for jid in ids:
+        got, _, _ = srv.wait(jid)
+        assert got == before
+    srv.close()
+    assert during == before
+    assert before == orc.general_prove(32, 5, A, B, Cm, w)
