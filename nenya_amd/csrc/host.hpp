@@ -16,8 +16,8 @@
 namespace nenya {
 
 // ---------------------------------------------------------------------------
-// Caching device allocator: blocks are returned to per-size free lists and
-// reused by later proofs with the same parameters (no hipMalloc in steady state).
+// Caching device allocator: blocks are returned to per-size-class free lists
+// and reused by later proofs (no hipMalloc in steady state).
 // ---------------------------------------------------------------------------
 // Debug switch: NENYA_POISON=1 fills every block the pool hands out with
 // 0xA5 bytes, so a kernel that reads memory nobody wrote shows up as a
@@ -49,14 +49,28 @@ inline std::atomic<int>& live_servers() {
   return n;
 }
 
+// Allocation size classes: powers of two up to 64 KiB, then four classes per
+// octave (a block is at most 25 % larger than the request), so blocks freed
+// by one proof serve later proofs of other shapes and widths.
+inline size_t size_class(size_t bytes) {
+  if (bytes <= 256) return 256;
+  if (bytes <= (size_t(1) << 16)) {
+    size_t c = 512;
+    while (c < bytes) c <<= 1;
+    return c;
+  }
+  const int e = 63 - __builtin_clzll(bytes);  // 2^e <= bytes < 2^(e+1)
+  const size_t q = size_t(1) << (e - 2);
+  return (bytes + q - 1) / q * q;
+}
+
 struct Pool {
   std::multimap<size_t, void*> free_;
   std::vector<std::pair<void*, size_t>> all_;
   size_t live = 0, peak = 0;
   std::mutex mu;
   void* get(size_t bytes) {
-    bytes = (bytes + 255) & ~size_t(255);
-    if (bytes == 0) bytes = 256;
+    bytes = size_class(bytes);
     std::lock_guard<std::mutex> g(mu);
     auto it = free_.find(bytes);
     if (it != free_.end()) {
@@ -133,8 +147,7 @@ struct Pool {
   }
   void put(void* p, size_t bytes) {
     if (!p) return;
-    bytes = (bytes + 255) & ~size_t(255);
-    if (bytes == 0) bytes = 256;
+    bytes = size_class(bytes);
     if (pool_guard()) check_guards(p, bytes);
     std::lock_guard<std::mutex> g(mu);
     free_.insert({bytes, p});

@@ -95,8 +95,9 @@ struct Pinned {
       if (p) (void)hipHostFree(p);
       p = nullptr;
       cap = 0;
-      HIP_CHECK(hipHostMalloc((void**)&p, n ? n : 1, flags));
-      cap = n;
+      const size_t c = size_class(n);  // headroom for a slightly larger next proof
+      HIP_CHECK(hipHostMalloc((void**)&p, c, flags));
+      cap = c;
     }
     return p;
   }

@@ -39,12 +39,26 @@ struct Job {
   bool done = false;
 };
 
+// One stream and one device pool per worker, shared by its per-width
+// contexts: the runtime maps streams onto a few hardware queues
+// (GPU_MAX_HW_QUEUES, 4 by default), and two streams on one queue run in
+// order, so a stream per width would let two workers' proofs serialize on a
+// shared queue; one pool lets blocks freed by a proof of one width serve the
+// next proof of another.  A worker runs one proof at a time, so its contexts
+// never overlap each other.
 struct Worker {
+  Pool pool;  // destroyed after the contexts
+  hipStream_t st = nullptr;
   std::unique_ptr<CtxBase> ctx[4];  // per width: 8, 16, 32, 64
+  Worker() { HIP_CHECK(hipStreamCreate(&st)); }
+  ~Worker() {
+    for (auto& c : ctx) c.reset();
+    if (st) (void)hipStreamDestroy(st);
+  }
   template <class T> Ctx<T>& get() {
     const int i = T::K == 8 ? 0 : T::K == 16 ? 1 : T::K == 32 ? 2 : 3;
     if (!ctx[i]) {
-      ctx[i].reset(new Ctx<T>());
+      ctx[i].reset(new Ctx<T>(st, pool));
       // several workers drive the device: no kernel waits on a host thread
       static_cast<Ctx<T>&>(*ctx[i]).eng->pipelined = false;
     }
@@ -119,7 +133,16 @@ struct Server {
     std::lock_guard<std::mutex> g(mu);
     Job* j = find(id);
     if (!j->done) throw Error(NENYA_E_ARG, "job still running");
-    if (j->proof.p) spare.push_back(std::move(j->proof));
+    if (j->proof.p) {
+      spare.push_back(std::move(j->proof));
+      // keep a few more buffers than there are workers: drop the smallest
+      if (spare.size() > 2 * workers.size() + 2) {
+        size_t lo = 0;
+        for (size_t i = 1; i < spare.size(); i++)
+          if (spare[i].cap < spare[lo].cap) lo = i;
+        spare.erase(spare.begin() + lo);
+      }
+    }
     jobs.erase(id);
   }
 
@@ -137,6 +160,9 @@ struct Server {
       v = sp->prove(cx, j.m3[0], j.m3[1], j.m3[2], (const typename T::L*)j.w);
     }
     j.len = v.second;
+    size_t cur = j.params->proof_len.load();
+    while (v.second > cur && !j.params->proof_len.compare_exchange_weak(cur, v.second)) {
+    }
     // the proof was assembled in cx.proofbuf: hand that buffer to the job
     std::swap(j.proof, cx.proofbuf);
   }
@@ -157,10 +183,22 @@ struct Server {
         in_use += j->fp;
         running++;
         j->t_start = now_ms();
-        // a recycled pinned buffer becomes the context's proof buffer
+        // a recycled pinned buffer becomes the context's proof buffer: the
+        // smallest that holds the params' last proof, or (before their first
+        // proof) the largest; none if none is large enough, so a too-small
+        // buffer is not freed only to be allocated again
         if (!spare.empty()) {
-          j->proof = std::move(spare.back());
-          spare.pop_back();
+          const size_t want = j->params->proof_len.load();
+          size_t pick = spare.size();
+          for (size_t i = 0; i < spare.size(); i++) {
+            if (want ? spare[i].cap >= want && (pick == spare.size() || spare[i].cap < spare[pick].cap)
+                     : pick == spare.size() || spare[i].cap > spare[pick].cap)
+              pick = i;
+          }
+          if (pick < spare.size()) {
+            j->proof = std::move(spare[pick]);
+            spare.erase(spare.begin() + pick);
+          }
         }
       }
       int rc = NENYA_OK;

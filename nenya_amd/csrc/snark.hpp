@@ -14,25 +14,30 @@ namespace nenya {
 // switches.  Params own one (setup and direct calls); each server worker
 // owns its own, so proofs on different contexts run concurrently.
 struct CtxBase {
-  Pool pool;
+  Pool own_pool;
+  Pool& pool;  // own_pool, or a pool its owner shares between contexts
   hipStream_t st = nullptr;
   Pinned proofbuf;  // the last proof's bytes (valid until the next proof)
   KProf prof;
   bool timing = false;
   std::vector<std::pair<std::string, double>> last_timing;
-  CtxBase() { HIP_CHECK(hipStreamCreate(&st)); }
+  bool own_st = true;
+  CtxBase() : pool(own_pool) { HIP_CHECK(hipStreamCreate(&st)); }
+  // a context on a stream and a pool its owner keeps (and destroys after it)
+  CtxBase(hipStream_t shared, Pool& shared_pool) : pool(shared_pool), st(shared), own_st(false) {}
   CtxBase(const CtxBase&) = delete;
   CtxBase& operator=(const CtxBase&) = delete;
   virtual ~CtxBase() {
     if (st) {
       (void)hipStreamSynchronize(st);
-      (void)hipStreamDestroy(st);
+      if (own_st) (void)hipStreamDestroy(st);
     }
   }
 };
 template <class T> struct Ctx : CtxBase {
   std::unique_ptr<Engine<T>> eng;
   Ctx() { eng.reset(new Engine<T>(&pool, st)); }
+  Ctx(hipStream_t shared, Pool& shared_pool) : CtxBase(shared, shared_pool) { eng.reset(new Engine<T>(&pool, st)); }
   ~Ctx() override { eng.reset(); }
 };
 
@@ -57,6 +62,9 @@ struct ParamsBase {
     while (bytes > cur && !footprint.compare_exchange_weak(cur, bytes)) {
     }
   }
+  // bytes of the last proof (0 before the first): sizes the proof buffer a
+  // server hands to the next job with these params
+  std::atomic<size_t> proof_len{0};
   virtual CtxBase& ctx0() = 0;
   virtual ~ParamsBase() = default;
 };
