@@ -1542,18 +1542,23 @@ __global__ void k_queries(const uint32_t* src, int parent, uint64_t t, uint32_t 
 // blocks) beside the prover's kernels.
 __global__ void __launch_bounds__(256) k_copy_out(const uint8_t* src, uint8_t* dst, uint64_t bytes) {
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nth = (uint64_t)gridDim.x * blockDim.x;
+  const uintptr_t al = (uintptr_t)src | (uintptr_t)dst;
   uint64_t done = 0;
-  if (((uintptr_t)dst & 15) == 0) {
+  if ((al & 15) == 0) {
     const uint64_t n16 = bytes / 16;
     const uint4* s4 = reinterpret_cast<const uint4*>(src);
     uint4* d4 = reinterpret_cast<uint4*>(dst);
     for (uint64_t i = tid; i < n16; i += nth) d4[i] = s4[i];
     done = n16 * 16;
   }
-  const uint64_t nw = (bytes - done) / 4;
-  const uint32_t* s1 = reinterpret_cast<const uint32_t*>(src + done);
-  uint32_t* d1 = reinterpret_cast<uint32_t*>(dst + done);
-  for (uint64_t i = tid; i < nw; i += nth) d1[i] = s1[i];
+  if ((al & 3) == 0) {
+    const uint64_t nw = (bytes - done) / 4;
+    const uint32_t* s1 = reinterpret_cast<const uint32_t*>(src + done);
+    uint32_t* d1 = reinterpret_cast<uint32_t*>(dst + done);
+    for (uint64_t i = tid; i < nw; i += nth) d1[i] = s1[i];
+    done += nw * 4;
+  }
+  for (uint64_t i = done + tid; i < bytes; i += nth) dst[i] = src[i];
 }
 
 // Serialized rotated m-bars (pcs.rs:426-471): per rotation a u64 length

@@ -106,6 +106,16 @@ struct Pinned {
   }
 };
 
+// Workgroups of the copy kernel that moves proof sections to pinned host
+// memory (and bulk uploads from it); NENYA_DL_BLOCKS overrides the default.
+inline unsigned dl_blocks() {
+  static const unsigned v = [] {
+    const char* e = std::getenv("NENYA_DL_BLOCKS");
+    return e ? (unsigned)std::max(1, std::atoi(e)) : 16u;
+  }();
+  return v;
+}
+
 // Write cursor over a pre-sized section of the proof buffer.
 struct Sink {
   uint8_t* p;
@@ -344,13 +354,20 @@ template <class T> struct Engine {
       HIP_CHECK(hipMemcpyAsync(d, slot, bytes, hipMemcpyHostToDevice, st));
       return;
     }
-    const size_t CH = size_t(8) << 20;  // bounce in chunks, waiting before reuse
+    // bounce in chunks, waiting before reuse; the copy kernel reads the
+    // pinned bounce buffer with the download's few workgroups (other
+    // streams' reads of host memory stay responsive)
+    const size_t CH = size_t(8) << 20;
     uint8_t* b = xfer.get(std::min(bytes, CH));
+    uint8_t* bd = nullptr;
+    HIP_CHECK(hipHostGetDevicePointer((void**)&bd, b, 0));
     for (size_t off = 0; off < bytes; off += CH) {
       const size_t c = std::min(CH, bytes - off);
       if (off) sync();
       std::memcpy(b, (const uint8_t*)h + off, c);
-      HIP_CHECK(hipMemcpyAsync((uint8_t*)d + off, b, c, hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(k_copy_out, dim3((unsigned)std::min<uint64_t>((c + 4095) / 4096, dl_blocks())), dim3(256), 0, st,
+                         (const uint8_t*)bd, (uint8_t*)d + off, (uint64_t)c);
+      HIP_CHECK(hipGetLastError());
     }
     sync();
   }
@@ -537,24 +554,33 @@ template <class T> struct Engine {
   // Enqueue the copy of a finished device buffer into the pinned proof
   // buffer at host address `dst` (after the prover stream's work so far);
   // the buffer is held until finish_downloads().
+  //
+  // The copy kernel runs dl_blocks() workgroups: that already fills the host
+  // link, and a copy with a workgroup per CU queues so many posted writes
+  // that every other queue's packet and kernel-argument reads from host
+  // memory wait behind them (a 64 MiB copy with 256 workgroups held a
+  // one-block kernel on another stream for 1.1 ms; with 16, for 70 us, at
+  // the same copy time).
+
   template <class X> void download(uint8_t* dst, DBuf<X>& src, size_t bytes) {
     if (!bytes) return;
+    uint8_t* ddst = nullptr;
+    HIP_CHECK(hipHostGetDevicePointer((void**)&ddst, dst, 0));
+    const dim3 grid((unsigned)std::min<uint64_t>((bytes + 4095) / 4096, dl_blocks()));
     if (!pipelined) {
       // contexts of a multi-worker server: in order on the prover stream
-      // (their streams already share the hardware queues)
-      HIP_CHECK(hipMemcpyAsync(dst, src.p, bytes, hipMemcpyDeviceToHost, st));
+      // (the workers' streams already overlap each other)
+      hipLaunchKernelGGL(k_copy_out, grid, dim3(256), 0, st, (const uint8_t*)src.p, ddst, (uint64_t)bytes);
+      HIP_CHECK(hipGetLastError());
       return;
     }
     if (!cst) {
       HIP_CHECK(hipStreamCreate(&cst));
       HIP_CHECK(hipEventCreateWithFlags(&cev, hipEventDisableTiming));
     }
-    uint8_t* ddst = nullptr;
-    HIP_CHECK(hipHostGetDevicePointer((void**)&ddst, dst, 0));
     HIP_CHECK(hipEventRecord(cev, st));
     HIP_CHECK(hipStreamWaitEvent(cst, cev, 0));
-    hipLaunchKernelGGL(k_copy_out, dim3((unsigned)std::min<uint64_t>((bytes + 4095) / 4096, 256)), dim3(256), 0, cst,
-                       (const uint8_t*)src.p, ddst, (uint64_t)bytes);
+    hipLaunchKernelGGL(k_copy_out, grid, dim3(256), 0, cst, (const uint8_t*)src.p, ddst, (uint64_t)bytes);
     HIP_CHECK(hipGetLastError());
     held.push_back({src.p, src.n * sizeof(X)});
     src.p = nullptr;
