@@ -493,18 +493,50 @@ template <class T> struct Engine {
            (uint64_t)nblocks, ns, out.p);
     return out;
   }
-  // ntab inner products <A_y, B> (A_y = A + y*astride elements), one D2H.
-  template <int NA> std::vector<G> dots(const L* A, uint64_t astride, int ntab, const L* B, uint64_t n) {
+  // ntab inner products <A_y, B> (A_y = A + y*astride elements), left on
+  // the device (ntab contiguous elements).
+  template <int NA> DBuf<L> dots_dev(const L* A, uint64_t astride, int ntab, const L* B, uint64_t n) {
     unsigned nb = grid_for(n, GPB, ntab > 1 ? 256 : 2048);
     DBuf<L> part(pool, (size_t)nb * ntab * E);
     launch("k_dot", n * (ntab + 1.0) * EB, n * ntab * NA * PM, (k_dot<T, NA>), dim3(nb, ntab), dim3(GB), A, astride, B, n,
            part.p);
-    DBuf<L> r = reduce(part, nb, ntab);
+    return reduce(part, nb, ntab);
+  }
+  // the same, one D2H
+  template <int NA> std::vector<G> dots(const L* A, uint64_t astride, int ntab, const L* B, uint64_t n) {
+    DBuf<L> r = dots_dev<NA>(A, astride, ntab, B, n);
     std::vector<G> out(ntab);
     d2h(out.data(), r.p, ntab * sizeof(G));
     return out;
   }
   template <int NA> G dot(const L* A, const L* B, uint64_t n) { return dots<NA>(A, 0, 1, B, n)[0]; }
+  // Several picks (element 0 of cnt tables spaced `stride` elements apart;
+  // stride 1 reads cnt consecutive elements) with one synchronization: the
+  // pick kernels write straight into a pinned staging area.
+  struct PickSeg {
+    const L* src;
+    uint64_t stride, cnt;
+  };
+  Pinned pickbuf;
+  std::vector<G> pick_many(std::initializer_list<PickSeg> segs) {
+    uint64_t tot = 0;
+    for (const PickSeg& g : segs) tot += g.cnt;
+    std::vector<G> out(tot);
+    if (!tot) return out;
+    L* h = (L*)pickbuf.get(tot * E * sizeof(L));
+    L* d = nullptr;
+    HIP_CHECK(hipHostGetDevicePointer((void**)&d, h, 0));
+    uint64_t off = 0;
+    for (const PickSeg& g : segs) {
+      if (g.cnt)
+        launch("k_pick", 0, 0, (k_pick<T>), dim3(grid_for(g.cnt * E, 256)), dim3(256), g.src, g.stride, g.cnt,
+               d + off * E);
+      off += g.cnt;
+    }
+    sync();
+    std::memcpy(out.data(), h, tot * sizeof(G));
+    return out;
+  }
   // element 0 of ntab tables spaced `stride` elements apart, one D2H
   std::vector<G> pick(const L* src, uint64_t stride, int ntab) {
     DBuf<L> o(pool, (size_t)ntab * E);
@@ -943,6 +975,14 @@ template <class T> struct Engine {
   }
 
   G open(const PcsDev<T>& pc, const CommitDev<T>& c, const G* z, const G* weights, size_t s, Sink& out) {
+    DBuf<L> y = open_dev(pc, c, z, weights, s, out);
+    G v;
+    d2h(&v, y.p, sizeof(G));
+    return v;
+  }
+  // The opening with its value y = <m-bar, e1> left on the device (no
+  // synchronization; callers that need y only later fetch it then).
+  DBuf<L> open_dev(const PcsDev<T>& pc, const CommitDev<T>& c, const G* z, const G* weights, size_t s, Sink& out) {
     const PcsParams& P = pc.P;
     size_t lk1 = (size_t)__builtin_ctzll(P.k1), lk2 = (size_t)__builtin_ctzll(P.k2);
     DBuf<L> e0 = eq_table(z, lk1), e1 = eq_table(z + lk1, lk2);
@@ -965,7 +1005,7 @@ template <class T> struct Engine {
     else
       launch("k_mbar", mbb, rows * P.k2 * 32 * PM, (k_mbar<T, 32>), dim3(grid_for(P.k2, GPB)), dim3(GB), (const L*)e0s.p,
              c.msg, rows, (uint64_t)P.k2, mbar.p, (uint64_t)0);
-    G y = dot<32>(mbar.p, e1.p, P.k2);
+    DBuf<L> y = dots_dev<32>(mbar.p, 0, 1, e1.p, P.k2);
     out.u64(P.k2);
     uint8_t* mdst = out.take(P.k2 * E * sizeof(L));
     open_tail(pc, c, mbar.p, P.k2, out);
@@ -977,6 +1017,14 @@ template <class T> struct Engine {
   // (rho_l[q] = pack_r2(rot_l(slots_q)), the lifted messages of the rotations).
   std::vector<G> rotated_open(const PcsDev<T>& pc, const CommitDev<T>& c, const L* rho, uint64_t nb, const G* z, int nl,
                               Sink& out) {
+    DBuf<L> y = rotated_open_dev(pc, c, rho, nb, z, nl, out);
+    std::vector<G> ys(nl);
+    d2h(ys.data(), y.p, (size_t)nl * sizeof(G));
+    return ys;
+  }
+  // The rotated opening with its nl values left on the device.
+  DBuf<L> rotated_open_dev(const PcsDev<T>& pc, const CommitDev<T>& c, const L* rho, uint64_t nb, const G* z, int nl,
+                           Sink& out) {
     const PcsParams& P = pc.P;
     size_t lk1 = (size_t)__builtin_ctzll(P.k1), lk2 = (size_t)__builtin_ctzll(P.k2);
     DBuf<L> e0 = eq_table(z, lk1), e1 = eq_table(z + lk1, lk2);
@@ -987,7 +1035,7 @@ template <class T> struct Engine {
       launch("k_rot_mbar", (double)nl * P.k1 * P.k2 * EB, (double)nl * P.k1 * P.k2 * 16 * PM, (k_rot_mbar<T>),
              dim3(grid_for((uint64_t)nl * P.k2, GPB)), dim3(GB), (const L*)e0.p, rho, nb, (uint64_t)P.k1, (uint64_t)P.k2,
              nl, mbar.p);
-    std::vector<G> ys = dots<32>(mbar.p, P.k2, nl, e1.p, P.k2);
+    DBuf<L> ys = dots_dev<32>(mbar.p, P.k2, nl, e1.p, P.k2);
     out.u64(nl);
     // the nl length-prefixed m-bars, laid out on the device, one download
     const uint64_t per = P.k2 * E * sizeof(L) / 8, recb = (size_t)nl * (per + 1) * 8;
@@ -1093,10 +1141,11 @@ template <class T> struct Engine {
       if (pipe) host_round();
       vdown = vnext;
     }
-    o.chal.resize(nv);
-    d2h(o.chal.data(), dch.p, nv * sizeof(G));
+    // challenges and finals with one synchronization
+    std::vector<G> tail = pick_many({{dch.p, 1, nv}, {work.p, N / 2, (uint64_t)ntab}});
     check_waits();
-    o.finals = pick(work.p, N / 2, ntab);
+    o.chal.assign(tail.begin(), tail.begin() + nv);
+    o.finals.assign(tail.begin() + nv, tail.end());
     return o;
   }
 
@@ -1167,12 +1216,10 @@ template <class T> struct Engine {
       if (pipe) host_round();
       vdown = vnext;
     }
-    o.chal.resize(nv);
-    d2h(o.chal.data(), dch.p, nv * sizeof(G));
+    std::vector<G> tail = pick_many({{dch.p, 1, nv}, {f, fs, (uint64_t)ntab}, {g, gs, (uint64_t)ntab}});
     check_waits();
-    o.finals = pick(f, fs, ntab);
-    std::vector<G> gf = pick(g, gs, ntab);
-    o.finals.insert(o.finals.end(), gf.begin(), gf.end());
+    o.chal.assign(tail.begin(), tail.begin() + nv);
+    o.finals.assign(tail.begin() + nv, tail.end());
     return o;
   }
 };

@@ -604,9 +604,11 @@ template <class T> struct MimdParams : ParamsBase {
     // 6. openings at s1 (mimd.rs:340-352)
     G wts[3] = {G::one(), gam, gam2};
     Sink k_es(o_es, s_es), k_abc(o_abc, s_abc), k_em(o_em, s_em);
+    // estack's value enters the linear claim; the other two only the final
+    // self-check, so they stay on the device until then
     G estack_val = e.open(pe, ces, mp.chal.data(), wts, 3, k_es);
-    G abc_val = e.open(pabc, cabc, mp.chal.data(), wts, 3, k_abc);
-    G em_val = e.open(pe, cem, mp.chal.data(), nullptr, 1, k_em);
+    DBuf<L> abc_y = e.open_dev(pabc, cabc, mp.chal.data(), wts, 3, k_abc);
+    DBuf<L> em_y = e.open_dev(pe, cem, mp.chal.data(), nullptr, 1, k_em);
     stg.mark("open_s1");
     // 7. M_gamma row (mimd.rs:196-219, eq_s1 for all three matrices)
     DBuf<L> eqs(&pool, 3 * n * E), mrow(&pool, m * E);
@@ -628,8 +630,11 @@ template <class T> struct MimdParams : ParamsBase {
     stg.mark("linear_sumcheck");
     // 9. open w at s2
     Sink k_w(o_w, s_w);
-    G yw = e.open(pw, cw, lp.chal.data(), nullptr, 1, k_w);
+    DBuf<L> yw_d = e.open_dev(pw, cw, lp.chal.data(), nullptr, 1, k_w);
     stg.mark("open_w");
+    // the opening values of the self-check, one synchronization
+    std::vector<G> late = e.pick_many({{abc_y.p, 1, 1}, {em_y.p, 1, 1}, {yw_d.p, 1, 1}});
+    const G abc_val = late[0], em_val = late[1], yw = late[2];
     Sink hd(buf, 128 + s_mult + s_lin);
     for (auto* r : {cw.root, cabc.root, cem.root, ces.root}) hd.bytes(r, 32);
     hd.u64(mp.chal.size());
@@ -933,9 +938,11 @@ template <class T> struct GeneralParams : ParamsBase {
     // 8. openings at s1 (general.rs:475-478)
     G wts[3] = {G::one(), gam, gam2};
     Sink k_es(o_es, s_es), k_abc(o_abc, s_abc), k_em(o_em, s_em);
+    // estack's value enters the linear claim; the other two only the final
+    // self-check, so they stay on the device until then
     G estack_val = e.open(pe, ces, mp.chal.data(), wts, 3, k_es);
-    G abc_val = e.open(pabc, cabc, mp.chal.data(), wts, 3, k_abc);
-    G em_val = e.open(pe, cem, mp.chal.data(), nullptr, 1, k_em);
+    DBuf<L> abc_y = e.open_dev(pabc, cabc, mp.chal.data(), wts, 3, k_abc);
+    DBuf<L> em_y = e.open_dev(pe, cem, mp.chal.data(), nullptr, 1, k_em);
     stg.mark("open_s1");
     // 9. layer rows (general.rs:233-271) as sum over cells of
     //    {eq_s1, gamma eq_s1, gamma^2 eq_s1}[p] * D[cell]
@@ -959,8 +966,12 @@ template <class T> struct GeneralParams : ParamsBase {
     stg.mark("linear_sumcheck");
     // 11. rotated openings of w at s2 (pcs.rs:973-1040)
     Sink k_rho(o_rho, s_rho);
-    std::vector<G> ys = e.rotated_open(pw, cw, rho.p, nb, lp.chal.data(), 32, k_rho);
+    DBuf<L> ys_d = e.rotated_open_dev(pw, cw, rho.p, nb, lp.chal.data(), 32, k_rho);
     stg.mark("open_rho");
+    // the opening values of the self-check, one synchronization
+    std::vector<G> late = e.pick_many({{abc_y.p, 1, 1}, {em_y.p, 1, 1}, {ys_d.p, 1, 32}});
+    const G abc_val = late[0], em_val = late[1];
+    const G* ys = late.data() + 2;
     Sink hd(buf, 128 + s_mult + s_lin);
     for (auto* r : {cw.root, cabc.root, cem.root, ces.root}) hd.bytes(r, 32);
     hd.u64(mp.chal.size());
