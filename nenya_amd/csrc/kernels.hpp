@@ -1561,6 +1561,28 @@ __global__ void __launch_bounds__(256) k_copy_out(const uint8_t* src, uint8_t* d
   for (uint64_t i = done + tid; i < bytes; i += nth) dst[i] = src[i];
 }
 
+// slots[j][s] = w[s][j] for the 32 witnesses of m limbs each (the slot
+// layout of simd_to_packed, reduce.rs:33-60): a block moves 64 columns
+// through LDS so both the reads and the writes are contiguous.
+template <class T>
+__global__ void __launch_bounds__(256) k_slots(const LT<T>* w, uint64_t m, LT<T>* slots) {
+  __shared__ LT<T> t[32][65];
+  for (uint64_t j0 = (uint64_t)blockIdx.x * 64; j0 < m; j0 += (uint64_t)gridDim.x * 64) {
+    for (int x = threadIdx.x; x < 32 * 64; x += 256) {
+      const int s = x >> 6, jj = x & 63;
+      const uint64_t j = j0 + jj;
+      t[s][jj] = j < m ? w[s * m + j] : (LT<T>)0;
+    }
+    __syncthreads();
+    for (int x = threadIdx.x; x < 32 * 64; x += 256) {
+      const int jj = x >> 5, s = x & 31;
+      const uint64_t j = j0 + jj;
+      if (j < m) slots[j * 32 + s] = t[s][jj];
+    }
+    __syncthreads();
+  }
+}
+
 // Serialized rotated m-bars (pcs.rs:426-471): per rotation a u64 length
 // `cnt`, then the rotation's `per` 8-byte words of src.
 __global__ void k_len_prefixed(const uint64_t* src, uint64_t nrec, uint64_t per, uint64_t cnt, uint64_t* dst) {

@@ -107,7 +107,7 @@ struct Pinned {
 };
 
 // Workgroups of the copy kernel that moves proof sections to pinned host
-// memory (and bulk uploads from it); NENYA_DL_BLOCKS overrides the default.
+// memory; NENYA_DL_BLOCKS overrides the default.
 inline unsigned dl_blocks() {
   static const unsigned v = [] {
     const char* e = std::getenv("NENYA_DL_BLOCKS");
@@ -354,20 +354,13 @@ template <class T> struct Engine {
       HIP_CHECK(hipMemcpyAsync(d, slot, bytes, hipMemcpyHostToDevice, st));
       return;
     }
-    // bounce in chunks, waiting before reuse; the copy kernel reads the
-    // pinned bounce buffer with the download's few workgroups (other
-    // streams' reads of host memory stay responsive)
-    const size_t CH = size_t(8) << 20;
+    const size_t CH = size_t(8) << 20;  // bounce in chunks, waiting before reuse
     uint8_t* b = xfer.get(std::min(bytes, CH));
-    uint8_t* bd = nullptr;
-    HIP_CHECK(hipHostGetDevicePointer((void**)&bd, b, 0));
     for (size_t off = 0; off < bytes; off += CH) {
       const size_t c = std::min(CH, bytes - off);
       if (off) sync();
       std::memcpy(b, (const uint8_t*)h + off, c);
-      hipLaunchKernelGGL(k_copy_out, dim3((unsigned)std::min<uint64_t>((c + 4095) / 4096, dl_blocks())), dim3(256), 0, st,
-                         (const uint8_t*)bd, (uint8_t*)d + off, (uint64_t)c);
-      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipMemcpyAsync((uint8_t*)d + off, b, c, hipMemcpyHostToDevice, st));
     }
     sync();
   }

@@ -4,6 +4,8 @@
 #include <algorithm>
 #include <atomic>
 #include <numeric>
+#include <thread>
+#include <tuple>
 
 #include "prover.hpp"
 
@@ -78,38 +80,65 @@ struct HostCoo {
 };
 
 template <class T> struct Csr {
+  using L = typename T::L;
   DBuf<uint32_t> ptr, idx;
-  DBuf<typename T::L> val;
+  DBuf<L> val;
   size_t nnz = 0;
-  // group COO by `major` (rows if !by_col), stable
-  template <class Eng> void build(Eng& e, const HostCoo& m, bool by_col) {
-    Pool* pool = e.pool;
-    using L = typename T::L;
-    size_t nmaj = by_col ? m.cols : m.rows;
-    nnz = m.nnz;
-    std::vector<uint32_t> p(nmaj + 1, 0), ix(m.nnz);
-    std::vector<L> v(m.nnz);
+  // the grouped COO on the host
+  struct Host {
+    std::vector<uint32_t> p, ix;
+    std::vector<L> v;
+  };
+  // group COO by `major` (rows if !by_col), stable (a counting sort)
+  static Host group(const HostCoo& m, bool by_col) {
+    Host h;
+    const size_t nmaj = by_col ? m.cols : m.rows;
+    h.p.assign(nmaj + 1, 0);
+    h.ix.resize(m.nnz);
+    h.v.resize(m.nnz);
     const L* vals = (const L*)m.v;
-    for (size_t k = 0; k < m.nnz; k++) p[(by_col ? m.c[k] : m.r[k]) + 1]++;
-    for (size_t i = 0; i < nmaj; i++) p[i + 1] += p[i];
-    std::vector<uint32_t> pos(p.begin(), p.end() - 1);
+    for (size_t k = 0; k < m.nnz; k++) h.p[(by_col ? m.c[k] : m.r[k]) + 1]++;
+    for (size_t i = 0; i < nmaj; i++) h.p[i + 1] += h.p[i];
+    std::vector<uint32_t> pos(h.p.begin(), h.p.end() - 1);
     for (size_t k = 0; k < m.nnz; k++) {
       uint32_t maj = by_col ? m.c[k] : m.r[k];
       uint32_t o = pos[maj]++;
-      ix[o] = by_col ? m.r[k] : m.c[k];
-      v[o] = vals[k];
+      h.ix[o] = by_col ? m.r[k] : m.c[k];
+      h.v[o] = vals[k];
     }
-    ptr = DBuf<uint32_t>(pool, nmaj + 1);
-    idx = DBuf<uint32_t>(pool, std::max<size_t>(m.nnz, 1));
-    val = DBuf<L>(pool, std::max<size_t>(m.nnz, 1));
-    e.h2d(ptr.p, p.data(), p.size() * 4);
-    if (m.nnz) {
-      e.h2d(idx.p, ix.data(), ix.size() * 4);
-      e.h2d(val.p, v.data(), v.size() * sizeof(L));
-    }
-    e.sync();
+    return h;
   }
+  // h2d returns once the host vectors may be reused
+  template <class Eng> void upload(Eng& e, const Host& h) {
+    Pool* pool = e.pool;
+    nnz = h.ix.size();
+    ptr = DBuf<uint32_t>(pool, h.p.size());
+    idx = DBuf<uint32_t>(pool, std::max<size_t>(nnz, 1));
+    val = DBuf<L>(pool, std::max<size_t>(nnz, 1));
+    e.h2d(ptr.p, h.p.data(), h.p.size() * 4);
+    if (nnz) {
+      e.h2d(idx.p, h.ix.data(), h.ix.size() * 4);
+      e.h2d(val.p, h.v.data(), h.v.size() * sizeof(L));
+    }
+  }
+  template <class Eng> void build(Eng& e, const HostCoo& m, bool by_col) { upload(e, group(m, by_col)); }
 };
+
+// Several CSR builds: the host grouping of each runs on its own thread, the
+// uploads follow in order on the engine's stream.
+template <class T, class Eng>
+void build_csrs(Eng& e, std::initializer_list<std::tuple<Csr<T>*, const HostCoo*, bool>> jobs) {
+  std::vector<typename Csr<T>::Host> hs(jobs.size());
+  std::vector<std::thread> th;
+  size_t i = 0;
+  for (const auto& j : jobs) {
+    th.emplace_back([&hs, i, j] { hs[i] = Csr<T>::group(*std::get<1>(j), std::get<2>(j)); });
+    i++;
+  }
+  for (auto& t : th) t.join();
+  i = 0;
+  for (const auto& j : jobs) std::get<0>(j)->upload(e, hs[i++]);
+}
 
 // Uploads a CSR of (dk, sel, idx) entries (the k_spmv_gr layout).
 template <class T>
@@ -183,21 +212,13 @@ template <class T> struct SimdParams : ParamsBase {
     stg.st = st;
     stg.start();
     Csr<T> ar, br, cr, ac, bc, cc;
-    ar.build(e, A, false);
-    br.build(e, B, false);
-    cr.build(e, Cm, false);
-    ac.build(e, A, true);
-    bc.build(e, B, true);
-    cc.build(e, Cm, true);
-    // witnesses [32][m] and their slot transpose [m][32]
+    build_csrs<T>(e, {{&ar, &A, false}, {&br, &B, false}, {&cr, &Cm, false}, {&ac, &A, true}, {&bc, &B, true},
+                      {&cc, &Cm, true}});
+    // witnesses [32][m] and their slot transpose [m][32] (made on the device)
     DBuf<L> dw(&pool, 32 * m), slots(&pool, m * 32);
-    {
-      std::vector<L> tr(m * 32);
-      for (size_t j = 0; j < m; j++)
-        for (int kk = 0; kk < 32; kk++) tr[j * 32 + kk] = wit[kk * m + j];
-      e.h2d(dw.p, wit, 32 * m * sizeof(L));
-      e.h2d(slots.p, tr.data(), tr.size() * sizeof(L));
-    }
+    e.h2d(dw.p, wit, 32 * m * sizeof(L));
+    e.launch("k_slots", 64.0 * m * sizeof(L), 0, (k_slots<T>), dim3(grid_for(m, 64, 1u << 16)), dim3(256),
+             (const L*)dw.p, (uint64_t)m, slots.p);
     stg.mark("upload");
     // 1. simd_to_packed (reduce.rs:33-104): w_phi = phi(slots); x_i = (A w_k)_i, y_i = (B w_k)_i
     //    and e_i = phi(x_i) phi(y_i) - phi(x_i . y_i)  (= q1 v_alpha + q2 v_beta exactly,
