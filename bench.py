@@ -49,6 +49,21 @@ def cpu_baseline(k, log_n, seed):
             "sample": f"oracle prove_general at 2^{log_n} u{k} constraints took {dt:.2f} s; scaled x{scale} to 2^20"}
 
 
+_LIMB = {8: "uint8", 16: "uint16", 32: "uint32", 64: "uint64"}
+
+
+def as_limbs(x, k):
+    """Input arrays in the proof's limb type (set-up work, outside the timed
+    region: the instance generators draw uint64 values)."""
+    import numpy as np
+    return np.ascontiguousarray(np.asarray(x).astype(_LIMB[k]))
+
+
+def coo_limbs(m, k):
+    r, c, v = m
+    return (r, c, as_limbs(v, k))
+
+
 def mixed_jobs(na, njobs, seed):
     """The mixed batch: logN ~ U{12..20}, k ~ U{8,16,32,64}, model ~ U{SIMD,
     MIMD, General} (SIMD/MIMD per-instance log = logN - 5, General
@@ -62,15 +77,16 @@ def mixed_jobs(na, njobs, seed):
         if model == 1:
             A, B, C, w = general_instance(log_n, k, seed + i)
             p = na.GeneralSnarkParams.secure(log_n - 5, (A, B, C), 0xDEAD, 0xCAFE, k=k)
-            jobs.append(("general", p, w))
+            jobs.append(("general", p, as_limbs(w, k)))
         elif model == 2:
             As, Bs, Cs, wit = mimd_instance(log_n - 5, k, seed + i)
             p = na.MimdSnarkParams.secure(log_n - 5, log_n - 5, (As, Bs, Cs), 0xDEAD, 0xCAFE, k=k)
-            jobs.append(("mimd", p, wit))
+            jobs.append(("mimd", p, as_limbs(wit, k)))
         else:
             A, B, C, wit = simd_instance(log_n - 5, k, seed + i)
             p = na.SimdSnarkParams.secure(log_n - 5, log_n - 5, 0xDEAD, 0xCAFE, k=k)
-            jobs.append(("simd", p, na.SimdR1csInstance(A, B, C, wit)))
+            jobs.append(("simd", p, na.SimdR1csInstance(coo_limbs(A, k), coo_limbs(B, k), coo_limbs(C, k),
+                                                        as_limbs(wit, k))))
     return jobs
 
 
@@ -91,7 +107,7 @@ def mimd_serving_jobs(na, njobs, seed):
             As, Bs, Cs, _ = mimd_instance(lg, 64, seed + 1000 * lg, max_a=4)
             circ[lg] = (na.MimdSnarkParams.secure(lg, lg, (As, Bs, Cs), 0xDEAD, 0xCAFE, k=64), As, Bs)
         p, As, Bs = circ[lg]
-        jobs.append(("mimd", p, mimd_witnesses(As, Bs, lg, 64, seed + i)))
+        jobs.append(("mimd", p, as_limbs(mimd_witnesses(As, Bs, lg, 64, seed + i), 64)))
     return jobs, sorted(set(sizes))
 
 
@@ -118,13 +134,13 @@ def width_sweep(na, seed, reps=5):
     for k in (8, 16, 32, 64):
         A, B, C, wit = simd_instance(17, k, seed + k)
         p = na.SimdSnarkParams.secure(17, 17, 0xDEAD, 0xCAFE, k=k)
-        inst = na.SimdR1csInstance(A, B, C, wit)
+        inst = na.SimdR1csInstance(coo_limbs(A, k), coo_limbs(B, k), coo_limbs(C, k), as_limbs(wit, k))
         out[f"simd_u{k}_2p17_ms"] = round(time_proofs(lambda: na.prove_simd(inst, p), reps), 2)
         log(f"  simd u{k} 2^17: {out[f'simd_u{k}_2p17_ms']} ms/proof")
         del p, inst
     ln, lm, A, B, C, wit = png_instance(20)
     p = na.SimdSnarkParams.secure(ln, lm, 0xDEAD, 0xCAFE, k=8)
-    inst = na.SimdR1csInstance(A, B, C, wit)
+    inst = na.SimdR1csInstance(coo_limbs(A, 8), coo_limbs(B, 8), coo_limbs(C, 8), as_limbs(wit, 8))
     out["png_u8_2p20_bytes_ms"] = round(time_proofs(lambda: na.prove_simd(inst, p), reps), 2)
     out["png_shape"] = {"n": 2 ** ln, "m": 2 ** lm}
     log(f"  png u8 log_bytes=20 (n=2^{ln}, m=2^{lm}): {out['png_u8_2p20_bytes_ms']} ms/proof")
@@ -372,6 +388,7 @@ def main():
         t0 = time.perf_counter()
         A2, B2, C2, w2 = general_instance(25, 32, args.seed)
         p2 = na.GeneralSnarkParams.secure(20, (A2, B2, C2), 0xDEAD, 0xCAFE, k=32)
+        w2 = as_limbs(w2, 32)
         log(f"general u32 2^25: set up in {time.perf_counter() - t0:.1f}s")
         ms25 = time_proofs(lambda: na.prove_general(w2, p2), 2)
         large["general_u32_2p25_ms"] = round(ms25, 1)

@@ -79,12 +79,21 @@ def _width(k):
     return _DT[k]
 
 
+def _limbs(x, k):
+    """x as a contiguous array of k-bit limbs (wrapping); no copy when it
+    already is one (the library then borrows the caller's array)."""
+    a = np.asarray(x)
+    if a.dtype != _DT[k]:
+        a = a.astype(_DT[k])
+    return np.ascontiguousarray(a)
+
+
 def _coo(m, k):
     _width(k)
     r, c, v = m
     r = np.ascontiguousarray(r, np.uint32)
     c = np.ascontiguousarray(c, np.uint32)
-    v = np.ascontiguousarray(np.asarray(v).astype(_DT[k]))
+    v = _limbs(v, k)
     if not (len(r) == len(c) == len(v)):
         raise NenyaError(1, "COO arrays differ in length")
     return (r, c, v)
@@ -166,7 +175,7 @@ def prove_simd(inst, params):
     """prove_simd(&SimdR1csInstance, &SimdSnarkParams) -> proof bytes."""
     k = params.k
     mats = [_coo(m, k) for m in (inst.a, inst.b, inst.c)]
-    wit = np.ascontiguousarray(np.asarray(inst.witnesses).astype(_DT[k]))
+    wit = _limbs(inst.witnesses, k)
     if wit.shape != (32, 1 << params.log_m):
         raise NenyaError(1, f"witnesses must have shape (32, {1 << params.log_m})")
     out, n = C.c_void_p(), C.c_size_t()
@@ -209,7 +218,7 @@ class MimdSnarkParams(_Params):
 
 
 def _mimd_witnesses(witnesses, params):
-    wit = np.ascontiguousarray(np.asarray(witnesses).astype(_DT[params.k]))
+    wit = _limbs(witnesses, params.k)
     if wit.shape != (32, 1 << params.log_m):
         raise NenyaError(1, f"witnesses must have shape (32, {1 << params.log_m})")
     return wit
@@ -242,7 +251,7 @@ class GeneralSnarkParams(_Params):
 
 def prove_general(w, params):
     """prove_general(w, &GeneralSnarkParams) -> proof bytes."""
-    w = np.ascontiguousarray(np.asarray(w).astype(_DT[params.k]))
+    w = _limbs(w, params.k)
     out, n = C.c_void_p(), C.c_size_t()
     _check(_lib().nenya_prove_general(params._h, w.ctypes.data, len(w), C.byref(out), C.byref(n)))
     return _take_proof(out, n)
@@ -297,7 +306,9 @@ class Server:
     """Batched / served proofs on one GPU (nenya_server_* in include/nenya.h):
     `workers` host threads with their own streams drain a FIFO job queue,
     admitting jobs by device-memory footprint.  Inputs are kept alive by the
-    Server until the job is collected."""
+    Server until the job is collected; inputs that already are contiguous
+    arrays of the params' limb type are borrowed, not copied, so they must
+    not change until then (the reference prover borrows its inputs too)."""
 
     def __init__(self, workers=4, mem_budget=0):
         h = C.c_void_p()
@@ -306,7 +317,7 @@ class Server:
         self._keep = {}
 
     def submit_general(self, params, w):
-        w = np.ascontiguousarray(np.asarray(w).astype(_DT[params.k]))
+        w = _limbs(w, params.k)
         j = C.c_uint64()
         _check(_lib().nenya_submit_general(self._h, params._h, w.ctypes.data, len(w), C.byref(j)))
         self._keep[j.value] = (params, w)
@@ -315,7 +326,7 @@ class Server:
     def submit_simd(self, inst, params):
         k = params.k
         mats = [_coo(m, k) for m in (inst.a, inst.b, inst.c)]
-        wit = np.ascontiguousarray(np.asarray(inst.witnesses).astype(_DT[k]))
+        wit = _limbs(inst.witnesses, k)
         if wit.shape != (32, 1 << params.log_m):
             raise NenyaError(1, f"witnesses must have shape (32, {1 << params.log_m})")
         args = [self._h, params._h]
