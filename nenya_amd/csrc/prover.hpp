@@ -106,6 +106,25 @@ struct Pinned {
   }
 };
 
+// memcpy of a large host block on up to 8 threads (one core's copy rate is
+// well under the host link's).
+inline void par_memcpy(uint8_t* dst, const uint8_t* src, size_t n) {
+  const size_t PART = size_t(1) << 20;
+  const int nt = (int)std::min<size_t>(8, n / PART);
+  if (nt < 2) {
+    std::memcpy(dst, src, n);
+    return;
+  }
+  const size_t per = (n / nt + 63) & ~size_t(63);
+  std::vector<std::thread> th;
+  for (int i = 1; i < nt; i++) {
+    const size_t a = std::min(n, i * per), e = std::min(n, a + per);
+    if (e > a) th.emplace_back([=] { std::memcpy(dst + a, src + a, e - a); });
+  }
+  std::memcpy(dst, src, std::min(n, per));
+  for (auto& t : th) t.join();
+}
+
 // Workgroups of the copy kernel that moves proof sections to pinned host
 // memory; NENYA_DL_BLOCKS overrides the default.
 inline unsigned dl_blocks() {
@@ -359,7 +378,7 @@ template <class T> struct Engine {
     for (size_t off = 0; off < bytes; off += CH) {
       const size_t c = std::min(CH, bytes - off);
       if (off) sync();
-      std::memcpy(b, (const uint8_t*)h + off, c);
+      par_memcpy(b, (const uint8_t*)h + off, c);
       HIP_CHECK(hipMemcpyAsync((uint8_t*)d + off, b, c, hipMemcpyHostToDevice, st));
     }
     sync();
