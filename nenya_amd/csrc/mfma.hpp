@@ -80,22 +80,24 @@ template <int ND> __device__ __forceinline__ void stage_frags(const v4i* gA, v4i
 __device__ __forceinline__ int frag_at(int p, int kb, int rb) { return (p * 4 + kb) * 4 + rb; }
 
 // v[i] = Y^i v[0] for i < 32 by Y^32 = -Y^3 - Y - alpha (all threads of the
-// block call this; threads < 128 own one limb each).
+// block call this).  Doubling: level k makes v[k..2k) = Y^k v[0..k), five
+// levels with one barrier each.  Multiplying by Y^k (k <= 16) shifts by k
+// and sends each of the k top elements h = o[s + 32 - k] (s < k) to
+// -alpha h at Y^s, -h at Y^(s+1) and -h at Y^(s+3), all below Y^32.
 template <class C> __device__ __forceinline__ void mulmat_chain(C (*v)[E]) {
-  const int tid = threadIdx.x;
-  for (int i = 1; i < 32; i++) {
-    if (tid < E) {
-      const int t = tid >> 2, c = tid & 3;
-      const C* o = v[i - 1];
-      C x;
-      if (t == 0) {  // -alpha * o_31 = (h3, h3 - h0, -h1, -h2)
-        const C h0 = o[124], h1 = o[125], h2 = o[126], h3 = o[127];
-        x = (c == 0) ? h3 : (c == 1) ? (C)(h3 - h0) : (c == 2) ? (C)(0 - h1) : (C)(0 - h2);
-      } else {
-        x = o[tid - 4];
-        if (t == 1 || t == 3) x -= o[124 + c];
+  for (int k = 1; k < 32; k <<= 1) {
+    for (int id = threadIdx.x; id < k * E; id += blockDim.x) {
+      const int i = k + id / E, l = id % E;
+      const int t = l >> 2, c = l & 3;
+      const C* o = v[i - k];
+      C x = t >= k ? o[l - 4 * k] : (C)0;
+      if (t < k) {  // -alpha * h = (h3, h3 - h0, -h1, -h2)
+        const C* h = o + 4 * (t + 32 - k);
+        x += (c == 0) ? h[3] : (c == 1) ? (C)(h[3] - h[0]) : (c == 2) ? (C)(0 - h[1]) : (C)(0 - h[2]);
       }
-      v[i][tid] = x;
+      if (t >= 1 && t <= k) x -= o[4 * (t - 1 + 32 - k) + c];
+      if (t >= 3 && t <= k + 2) x -= o[4 * (t - 3 + 32 - k) + c];
+      v[i][l] = x;
     }
     __syncthreads();
   }
