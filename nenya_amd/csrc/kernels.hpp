@@ -859,12 +859,16 @@ constexpr int DPP_HALF_MIRROR = 0x141; // lane i <-> 7-i within 8 lanes
 // matrix transposed ([j][o], computed on the host), so the lift is one matvec
 // and `squarings` = k-1 squarings.
 //
-// Squaring with 256 threads: wave w owns result coefficients t = 8w..8w+7,
-// lane (t, dl) takes the unordered pairs {i, j}, i + j = t mod 32, numbered
-// d = dl and dl+8 (i = t/2 - d, j = t - i), plus d = 16 on dl = 0 for even t.
-// Off-diagonal pairs count twice.  The 8 lanes of a t sum their raw circ/hi
-// terms with a DPP transpose-reduction; the reduction Y^32 = -Y^3 - Y - alpha
-// is applied by 128 threads reading the sums from LDS.
+// Squaring with 256 threads: wave w owns result coefficients t = 8w..8w+7.
+// The unordered pairs {i, j} with i + j = t mod 32 split into n0 = t/2 + 1
+// unwrapped ones (i + j = t: i = q, j = t - q) and n1 = (32 - t)/2 wrapped
+// ones (i + j = t + 32: i = t + 1 + q, j = 31 - q); lane (t, dl) takes three
+// consecutive pairs of one kind (lanes dl < ceil(n0/3) the unwrapped ones),
+// so it keeps a single accumulator and its kind is fixed.  Off-diagonal pairs
+// count twice: they read the left factor from sx2 = 2 x.  The 8 lanes of a t
+// sum their circ (all) and hi (wrapped) terms with a DPP transpose-reduction;
+// the reduction Y^32 = -Y^3 - Y - alpha is applied by 128 threads reading the
+// sums from LDS.
 // Lifts src (device or host-pinned memory) into el with a 256-thread block;
 // the lifted value is left in sx (LDS, C-typed, 16-byte aligned) as well.
 template <class T>
@@ -872,6 +876,7 @@ __device__ __forceinline__ void teich_lift_block(const LT<T>* src, LT<T>* el, co
                                                  CT<T>* sx) {
   using C = CT<T>;
   __shared__ C part[E];
+  __shared__ __attribute__((aligned(16))) C sx2[E];
   __shared__ C sred[2][32][4];  // [0] reduced circ, [1] reduced hi (before the cascade)
   const int tid = threadIdx.x;
   if (tid < E) sx[tid] = (C)src[tid];
@@ -887,40 +892,41 @@ __device__ __forceinline__ void teich_lift_block(const LT<T>* src, LT<T>* el, co
     for (int j = 0; j < 64; j++) s += f[j] * sx[64 * h + j];
     if (h) part[o] = s;
     __syncthreads();
-    if (!h) sx[o] = s + part[o];
+    if (!h) {
+      const C y = s + part[o];
+      sx[o] = y;
+      sx2[o] = y + y;
+    }
     __syncthreads();
   }
   const int lane = tid & 63, dl = lane & 7;
-  const int t = 8 * (tid >> 6) + (lane >> 3), h0 = t >> 1, h1 = t - h0;
-  const bool even = !(t & 1);
+  const int t = 8 * (tid >> 6) + (lane >> 3);
   const bool b2 = dl & 4, b1 = dl & 2, b0 = dl & 1;
+  const int n0 = (t >> 1) + 1, n1 = (32 - t) >> 1, a0 = (n0 + 2) / 3;
+  const bool wrap = dl >= a0;
+  const int q0 = 3 * (wrap ? dl - a0 : dl), nq = wrap ? n1 : n0;
+  const C hm = wrap ? (C)~(C)0 : (C)0;
   for (int it = 0; it < squarings; it++) {
-    C ac[7], ah[7];
+    C ac[7];
 #pragma unroll
-    for (int c = 0; c < 7; c++) ac[c] = ah[c] = 0;
-    auto pair = [&](int d) {
-      const int i = (h0 - d) & 31, j = (h1 + d) & 31;
-      C a[4], b[4], p[7];
-      ldc4(sx + 4 * i, a);
-      ldc4(sx + 4 * j, b);
-      if (!(even && (d == 0 || d == 16))) {
+    for (int c = 0; c < 7; c++) ac[c] = 0;
 #pragma unroll
-        for (int c = 0; c < 4; c++) a[c] += a[c];
+    for (int k = 0; k < 3; k++) {
+      const int q = q0 + k;
+      if (q < nq) {
+        const int i = wrap ? t + 1 + q : q, j = wrap ? 31 - q : t - q;
+        C a[4], b[4], p[7];
+        ldc4((i != j ? sx2 : sx) + 4 * i, a);
+        ldc4(sx + 4 * j, b);
+        d_raw(a, b, p);
+#pragma unroll
+        for (int c = 0; c < 7; c++) ac[c] += p[c];
       }
-      d_raw(a, b, p);
-      const C m = (i > t) ? (C)~(C)0 : (C)0;
-#pragma unroll
-      for (int c = 0; c < 7; c++) {
-        ac[c] += p[c];
-        ah[c] += p[c] & m;
-      }
-    };
-    pair(dl);
-    pair(dl + 8);
-    if (dl == 0 && even) pair(16);
+    }
     C v[8];
     d_red(ac, v);
-    d_red(ah, v + 4);
+#pragma unroll
+    for (int c = 0; c < 4; c++) v[4 + c] = v[c] & hm;
     // lane dl ends with the sum over the 8 lanes of v[dl]
     C k4[4], k2[2];
 #pragma unroll
@@ -947,7 +953,9 @@ __device__ __forceinline__ void teich_lift_block(const LT<T>* src, LT<T>* el, co
       const C lo = sred[0][u][c] - sred[1][u][c] - (hc(u >= 1 ? u - 1 : 0, c) & m1) - (hc(u >= 3 ? u - 3 : 0, c) & m3);
       // -alpha * hi(u): c0 += h3, c1 += h3 - h0, c2 -= h1, c3 -= h2
       const C ma = c <= 1 ? ~(C)0 : (C)0, mb = c >= 1 ? ~(C)0 : (C)0;
-      sx[tid] = lo + (hc(u, 3) & ma) - (hc(u, (c + 3) & 3) & mb);
+      const C y = lo + (hc(u, 3) & ma) - (hc(u, (c + 3) & 3) & mb);
+      sx[tid] = y;
+      sx2[tid] = y + y;
     }
     __syncthreads();
   }
