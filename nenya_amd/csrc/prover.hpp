@@ -117,11 +117,17 @@ inline void par_memcpy(uint8_t* dst, const uint8_t* src, size_t n) {
   }
   const size_t per = (n / nt + 63) & ~size_t(63);
   std::vector<std::thread> th;
-  for (int i = 1; i < nt; i++) {
-    const size_t a = std::min(n, i * per), e = std::min(n, a + per);
-    if (e > a) th.emplace_back([=] { std::memcpy(dst + a, src + a, e - a); });
+  size_t done = std::min(n, per);  // parts [0, done) copied here, the rest by threads
+  try {
+    for (int i = 1; i < nt; i++) {
+      const size_t a = std::min(n, i * per), e = std::min(n, a + per);
+      if (e > a) th.emplace_back([=] { std::memcpy(dst + a, src + a, e - a); });
+    }
+  } catch (...) {  // a thread that could not start: copy its part and the rest here
+    const size_t from = std::min(n, (th.size() + 1) * per);
+    std::memcpy(dst + from, src + from, n - from);
   }
-  std::memcpy(dst, src, std::min(n, per));
+  std::memcpy(dst, src, done);
   for (auto& t : th) t.join();
 }
 

@@ -131,9 +131,14 @@ void build_csrs(Eng& e, std::initializer_list<std::tuple<Csr<T>*, const HostCoo*
   std::vector<typename Csr<T>::Host> hs(jobs.size());
   std::vector<std::thread> th;
   size_t i = 0;
-  for (const auto& j : jobs) {
-    th.emplace_back([&hs, i, j] { hs[i] = Csr<T>::group(*std::get<1>(j), std::get<2>(j)); });
-    i++;
+  try {
+    for (const auto& j : jobs) {
+      th.emplace_back([&hs, i, j] { hs[i] = Csr<T>::group(*std::get<1>(j), std::get<2>(j)); });
+      i++;
+    }
+  } catch (...) {  // a thread that could not start: finish the started ones first
+    for (auto& t : th) t.join();
+    throw;
   }
   for (auto& t : th) t.join();
   i = 0;
