@@ -52,6 +52,17 @@ int nenya_prove_simd(const nenya_params* p,
                      const uint32_t* c_rows, const uint32_t* c_cols, const void* c_vals, size_t c_nnz,
                      const void* witnesses, uint8_t** proof, size_t* proof_len);
 
+/* Same as nenya_prove_simd, but *proof points into a pinned host buffer owned
+ * by `p` (no copy, nothing allocated per call); the bytes stay valid until
+ * the next proof made with `p` or nenya_params_free(p).  No reference
+ * counterpart (the reference returns an owned proof): for callers that read
+ * the proof once, e.g. to send it, without keeping a copy. */
+int nenya_prove_simd_view(const nenya_params* p,
+                          const uint32_t* a_rows, const uint32_t* a_cols, const void* a_vals, size_t a_nnz,
+                          const uint32_t* b_rows, const uint32_t* b_cols, const void* b_vals, size_t b_nnz,
+                          const uint32_t* c_rows, const uint32_t* c_cols, const void* c_vals, size_t c_nnz,
+                          const void* witnesses, const uint8_t** proof, size_t* proof_len);
+
 /* MimdSnarkParams::secure(log_n, log_m, pack_mimd_matrices(a, b, c), seed_w,
  * seed_e) (snark-ring/src/mimd.rs:84-107, reduction/src/reduce.rs:249-273):
  * 32 independent n x m instances (n = 2^log_n, m = 2^log_m).  Each matrix
@@ -69,6 +80,10 @@ int nenya_params_mimd_secure(int k, int log_n, int log_m,
  * MimdSnarkProof::serialize. */
 int nenya_prove_mimd(const nenya_params* p, const void* witnesses, uint8_t** proof, size_t* proof_len);
 
+/* View variant of nenya_prove_mimd (as nenya_prove_simd_view; no reference
+ * counterpart). */
+int nenya_prove_mimd_view(const nenya_params* p, const void* witnesses, const uint8_t** proof, size_t* proof_len);
+
 /* GeneralSnarkParams::secure(log_blocks, pack_general_matrices(a, b, c),
  * seed_w, seed_e) (snark-ring/src/general.rs:69-95,
  * reduction/src/reduce.rs:618-651): the cyclic-diagonal packing of the
@@ -83,6 +98,11 @@ int nenya_params_general_secure(int k, int log_blocks,
  * n = 32 * 2^log_blocks limbs in host memory.  Returns
  * GeneralSnarkProof::serialize. */
 int nenya_prove_general(const nenya_params* p, const void* w, size_t n, uint8_t** proof, size_t* proof_len);
+
+/* View variant of nenya_prove_general (as nenya_prove_simd_view; no
+ * reference counterpart). */
+int nenya_prove_general_view(const nenya_params* p, const void* w, size_t n, const uint8_t** proof,
+                             size_t* proof_len);
 
 /* Same as nenya_prove_general with `w` already resident in device memory
  * (a hipMalloc'd pointer on the current device).  No reference counterpart:

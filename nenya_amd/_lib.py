@@ -32,6 +32,9 @@ def _lib():
         coo = [vp, vp, vp, sz]
         L.nenya_params_simd_secure.argtypes = [C.c_int, C.c_int, C.c_int, u64, u64, C.POINTER(vp)]
         L.nenya_prove_simd.argtypes = [vp] + coo * 3 + [vp, C.POINTER(vp), C.POINTER(sz)]
+        L.nenya_prove_simd_view.argtypes = [vp] + coo * 3 + [vp, C.POINTER(vp), C.POINTER(sz)]
+        L.nenya_prove_mimd_view.argtypes = [vp, vp, C.POINTER(vp), C.POINTER(sz)]
+        L.nenya_prove_general_view.argtypes = [vp, vp, sz, C.POINTER(vp), C.POINTER(sz)]
         L.nenya_params_general_secure.argtypes = [C.c_int, C.c_int] + coo * 3 + [u64, u64, C.POINTER(vp)]
         mset = [vp, vp, vp, vp]
         L.nenya_params_mimd_secure.argtypes = [C.c_int, C.c_int, C.c_int] + mset * 3 + [u64, u64, C.POINTER(vp)]
@@ -171,8 +174,14 @@ class SimdR1csInstance:
     witnesses: np.ndarray  # shape (32, m)
 
 
-def prove_simd(inst, params):
-    """prove_simd(&SimdR1csInstance, &SimdSnarkParams) -> proof bytes."""
+def _view(out, n):
+    return memoryview((C.c_ubyte * n.value).from_address(out.value)).toreadonly()
+
+
+def prove_simd(inst, params, view=False):
+    """prove_simd(&SimdR1csInstance, &SimdSnarkParams) -> proof bytes.
+    view=True returns a read-only memoryview of the params' pinned proof
+    buffer instead (no copy; valid until the next proof with these params)."""
     k = params.k
     mats = [_coo(m, k) for m in (inst.a, inst.b, inst.c)]
     wit = _limbs(inst.witnesses, k)
@@ -182,6 +191,9 @@ def prove_simd(inst, params):
     args = [params._h]
     for m in mats:
         args += _coo_args(m)
+    if view:
+        _check(_lib().nenya_prove_simd_view(*args, wit.ctypes.data, C.byref(out), C.byref(n)))
+        return _view(out, n)
     _check(_lib().nenya_prove_simd(*args, wit.ctypes.data, C.byref(out), C.byref(n)))
     return _take_proof(out, n)
 
@@ -224,10 +236,14 @@ def _mimd_witnesses(witnesses, params):
     return wit
 
 
-def prove_mimd(witnesses, params):
-    """prove_mimd(&[Vec<Gr1>; 32], &MimdSnarkParams) -> proof bytes."""
+def prove_mimd(witnesses, params, view=False):
+    """prove_mimd(&[Vec<Gr1>; 32], &MimdSnarkParams) -> proof bytes (view=True:
+    a read-only view of the params' pinned proof buffer, as prove_simd)."""
     wit = _mimd_witnesses(witnesses, params)
     out, n = C.c_void_p(), C.c_size_t()
+    if view:
+        _check(_lib().nenya_prove_mimd_view(params._h, wit.ctypes.data, C.byref(out), C.byref(n)))
+        return _view(out, n)
     _check(_lib().nenya_prove_mimd(params._h, wit.ctypes.data, C.byref(out), C.byref(n)))
     return _take_proof(out, n)
 
@@ -249,10 +265,14 @@ class GeneralSnarkParams(_Params):
         return p
 
 
-def prove_general(w, params):
-    """prove_general(w, &GeneralSnarkParams) -> proof bytes."""
+def prove_general(w, params, view=False):
+    """prove_general(w, &GeneralSnarkParams) -> proof bytes (view=True: a
+    read-only view of the params' pinned proof buffer, as prove_simd)."""
     w = _limbs(w, params.k)
     out, n = C.c_void_p(), C.c_size_t()
+    if view:
+        _check(_lib().nenya_prove_general_view(params._h, w.ctypes.data, len(w), C.byref(out), C.byref(n)))
+        return _view(out, n)
     _check(_lib().nenya_prove_general(params._h, w.ctypes.data, len(w), C.byref(out), C.byref(n)))
     return _take_proof(out, n)
 

@@ -147,12 +147,13 @@ int nenya_params_simd_secure(int k, int log_n, int log_m, uint64_t seed_w, uint6
   });
 }
 
-int nenya_prove_simd(const nenya_params* p, const uint32_t* ar, const uint32_t* ac, const void* av, size_t an,
-                     const uint32_t* br, const uint32_t* bc, const void* bv, size_t bn, const uint32_t* cr,
-                     const uint32_t* cc, const void* cv, size_t cn, const void* witnesses, uint8_t** proof,
-                     size_t* proof_len) {
+static int simd_common(const nenya_params* p, const uint32_t* ar, const uint32_t* ac, const void* av, size_t an,
+                       const uint32_t* br, const uint32_t* bc, const void* bv, size_t bn, const uint32_t* cr,
+                       const uint32_t* cc, const void* cv, size_t cn, const void* witnesses, uint8_t** proof,
+                       size_t* proof_len, const uint8_t** view) {
   return guard([&] {
-    if (!p || !p->impl || p->impl->kind != 0 || !witnesses || !proof || !proof_len) throw Error(NENYA_E_ARG, "bad argument");
+    if (!p || !p->impl || p->impl->kind != 0 || !witnesses || (!proof && !view) || !proof_len)
+      throw Error(NENYA_E_ARG, "bad argument");
     auto* np = const_cast<nenya_params*>(p);
     std::lock_guard<std::mutex> g(np->mu);
     ParamsBase* b = p->impl;
@@ -174,9 +175,28 @@ int nenya_prove_simd(const nenya_params* p, const uint32_t* ar, const uint32_t* 
       case 32: v = simd_prove_t<W<32>>(b, m3, witnesses); break;
       default: v = simd_prove_t<W<64>>(b, m3, witnesses); break;
     }
-    emit(v, proof, proof_len);
+    if (view) {
+      *view = v.first;
+      *proof_len = v.second;
+    } else {
+      emit(v, proof, proof_len);
+    }
     return NENYA_OK;
   });
+}
+
+int nenya_prove_simd(const nenya_params* p, const uint32_t* ar, const uint32_t* ac, const void* av, size_t an,
+                     const uint32_t* br, const uint32_t* bc, const void* bv, size_t bn, const uint32_t* cr,
+                     const uint32_t* cc, const void* cv, size_t cn, const void* witnesses, uint8_t** proof,
+                     size_t* proof_len) {
+  return simd_common(p, ar, ac, av, an, br, bc, bv, bn, cr, cc, cv, cn, witnesses, proof, proof_len, nullptr);
+}
+
+int nenya_prove_simd_view(const nenya_params* p, const uint32_t* ar, const uint32_t* ac, const void* av, size_t an,
+                          const uint32_t* br, const uint32_t* bc, const void* bv, size_t bn, const uint32_t* cr,
+                          const uint32_t* cc, const void* cv, size_t cn, const void* witnesses, const uint8_t** proof,
+                          size_t* proof_len) {
+  return simd_common(p, ar, ac, av, an, br, bc, bv, bn, cr, cc, cv, cn, witnesses, nullptr, proof_len, proof);
 }
 
 int nenya_params_mimd_secure(int k, int log_n, int log_m, const size_t* a_off, const uint32_t* ar, const uint32_t* ac,
@@ -215,9 +235,11 @@ int nenya_params_mimd_secure(int k, int log_n, int log_m, const size_t* a_off, c
   });
 }
 
-int nenya_prove_mimd(const nenya_params* p, const void* witnesses, uint8_t** proof, size_t* len) {
+static int mimd_common(const nenya_params* p, const void* witnesses, uint8_t** proof, size_t* len,
+                       const uint8_t** view) {
   return guard([&] {
-    if (!p || !p->impl || p->impl->kind != 2 || !witnesses || !proof || !len) throw Error(NENYA_E_ARG, "bad argument");
+    if (!p || !p->impl || p->impl->kind != 2 || !witnesses || (!proof && !view) || !len)
+      throw Error(NENYA_E_ARG, "bad argument");
     auto* np = const_cast<nenya_params*>(p);
     std::lock_guard<std::mutex> g(np->mu);
     ProofView v;
@@ -227,9 +249,22 @@ int nenya_prove_mimd(const nenya_params* p, const void* witnesses, uint8_t** pro
       case 32: v = mimd_t<W<32>>(p->impl, witnesses); break;
       default: v = mimd_t<W<64>>(p->impl, witnesses); break;
     }
-    emit(v, proof, len);
+    if (view) {
+      *view = v.first;
+      *len = v.second;
+    } else {
+      emit(v, proof, len);
+    }
     return NENYA_OK;
   });
+}
+
+int nenya_prove_mimd(const nenya_params* p, const void* witnesses, uint8_t** proof, size_t* len) {
+  return mimd_common(p, witnesses, proof, len, nullptr);
+}
+
+int nenya_prove_mimd_view(const nenya_params* p, const void* witnesses, const uint8_t** proof, size_t* len) {
+  return mimd_common(p, witnesses, nullptr, len, proof);
 }
 
 int nenya_params_general_secure(int k, int log_blocks, const uint32_t* ar, const uint32_t* ac, const void* av, size_t an,
@@ -271,6 +306,10 @@ int nenya_prove_general_device(const nenya_params* p, const void* d_w, size_t n,
 int nenya_prove_general_device_view(const nenya_params* p, const void* d_w, size_t n, const uint8_t** proof,
                                     size_t* len) {
   return general_common(p, d_w, n, nullptr, len, true, proof);
+}
+
+int nenya_prove_general_view(const nenya_params* p, const void* w, size_t n, const uint8_t** proof, size_t* len) {
+  return general_common(p, w, n, nullptr, len, false, proof);
 }
 
 int nenya_verify_simd(const nenya_params* p, const uint32_t* ar, const uint32_t* ac, const void* av, size_t an,

@@ -71,6 +71,25 @@ def test_png_bytes_equal_oracle(orc, log_bytes):
     assert orc.simd_verify(8, ln, lm, A, B, Cm, got) == "Ok"
 
 
+def test_proof_views_equal_owned_proofs():
+    """The *_view entry points leave the same bytes in the params' pinned
+    buffer that the copying entry points return."""
+    print("views: simd k=32 log=7, mimd k=16 log=6, general k=8 log_n=10", flush=True)
+    A, B, Cm, wit = simd_instance(7, 32, 21)
+    ps = na.SimdSnarkParams.secure(7, 7, 0xDEAD, 0xCAFE, k=32)
+    inst = na.SimdR1csInstance(A, B, Cm, wit)
+    owned = na.prove_simd(inst, ps)
+    assert bytes(na.prove_simd(inst, ps, view=True)) == owned
+    As, Bs, Cs, mw = mimd_instance(6, 16, 22)
+    pm = na.MimdSnarkParams.secure(6, 6, (As, Bs, Cs), 0xDEAD, 0xCAFE, k=16)
+    owned = na.prove_mimd(mw, pm)
+    assert bytes(na.prove_mimd(mw, pm, view=True)) == owned
+    A, B, Cm, w = general_instance(10, 8, 23)
+    pg = na.GeneralSnarkParams.secure(5, (A, B, Cm), 0xDEAD, 0xCAFE, k=8)
+    owned = na.prove_general(w, pg)
+    assert bytes(na.prove_general(w, pg, view=True)) == owned
+
+
 def test_proofs_match_committed_golden_digests():
     """The GPU proofs of the committed fixtures hash to the digests stored in
     tests/golden/oracle_c0.json (no oracle call at run time)."""
