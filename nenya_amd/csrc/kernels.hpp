@@ -1095,21 +1095,69 @@ __global__ void k_mimd_slots(const uint64_t* ptr, const uint32_t* col, const LT<
 
 // out[o] (+)= sum_{k in ptr[o]..ptr[o+1]} val[k] * X[idx[k]]   (R0 scalar x R2,
 // the mul_by_gr1 matvecs of matrix.rs:93-105 and simd.rs:570-580)
+// Outputs with heavy_min or more entries keep only the accumulate baseline
+// here; k_spmv_heavy / k_spmv_heavy_sum add their sums.
 template <class T>
 __global__ void __launch_bounds__(GB) k_spmv_gr1(const uint32_t* ptr, const uint32_t* idx, const LT<T>* val,
-                                                 const LT<T>* X, LT<T>* out, uint64_t nout, int accumulate) {
+                                                 const LT<T>* X, LT<T>* out, uint64_t nout, int accumulate,
+                                                 uint32_t heavy_min) {
   int g = threadIdx.x >> 5, t = threadIdx.x & 31;
   for (uint64_t o = (uint64_t)blockIdx.x * GPB + g; o < nout; o += (uint64_t)gridDim.x * GPB) {
     CT<T> acc[4] = {0, 0, 0, 0};
     if (accumulate) ld4<T>(out + o * E + 4 * t, acc);
-    for (uint32_t k = ptr[o]; k < ptr[o + 1]; k++) {
-      CT<T> x[4];
-      ld4<T>(X + (uint64_t)idx[k] * E + 4 * t, x);
-      CT<T> v = val[k];
-      for (int c = 0; c < 4; c++) acc[c] += v * x[c];
+    const uint32_t k0 = ptr[o], k1 = ptr[o + 1];
+    if (k1 - k0 < heavy_min) {
+      for (uint32_t k = k0; k < k1; k++) {
+        CT<T> x[4];
+        ld4<T>(X + (uint64_t)idx[k] * E + 4 * t, x);
+        CT<T> v = val[k];
+        for (int c = 0; c < 4; c++) acc[c] += v * x[c];
+      }
     }
     st4<T>(out + o * E + 4 * t, acc);
   }
+}
+
+// Outputs with many entries (a column every constraint uses, say): block
+// (c, h) sums slice c of the entries of output heavy[h] (its groups stride
+// through the slice) into part[h][c]; k_spmv_heavy_sum adds the slices.
+template <class T>
+__global__ void __launch_bounds__(GB) k_spmv_heavy(const uint32_t* ptr, const uint32_t* idx, const LT<T>* val,
+                                                   const LT<T>* X, const uint32_t* heavy, LT<T>* part) {
+  using C = CT<T>;
+  __shared__ C red[GPB][E];
+  const int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  const uint32_t o = heavy[blockIdx.y];
+  const uint32_t k0 = ptr[o], k1 = ptr[o + 1];
+  const uint32_t per = (k1 - k0 + gridDim.x - 1) / gridDim.x;
+  const uint32_t a = min(k1, k0 + blockIdx.x * per), b = min(k1, a + per);
+  C acc[4] = {0, 0, 0, 0};
+  for (uint32_t k = a + g; k < b; k += GPB) {
+    C x[4];
+    ld4<T>(X + (uint64_t)idx[k] * E + 4 * t, x);
+    const C v = val[k];
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[c] += v * x[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) red[g][4 * t + c] = acc[c];
+  __syncthreads();
+  for (int l = threadIdx.x; l < E; l += GB) {
+    C s = 0;
+#pragma unroll
+    for (int q = 0; q < GPB; q++) s += red[q][l];
+    part[((uint64_t)blockIdx.y * gridDim.x + blockIdx.x) * E + l] = (LT<T>)s;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(E) k_spmv_heavy_sum(const LT<T>* part, uint32_t nch, const uint32_t* heavy,
+                                                      LT<T>* out) {
+  const uint64_t o = heavy[blockIdx.x];
+  const int l = threadIdx.x;
+  CT<T> s = (CT<T>)out[o * E + l];
+  for (uint32_t c = 0; c < nch; c++) s += (CT<T>)part[((uint64_t)blockIdx.x * nch + c) * E + l];
+  out[o * E + l] = (LT<T>)s;
 }
 
 // Sparse R2 x R2 accumulation: out[o] = sum_{k in ptr[o]..} D[k] * X[xsel[k]][xidx[k]]

@@ -84,6 +84,11 @@ template <class T> struct Csr {
   DBuf<uint32_t> ptr, idx;
   DBuf<L> val;
   size_t nnz = 0;
+  // majors with at least HEAVY entries (summed by k_spmv_heavy)
+  static constexpr uint32_t HEAVY = 1024;
+  DBuf<uint32_t> heavy;
+  size_t nheavy = 0;
+  uint32_t maxheavy = 0;
   // the grouped COO on the host
   struct Host {
     std::vector<uint32_t> p, ix;
@@ -120,9 +125,42 @@ template <class T> struct Csr {
       e.h2d(idx.p, h.ix.data(), h.ix.size() * 4);
       e.h2d(val.p, h.v.data(), h.v.size() * sizeof(L));
     }
+    std::vector<uint32_t> hv;
+    maxheavy = 0;
+    for (size_t o = 0; o + 1 < h.p.size(); o++) {
+      const uint32_t c = h.p[o + 1] - h.p[o];
+      if (c >= HEAVY) {
+        hv.push_back((uint32_t)o);
+        maxheavy = std::max(maxheavy, c);
+      }
+    }
+    nheavy = hv.size();
+    if (nheavy) {
+      heavy = DBuf<uint32_t>(pool, nheavy);
+      e.h2d(heavy.p, hv.data(), nheavy * 4);
+    }
   }
   template <class Eng> void build(Eng& e, const HostCoo& m, bool by_col) { upload(e, group(m, by_col)); }
 };
+
+// out[o] (+)= sum over o's entries of val * X[idx] (mul_by_gr1 matvecs,
+// matrix.rs:93-105, simd.rs:570-580): a 32-lane group per output, and the
+// outputs with at least Csr::HEAVY entries split over blocks of ~128
+// entries each (one group walking 32k entries took 10 ms).
+template <class T, class Eng>
+void spmv_gr1(Eng& e, const Csr<T>& m, const typename T::L* X, typename T::L* out, uint64_t nout, int accumulate) {
+  using L = typename T::L;
+  e.launch("k_spmv_gr1", 0, 0, (k_spmv_gr1<T>), dim3(grid_for(nout, GPB)), dim3(GB), (const uint32_t*)m.ptr.p,
+           (const uint32_t*)m.idx.p, (const L*)m.val.p, X, out, nout, accumulate,
+           m.nheavy ? Csr<T>::HEAVY : 0xFFFFFFFFu);
+  if (!m.nheavy) return;
+  const uint32_t nch = std::min<uint32_t>(1024, (m.maxheavy + 127) / 128);
+  DBuf<L> part(e.pool, m.nheavy * nch * E);
+  e.launch("k_spmv_heavy", 0, 0, (k_spmv_heavy<T>), dim3(nch, (unsigned)m.nheavy), dim3(GB), (const uint32_t*)m.ptr.p,
+           (const uint32_t*)m.idx.p, (const L*)m.val.p, X, (const uint32_t*)m.heavy.p, part.p);
+  e.launch("k_spmv_heavy_sum", 0, 0, (k_spmv_heavy_sum<T>), dim3((unsigned)m.nheavy), dim3(E), (const L*)part.p, nch,
+           (const uint32_t*)m.heavy.p, out);
+}
 
 // Several CSR builds: the host grouping of each runs on its own thread, the
 // uploads follow in order on the engine's stream.
@@ -259,10 +297,9 @@ template <class T> struct SimdParams : ParamsBase {
     std::vector<G> rt = e.sample(rs, log_n);
     // 4. a_phi, b_phi, c_phi (matrix.rs:93-105)
     DBuf<L> aw(&pool, n * E), bw(&pool, n * E), cwv(&pool, n * E);
-    unsigned gs = grid_for(n, GPB);
-    e.launch("k_spmv_gr1", 0, 0, (k_spmv_gr1<T>), dim3(gs), dim3(GB), (const uint32_t*)ar.ptr.p, (const uint32_t*)ar.idx.p, (const L*)ar.val.p, (const L*)wphi.p, aw.p, (uint64_t)n, 0);
-    e.launch("k_spmv_gr1", 0, 0, (k_spmv_gr1<T>), dim3(gs), dim3(GB), (const uint32_t*)br.ptr.p, (const uint32_t*)br.idx.p, (const L*)br.val.p, (const L*)wphi.p, bw.p, (uint64_t)n, 0);
-    e.launch("k_spmv_gr1", 0, 0, (k_spmv_gr1<T>), dim3(gs), dim3(GB), (const uint32_t*)cr.ptr.p, (const uint32_t*)cr.idx.p, (const L*)cr.val.p, (const L*)wphi.p, cwv.p, (uint64_t)n, 0);
+    spmv_gr1(e, ar, (const L*)wphi.p, aw.p, (uint64_t)n, 0);
+    spmv_gr1(e, br, (const L*)wphi.p, bw.p, (uint64_t)n, 0);
+    spmv_gr1(e, cr, (const L*)wphi.p, cwv.p, (uint64_t)n, 0);
     // 5. claim_mult = c_phi(r) + e(r)
     DBuf<L> eq_r = e.eq_table(rt.data(), log_n);
     G cw_at_r = e.template dot<32>(eq_r.p, cwv.p, n);
@@ -287,10 +324,9 @@ template <class T> struct SimdParams : ParamsBase {
     DBuf<L> dg = e.put(&gam, 1), dg2 = e.put(&gam2, 1);
     e.scale(dg.p, eq_s1.p, geq.p, n);
     e.scale(dg2.p, eq_r.p, g2eq.p, n);
-    unsigned gm = grid_for(m, GPB);
-    e.launch("k_spmv_gr1", 0, 0, (k_spmv_gr1<T>), dim3(gm), dim3(GB), (const uint32_t*)ac.ptr.p, (const uint32_t*)ac.idx.p, (const L*)ac.val.p, (const L*)eq_s1.p, mrow.p, (uint64_t)m, 0);
-    e.launch("k_spmv_gr1", 0, 0, (k_spmv_gr1<T>), dim3(gm), dim3(GB), (const uint32_t*)bc.ptr.p, (const uint32_t*)bc.idx.p, (const L*)bc.val.p, (const L*)geq.p, mrow.p, (uint64_t)m, 1);
-    e.launch("k_spmv_gr1", 0, 0, (k_spmv_gr1<T>), dim3(gm), dim3(GB), (const uint32_t*)cc.ptr.p, (const uint32_t*)cc.idx.p, (const L*)cc.val.p, (const L*)g2eq.p, mrow.p, (uint64_t)m, 1);
+    spmv_gr1(e, ac, (const L*)eq_s1.p, mrow.p, (uint64_t)m, 0);
+    spmv_gr1(e, bc, (const L*)geq.p, mrow.p, (uint64_t)m, 1);
+    spmv_gr1(e, cc, (const L*)g2eq.p, mrow.p, (uint64_t)m, 1);
     // 9. claim_lin
     G claim_lin = mp.finals[0] + gam * mp.finals[1] + gam2 * cw_at_r;
     tr.elems("claim_lin", &claim_lin, 1);

@@ -558,13 +558,9 @@ template <class T> struct Verifier {
       DBuf<L> dg = e.put(&gam, 1), dg2 = e.put(&gam2, 1);
       e.scale(dg.p, eq_s1.p, geq.p, n);
       e.scale(dg2.p, eq_r.p, g2eq.p, n);
-      const unsigned gm = grid_for(m, GPB);
-      e.launch("k_spmv_gr1", 0, 0, (k_spmv_gr1<T>), dim3(gm), dim3(GB), (const uint32_t*)ac.ptr.p,
-               (const uint32_t*)ac.idx.p, (const L*)ac.val.p, (const L*)eq_s1.p, mrow.p, (uint64_t)m, 0);
-      e.launch("k_spmv_gr1", 0, 0, (k_spmv_gr1<T>), dim3(gm), dim3(GB), (const uint32_t*)bc.ptr.p,
-               (const uint32_t*)bc.idx.p, (const L*)bc.val.p, (const L*)geq.p, mrow.p, (uint64_t)m, 1);
-      e.launch("k_spmv_gr1", 0, 0, (k_spmv_gr1<T>), dim3(gm), dim3(GB), (const uint32_t*)cc.ptr.p,
-               (const uint32_t*)cc.idx.p, (const L*)cc.val.p, (const L*)g2eq.p, mrow.p, (uint64_t)m, 1);
+      spmv_gr1(e, ac, (const L*)eq_s1.p, mrow.p, (uint64_t)m, 0);
+      spmv_gr1(e, bc, (const L*)geq.p, mrow.p, (uint64_t)m, 1);
+      spmv_gr1(e, cc, (const L*)g2eq.p, mrow.p, (uint64_t)m, 1);
       at_s2 = e.template dot<32>(eq_s2.p, mrow.p, m);
     }
     if (!(at_s2 == lp.fin[0])) return V_AROW;
