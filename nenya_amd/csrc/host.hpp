@@ -69,6 +69,24 @@ struct Pool {
   std::vector<std::pair<void*, size_t>> all_;
   size_t live = 0, peak = 0;
   std::mutex mu;
+  // While deferring (work on several streams), returned blocks wait here
+  // until the streams have joined: one stream may still read a block
+  // another has let go of.
+  int defer = 0;
+  std::vector<std::pair<void*, size_t>> deferred;
+  void defer_frees(bool on) {
+    std::lock_guard<std::mutex> g(mu);
+    if (on) {
+      defer++;
+      return;
+    }
+    if (defer == 0 || --defer > 0) return;
+    for (auto& d : deferred) {
+      free_.insert({d.second, d.first});
+      live -= d.second;
+    }
+    deferred.clear();
+  }
   void* get(size_t bytes) {
     bytes = size_class(bytes);
     std::lock_guard<std::mutex> g(mu);
@@ -150,6 +168,10 @@ struct Pool {
     bytes = size_class(bytes);
     if (pool_guard()) check_guards(p, bytes);
     std::lock_guard<std::mutex> g(mu);
+    if (defer) {
+      deferred.push_back({p, bytes});
+      return;
+    }
     free_.insert({bytes, p});
     live -= bytes;
   }

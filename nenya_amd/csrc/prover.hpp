@@ -338,7 +338,12 @@ template <class T> struct Engine {
   size_t up_off = 0;
   uint8_t* ring_slot(size_t bytes) {
     if (up_off + bytes > UPCAP) {
+      // every stream that may still copy from the ring
       sync();
+      if (forked) {
+        HIP_CHECK(hipStreamSynchronize(st_main));
+        for (int i = 0; i < forked; i++) HIP_CHECK(hipStreamSynchronize(sst[i]));
+      }
       up_off = 0;
     }
     uint8_t* slot = upbuf.get(UPCAP) + up_off;
@@ -636,7 +641,57 @@ template <class T> struct Engine {
     src.p = nullptr;
     src.n = 0;
   }
+  // ---------------- concurrent phases ----------------
+  // Independent work of a proof (its commitments) runs on up to four
+  // streams: fork() makes the side streams wait for the prover stream's work
+  // so far, on(i) sends launches to stream i (0: the prover stream), join()
+  // makes the prover stream wait for the side streams.  Blocks freed in
+  // between go back to the pool at join().  Server contexts stay on one
+  // stream (their workers already overlap), and so do the debug pools.
+  static constexpr int NSIDE = 3;
+  hipStream_t sst[NSIDE] = {};
+  hipEvent_t fev = nullptr, jev[NSIDE] = {};
+  hipStream_t st_main = nullptr;
+  int forked = 0;
+  void fork(int ns) {
+    st_main = st;
+    forked = (pipelined && !pool_poison() && !pool_guard()) ? std::min(ns - 1, NSIDE) : 0;
+    if (forked <= 0) {
+      forked = 0;
+      return;
+    }
+    if (!fev) {
+      HIP_CHECK(hipEventCreateWithFlags(&fev, hipEventDisableTiming));
+      for (int i = 0; i < NSIDE; i++) {
+        HIP_CHECK(hipEventCreateWithFlags(&jev[i], hipEventDisableTiming));
+        HIP_CHECK(hipStreamCreate(&sst[i]));
+      }
+    }
+    HIP_CHECK(hipEventRecord(fev, st));
+    for (int i = 0; i < forked; i++) HIP_CHECK(hipStreamWaitEvent(sst[i], fev, 0));
+    pool->defer_frees(true);
+  }
+  void on(int i) {
+    if (forked) st = i == 0 ? st_main : sst[(i - 1) % forked];
+  }
+  void join() {
+    if (!forked) return;
+    for (int i = 0; i < forked; i++) {
+      HIP_CHECK(hipEventRecord(jev[i], sst[i]));
+      HIP_CHECK(hipStreamWaitEvent(st_main, jev[i], 0));
+    }
+    st = st_main;
+    forked = 0;
+    pool->defer_frees(false);
+  }
   void finish_downloads() {
+    // a proof that failed between fork() and join() left the streams forked
+    if (forked) {
+      st = st_main;
+      for (int i = 0; i < forked; i++) (void)hipStreamSynchronize(sst[i]);
+      forked = 0;
+      pool->defer_frees(false);
+    }
     // the prover stream's own writes into the proof buffer (server contexts
     // copy their sections on it; k_queries writes indices directly) and any
     // commitment roots a failed proof left unresolved
@@ -648,6 +703,14 @@ template <class T> struct Engine {
     held.clear();
   }
   ~Engine() {
+    for (int i = 0; i < NSIDE; i++)
+      if (sst[i]) {
+        (void)hipStreamSynchronize(sst[i]);
+        (void)hipStreamDestroy(sst[i]);
+      }
+    if (fev) (void)hipEventDestroy(fev);
+    for (int i = 0; i < NSIDE; i++)
+      if (jev[i]) (void)hipEventDestroy(jev[i]);
     if (cst) {
       (void)hipStreamSynchronize(cst);
       for (auto& h : held) pool->put(h.first, h.second);

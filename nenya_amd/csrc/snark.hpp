@@ -285,8 +285,13 @@ template <class T> struct SimdParams : ParamsBase {
     e.template mul_sub<16>(px.p, py.p, ev.p, ev.p, n);
     stg.mark("reduce");
     // 2. commits
-    CommitDev<T> cw = e.commit(pw, slots.p, pw.P.k1, wphi.p);
+    // the commitments are independent: one stream each
+    e.fork(2);
+    e.on(1);
     CommitDev<T> ce = e.commit(pq, ev.p, pq.P.k1, ev.p);
+    e.on(0);
+    CommitDev<T> cw = e.commit(pw, slots.p, pw.P.k1, wphi.p);
+    e.join();
     e.resolve_roots({&cw, &ce});
     stg.mark("commit");
     // 3. transcript
@@ -630,10 +635,17 @@ template <class T> struct MimdParams : ParamsBase {
     e.sub(estack.p, hats.p, estack.p, 3 * n * E);
     stg.mark("reduce_matvec");
     // 4. commits (mimd.rs:302-316)
-    CommitDev<T> cw = e.commit(pw, slots.p, pw.P.k1, wphi.p);
-    CommitDev<T> cabc = e.commit(pabc, abc_slots.p, 3 * pabc.P.k1, hats.p);
-    CommitDev<T> cem = e.commit(pe, emult, pe.P.k1, emult);
+    // the four commitments are independent: one stream each
+    e.fork(4);
+    e.on(1);
     CommitDev<T> ces = e.commit(pe, estack.p, 3 * pe.P.k1, estack.p);
+    e.on(2);
+    CommitDev<T> cabc = e.commit(pabc, abc_slots.p, 3 * pabc.P.k1, hats.p);
+    e.on(3);
+    CommitDev<T> cem = e.commit(pe, emult, pe.P.k1, emult);
+    e.on(0);
+    CommitDev<T> cw = e.commit(pw, slots.p, pw.P.k1, wphi.p);
+    e.join();
     e.resolve_roots({&cw, &cabc, &cem, &ces});
     stg.mark("commit");
     Transcript tr("mimd");
@@ -962,10 +974,17 @@ template <class T> struct GeneralParams : ParamsBase {
     sums.release();
     stg.mark("reduce_matvec");
     // 6. commits (general.rs:434-451)
-    CommitDev<T> cw = e.commit(pw, w, pw.P.k1, rho.p);
-    CommitDev<T> cabc = e.commit(pabc, abc_slots.p, 3 * pabc.P.k1, hats.p);
-    CommitDev<T> cem = e.commit(pe, emult, pe.P.k1, emult);
+    // the four commitments are independent: one stream each
+    e.fork(4);
+    e.on(1);
     CommitDev<T> ces = e.commit(pe, estack.p, 3 * pe.P.k1, estack.p);
+    e.on(2);
+    CommitDev<T> cabc = e.commit(pabc, abc_slots.p, 3 * pabc.P.k1, hats.p);
+    e.on(3);
+    CommitDev<T> cem = e.commit(pe, emult, pe.P.k1, emult);
+    e.on(0);
+    CommitDev<T> cw = e.commit(pw, w, pw.P.k1, rho.p);
+    e.join();
     e.resolve_roots({&cw, &cabc, &cem, &ces});
     stg.mark("commit");
     Transcript tr("general");
