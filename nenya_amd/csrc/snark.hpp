@@ -944,15 +944,21 @@ template <class T> struct GeneralParams : ParamsBase {
     DBuf<L> sums(&pool, 3 * nb * E);
     if constexpr (kMx<T>) {
       // sum_{s,s'} C_p[s][s'] b_s b_s' on the matrix cores
-      DBuf<L> cm(&pool, nb * 1024);
+      // the three matrices are independent: one stream each (k_cgemm is
+      // bound by load latency, so three in flight overlap well)
+      DBuf<L> cm(&pool, 3 * nb * 1024);
+      e.fork(3);
       for (int m = 0; m < 3; m++) {
+        e.on(m);
+        L* cmm = cm.p + (size_t)m * nb * 1024;
         e.launch("k_cmat", (double)nb * 1024 * sizeof(L), 0, (k_cmat<T>), dim3(grid_for(nb, 8, 4096)), dim3(256),
-                 (const uint32_t*)mr[m].ptr.p, (const uint32_t*)mr[m].idx.p, (const L*)mr[m].val.p, w, cm.p,
+                 (const uint32_t*)mr[m].ptr.p, (const uint32_t*)mr[m].idx.p, (const L*)mr[m].val.p, w, cmm,
                  (uint64_t)nb);
         e.launch("k_cgemm", (double)nb * (1024 * sizeof(L) + Engine<T>::EB), (double)nb * 1024 * E, (k_cgemm<T>),
-                 dim3(grid_for(nb, 32, 1024)), dim3(256), (const uint32_t*)e.d_b2frag.p, (const L*)cm.p,
+                 dim3(grid_for(nb, 32, 1024)), dim3(256), (const uint32_t*)e.d_b2frag.p, (const L*)cmm,
                  sums.p + m * nb * E, (uint64_t)nb);
       }
+      e.join();
     } else {
       for (int m = 0; m < 3; m++)
         e.launch("k_spmv_gr", ncell[m] * 2.0 * Engine<T>::EB, ncell[m] * 16 * Engine<T>::PM, (k_spmv_gr<T>),
