@@ -554,15 +554,20 @@ template <class T> struct MimdParams : ParamsBase {
   // sum_c Xhat[r, c] w_phi[c] for the three matrices into out (3 x n)
   void matvecs(Engine<T>& e, const L* slots, const L* wphi, L* out) {
     if constexpr (kMx<T>) {
-      DBuf<L> cm(e.pool, n * 1024);
+      // the three matrices are independent: one stream each
+      DBuf<L> cm(e.pool, 3 * n * 1024);
+      e.fork(3);
       for (int w = 0; w < 3; w++) {
+        e.on(w);
+        L* cmw = cm.p + (size_t)w * n * 1024;
         e.launch("k_cmat_mimd", (double)n * 1024 * sizeof(L), (double)ncell[w] * 1024, (k_cmat_mimd<T>),
                  dim3(grid_for(n, 8, 4096)), dim3(256), (const uint64_t*)(cptr.p + w * (n + 1)), (const uint32_t*)ccol.p,
-                 (const L*)cval.p, slots, cm.p, (uint64_t)n);
+                 (const L*)cval.p, slots, cmw, (uint64_t)n);
         e.launch("k_cgemm", (double)n * (1024 * sizeof(L) + Engine<T>::EB), (double)n * 1024 * E, (k_cgemm<T>),
-                 dim3(grid_for(n, 32, 1024)), dim3(256), (const uint32_t*)e.d_b2frag.p, (const L*)cm.p,
+                 dim3(grid_for(n, 32, 1024)), dim3(256), (const uint32_t*)e.d_b2frag.p, (const L*)cmw,
                  out + w * n * E, (uint64_t)n);
       }
+      e.join();
     } else {
       (void)slots;
       for (int w = 0; w < 3; w++)
