@@ -923,6 +923,29 @@ template <class T> struct GeneralParams : ParamsBase {
     }
   }
 
+  // layer_rows with eqs = {eq_s1, gamma eq_s1, gamma^2 eq_s1} formed here
+  // (k <= 32): each w's third of eqs and of the products G on its own
+  // stream, so the scalings overlap the other thirds' products.
+  void layer_rows_from(Engine<T>& e, const L* eq1, const L* dg, const L* dg2, L* eqs, L* rows) {
+    DBuf<L> G(e.pool, 3 * nb * 16 * E);
+    const uint64_t ntiles = (nb + 31) / 32;
+    e.fork(3);
+    for (int w = 0; w < 3; w++) {
+      e.on(w);
+      L* ew = eqs + (size_t)w * nb * E;
+      if (w == 0) e.d2d(ew, eq1, nb * E * sizeof(L));
+      else e.scale(w == 1 ? dg : dg2, eq1, ew, nb);
+      e.launch("k_mulfix_mx", nb * 17.0 * Engine<T>::EB, nb * 16.0 * E * E, (k_mulfix_mx<T>),
+               dim3(grid_for(ntiles, 4, 32), 16), dim3(256), (const uint32_t*)e.d_lbfrag.p, (const L*)ew, (uint64_t)nb,
+               G.p + (size_t)w * nb * 16 * E);
+    }
+    e.join();
+    const double ne = (double)(mr[0].nnz + mr[1].nnz + mr[2].nnz);
+    e.launch("k_rows_gather", ne * (Engine<T>::EB + 8) + 32.0 * nb * Engine<T>::EB, ne * E, (k_rows_gather<T>),
+             dim3(grid_for(32 * nb, 8, 1u << 16)), dim3(256), (const uint64_t*)eptr.p, (const uint32_t*)ecode.p,
+             (const L*)eval.p, (const L*)G.p, (uint64_t)nb, rows, (uint64_t)(32 * nb));
+  }
+
   ProofView prove_device(Ctx<T>& cx, const L* w /* device, N limbs */) {
     Engine<T>& e = *cx.eng;
     e.finish_downloads();  // a failed earlier proof may have left copies in flight
@@ -1039,15 +1062,19 @@ template <class T> struct GeneralParams : ParamsBase {
     // 9. layer rows (general.rs:233-271) as sum over cells of
     //    {eq_s1, gamma eq_s1, gamma^2 eq_s1}[p] * D[cell]
     DBuf<L> eqs(&pool, 3 * nb * E);
+    DBuf<L> rows(&pool, 32 * nb * E);
     {
       DBuf<L> eq1 = e.eq_table(mp.chal.data(), log_b);
-      e.d2d(eqs.p, eq1.p, nb * E * sizeof(L));
       DBuf<L> dg = e.put(&gam, 1), dg2 = e.put(&gam2, 1);
-      e.scale(dg.p, eq1.p, eqs.p + nb * E, nb);
-      e.scale(dg2.p, eq1.p, eqs.p + 2 * nb * E, nb);
+      if constexpr (kMx<T>) {
+        layer_rows_from(e, eq1.p, dg.p, dg2.p, eqs.p, rows.p);
+      } else {
+        e.d2d(eqs.p, eq1.p, nb * E * sizeof(L));
+        e.scale(dg.p, eq1.p, eqs.p + nb * E, nb);
+        e.scale(dg2.p, eq1.p, eqs.p + 2 * nb * E, nb);
+        layer_rows(e, eqs.p, rows.p);
+      }
     }
-    DBuf<L> rows(&pool, 32 * nb * E);
-    layer_rows(e, eqs.p, rows.p);
     eqs.release();
     stg.mark("layer_rows");
     // 10. linear check over the 32 (rho_l, row_l) pairs; rho is low-degree
