@@ -394,6 +394,37 @@ template <class T> struct Engine {
     }
     sync();
   }
+  // Several uploads with one synchronization: small ones share the pinned
+  // bounce buffer (a synchronization only when it fills), larger ones take
+  // the chunked path.
+  struct Up {
+    void* d;
+    const void* h;
+    size_t bytes;
+  };
+  void h2d_many(const std::vector<Up>& ups) {
+    const size_t CH = size_t(8) << 20;
+    uint8_t* b = xfer.get(CH);
+    size_t off = 0;
+    for (const Up& u : ups) {
+      if (!u.bytes) continue;
+      const size_t sz = (u.bytes + 255) & ~size_t(255);
+      if (sz > CH / 2) {
+        if (off) sync();
+        off = 0;
+        h2d(u.d, u.h, u.bytes);  // synchronizes at its end
+        continue;
+      }
+      if (off + sz > CH) {
+        sync();
+        off = 0;
+      }
+      par_memcpy(b + off, (const uint8_t*)u.h, u.bytes);
+      HIP_CHECK(hipMemcpyAsync(u.d, b + off, u.bytes, hipMemcpyHostToDevice, st));
+      off += sz;
+    }
+    sync();
+  }
   void d2h(void* h, const void* d, size_t bytes) {
     if (!bytes) return;
     uint8_t* b = xfer.get(bytes);

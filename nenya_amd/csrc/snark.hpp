@@ -113,19 +113,20 @@ template <class T> struct Csr {
     }
     return h;
   }
-  // h2d returns once the host vectors may be reused
-  template <class Eng> void upload(Eng& e, const Host& h) {
+  // Allocates the device arrays and lists their uploads from h (and from
+  // hv, which receives the heavy majors; both must outlive the uploads).
+  template <class Eng> void prepare(Eng& e, const Host& h, std::vector<uint32_t>& hv, std::vector<typename Eng::Up>& ups) {
     Pool* pool = e.pool;
     nnz = h.ix.size();
     ptr = DBuf<uint32_t>(pool, h.p.size());
     idx = DBuf<uint32_t>(pool, std::max<size_t>(nnz, 1));
     val = DBuf<L>(pool, std::max<size_t>(nnz, 1));
-    e.h2d(ptr.p, h.p.data(), h.p.size() * 4);
+    ups.push_back({ptr.p, h.p.data(), h.p.size() * 4});
     if (nnz) {
-      e.h2d(idx.p, h.ix.data(), h.ix.size() * 4);
-      e.h2d(val.p, h.v.data(), h.v.size() * sizeof(L));
+      ups.push_back({idx.p, h.ix.data(), h.ix.size() * 4});
+      ups.push_back({val.p, h.v.data(), h.v.size() * sizeof(L)});
     }
-    std::vector<uint32_t> hv;
+    hv.clear();
     maxheavy = 0;
     for (size_t o = 0; o + 1 < h.p.size(); o++) {
       const uint32_t c = h.p[o + 1] - h.p[o];
@@ -137,8 +138,14 @@ template <class T> struct Csr {
     nheavy = hv.size();
     if (nheavy) {
       heavy = DBuf<uint32_t>(pool, nheavy);
-      e.h2d(heavy.p, hv.data(), nheavy * 4);
+      ups.push_back({heavy.p, hv.data(), nheavy * 4});
     }
+  }
+  template <class Eng> void upload(Eng& e, const Host& h) {
+    std::vector<uint32_t> hv;
+    std::vector<typename Eng::Up> ups;
+    prepare(e, h, hv, ups);
+    e.h2d_many(ups);
   }
   template <class Eng> void build(Eng& e, const HostCoo& m, bool by_col) { upload(e, group(m, by_col)); }
 };
@@ -179,8 +186,15 @@ void build_csrs(Eng& e, std::initializer_list<std::tuple<Csr<T>*, const HostCoo*
     throw;
   }
   for (auto& t : th) t.join();
+  // every array of every CSR in one batch of uploads
+  std::vector<std::vector<uint32_t>> hv(jobs.size());
+  std::vector<typename Eng::Up> ups;
   i = 0;
-  for (const auto& j : jobs) std::get<0>(j)->upload(e, hs[i++]);
+  for (const auto& j : jobs) {
+    std::get<0>(j)->prepare(e, hs[i], hv[i], ups);
+    i++;
+  }
+  e.h2d_many(ups);
 }
 
 // Uploads a CSR of (dk, sel, idx) entries (the k_spmv_gr layout).
