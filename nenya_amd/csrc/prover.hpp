@@ -334,7 +334,8 @@ template <class T> struct Engine {
   // bounce buffer.  A ring slot is recycled only after a synchronization
   // that follows its use (the ring rotates; it is not reset per round).
   static constexpr size_t UPCAP = 1 << 16;
-  Pinned upbuf, xfer;
+  Pinned upbuf, xfer, xfer2;
+  hipEvent_t xev[2] = {};
   size_t up_off = 0;
   uint8_t* ring_slot(size_t bytes) {
     if (up_off + bytes > UPCAP) {
@@ -384,13 +385,22 @@ template <class T> struct Engine {
       HIP_CHECK(hipMemcpyAsync(d, slot, bytes, hipMemcpyHostToDevice, st));
       return;
     }
-    const size_t CH = size_t(8) << 20;  // bounce in chunks, waiting before reuse
-    uint8_t* b = xfer.get(std::min(bytes, CH));
-    for (size_t off = 0; off < bytes; off += CH) {
+    // bounce in chunks through two pinned buffers, so the host fills one
+    // while the other's copy runs (a buffer is refilled once its last copy
+    // has ended)
+    const size_t CH = size_t(8) << 20;
+    uint8_t* bb[2] = {xfer.get(std::min(bytes, CH)), bytes > CH ? xfer2.get(CH) : nullptr};
+    if (bytes > CH && !xev[0]) {
+      HIP_CHECK(hipEventCreateWithFlags(&xev[0], hipEventDisableTiming));
+      HIP_CHECK(hipEventCreateWithFlags(&xev[1], hipEventDisableTiming));
+    }
+    int k = 0;
+    for (size_t off = 0; off < bytes; off += CH, k ^= 1) {
       const size_t c = std::min(CH, bytes - off);
-      if (off) sync();
-      par_memcpy(b, (const uint8_t*)h + off, c);
-      HIP_CHECK(hipMemcpyAsync((uint8_t*)d + off, b, c, hipMemcpyHostToDevice, st));
+      if (off >= 2 * CH) HIP_CHECK(hipEventSynchronize(xev[k]));
+      par_memcpy(bb[k], (const uint8_t*)h + off, c);
+      HIP_CHECK(hipMemcpyAsync((uint8_t*)d + off, bb[k], c, hipMemcpyHostToDevice, st));
+      if (bytes > CH) HIP_CHECK(hipEventRecord(xev[k], st));
     }
     sync();
   }
@@ -740,6 +750,8 @@ template <class T> struct Engine {
         (void)hipStreamDestroy(sst[i]);
       }
     if (fev) (void)hipEventDestroy(fev);
+    for (auto& x : xev)
+      if (x) (void)hipEventDestroy(x);
     for (int i = 0; i < NSIDE; i++)
       if (jev[i]) (void)hipEventDestroy(jev[i]);
     if (cst) {
