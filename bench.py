@@ -33,20 +33,50 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+GOLDEN = os.path.join(ROOT, "tests", "golden", "oracle_bench.json")
+
+
+def golden(name):
+    """The oracle's digest record for a benchmark workload, if committed."""
+    try:
+        return json.load(open(GOLDEN)).get(name)
+    except (OSError, ValueError):
+        return None
+
+
+def digest(b):
+    import hashlib
+    return hashlib.blake2b(bytes(b), digest_size=16).hexdigest()
+
+
+def check_proof(proof, gold, verdict):
+    """Verification record of a timed proof: the library verifier's verdict
+    and, where the oracle's digest for this workload is committed, whether
+    the bytes match it."""
+    out = {"verifier": verdict}
+    if gold is not None:
+        out["oracle_digest_match"] = len(proof) == gold["len"] and digest(proof) == gold["digest"]
+    out["ok"] = verdict == "Ok" and out.get("oracle_digest_match", True)
+    return out
+
+
 def cpu_baseline(k, log_n, seed):
-    """The oracle (C restatement of the reference prover, one thread) timed on
-    a bounded sample: one proof at 2^log_n constraints, scaled linearly to
-    2^20 (the prover is linear in n up to log factors)."""
+    """The oracle (the C restatement of the reference prover's algorithm,
+    including the per-triple quotients and the reference's lift) proving the
+    same workload on the host's cores (OpenMP threads over its loops), timed
+    once; its bytes are checked against the committed digest."""
     from oracle import load
     from nenya_amd.instances import general_instance
     o = load()
     A, B, C, w = general_instance(log_n, k, seed)
     t = time.perf_counter()
-    o.general_prove(k, log_n - 5, A, B, C, w)
+    p = o.general_prove(k, log_n - 5, A, B, C, w)
     dt = time.perf_counter() - t
-    scale = 2 ** (20 - log_n)
-    return {"value": round(dt * scale, 3), "unit": "s", "cores": 1, "kind": "port",
-            "sample": f"oracle prove_general at 2^{log_n} u{k} constraints took {dt:.2f} s; scaled x{scale} to 2^20"}
+    g = golden(f"general_u{k}_logn{log_n}_seed{seed}")
+    return {"value": round(dt, 3), "unit": "s", "cores": o.threads(), "nproc": os.cpu_count(), "kind": "port",
+            "sample": f"one oracle prove_general at 2^{log_n} u{k} constraints (the full workload, not scaled), "
+                      f"{o.threads()} OpenMP threads",
+            "oracle_digest_match": (g is not None and digest(p) == g["digest"]) if g else None}
 
 
 _LIMB = {8: "uint8", 16: "uint16", 32: "uint32", 64: "uint64"}
@@ -131,21 +161,27 @@ def width_sweep(na, seed, reps=5):
     filter batch (u8, 32 images x 2^15 bytes) -- ms per proof, host inputs,
     the proof left in the params' pinned buffer (a view, no copy)."""
     from nenya_amd.instances import png_instance, simd_instance
-    out = {}
+    out, ver = {}, {}
     for k in (8, 16, 32, 64):
         A, B, C, wit = simd_instance(17, k, seed + k)
         p = na.SimdSnarkParams.secure(17, 17, 0xDEAD, 0xCAFE, k=k)
         inst = na.SimdR1csInstance(coo_limbs(A, k), coo_limbs(B, k), coo_limbs(C, k), as_limbs(wit, k))
         out[f"simd_u{k}_2p17_ms"] = round(time_proofs(lambda: na.prove_simd(inst, p, view=True), reps), 2)
-        log(f"  simd u{k} 2^17: {out[f'simd_u{k}_2p17_ms']} ms/proof")
+        last = bytes(na.prove_simd(inst, p, view=True))
+        ver[f"simd_u{k}"] = check_proof(last, golden(f"simd_u{k}_log17_seed{seed + k}"), na.verify_simd(inst, p, last))
+        log(f"  simd u{k} 2^17: {out[f'simd_u{k}_2p17_ms']} ms/proof  {ver[f'simd_u{k}']}")
         del p, inst
-    ln, lm, A, B, C, wit = png_instance(20)
+    ln, lm, A, B, C, wit = png_instance(20, "gradient")
     p = na.SimdSnarkParams.secure(ln, lm, 0xDEAD, 0xCAFE, k=8)
     inst = na.SimdR1csInstance(coo_limbs(A, 8), coo_limbs(B, 8), coo_limbs(C, 8), as_limbs(wit, 8))
     out["png_u8_2p20_bytes_ms"] = round(time_proofs(lambda: na.prove_simd(inst, p, view=True), reps), 2)
+    last = bytes(na.prove_simd(inst, p, view=True))
+    ver["png_u8"] = check_proof(last, golden("png_u8_logbytes20_gradient"), na.verify_simd(inst, p, last))
     out["proofs"] = "left in the params' pinned buffer (view), as the headline's"
-    out["png_shape"] = {"n": 2 ** ln, "m": 2 ** lm}
-    log(f"  png u8 log_bytes=20 (n=2^{ln}, m=2^{lm}): {out['png_u8_2p20_bytes_ms']} ms/proof")
+    out["png_shape"] = {"n": 2 ** ln, "m": 2 ** lm,
+                        "images": "32 per-row linear gradients + uniform +-16 noise (nenya_amd.instances.png_instance)"}
+    out["verified"] = ver
+    log(f"  png u8 log_bytes=20 (n=2^{ln}, m=2^{lm}): {out['png_u8_2p20_bytes_ms']} ms/proof  {ver['png_u8']}")
     return out
 
 
@@ -173,6 +209,34 @@ def run_queue(srv, jobs, arrivals=None):
         lat.append(q + r)
     wall = time.perf_counter() - t0
     return wall, np.array(lat)
+
+
+def prove_direct(na, job):
+    """One job through the drop-in entry points (nenya_prove_*_view: the
+    single-proof path with pipelined rounds and forked commitments), the
+    proof left in its params' pinned buffer."""
+    model, p, x = job
+    if model == "general":
+        return na.prove_general(x, p, view=True)
+    if model == "mimd":
+        return na.prove_mimd(x, p, view=True)
+    return na.prove_simd(x, p, view=True)
+
+
+def run_plain(na, jobs, arrivals=None):
+    """The plain path: the jobs proved one at a time, in order, by direct
+    calls (all available at t=0, or each from its arrival offset in s on);
+    returns (wall s, latencies ms from each job's arrival to its proof)."""
+    import numpy as np
+    t0 = time.perf_counter()
+    lat = []
+    for i, job in enumerate(jobs):
+        a = 0.0 if arrivals is None else arrivals[i]
+        while time.perf_counter() - t0 < a:
+            time.sleep(0.0002)
+        prove_direct(na, job)
+        lat.append((time.perf_counter() - t0 - a) * 1e3)
+    return time.perf_counter() - t0, np.array(lat)
 
 
 def verify_queue(na, jobs, workers):
@@ -203,30 +267,38 @@ def verify_queue(na, jobs, workers):
 
 def queue_stats(na, jobs, workers, seed):
     """Batch (all jobs at t=0) and serving (Poisson arrivals at 70% of the
-    batch throughput) figures for `workers` streams and for the plain
-    one-at-a-time path (one worker).  Each server first proves the whole job
-    list once, untimed, so its device pools and pinned proof buffers have
-    their steady-state sizes, as in a long-running server."""
+    batch throughput) figures for the server with `workers` streams and for
+    the plain path: the same jobs proved one at a time by direct calls to
+    the drop-in entry points.  Each path first proves the whole job list
+    once, untimed, so device pools and pinned proof buffers have their
+    steady-state sizes, as in a long-running process."""
     import numpy as np
+
+    def stats(wall, lat, batch):
+        d = {"p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2)}
+        if batch:
+            d = {"proofs_per_s": round(len(jobs) / wall, 2), **d}
+        return d
+
     out, serve = {}, {}
-    arr = None
-    for tag, wk in (("batched", workers), ("plain", 1)):
-        srv = na.Server(workers=wk)
-        try:
-            run_queue(srv, jobs)
-            wall, lat = run_queue(srv, jobs)
-            out[tag] = {"proofs_per_s": round(len(jobs) / wall, 2), "p50_ms": round(float(np.percentile(lat, 50)), 2),
-                        "p99_ms": round(float(np.percentile(lat, 99)), 2)}
-            log(f"  batch {tag:8s} workers={wk}: {out[tag]}")
-            if arr is None:
-                lam = 0.7 * out["batched"]["proofs_per_s"]
-                arr = np.cumsum(np.random.default_rng(seed).exponential(1.0 / lam, len(jobs)))
-                serve["lambda_per_s"] = round(lam, 2)
-            wall, lat = run_queue(srv, jobs, arrivals=arr)
-            serve[tag] = {"p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2)}
-            log(f"  serve {tag:8s} workers={wk} lambda={serve['lambda_per_s']:.1f}/s: {serve[tag]}")
-        finally:
-            srv.close()
+    srv = na.Server(workers=workers)
+    try:
+        run_queue(srv, jobs)
+        out["batched"] = stats(*run_queue(srv, jobs), True)
+        log(f"  batch batched  workers={workers}: {out['batched']}")
+        lam = 0.7 * out["batched"]["proofs_per_s"]
+        arr = np.cumsum(np.random.default_rng(seed).exponential(1.0 / lam, len(jobs)))
+        serve["lambda_per_s"] = round(lam, 2)
+        serve["batched"] = stats(*run_queue(srv, jobs, arrivals=arr), False)
+        log(f"  serve batched  workers={workers} lambda={lam:.1f}/s: {serve['batched']}")
+    finally:
+        srv.close()
+    run_plain(na, jobs)
+    out["plain"] = stats(*run_plain(na, jobs), True)
+    log(f"  batch plain (direct calls): {out['plain']}")
+    serve["plain"] = stats(*run_plain(na, jobs, arrivals=arr), False)
+    log(f"  serve plain (direct calls) lambda={lam:.1f}/s: {serve['plain']}")
+    out["ratio"] = round(out["batched"]["proofs_per_s"] / out["plain"]["proofs_per_s"], 3)
     return out, serve
 
 
@@ -251,7 +323,7 @@ def main():
     ap.add_argument("--log-n", type=int, default=20)
     ap.add_argument("--k", type=int, default=32)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-log-n", type=int, default=17)
+    ap.add_argument("--cpu-log-n", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=64, help="mixed batch size (0 skips batch/serve figures)")
     ap.add_argument("--workers", type=int, default=4)
@@ -266,7 +338,6 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
     import nenya_amd as na
     from nenya_amd.instances import general_instance
 
@@ -277,27 +348,27 @@ def main():
     t0 = time.perf_counter()
     params = na.GeneralSnarkParams.secure(log_n - 5, (A, B, C), 0xDEAD, 0xCAFE, k=k)
     log(f"setup (diagonal packing + PCS params) {time.perf_counter() - t0:.1f}s")
-    npdt = {8: np.uint8, 16: np.int16, 32: np.int32, 64: np.int64}[k]
-    wd = torch.from_numpy(w.astype({8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}[k]).view(npdt)).to("cuda:0")
-    torch.cuda.synchronize()
+    wd = na.DeviceArray(as_limbs(w, k))
+    na.device_synchronize()
     n = len(w)
 
     if args.copy_calib:
-        src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda:0")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
+        src = na.DeviceArray(np.zeros(256 << 20, np.uint8))
+        dst = na.DeviceArray(np.zeros(256 << 20, np.uint8))
+        na._lib.device_copy(dst, src)
+        na.device_synchronize()
+        del src, dst
         log("copy calibration: 268435456 bytes device-to-device")
 
     proofs = []
     for i in range(args.warmup):
-        proofs.append(na.prove_general_device(wd.data_ptr(), n, params))
+        proofs.append(na.prove_general_device(wd.ptr, n, params))
         log(f"warmup {i}: proof {len(proofs[-1])} bytes")
 
     # profiled proof: per-kernel HIP-event durations + stage breakdown
     params.set_profile(True)
     params.set_timing(True)
-    na.prove_general_device(wd.data_ptr(), n, params)
+    na.prove_general_device(wd.ptr, n, params)
     stats = params.kernel_stats()
     stages = params.last_timing()
     params.set_timing(False)
@@ -314,14 +385,14 @@ def main():
     # events on every launch of every proof would slow the proofs measured)
     params.set_profile(False)
     params.set_profile_only(dom_name)
-    torch.cuda.synchronize()
+    na.device_synchronize()
     t = time.perf_counter()
     for i in range(args.steps):
         if i == args.steps - 1:
             params.set_profile(True)
         # the proof lands in a pinned host buffer owned by the params (no copy)
-        pv = na.prove_general_device(wd.data_ptr(), n, params, view=True)
-    torch.cuda.synchronize()
+        pv = na.prove_general_device(wd.ptr, n, params, view=True)
+    na.device_synchronize()
     elapsed = time.perf_counter() - t
     dom = params.kernel_stats()[dom_name]
     params.set_profile(False)
@@ -331,6 +402,18 @@ def main():
     log(f"timed {args.steps} proofs: {ms:.2f} ms/proof")
     if proofs and p != proofs[0]:
         raise SystemExit("non-deterministic proof bytes")
+    verified = check_proof(p, golden(f"general_u{k}_logn{log_n}_seed{args.seed}"), na.verify_general(params, p))
+    log(f"headline proof: {verified}")
+
+    # the drop-in boundary: host witness in, owned proof bytes out
+    # (nenya_prove_general, as a Rust caller of prove_general would use it),
+    # and the same with the proof left as a view of the pinned buffer
+    wh = as_limbs(w, k)
+    dropin_ms = time_proofs(lambda: na.prove_general(wh, params, copy=False).close(), 5)
+    dropin_bytes_ms = time_proofs(lambda: na.prove_general(wh, params), 5)
+    dropin_view_ms = time_proofs(lambda: na.prove_general(wh, params, view=True), 5)
+    log(f"drop-in (host witness, owned proof): {dropin_ms:.2f} ms/proof; as Python bytes {dropin_bytes_ms:.2f} ms; "
+        f"host witness, view: {dropin_view_ms:.2f} ms")
 
     per_launch_ms = dom["ms"] / dom["count"]
     per_launch_bytes = dom["bytes"] / dom["count"]
@@ -362,8 +445,10 @@ def main():
         batch = {"jobs": len(jobs), "workers": args.workers, "proofs_per_s": q["batched"]["proofs_per_s"],
                  "verified": f"{nver}/{len(jobs)}",
                  "p50_ms": q["batched"]["p50_ms"], "p99_ms": q["batched"]["p99_ms"], "plain": q["plain"],
+                 "ratio_vs_plain": q["ratio"],
+                 "plain_path": "the same jobs one at a time through nenya_prove_*_view (direct calls)",
                  "mix": "logN~U{12..20}, k~U{8,16,32,64}, model~U{SIMD,MIMD,General}, host inputs, seed_i=base+i",
-                 "warm": "each server (batched and plain) proves the job list once, untimed, before its timed pass"}
+                 "warm": "each path (server and direct calls) proves the job list once, untimed, before its timed pass"}
         serve = {"lambda_per_s": srv["lambda_per_s"], "jobs": len(jobs), "workers": args.workers,
                  "p50_ms": srv["batched"]["p50_ms"], "p99_ms": srv["batched"]["p99_ms"], "plain": srv["plain"],
                  "arrivals": "Poisson at 70% of the batched throughput, same job mix"}
@@ -386,22 +471,33 @@ def main():
                                     "mix": "k=64, logN~U{10..22}, one circuit per size, witness per job"}}
         del mj
         del params, wd
-        torch.cuda.empty_cache()
         t0 = time.perf_counter()
         A2, B2, C2, w2 = general_instance(25, 32, args.seed)
         p2 = na.GeneralSnarkParams.secure(20, (A2, B2, C2), 0xDEAD, 0xCAFE, k=32)
         w2 = as_limbs(w2, 32)
         log(f"general u32 2^25: set up in {time.perf_counter() - t0:.1f}s")
-        ms25 = time_proofs(lambda: na.prove_general(w2, p2, view=True), 2)
+        # three proofs, each timed alone, copied out and verified after its timing
+        ts, checks = [], []
+        for _ in range(3):
+            t = time.perf_counter()
+            pv25 = na.prove_general(w2, p2, view=True)
+            ts.append((time.perf_counter() - t) * 1e3)
+            pf25 = bytes(pv25)
+            del pv25
+            checks.append(dict(check_proof(pf25, None, na.verify_general(p2, pf25)), digest=digest(pf25)))
+            del pf25
+        ms25 = sorted(ts[1:])[0] if len(ts) > 1 else ts[0]
         large["general_u32_2p25_ms"] = round(ms25, 1)
-        log(f"  general u32 2^25: {ms25:.1f} ms/proof (host witness)")
+        large["general_u32_2p25_ms_each"] = [round(x, 1) for x in ts]
+        large["general_u32_2p25_verified"] = checks
+        log(f"  general u32 2^25: {[round(x, 1) for x in ts]} ms/proof (host witness) {checks}")
         del p2, A2, B2, C2, w2
 
     cpu = None
     if not args.no_cpu_baseline:
         log(f"cpu baseline: oracle at 2^{args.cpu_log_n} ...")
         cpu = cpu_baseline(k, args.cpu_log_n, args.seed)
-        log(f"cpu baseline {cpu['value']:.1f} s (scaled)")
+        log(f"cpu baseline {cpu['value']:.1f} s on {cpu['cores']} threads")
 
     out = {
         "metric": "prover sec @2^20 u32 constraints; batched proofs/s + p99 vs plain GPU path",
@@ -417,9 +513,15 @@ def main():
         "dtype": f"u{k}",
         "data": "synthetic",
         "config": {"workload": f"General-model proof, Z/2^{k}, 2^{log_n} constraints (log_blocks {log_n - 5}), "
-                               "random R1CS seed 1, witness resident in HBM",
+                               f"random R1CS seed {args.seed}, witness resident in HBM, proof left as a view of "
+                               "the params' pinned host buffer",
                    "constraints": 2 ** log_n, "log_blocks": log_n - 5, "k": k, "proof_bytes": len(p),
-                   "proofs_per_s": round(1e3 / ms, 3)},
+                   "proofs_per_s": round(1e3 / ms, 3), "verified": verified,
+                   "dropin_ms": round(dropin_ms, 3), "dropin_bytes_ms": round(dropin_bytes_ms, 3),
+                   "dropin_view_ms": round(dropin_view_ms, 3),
+                   "dropin": "nenya_prove_general: host witness uploaded, owned proof returned and released with "
+                             "nenya_free; dropin_bytes_ms also copies it into a Python bytes object; "
+                             "dropin_view_ms leaves it as a view of the params' buffer"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "stages_ms": {k2: round(v, 3) for k2, v in stages.items()},

@@ -17,6 +17,10 @@
  *     (the reference asserts on shape misuse: simd.rs:224-231,
  *     reduce.rs:463-467).
  *   - Proof buffers are library-allocated; release them with nenya_free().
+ *     An owned proof is the pinned host buffer it was assembled in, handed
+ *     over without a copy; nenya_free keeps up to four released buffers for
+ *     later proofs, so a caller that frees each proof before the next one
+ *     allocates nothing in steady state.
  *   - A params object is immutable after creation and owns its device memory
  *     and HIP stream; calls on one params object are serialized internally.
  */
@@ -175,6 +179,11 @@ int nenya_wait(nenya_server* s, uint64_t job, const uint8_t** proof, size_t* pro
                double* run_ms);
 int nenya_job_release(nenya_server* s, uint64_t job);
 void nenya_server_free(nenya_server* s);
+
+/* Device-to-device copy of `bytes` bytes by the library's copy kernel
+ * (2048 workgroups at most), synchronized.  No reference counterpart: a
+ * transfer of known size for calibrating hardware traffic counters. */
+int nenya_device_copy(void* dst, const void* src, size_t bytes);
 
 void nenya_params_free(nenya_params* p);
 void nenya_free(void* ptr);

@@ -29,6 +29,7 @@ Every call runs the HIP library nenya_amd/libnenya_amd.so; there is no CPU
 fallback: without the library or a GPU the calls raise.
 """
 from ._lib import (  # noqa: F401
-    NenyaError, GeneralSnarkParams, MimdSnarkParams, Server, SimdR1csInstance, SimdSnarkParams, device_count,
-    lib_path, prove_general, prove_general_device, prove_mimd, prove_simd, verify_general, verify_mimd, verify_simd,
+    DeviceArray, NenyaError, GeneralSnarkParams, MimdSnarkParams, Server, SimdR1csInstance, SimdSnarkParams,
+    device_count, device_synchronize, lib_path, prove_general, prove_general_device, prove_mimd, prove_simd,
+    verify_general, verify_mimd, verify_simd,
 )

@@ -63,6 +63,7 @@ def _lib():
         L.nenya_free.argtypes = [vp]
         L.nenya_last_error.restype = C.c_char_p
         L.nenya_device_count.restype = C.c_int
+        L.nenya_device_copy.argtypes = [vp, vp, sz]
         _LIB = L
     return _LIB
 
@@ -112,6 +113,51 @@ def _take_proof(buf, n):
         return C.string_at(buf, n.value)
     finally:
         _lib().nenya_free(buf)
+
+
+class OwnedProof:
+    """An owned proof as the C ABI returns it: the pinned buffer the proof
+    was assembled in, handed over without a copy and released with
+    nenya_free by close() (or when the object is collected).  bytes(p) copies
+    it out; views from view() must not outlive close()."""
+
+    def __init__(self, buf, n):
+        self._buf, self._n = buf, n.value
+
+    def __len__(self):
+        return self._n
+
+    def __bytes__(self):
+        if self._buf is None:
+            raise ValueError("proof released")
+        return C.string_at(self._buf, self._n)
+
+    def view(self):
+        if self._buf is None:
+            raise ValueError("proof released")
+        return _view(self._buf, C.c_size_t(self._n), self)
+
+    def close(self):
+        if self._buf is not None and _LIB is not None:
+            _LIB.nenya_free(self._buf)
+        self._buf = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+
+def _result(out, n, params, view, copy):
+    if view:
+        return _view(out, n, params)
+    if not copy:
+        return OwnedProof(out, n)
+    return _take_proof(out, n)
 
 
 class _Params:
@@ -174,8 +220,12 @@ class SimdR1csInstance:
     witnesses: np.ndarray  # shape (32, m)
 
 
-def _view(out, n):
-    return memoryview((C.c_ubyte * n.value).from_address(out.value)).toreadonly()
+def _view(out, n, params):
+    """Read-only bytes view of a proof in the params' pinned buffer; the view
+    keeps `params` (the buffer's owner) alive."""
+    arr = (C.c_ubyte * n.value).from_address(out.value)
+    arr._owner = params
+    return memoryview(arr).cast("B").toreadonly()
 
 
 def prove_simd(inst, params, view=False):
@@ -193,7 +243,7 @@ def prove_simd(inst, params, view=False):
         args += _coo_args(m)
     if view:
         _check(_lib().nenya_prove_simd_view(*args, wit.ctypes.data, C.byref(out), C.byref(n)))
-        return _view(out, n)
+        return _view(out, n, params)
     _check(_lib().nenya_prove_simd(*args, wit.ctypes.data, C.byref(out), C.byref(n)))
     return _take_proof(out, n)
 
@@ -243,7 +293,7 @@ def prove_mimd(witnesses, params, view=False):
     out, n = C.c_void_p(), C.c_size_t()
     if view:
         _check(_lib().nenya_prove_mimd_view(params._h, wit.ctypes.data, C.byref(out), C.byref(n)))
-        return _view(out, n)
+        return _view(out, n, params)
     _check(_lib().nenya_prove_mimd(params._h, wit.ctypes.data, C.byref(out), C.byref(n)))
     return _take_proof(out, n)
 
@@ -265,30 +315,91 @@ class GeneralSnarkParams(_Params):
         return p
 
 
-def prove_general(w, params, view=False):
+def prove_general(w, params, view=False, copy=True):
     """prove_general(w, &GeneralSnarkParams) -> proof bytes (view=True: a
-    read-only view of the params' pinned proof buffer, as prove_simd)."""
+    read-only view of the params' pinned proof buffer, as prove_simd;
+    copy=False: the owned buffer as the C ABI returns it, an OwnedProof)."""
     w = _limbs(w, params.k)
     out, n = C.c_void_p(), C.c_size_t()
     if view:
         _check(_lib().nenya_prove_general_view(params._h, w.ctypes.data, len(w), C.byref(out), C.byref(n)))
-        return _view(out, n)
-    _check(_lib().nenya_prove_general(params._h, w.ctypes.data, len(w), C.byref(out), C.byref(n)))
-    return _take_proof(out, n)
+    else:
+        _check(_lib().nenya_prove_general(params._h, w.ctypes.data, len(w), C.byref(out), C.byref(n)))
+    return _result(out, n, params, view, copy)
 
 
 def prove_general_device(dev_ptr, n_limbs, params, view=False):
-    """prove_general with the witness already in device memory (e.g. a torch
-    tensor's data_ptr()).  view=True returns a read-only memoryview of the
+    """prove_general with the witness already in device memory (e.g. a
+    DeviceArray's ptr).  view=True returns a read-only memoryview of the
     proof in the params' pinned buffer (no copy; valid until the next proof
     with these params)."""
     out, n = C.c_void_p(), C.c_size_t()
     if view:
         _check(_lib().nenya_prove_general_device_view(params._h, C.c_void_p(dev_ptr), n_limbs, C.byref(out),
                                                       C.byref(n)))
-        return memoryview((C.c_ubyte * n.value).from_address(out.value)).toreadonly()
+        return _view(out, n, params)
     _check(_lib().nenya_prove_general_device(params._h, C.c_void_p(dev_ptr), n_limbs, C.byref(out), C.byref(n)))
     return _take_proof(out, n)
+
+
+_HIP = None
+
+
+def _hip():
+    """The HIP runtime the library itself runs on (libamdhip64.so.7, already
+    loaded with libnenya_amd.so): device buffers for callers that hand the
+    prover a witness in device memory."""
+    global _HIP
+    if _HIP is None:
+        _lib()
+        H = C.CDLL("libamdhip64.so.7")
+        H.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        H.hipFree.argtypes = [C.c_void_p]
+        H.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        H.hipDeviceSynchronize.argtypes = []
+        _HIP = H
+    return _HIP
+
+
+def device_synchronize():
+    if _hip().hipDeviceSynchronize() != 0:
+        raise NenyaError(3, "hipDeviceSynchronize failed")
+
+
+def device_copy(dst, src):
+    """dst <- src (DeviceArrays of equal size) with the library's copy kernel."""
+    if dst.nbytes != src.nbytes:
+        raise NenyaError(1, "device_copy: sizes differ")
+    _check(_lib().nenya_device_copy(C.c_void_p(dst.ptr), C.c_void_p(src.ptr), src.nbytes))
+
+
+class DeviceArray:
+    """A copy of a host array in device memory (hipMalloc'd, freed with the
+    object); `.ptr` is the device address, `.size` the element count."""
+
+    def __init__(self, host):
+        a = np.ascontiguousarray(host)
+        self.size, self.nbytes, self.dtype = a.size, a.nbytes, a.dtype
+        p = C.c_void_p()
+        if _hip().hipMalloc(C.byref(p), max(a.nbytes, 1)) != 0:
+            raise NenyaError(2, f"hipMalloc of {a.nbytes} bytes failed")
+        self._p = p
+        if a.nbytes and _hip().hipMemcpy(p, a.ctypes.data, a.nbytes, 1) != 0:  # hipMemcpyHostToDevice
+            self.free()
+            raise NenyaError(3, "hipMemcpy to the device failed")
+
+    @property
+    def ptr(self):
+        return self._p.value
+
+    def free(self):
+        p = getattr(self, "_p", None)
+        if p is not None and p.value and _HIP is not None:
+            _HIP.hipFree(p)
+        self._p = None
+
+    def __del__(self):
+        self.free()
 
 
 VERIFY_ERRORS = {0: "Ok", 1: "MultSumcheck", 2: "LinSumcheck", 3: "LinClaimMismatch", 4: "LinARowMismatch",

@@ -1,6 +1,8 @@
 // extern "C" boundary of the MI355X Nenya prover (include/nenya.h).
+#include <algorithm>
 #include <cstdlib>
 #include <mutex>
+#include <unordered_map>
 
 #include "../../include/nenya.h"
 #include "server.hpp"
@@ -41,13 +43,64 @@ static void need_gpu() {
 static void need_width(int k) {
   if (k != 8 && k != 16 && k != 32 && k != 64) throw Error(NENYA_E_ARG, "k must be 8, 16, 32 or 64");
 }
-static uint8_t* emit(const ProofView& v, uint8_t** proof, size_t* len) {
-  uint8_t* b = (uint8_t*)std::malloc(v.second ? v.second : 1);
-  if (!b) throw std::bad_alloc();
-  std::memcpy(b, v.first, v.second);
-  *proof = b;
+// Owned proofs: the proof is assembled straight in a pinned buffer that is
+// then handed to the caller (no copy); nenya_free takes it back into a small
+// pool of spare buffers, so a caller that frees each proof before the next
+// one allocates nothing in steady state.
+struct Handout {
+  std::mutex mu;
+  std::unordered_map<const void*, Pinned> out;  // buffers the callers hold
+  std::vector<Pinned> spare;                    // returned ones, largest last
+  static constexpr size_t SPARE_MAX = 4;
+  static Handout& get() {
+    static Handout* h = new Handout;  // never destroyed: callers may free at exit
+    return *h;
+  }
+  Pinned take() {
+    std::lock_guard<std::mutex> g(mu);
+    if (spare.empty()) return Pinned();
+    Pinned b = std::move(spare.back());
+    spare.pop_back();
+    return b;
+  }
+  uint8_t* give(Pinned&& b) {
+    std::lock_guard<std::mutex> g(mu);
+    uint8_t* p = b.p;
+    out.emplace(p, std::move(b));
+    return p;
+  }
+  // true when p was one of ours
+  bool back(const void* p) {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = out.find(p);
+    if (it == out.end()) return false;
+    spare.push_back(std::move(it->second));
+    out.erase(it);
+    std::sort(spare.begin(), spare.end(), [](const Pinned& a, const Pinned& b) { return a.cap < b.cap; });
+    if (spare.size() > SPARE_MAX) spare.erase(spare.begin());  // drops (frees) the smallest
+    return true;
+  }
+};
+
+// Runs `prove` (which returns a view into the context's proof buffer) with
+// a handout buffer swapped in as that buffer, then gives the buffer to the
+// caller as an owned proof.
+template <class F> static void prove_owned(CtxBase& cx, F&& prove, uint8_t** proof, size_t* len) {
+  Pinned b = Handout::get().take();
+  std::swap(cx.proofbuf, b);
+  struct Restore {
+    CtxBase& cx;
+    Pinned& b;
+    ~Restore() { std::swap(cx.proofbuf, b); }
+  };
+  ProofView v;
+  {
+    Restore r{cx, b};
+    v = prove();
+  }
+  if (v.first != b.p) throw Error(NENYA_E_DEVICE, "proof not in its buffer");
   *len = v.second;
-  return b;
+  *proof = Handout::get().give(std::move(b));
 }
 
 template <class T>
@@ -76,18 +129,20 @@ static int general_common(const nenya_params* p, const void* w, size_t n, uint8_
     if (!p || !p->impl || p->impl->kind != 1 || !w || !len || (!proof && !view)) throw Error(NENYA_E_ARG, "bad argument");
     auto* np = const_cast<nenya_params*>(p);
     std::lock_guard<std::mutex> g(np->mu);
-    ProofView v;
-    switch (p->impl->k) {
-      case 8: v = general_t<W<8>>(p->impl, w, n, dev); break;
-      case 16: v = general_t<W<16>>(p->impl, w, n, dev); break;
-      case 32: v = general_t<W<32>>(p->impl, w, n, dev); break;
-      default: v = general_t<W<64>>(p->impl, w, n, dev); break;
-    }
+    auto run = [&]() -> ProofView {
+      switch (p->impl->k) {
+        case 8: return general_t<W<8>>(p->impl, w, n, dev);
+        case 16: return general_t<W<16>>(p->impl, w, n, dev);
+        case 32: return general_t<W<32>>(p->impl, w, n, dev);
+        default: return general_t<W<64>>(p->impl, w, n, dev);
+      }
+    };
     if (view) {
+      ProofView v = run();
       *view = v.first;
       *len = v.second;
     } else {
-      emit(v, proof, len);
+      prove_owned(p->impl->ctx0(), run, proof, len);
     }
     return NENYA_OK;
   });
@@ -116,7 +171,23 @@ int nenya_device_count(void) {
 
 const char* nenya_last_error(void) { return g_err.c_str(); }
 
-void nenya_free(void* p) { std::free(p); }
+int nenya_device_copy(void* dst, const void* src, size_t bytes) {
+  return guard([&] {
+    if ((!dst || !src) && bytes) throw Error(NENYA_E_ARG, "bad argument");
+    need_gpu();
+    if (!bytes) return NENYA_OK;
+    const unsigned grid = (unsigned)std::min<size_t>(2048, (bytes + 4095) / 4096);
+    hipLaunchKernelGGL(k_copy_out, dim3(grid), dim3(256), 0, nullptr, (const uint8_t*)src, (uint8_t*)dst,
+                       (uint64_t)bytes);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipDeviceSynchronize());
+    return NENYA_OK;
+  });
+}
+
+void nenya_free(void* p) {
+  if (p && !Handout::get().back(p)) std::free(p);
+}
 
 void nenya_params_free(nenya_params* p) {
   if (!p) return;
@@ -168,18 +239,20 @@ static int simd_common(const nenya_params* p, const uint32_t* ar, const uint32_t
     if ((an && (!ar || !ac || !av)) || (bn && (!br || !bc || !bv)) || (cn && (!cr || !cc || !cv)))
       throw Error(NENYA_E_ARG, "null matrix array");
     HostCoo m3[3] = {{n, m, an, ar, ac, av}, {n, m, bn, br, bc, bv}, {n, m, cn, cr, cc, cv}};
-    ProofView v;
-    switch (b->k) {
-      case 8: v = simd_prove_t<W<8>>(b, m3, witnesses); break;
-      case 16: v = simd_prove_t<W<16>>(b, m3, witnesses); break;
-      case 32: v = simd_prove_t<W<32>>(b, m3, witnesses); break;
-      default: v = simd_prove_t<W<64>>(b, m3, witnesses); break;
-    }
+    auto run = [&]() -> ProofView {
+      switch (b->k) {
+        case 8: return simd_prove_t<W<8>>(b, m3, witnesses);
+        case 16: return simd_prove_t<W<16>>(b, m3, witnesses);
+        case 32: return simd_prove_t<W<32>>(b, m3, witnesses);
+        default: return simd_prove_t<W<64>>(b, m3, witnesses);
+      }
+    };
     if (view) {
+      ProofView v = run();
       *view = v.first;
       *proof_len = v.second;
     } else {
-      emit(v, proof, proof_len);
+      prove_owned(b->ctx0(), run, proof, proof_len);
     }
     return NENYA_OK;
   });
@@ -242,18 +315,20 @@ static int mimd_common(const nenya_params* p, const void* witnesses, uint8_t** p
       throw Error(NENYA_E_ARG, "bad argument");
     auto* np = const_cast<nenya_params*>(p);
     std::lock_guard<std::mutex> g(np->mu);
-    ProofView v;
-    switch (p->impl->k) {
-      case 8: v = mimd_t<W<8>>(p->impl, witnesses); break;
-      case 16: v = mimd_t<W<16>>(p->impl, witnesses); break;
-      case 32: v = mimd_t<W<32>>(p->impl, witnesses); break;
-      default: v = mimd_t<W<64>>(p->impl, witnesses); break;
-    }
+    auto run = [&]() -> ProofView {
+      switch (p->impl->k) {
+        case 8: return mimd_t<W<8>>(p->impl, witnesses);
+        case 16: return mimd_t<W<16>>(p->impl, witnesses);
+        case 32: return mimd_t<W<32>>(p->impl, witnesses);
+        default: return mimd_t<W<64>>(p->impl, witnesses);
+      }
+    };
     if (view) {
+      ProofView v = run();
       *view = v.first;
       *len = v.second;
     } else {
-      emit(v, proof, len);
+      prove_owned(p->impl->ctx0(), run, proof, len);
     }
     return NENYA_OK;
   });

@@ -134,17 +134,24 @@ def set_arrays(ms):
     return off, rows, cols, vals
 
 
-def png_instance(log_bytes):
+def png_instance(log_bytes, image="example"):
     """The PNG-filter batch of png_filter.rs (snark-ring/examples/png_filter.rs):
     32 grayscale images of b = 2^(log_bytes-5) bytes, W = min(256, b) columns,
     row r filtered with Sub (left neighbour) when r is even and Up (the row
-    above) when odd (lines 84-103); raw bytes from the example's own SplitMix64
-    byte generator (lines 67-73, seed (slot << 40) ^ i * 0x10000001).  The
-    ring-u8 circuit (build_ring, lines 126-163): row i says
-    Orig[i] * 1 = Filt[i] + Orig[pred(i)] over the witness
-    w = (1, Orig[0], Filt[0], Orig[1], Filt[1], ...), n = b rows and
+    above) when odd (lines 84-103).  The ring-u8 circuit (build_ring, lines
+    126-163): row i says Orig[i] * 1 = Filt[i] + Orig[pred(i)] over the
+    witness w = (1, Orig[0], Filt[0], Orig[1], Filt[1], ...), n = b rows and
     m = 2^ceil_log2(1 + 2b) columns (lines 270-273).  Returns
-    (log_n, log_m, A, B, C, wit[32, m]) with k = 8."""
+    (log_n, log_m, A, B, C, wit[32, m]) with k = 8.
+
+    image="example": raw bytes from the example's own SplitMix64 byte
+    generator (lines 67-73, seed (slot << 40) ^ i * 0x10000001).
+    image="gradient": each row a linear gradient (start byte and slope in
+    [-4, 4] per row and image) plus uniform noise in [-16, 16], clamped to
+    [0, 255], drawn from SplitMix64 streams of the image index.  The circuit
+    does not depend on the bytes, only the witness does."""
+    if image not in ("example", "gradient"):
+        raise ValueError("image must be 'example' or 'gradient'")
     ls = log_bytes - 5
     b = 1 << ls
     log_m = (2 * b).bit_length()  # ceil_log2(1 + 2b) for b a power of two
@@ -161,10 +168,17 @@ def png_instance(log_bytes):
          np.concatenate([2 + 2 * i, 1 + 2 * pred[has]]).astype(np.uint32),
          np.ones(b + int(has.sum()), np.uint64))
     wit = np.zeros((32, m), np.uint64)
+    nrows = b // w_cols
     with np.errstate(over="ignore"):
         for s in range(32):
-            seed = (np.uint64(s) << np.uint64(40)) ^ (i.astype(np.uint64) * np.uint64(0x10000001))
-            orig = _mix(seed + _G) & np.uint64(0xFF)
+            if image == "example":
+                seed = (np.uint64(s) << np.uint64(40)) ^ (i.astype(np.uint64) * np.uint64(0x10000001))
+                orig = _mix(seed + _G) & np.uint64(0xFF)
+            else:
+                start = (stream(s, 400, nrows) % np.uint64(256)).astype(np.int64)
+                slope = (stream(s, 401, nrows) % np.uint64(9)).astype(np.int64) - 4
+                noise = (stream(s, 402, b) % np.uint64(33)).astype(np.int64) - 16
+                orig = np.clip(start[r] + slope[r] * c + noise, 0, 255).astype(np.uint64)
             pv = np.where(has, orig[np.maximum(pred, 0)], np.uint64(0))
             filt = (orig - pv) & np.uint64(0xFF)
             wit[s, 0] = 1

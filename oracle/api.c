@@ -9,6 +9,14 @@
 #define API __attribute__((visibility("default")))
 
 API void orc_free(void *p) { free(p); }
+/* threads the prover loops run on (OpenMP; OMP_NUM_THREADS sets it) */
+API int orc_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 API void orc_set_width(int k) { or_set_width(k); }
 
 /* ---- ring primitives ---- */

@@ -272,4 +272,17 @@ int mimd_verify(int log_n, int log_m, const mmats_t *mm, u64 seed_w, u64 seed_e,
 /* ---------------- misc ---------------- */
 void *xmalloc(size_t n);
 void *xcalloc(size_t n, size_t s);
+
+/* ---------------- threads ----------------
+ * The prover loops run on OpenMP threads (the CPU-baseline timing runs the
+ * reference algorithm over all host cores).  Every sum is exact arithmetic
+ * mod 2^k, so partial sums combined in any order give the same bytes as one
+ * thread; scatters are split so that no two threads write one output. */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#pragma omp declare reduction(gr_plus : gr : gr_add(&omp_out, &omp_out, &omp_in)) initializer(gr_zero(&omp_priv))
+/* starts of the runs of equal keys in key[0..n): writes them to *out
+ * (allocated, n_runs + 1 entries, the last = n) and returns n_runs */
+size_t key_runs(const u32 *key, size_t n, size_t **out);
 #endif

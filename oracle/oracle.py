@@ -84,6 +84,11 @@ class Oracle:
         L.orc_mimd_verify.argtypes = [C.c_int, C.c_int, C.c_int] + mset * 3 + [
             C.c_uint64, C.c_uint64, C.c_char_p, _sz]
         L.orc_free.argtypes = [C.c_void_p]
+        L.orc_threads.restype = C.c_int
+
+    def threads(self):
+        """OpenMP threads the prover loops run on."""
+        return int(self.L.orc_threads())
 
     # ---- ring ----
     def gr_mul(self, k, a, b):

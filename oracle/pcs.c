@@ -71,6 +71,7 @@ void merkle_build(merkle_t *m, u8 (*leaves)[32], size_t nl) {
   size_t start = 0, len = n;
   while (len > 1) {
     size_t next = start + len;
+#pragma omp parallel for if (len >= 512)
     for (size_t i = 0; i < len / 2; i++) {
       u8 pair[64];
       memcpy(pair, m->nodes[start + 2 * i], 32);
@@ -169,19 +170,23 @@ void r2pcs_setup(pcs_t *p, size_t nv, size_t q, double delta, size_t lambda, u64
 static void hash_columns(const pcs_t *p, comm_t *c) {
   size_t n = p->raa.n, w = p->packed ? 32 : E;
   u8 (*leaves)[32] = xmalloc(n * 32);
-  u8 *tmp = xmalloc(c->rows * w * 8);
-  for (size_t j = 0; j < n; j++) {
-    size_t o = 0;
-    for (size_t i = 0; i < c->rows; i++) {
-      const u64 *ent = &c->encoded[(i * n + j) * w];
-      for (size_t s = 0; s < w; s++) for (int b = 0; b < g_nb; b++) tmp[o++] = (u8)(ent[s] >> (8 * b));
+#pragma omp parallel
+  {
+    u8 *tmp = xmalloc(c->rows * w * 8);
+#pragma omp for
+    for (size_t j = 0; j < n; j++) {
+      size_t o = 0;
+      for (size_t i = 0; i < c->rows; i++) {
+        const u64 *ent = &c->encoded[(i * n + j) * w];
+        for (size_t s = 0; s < w; s++) for (int b = 0; b < g_nb; b++) tmp[o++] = (u8)(ent[s] >> (8 * b));
+      }
+      b3_hash(tmp, o, leaves[j]);
     }
-    b3_hash(tmp, o, leaves[j]);
+    free(tmp);
   }
   merkle_build(&c->tree, leaves, n);
   memcpy(c->root, c->tree.nodes[2 * c->tree.n - 2], 32);
   free(leaves);
-  free(tmp);
 }
 void pcs_commit_stacked(const pcs_t *p, const u64 *const *msgs, size_t s, comm_t *c) {
   size_t n = p->raa.n, k1 = p->k1, k2 = p->k2, rows = s * k1;
@@ -190,12 +195,14 @@ void pcs_commit_stacked(const pcs_t *p, const u64 *const *msgs, size_t s, comm_t
   c->encoded = xmalloc(rows * n * 32 * sizeof(u64));
   c->slots = xmalloc(rows * k2 * 32 * sizeof(u64));
   c->message = xmalloc(rows * k2 * sizeof(gr));
+#pragma omp parallel for
   for (size_t i = 0; i < rows; i++) {
     const u64 *row = &msgs[i / k1][(i % k1) * k2 * 32];
     memcpy(&c->slots[i * k2 * 32], row, k2 * 32 * sizeof(u64));
     raa_encode_packed(&p->raa, row, &c->encoded[i * n * 32]);
   }
   hash_columns(p, c);
+#pragma omp parallel for
   for (size_t i = 0; i < rows * k2; i++) pack_r2(&c->message[i], &c->slots[i * 32]);
 }
 void pcs_commit(const pcs_t *p, const u64 *slots, comm_t *c) { pcs_commit_stacked(p, &slots, 1, c); }
@@ -205,6 +212,7 @@ void r2pcs_commit_stacked(const pcs_t *p, const gr *const *msgs, size_t s, comm_
   c->rows = rows; c->s = s;
   c->encoded = xmalloc(rows * n * E * sizeof(u64));
   c->message = xmalloc(rows * k2 * sizeof(gr));
+#pragma omp parallel for
   for (size_t i = 0; i < rows; i++) {
     const gr *row = &msgs[i / k1][(i % k1) * k2];
     memcpy(&c->message[i * k2], row, k2 * sizeof(gr));
@@ -264,6 +272,7 @@ static void open_columns(const pcs_t *p, const comm_t *c, open_t *o) {
   o->cols = xmalloc(o->t * c->rows * w * sizeof(u64));
   o->plen = (size_t)__builtin_ctzll(c->tree.n);
   o->paths = xmalloc((o->t * o->plen + 1) * 32);
+#pragma omp parallel for
   for (size_t q = 0; q < o->t; q++) {
     size_t j = o->idx[q];
     for (size_t i = 0; i < c->rows; i++)
@@ -282,9 +291,10 @@ void pcs_open_weighted(const pcs_t *p, const comm_t *c, const gr *z, const gr *w
   o->nmbar = 1; o->k2 = k2;
   o->mbar = xmalloc(k2 * sizeof(gr));
   gr t;
+#pragma omp parallel for
   for (size_t j = 0; j < k2; j++) {
-    gr acc; gr_zero(&acc);
-    for (size_t i = 0; i < rows; i++) { gr_mul(&t, &e0s[i], &c->message[i * k2 + j]); gr_add(&acc, &acc, &t); }
+    gr acc, tj; gr_zero(&acc);
+    for (size_t i = 0; i < rows; i++) { gr_mul(&tj, &e0s[i], &c->message[i * k2 + j]); gr_add(&acc, &acc, &tj); }
     o->mbar[j] = acc;
   }
   gr_zero(y);
@@ -303,20 +313,22 @@ void pcs_rotated_open_batch(const pcs_t *p, const comm_t *c, const gr *z, size_t
   memset(o, 0, sizeof *o);
   o->nmbar = nl; o->k2 = k2;
   o->mbar = xmalloc(nl * k2 * sizeof(gr));
-  gr t, pk;
-  for (size_t l = 0; l < nl; l++) {
+  gr t;
+#pragma omp parallel for collapse(2)
+  for (size_t l = 0; l < nl; l++)
     for (size_t j = 0; j < k2; j++) {
-      gr acc; gr_zero(&acc);
+      gr acc, pk, tj; gr_zero(&acc);
       for (size_t i = 0; i < k1; i++) {
         const u64 *sl = &c->slots[(i * k2 + j) * 32];
         u64 rot[32];
         for (int s = 0; s < 32; s++) rot[s] = sl[(s + l) % 32];
         pack_r2(&pk, rot);
-        gr_mul(&t, &e0[i], &pk);
-        gr_add(&acc, &acc, &t);
+        gr_mul(&tj, &e0[i], &pk);
+        gr_add(&acc, &acc, &tj);
       }
       o->mbar[l * k2 + j] = acc;
     }
+  for (size_t l = 0; l < nl; l++) {
     gr_zero(&ys[l]);
     for (size_t j = 0; j < k2; j++) { gr_mul(&t, &o->mbar[l * k2 + j], &e1[j]); gr_add(&ys[l], &ys[l], &t); }
   }

@@ -19,6 +19,7 @@ void build_eq_table(const gr *r, size_t n, gr *out) {
   size_t len = 1;
   gr *tmp = xmalloc(((size_t)1 << n) * sizeof(gr));
   for (size_t l = 0; l < n; l++) {
+#pragma omp parallel for if (len >= 256)
     for (size_t k = 0; k < len; k++) {
       gr hi; gr_mul(&hi, &out[k], &r[l]);
       gr_sub(&tmp[2 * k], &out[k], &hi);
@@ -32,6 +33,7 @@ void build_eq_table(const gr *r, size_t n, gr *out) {
 
 void mle_fold(gr *t, size_t len, const gr *r) {
   size_t m = len / 2;
+#pragma omp parallel for if (m >= 256)
   for (size_t i = 0; i < m; i++) {
     gr d, p;
     gr_sub(&d, &t[i + m], &t[i]);
@@ -51,6 +53,7 @@ static void eq_one(gr *o, const gr *t, const gr *r) { /* t*r + (1-t)(1-r) */
 }
 static void marginalize_and_scale(gr *eg, size_t len, const gr *scale) {
   size_t m = len / 2;
+#pragma omp parallel for if (m >= 256)
   for (size_t i = 0; i < m; i++) {
     gr s; gr_add(&s, &eg[i], &eg[i + m]);
     gr_mul(&eg[i], scale, &s);
@@ -74,10 +77,11 @@ void prove_mult_check(const gr *a0, const gr *b0, size_t nv, const gr *rt, const
   size_t len = N;
   for (size_t r = 0; r < nv; r++) {
     size_t m = len / 2;
-    gr s0, s1, si, t, u;
+    gr s0, s1, si;
     gr_zero(&s0); gr_zero(&s1); gr_zero(&si);
+#pragma omp parallel for if (m >= 64) reduction(gr_plus : s0, s1, si)
     for (size_t i = 0; i < m; i++) {
-      gr da, db;
+      gr da, db, t, u;
       gr_sub(&da, &a[i + m], &a[i]);
       gr_sub(&db, &b[i + m], &b[i]);
       gr_mul(&t, &a[i], &b[i]); gr_mul(&u, &eg[i], &t); gr_add(&s0, &s0, &u);
@@ -146,10 +150,11 @@ void prove_mult_check_sub(const gr *a0, const gr *b0, const gr *c0, const gr *e0
   size_t len = N;
   for (size_t r = 0; r < nv; r++) {
     size_t m = len / 2;
-    gr s0, s1, si, t, u;
+    gr s0, s1, si;
     gr_zero(&s0); gr_zero(&s1); gr_zero(&si);
+#pragma omp parallel for if (m >= 64) reduction(gr_plus : s0, s1, si)
     for (size_t i = 0; i < m; i++) {
-      gr da, db;
+      gr da, db, t, u;
       gr_sub(&da, &a[i + m], &a[i]);
       gr_sub(&db, &b[i + m], &b[i]);
       gr_mul(&t, &a[i], &b[i]); gr_sub(&t, &t, &c[i]); gr_sub(&t, &t, &e[i]); gr_mul(&u, &eg[i], &t); gr_add(&s0, &s0, &u);
@@ -197,8 +202,9 @@ void prove_linear_simd(gr *f, gr *g, size_t nv, const gr *claim, ch_t *ch, sc_t 
     size_t m = len / 2;
     gr s0, si;
     gr_zero(&s0); gr_zero(&si);
+#pragma omp parallel for if (m >= 64) reduction(gr_plus : s0, si)
     for (size_t i = 0; i < m; i++) {
-      gr df, dg;
+      gr df, dg, t;
       gr_mul(&t, &f[i], &g[i]); gr_add(&s0, &s0, &t);
       gr_sub(&df, &f[i + m], &f[i]); gr_sub(&dg, &g[i + m], &g[i]);
       gr_mul(&t, &df, &dg); gr_add(&si, &si, &t);
@@ -250,9 +256,10 @@ void prove_linear_general(gr *const *rows, gr *const *rhos, size_t nv, const gr 
     size_t m = len / 2;
     gr s0, si;
     gr_zero(&s0); gr_zero(&si);
+#pragma omp parallel for collapse(2) if (m >= 4) reduction(gr_plus : s0, si)
     for (int l = 0; l < 32; l++)
       for (size_t i = 0; i < m; i++) {
-        gr dr, dp;
+        gr dr, dp, t;
         gr_mul(&t, &rows[l][i], &rhos[l][i]); gr_add(&s0, &s0, &t);
         gr_sub(&dr, &rows[l][i + m], &rows[l][i]); gr_sub(&dp, &rhos[l][i + m], &rhos[l][i]);
         gr_mul(&t, &dr, &dp); gr_add(&si, &si, &t);
@@ -264,8 +271,8 @@ void prove_linear_general(gr *const *rows, gr *const *rhos, size_t nv, const gr 
     ch_observe(ch, q, 3);
     ch_sample(ch, &chal[r]);
     gr_mul(&t, &chal[r], &q[2]); gr_add(&t, &q[1], &t); gr_mul(&t, &chal[r], &t); gr_add(&c, &q[0], &t);
-    for (int l = 0; l < 32; l++) mle_fold(rows[l], len, &chal[r]);
-    for (int l = 0; l < 32; l++) mle_fold(rhos[l], len, &chal[r]);
+#pragma omp parallel for if (m >= 4)
+    for (int l = 0; l < 64; l++) mle_fold(l < 32 ? rows[l] : rhos[l - 32], len, &chal[r]);
     len = m;
   }
   pf->nfin = 32;

@@ -33,6 +33,14 @@ void *xcalloc(size_t n, size_t s) {
   if (!p) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
   return p;
 }
+size_t key_runs(const u32 *key, size_t n, size_t **out) {
+  size_t *st = xmalloc((n + 1) * sizeof(size_t)), nr = 0;
+  for (size_t i = 0; i < n; i++)
+    if (i == 0 || key[i] != key[i - 1]) st[nr++] = i;
+  st[nr] = n;
+  *out = st;
+  return nr;
+}
 
 /* limb.rs:91-101 -- 6 Newton doublings */
 u64 inv_2adic(u64 a) {
@@ -287,12 +295,19 @@ static void consts_compute(consts_t *C) {
 
 const consts_t *consts(void) {
   int w = g_k == 8 ? 0 : g_k == 16 ? 1 : g_k == 32 ? 2 : 3;
-  if (!g_consts[w]) {
-    consts_t *C = xcalloc(1, sizeof(consts_t));
-    consts_compute(C);
-    g_consts[w] = C;
+  consts_t *c = __atomic_load_n(&g_consts[w], __ATOMIC_ACQUIRE);
+  if (c) return c;
+  /* first use of a width, possibly from several threads: compute once */
+#pragma omp critical(oracle_consts)
+  {
+    c = __atomic_load_n(&g_consts[w], __ATOMIC_ACQUIRE);
+    if (!c) {
+      c = xcalloc(1, sizeof(consts_t));
+      consts_compute(c);
+      __atomic_store_n(&g_consts[w], c, __ATOMIC_RELEASE);
+    }
   }
-  return g_consts[w];
+  return c;
 }
 
 /* embedding.rs:30-65: phi_2 on the flat 32-vector, 16x2 view m[i][j]=a[2i+j] */

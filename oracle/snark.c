@@ -47,9 +47,11 @@ static gr *accumulate_row(const coo_t *m, const gr *eq, size_t cols) { /* simd.r
   return o;
 }
 static void inner(const gr *a, const gr *b, size_t n, gr *o) {
-  gr t;
-  gr_zero(o);
-  for (size_t i = 0; i < n; i++) { gr_mul(&t, &a[i], &b[i]); gr_add(o, o, &t); }
+  gr s;
+  gr_zero(&s);
+#pragma omp parallel for if (n >= 256) reduction(gr_plus : s)
+  for (size_t i = 0; i < n; i++) { gr t; gr_mul(&t, &a[i], &b[i]); gr_add(&s, &s, &t); }
+  *o = s;
 }
 
 static void ser_mult(buf_t *b, const sc_t *s, int with_claim) {
@@ -81,13 +83,16 @@ int simd_prove(int log_n, int log_m, const coo_t *A, const coo_t *B, const coo_t
   /* 1. simd_to_packed */
   gr *w_phi = xmalloc(m * sizeof(gr));
   u64 *w_slots = xmalloc(m * 32 * sizeof(u64));
+#pragma omp parallel for
   for (size_t j = 0; j < m; j++) {
     for (int k = 0; k < 32; k++) w_slots[j * 32 + k] = wit[(size_t)k * m + j];
     pack_r2(&w_phi[j], &w_slots[j * 32]);
   }
   u64 *aw0[32], *bw0[32];
+#pragma omp parallel for
   for (int k = 0; k < 32; k++) { aw0[k] = matvec_r0(A, wit + (size_t)k * m); bw0[k] = matvec_r0(B, wit + (size_t)k * m); }
   gr *e = xmalloc(n * sizeof(gr));
+#pragma omp parallel for
   for (size_t i = 0; i < n; i++) {
     u64 x[32], y[32];
     for (int k = 0; k < 32; k++) { x[k] = aw0[k][i]; y[k] = bw0[k][i]; }
@@ -382,18 +387,24 @@ static void general_params(int log_b, u64 seed_w, u64 seed_e, pcs_t *pw, pcs_t *
 }
 static gr *layered_matvec(const gdiag_t *g, int w, gr *const *rho, size_t nb) { /* general.rs:211-230 */
   gr *o = xcalloc(nb, sizeof(gr));
-  gr t;
-  for (int l = 0; l < 32; l++)
-    for (size_t c = 0; c < g->ncell[w][l]; c++) {
-      gr_mul(&t, &g->dhat[w][l][c], &rho[l][g->q[w][l][c]]);
-      gr_add(&o[g->p[w][l][c]], &o[g->p[w][l][c]], &t);
-    }
+  for (int l = 0; l < 32; l++) {
+    size_t *run, nr = key_runs(g->p[w][l], g->ncell[w][l], &run);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (size_t j = 0; j < nr; j++)
+      for (size_t c = run[j]; c < run[j + 1]; c++) {
+        gr t;
+        gr_mul(&t, &g->dhat[w][l][c], &rho[l][g->q[w][l][c]]);
+        gr_add(&o[g->p[w][l][c]], &o[g->p[w][l][c]], &t);
+      }
+    free(run);
+  }
   return o;
 }
 /* general.rs:233-271 */
 static void build_layer_rows(const gdiag_t *g, const gr *eq, const gr *gam, const gr *gam2, size_t nb, gr **rows) {
-  gr t;
+#pragma omp parallel for schedule(dynamic, 1)
   for (int l = 0; l < 32; l++) {
+    gr t;
     gr *acc[3];
     for (int w = 0; w < 3; w++) {
       acc[w] = xcalloc(nb, sizeof(gr));
@@ -419,8 +430,9 @@ int general_prove(int log_b, const gdiag_t *g, const u64 *w, u64 seed_w, u64 see
   general_params(log_b, seed_w, seed_e, &pw, &pabc, &pe);
   /* 1. general_reduce_witness (reduce.rs:670-744) */
   gr *rho[32];
+  for (int l = 0; l < 32; l++) rho[l] = xmalloc(nb * sizeof(gr));
+#pragma omp parallel for collapse(2)
   for (int l = 0; l < 32; l++) {
-    rho[l] = xmalloc(nb * sizeof(gr));
     for (size_t q = 0; q < nb; q++) {
       u64 rot[32];
       for (int i = 0; i < 32; i++) rot[i] = w[32 * q + (i + l) % 32];
@@ -431,31 +443,41 @@ int general_prove(int log_b, const gdiag_t *g, const u64 *w, u64 seed_w, u64 see
   for (int mtx = 0; mtx < 2; mtx++) {
     q1[mtx] = xcalloc(nb, sizeof(gr));
     q2[mtx] = xcalloc(nb, sizeof(gr));
-    for (int l = 0; l < 32; l++)
-      for (size_t c = 0; c < g->ncell[mtx][l]; c++) {
-        u64 rv[32];
-        size_t q = g->q[mtx][l][c], p = g->p[mtx][l][c];
-        for (int i = 0; i < 32; i++) rv[i] = w[32 * q + (i + l) % 32];
-        gr d1, d2;
-        mul_with_quotients_only(&d1, &d2, &g->d[mtx][l][c * 32], rv);
-        gr_add(&q1[mtx][p], &q1[mtx][p], &d1);
-        gr_add(&q2[mtx][p], &q2[mtx][p], &d2);
-      }
+    for (int l = 0; l < 32; l++) {
+      /* cells are (p, q) sorted: a run of equal p is one thread's */
+      size_t *run, nr = key_runs(g->p[mtx][l], g->ncell[mtx][l], &run);
+#pragma omp parallel for schedule(dynamic, 16)
+      for (size_t j = 0; j < nr; j++)
+        for (size_t c = run[j]; c < run[j + 1]; c++) {
+          u64 rv[32];
+          size_t q = g->q[mtx][l][c], p = g->p[mtx][l][c];
+          for (int i = 0; i < 32; i++) rv[i] = w[32 * q + (i + l) % 32];
+          gr d1, d2;
+          mul_with_quotients_only(&d1, &d2, &g->d[mtx][l][c * 32], rv);
+          gr_add(&q1[mtx][p], &q1[mtx][p], &d1);
+          gr_add(&q2[mtx][p], &q2[mtx][p], &d2);
+        }
+      free(run);
+    }
   }
   /* 2. errors and packed matvec outputs (general.rs:389-428) */
   gr va; gr_from_g4(&va, &K->v_alpha);
   gr *ea = xmalloc(nb * sizeof(gr)), *eb = xmalloc(nb * sizeof(gr)), *ec = xmalloc(nb * sizeof(gr));
   gr *e_[2] = {ea, eb};
-  gr t, u;
+  gr t;
+#pragma omp parallel for collapse(2)
   for (int mtx = 0; mtx < 2; mtx++)
     for (size_t i = 0; i < nb; i++) {
-      gr_mul(&t, &q1[mtx][i], &va); gr_mul(&u, &q2[mtx][i], &K->v_beta); gr_add(&e_[mtx][i], &t, &u);
+      gr t1, u1;
+      gr_mul(&t1, &q1[mtx][i], &va); gr_mul(&u1, &q2[mtx][i], &K->v_beta); gr_add(&e_[mtx][i], &t1, &u1);
     }
   for (int mtx = 0; mtx < 2; mtx++) { free(q1[mtx]); free(q2[mtx]); }
   gr *asum = layered_matvec(g, 0, rho, nb), *bsum = layered_matvec(g, 1, rho, nb), *csum = layered_matvec(g, 2, rho, nb);
   gr *ah = xmalloc(nb * sizeof(gr)), *bh = xmalloc(nb * sizeof(gr)), *chh = xmalloc(nb * sizeof(gr)), *em = xmalloc(nb * sizeof(gr));
   u64 *as = xmalloc(nb * 32 * sizeof(u64)), *bs = xmalloc(nb * 32 * sizeof(u64)), *cs = xmalloc(nb * 32 * sizeof(u64));
+#pragma omp parallel for
   for (size_t i = 0; i < nb; i++) {
+    gr t;
     gr_sub(&ah[i], &asum[i], &ea[i]);
     gr_sub(&bh[i], &bsum[i], &eb[i]);
     u64 sl[32];

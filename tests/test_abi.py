@@ -62,3 +62,24 @@ def test_mimd_and_verify_argument_errors():
     assert v.value == -1
     buf, n = ctypes.c_void_p(), ctypes.c_size_t()
     assert lib.nenya_prove_mimd(None, None, ctypes.byref(buf), ctypes.byref(n)) == 4
+
+
+def test_proof_view_keeps_its_params_alive():
+    """A *_view proof reads the params' pinned buffer, so the view holds the
+    params object until the view is dropped."""
+    import gc
+    import weakref
+
+    class Owner:
+        pass
+
+    buf = ctypes.create_string_buffer(b"proofbytes")
+    owner = Owner()
+    ref = weakref.ref(owner)
+    v = na._lib._view(ctypes.c_void_p(ctypes.addressof(buf)), ctypes.c_size_t(10), owner)
+    del owner
+    gc.collect()
+    assert ref() is not None and bytes(v) == b"proofbytes" and v[0] == ord("p") and v.readonly
+    del v
+    gc.collect()
+    assert ref() is None
