@@ -9,9 +9,19 @@
       HBM bytes per launch of kernel NAME from two `rocprofv3 --pmc FETCH_SIZE`
       / `--pmc WRITE_SIZE` counter_collection.csv files (KiB per dispatch).
       Each counter is scaled by what it reads for the calibration copy of B
-      bytes in the same run (the copyBuffer dispatch with the largest
-      WRITE_SIZE), so the factor absorbs the counter's units.  The result is
-      merged into the JSON file under KEY.
+      bytes in the same run (the dispatch of the calibration kernel, k_copy_out
+      by default, with the largest WRITE_SIZE), so the factor absorbs the
+      counter's units.  The result is merged into the JSON file under KEY.
+
+  python -m nenya_amd.profile_summary kernels --stats KERNEL_STATS_CSV
+          [--mfma M.csv] [--fetch F.csv --write W.csv] [--calib-bytes B]
+          [--kernels K1,K2,...] [--cmd CMD]
+      Markdown table per kernel: calls, average duration (kernel trace), and
+      from counter runs the int8 MFMA operations per dispatch
+      (SQ_INSTS_VALU_MFMA_MOPS_I8 x 512) as TOP/s against the 5.0 POP/s dense
+      int8 peak, the MFMA busy share (SQ_VALU_MFMA_BUSY_CYCLES over
+      GRBM_GUI_ACTIVE x 1024 SIMDs), and HBM bytes per dispatch as GB/s
+      against 8 TB/s.
 """
 import argparse
 import csv
@@ -45,14 +55,73 @@ def load_counters(path):
     return list(csv.DictReader(open(path)))
 
 
+def calib_scales(fr, wr, calib_kernel, calib_bytes):
+    """FETCH_SIZE / WRITE_SIZE scale factors (bytes per counted unit... as KiB
+    ratios) from the calibration copy: the dispatch of `calib_kernel` with
+    the largest WRITE_SIZE."""
+    cal_w = max((r for r in wr if calib_kernel in r["Kernel_Name"]), key=lambda r: float(r["Counter_Value"]))
+    cal_f = max((r for r in fr if calib_kernel in r["Kernel_Name"]), key=lambda r: float(r["Counter_Value"]))
+    kib = calib_bytes / 1024.0
+    return kib / float(cal_f["Counter_Value"]), kib / float(cal_w["Counter_Value"]), cal_f, cal_w
+
+
+def _per_kernel(rows, counter=None):
+    out = {}
+    for r in rows:
+        if counter and r["Counter_Name"] != counter:
+            continue
+        out.setdefault(short(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+    return out
+
+
+def kernels(args):
+    st = {short(r["Name"]): r for r in csv.DictReader(open(args.stats))}
+    names = args.kernels.split(",") if args.kernels else sorted(st, key=lambda n: -float(st[n]["TotalDurationNs"]))
+    mf = load_counters(args.mfma) if args.mfma else []
+    mops, busy, act = (_per_kernel(mf, c) for c in ("SQ_INSTS_VALU_MFMA_MOPS_I8", "SQ_VALU_MFMA_BUSY_CYCLES",
+                                                   "GRBM_GUI_ACTIVE"))
+    fet = wri = {}
+    fs = ws = None
+    if args.fetch and args.write:
+        fr, wr = load_counters(args.fetch), load_counters(args.write)
+        fs, ws, cf, cw = calib_scales(fr, wr, args.calib_kernel, args.calib_bytes)
+        fet, wri = _per_kernel(fr), _per_kernel(wr)
+    out = []
+    if args.cmd:
+        out.append(f"Command: `{args.cmd}`\n")
+    if fs:
+        out.append(f"Traffic scale (calibration copy of {args.calib_bytes} bytes by {args.calib_kernel}): "
+                   f"FETCH_SIZE x{fs:.3f}, WRITE_SIZE x{ws:.3f}.\n")
+    out.append("| kernel | calls | avg us | int8 TOP/s | of 5.0 POP/s | MFMA busy | HBM MB/launch | GB/s | of 8 TB/s |")
+    out.append("|---|---:|---:|---:|---:|---:|---:|---:|---:|")
+    mean = lambda v: sum(v) / len(v)  # noqa: E731
+    for n in names:
+        matches = [k for k in st if k == n or k.startswith(n + "<")]
+        for k in matches:
+            r = st[k]
+            us = float(r["AverageNs"]) / 1e3
+            cells = [k, r["Calls"], f"{us:.1f}"]
+            if k in mops and mean(mops[k]) > 0:
+                ops = mean(mops[k]) * 512
+                tops = ops / (us * 1e-6) / 1e12
+                cells += [f"{tops:.0f}", f"{tops / 5000:.3f}",
+                          f"{100 * mean(busy[k]) / (mean(act[k]) * 1024):.1f}%" if k in busy and k in act else "-"]
+            else:
+                cells += ["-", "-", "-"]
+            if k in fet and k in wri:
+                b = (mean(fet[k]) * fs + mean(wri[k]) * ws) * 1024
+                gbs = b / (us * 1e-6) / 1e9
+                cells += [f"{b / 1e6:.2f}", f"{gbs:.0f}", f"{gbs / 8000:.3f}"]
+            else:
+                cells += ["-", "-", "-"]
+            out.append("| " + " | ".join(cells) + " |")
+    print("\n".join(out))
+
+
 def traffic(args):
     fr, wr = load_counters(args.fetch), load_counters(args.write)
-    cal_w = max((r for r in wr if "copyBuffer" in r["Kernel_Name"]), key=lambda r: float(r["Counter_Value"]))
+    f_scale, w_scale, cal_f, cal_w = calib_scales(fr, wr, args.calib_kernel, args.calib_bytes)
     cal_id = cal_w["Dispatch_Id"]
-    cal_f = next(r for r in fr if r["Dispatch_Id"] == cal_id and "copyBuffer" in r["Kernel_Name"])
-    kib = args.calib_bytes / 1024.0
-    f_scale = kib / float(cal_f["Counter_Value"])
-    w_scale = kib / float(cal_w["Counter_Value"])
     fk = [float(r["Counter_Value"]) for r in fr if short(r["Kernel_Name"]).startswith(args.kernel + "<")
           or short(r["Kernel_Name"]) == args.kernel]
     wk = [float(r["Counter_Value"]) for r in wr if short(r["Kernel_Name"]).startswith(args.kernel + "<")
@@ -90,8 +159,18 @@ def main():
     t.add_argument("--key", required=True)
     t.add_argument("--calib-bytes", type=int, default=1 << 28)
     t.add_argument("--out", required=True)
+    t.add_argument("--calib-kernel", default="k_copy_out")
+    k = sub.add_parser("kernels")
+    k.add_argument("--stats", required=True)
+    k.add_argument("--mfma")
+    k.add_argument("--fetch")
+    k.add_argument("--write")
+    k.add_argument("--calib-bytes", type=int, default=1 << 28)
+    k.add_argument("--calib-kernel", default="k_copy_out")
+    k.add_argument("--kernels", default="")
+    k.add_argument("--cmd", default="")
     args = ap.parse_args()
-    stats(args) if args.cmd_name == "stats" else traffic(args)
+    {"stats": stats, "traffic": traffic, "kernels": kernels}[args.cmd_name](args)
 
 
 if __name__ == "__main__":
