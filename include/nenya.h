@@ -119,6 +119,21 @@ int nenya_prove_general_device(const nenya_params* p, const void* d_w, size_t n,
 int nenya_prove_general_device_view(const nenya_params* p, const void* d_w, size_t n, const uint8_t** proof,
                                     size_t* proof_len);
 
+/* Same proof as nenya_prove_general_device, for callers that cannot allocate
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The call allocates no
+ * device or pinned memory and creates no stream or event: every device
+ * buffer of the proof is carved out of a static 256 MiB device array (one
+ * per GPU, held until the process ends), the pinned staging areas and proof
+ * buffer were sized when `p` was created, and the proof is copied into the
+ * caller's `proof` buffer of `proof_cap` bytes.  Limit: log_blocks <= 10
+ * (n <= 2^15 constraints); larger params get NENYA_E_SHAPE, a buffer smaller
+ * than the proof gets NENYA_E_ARG with the needed size in *proof_len.  Calls
+ * on one GPU share its array (whose contents the last call leaves behind),
+ * so the library runs them one at a time. */
+int nenya_prove_general_device_noalloc(const nenya_params* p, const void* d_w, size_t n, uint8_t* proof,
+                                       size_t proof_cap, size_t* proof_len);
+
 /* Per-stage wall-clock timing (ms) of the last proof made with `p` when
  * enabled (GeneralProveTiming / SimdProveTiming, general.rs:153-173,
  * simd.rs:177-192).  Enabling adds stream synchronizations at stage

@@ -46,6 +46,7 @@ def _lib():
         L.nenya_prove_general.argtypes = [vp, vp, sz, C.POINTER(vp), C.POINTER(sz)]
         L.nenya_prove_general_device.argtypes = [vp, vp, sz, C.POINTER(vp), C.POINTER(sz)]
         L.nenya_prove_general_device_view.argtypes = [vp, vp, sz, C.POINTER(vp), C.POINTER(sz)]
+        L.nenya_prove_general_device_noalloc.argtypes = [vp, vp, sz, vp, sz, C.POINTER(sz)]
         L.nenya_set_timing.argtypes = [vp, C.c_int]
         L.nenya_last_timing.argtypes = [vp, C.POINTER(C.c_double), C.c_int, C.c_char_p, sz]
         L.nenya_set_profile.argtypes = [vp, C.c_int]
@@ -400,6 +401,24 @@ class DeviceArray:
 
     def __del__(self):
         self.free()
+
+
+def prove_general_device_noalloc(dev_ptr, n_limbs, params, out=None):
+    """prove_general with the witness in device memory through the entry
+    point that allocates nothing (nenya_prove_general_device_noalloc): the
+    proof is written into `out` (a writable buffer of at least the proof's
+    size; one is made when None) and returned as bytes."""
+    need = C.c_size_t()
+    if out is None:
+        rc = _lib().nenya_prove_general_device_noalloc(params._h, C.c_void_p(dev_ptr), n_limbs, None, 0,
+                                                       C.byref(need))
+        if rc != 4 or need.value == 0:
+            _check(rc)
+        out = bytearray(need.value)
+    buf = (C.c_ubyte * len(out)).from_buffer(out)
+    _check(_lib().nenya_prove_general_device_noalloc(params._h, C.c_void_p(dev_ptr), n_limbs, buf, len(out),
+                                                     C.byref(need)))
+    return bytes(out[:need.value])
 
 
 VERIFY_ERRORS = {0: "Ok", 1: "MultSumcheck", 2: "LinSumcheck", 3: "LinClaimMismatch", 4: "LinARowMismatch",

@@ -148,6 +148,77 @@ static int general_common(const nenya_params* p, const void* w, size_t n, uint8_
   });
 }
 
+// ---------------- no-alloc General proofs ----------------
+// Device scratch of nenya_prove_general_device_noalloc: every device buffer
+// of such a proof is carved out of this array (Arena, first fit).  Each GPU
+// the library runs on holds its own copy of it until the process ends.  In
+// host code `scratch` only names the symbol: its device address on the
+// current GPU comes from hipGetSymbolAddress (NoAllocDev::ptr), and only
+// that address is ever handed to kernels or runtime calls.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(256) << 20;
+constexpr int NOALLOC_MAX_LOG_BLOCKS = 10;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+struct NoAllocDev {
+  std::mutex mu;         // calls on one GPU share its scratch: one at a time
+  void* ptr = nullptr;   // device address of `scratch` on this GPU
+  Arena arena;
+};
+// The state of GPU `dev`; the first call on a GPU (which is then the current
+// device) looks up the scratch address there, once.
+static NoAllocDev* noalloc_dev(int dev) {
+  static std::mutex mu;
+  static NoAllocDev* devs[64] = {};  // kept until the process ends, like the arrays
+  if (dev < 0 || dev >= 64) throw Error(NENYA_E_ARG, "device index out of range");
+  std::lock_guard<std::mutex> g(mu);
+  if (!devs[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      throw Error(NENYA_E_DEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    }
+    auto* d = new NoAllocDev;
+    d->ptr = ptr;
+    devs[dev] = d;
+  }
+  return devs[dev];
+}
+
+// prove_device with every device buffer taken from the GPU's scratch, one
+// stream, and the pinned areas the params reserved at setup.
+template <class T> static ProofView general_noalloc_t(ParamsBase* b, NoAllocDev* d, const void* w) {
+  auto* gp = static_cast<GeneralParams<T>*>(b);
+  Ctx<T>& cx = gp->cx0;
+  Engine<T>& e = *cx.eng;
+  e.finish_downloads();  // an earlier proof's copies release their (non-scratch) blocks
+  d->arena.reset(d->ptr, NOALLOC_SCRATCH_BYTES);
+  struct Scope {
+    Ctx<T>& cx;
+    bool prof, timing;
+    int n0 = std::uncaught_exceptions();
+    Scope(Ctx<T>& c, Arena* a) : cx(c), prof(c.prof.on), timing(c.timing) {
+      cx.pool.arena = a;
+      cx.eng->serial = true;
+      pinned_no_grow() = true;
+      cx.prof.on = false;
+      cx.timing = false;
+    }
+    ~Scope() {
+      // a failed proof may leave kernels on the stream that use scratch blocks
+      if (std::uncaught_exceptions() > n0) (void)hipStreamSynchronize(cx.st);
+      cx.pool.arena = nullptr;
+      cx.eng->serial = false;
+      pinned_no_grow() = false;
+      cx.prof.on = prof;
+      cx.timing = timing;
+    }
+  } scope(cx, &d->arena);
+  ProofView v = gp->prove_device(cx, (const typename T::L*)w);
+  if (d->arena.used != 0) throw Error(NENYA_E_DEVICE, "scratch blocks outlived a no-alloc proof");
+  return v;
+}
+
 template <class F> static int verify_common(const nenya_params* p, int kind, int* verdict, F&& f) {
   return guard([&] {
     if (!p || !p->impl || p->impl->kind != kind || !verdict) throw Error(NENYA_E_ARG, "bad argument");
@@ -385,6 +456,52 @@ int nenya_prove_general_device_view(const nenya_params* p, const void* d_w, size
 
 int nenya_prove_general_view(const nenya_params* p, const void* w, size_t n, const uint8_t** proof, size_t* len) {
   return general_common(p, w, n, nullptr, len, false, proof);
+}
+
+int nenya_prove_general_device_noalloc(const nenya_params* p, const void* d_w, size_t n, uint8_t* proof,
+                                       size_t proof_cap, size_t* proof_len) {
+  return guard([&] {
+    if (!p || !p->impl || p->impl->kind != 1 || !d_w || !proof_len) throw Error(NENYA_E_ARG, "bad argument");
+    ParamsBase* b = p->impl;
+    size_t need = 0, N = 0;
+    int log_b = 0;
+    auto shape = [&](auto* gp) {
+      need = gp->proof_size();
+      N = gp->N;
+      log_b = gp->log_b;
+    };
+    switch (b->k) {
+      case 8: shape(static_cast<GeneralParams<W<8>>*>(b)); break;
+      case 16: shape(static_cast<GeneralParams<W<16>>*>(b)); break;
+      case 32: shape(static_cast<GeneralParams<W<32>>*>(b)); break;
+      default: shape(static_cast<GeneralParams<W<64>>*>(b)); break;
+    }
+    if (log_b > NOALLOC_MAX_LOG_BLOCKS)
+      throw Error(NENYA_E_SHAPE, "log_blocks " + std::to_string(log_b) + " exceeds the no-alloc limit of " +
+                                     std::to_string(NOALLOC_MAX_LOG_BLOCKS) + " (256 MiB device scratch)");
+    if (n != N) throw Error(NENYA_E_SHAPE, "witness length must be 32 * 2^log_blocks");
+    *proof_len = need;
+    if (!proof || proof_cap < need)
+      throw Error(NENYA_E_ARG, "proof buffer of " + std::to_string(proof_cap) + " bytes is smaller than the proof (" +
+                                   std::to_string(need) + " bytes, written to *proof_len)");
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    NoAllocDev* d = noalloc_dev(dev);
+    std::lock_guard<std::mutex> g1(d->mu);
+    auto* np = const_cast<nenya_params*>(p);
+    std::lock_guard<std::mutex> g2(np->mu);
+    ProofView v;
+    switch (b->k) {
+      case 8: v = general_noalloc_t<W<8>>(b, d, d_w); break;
+      case 16: v = general_noalloc_t<W<16>>(b, d, d_w); break;
+      case 32: v = general_noalloc_t<W<32>>(b, d, d_w); break;
+      default: v = general_noalloc_t<W<64>>(b, d, d_w); break;
+    }
+    if (v.second != need) throw Error(NENYA_E_DEVICE, "proof size mismatch");
+    std::memcpy(proof, v.first, v.second);
+    *proof_len = v.second;
+    return NENYA_OK;
+  });
 }
 
 int nenya_verify_simd(const nenya_params* p, const uint32_t* ar, const uint32_t* ac, const void* av, size_t an,

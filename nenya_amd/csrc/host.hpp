@@ -1,6 +1,7 @@
 // Host-side building blocks: device memory pool, per-width constants,
 // PCS parameters, Fiat-Shamir transcript.
 #pragma once
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -64,6 +65,73 @@ inline size_t size_class(size_t bytes) {
   return (bytes + q - 1) / q * q;
 }
 
+// A fixed device range handed out first-fit, for calls that must not
+// allocate device memory: blocks are 256-byte aligned offsets into
+// [base, base + size), free space is a sorted list of at most MAXB ranges in
+// this object's own storage (no heap use), and neighbouring free ranges
+// merge when a block comes back.
+struct Arena {
+  static constexpr int MAXB = 2048;
+  struct Range {
+    size_t off, len;
+  };
+  uint8_t* base = nullptr;
+  size_t size = 0, used = 0, peak = 0;
+  int nfree = 0;
+  Range fr[MAXB];
+  void reset(void* b, size_t s) {
+    base = (uint8_t*)b;
+    size = s;
+    used = peak = 0;
+    nfree = 1;
+    fr[0] = {0, s};
+  }
+  static size_t round(size_t bytes) { return (std::max<size_t>(bytes, 1) + 255) & ~size_t(255); }
+  bool owns(const void* p) const { return base && p >= base && p < base + size; }
+  void* get(size_t bytes) {
+    bytes = round(bytes);
+    for (int i = 0; i < nfree; i++) {
+      if (fr[i].len < bytes) continue;
+      const size_t off = fr[i].off;
+      fr[i].off += bytes;
+      fr[i].len -= bytes;
+      if (fr[i].len == 0) {
+        for (int j = i; j + 1 < nfree; j++) fr[j] = fr[j + 1];
+        nfree--;
+      }
+      used += bytes;
+      peak = std::max(peak, used);
+      return base + off;
+    }
+    throw Error(NENYA_E_OOM, "no-alloc scratch exhausted: " + std::to_string(bytes) + " bytes requested with " +
+                                 std::to_string(size - used) + " of " + std::to_string(size) + " free");
+  }
+  void put(const void* p, size_t bytes) {
+    bytes = round(bytes);
+    const size_t off = (size_t)((const uint8_t*)p - base);
+    int i = 0;
+    while (i < nfree && fr[i].off < off) i++;
+    const bool left = i > 0 && fr[i - 1].off + fr[i - 1].len == off;
+    const bool right = i < nfree && off + bytes == fr[i].off;
+    if (left && right) {
+      fr[i - 1].len += bytes + fr[i].len;
+      for (int j = i; j + 1 < nfree; j++) fr[j] = fr[j + 1];
+      nfree--;
+    } else if (left) {
+      fr[i - 1].len += bytes;
+    } else if (right) {
+      fr[i].off = off;
+      fr[i].len += bytes;
+    } else {
+      if (nfree == MAXB) throw Error(NENYA_E_DEVICE, "no-alloc scratch free list full");
+      for (int j = nfree; j > i; j--) fr[j] = fr[j - 1];
+      fr[i] = {off, bytes};
+      nfree++;
+    }
+    used -= bytes;
+  }
+};
+
 struct Pool {
   std::multimap<size_t, void*> free_;
   std::vector<std::pair<void*, size_t>> all_;
@@ -87,7 +155,14 @@ struct Pool {
     }
     deferred.clear();
   }
+  // While set, every block comes from this arena (the no-alloc entry point);
+  // blocks of the arena go back to it, others to the free lists.
+  Arena* arena = nullptr;
   void* get(size_t bytes) {
+    if (arena) {
+      std::lock_guard<std::mutex> g(mu);
+      return arena->get(bytes);
+    }
     bytes = size_class(bytes);
     std::lock_guard<std::mutex> g(mu);
     auto it = free_.find(bytes);
@@ -165,6 +240,11 @@ struct Pool {
   }
   void put(void* p, size_t bytes) {
     if (!p) return;
+    if (arena && arena->owns(p)) {
+      std::lock_guard<std::mutex> g(mu);
+      arena->put(p, bytes);
+      return;
+    }
     bytes = size_class(bytes);
     if (pool_guard()) check_guards(p, bytes);
     std::lock_guard<std::mutex> g(mu);

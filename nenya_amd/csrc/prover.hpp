@@ -72,6 +72,13 @@ struct KProf {
   }
 };
 
+// Set on a thread while it runs a no-alloc call: a pinned buffer that would
+// have to grow fails the call instead of allocating.
+inline bool& pinned_no_grow() {
+  static thread_local bool v = false;
+  return v;
+}
+
 // Pinned host buffer the proof is assembled in (grow-only, owned by params).
 struct Pinned {
   uint8_t* p = nullptr;
@@ -92,6 +99,9 @@ struct Pinned {
   }
   uint8_t* get(size_t n) {
     if (n > cap) {
+      if (pinned_no_grow())
+        throw Error(NENYA_E_OOM, "a pinned staging buffer of " + std::to_string(cap) + " bytes would have to grow to " +
+                                     std::to_string(n) + " in a no-alloc call");
       if (p) (void)hipHostFree(p);
       p = nullptr;
       cap = 0;
@@ -251,6 +261,9 @@ template <class T> struct Engine {
   // post each challenge before the lift that reads it is enqueued, so no
   // kernel waits on a host thread.
   bool pipelined = true;
+  // No-alloc calls: one stream only (no forks, downloads in order on the
+  // prover stream), so no stream or event is created during the call.
+  bool serial = false;
   bool pipeline_rounds() const { return pipelined && live_servers().load() == 0 && !pool_poison() && !pool_guard(); }
   DBuf<uint32_t> d_count;  // last-block counter of k_sum_post
   DBuf<L> d_sums;          // reduced round sums (3 elements)
@@ -375,6 +388,16 @@ template <class T> struct Engine {
     return (L*)dptr;
   }
   void put_async(void* dst, const void* src, size_t bytes) { h2d(dst, src, bytes); }
+  // Sizes every pinned staging area a device-witness proof uses (small
+  // uploads, downloads of up to 1 MiB, commitment roots, results, picks), so
+  // a no-alloc call finds them allocated.
+  void reserve_staging() {
+    upbuf.get(UPCAP);
+    xfer.get(size_t(1) << 20);
+    rootbuf.get(ROOT_SLOTS * 32);
+    res_host();
+    pickbuf.get(128 * E * sizeof(L));
+  }
 
   // Upload (stream-ordered; the host buffer may be reused on return).
   void h2d(void* d, const void* h, size_t bytes) {
@@ -663,8 +686,8 @@ template <class T> struct Engine {
     uint8_t* ddst = nullptr;
     HIP_CHECK(hipHostGetDevicePointer((void**)&ddst, dst, 0));
     const dim3 grid((unsigned)std::min<uint64_t>((bytes + 4095) / 4096, dl_blocks()));
-    if (!pipelined) {
-      // contexts of a multi-worker server: in order on the prover stream
+    if (!pipelined || serial) {
+      // contexts of a multi-worker server and no-alloc calls: in order on the prover stream
       // (the workers' streams already overlap each other)
       hipLaunchKernelGGL(k_copy_out, grid, dim3(256), 0, st, (const uint8_t*)src.p, ddst, (uint64_t)bytes);
       HIP_CHECK(hipGetLastError());
@@ -696,7 +719,7 @@ template <class T> struct Engine {
   int forked = 0;
   void fork(int ns) {
     st_main = st;
-    forked = (pipelined && !pool_poison() && !pool_guard()) ? std::min(ns - 1, NSIDE) : 0;
+    forked = (pipelined && !serial && !pool_poison() && !pool_guard()) ? std::min(ns - 1, NSIDE) : 0;
     if (forked <= 0) {
       forked = 0;
       return;

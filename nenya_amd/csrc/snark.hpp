@@ -902,6 +902,10 @@ template <class T> struct GeneralParams : ParamsBase {
       eng->h2d(eval.p, ev.data(), ev.size() * sizeof(L));
       eng->sync();
     }
+    // the proof buffer and the pinned staging areas exist before the first
+    // proof (a no-alloc call may be the first)
+    cx0.proofbuf.get(proof_size());
+    eng->reserve_staging();
     eng->sync();
   }
   ~GeneralParams() override {
@@ -958,6 +962,24 @@ template <class T> struct GeneralParams : ParamsBase {
     e.launch("k_rows_gather", ne * (Engine<T>::EB + 8) + 32.0 * nb * Engine<T>::EB, ne * E, (k_rows_gather<T>),
              dim3(grid_for(32 * nb, 8, 1u << 16)), dim3(256), (const uint64_t*)eptr.p, (const uint32_t*)ecode.p,
              (const L*)eval.p, (const L*)G.p, (uint64_t)nb, rows, (uint64_t)(32 * nb));
+  }
+
+  // Section sizes of the serialized proof (general.rs:120-150): mult check,
+  // linear check, and the openings of e_mult, abc, the stacked errors and the
+  // rotated rho batch; returns the total with the four roots.
+  size_t sections(size_t& s_mult, size_t& s_lin, size_t& s_em, size_t& s_abc, size_t& s_es, size_t& s_rho) const {
+    const size_t EBs = E * sizeof(L);
+    s_mult = 8 + 3 * (size_t)log_b * EBs + 4 * EBs;
+    s_lin = EBs + 8 + 3 * (size_t)log_b * EBs + 64 * EBs;
+    s_em = opening_bytes(pe.P, pe.P.k1, sizeof(L), 0);
+    s_abc = opening_bytes(pabc.P, 3 * pabc.P.k1, sizeof(L), 0);
+    s_es = opening_bytes(pe.P, 3 * pe.P.k1, sizeof(L), 0);
+    s_rho = opening_bytes(pw.P, pw.P.k1, sizeof(L), 32);
+    return 128 + s_mult + s_lin + s_em + s_abc + s_es + s_rho;
+  }
+  size_t proof_size() const {
+    size_t a, b, c, d, e, f;
+    return sections(a, b, c, d, e, f);
   }
 
   ProofView prove_device(Ctx<T>& cx, const L* w /* device, N limbs */) {
@@ -1052,11 +1074,8 @@ template <class T> struct GeneralParams : ParamsBase {
     stg.mark("mult_sumcheck");
     // proof layout (general.rs:120-150): roots, mult, lin, open_emult,
     // open_abc, open_estack, open_rho -- sections sized up front
-    const size_t EBs = E * sizeof(L);
-    size_t s_mult = 8 + 3 * (size_t)log_b * EBs + 4 * EBs, s_lin = EBs + 8 + 3 * (size_t)log_b * EBs + 64 * EBs;
-    size_t s_em = opening_bytes(pe.P, pe.P.k1, sizeof(L), 0), s_abc = opening_bytes(pabc.P, 3 * pabc.P.k1, sizeof(L), 0);
-    size_t s_es = opening_bytes(pe.P, 3 * pe.P.k1, sizeof(L), 0), s_rho = opening_bytes(pw.P, pw.P.k1, sizeof(L), 32);
-    size_t total = 128 + s_mult + s_lin + s_em + s_abc + s_es + s_rho;
+    size_t s_mult, s_lin, s_em, s_abc, s_es, s_rho;
+    const size_t total = sections(s_mult, s_lin, s_em, s_abc, s_es, s_rho);
     uint8_t* buf = cx.proofbuf.get(total);
     uint8_t* o_mult = buf + 128;
     uint8_t* o_lin = o_mult + s_mult;
