@@ -23,6 +23,15 @@
  *     allocates nothing in steady state.
  *   - A params object is immutable after creation and owns its device memory
  *     and HIP stream; calls on one params object are serialized internally.
+ *   - Threading: a single-proof call (nenya_prove_*) that finds no other
+ *     library call running and no server alive pipelines its sumcheck rounds:
+ *     its kernels wait for challenges its thread posts.  Every other library
+ *     call waits until it is done, and it waits for calls already running
+ *     (then it runs without pipelining), so library calls from several
+ *     threads are safe.  Other HIP runtime calls the caller makes from other
+ *     threads meanwhile (allocation, frees, synchronization) can stall such a
+ *     proof until its device waits time out (10 s, NENYA_E_DEVICE; never a
+ *     wrong proof): drive the device from one thread, or use a server.
  */
 #ifndef NENYA_H
 #define NENYA_H
@@ -39,6 +48,7 @@ extern "C" {
 #define NENYA_E_DEVICE 3 /* HIP runtime error or internal self-check */
 #define NENYA_E_ARG 4    /* invalid argument (width, null pointer, ...) */
 #define NENYA_E_NOGPU 5  /* no HIP device visible */
+#define NENYA_E_CANCELED 6 /* a served job canceled with nenya_cancel */
 
 typedef struct nenya_params nenya_params;
 
@@ -193,6 +203,16 @@ int nenya_poll(nenya_server* s, uint64_t job);
 int nenya_wait(nenya_server* s, uint64_t job, const uint8_t** proof, size_t* proof_len, double* queued_ms,
                double* run_ms);
 int nenya_job_release(nenya_server* s, uint64_t job);
+/* Cancels a job: a queued job is dropped before it starts; a running one
+ * stops at its next checkpoint (each sumcheck round, each commitment and
+ * opening), its kernels ending normally.  nenya_wait on it then returns
+ * NENYA_E_CANCELED; other jobs are unaffected.  NENYA_E_ARG when the job is
+ * unknown or already done.  A job whose device footprint exceeds the
+ * server's budget is refused at submit with NENYA_E_OOM.  Test switch:
+ * NENYA_INJECT_OOM=<job>:<n>, read when a server is created, makes the n-th
+ * device allocation of that job (job ids count from 1) fail with
+ * NENYA_E_OOM. */
+int nenya_cancel(nenya_server* s, uint64_t job);
 void nenya_server_free(nenya_server* s);
 
 /* Device-to-device copy of `bytes` bytes by the library's copy kernel

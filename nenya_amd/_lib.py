@@ -8,7 +8,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 _DT = {8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}
-_CODES = {1: "shape", 2: "out of device memory", 3: "device", 4: "argument", 5: "no GPU"}
+_CODES = {1: "shape", 2: "out of device memory", 3: "device", 4: "argument", 5: "no GPU", 6: "canceled"}
 
 
 class NenyaError(RuntimeError):
@@ -59,6 +59,7 @@ def _lib():
         L.nenya_poll.argtypes = [vp, u64]
         L.nenya_wait.argtypes = [vp, u64, C.POINTER(vp), C.POINTER(sz), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.nenya_job_release.argtypes = [vp, u64]
+        L.nenya_cancel.argtypes = [vp, u64]
         L.nenya_server_free.argtypes = [vp]
         L.nenya_params_free.argtypes = [vp]
         L.nenya_free.argtypes = [vp]
@@ -493,6 +494,11 @@ class Server:
         _check(_lib().nenya_submit_mimd(self._h, params._h, wit.ctypes.data, C.byref(j)))
         self._keep[j.value] = (params, wit)
         return j.value
+
+    def cancel(self, job):
+        """Cancel a job (queued: dropped; running: stops at its next
+        checkpoint); wait() on it then raises NenyaError with code 6."""
+        _check(_lib().nenya_cancel(self._h, job))
 
     def poll(self, job):
         r = _lib().nenya_poll(self._h, job)

@@ -129,6 +129,7 @@ static int general_common(const nenya_params* p, const void* w, size_t n, uint8_
     if (!p || !p->impl || p->impl->kind != 1 || !w || !len || (!proof && !view)) throw Error(NENYA_E_ARG, "bad argument");
     auto* np = const_cast<nenya_params*>(p);
     std::lock_guard<std::mutex> g(np->mu);
+    CallTurn turn(true);
     auto run = [&]() -> ProofView {
       switch (p->impl->k) {
         case 8: return general_t<W<8>>(p->impl, w, n, dev);
@@ -224,6 +225,7 @@ template <class F> static int verify_common(const nenya_params* p, int kind, int
     if (!p || !p->impl || p->impl->kind != kind || !verdict) throw Error(NENYA_E_ARG, "bad argument");
     auto* np = const_cast<nenya_params*>(p);
     std::lock_guard<std::mutex> g(np->mu);
+    CallTurn turn(false);
     *verdict = f(p->impl);
     return NENYA_OK;
   });
@@ -247,6 +249,7 @@ int nenya_device_copy(void* dst, const void* src, size_t bytes) {
     if ((!dst || !src) && bytes) throw Error(NENYA_E_ARG, "bad argument");
     need_gpu();
     if (!bytes) return NENYA_OK;
+    CallTurn turn(false);
     const unsigned grid = (unsigned)std::min<size_t>(2048, (bytes + 4095) / 4096);
     hipLaunchKernelGGL(k_copy_out, dim3(grid), dim3(256), 0, nullptr, (const uint8_t*)src, (uint8_t*)dst,
                        (uint64_t)bytes);
@@ -257,11 +260,14 @@ int nenya_device_copy(void* dst, const void* src, size_t bytes) {
 }
 
 void nenya_free(void* p) {
-  if (p && !Handout::get().back(p)) std::free(p);
+  if (!p) return;
+  CallTurn turn(false);  // may free a pinned buffer
+  if (!Handout::get().back(p)) std::free(p);
 }
 
 void nenya_params_free(nenya_params* p) {
   if (!p) return;
+  CallTurn turn(false);
   delete p->impl;
   delete p;
 }
@@ -272,6 +278,7 @@ int nenya_params_simd_secure(int k, int log_n, int log_m, uint64_t seed_w, uint6
     need_width(k);
     if (log_n < 2 || log_m < 2 || log_n > 30 || log_m > 30) throw Error(NENYA_E_SHAPE, "log_n/log_m out of range");
     need_gpu();
+    CallTurn turn(false);
     auto* p = new nenya_params;
     try {
       switch (k) {
@@ -298,6 +305,7 @@ static int simd_common(const nenya_params* p, const uint32_t* ar, const uint32_t
       throw Error(NENYA_E_ARG, "bad argument");
     auto* np = const_cast<nenya_params*>(p);
     std::lock_guard<std::mutex> g(np->mu);
+    CallTurn turn(true);
     ParamsBase* b = p->impl;
     int ln = 0, lm = 0;
     switch (b->k) {
@@ -362,6 +370,7 @@ int nenya_params_mimd_secure(int k, int log_n, int log_m, const size_t* a_off, c
       sets[w] = {offs[w], rs[w], cs[w], vs[w]};
     }
     need_gpu();
+    CallTurn turn(false);
     auto* p = new nenya_params;
     try {
       switch (k) {
@@ -386,6 +395,7 @@ static int mimd_common(const nenya_params* p, const void* witnesses, uint8_t** p
       throw Error(NENYA_E_ARG, "bad argument");
     auto* np = const_cast<nenya_params*>(p);
     std::lock_guard<std::mutex> g(np->mu);
+    CallTurn turn(true);
     auto run = [&]() -> ProofView {
       switch (p->impl->k) {
         case 8: return mimd_t<W<8>>(p->impl, witnesses);
@@ -424,6 +434,7 @@ int nenya_params_general_secure(int k, int log_blocks, const uint32_t* ar, const
     if ((an && (!ar || !ac || !av)) || (bn && (!br || !bc || !bv)) || (cn && (!cr || !cc || !cv)))
       throw Error(NENYA_E_ARG, "null matrix array");
     need_gpu();
+    CallTurn turn(false);
     size_t n = size_t(32) << log_blocks;
     HostCoo m3[3] = {{n, n, an, ar, ac, av}, {n, n, bn, br, bc, bv}, {n, n, cn, cr, cc, cv}};
     auto* p = new nenya_params;
@@ -490,6 +501,7 @@ int nenya_prove_general_device_noalloc(const nenya_params* p, const void* d_w, s
     std::lock_guard<std::mutex> g1(d->mu);
     auto* np = const_cast<nenya_params*>(p);
     std::lock_guard<std::mutex> g2(np->mu);
+    CallTurn turn(true);
     ProofView v;
     switch (b->k) {
       case 8: v = general_noalloc_t<W<8>>(b, d, d_w); break;
@@ -615,6 +627,7 @@ int nenya_server_create(int workers, size_t mem_budget, nenya_server** out) {
   return guard([&] {
     if (!out || workers < 1 || workers > 64) throw Error(NENYA_E_ARG, "workers must be 1..64");
     need_gpu();
+    CallTurn turn(false);
     auto* s = new nenya_server;
     try {
       s->impl = new Server(workers, mem_budget);
@@ -714,8 +727,17 @@ int nenya_job_release(nenya_server* s, uint64_t job) {
   });
 }
 
+int nenya_cancel(nenya_server* s, uint64_t job) {
+  return guard([&] {
+    if (!s || !s->impl) throw Error(NENYA_E_ARG, "bad argument");
+    s->impl->cancel(job);
+    return NENYA_OK;
+  });
+}
+
 void nenya_server_free(nenya_server* s) {
   if (!s) return;
+  CallTurn turn(false);
   delete s->impl;
   delete s;
 }

@@ -9,6 +9,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 
 #include "blake3.hpp"
 #include "common.hpp"
@@ -49,6 +50,48 @@ inline std::atomic<int>& live_servers() {
   static std::atomic<int> n{0};
   return n;
 }
+
+// Turns on the device.  A single-proof call pipelines its sumcheck rounds
+// (kernels wait for challenges the calling thread posts) only while no other
+// call of the library runs: another thread's runtime calls could wait for
+// the device while holding locks the posting thread needs.  Such a call
+// holds device_turns() exclusively; every other call (proofs that do not
+// pipeline, server jobs, verifiers) holds it shared, so they overlap each
+// other but never a pipelining proof.
+inline std::shared_mutex& device_turns() {
+  static std::shared_mutex m;
+  return m;
+}
+inline bool& solo_call() {
+  static thread_local bool v = false;
+  return v;
+}
+struct CallTurn {
+  bool excl = false;
+  // want_solo: the call would like to pipeline its rounds; it does when it
+  // gets the device to itself right away (and no server exists)
+  explicit CallTurn(bool want_solo) {
+    if (want_solo && device_turns().try_lock()) {
+      if (live_servers().load() == 0) {
+        excl = true;
+        solo_call() = true;
+        return;
+      }
+      device_turns().unlock();
+    }
+    device_turns().lock_shared();
+  }
+  CallTurn(const CallTurn&) = delete;
+  CallTurn& operator=(const CallTurn&) = delete;
+  ~CallTurn() {
+    if (excl) {
+      solo_call() = false;
+      device_turns().unlock();
+    } else {
+      device_turns().unlock_shared();
+    }
+  }
+};
 
 // Allocation size classes: powers of two up to 64 KiB, then four classes per
 // octave (a block is at most 25 % larger than the request), so blocks freed
@@ -158,7 +201,11 @@ struct Pool {
   // While set, every block comes from this arena (the no-alloc entry point);
   // blocks of the arena go back to it, others to the free lists.
   Arena* arena = nullptr;
+  // Test switch of the server (NENYA_INJECT_OOM): when > 0, counts down per
+  // request and the request that reaches 0 fails as if the device were full.
+  long fail_in = 0;
   void* get(size_t bytes) {
+    if (fail_in > 0 && --fail_in == 0) throw Error(NENYA_E_OOM, "injected device allocation failure");
     if (arena) {
       std::lock_guard<std::mutex> g(mu);
       return arena->get(bytes);

@@ -256,15 +256,22 @@ template <class T> struct Engine {
   // Pipelined rounds (a kernel waits for the host's challenge) only where
   // one thread drives the device: allocation and free calls of other threads
   // can wait for the device while holding runtime locks this thread's
-  // launches need.  Contexts of a multi-worker server (and every context
-  // while a server exists, or with the pool's synchronizing debug modes)
-  // post each challenge before the lift that reads it is enqueued, so no
-  // kernel waits on a host thread.
+  // launches need.  So only a call that holds the device to itself
+  // (CallTurn, solo_call()) pipelines; server contexts, calls that overlap
+  // others and the pool's synchronizing debug modes post each challenge
+  // before the lift that reads it is enqueued, so no kernel waits on a host
+  // thread.
   bool pipelined = true;
+  // Served jobs: a cancel request this engine checks between phases
+  // (checkpoint()); the job's proof then fails with NENYA_E_CANCELED.
+  const std::atomic<bool>* cancel = nullptr;
+  void checkpoint() const {
+    if (cancel && cancel->load(std::memory_order_relaxed)) throw Error(NENYA_E_CANCELED, "job canceled");
+  }
   // No-alloc calls: one stream only (no forks, downloads in order on the
   // prover stream), so no stream or event is created during the call.
   bool serial = false;
-  bool pipeline_rounds() const { return pipelined && live_servers().load() == 0 && !pool_poison() && !pool_guard(); }
+  bool pipeline_rounds() const { return pipelined && solo_call() && !pool_poison() && !pool_guard(); }
   DBuf<uint32_t> d_count;  // last-block counter of k_sum_post
   DBuf<L> d_sums;          // reduced round sums (3 elements)
 
@@ -933,6 +940,7 @@ template <class T> struct Engine {
   // `rows` message rows (k2 entries of W limbs, contiguous), hash the n
   // columns, build the Merkle tree.
   CommitDev<T> commit(const PcsDev<T>& pc, const L* rowsbuf, size_t rows, const L* lifted) {
+    checkpoint();
     const PcsParams& P = pc.P;
     const int W = P.packed ? 32 : E;
     CommitDev<T> c;
@@ -1057,6 +1065,7 @@ template <class T> struct Engine {
   // domain || root || m-bar), columns, Merkle paths; D2H straight into the
   // pinned proof buffer.
   void open_tail(const PcsDev<T>& pc, const CommitDev<T>& c, const L* d_mbar, size_t nmbar, Sink& out) {
+    checkpoint();
     const PcsParams& P = pc.P;
     const int W = P.packed ? 32 : E;
     const char* dom = P.packed ? "pcs-ring/fs/queries" : "snark-ring/r2-pcs/fs/queries";
@@ -1258,6 +1267,7 @@ template <class T> struct Engine {
       const uint32_t vup = ++mseq;
       // host side of round r: its polynomial, the transcript, the challenge
       auto host_round = [&] {
+        checkpoint();
         G q[3];
         wait_round((int)r, vdown, q);
         tr.elems("sc/round", q, 3);
@@ -1345,6 +1355,7 @@ template <class T> struct Engine {
       MulMat mr;
       const uint32_t vup = ++mseq;
       auto host_round = [&] {
+        checkpoint();
         G q[3];
         wait_round((int)r, vdown, q);
         tr.elems("sc/round", q, 3);

@@ -36,7 +36,8 @@ struct Job {
   int rc = NENYA_OK;
   std::string err;
   double t_submit = 0, t_start = 0, t_end = 0;
-  bool done = false;
+  bool done = false, started = false;
+  std::atomic<bool> cancel{false};
 };
 
 // One stream and one device pool per worker, shared by its per-width
@@ -79,6 +80,10 @@ struct Server {
   uint64_t next_id = 1;
   bool stop = false;
   int device = 0;
+  // test switch NENYA_INJECT_OOM=<job>:<n>: the n-th device allocation of
+  // job <job> fails
+  uint64_t inject_job = 0;
+  long inject_at = 0;
 
   Server(int nworkers, size_t budget_bytes) {
     HIP_CHECK(hipGetDevice(&device));
@@ -88,6 +93,14 @@ struct Server {
       budget_bytes = fr / 10 * 8;
     }
     budget = budget_bytes;
+    if (const char* e = std::getenv("NENYA_INJECT_OOM")) {
+      unsigned long long jid = 0;
+      long at = 0;
+      if (std::sscanf(e, "%llu:%ld", &jid, &at) == 2 && jid > 0 && at > 0) {
+        inject_job = jid;
+        inject_at = at;
+      }
+    }
     for (int i = 0; i < nworkers; i++) workers.emplace_back(new Worker());
     live_servers()++;  // before any job can be submitted
     for (int i = 0; i < nworkers; i++) threads.emplace_back([this, i] { loop(i); });
@@ -105,8 +118,11 @@ struct Server {
 
   uint64_t submit(std::unique_ptr<Job> j) {
     std::lock_guard<std::mutex> g(mu);
-    j->id = next_id++;
     j->fp = j->params->footprint;
+    if (j->fp > budget)
+      throw Error(NENYA_E_OOM, "job footprint of " + std::to_string(j->fp) + " device bytes exceeds the server's budget of " +
+                                   std::to_string(budget));
+    j->id = next_id++;
     j->t_submit = now_ms();
     Job* raw = j.get();
     jobs[raw->id] = std::move(j);
@@ -129,6 +145,28 @@ struct Server {
     std::lock_guard<std::mutex> g(mu);
     return find(id)->done;
   }
+  // A queued job leaves the queue done (canceled); a running one is flagged
+  // and stops at its next checkpoint.
+  void cancel(uint64_t id) {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      Job* j = find(id);
+      if (j->done) throw Error(NENYA_E_ARG, "job already done");
+      j->cancel.store(true);
+      if (j->started) return;
+      for (auto it = queue.begin(); it != queue.end(); ++it)
+        if (*it == j) {
+          queue.erase(it);
+          break;
+        }
+      j->rc = NENYA_E_CANCELED;
+      j->err = "job canceled before it started";
+      j->t_start = j->t_end = now_ms();
+      j->done = true;
+    }
+    cv_done.notify_all();
+    cv_work.notify_all();
+  }
   void release(uint64_t id) {
     std::lock_guard<std::mutex> g(mu);
     Job* j = find(id);
@@ -148,6 +186,12 @@ struct Server {
 
   template <class T> void run_t(Worker& wk, Job& j) {
     Ctx<T>& cx = wk.get<T>();
+    CallTurn turn(false);
+    struct Cancel {
+      Engine<T>& e;
+      Cancel(Engine<T>& en, const std::atomic<bool>* c) : e(en) { e.cancel = c; }
+      ~Cancel() { e.cancel = nullptr; }
+    } can(*cx.eng, &j.cancel);
     ProofView v;
     if (j.kind == 1) {
       auto* gp = static_cast<GeneralParams<T>*>(j.params);
@@ -182,6 +226,7 @@ struct Server {
         queue.pop_front();
         in_use += j->fp;
         running++;
+        j->started = true;
         j->t_start = now_ms();
         // a recycled pinned buffer becomes the context's proof buffer: the
         // smallest that holds the params' last proof, or (before their first
@@ -203,6 +248,7 @@ struct Server {
       }
       int rc = NENYA_OK;
       std::string err;
+      wk.pool.fail_in = j->id == inject_job ? inject_at : 0;
       try {
         switch (j->params->k) {
           case 8: swap_in<W<8>>(wk, *j); run_t<W<8>>(wk, *j); break;
@@ -216,6 +262,11 @@ struct Server {
       } catch (const std::exception& e) {
         rc = NENYA_E_DEVICE;
         err = e.what();
+      }
+      wk.pool.fail_in = 0;
+      if (rc == NENYA_OK && j->cancel.load()) {
+        rc = NENYA_E_CANCELED;
+        err = "job canceled while it ran";
       }
       {
         std::lock_guard<std::mutex> g(mu);

@@ -568,12 +568,14 @@ template <class T> struct MimdParams : ParamsBase {
   // sum_c Xhat[r, c] w_phi[c] for the three matrices into out (3 x n)
   void matvecs(Engine<T>& e, const L* slots, const L* wphi, L* out) {
     if constexpr (kMx<T>) {
-      // the three matrices are independent: one stream each
-      DBuf<L> cm(e.pool, 3 * n * 1024);
+      // the three matrices are independent: one stream each, with a C-matrix
+      // slice per stream (one slice when the work stays on one stream)
       e.fork(3);
+      const size_t ns = (size_t)e.forked + 1;
+      DBuf<L> cm(e.pool, ns * n * 1024);
       for (int w = 0; w < 3; w++) {
         e.on(w);
-        L* cmw = cm.p + (size_t)w * n * 1024;
+        L* cmw = cm.p + (size_t)(w % ns) * n * 1024;
         e.launch("k_cmat_mimd", (double)n * 1024 * sizeof(L), (double)ncell[w] * 1024, (k_cmat_mimd<T>),
                  dim3(grid_for(n, 8, 4096)), dim3(256), (const uint64_t*)(cptr.p + w * (n + 1)), (const uint32_t*)ccol.p,
                  (const L*)cval.p, slots, cmw, (uint64_t)n);
@@ -1010,11 +1012,14 @@ template <class T> struct GeneralParams : ParamsBase {
       // sum_{s,s'} C_p[s][s'] b_s b_s' on the matrix cores
       // the three matrices are independent: one stream each (k_cgemm is
       // bound by load latency, so three in flight overlap well)
-      DBuf<L> cm(&pool, 3 * nb * 1024);
+      // (a C-matrix slice per stream; one slice when the work stays on one
+      // stream)
       e.fork(3);
+      const size_t ns = (size_t)e.forked + 1;
+      DBuf<L> cm(&pool, ns * nb * 1024);
       for (int m = 0; m < 3; m++) {
         e.on(m);
-        L* cmm = cm.p + (size_t)m * nb * 1024;
+        L* cmm = cm.p + (size_t)(m % ns) * nb * 1024;
         e.launch("k_cmat", (double)nb * 1024 * sizeof(L), 0, (k_cmat<T>), dim3(grid_for(nb, 8, 4096)), dim3(256),
                  (const uint32_t*)mr[m].ptr.p, (const uint32_t*)mr[m].idx.p, (const L*)mr[m].val.p, w, cmm,
                  (uint64_t)nb);

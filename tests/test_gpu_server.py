@@ -93,3 +93,105 @@ def test_single_proof_beside_a_live_server(orc):
     srv.close()
     assert during == before
     assert before == orc.general_prove(32, 5, A, B, Cm, w)
+
+
+def test_direct_calls_from_several_threads_match_sequential():
+    """Single-proof calls from four threads at once on different params (one
+    gets the device to itself and pipelines its rounds, the others wait their
+    turn) return the bytes of the same calls made one after another."""
+    import threading
+    print("direct calls: 4 threads x 3 proofs", flush=True)
+    specs = [(32, 12), (8, 11), (64, 10), (16, 12)]
+    made = []
+    for i, (k, log_n) in enumerate(specs):
+        A, B, Cm, w = general_instance(log_n, k, 70 + i)
+        made.append((na.GeneralSnarkParams.secure(log_n - 5, (A, B, Cm), 0xDEAD, 0xCAFE, k=k), w))
+    want = [na.prove_general(w, p) for p, w in made]
+    got = [[None] * 3 for _ in made]
+    errs = []
+
+    def work(i):
+        try:
+            p, w = made[i]
+            for r in range(3):
+                got[i][r] = na.prove_general(w, p)
+        except Exception as e:  # reported below
+            errs.append(e)
+
+    th = [threading.Thread(target=work, args=(i,)) for i in range(len(made))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not errs, errs
+    for i in range(len(made)):
+        assert all(g == want[i] for g in got[i]), f"thread {i}"
+
+
+def _general(k, log_n, seed):
+    A, B, Cm, w = general_instance(log_n, k, seed)
+    return na.GeneralSnarkParams.secure(log_n - 5, (A, B, Cm), 0xDEAD, 0xCAFE, k=k), w
+
+
+def test_job_over_budget_is_refused_at_submit():
+    print("server: budget 1 MiB, 2^10 job", flush=True)
+    p, w = _general(32, 10, 80)
+    srv = na.Server(workers=2, mem_budget=1 << 20)
+    with pytest.raises(na.NenyaError) as e:
+        srv.submit_general(p, w)
+    assert e.value.code == 2
+    srv.close()
+
+
+def test_cancel_queued_and_running_jobs_leaves_the_others_intact():
+    import time
+    print("server: 1 worker, cancel a queued job and a running one", flush=True)
+    big, wb = _general(64, 18, 81)
+    small, ws = _general(32, 11, 82)
+    want_small = na.prove_general(ws, small)
+    srv = na.Server(workers=1)
+    a = srv.submit_general(big, wb)
+    b = srv.submit_general(small, ws)
+    c = srv.submit_general(small, ws)
+    srv.cancel(b)  # queued behind the big job on the only worker
+    with pytest.raises(na.NenyaError) as e:
+        srv.wait(b)
+    assert e.value.code == 6
+    assert srv.wait(c)[0] == want_small
+    assert na.verify_general(big, srv.wait(a)[0]) == "Ok"
+    # a running job: the big proof takes long enough to be caught running
+    d = srv.submit_general(big, wb)
+    time.sleep(0.02)
+    try:
+        srv.cancel(d)
+        canceled = True
+    except na.NenyaError as err:  # it had finished already
+        assert err.code == 4
+        canceled = False
+    print(f"running job canceled: {canceled}", flush=True)
+    if canceled:
+        with pytest.raises(na.NenyaError) as e:
+            srv.wait(d)
+        assert e.value.code == 6
+    else:
+        srv.wait(d)
+    # the worker goes on proving correctly after the canceled proof
+    assert srv.wait(srv.submit_general(small, ws))[0] == want_small
+    srv.close()
+
+
+def test_injected_allocation_failure_fails_only_its_job(monkeypatch):
+    print("server: NENYA_INJECT_OOM=2:5, three jobs", flush=True)
+    ps = [_general(k, n, 83 + i) for i, (k, n) in enumerate([(32, 11), (16, 12), (8, 11)])]
+    want = [na.prove_general(w, p) for p, w in ps]
+    monkeypatch.setenv("NENYA_INJECT_OOM", "2:5")
+    srv = na.Server(workers=2)
+    ids = [srv.submit_general(p, w) for p, w in ps]
+    assert srv.wait(ids[0])[0] == want[0]
+    with pytest.raises(na.NenyaError) as e:
+        srv.wait(ids[1])
+    assert e.value.code == 2 and "injected" in str(e.value)
+    assert srv.wait(ids[2])[0] == want[2]
+    # the same params prove correctly afterwards on the same server
+    assert srv.wait(srv.submit_general(*ps[1]))[0] == want[1]
+    srv.close()
