@@ -181,6 +181,21 @@ int nenya_verify_simd(const nenya_params* p,
 int nenya_verify_mimd(const nenya_params* p, const uint8_t* proof, size_t proof_len, int* verdict);
 int nenya_verify_general(const nenya_params* p, const uint8_t* proof, size_t proof_len, int* verdict);
 
+/* ---- Proof container (no reference counterpart: the reference only
+ * serializes proof structs).  A 72-byte versioned header in front of the
+ * serialized proof: magic "NNYA", version 1, model (0 SIMD, 1 General,
+ * 2 MIMD), k, log_n / log_blocks, log_m, the two PCS seeds, the payload
+ * length and a BLAKE3 digest of the payload.  nenya_proof_container wraps
+ * the proof made with `p` (output freed with nenya_free);
+ * nenya_proof_payload checks a container's header and digest and points at
+ * its payload (NENYA_E_ARG when it is not an intact container).  The
+ * nenya_verify_* calls accept a container in place of the raw bytes; one
+ * made for params of another model, width, shape or seeds, or with a wrong
+ * digest, gets verdict 8. */
+int nenya_proof_container(const nenya_params* p, const uint8_t* proof, size_t len, uint8_t** out, size_t* out_len);
+int nenya_proof_payload(const uint8_t* c, size_t len, const uint8_t** payload, size_t* payload_len, int* model,
+                        int* k);
+
 /* ---- Batched and served proofs (no reference counterpart: the reference
  * proves one instance per call, general.rs:367-372, simd.rs:212-217,
  * mimd.rs:235-240).

@@ -30,6 +30,6 @@ fallback: without the library or a GPU the calls raise.
 """
 from ._lib import (  # noqa: F401
     DeviceArray, NenyaError, GeneralSnarkParams, MimdSnarkParams, Server, SimdR1csInstance, SimdSnarkParams,
-    device_copy, device_count, device_synchronize, lib_path, prove_general, prove_general_device, prove_mimd, prove_simd,
+    device_copy, device_count, device_synchronize, lib_path, proof_container, proof_payload, prove_general, prove_general_device, prove_mimd, prove_simd,
     verify_general, verify_mimd, verify_simd,
 )

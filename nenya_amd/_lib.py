@@ -60,6 +60,9 @@ def _lib():
         L.nenya_wait.argtypes = [vp, u64, C.POINTER(vp), C.POINTER(sz), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.nenya_job_release.argtypes = [vp, u64]
         L.nenya_cancel.argtypes = [vp, u64]
+        L.nenya_proof_container.argtypes = [vp, C.c_char_p, sz, C.POINTER(vp), C.POINTER(sz)]
+        L.nenya_proof_payload.argtypes = [C.c_char_p, sz, C.POINTER(vp), C.POINTER(sz), C.POINTER(C.c_int),
+                                          C.POINTER(C.c_int)]
         L.nenya_server_free.argtypes = [vp]
         L.nenya_params_free.argtypes = [vp]
         L.nenya_free.argtypes = [vp]
@@ -420,6 +423,24 @@ def prove_general_device_noalloc(dev_ptr, n_limbs, params, out=None):
     _check(_lib().nenya_prove_general_device_noalloc(params._h, C.c_void_p(dev_ptr), n_limbs, buf, len(out),
                                                      C.byref(need)))
     return bytes(out[:need.value])
+
+
+def proof_container(params, proof):
+    """The versioned container of a proof made with `params` (header + the
+    serialized bytes; nenya_proof_container)."""
+    out, n = C.c_void_p(), C.c_size_t()
+    b = bytes(proof)
+    _check(_lib().nenya_proof_container(params._h, b, len(b), C.byref(out), C.byref(n)))
+    return _take_proof(out, n)
+
+
+def proof_payload(container):
+    """(payload bytes, model, k) of an intact container (nenya_proof_payload);
+    raises NenyaError (code 4) otherwise."""
+    b = bytes(container)
+    pl, n, model, k = C.c_void_p(), C.c_size_t(), C.c_int(), C.c_int()
+    _check(_lib().nenya_proof_payload(b, len(b), C.byref(pl), C.byref(n), C.byref(model), C.byref(k)))
+    return b[72:72 + n.value], model.value, k.value
 
 
 VERIFY_ERRORS = {0: "Ok", 1: "MultSumcheck", 2: "LinSumcheck", 3: "LinClaimMismatch", 4: "LinARowMismatch",

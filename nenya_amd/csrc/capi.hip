@@ -5,6 +5,7 @@
 #include <unordered_map>
 
 #include "../../include/nenya.h"
+#include "container.hpp"
 #include "server.hpp"
 #include "verify.hpp"
 
@@ -521,6 +522,7 @@ int nenya_verify_simd(const nenya_params* p, const uint32_t* ar, const uint32_t*
                       const uint32_t* cc, const void* cv, size_t cn, const uint8_t* proof, size_t len, int* verdict) {
   return verify_common(p, 0, verdict, [&](ParamsBase* b) -> int {
     if (!proof && len) throw Error(NENYA_E_ARG, "null proof");
+    if (!container::unwrap_for(*b, proof, len)) return 8;
     if ((an && (!ar || !ac || !av)) || (bn && (!br || !bc || !bv)) || (cn && (!cr || !cc || !cv)))
       throw Error(NENYA_E_ARG, "null matrix array");
     auto run = [&](auto* sp) {
@@ -541,6 +543,7 @@ int nenya_verify_simd(const nenya_params* p, const uint32_t* ar, const uint32_t*
 int nenya_verify_mimd(const nenya_params* p, const uint8_t* proof, size_t len, int* verdict) {
   return verify_common(p, 2, verdict, [&](ParamsBase* b) -> int {
     if (!proof && len) throw Error(NENYA_E_ARG, "null proof");
+    if (!container::unwrap_for(*b, proof, len)) return 8;
     switch (b->k) {
       case 8: return Verifier<W<8>>::mimd(*static_cast<MimdParams<W<8>>*>(b), proof, len);
       case 16: return Verifier<W<16>>::mimd(*static_cast<MimdParams<W<16>>*>(b), proof, len);
@@ -553,6 +556,7 @@ int nenya_verify_mimd(const nenya_params* p, const uint8_t* proof, size_t len, i
 int nenya_verify_general(const nenya_params* p, const uint8_t* proof, size_t len, int* verdict) {
   return verify_common(p, 1, verdict, [&](ParamsBase* b) -> int {
     if (!proof && len) throw Error(NENYA_E_ARG, "null proof");
+    if (!container::unwrap_for(*b, proof, len)) return 8;
     switch (b->k) {
       case 8: return Verifier<W<8>>::general(*static_cast<GeneralParams<W<8>>*>(b), proof, len);
       case 16: return Verifier<W<16>>::general(*static_cast<GeneralParams<W<16>>*>(b), proof, len);
@@ -723,6 +727,32 @@ int nenya_job_release(nenya_server* s, uint64_t job) {
   return guard([&] {
     if (!s || !s->impl) throw Error(NENYA_E_ARG, "bad argument");
     s->impl->release(job);
+    return NENYA_OK;
+  });
+}
+
+int nenya_proof_container(const nenya_params* p, const uint8_t* proof, size_t len, uint8_t** out, size_t* out_len) {
+  return guard([&] {
+    if (!p || !p->impl || (!proof && len) || !out || !out_len) throw Error(NENYA_E_ARG, "bad argument");
+    if (container::is_container(proof, len)) throw Error(NENYA_E_ARG, "already a container");
+    uint8_t* b = (uint8_t*)std::malloc(container::HEADER + len);
+    if (!b) throw std::bad_alloc();
+    container::wrap(*p->impl, proof, len, b);
+    *out = b;
+    *out_len = container::HEADER + len;
+    return NENYA_OK;
+  });
+}
+
+int nenya_proof_payload(const uint8_t* c, size_t len, const uint8_t** payload, size_t* payload_len, int* model,
+                        int* k) {
+  return guard([&] {
+    if (!c || !payload || !payload_len) throw Error(NENYA_E_ARG, "bad argument");
+    container::Info in;
+    if (!container::parse(c, len, in, payload)) throw Error(NENYA_E_ARG, "not an intact proof container");
+    *payload_len = (size_t)in.len;
+    if (model) *model = in.model;
+    if (k) *k = in.k;
     return NENYA_OK;
   });
 }

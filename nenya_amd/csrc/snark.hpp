@@ -56,6 +56,10 @@ inline bool skip_selfcheck() {
 struct ParamsBase {
   int k = 32;
   int kind = 0;  // 0 simd, 1 general, 2 mimd
+  // the shape and seeds the params were made with (log_n, log_m for SIMD and
+  // MIMD; log_blocks, 0 for General), recorded in proof containers
+  int shape[2] = {0, 0};
+  uint64_t seeds[2] = {0, 0};
   // device bytes of one proof: an estimate until a proof has run, then the
   // largest measured pool growth of a proof
   std::atomic<size_t> footprint{0};
@@ -237,6 +241,10 @@ template <class T> struct SimdParams : ParamsBase {
   SimdParams(int ln, int lm, uint64_t seed_w, uint64_t seed_q) {
     k = T::K;
     kind = 0;
+    shape[0] = ln;
+    shape[1] = lm;
+    seeds[0] = seed_w;
+    seeds[1] = seed_q;
     log_n = ln;
     log_m = lm;
     pw.P = PcsParams::setup(lm, seed_w, T::K);
@@ -431,6 +439,10 @@ template <class T> struct MimdParams : ParamsBase {
   MimdParams(int ln, int lm, const MimdSet* sets, uint64_t seed_w, uint64_t seed_e) {
     k = T::K;
     kind = 2;
+    shape[0] = ln;
+    shape[1] = lm;
+    seeds[0] = seed_w;
+    seeds[1] = seed_e;
     log_n = ln;
     log_m = lm;
     n = size_t(1) << ln;
@@ -780,6 +792,9 @@ template <class T> struct GeneralParams : ParamsBase {
   GeneralParams(int lb, const HostCoo* m3, uint64_t seed_w, uint64_t seed_e) {
     k = T::K;
     kind = 1;
+    shape[0] = lb;
+    seeds[0] = seed_w;
+    seeds[1] = seed_e;
     log_b = lb;
     nb = size_t(1) << lb;
     N = 32 * nb;

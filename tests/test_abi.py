@@ -83,3 +83,28 @@ def test_proof_view_keeps_its_params_alive():
     del v
     gc.collect()
     assert ref() is None
+
+
+def _container(model, k, s0, s1, seed_w, seed_e, payload, orc):
+    import struct
+    head = b"NNYA" + struct.pack("<HBBBB6xQQQ", 1, model, k, s0, s1, seed_w, seed_e, len(payload))
+    return head + orc.blake3(payload, 32) + payload
+
+
+def test_proof_container_format(orc):
+    """nenya_proof_payload accepts the documented layout (digest from the
+    oracle's BLAKE3) and refuses a wrong digest, version or length."""
+    payload = bytes(range(256)) * 5
+    c = _container(1, 32, 15, 0, 0xDEAD, 0xCAFE, payload, orc)
+    got, model, k = na.proof_payload(c)
+    assert got == payload and model == 1 and k == 32
+    bad = bytearray(c)
+    bad[100] ^= 1  # payload byte: digest mismatch
+    with pytest.raises(na.NenyaError):
+        na.proof_payload(bytes(bad))
+    bad = bytearray(c)
+    bad[4] = 2  # version
+    with pytest.raises(na.NenyaError):
+        na.proof_payload(bytes(bad))
+    with pytest.raises(na.NenyaError):
+        na.proof_payload(c[:-1])  # length

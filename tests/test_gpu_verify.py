@@ -91,3 +91,27 @@ def test_verify_simd_larger_accepts(orc, k, log):
     bad = bytearray(p)
     bad[len(p) // 2] ^= 1
     assert na.verify_simd(inst, params, bytes(bad)) == orc.simd_verify(k, log, log, A, B, Cm, bytes(bad))
+
+
+def test_verifiers_accept_containers_of_their_params_only():
+    """A proof in the versioned container verifies like the raw bytes; a
+    container for other params (seeds), with a changed payload byte, or
+    truncated gets verdict Parse."""
+    print("containers: general k=32 log_n=10, simd k=16 log=6", flush=True)
+    A, B, Cm, w = general_instance(10, 32, 91)
+    p = na.GeneralSnarkParams.secure(5, (A, B, Cm), 0xDEAD, 0xCAFE, k=32)
+    other = na.GeneralSnarkParams.secure(5, (A, B, Cm), 0xDEAD, 0xBEEF, k=32)
+    raw = na.prove_general(w, p)
+    c = na.proof_container(p, raw)
+    assert len(c) == len(raw) + 72 and na.proof_payload(c) == (raw, 1, 32)
+    assert na.verify_general(p, c) == "Ok"
+    assert na.verify_general(other, c) == "Parse"
+    bad = bytearray(c)
+    bad[72 + 5000] ^= 1
+    assert na.verify_general(p, bytes(bad)) == "Parse"
+    assert na.verify_general(p, c[:-3]) == "Parse"
+    A, B, Cm, wit = simd_instance(6, 16, 92)
+    ps = na.SimdSnarkParams.secure(6, 6, 0xDEAD, 0xCAFE, k=16)
+    inst = na.SimdR1csInstance(A, B, Cm, wit)
+    cs = na.proof_container(ps, na.prove_simd(inst, ps))
+    assert na.verify_simd(inst, ps, cs) == "Ok"
