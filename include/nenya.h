@@ -226,7 +226,10 @@ int nenya_job_release(nenya_server* s, uint64_t job);
  * server's budget is refused at submit with NENYA_E_OOM.  Test switch:
  * NENYA_INJECT_OOM=<job>:<n>, read when a server is created, makes the n-th
  * device allocation of that job (job ids count from 1) fail with
- * NENYA_E_OOM. */
+ * NENYA_E_OOM.  NENYA_DEVICE_FS=1, read when a server is created, makes its
+ * workers draw every sumcheck round challenge on the device (the
+ * transcript's BLAKE3 state and the ChaCha12 sampler run in the kernel that
+ * lifts the challenge), so rounds need no host round trip; same bytes. */
 int nenya_cancel(nenya_server* s, uint64_t job);
 void nenya_server_free(nenya_server* s);
 

@@ -129,8 +129,10 @@ __host__ __device__ inline void hash64(const uint32_t l[8], const uint32_t r[8],
   for (int i = 0; i < 8; i++) out[i] = o[i];
 }
 
-// ---------------- host incremental hasher with XOF ----------------
-struct Hasher {
+// ---------------- incremental hasher with XOF ----------------
+// The hasher's state as plain data, identical on host and device, so a
+// transcript can move between them as bytes.
+struct State {
   uint32_t key[8];
   uint32_t cv[8];
   uint64_t chunk;
@@ -139,7 +141,10 @@ struct Hasher {
   uint32_t blocks;
   uint32_t stack[56][8];
   int sp;
+  int pad_;
+};
 
+struct Hasher : State {
   Hasher() { reset(); }
   void reset() {
     iv(key);
@@ -149,6 +154,7 @@ struct Hasher {
     blen = 0;
     blocks = 0;
     sp = 0;
+    pad_ = 0;
   }
   static void words(const uint8_t* b, uint32_t* w) {
     for (int i = 0; i < 16; i++)

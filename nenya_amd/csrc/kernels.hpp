@@ -685,8 +685,10 @@ __global__ void __launch_bounds__(1024) k_sum_post(const LT<T>* partials, uint64
   __syncthreads();
   if (threadIdx.x == 0) {
     if (!single) *count = 0;
-    __threadfence_system();
-    __hip_atomic_store(flag, val, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (flag) {  // no flag: the round's Fiat-Shamir runs on the device
+      __threadfence_system();
+      __hip_atomic_store(flag, val, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
   }
 }
 

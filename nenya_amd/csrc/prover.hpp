@@ -12,6 +12,7 @@
 #include <map>
 #include <thread>
 
+#include "fsdev.hpp"
 #include "host.hpp"
 #include "kernels.hpp"
 #include "mfma.hpp"
@@ -268,6 +269,11 @@ template <class T> struct Engine {
   void checkpoint() const {
     if (cancel && cancel->load(std::memory_order_relaxed)) throw Error(NENYA_E_CANCELED, "job canceled");
   }
+  // Contexts that do not pipeline may draw their round challenges on the
+  // device (fsdev.hpp): no host round trip per round, at the price of the
+  // transcript's BLAKE3 work in the lift kernel (servers created with
+  // NENYA_DEVICE_FS=1).
+  bool devfs = false;
   // No-alloc calls: one stream only (no forks, downloads in order on the
   // prover stream), so no stream or event is created during the call.
   bool serial = false;
@@ -811,6 +817,36 @@ template <class T> struct Engine {
            d_sums.p, qd, mbd<L>(mb_down(s)), d_count.p, mbd<uint32_t>(mb_down_flag(s)), v);
     return v;
   }
+  // Enqueue: reduce a round's partial sums into q at qh (device), for the
+  // device transcript (no host posting).
+  void post_round_dev(const L* part, uint64_t nparts, int ns, int mode, const L* c, L* qd, L* qh) {
+    const unsigned grid = nparts <= 64 ? 1 : ns * 4;
+    launch("k_sum_post", (double)nparts * ns * EB, 0, (k_sum_post<T>), dim3(grid), dim3(1024), part, nparts, ns, mode, c,
+           d_sums.p, qd, qh, d_count.p, (uint32_t*)nullptr, 0u);
+  }
+  // Enqueue: absorb q (device) into the device transcript st, draw the
+  // round challenge there and lift it into d (with k <= 32 its fragments go
+  // to frag and mm points at them).
+  void lift_device_fs(b3::State* st, const L* q, L* d, uint32_t* frag, MulMat* mm) {
+    const FsArg<T> f{st, q};
+    const double lm = E * E + (T::K - 1) * 16.5 * PM;
+    mm->r = d;
+    if constexpr (kMx<T>) {
+      mm->fp = frag;
+      launch("k_lift_mulmat_fs", 5 * EB, lm, (k_lift_mulmat_fs<T>), dim3(1), dim3(256), f, d, d_frob.p, T::K - 1, frag);
+    } else {
+      launch("k_lift_fs", 5 * EB, lm, (k_lift_fs<T>), dim3(1), dim3(256), f, d, d_frob.p, T::K - 1);
+    }
+  }
+  // The device transcript of a sumcheck: the host transcript's state
+  // uploaded before the rounds, fetched back with the sumcheck's tail.
+  Pinned trbuf;
+  void fetch_transcript_async(const b3::State* d) {
+    uint8_t* h = trbuf.get(sizeof(b3::State));
+    HIP_CHECK(hipMemcpyAsync(h, d, sizeof(b3::State), hipMemcpyDeviceToHost, st));
+  }
+  void adopt_transcript(Transcript& tr) { std::memcpy(static_cast<b3::State*>(&tr.h), trbuf.p, sizeof(b3::State)); }
+
   // Host: wait for slot s's round polynomial (bounded; device errors raise).
   void wait_round(int s, uint32_t v, G q[3]) {
     const uint32_t* f = mbh<uint32_t>(mb_down_flag(s));
@@ -1244,6 +1280,11 @@ template <class T> struct Engine {
     DBuf<uint32_t> frags(pool, kMx<T> ? nv * kND<T> * 4096 : 1);
     MbGuard guard(*this);
     const bool pipe = pipeline_rounds();
+    // contexts that do not pipeline: Fiat-Shamir of the rounds on the device
+    const bool dfs = !pipe && devfs;
+    DBuf<L> drounds(pool, dfs ? 3 * nv * E : 1);
+    DBuf<b3::State> dtr(pool, 1);
+    if (dfs) h2d(dtr.p, static_cast<const b3::State*>(&tr.h), sizeof(b3::State));
     wait_ticks = spin_ticks();
     // enqueue the sums of round r over tables of 2m entries; returns the flag value
     auto round_sums = [&](size_t r, size_t m) -> uint32_t {
@@ -1257,6 +1298,10 @@ template <class T> struct Engine {
       else
         launch("k_mult_round", m * (2.0 * ntab + 1) * EB, m * 6 * 32 * PM, (k_mult_round<T, false>), dim3(nbk),
                dim3(GB), (const L*)eg.p, cur[0], cur[1], c3, e3, (uint64_t)m, part.p);
+      if (dfs) {
+        post_round_dev(part.p, nbk, 3, 0, nullptr, nullptr, drounds.p + r * 3 * E);
+        return 0u;
+      }
       return post_round(part.p, nbk, 3, 0, nullptr, nullptr, (int)r);
     };
     uint32_t vdown = round_sums(0, len / 2);
@@ -1274,8 +1319,13 @@ template <class T> struct Engine {
         for (int i = 0; i < 3; i++) o.rounds.push_back(q[i]);
         post_challenge(tr, "sc/challenge", (int)r, vup);
       };
-      if (!pipe) host_round();
-      lift_posted((int)r, vup, rc, frags.p + (kMx<T> ? r * kND<T> * 4096 : 0), &mr);
+      if (dfs) {
+        checkpoint();
+        lift_device_fs(dtr.p, drounds.p + r * 3 * E, rc, frags.p + (kMx<T> ? r * kND<T> * 4096 : 0), &mr);
+      } else {
+        if (!pipe) host_round();
+        lift_posted((int)r, vup, rc, frags.p + (kMx<T> ? r * kND<T> * 4096 : 0), &mr);
+      }
       // the next round's eq factor eq_one(rt[r], r_r) rides along with the fold
       Side sd;
       if (r + 1 < nv) sd = Side{1, drt.p + r * E, rc, dsc.p};
@@ -1298,11 +1348,18 @@ template <class T> struct Engine {
       if (pipe) host_round();
       vdown = vnext;
     }
-    // challenges and finals with one synchronization
-    std::vector<G> tail = pick_many({{dch.p, 1, nv}, {work.p, N / 2, (uint64_t)ntab}});
+    // challenges and finals (and, with the device transcript, the round
+    // polynomials and the transcript state) with one synchronization
+    if (dfs) fetch_transcript_async(dtr.p);
+    std::vector<G> tail =
+        pick_many({{dch.p, 1, nv}, {work.p, N / 2, (uint64_t)ntab}, {drounds.p, 1, dfs ? 3 * (uint64_t)nv : 0}});
     check_waits();
     o.chal.assign(tail.begin(), tail.begin() + nv);
-    o.finals.assign(tail.begin() + nv, tail.end());
+    o.finals.assign(tail.begin() + nv, tail.begin() + nv + ntab);
+    if (dfs) {
+      o.rounds.assign(tail.begin() + nv + ntab, tail.end());
+      adopt_transcript(tr);
+    }
     return o;
   }
 
@@ -1331,11 +1388,22 @@ template <class T> struct Engine {
     DBuf<uint32_t> frags(pool, kMx<T> ? nv * kND<T> * 4096 : 1);
     MbGuard guard(*this);
     const bool pipe = pipeline_rounds();
+    const bool dfs = !pipe && devfs;  // as in mult_check
+    DBuf<L> drounds(pool, dfs ? 3 * nv * E : 1);
+    DBuf<b3::State> dtr(pool, 1);
+    if (dfs) h2d(dtr.p, static_cast<const b3::State*>(&tr.h), sizeof(b3::State));
     wait_ticks = spin_ticks();
+    auto post = [&](size_t r, unsigned nparts) -> uint32_t {
+      if (dfs) {
+        post_round_dev(part.p, nparts, 2, 1, dc.p, dq.p, drounds.p + r * 3 * E);
+        return 0u;
+      }
+      return post_round(part.p, nparts, 2, 1, dc.p, dq.p, (int)r);
+    };
     auto round_sums = [&](size_t r, size_t m) -> uint32_t {
       if constexpr (kMx<T>) {
         const unsigned nwg = gram_lin(f, fs, g, gs, (uint64_t)ntab, (uint64_t)m, (r == 0 && NAF0 == 16) ? 64 : 128, part.p);
-        return post_round(part.p, nwg, 2, 1, dc.p, dq.p, (int)r);
+        return post(r, nwg);
       } else {
         const unsigned nbk = grid_for(m * ntab, GPB, 2048);
         double lb = m * ntab * 4.0 * EB;
@@ -1345,7 +1413,7 @@ template <class T> struct Engine {
         else
           launch("k_lin_round", lb, m * ntab * 2 * 32 * PM, (k_lin_round<T, 32>), dim3(nbk), dim3(GB), f, fs, g, gs,
                  (uint64_t)ntab, (uint64_t)m, part.p);
-        return post_round(part.p, nbk, 2, 1, dc.p, dq.p, (int)r);
+        return post(r, nbk);
       }
     };
     uint32_t vdown = round_sums(0, len / 2);
@@ -1362,8 +1430,13 @@ template <class T> struct Engine {
         for (int i = 0; i < 3; i++) o.rounds.push_back(q[i]);
         post_challenge(tr, "sc/challenge", (int)r, vup);
       };
-      if (!pipe) host_round();
-      lift_posted((int)r, vup, rc, frags.p + (kMx<T> ? r * kND<T> * 4096 : 0), &mr);
+      if (dfs) {
+        checkpoint();
+        lift_device_fs(dtr.p, drounds.p + r * 3 * E, rc, frags.p + (kMx<T> ? r * kND<T> * 4096 : 0), &mr);
+      } else {
+        if (!pipe) host_round();
+        lift_posted((int)r, vup, rc, frags.p + (kMx<T> ? r * kND<T> * 4096 : 0), &mr);
+      }
       // the next claim q_r(r_r) rides along with the folds
       fold2(mr, m, f, fw.p, ntab, fs, H, g, gw.p, ntab, gs, H, Side{2, dq.p, rc, dc.p});
       f = fw.p;
@@ -1374,10 +1447,16 @@ template <class T> struct Engine {
       if (pipe) host_round();
       vdown = vnext;
     }
-    std::vector<G> tail = pick_many({{dch.p, 1, nv}, {f, fs, (uint64_t)ntab}, {g, gs, (uint64_t)ntab}});
+    if (dfs) fetch_transcript_async(dtr.p);
+    std::vector<G> tail = pick_many({{dch.p, 1, nv}, {f, fs, (uint64_t)ntab}, {g, gs, (uint64_t)ntab},
+                                     {drounds.p, 1, dfs ? 3 * (uint64_t)nv : 0}});
     check_waits();
     o.chal.assign(tail.begin(), tail.begin() + nv);
-    o.finals.assign(tail.begin() + nv, tail.end());
+    o.finals.assign(tail.begin() + nv, tail.begin() + nv + 2 * (size_t)ntab);
+    if (dfs) {
+      o.rounds.assign(tail.begin() + nv + 2 * (size_t)ntab, tail.end());
+      adopt_transcript(tr);
+    }
     return o;
   }
 };

@@ -51,7 +51,8 @@ struct Worker {
   Pool pool;  // destroyed after the contexts
   hipStream_t st = nullptr;
   std::unique_ptr<CtxBase> ctx[4];  // per width: 8, 16, 32, 64
-  Worker() { HIP_CHECK(hipStreamCreate(&st)); }
+  bool devfs = false;               // round challenges drawn on the device
+  explicit Worker(bool dfs) : devfs(dfs) { HIP_CHECK(hipStreamCreate(&st)); }
   ~Worker() {
     for (auto& c : ctx) c.reset();
     if (st) (void)hipStreamDestroy(st);
@@ -62,6 +63,7 @@ struct Worker {
       ctx[i].reset(new Ctx<T>(st, pool));
       // several workers drive the device: no kernel waits on a host thread
       static_cast<Ctx<T>&>(*ctx[i]).eng->pipelined = false;
+      static_cast<Ctx<T>&>(*ctx[i]).eng->devfs = devfs;
     }
     return static_cast<Ctx<T>&>(*ctx[i]);
   }
@@ -101,7 +103,8 @@ struct Server {
         inject_at = at;
       }
     }
-    for (int i = 0; i < nworkers; i++) workers.emplace_back(new Worker());
+    const char* dfs = std::getenv("NENYA_DEVICE_FS");
+    for (int i = 0; i < nworkers; i++) workers.emplace_back(new Worker(dfs && dfs[0] == '1'));
     live_servers()++;  // before any job can be submitted
     for (int i = 0; i < nworkers; i++) threads.emplace_back([this, i] { loop(i); });
   }

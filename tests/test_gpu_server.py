@@ -195,3 +195,35 @@ def test_injected_allocation_failure_fails_only_its_job(monkeypatch):
     # the same params prove correctly afterwards on the same server
     assert srv.wait(srv.submit_general(*ps[1]))[0] == want[1]
     srv.close()
+
+
+def test_device_fiat_shamir_server_matches_direct_path(monkeypatch):
+    """Workers that draw the round challenges on the device (BLAKE3 absorb
+    and squeeze, ChaCha12 sampling and lift in one kernel) produce the bytes
+    of the direct path, which runs the transcript on the host."""
+    print("server with NENYA_DEVICE_FS=1: general/simd/mimd, all widths", flush=True)
+    jobs = []
+    for i, (model, k, log) in enumerate([("g", 32, 11), ("g", 64, 10), ("s", 8, 7), ("s", 16, 6), ("m", 64, 6),
+                                         ("m", 32, 5), ("g", 8, 12), ("g", 16, 10)]):
+        if model == "g":
+            A, B, Cm, w = general_instance(log, k, 60 + i)
+            p = na.GeneralSnarkParams.secure(log - 5, (A, B, Cm), 0xDEAD, 0xCAFE, k=k)
+            jobs.append(("g", p, w, na.prove_general(w, p)))
+        elif model == "m":
+            As, Bs, Cs, wit = mimd_instance(log, k, 60 + i)
+            p = na.MimdSnarkParams.secure(log, log, (As, Bs, Cs), 0xDEAD, 0xCAFE, k=k)
+            jobs.append(("m", p, wit, na.prove_mimd(wit, p)))
+        else:
+            A, B, Cm, wit = simd_instance(log, k, 60 + i)
+            p = na.SimdSnarkParams.secure(log, log, 0xDEAD, 0xCAFE, k=k)
+            inst = na.SimdR1csInstance(A, B, Cm, wit)
+            jobs.append(("s", p, inst, na.prove_simd(inst, p)))
+    monkeypatch.setenv("NENYA_DEVICE_FS", "1")
+    srv = na.Server(workers=3)
+    ids = []
+    for model, p, x, _ in jobs:
+        ids.append(srv.submit_general(p, x) if model == "g" else srv.submit_mimd(p, x) if model == "m"
+                   else srv.submit_simd(x, p))
+    for jid, (model, p, x, want) in zip(ids, jobs):
+        assert srv.wait(jid)[0] == want, model
+    srv.close()
