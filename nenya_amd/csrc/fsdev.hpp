@@ -102,6 +102,36 @@ __device__ __forceinline__ void compress(const uint32_t* cv, const uint32_t* m, 
   wsync();
 }
 
+// The same compression with this quad's chaining value in registers: lane c
+// holds cv[c] in ca and cv[4+c] in cb, and receives the new chaining value
+// (output words c and 4+c) there.  m: 16 words in LDS.  Only the quads that
+// call it take part (each quad is entirely active or not).
+__device__ __forceinline__ void compress_cv(uint32_t& ca, uint32_t& cb, const uint32_t* m, uint64_t counter,
+                                            uint32_t blen, uint32_t flags) {
+  const int c = threadIdx.x & 3;
+  uint32_t a = ca, b = cb;
+  uint32_t cc = c == 0 ? 0x6A09E667u : c == 1 ? 0xBB67AE85u : c == 2 ? 0x3C6EF372u : 0xA54FF53Au;
+  uint32_t d = c == 0 ? (uint32_t)counter : c == 1 ? (uint32_t)(counter >> 32) : c == 2 ? blen : flags;
+  uint32_t mx[28];
+#pragma unroll
+  for (int r = 0; r < 7; r++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) mx[4 * r + j] = m[sched_idx(r, j, c)];
+#pragma unroll
+  for (int r = 0; r < 7; r++) {
+    g_mix(a, b, cc, d, mx[4 * r], mx[4 * r + 1]);
+    b = qperm<Q1>(b);
+    cc = qperm<Q2>(cc);
+    d = qperm<Q3>(d);
+    g_mix(a, b, cc, d, mx[4 * r + 2], mx[4 * r + 3]);
+    b = qperm<Q3>(b);
+    cc = qperm<Q2>(cc);
+    d = qperm<Q1>(d);
+  }
+  ca = a ^ cc;
+  cb = b ^ d;
+}
+
 // Scratch of the wave-level hasher (LDS).
 struct Work {
   uint32_t out[16];
@@ -165,6 +195,74 @@ __device__ __forceinline__ void update(b3::State* S, const uint8_t* msg, uint32_
     S->blen = blen;
     S->blocks = blocks;
     S->chunk = chunk;
+    S->sp = sp;
+  }
+  wsync();
+}
+
+// Appends len bytes of msg (LDS) to S like update(), with the chunks the
+// input completes compressed in parallel: the current block's bytes and msg
+// are laid out from the current block's start in buf (LDS, 64-byte blocks,
+// zero padded); every block but the last is compressed, quad q taking the
+// blocks of the q-th chunk from the current one (its chaining value in
+// registers), and the last block stays buffered, as the lazy host hasher
+// leaves it.  Then the finished chunks' chaining values enter the CV stack in
+// order.  At most 16 chunks (len < 15 KiB).
+__device__ __forceinline__ void absorb(b3::State* S, const uint8_t* msg, uint32_t len, Work* w, uint8_t* buf,
+                                       uint32_t (*cvs)[8]) {
+  const int lane = threadIdx.x & 63, c = lane & 3, q = lane >> 2;
+  if (len == 0) return;
+  const uint32_t blen0 = S->blen, blocks0 = S->blocks;
+  const uint64_t chunk0 = S->chunk;
+  const uint32_t tot = blen0 + len, nbk = (tot + 63) / 64, ncomp = nbk - 1;
+  for (uint32_t i = lane; i < 64 * nbk; i += 64) buf[i] = i < blen0 ? S->block[i] : i < tot ? msg[i - blen0] : (uint8_t)0;
+  wsync();
+  const uint32_t last = (blocks0 + ncomp) / 16;  // chunk (from the current one) of the buffered block
+  if ((uint32_t)q <= last) {
+    uint32_t ca = q == 0 ? S->cv[c] : S->key[c], cb = q == 0 ? S->cv[4 + c] : S->key[4 + c];
+    // blocks j of chunk q: blocks0 + j in [16q, 16q + 16), j < ncomp
+    const uint32_t jb = q == 0 ? 0u : 16u * q - blocks0;
+    const uint32_t je = min(16u * (q + 1) - blocks0, ncomp);
+    for (uint32_t j = jb; j < je; j++) {
+      const uint32_t wb = (blocks0 + j) & 15u;
+      compress_cv(ca, cb, reinterpret_cast<const uint32_t*>(buf + 64 * j), chunk0 + q, 64,
+                  (wb == 0 ? b3::CHUNK_START : 0u) | (wb == 15 ? b3::CHUNK_END : 0u));
+    }
+    if ((lane & ~3) == 4 * q) {  // one quad per chunk writes
+      if ((uint32_t)q < last) {
+        cvs[q][c] = ca;
+        cvs[q][4 + c] = cb;
+      } else {
+        S->cv[c] = ca;
+        S->cv[4 + c] = cb;
+      }
+    }
+  }
+  wsync();
+  int sp = S->sp;
+  for (uint32_t k = 0; k < last; k++) {
+    if (lane < 8) w->out[lane] = cvs[k][lane];
+    wsync();
+    uint64_t total = chunk0 + k + 1;
+    while ((total & 1) == 0) {
+      sp--;
+      if (lane < 8) {
+        w->pm[lane] = S->stack[sp][lane];
+        w->pm[8 + lane] = w->out[lane];
+      }
+      wsync();
+      compress(S->key, w->pm, 0, 64, b3::PARENT, w->out);
+      total >>= 1;
+    }
+    if (lane < 8) S->stack[sp][lane] = w->out[lane];
+    wsync();
+    sp++;
+  }
+  if (lane < 16) reinterpret_cast<uint32_t*>(S->block)[lane] = reinterpret_cast<const uint32_t*>(buf + 64 * ncomp)[lane];
+  if (lane == 0) {
+    S->blen = tot - 64 * ncomp;
+    S->blocks = blocks0 + ncomp - 16 * last;
+    S->chunk = chunk0 + last;
     S->sp = sp;
   }
   wsync();
@@ -247,12 +345,15 @@ __device__ __forceinline__ void chacha12(const uint32_t key[8], int nblocks, uin
 
 // LDS of one device Fiat-Shamir round.
 template <class T> struct RoundLds {
+  static constexpr int MSG = 40 + 3 * (1 + E * sizeof(LT<T>));  // round label, count, q, challenge label
   b3::State st;
   Work w;
   uint32_t ocv[8];
   uint32_t key[8];
   uint32_t ks[256];
-  uint8_t msg[20 + 3 * (1 + E * sizeof(LT<T>))];
+  uint32_t cvs[16][8];
+  uint8_t msg[MSG];
+  __attribute__((aligned(16))) uint8_t buf[(MSG + 64 + 63) / 64 * 64];
 };
 
 // Label bytes "L" || u32 len || label at dst; returns the length.
@@ -291,11 +392,9 @@ __device__ __forceinline__ void round_challenge(b3::State* gst, const LT<T>* q, 
     L.msg[n + i] = o == 0 ? (uint8_t)'E' : qb[e * EB + o - 1];
   }
   n += 3 * (1 + EB);
+  n += put_label(L.msg + n, "sc/challenge", 12, lane);
   wsync();
-  update(&L.st, L.msg, n, &L.w);
-  n = put_label(L.msg, "sc/challenge", 12, lane);
-  wsync();
-  update(&L.st, L.msg, n, &L.w);
+  absorb(&L.st, L.msg, n, &L.w, L.buf, L.cvs);
   const uint64_t seed = squeeze8(&L.st, &L.w, L.ocv);
   if (lane < 9) L.msg[lane] = lane == 0 ? (uint8_t)'S' : (uint8_t)(seed >> (8 * (lane - 1)));
   wsync();

@@ -509,7 +509,20 @@ template <class T> struct Engine {
   template <int NA> void mul_sub(const L* A, const L* B, const L* S, L* out, uint64_t n) {
     launch("k_mul_sub", n * 4 * EB, n * NA * PM, (k_mul_sub<T, NA>), dim3(grid_for(n, GPB)), dim3(GB), A, B, S, out, n);
   }
+  // out[i] = s X[i]: with k <= 32 and many elements, x -> s x as
+  // matrix-core fragments (k_mulmat) applied to every X[i] (one fixed
+  // matrix, k_mulfix_mx); otherwise group products on the vector ALUs.
   void scale(const L* s, const L* X, L* out, uint64_t n) {
+    if constexpr (kMx<T>) {
+      if (n >= 1024) {
+        DBuf<uint32_t> fr(pool, (size_t)kND<T> * 4096);
+        launch("k_mulmat", EB + kND<T> * 16384.0, 0, (k_mulmat<T>), dim3(1), dim3(256), s, fr.p);
+        const uint64_t ntiles = (n + 31) / 32;
+        launch("k_scale_mx", n * 2 * EB, (double)n * E * E, (k_mulfix_mx<T>), dim3(grid_for(ntiles, 4, 1024), 1),
+               dim3(256), (const uint32_t*)fr.p, X, n, out);
+        return;
+      }
+    }
     launch("k_scale", n * 2 * EB, n * 32 * PM, (k_scale<T>), dim3(grid_for(n, GPB)), dim3(GB), s, X, out, n);
   }
   void sub(const L* a, const L* b, L* o, uint64_t nl) {
