@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
 INT8_PEAK_TOPS = 5000.0    # MI355X dense int8 MFMA, TOP/s
-MX_KERNELS = ("k_fold_mx", "k_gram_lin", "k_mulfix_mx", "k_cgemm")  # int8 digit-plane GEMMs
+MX_KERNELS = ("k_fold_mx", "k_gram_lin", "k_mulfix_mx", "k_cgemm", "k_scale_mx")  # int8 digit-plane GEMMs
 
 
 def log(*a):
@@ -329,6 +329,7 @@ def main():
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the width sweep, the MIMD serving mix and the 2^25 General proof")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch shapes")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a 256 MiB device copy (PMC calibration of FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -350,6 +351,21 @@ def main():
     log(f"setup (diagonal packing + PCS params) {time.perf_counter() - t0:.1f}s")
     wd = na.DeviceArray(as_limbs(w, k))
     na.device_synchronize()
+    # launch shapes of the sumcheck kernels picked by the autotuner (every
+    # width, fold / Gram work sizes 2^8 .. 2^20); all later proofs use them
+    tune = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        rep = na.autotune(log_min=8, log_max=20)
+        tune = {"seconds": round(time.perf_counter() - t0, 3), "picks": {}}
+        for kk, kernels in rep.items():
+            if kk == "seconds":
+                continue
+            tune["picks"][kk] = {kern: {sz: (f"{e['pick']}/{e['workgroups']}" if "pick" in e else e["workgroups"])
+                                        for sz, e in d.items()} for kern, d in kernels.items()}
+            tune["picks"][kk]["us_saved_vs_builtin"] = {
+                kern: round(sum(max(0.0, e["builtin_us"] - e["us"]) for e in d.values()), 1) for kern, d in kernels.items()}
+        log(f"autotune: {tune['seconds']} s")
     n = len(w)
 
     if args.copy_calib:
@@ -523,6 +539,7 @@ def main():
                              "nenya_free; dropin_bytes_ms also copies it into a Python bytes object; "
                              "dropin_view_ms leaves it as a view of the params' buffer"},
         "roofline": roofline,
+        "autotune": tune,
         "cpu_baseline": cpu,
         "stages_ms": {k2: round(v, 3) for k2, v in stages.items()},
         "batch": batch,

@@ -233,6 +233,17 @@ int nenya_job_release(nenya_server* s, uint64_t job);
 int nenya_cancel(nenya_server* s, uint64_t job);
 void nenya_server_free(nenya_server* s);
 
+/* Autotuner (no reference counterpart): for the widths in k_mask (bit 0 k=8,
+ * bit 1 k=16, bit 2 k=32, bit 3 k=64) and every work size 2^log_min ..
+ * 2^log_max elements (8 <= log_min <= log_max <= 22), times the candidate
+ * launch shapes of the sumcheck folds (k_fold_mx split or bulk mode and its
+ * workgroup count; k_fold's workgroup cap at k=64) and of the linear-check
+ * Gram sums (k_gram_lin workgroups) on synthetic tables, and makes the
+ * fastest the shape every later launch of the process uses.  The picks,
+ * with their times and the built-in choices' times, go to `report` as JSON
+ * (up to report_cap bytes). */
+int nenya_autotune(int k_mask, int log_min, int log_max, char* report, size_t report_cap);
+
 /* Device-to-device copy of `bytes` bytes by the library's copy kernel
  * (2048 workgroups at most), synchronized.  No reference counterpart: a
  * transfer of known size for calibrating hardware traffic counters. */

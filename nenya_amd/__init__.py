@@ -29,7 +29,7 @@ Every call runs the HIP library nenya_amd/libnenya_amd.so; there is no CPU
 fallback: without the library or a GPU the calls raise.
 """
 from ._lib import (  # noqa: F401
-    DeviceArray, NenyaError, GeneralSnarkParams, MimdSnarkParams, Server, SimdR1csInstance, SimdSnarkParams,
+    DeviceArray, NenyaError, autotune, GeneralSnarkParams, MimdSnarkParams, Server, SimdR1csInstance, SimdSnarkParams,
     device_copy, device_count, device_synchronize, lib_path, proof_container, proof_payload, prove_general, prove_general_device, prove_mimd, prove_simd,
     verify_general, verify_mimd, verify_simd,
 )

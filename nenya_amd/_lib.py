@@ -69,6 +69,7 @@ def _lib():
         L.nenya_last_error.restype = C.c_char_p
         L.nenya_device_count.restype = C.c_int
         L.nenya_device_copy.argtypes = [vp, vp, sz]
+        L.nenya_autotune.argtypes = [C.c_int, C.c_int, C.c_int, C.c_char_p, sz]
         _LIB = L
     return _LIB
 
@@ -369,6 +370,18 @@ def _hip():
 def device_synchronize():
     if _hip().hipDeviceSynchronize() != 0:
         raise NenyaError(3, "hipDeviceSynchronize failed")
+
+
+def autotune(widths=(8, 16, 32, 64), log_min=8, log_max=20):
+    """Run the launch-shape autotuner (nenya_autotune) for the given widths;
+    later proofs in this process use its picks.  Returns the report (dict)."""
+    import json
+    mask = 0
+    for k in widths:
+        mask |= 1 << {8: 0, 16: 1, 32: 2, 64: 3}[k]
+    buf = C.create_string_buffer(1 << 16)
+    _check(_lib().nenya_autotune(mask, log_min, log_max, buf, len(buf)))
+    return json.loads(buf.value.decode())
 
 
 def device_copy(dst, src):

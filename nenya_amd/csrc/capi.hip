@@ -7,6 +7,7 @@
 #include "../../include/nenya.h"
 #include "container.hpp"
 #include "server.hpp"
+#include "tune.hpp"
 #include "verify.hpp"
 
 using namespace nenya;
@@ -753,6 +754,38 @@ int nenya_proof_payload(const uint8_t* c, size_t len, const uint8_t** payload, s
     *payload_len = (size_t)in.len;
     if (model) *model = in.model;
     if (k) *k = in.k;
+    return NENYA_OK;
+  });
+}
+
+int nenya_autotune(int k_mask, int log_min, int log_max, char* report, size_t report_cap) {
+  return guard([&] {
+    if (log_min < 8 || log_max > 22 || log_min > log_max || (k_mask & ~15) || !k_mask)
+      throw Error(NENYA_E_ARG, "autotune: k_mask in 1..15, 8 <= log_min <= log_max <= 22");
+    need_gpu();
+    CallTurn turn(false);
+    const auto t0 = std::chrono::steady_clock::now();
+    std::ostringstream js;
+    js << "{";
+    bool first = true;
+    for (int b = 0; b < 4; b++) {
+      if (!(k_mask & (1 << b))) continue;
+      if (!first) js << ",";
+      first = false;
+      switch (b) {
+        case 0: tune_width<W<8>>(js, log_min, log_max); break;
+        case 1: tune_width<W<16>>(js, log_min, log_max); break;
+        case 2: tune_width<W<32>>(js, log_min, log_max); break;
+        default: tune_width<W<64>>(js, log_min, log_max); break;
+      }
+    }
+    js << ",\"seconds\":" << std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() << "}";
+    const std::string out = js.str();
+    if (report && report_cap) {
+      const size_t c = std::min(out.size(), report_cap - 1);
+      std::memcpy(report, out.data(), c);
+      report[c] = 0;
+    }
     return NENYA_OK;
   });
 }

@@ -45,6 +45,31 @@ inline bool pool_guard() {
 }
 constexpr size_t GUARD = 1 << 16;
 
+// Launch shapes picked by the autotuner (tune.hpp), by limb width (index
+// log2(k) - 3) and the log2 of the launch's work items; 0 keeps the built-in
+// heuristic.  Written by nenya_autotune, read by every later launch.
+struct Tuning {
+  static constexpr int NL = 40;
+  // k_fold_mx: 1 split mode (a block per 32-element tile), 2 bulk mode;
+  // fold_cap: bulk-mode workgroups (by log2 of the tile count)
+  std::atomic<int> fold_mode[4][NL], fold_cap[4][NL];
+  // k_gram_lin workgroups (by log2 of the 32-element chunk count)
+  std::atomic<int> gram_blocks[4][NL];
+  // k_fold (64-bit limbs, vector ALUs): workgroup cap (by log2 of the elements)
+  std::atomic<int> vfold_cap[NL];
+  Tuning() {
+    for (int k = 0; k < 4; k++)
+      for (int i = 0; i < NL; i++) fold_mode[k][i] = fold_cap[k][i] = gram_blocks[k][i] = 0;
+    for (int i = 0; i < NL; i++) vfold_cap[i] = 0;
+  }
+  static Tuning& get() {
+    static Tuning t;
+    return t;
+  }
+  static int kidx(int k) { return k == 8 ? 0 : k == 16 ? 1 : k == 32 ? 2 : 3; }
+  static int lg(uint64_t n) { return n <= 1 ? 0 : std::min(NL - 1, 64 - __builtin_clzll(n - 1)); }
+};
+
 // Number of live batched-proof servers in the process.
 inline std::atomic<int>& live_servers() {
   static std::atomic<int> n{0};

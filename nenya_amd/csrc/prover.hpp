@@ -529,8 +529,9 @@ template <class T> struct Engine {
     launch("k_sub", nl * 3.0 * sizeof(L), 0, (k_sub<T>), dim3(grid_for(nl, 256, 1u << 20)), dim3(256), a, b, o, nl);
   }
   void fold(const L* r, const L* src, L* dst, uint64_t m, uint64_t ntab, uint64_t sstride, uint64_t dstride) {
-    launch("k_fold", m * ntab * 3 * EB, m * ntab * 32 * PM, (k_fold<T>), dim3(grid_for(m * ntab, GPB)), dim3(GB), r, src,
-           dst, m, ntab, sstride, dstride);
+    const int cap = Tuning::get().vfold_cap[Tuning::lg(m * ntab)].load(std::memory_order_relaxed);
+    launch("k_fold", m * ntab * 3 * EB, m * ntab * 32 * PM, (k_fold<T>), dim3(grid_for(m * ntab, GPB, cap ? cap : 16384)),
+           dim3(GB), r, src, dst, m, ntab, sstride, dstride);
   }
   // x -> r*x as matrix-core fragments (k <= 32); folds by the same
   // challenge share one.
@@ -575,8 +576,14 @@ template <class T> struct Engine {
       FoldJob<T> J{{s0, s1 ? s1 : s0}, {d0, d1 ? d1 : d0}, {n0, s1 ? n1 : 0}, {ss0, ss1}, {ds0, ds1}, m,
                    sd.kind, sd.sa, sd.sb, sd.so};
       const uint64_t ntiles = (m * (J.ntab[0] + J.ntab[1]) + 31) / 32;
-      const int split = ntiles <= 256;  // few tiles: 4 waves per tile, fragments from L2
-      const unsigned nb = split ? (unsigned)ntiles : grid_for(ntiles, 4, 512);
+      // few tiles: 4 waves per tile, fragments from L2 (the autotuner's
+      // picks, when it ran, replace the built-in choice)
+      const Tuning& tu = Tuning::get();
+      const int lt = Tuning::lg(ntiles), kx = Tuning::kidx(T::K);
+      const int tmode = tu.fold_mode[kx][lt].load(std::memory_order_relaxed);
+      const int tcap = tu.fold_cap[kx][lt].load(std::memory_order_relaxed);
+      const int split = tmode ? tmode == 1 : ntiles <= 256;
+      const unsigned nb = split ? (unsigned)ntiles : grid_for(ntiles, 4, tcap ? (unsigned)tcap : 512u);
       launch("k_fold_mx", m * (J.ntab[0] + J.ntab[1]) * 3 * EB, m * (J.ntab[0] + J.ntab[1]) * 32 * PM, (k_fold_mx<T>),
              dim3(nb + (sd.kind ? 1 : 0)), dim3(256), mm.fp, J, split);
     } else {
@@ -589,10 +596,12 @@ template <class T> struct Engine {
   // on the matrix cores; x rows past `xrows` limbs are zero.
   // Partials [block][2] of the two sums into `part` (room for 2 * 256
   // elements); returns the block count.
-  static constexpr unsigned GRAM_BLOCKS = 256;
+  static constexpr unsigned GRAM_BLOCKS = 256;      // built-in workgroup count
+  static constexpr unsigned GRAM_BLOCKS_MAX = 1024;  // room for the autotuner's picks
   unsigned gram_lin(const L* f, uint64_t fs, const L* g, uint64_t gs, uint64_t ntab, uint64_t m, int xrows, L* part) {
     const uint64_t nch = (m * ntab + 31) / 32;
-    const unsigned nwg = (unsigned)std::min<uint64_t>(nch, GRAM_BLOCKS);
+    const int tb = Tuning::get().gram_blocks[Tuning::kidx(T::K)][Tuning::lg(nch)].load(std::memory_order_relaxed);
+    const unsigned nwg = (unsigned)std::min<uint64_t>(nch, tb ? (unsigned)tb : GRAM_BLOCKS);
     launch("k_gram_lin", m * ntab * 4.0 * EB, m * ntab * 2.0 * E * E, (k_gram_lin<T>), dim3(nwg), dim3(1024), f, fs, g,
            gs, ntab, m, xrows, part);
     return nwg;
@@ -1396,7 +1405,7 @@ template <class T> struct Engine {
     DBuf<L> dc = put(&claim, 1), dq(pool, 3 * E), dch(pool, nv * E);
     size_t len = N;
     // allocated before the first waiting kernel (see mult_check)
-    const size_t nparts = kMx<T> ? GRAM_BLOCKS : grid_for((uint64_t)(N / 2 > 0 ? N / 2 : 1) * ntab, GPB, 2048);
+    const size_t nparts = kMx<T> ? GRAM_BLOCKS_MAX : grid_for((uint64_t)(N / 2 > 0 ? N / 2 : 1) * ntab, GPB, 2048);
     DBuf<L> part(pool, nparts * 2 * E);
     DBuf<uint32_t> frags(pool, kMx<T> ? nv * kND<T> * 4096 : 1);
     MbGuard guard(*this);
