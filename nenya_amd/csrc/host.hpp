@@ -445,6 +445,7 @@ template <class T> struct Consts {
   std::vector<uint32_t> b2frag;  // MFMA fragments of [b_s b_s'] (128 x 1024), k <= 32
   std::vector<uint32_t> lbfrag;  // MFMA fragments of x -> l_beta_i x for i < 16, k <= 32
   std::vector<uint32_t> packfrag;  // MFMA fragments of pack_r2 o rot_L (128 x 32) for L < 32, k <= 32
+  std::vector<L> b2plain;          // [b_s b_s'] as 1024 elements [32 s + s'][limb], k = 64
   L inv15;
 
   Consts() {
@@ -512,6 +513,15 @@ template <class T> struct Consts {
       }
     }
     frobenius_setup();
+    if (K == 64) {
+      // the products b_s b_s' for the vector-ALU C-matrix matvec
+      b2plain.assign((size_t)1024 * E, 0);
+      for (int a = 0; a < 32; a++)
+        for (int b = a; b < 32; b++) {
+          const Gr<T> p = basis[a] * basis[b];
+          for (int l = 0; l < E; l++) b2plain[(size_t)(32 * a + b) * E + l] = b2plain[(size_t)(32 * b + a) * E + l] = p.v[l];
+        }
+    }
     if (K <= 32) {
       // products b_s b_s' of the packing basis, column k = 32 s + s'
       std::vector<Gr<T>> b2(1024);

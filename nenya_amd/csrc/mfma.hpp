@@ -604,6 +604,47 @@ __global__ void __launch_bounds__(256) k_cgemm(const uint32_t* frag, const LT<T>
   }
 }
 
+// The same product on the vector ALUs (64-bit limbs):
+//   out[p][limb] = sum_{j < 1024} B2[j][limb] C_p[j],  j = 32 s + s',
+// B2 the basis products as plain limbs ([j][limb]).  A block takes 32 block
+// rows: thread (g, limb) accumulates 16 of them; per s it stages the 32
+// C_p[s][.] values of the block's rows in LDS and reads its 32 B2 limbs.
+template <class T>
+__global__ void __launch_bounds__(256) k_cgemm_valu(const LT<T>* b2, const LT<T>* cmat, LT<T>* out, uint64_t nb) {
+  using C = CT<T>;
+  constexpr int PB = 32, PG = 16;
+  __shared__ C cs[PB][32];
+  const int limb = threadIdx.x & (E - 1), g = threadIdx.x >> 7;
+  for (uint64_t p0 = (uint64_t)blockIdx.x * PB; p0 < nb; p0 += (uint64_t)gridDim.x * PB) {
+    C acc[PG];
+#pragma unroll
+    for (int i = 0; i < PG; i++) acc[i] = 0;
+    for (int s = 0; s < 32; s++) {
+      __syncthreads();
+      for (int t = threadIdx.x; t < PB * 32; t += blockDim.x) {
+        const uint64_t p = p0 + t / 32;
+        cs[t / 32][t % 32] = p < nb ? (C)cmat[((uint64_t)s * nb + p) * 32 + (t % 32)] : (C)0;
+      }
+      __syncthreads();
+      C bv[32];
+#pragma unroll
+      for (int sp = 0; sp < 32; sp++) bv[sp] = (C)b2[(size_t)(32 * s + sp) * E + limb];
+#pragma unroll
+      for (int i = 0; i < PG; i++) {
+        C a = acc[i];
+#pragma unroll
+        for (int sp = 0; sp < 32; sp++) a += bv[sp] * cs[g * PG + i][sp];
+        acc[i] = a;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < PG; i++) {
+      const uint64_t p = p0 + g * PG + i;
+      if (p < nb) out[p * E + limb] = (LT<T>)acc[i];
+    }
+  }
+}
+
 // m-bars on the matrix cores (pcs.rs:554-561, 984-1001):
 //   out[l][j] = sum_{i < rows} e_i X[l * lstride + i * k2 + j]
 // with the M_{e_i} fragments (frag + i * ND * 4096 words, from k_mulmat).

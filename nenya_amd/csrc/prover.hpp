@@ -231,6 +231,7 @@ template <class T> struct Engine {
   DBuf<uint32_t> d_b2frag;  // MFMA fragments of the packing-basis products
   DBuf<uint32_t> d_lbfrag;  // MFMA fragments of x -> l_beta_i x
   DBuf<uint32_t> d_packfrag;  // MFMA fragments of pack_r2 o rot_L
+  DBuf<L> d_b2;               // basis products b_s b_s' as limbs (k = 64)
   KProf* prof = nullptr;
 
   // Host-device mailbox for the sumcheck rounds (coherent pinned memory):
@@ -336,6 +337,10 @@ template <class T> struct Engine {
     h2d(d_pw.p, K.psi_pw.data(), K.psi_pw.size() * sizeof(L));
     h2d(d_one.p, one.v, E * sizeof(L));
     h2d(d_frob.p, K.frob_t.data(), E * E * sizeof(L));
+    if (!K.b2plain.empty()) {
+      d_b2 = DBuf<L>(pool, K.b2plain.size());
+      h2d(d_b2.p, K.b2plain.data(), K.b2plain.size() * sizeof(L));
+    }
     if (!K.b2frag.empty()) {
       d_b2frag = DBuf<uint32_t>(pool, K.b2frag.size());
       h2d(d_b2frag.p, K.b2frag.data(), K.b2frag.size() * 4);

@@ -425,8 +425,8 @@ template <class T> struct MimdParams : ParamsBase {
   DBuf<L> cval, dhat;
   // vector-ALU path (k = 64): matvec entries by row (k_spmv_gr layout) and
   // M_gamma-row entries by column, over the packed cells dhat
-  DBuf<uint64_t> mvptr, rwptr;
-  DBuf<uint32_t> mvdk, mvsel, mvidx, rwdk, rwsel, rwidx;
+  DBuf<uint64_t> rwptr;
+  DBuf<uint32_t> rwdk, rwsel, rwidx;
   // matrix-core path (k <= 32): M_gamma-row entries by column, one per
   // nonzero instance value, code = y << 7 | w << 5 | s (k_rows_gather)
   DBuf<uint64_t> eptr;
@@ -536,16 +536,6 @@ template <class T> struct MimdParams : ParamsBase {
     }
     dhat = DBuf<L>(&pool, std::max<size_t>(total, 1) * E);
     if (total) eng->pack(cval.p, dhat.p, total, 1);
-    // matvec entries: row r of matrix w sums dhat[g] * w_phi[col[g]]
-    {
-      std::vector<uint64_t> ptr(rowptr);
-      std::vector<uint32_t> dk(total), sel(total, 0), idx(total);
-      for (size_t g = 0; g < total; g++) {
-        dk[g] = (uint32_t)g;
-        idx[g] = col[g];
-      }
-      upload_csr(*eng, pool, ptr, dk, sel, idx, mvptr, mvdk, mvsel, mvidx);
-    }
     // M_gamma row entries by column: dhat[g] * eqs[w][row(g)] (mimd.rs:196-219)
     {
       std::vector<uint64_t> ptr(m + 1, 0);
@@ -565,14 +555,14 @@ template <class T> struct MimdParams : ParamsBase {
     }
     // setup data is read by other streams (server workers): finish it here
     eng->sync();
-    footprint = (size_t)((24.0 * E + 3 * 32) * sizeof(L) * n + (8.0 * E + 64) * sizeof(L) * m);
+    footprint = (size_t)((24.0 * E + 3 * 32) * sizeof(L) * n + (8.0 * E + 64) * sizeof(L) * m +
+                         1024 * sizeof(L) * n);
   }
   ~MimdParams() override {
     pw = PcsDev<T>();
     pabc = PcsDev<T>();
     pe = PcsDev<T>();
     cptr.release(); ccol.release(); cval.release(); dhat.release();
-    mvptr.release(); mvdk.release(); mvsel.release(); mvidx.release();
     rwptr.release(); rwdk.release(); rwsel.release(); rwidx.release();
     eptr.release(); ecode.release(); eval.release();
   }
@@ -597,12 +587,18 @@ template <class T> struct MimdParams : ParamsBase {
       }
       e.join();
     } else {
-      (void)slots;
-      for (int w = 0; w < 3; w++)
-        e.launch("k_spmv_gr", (double)ncell[w] * 2 * Engine<T>::EB, (double)ncell[w] * 16 * Engine<T>::PM,
-                 (k_spmv_gr<T>), dim3(grid_for(n, GPB)), dim3(GB), (const uint64_t*)(mvptr.p + w * (n + 1)),
-                 (const uint32_t*)mvdk.p, (const uint32_t*)mvsel.p, (const uint32_t*)mvidx.p, (const L*)dhat.p, wphi,
-                 (uint64_t)0, out + w * n * E, (uint64_t)n);
+      // C_r from the cells' instance values and the witness slots, then the
+      // basis products on the vector ALUs (as the layered matvec)
+      (void)wphi;
+      DBuf<L> cm(e.pool, n * 1024);
+      for (int w = 0; w < 3; w++) {
+        e.launch("k_cmat_mimd", (double)n * 1024 * sizeof(L), (double)ncell[w] * 1024, (k_cmat_mimd<T>),
+                 dim3(grid_for(n, 8, 4096)), dim3(256), (const uint64_t*)(cptr.p + w * (n + 1)), (const uint32_t*)ccol.p,
+                 (const L*)cval.p, slots, cm.p, (uint64_t)n);
+        e.launch("k_cgemm_valu", (double)n * (1024 * sizeof(L) + Engine<T>::EB), (double)n * 1024 * E,
+                 (k_cgemm_valu<T>), dim3(grid_for(n, 32, 2048)), dim3(256), (const L*)e.d_b2.p, (const L*)cm.p,
+                 out + w * n * E, (uint64_t)n);
+      }
     }
   }
   // build_combined_row (mimd.rs:196-219): M[col] = sum over the three
@@ -778,8 +774,6 @@ template <class T> struct GeneralParams : ParamsBase {
   Csr<T> mr[3];  // A, B, C by row (base-ring matvecs)
   DBuf<L> dhat;  // packed diagonal cells, A then B then C
   size_t ncell[3] = {0, 0, 0};
-  DBuf<uint64_t> lptr[3];
-  DBuf<uint32_t> ldk[3], lsel[3], lidx[3];
   DBuf<uint64_t> rptr;
   DBuf<uint32_t> rdk, rsel, ridx;
   DBuf<uint64_t> eptr;  // layer-row entries by output (matrix-core path)
@@ -801,7 +795,7 @@ template <class T> struct GeneralParams : ParamsBase {
     Pool& pool = cx0.pool;
     hipStream_t st = cx0.st;
     Engine<T>* eng = cx0.eng.get();
-    footprint = (size_t)(170.0 * E * sizeof(L) * nb) + (kMx<T> ? (size_t)nb * 1024 * sizeof(L) : 0);
+    footprint = (size_t)(170.0 * E * sizeof(L) * nb) + (size_t)nb * 1024 * sizeof(L);
     pw.P = PcsParams::setup_batched(lb, seed_w, 32, T::K);
     pabc.P = PcsParams::setup_stacked(lb, seed_w ^ 0xABC, 3, T::K);
     pe.P = PcsParams::setup_r2(lb, seed_e, T::K);
@@ -853,21 +847,6 @@ template <class T> struct GeneralParams : ParamsBase {
       if (total) {
         eng->h2d(dsl.p, cslots.data(), cslots.size() * sizeof(L));
         eng->pack(dsl.p, dhat.p, total, 1);
-      }
-      // layered_matvec CSR per matrix, grouped by output block row p
-      for (int w = 0; w < 3; w++) {
-        std::vector<uint64_t> ptr(nb + 1, 0);
-        for (size_t i = 0; i < ncell[w]; i++) ptr[cp[w][i] + 1]++;
-        for (size_t p = 0; p < nb; p++) ptr[p + 1] += ptr[p];
-        std::vector<uint64_t> pos(ptr.begin(), ptr.end() - 1);
-        std::vector<uint32_t> dk(ncell[w]), sel(ncell[w]), idx(ncell[w]);
-        for (size_t i = 0; i < ncell[w]; i++) {
-          uint64_t o = pos[cp[w][i]]++;
-          dk[o] = (uint32_t)(cell_base[w] + i);
-          sel[o] = cl[w][i];
-          idx[o] = cq[w][i];
-        }
-        upload_csr(*eng, pool, ptr, dk, sel, idx, lptr[w], ldk[w], lsel[w], lidx[w]);
       }
       // build_layer_rows CSR over (l, q) merging A, B, C (xsel = matrix id)
       {
@@ -931,7 +910,6 @@ template <class T> struct GeneralParams : ParamsBase {
     pe = PcsDev<T>();
     for (auto& m : mr) m = Csr<T>();
     dhat.release();
-    for (int w = 0; w < 3; w++) { lptr[w].release(); ldk[w].release(); lsel[w].release(); lidx[w].release(); }
     rptr.release(); rdk.release(); rsel.release(); ridx.release();
     eptr.release(); ecode.release(); eval.release();
   }
@@ -1044,11 +1022,18 @@ template <class T> struct GeneralParams : ParamsBase {
       }
       e.join();
     } else {
-      for (int m = 0; m < 3; m++)
-        e.launch("k_spmv_gr", ncell[m] * 2.0 * Engine<T>::EB, ncell[m] * 16 * Engine<T>::PM, (k_spmv_gr<T>),
-                 dim3(grid_for(nb, GPB)), dim3(GB), (const uint64_t*)lptr[m].p, (const uint32_t*)ldk[m].p,
-                 (const uint32_t*)lsel[m].p, (const uint32_t*)lidx[m].p, (const L*)dhat.p, (const L*)rho.p,
-                 (uint64_t)nb, sums.p + m * nb * E, (uint64_t)nb);
+      // the same factorization on the vector ALUs: C_p, then the products
+      // b_s b_s' (131072 limb-MACs per block row instead of a full ring
+      // product per diagonal cell)
+      DBuf<L> cm(&pool, nb * 1024);
+      for (int m = 0; m < 3; m++) {
+        e.launch("k_cmat", (double)nb * 1024 * sizeof(L), 0, (k_cmat<T>), dim3(grid_for(nb, 8, 4096)), dim3(256),
+                 (const uint32_t*)mr[m].ptr.p, (const uint32_t*)mr[m].idx.p, (const L*)mr[m].val.p, w, cm.p,
+                 (uint64_t)nb);
+        e.launch("k_cgemm_valu", (double)nb * (1024 * sizeof(L) + Engine<T>::EB), (double)nb * 1024 * E,
+                 (k_cgemm_valu<T>), dim3(grid_for(nb, 32, 2048)), dim3(256), (const L*)e.d_b2.p, (const L*)cm.p,
+                 sums.p + m * nb * E, (uint64_t)nb);
+      }
     }
     // 4. c_hat = pack_r2(psi(c_sum)) (general.rs:410-414); a_hat, b_hat = phi(slots)
     e.psi(sums.p + 2 * nb * E, abc_slots.p + 2 * N, nb);
