@@ -1598,7 +1598,7 @@ __global__ void k_queries(const uint32_t* src, int parent, uint64_t t, uint32_t 
 // the pinned proof buffer through its device address; 16-byte stores when
 // the destination allows them.  Runs on the download stream (at most 256
 // blocks) beside the prover's kernels.
-__global__ void __launch_bounds__(256) k_copy_out(const uint8_t* src, uint8_t* dst, uint64_t bytes) {
+__device__ __forceinline__ void copy_bytes(const uint8_t* src, uint8_t* dst, uint64_t bytes) {
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nth = (uint64_t)gridDim.x * blockDim.x;
   const uintptr_t al = (uintptr_t)src | (uintptr_t)dst;
   uint64_t done = 0;
@@ -1617,6 +1617,14 @@ __global__ void __launch_bounds__(256) k_copy_out(const uint8_t* src, uint8_t* d
     done += nw * 4;
   }
   for (uint64_t i = done + tid; i < bytes; i += nth) dst[i] = src[i];
+}
+__global__ void __launch_bounds__(256) k_copy_out(const uint8_t* src, uint8_t* dst, uint64_t bytes) {
+  copy_bytes(src, dst, bytes);
+}
+// Small uploads and root downloads between device memory and mapped pinned
+// host memory (a kernel of our own rather than the runtime's copy path).
+__global__ void __launch_bounds__(256) k_copy_small(const uint8_t* src, uint8_t* dst, uint64_t bytes) {
+  copy_bytes(src, dst, bytes);
 }
 
 // slots[j][s] = w[s][j] for the 32 witnesses of m limbs each (the slot

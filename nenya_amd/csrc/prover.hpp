@@ -376,6 +376,10 @@ template <class T> struct Engine {
         HIP_CHECK(hipStreamSynchronize(st_main));
         for (int i = 0; i < forked; i++) HIP_CHECK(hipStreamSynchronize(sst[i]));
       }
+      if (bg_live) {
+        HIP_CHECK(hipStreamSynchronize(st_fg));
+        HIP_CHECK(hipStreamSynchronize(sbg));
+      }
       up_off = 0;
     }
     uint8_t* slot = upbuf.get(UPCAP) + up_off;
@@ -388,10 +392,14 @@ template <class T> struct Engine {
     if (bytes > UPCAP) throw Error(NENYA_E_DEVICE, "staging request too large");
     uint8_t* slot = ring_slot(bytes);
     std::memcpy(slot, src, bytes);
-    void* dptr = nullptr;
-    HIP_CHECK(hipHostGetDevicePointer(&dptr, slot, 0));
-    return dptr;
+    uint8_t* base = upbuf.p;
+    if (base != up_hbase) {
+      HIP_CHECK(hipHostGetDevicePointer((void**)&up_dbase, base, 0));
+      up_hbase = base;
+    }
+    return up_dbase + (slot - base);
   }
+  uint8_t *up_hbase = nullptr, *up_dbase = nullptr;  // the ring's host and device addresses
   // Commitment roots in flight to the host (see commit / resolve_roots).
   static constexpr int ROOT_SLOTS = 16;
   Pinned rootbuf;
@@ -421,9 +429,10 @@ template <class T> struct Engine {
   void h2d(void* d, const void* h, size_t bytes) {
     if (!bytes) return;
     if (bytes <= UPCAP / 4) {
-      uint8_t* slot = ring_slot(bytes);
-      std::memcpy(slot, h, bytes);
-      HIP_CHECK(hipMemcpyAsync(d, slot, bytes, hipMemcpyHostToDevice, st));
+      // the copy kernel reads the ring slot through its device address
+      const void* src = stage_host(h, bytes);
+      launch("k_copy_small", 2.0 * bytes, 0, k_copy_small, dim3((unsigned)std::min<size_t>((bytes + 4095) / 4096, 4)),
+             dim3(256), (const uint8_t*)src, (uint8_t*)d, (uint64_t)bytes);
       return;
     }
     // bounce in chunks through two pinned buffers, so the host fills one
@@ -778,6 +787,54 @@ template <class T> struct Engine {
   void on(int i) {
     if (forked) st = i == 0 ? st_main : sst[(i - 1) % forked];
   }
+  // A background stream for work whose results are needed only at the end
+  // of a proof (opening tails, openings whose values enter only the final
+  // self-check): bg_begin() makes it wait for the prover stream's work so
+  // far and sends launches to it until the matching bg_end(); bg_join()
+  // makes the prover stream wait for it.  Regions may nest and may reopen
+  // after bg_end().  Frees are deferred from the first bg_begin() to
+  // bg_join().  Contexts that stay on one stream run the same work in place.
+  hipStream_t sbg = nullptr, st_fg = nullptr;
+  hipEvent_t bgev = nullptr, bgjev = nullptr;
+  bool bg_live = false;
+  int bg_nest = 0;
+  void bg_begin() {
+    if (forked || !pipelined || serial || pool_poison() || pool_guard()) return;
+    if (bg_live && st == sbg) {
+      bg_nest++;
+      return;
+    }
+    if (!sbg) {
+      HIP_CHECK(hipStreamCreate(&sbg));
+      HIP_CHECK(hipEventCreateWithFlags(&bgev, hipEventDisableTiming));
+      HIP_CHECK(hipEventCreateWithFlags(&bgjev, hipEventDisableTiming));
+    }
+    HIP_CHECK(hipEventRecord(bgev, st));
+    HIP_CHECK(hipStreamWaitEvent(sbg, bgev, 0));
+    if (!bg_live) {
+      st_fg = st;
+      bg_live = true;
+      pool->defer_frees(true);
+    }
+    st = sbg;
+  }
+  void bg_end() {
+    if (!bg_live || st != sbg) return;
+    if (bg_nest > 0) {
+      bg_nest--;
+      return;
+    }
+    st = st_fg;
+  }
+  void bg_join() {
+    if (!bg_live) return;
+    HIP_CHECK(hipEventRecord(bgjev, sbg));
+    HIP_CHECK(hipStreamWaitEvent(st_fg, bgjev, 0));
+    st = st_fg;
+    bg_live = false;
+    bg_nest = 0;
+    pool->defer_frees(false);
+  }
   void join() {
     if (!forked) return;
     for (int i = 0; i < forked; i++) {
@@ -796,6 +853,14 @@ template <class T> struct Engine {
       forked = 0;
       pool->defer_frees(false);
     }
+    // ... or its background work running
+    if (bg_live) {
+      st = st_fg;
+      (void)hipStreamSynchronize(sbg);
+      bg_live = false;
+      bg_nest = 0;
+      pool->defer_frees(false);
+    }
     // the prover stream's own writes into the proof buffer (server contexts
     // copy their sections on it; k_queries writes indices directly) and any
     // commitment roots a failed proof left unresolved
@@ -807,6 +872,12 @@ template <class T> struct Engine {
     held.clear();
   }
   ~Engine() {
+    if (sbg) {
+      (void)hipStreamSynchronize(sbg);
+      (void)hipStreamDestroy(sbg);
+      (void)hipEventDestroy(bgev);
+      (void)hipEventDestroy(bgjev);
+    }
     for (int i = 0; i < NSIDE; i++)
       if (sst[i]) {
         (void)hipStreamSynchronize(sst[i]);
@@ -1037,7 +1108,10 @@ template <class T> struct Engine {
     // caller's resolve_roots() waits once for a group of commitments
     if (root_used >= ROOT_SLOTS) throw Error(NENYA_E_DEVICE, "too many unresolved commitment roots");
     uint8_t* slot = rootbuf.get(ROOT_SLOTS * 32) + 32 * root_used++;
-    HIP_CHECK(hipMemcpyAsync(slot, c.nodes.p + (2 * n - 2) * 8, 32, hipMemcpyDeviceToHost, st));
+    uint8_t* dslot = nullptr;
+    HIP_CHECK(hipHostGetDevicePointer((void**)&dslot, slot, 0));
+    launch("k_copy_small", 64, 0, k_copy_small, dim3(1), dim3(256), (const uint8_t*)(c.nodes.p + (2 * n - 2) * 8), dslot,
+           (uint64_t)32);
     c.root_stage = slot;
     return c;
   }
@@ -1193,15 +1267,19 @@ template <class T> struct Engine {
              dim3(256), (const L*)part.p, nsplit, nl * k2, out);
   }
 
-  G open(const PcsDev<T>& pc, const CommitDev<T>& c, const G* z, const G* weights, size_t s, Sink& out) {
-    DBuf<L> y = open_dev(pc, c, z, weights, s, out);
+  G open(const PcsDev<T>& pc, const CommitDev<T>& c, const G* z, const G* weights, size_t s, Sink& out,
+         bool bg_tail = false) {
+    DBuf<L> y = open_dev(pc, c, z, weights, s, out, bg_tail);
     G v;
     d2h(&v, y.p, sizeof(G));
     return v;
   }
   // The opening with its value y = <m-bar, e1> left on the device (no
   // synchronization; callers that need y only later fetch it then).
-  DBuf<L> open_dev(const PcsDev<T>& pc, const CommitDev<T>& c, const G* z, const G* weights, size_t s, Sink& out) {
+  // bg_tail: the m-bar's hashing, the query columns, the paths and the
+  // downloads go to the background stream (y does not depend on them).
+  DBuf<L> open_dev(const PcsDev<T>& pc, const CommitDev<T>& c, const G* z, const G* weights, size_t s, Sink& out,
+                   bool bg_tail = false) {
     const PcsParams& P = pc.P;
     size_t lk1 = (size_t)__builtin_ctzll(P.k1), lk2 = (size_t)__builtin_ctzll(P.k2);
     DBuf<L> e0 = eq_table(z, lk1), e1 = eq_table(z + lk1, lk2);
@@ -1227,8 +1305,10 @@ template <class T> struct Engine {
     DBuf<L> y = dots_dev<32>(mbar.p, 0, 1, e1.p, P.k2);
     out.u64(P.k2);
     uint8_t* mdst = out.take(P.k2 * E * sizeof(L));
+    if (bg_tail) bg_begin();
     open_tail(pc, c, mbar.p, P.k2, out);
     download(mdst, mbar, P.k2 * E * sizeof(L));
+    if (bg_tail) bg_end();
     return y;
   }
 

@@ -322,6 +322,14 @@ template <class T> struct SimdParams : ParamsBase {
     tr.root(ce.root);
     StdRng rs = tr.sampler("mult_check_target");
     std::vector<G> rt = e.sample(rs, log_n);
+    // proof layout (simd.rs:150-174): roots, claims and sumchecks, then the
+    // openings of w at s2 and of e at r
+    const size_t EBs = E * sizeof(L);
+    size_t s_mult = EBs + 8 + 3 * (size_t)log_n * EBs + 2 * EBs, s_lin = EBs + 8 + 3 * (size_t)log_m * EBs + 2 * EBs;
+    size_t s_ow = opening_bytes(pw.P, pw.P.k1, sizeof(L), 0), s_oe = opening_bytes(pq.P, pq.P.k1, sizeof(L), 0);
+    size_t total = 64 + s_mult + s_lin + s_ow + s_oe;
+    uint8_t* buf = cx.proofbuf.get(total);
+    Sink k_oe(buf + total - s_oe, s_oe);
     // 4. a_phi, b_phi, c_phi (matrix.rs:93-105)
     DBuf<L> aw(&pool, n * E), bw(&pool, n * E), cwv(&pool, n * E);
     spmv_gr1(e, ar, (const L*)wphi.p, aw.p, (uint64_t)n, 0);
@@ -329,6 +337,11 @@ template <class T> struct SimdParams : ParamsBase {
     spmv_gr1(e, cr, (const L*)wphi.p, cwv.p, (uint64_t)n, 0);
     // 5. claim_mult = c_phi(r) + e(r)
     DBuf<L> eq_r = e.eq_table(rt.data(), log_n);
+    // the opening of e at r needs only r: it runs on the background stream,
+    // beside the rest of the proof (enqueued once the matvecs are)
+    e.bg_begin();
+    DBuf<L> ye_d = e.open_dev(pq, ce, rt.data(), nullptr, 1, k_oe);
+    e.bg_end();
     G cw_at_r = e.template dot<32>(eq_r.p, cwv.p, n);
     G e_at_r = e.template dot<32>(eq_r.p, ev.p, n);
     G claim_mult = cw_at_r + e_at_r;
@@ -365,13 +378,8 @@ template <class T> struct SimdParams : ParamsBase {
     G fin_lin[2] = {final_row, final_w};
     tr.elems("lin/finals", fin_lin, 2);
     stg.mark("linear_sumcheck");
-    // 11. serialize (simd.rs:150-174) with the two openings
-    const size_t EBs = E * sizeof(L);
-    size_t s_mult = EBs + 8 + 3 * (size_t)log_n * EBs + 2 * EBs, s_lin = EBs + 8 + 3 * (size_t)log_m * EBs + 2 * EBs;
-    size_t s_ow = opening_bytes(pw.P, pw.P.k1, sizeof(L), 0), s_oe = opening_bytes(pq.P, pq.P.k1, sizeof(L), 0);
-    size_t total = 64 + s_mult + s_lin + s_ow + s_oe;
-    uint8_t* buf = cx.proofbuf.get(total);
-    Sink out(buf, total);
+    // 11. serialize (simd.rs:150-174) with the opening of w
+    Sink out(buf, total - s_oe);
     out.bytes(cw.root, 32);
     out.bytes(ce.root, 32);
     out.gr(claim_mult);
@@ -385,11 +393,12 @@ template <class T> struct SimdParams : ParamsBase {
     out.gr(final_row);
     out.gr(final_w);
     G yw = e.open(pw, cw, lp.chal.data(), nullptr, 1, out);
-    G ye = e.open(pq, ce, rt.data(), nullptr, 1, out);
+    e.bg_join();
+    G ye = e.get(ye_d.p);
     stg.mark("open");
     if ((!(yw == final_w) || !(ye == e_at_r)) && !skip_selfcheck())
       throw Error(NENYA_E_DEVICE, "internal opening self-check failed");
-    if (!out.full()) throw Error(NENYA_E_DEVICE, "proof size mismatch");
+    if (!out.full() || !k_oe.full()) throw Error(NENYA_E_DEVICE, "proof size mismatch");
     e.finish_downloads();
     cx.last_timing = stg.t;
     if (prof.on) prof.collect();
@@ -707,11 +716,11 @@ template <class T> struct MimdParams : ParamsBase {
     // 6. openings at s1 (mimd.rs:340-352)
     G wts[3] = {G::one(), gam, gam2};
     Sink k_es(o_es, s_es), k_abc(o_abc, s_abc), k_em(o_em, s_em);
-    // estack's value enters the linear claim; the other two only the final
-    // self-check, so they stay on the device until then
-    G estack_val = e.open(pe, ces, mp.chal.data(), wts, 3, k_es);
-    DBuf<L> abc_y = e.open_dev(pabc, cabc, mp.chal.data(), wts, 3, k_abc);
-    DBuf<L> em_y = e.open_dev(pe, cem, mp.chal.data(), nullptr, 1, k_em);
+    // estack's value enters the linear claim: it is formed first on the
+    // prover stream (the rest of its opening in the background); the other
+    // two openings feed only the final self-check and run on the background
+    // stream, enqueued after the M_gamma row (critical path first)
+    DBuf<L> es_y = e.open_dev(pe, ces, mp.chal.data(), wts, 3, k_es, true);
     stg.mark("open_s1");
     // 7. M_gamma row (mimd.rs:196-219, eq_s1 for all three matrices)
     DBuf<L> eqs(&pool, 3 * n * E), mrow(&pool, m * E);
@@ -725,6 +734,11 @@ template <class T> struct MimdParams : ParamsBase {
     m_row(e, eqs.p, mrow.p);
     eqs.release();
     stg.mark("linear_rows");
+    e.bg_begin();
+    DBuf<L> abc_y = e.open_dev(pabc, cabc, mp.chal.data(), wts, 3, k_abc);
+    DBuf<L> em_y = e.open_dev(pe, cem, mp.chal.data(), nullptr, 1, k_em);
+    e.bg_end();
+    const G estack_val = e.get(es_y.p);
     // 8. linear check on (M_gamma row, w_phi); w_phi is low-degree
     G claim_lin = mp.finals[0] + gam * mp.finals[1] + gam2 * mp.finals[2] + estack_val;
     tr.label("linear_check_sumcheck");
@@ -736,6 +750,7 @@ template <class T> struct MimdParams : ParamsBase {
     DBuf<L> yw_d = e.open_dev(pw, cw, lp.chal.data(), nullptr, 1, k_w);
     stg.mark("open_w");
     // the opening values of the self-check, one synchronization
+    e.bg_join();
     std::vector<G> late = e.pick_many({{abc_y.p, 1, 1}, {em_y.p, 1, 1}, {yw_d.p, 1, 1}});
     const G abc_val = late[0], em_val = late[1], yw = late[2];
     Sink hd(buf, 128 + s_mult + s_lin);
@@ -1091,11 +1106,13 @@ template <class T> struct GeneralParams : ParamsBase {
     // 8. openings at s1 (general.rs:475-478)
     G wts[3] = {G::one(), gam, gam2};
     Sink k_es(o_es, s_es), k_abc(o_abc, s_abc), k_em(o_em, s_em);
-    // estack's value enters the linear claim; the other two only the final
-    // self-check, so they stay on the device until then
-    G estack_val = e.open(pe, ces, mp.chal.data(), wts, 3, k_es);
-    DBuf<L> abc_y = e.open_dev(pabc, cabc, mp.chal.data(), wts, 3, k_abc);
-    DBuf<L> em_y = e.open_dev(pe, cem, mp.chal.data(), nullptr, 1, k_em);
+    // estack's value enters the linear claim: it is formed first on the
+    // prover stream (the rest of its opening in the background).  The other
+    // two values enter only the final self-check: those openings run on the
+    // background stream, beside the layer rows and the linear check, and are
+    // enqueued after the layer rows (the host's launches are the serial
+    // resource here, so the critical path goes first).
+    DBuf<L> es_y = e.open_dev(pe, ces, mp.chal.data(), wts, 3, k_es, true);
     stg.mark("open_s1");
     // 9. layer rows (general.rs:233-271) as sum over cells of
     //    {eq_s1, gamma eq_s1, gamma^2 eq_s1}[p] * D[cell]
@@ -1115,6 +1132,11 @@ template <class T> struct GeneralParams : ParamsBase {
     }
     eqs.release();
     stg.mark("layer_rows");
+    e.bg_begin();
+    DBuf<L> abc_y = e.open_dev(pabc, cabc, mp.chal.data(), wts, 3, k_abc);
+    DBuf<L> em_y = e.open_dev(pe, cem, mp.chal.data(), nullptr, 1, k_em);
+    e.bg_end();
+    const G estack_val = e.get(es_y.p);
     // 10. linear check over the 32 (rho_l, row_l) pairs; rho is low-degree
     G claim_lin = mp.finals[0] + gam * mp.finals[1] + gam2 * mp.finals[2] + estack_val;
     tr.label("linear_check_sumcheck");
@@ -1126,6 +1148,7 @@ template <class T> struct GeneralParams : ParamsBase {
     DBuf<L> ys_d = e.rotated_open_dev(pw, cw, rho.p, nb, lp.chal.data(), 32, k_rho);
     stg.mark("open_rho");
     // the opening values of the self-check, one synchronization
+    e.bg_join();
     std::vector<G> late = e.pick_many({{abc_y.p, 1, 1}, {em_y.p, 1, 1}, {ys_d.p, 1, 32}});
     const G abc_val = late[0], em_val = late[1];
     const G* ys = late.data() + 2;
