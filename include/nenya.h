@@ -144,11 +144,13 @@ int nenya_prove_general_device_view(const nenya_params* p, const void* d_w, size
 int nenya_prove_general_device_noalloc(const nenya_params* p, const void* d_w, size_t n, uint8_t* proof,
                                        size_t proof_cap, size_t* proof_len);
 
-/* Per-stage wall-clock timing (ms) of the last proof made with `p` when
- * enabled (GeneralProveTiming / SimdProveTiming, general.rs:153-173,
- * simd.rs:177-192).  Enabling adds stream synchronizations at stage
- * boundaries.  nenya_last_timing writes up to `cap` entries and returns the
- * count; names are separated by '\n' in `names` (up to names_cap bytes). */
+/* Per-stage timing (ms) of the last proof made with `p` when enabled
+ * (GeneralProveTiming / SimdProveTiming, general.rs:153-173,
+ * simd.rs:177-192): HIP events on the prover stream at stage boundaries, no
+ * added synchronization, so a stage is the time the prover stream took to
+ * get from one boundary to the next in the overlapped schedule.
+ * nenya_last_timing writes up to `cap` entries and returns the count; names
+ * are separated by '\n' in `names` (up to names_cap bytes). */
 int nenya_set_timing(nenya_params* p, int enabled);
 int nenya_last_timing(const nenya_params* p, double* ms, int cap, char* names, size_t names_cap);
 

@@ -28,22 +28,42 @@ inline unsigned grid_for(uint64_t items, uint64_t per_block, unsigned cap = 1638
 
 // Stage timer (GeneralProveTiming / SimdProveTiming analogue); synchronizes the
 // stream at stage boundaries only when enabled.
+// Stage times of a proof from HIP events on the prover stream: mark() records
+// where the stream has got to, without a synchronization, so the stages keep
+// the proof's overlap (work on other streams shows only where the prover
+// stream waits for it, and host delays where the stream runs dry);
+// collect() turns the marks into milliseconds once the proof is done.
 struct Stages {
   bool on = false;
   hipStream_t st = nullptr;
-  std::vector<std::pair<std::string, double>> t;
-  std::chrono::steady_clock::time_point last;
+  std::vector<hipEvent_t> ev;
+  std::vector<std::string> names;
+  void rec(const char* name) {
+    hipEvent_t e = nullptr;
+    HIP_CHECK(hipEventCreate(&e));
+    HIP_CHECK(hipEventRecord(e, st));
+    ev.push_back(e);
+    names.push_back(name);
+  }
   void start() {
-    if (!on) return;
-    (void)hipStreamSynchronize(st);
-    last = std::chrono::steady_clock::now();
+    if (on) rec("");
   }
   void mark(const char* name) {
-    if (!on) return;
-    (void)hipStreamSynchronize(st);
-    auto now = std::chrono::steady_clock::now();
-    t.push_back({name, std::chrono::duration<double, std::milli>(now - last).count()});
-    last = now;
+    if (on) rec(name);
+  }
+  std::vector<std::pair<std::string, double>> collect() {
+    std::vector<std::pair<std::string, double>> t;
+    if (!on || ev.empty()) return t;
+    HIP_CHECK(hipEventSynchronize(ev.back()));
+    for (size_t i = 1; i < ev.size(); i++) {
+      float ms = 0;
+      HIP_CHECK(hipEventElapsedTime(&ms, ev[i - 1], ev[i]));
+      t.push_back({names[i], (double)ms});
+    }
+    return t;
+  }
+  ~Stages() {
+    for (auto e : ev) (void)hipEventDestroy(e);
   }
 };
 

@@ -400,7 +400,7 @@ template <class T> struct SimdParams : ParamsBase {
       throw Error(NENYA_E_DEVICE, "internal opening self-check failed");
     if (!out.full() || !k_oe.full()) throw Error(NENYA_E_DEVICE, "proof size mismatch");
     e.finish_downloads();
-    cx.last_timing = stg.t;
+    cx.last_timing = stg.collect();
     if (prof.on) prof.collect();
     note_footprint(pool.peak - pool_base);
     return {buf, total};
@@ -769,7 +769,7 @@ template <class T> struct MimdParams : ParamsBase {
                     yw == final_w;
     if (!ok && !skip_selfcheck()) throw Error(NENYA_E_DEVICE, "internal opening self-check failed");
     e.finish_downloads();
-    cx.last_timing = stg.t;
+    cx.last_timing = stg.collect();
     if (prof.on) prof.collect();
     note_footprint(pool.peak - pool_base);
     return {buf, total};
@@ -1169,7 +1169,7 @@ template <class T> struct GeneralParams : ParamsBase {
     for (int l = 0; l < 32; l++) ok = ok && ys[l] == lp.finals[l];
     if (!ok && !skip_selfcheck()) throw Error(NENYA_E_DEVICE, "internal opening self-check failed");
     e.finish_downloads();
-    cx.last_timing = stg.t;
+    cx.last_timing = stg.collect();
     if (prof.on) prof.collect();
     note_footprint(pool.peak - pool_base);
     return {buf, total};
