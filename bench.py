@@ -348,7 +348,9 @@ def main():
     log(f"instance 2^{log_n} u{k}: nnz A={len(A[0])} B={len(B[0])} C={len(C[0])} ({time.perf_counter() - t0:.1f}s)")
     t0 = time.perf_counter()
     params = na.GeneralSnarkParams.secure(log_n - 5, (A, B, C), 0xDEAD, 0xCAFE, k=k)
-    log(f"setup (diagonal packing + PCS params) {time.perf_counter() - t0:.1f}s")
+    setup = {"total_ms": round((time.perf_counter() - t0) * 1e3, 1),
+             **{k2: round(v, 1) for k2, v in params.setup_timing().items()}}
+    log(f"setup (matrix packing + PCS params): {setup}")
     wd = na.DeviceArray(as_limbs(w, k))
     na.device_synchronize()
     # launch shapes of the sumcheck kernels picked by the autotuner (every
@@ -481,21 +483,28 @@ def main():
         t0 = time.perf_counter()
         mj, sizes = mimd_serving_jobs(na, 32, 2000 + args.seed)
         log(f"mimd serving mix: {len(mj)} jobs over per-instance logs {sizes} set up in {time.perf_counter() - t0:.1f}s")
+        # setup stages of the largest circuit of the mix
+        pbig = max((j[1] for j in mj), key=lambda pp: pp.log_n)
+        mimd_setup = {"log_n": pbig.log_n, **{k2: round(v, 1) for k2, v in pbig.setup_timing().items()}}
+        log(f"  mimd u64 setup of the largest circuit: {mimd_setup}")
         q, srv = queue_stats(na, mj, args.workers, 99 + args.seed)
         nver = verify_queue(na, mj, args.workers)
         log(f"  mimd serving proofs accepted by the verifier: {nver}/{len(mj)}")
         large = {"mimd_u64_serve": {"jobs": len(mj), "workers": args.workers, "lambda_per_s": srv["lambda_per_s"],
                                     "verified": f"{nver}/{len(mj)}",
                                     "p50_ms": srv["batched"]["p50_ms"], "p99_ms": srv["batched"]["p99_ms"],
-                                    "plain": srv["plain"], "batch": q,
+                                    "plain": srv["plain"], "batch": q, "setup_ms": mimd_setup,
                                     "mix": "k=64, logN~U{10..22}, one circuit per size, witness per job"}}
         del mj
         del params, wd
         t0 = time.perf_counter()
         A2, B2, C2, w2 = general_instance(25, 32, args.seed)
         p2 = na.GeneralSnarkParams.secure(20, (A2, B2, C2), 0xDEAD, 0xCAFE, k=32)
+        setup25 = {"total_ms": round((time.perf_counter() - t0) * 1e3, 1),
+                   **{k2: round(v, 1) for k2, v in p2.setup_timing().items()}}
+        large["general_u32_2p25_setup_ms"] = setup25
         w2 = as_limbs(w2, 32)
-        log(f"general u32 2^25: set up in {time.perf_counter() - t0:.1f}s")
+        log(f"general u32 2^25: set up {setup25} (instance generation included in total_ms)")
         # three proofs, each timed alone, copied out and verified after its timing
         ts, checks = [], []
         for _ in range(3):
@@ -546,6 +555,7 @@ def main():
         "autotune": tune,
         "cpu_baseline": cpu,
         "stages_ms": {k2: round(v, 3) for k2, v in stages.items()},
+        "setup_ms": setup,
         "batch": batch,
         "serve": serve,
         "widths": widths,

@@ -49,6 +49,7 @@ def _lib():
         L.nenya_prove_general_device_noalloc.argtypes = [vp, vp, sz, vp, sz, C.POINTER(sz)]
         L.nenya_set_timing.argtypes = [vp, C.c_int]
         L.nenya_last_timing.argtypes = [vp, C.POINTER(C.c_double), C.c_int, C.c_char_p, sz]
+        L.nenya_setup_timing.argtypes = [vp, C.POINTER(C.c_double), C.c_int, C.c_char_p, sz]
         L.nenya_set_profile.argtypes = [vp, C.c_int]
         L.nenya_set_profile_only.argtypes = [vp, C.c_char_p]
         dp = C.POINTER(C.c_double)
@@ -198,9 +199,16 @@ class _Params:
                 for i in range(n)}
 
     def last_timing(self):
+        return self._timing(_lib().nenya_last_timing)
+
+    def setup_timing(self):
+        """Wall-clock ms of the setup stages that made these params."""
+        return self._timing(_lib().nenya_setup_timing)
+
+    def _timing(self, fn):
         ms = (C.c_double * 64)()
         names = C.create_string_buffer(4096)
-        n = _lib().nenya_last_timing(self._h, ms, 64, names, 4096)
+        n = fn(self._h, ms, 64, names, 4096)
         labels = names.value.decode().split("\n")[:n]
         return {labels[i]: ms[i] for i in range(n)}
 

@@ -605,9 +605,8 @@ int nenya_kernel_stats(const nenya_params* p, int cap, char* names, size_t names
   }
   return i;
 }
-int nenya_last_timing(const nenya_params* p, double* ms, int cap, char* names, size_t names_cap) {
-  if (!p || !p->impl) return 0;
-  const auto& t = p->impl->ctx0().last_timing;
+static int export_timing(const std::vector<std::pair<std::string, double>>& t, double* ms, int cap, char* names,
+                         size_t names_cap) {
   int n = (int)std::min<size_t>(t.size(), cap > 0 ? (size_t)cap : 0);
   std::string all;
   for (int i = 0; i < n; i++) {
@@ -621,6 +620,14 @@ int nenya_last_timing(const nenya_params* p, double* ms, int cap, char* names, s
     names[c] = 0;
   }
   return n;
+}
+int nenya_last_timing(const nenya_params* p, double* ms, int cap, char* names, size_t names_cap) {
+  if (!p || !p->impl) return 0;
+  return export_timing(p->impl->ctx0().last_timing, ms, cap, names, names_cap);
+}
+int nenya_setup_timing(const nenya_params* p, double* ms, int cap, char* names, size_t names_cap) {
+  if (!p || !p->impl) return 0;
+  return export_timing(p->impl->setup_timing, ms, cap, names, names_cap);
 }
 
 // ---------------- batched / served proofs ----------------

@@ -71,6 +71,16 @@ struct ParamsBase {
   // bytes of the last proof (0 before the first): sizes the proof buffer a
   // server hands to the next job with these params
   std::atomic<size_t> proof_len{0};
+  // wall-clock milliseconds of the setup's stages, in order (each stage's
+  // device work finished before its mark)
+  std::vector<std::pair<std::string, double>> setup_timing;
+  std::chrono::steady_clock::time_point setup_last = std::chrono::steady_clock::now();
+  void setup_mark(const char* name, hipStream_t st) {
+    HIP_CHECK(hipStreamSynchronize(st));
+    const auto now = std::chrono::steady_clock::now();
+    setup_timing.push_back({name, std::chrono::duration<double, std::milli>(now - setup_last).count()});
+    setup_last = now;
+  }
   virtual CtxBase& ctx0() = 0;
   virtual ~ParamsBase() = default;
 };
@@ -83,6 +93,8 @@ struct HostCoo {
   const void* v;  // nnz limbs of k bits (native width)
 };
 
+// A matrix grouped by major index (rows or columns) on the device, built by
+// DevSetup::csr.
 template <class T> struct Csr {
   using L = typename T::L;
   DBuf<uint32_t> ptr, idx;
@@ -93,65 +105,6 @@ template <class T> struct Csr {
   DBuf<uint32_t> heavy;
   size_t nheavy = 0;
   uint32_t maxheavy = 0;
-  // the grouped COO on the host
-  struct Host {
-    std::vector<uint32_t> p, ix;
-    std::vector<L> v;
-  };
-  // group COO by `major` (rows if !by_col), stable (a counting sort)
-  static Host group(const HostCoo& m, bool by_col) {
-    Host h;
-    const size_t nmaj = by_col ? m.cols : m.rows;
-    h.p.assign(nmaj + 1, 0);
-    h.ix.resize(m.nnz);
-    h.v.resize(m.nnz);
-    const L* vals = (const L*)m.v;
-    for (size_t k = 0; k < m.nnz; k++) h.p[(by_col ? m.c[k] : m.r[k]) + 1]++;
-    for (size_t i = 0; i < nmaj; i++) h.p[i + 1] += h.p[i];
-    std::vector<uint32_t> pos(h.p.begin(), h.p.end() - 1);
-    for (size_t k = 0; k < m.nnz; k++) {
-      uint32_t maj = by_col ? m.c[k] : m.r[k];
-      uint32_t o = pos[maj]++;
-      h.ix[o] = by_col ? m.r[k] : m.c[k];
-      h.v[o] = vals[k];
-    }
-    return h;
-  }
-  // Allocates the device arrays and lists their uploads from h (and from
-  // hv, which receives the heavy majors; both must outlive the uploads).
-  template <class Eng> void prepare(Eng& e, const Host& h, std::vector<uint32_t>& hv, std::vector<typename Eng::Up>& ups) {
-    Pool* pool = e.pool;
-    nnz = h.ix.size();
-    ptr = DBuf<uint32_t>(pool, h.p.size());
-    idx = DBuf<uint32_t>(pool, std::max<size_t>(nnz, 1));
-    val = DBuf<L>(pool, std::max<size_t>(nnz, 1));
-    ups.push_back({ptr.p, h.p.data(), h.p.size() * 4});
-    if (nnz) {
-      ups.push_back({idx.p, h.ix.data(), h.ix.size() * 4});
-      ups.push_back({val.p, h.v.data(), h.v.size() * sizeof(L)});
-    }
-    hv.clear();
-    maxheavy = 0;
-    for (size_t o = 0; o + 1 < h.p.size(); o++) {
-      const uint32_t c = h.p[o + 1] - h.p[o];
-      if (c >= HEAVY) {
-        hv.push_back((uint32_t)o);
-        maxheavy = std::max(maxheavy, c);
-      }
-    }
-    nheavy = hv.size();
-    if (nheavy) {
-      heavy = DBuf<uint32_t>(pool, nheavy);
-      ups.push_back({heavy.p, hv.data(), nheavy * 4});
-    }
-  }
-  template <class Eng> void upload(Eng& e, const Host& h) {
-    std::vector<uint32_t> hv;
-    std::vector<typename Eng::Up> ups;
-    prepare(e, h, hv, ups);
-    e.h2d_many(ups);
-  }
-  template <class Eng> void build(Eng& e, const HostCoo& m, bool by_col) { upload(e, group(m, by_col)); }
 };
 
 // out[o] (+)= sum over o's entries of val * X[idx] (mul_by_gr1 matvecs,
@@ -173,58 +126,17 @@ void spmv_gr1(Eng& e, const Csr<T>& m, const typename T::L* X, typename T::L* ou
            (const uint32_t*)m.heavy.p, out);
 }
 
-// Several CSR builds: the host grouping of each runs on its own thread, the
-// uploads follow in order on the engine's stream.
-template <class T, class Eng>
-void build_csrs(Eng& e, std::initializer_list<std::tuple<Csr<T>*, const HostCoo*, bool>> jobs) {
-  std::vector<typename Csr<T>::Host> hs(jobs.size());
-  std::vector<std::thread> th;
-  size_t i = 0;
-  try {
-    for (const auto& j : jobs) {
-      th.emplace_back([&hs, i, j] { hs[i] = Csr<T>::group(*std::get<1>(j), std::get<2>(j)); });
-      i++;
-    }
-  } catch (...) {  // a thread that could not start: finish the started ones first
-    for (auto& t : th) t.join();
-    throw;
-  }
-  for (auto& t : th) t.join();
-  // every array of every CSR in one batch of uploads
-  std::vector<std::vector<uint32_t>> hv(jobs.size());
-  std::vector<typename Eng::Up> ups;
-  i = 0;
-  for (const auto& j : jobs) {
-    std::get<0>(j)->prepare(e, hs[i], hv[i], ups);
-    i++;
-  }
-  e.h2d_many(ups);
-}
-
-// Uploads a CSR of (dk, sel, idx) entries (the k_spmv_gr layout).
-template <class T>
-void upload_csr(Engine<T>& eng, Pool& pool, const std::vector<uint64_t>& ptr, const std::vector<uint32_t>& dk,
-                const std::vector<uint32_t>& sel, const std::vector<uint32_t>& idx, DBuf<uint64_t>& dp,
-                DBuf<uint32_t>& ddk, DBuf<uint32_t>& dsel, DBuf<uint32_t>& didx) {
-  dp = DBuf<uint64_t>(&pool, ptr.size());
-  size_t nn = std::max<size_t>(dk.size(), 1);
-  ddk = DBuf<uint32_t>(&pool, nn);
-  dsel = DBuf<uint32_t>(&pool, nn);
-  didx = DBuf<uint32_t>(&pool, nn);
-  eng.h2d(dp.p, ptr.data(), ptr.size() * 8);
-  if (!dk.empty()) {
-    eng.h2d(ddk.p, dk.data(), dk.size() * 4);
-    eng.h2d(dsel.p, sel.data(), sel.size() * 4);
-    eng.h2d(didx.p, idx.data(), idx.size() * 4);
-  }
-  eng.sync();
-}
-
 inline void check_coo(const HostCoo& m, size_t rows, size_t cols) {
   if (m.rows != rows || m.cols != cols) throw Error(NENYA_E_SHAPE, "matrix shape mismatch");
   for (size_t k = 0; k < m.nnz; k++)
     if (m.r[k] >= rows || m.c[k] >= cols) throw Error(NENYA_E_SHAPE, "matrix entry out of range");
 }
+
+}  // namespace nenya
+
+#include "setup.hpp"  // device matrix packing (uses HostCoo and Csr above)
+
+namespace nenya {
 
 // ============================================================================
 // SIMD (simd.rs:220-373)
@@ -249,8 +161,10 @@ template <class T> struct SimdParams : ParamsBase {
     log_m = lm;
     pw.P = PcsParams::setup(lm, seed_w, T::K);
     pq.P = PcsParams::setup_r2(ln, seed_q, T::K);
+    setup_mark("pcs_params", cx0.st);
     pw.upload(*cx0.eng);
     pq.upload(*cx0.eng);
+    setup_mark("pcs_upload", cx0.st);
     // setup data is read by other streams (server workers): finish it here
     HIP_CHECK(hipStreamSynchronize(cx0.st));
     footprint = (size_t)((32.0 * 2 + 2 * E) * sizeof(L) * (1ull << lm) + (10.0 * E + 96) * sizeof(L) * (1ull << ln));
@@ -276,9 +190,23 @@ template <class T> struct SimdParams : ParamsBase {
     stg.on = cx.timing;
     stg.st = st;
     stg.start();
+    // the six CSRs (by row and by column) grouped on the device from one
+    // upload of each matrix's entries
     Csr<T> ar, br, cr, ac, bc, cc;
-    build_csrs<T>(e, {{&ar, &A, false}, {&br, &B, false}, {&cr, &Cm, false}, {&ac, &A, true}, {&bc, &B, true},
-                      {&cc, &Cm, true}});
+    {
+      DevSetup<T> ds(e);
+      DevCoo<T> coo(e, {&A, &B, &Cm});
+      const HostCoo* hm[3] = {&A, &B, &Cm};
+      Csr<T>* byrow[3] = {&ar, &br, &cr};
+      Csr<T>* bycol[3] = {&ac, &bc, &cc};
+      for (int w = 0; w < 3; w++) {
+        const uint32_t* r = coo.r.p + coo.off[w];
+        const uint32_t* c = coo.c.p + coo.off[w];
+        const L* v = coo.v.p + coo.off[w];
+        ds.csr(*byrow[w], r, c, v, hm[w]->nnz, n);
+        ds.csr(*bycol[w], c, r, v, hm[w]->nnz, m);
+      }
+    }
     // witnesses [32][m] and their slot transpose [m][32] (made on the device)
     DBuf<L> dw(&pool, 32 * m), slots(&pool, m * 32);
     e.h2d(dw.p, wit, 32 * m * sizeof(L));
@@ -461,84 +389,114 @@ template <class T> struct MimdParams : ParamsBase {
     pw.P = PcsParams::setup(lm, seed_w, T::K);
     pabc.P = PcsParams::setup_stacked(ln, seed_w ^ 0xABC, 3, T::K);
     pe.P = PcsParams::setup_r2(ln, seed_e, T::K);
+    setup_mark("pcs_params", cx0.st);
     pw.upload(*eng);
     pabc.upload(*eng);
     pe.upload(*eng);
-    std::vector<uint64_t> rowptr(3 * (n + 1), 0);
-    std::vector<uint32_t> col, cellrow;
-    std::vector<L> vals;
-    size_t base = 0;
+    setup_mark("pcs_upload", cx0.st);
+    // pack_mimd_matrices on the device: per matrix the entries of the 32
+    // instances sorted (stably) by (row, col); runs of one key are the
+    // union-support cells, each with its 32 instance values summed
+    std::vector<HostCoo> hs(3);
+    std::vector<uint64_t> hoff(3 * 33);
     for (int w = 0; w < 3; w++) {
       const MimdSet& S = sets[w];
-      const L* sv = (const L*)S.v;
       const size_t tot = S.off[32];
-      struct Item { uint64_t key; uint32_t s; L v; };
-      std::vector<Item> it;
-      it.reserve(tot);
       for (int inst = 0; inst < 32; inst++) {
-        if (S.off[inst + 1] < S.off[inst] || S.off[inst + 1] > tot) throw Error(NENYA_E_ARG, "bad instance offsets");
-        for (size_t j = S.off[inst]; j < S.off[inst + 1]; j++) {
-          if (S.r[j] >= n || S.c[j] >= m) throw Error(NENYA_E_SHAPE, "matrix entry out of range");
-          it.push_back({(uint64_t)S.r[j] * m + S.c[j], (uint32_t)inst, sv[j]});
-        }
+        if (S.off[inst + 1] < S.off[inst] || S.off[inst + 1] > tot || S.off[0] != 0)
+          throw Error(NENYA_E_ARG, "bad instance offsets");
       }
-      std::stable_sort(it.begin(), it.end(), [](const Item& a, const Item& b) { return a.key < b.key; });
-      uint64_t* rp = &rowptr[w * (n + 1)];
-      for (size_t i = 0; i < it.size(); i++) {
-        if (i == 0 || it[i].key != it[i - 1].key) {
-          const uint64_t r = it[i].key / m;
-          col.push_back((uint32_t)(it[i].key % m));
-          cellrow.push_back((uint32_t)r);
-          vals.resize(vals.size() + 32, (L)0);
-          rp[r + 1]++;
-        }
-        L& x = vals[vals.size() - 32 + it[i].s];
-        x = (L)(x + it[i].v);
-      }
-      ncell[w] = col.size() - base;
-      rp[0] = base;
-      for (size_t r = 0; r < n; r++) rp[r + 1] += rp[r];
-      base = col.size();
+      for (size_t j = 0; j < tot; j++)
+        if (S.r[j] >= n || S.c[j] >= m) throw Error(NENYA_E_SHAPE, "matrix entry out of range");
+      hs[w] = HostCoo{n, m, tot, S.r, S.c, S.v};
+      for (int i = 0; i <= 32; i++) hoff[w * 33 + i] = S.off[i];
     }
-    const size_t total = base;
-    if (total >= (size_t(1) << 32)) throw Error(NENYA_E_SHAPE, "too many matrix cells");
-    cptr = DBuf<uint64_t>(&pool, rowptr.size());
+    DevSetup<T> ds(*eng);
+    DevCoo<T> coo(*eng, {&hs[0], &hs[1], &hs[2]});
+    const uint64_t ne = coo.off[3];
+    if (ne >= (uint64_t(1) << 32)) throw Error(NENYA_E_SHAPE, "too many matrix entries");
+    DBuf<uint8_t> inst(&pool, std::max<uint64_t>(ne, 1));
+    {
+      DBuf<uint64_t> doff(&pool, hoff.size());
+      eng->h2d(doff.p, hoff.data(), hoff.size() * 8);
+      for (int w = 0; w < 3; w++)
+        eng->launch("k_inst_of", 0, 0, k_inst_of, dim3(ds.grid(hs[w].nnz)), dim3(256), (const uint64_t*)(doff.p + 33 * w),
+                    (uint64_t)hs[w].nnz, inst.p + coo.off[w]);
+      eng->sync();
+    }
+    DBuf<uint64_t> ckey(&pool, std::max<uint64_t>(ne, 1));
+    DBuf<L> slots(&pool, std::max<uint64_t>(ne, 1) * 32);
+    HIP_CHECK(hipMemsetAsync(slots.p, 0, std::max<uint64_t>(ne, 1) * 32 * sizeof(L), cx0.st));
+    uint64_t cbase[4] = {0, 0, 0, 0};
+    for (int w = 0; w < 3; w++) {
+      const uint64_t nw = hs[w].nnz;
+      cbase[w + 1] = cbase[w];
+      if (!nw) continue;
+      DBuf<uint64_t> key(&pool, nw), skey(&pool, nw);
+      DBuf<uint32_t> perm(&pool, nw), head(&pool, nw);
+      eng->launch("k_rc_keys", 0, 0, k_rc_keys, dim3(ds.grid(nw)), dim3(256), (const uint32_t*)(coo.r.p + coo.off[w]),
+                  (const uint32_t*)(coo.c.p + coo.off[w]), nw, (uint64_t)m, key.p);
+      {
+        DBuf<uint32_t> id = ds.iota(nw, (uint32_t)coo.off[w]);
+        ds.template sort_pairs<uint64_t>(key.p, skey.p, id.p, perm.p, nw, bits_for((uint64_t)n * m - 1));
+      }
+      eng->launch("k_run_heads", 0, 0, (k_run_heads<uint64_t>), dim3(ds.grid(nw)), dim3(256), (const uint64_t*)skey.p, nw,
+                  head.p);
+      uint32_t* id = reinterpret_cast<uint32_t*>(key.p);  // run numbers (1-based), over the unsorted keys
+      ds.scan(head.p, id, nw, true);
+      const uint64_t nc = ds.last_of(id, nw);
+      eng->launch("k_cell_sum", 0, 0, (k_cell_sum<L>), dim3(ds.grid(nw)), dim3(256), (const uint64_t*)skey.p,
+                  (const uint32_t*)perm.p, id, nw, (const uint8_t*)inst.p, (const L*)coo.v.p, ckey.p + cbase[w],
+                  slots.p + cbase[w] * 32);
+      ncell[w] = nc;
+      cbase[w + 1] = cbase[w] + nc;
+    }
+    const size_t total = cbase[3];
+    if (total >= (size_t(1) << 27)) throw Error(NENYA_E_SHAPE, "too many matrix cells");
     ccol = DBuf<uint32_t>(&pool, std::max<size_t>(total, 1));
     cval = DBuf<L>(&pool, std::max<size_t>(total, 1) * 32);
-    eng->h2d(cptr.p, rowptr.data(), rowptr.size() * 8);
+    DBuf<uint32_t> crow(&pool, std::max<size_t>(total, 1));
     if (total) {
-      eng->h2d(ccol.p, col.data(), total * 4);
-      eng->h2d(cval.p, vals.data(), vals.size() * sizeof(L));
+      eng->d2d(cval.p, slots.p, total * 32 * sizeof(L));
+      eng->launch("k_cell_rc", 0, 0, k_cell_rc, dim3(ds.grid(total)), dim3(256), (const uint64_t*)ckey.p, (uint64_t)total,
+                  (uint64_t)m, crow.p, ccol.p);
     }
+    slots.release();
+    // row pointers of each matrix's cells (global cell indices)
+    cptr = DBuf<uint64_t>(&pool, 3 * (n + 1));
+    for (int w = 0; w < 3; w++)
+      eng->launch("k_group_ptr", 0, 0, (k_group_ptr<uint32_t, uint64_t>), dim3(ds.grid(n + 1)), dim3(256),
+                  (const uint32_t*)(crow.p + cbase[w]), (uint64_t)ncell[w], (uint64_t)n, (uint64_t)cbase[w],
+                  cptr.p + w * (n + 1));
     if constexpr (kMx<T>) {
       if (n > (size_t(1) << 25)) throw Error(NENYA_E_SHAPE, "too many rows for the row-entry code");
-      std::vector<uint64_t> ptr(m + 1, 0);
-      size_t g = 0;
-      for (int w = 0; w < 3; w++)
-        for (size_t i = 0; i < ncell[w]; i++, g++)
-          for (int sl = 0; sl < 32; sl++)
-            if (vals[g * 32 + sl]) ptr[col[g] + 1]++;
-      for (size_t h = 0; h < m; h++) ptr[h + 1] += ptr[h];
-      std::vector<uint64_t> pos(ptr.begin(), ptr.end() - 1);
-      std::vector<uint32_t> code(std::max<uint64_t>(ptr[m], 1));
-      std::vector<L> ev(std::max<uint64_t>(ptr[m], 1));
-      g = 0;
-      for (int w = 0; w < 3; w++)
-        for (size_t i = 0; i < ncell[w]; i++, g++)
-          for (int sl = 0; sl < 32; sl++) {
-            const L v = vals[g * 32 + sl];
-            if (!v) continue;
-            const uint64_t o = pos[col[g]]++;
-            code[o] = (uint32_t)(((uint64_t)cellrow[g] << 7) | ((uint64_t)w << 5) | (uint64_t)sl);
-            ev[o] = v;
-          }
-      eptr = DBuf<uint64_t>(&pool, ptr.size());
-      ecode = DBuf<uint32_t>(&pool, code.size());
-      eval = DBuf<L>(&pool, ev.size());
-      eng->h2d(eptr.p, ptr.data(), ptr.size() * 8);
-      eng->h2d(ecode.p, code.data(), code.size() * 4);
-      eng->h2d(eval.p, ev.data(), ev.size() * sizeof(L));
+      // M_gamma-row entries: the nonzero instance values by column, in
+      // (cell, slot) order within a column
+      const uint64_t ns = (uint64_t)total * 32;
+      DBuf<uint32_t> flag(&pool, std::max<uint64_t>(ns, 1)), pos(&pool, std::max<uint64_t>(ns, 1));
+      if (ns) {
+        eng->launch("k_nz_flags", 0, 0, (k_nz_flags<L>), dim3(ds.grid(ns)), dim3(256), (const L*)cval.p, ns, flag.p);
+        ds.scan(flag.p, pos.p, ns, false);
+      }
+      const uint64_t nz = ns ? (uint64_t)ds.last_of(pos.p, ns) + ds.last_of(flag.p, ns) : 0;
+      eptr = DBuf<uint64_t>(&pool, m + 1);
+      ecode = DBuf<uint32_t>(&pool, std::max<uint64_t>(nz, 1));
+      eval = DBuf<L>(&pool, std::max<uint64_t>(nz, 1));
+      DBuf<uint32_t> key(&pool, std::max<uint64_t>(nz, 1)), at(&pool, std::max<uint64_t>(nz, 1));
+      DBuf<uint32_t> skey(&pool, std::max<uint64_t>(nz, 1)), perm(&pool, std::max<uint64_t>(nz, 1));
+      if (nz) {
+        eng->launch("k_nz_place", 0, 0, k_nz_place, dim3(ds.grid(ns)), dim3(256), (const uint32_t*)flag.p,
+                    (const uint32_t*)pos.p, (const uint32_t*)ccol.p, ns, key.p, at.p);
+        DBuf<uint32_t> id = ds.iota(nz);
+        ds.template sort_pairs<uint32_t>(key.p, skey.p, id.p, perm.p, nz, bits_for(m - 1));
+        eng->launch("k_nz_codes", 0, 0, (k_nz_codes<L>), dim3(ds.grid(nz)), dim3(256), (const uint32_t*)perm.p,
+                    (const uint32_t*)at.p, (const uint32_t*)crow.p, (const L*)cval.p, nz, (uint64_t)cbase[1],
+                    (uint64_t)cbase[2], ecode.p, eval.p);
+      }
+      eng->launch("k_group_ptr", 0, 0, (k_group_ptr<uint32_t, uint64_t>), dim3(ds.grid(m + 1)), dim3(256),
+                  (const uint32_t*)skey.p, nz, (uint64_t)m, (uint64_t)0, eptr.p);
       eng->sync();
+      setup_mark("matrices", cx0.st);
       footprint = (size_t)((24.0 * E + 3 * 32) * sizeof(L) * n + (8.0 * E + 64) * sizeof(L) * m +
                            (48.0 * E + 1024) * sizeof(L) * n);
       return;
@@ -547,21 +505,23 @@ template <class T> struct MimdParams : ParamsBase {
     if (total) eng->pack(cval.p, dhat.p, total, 1);
     // M_gamma row entries by column: dhat[g] * eqs[w][row(g)] (mimd.rs:196-219)
     {
-      std::vector<uint64_t> ptr(m + 1, 0);
-      for (size_t g = 0; g < total; g++) ptr[col[g] + 1]++;
-      for (size_t h = 0; h < m; h++) ptr[h + 1] += ptr[h];
-      std::vector<uint64_t> pos(ptr.begin(), ptr.end() - 1);
-      std::vector<uint32_t> dk(total), sel(total), idx(total);
-      size_t g = 0;
-      for (int w = 0; w < 3; w++)
-        for (size_t i = 0; i < ncell[w]; i++, g++) {
-          const uint64_t o = pos[col[g]]++;
-          dk[o] = (uint32_t)g;
-          sel[o] = (uint32_t)w;
-          idx[o] = cellrow[g];
-        }
-      upload_csr(*eng, pool, ptr, dk, sel, idx, rwptr, rwdk, rwsel, rwidx);
+      rwptr = DBuf<uint64_t>(&pool, m + 1);
+      const uint64_t nt = std::max<uint64_t>(total, 1);
+      rwdk = DBuf<uint32_t>(&pool, nt);
+      rwsel = DBuf<uint32_t>(&pool, nt);
+      rwidx = DBuf<uint32_t>(&pool, nt);
+      DBuf<uint32_t> skey(&pool, nt), perm(&pool, nt);
+      if (total) {
+        DBuf<uint32_t> id = ds.iota(total);
+        ds.template sort_pairs<uint32_t>(ccol.p, skey.p, id.p, perm.p, total, bits_for(m - 1));
+        eng->launch("k_cellcol_place", 0, 0, k_cellcol_place, dim3(ds.grid(total)), dim3(256), (const uint32_t*)perm.p,
+                    (const uint32_t*)crow.p, (uint64_t)total, (uint64_t)cbase[1], (uint64_t)cbase[2], rwdk.p, rwsel.p,
+                    rwidx.p);
+      }
+      eng->launch("k_group_ptr", 0, 0, (k_group_ptr<uint32_t, uint64_t>), dim3(ds.grid(m + 1)), dim3(256),
+                  (const uint32_t*)skey.p, (uint64_t)total, (uint64_t)m, (uint64_t)0, rwptr.p);
     }
+    setup_mark("matrices", cx0.st);
     // setup data is read by other streams (server workers): finish it here
     eng->sync();
     footprint = (size_t)((24.0 * E + 3 * 32) * sizeof(L) * n + (8.0 * E + 64) * sizeof(L) * m +
@@ -814,110 +774,106 @@ template <class T> struct GeneralParams : ParamsBase {
     pw.P = PcsParams::setup_batched(lb, seed_w, 32, T::K);
     pabc.P = PcsParams::setup_stacked(lb, seed_w ^ 0xABC, 3, T::K);
     pe.P = PcsParams::setup_r2(lb, seed_e, T::K);
+    setup_mark("pcs_params", st);
     pw.upload(*eng);
     pabc.upload(*eng);
     pe.upload(*eng);
-    for (int w = 0; w < 3; w++) {
-      check_coo(m3[w], N, N);
-      mr[w].build(*eng, m3[w], false);
-    }
-    // diagonal cells (used by the vector-ALU path, k = 64; the matrix-core
-    // widths form the same sums from the matrices' entries directly)
+    setup_mark("pcs_upload", st);
+    for (int w = 0; w < 3; w++) check_coo(m3[w], N, N);
+    // pack_general_matrices on the device (stable sorts of the uploaded
+    // entries): the row CSRs, then the diagonal cells (vector-ALU width) or
+    // the layer-row entries (matrix-core widths)
+    DevSetup<T> ds(*eng);
+    DevCoo<T> coo(*eng, {&m3[0], &m3[1], &m3[2]});
+    const uint64_t ne = coo.off[3];
+    if (ne >= (uint64_t(1) << 32)) throw Error(NENYA_E_SHAPE, "too many matrix entries");
+    for (int w = 0; w < 3; w++)
+      ds.csr(mr[w], coo.r.p + coo.off[w], coo.c.p + coo.off[w], coo.v.p + coo.off[w], m3[w].nnz, N);
     if (!kMx<T>) {
       // diagonal cells: nonzero (r,c) -> layer l = (c%32 - r%32) mod 32, p = r/32,
-      // q = c/32, slot i = r%32 (reduce.rs:387-397)
-      struct Cell { uint64_t key; uint32_t slot; uint64_t val; };
-      std::vector<std::vector<uint32_t>> cp(3), cq(3), cl(3);
-      std::vector<L> cslots;
-      size_t base = 0;
-      std::vector<size_t> cell_base(3);
+      // q = c/32, slot i = r%32 (reduce.rs:387-397); cells of A, B, C
+      // concatenated, each matrix's in (l, p, q) order
+      DBuf<uint8_t> slot(&pool, std::max<uint64_t>(ne, 1));
+      eng->launch("k_row_slot", 0, 0, k_row_slot, dim3(ds.grid(ne)), dim3(256), (const uint32_t*)coo.r.p, ne, slot.p);
+      DBuf<uint64_t> ckey(&pool, std::max<uint64_t>(ne, 1));
+      DBuf<L> cslots(&pool, std::max<uint64_t>(ne, 1) * 32);
+      HIP_CHECK(hipMemsetAsync(cslots.p, 0, std::max<uint64_t>(ne, 1) * 32 * sizeof(L), st));
+      uint64_t cbase[4] = {0, 0, 0, 0};
       for (int w = 0; w < 3; w++) {
-        const HostCoo& m = m3[w];
-        const L* vals = (const L*)m.v;
-        std::vector<Cell> items(m.nnz);
-        for (size_t k2 = 0; k2 < m.nnz; k2++) {
-          uint64_t r = m.r[k2], c = m.c[k2];
-          uint64_t l = ((c % 32) + 32 - (r % 32)) % 32;
-          items[k2] = {(l * nb + r / 32) * nb + c / 32, (uint32_t)(r % 32), (uint64_t)vals[k2]};
+        const uint64_t nw = m3[w].nnz;
+        cbase[w + 1] = cbase[w];
+        if (!nw) continue;
+        DBuf<uint64_t> key(&pool, nw), skey(&pool, nw);
+        DBuf<uint32_t> perm(&pool, nw), head(&pool, nw);
+        eng->launch("k_cell_keys", 0, 0, k_cell_keys, dim3(ds.grid(nw)), dim3(256),
+                    (const uint32_t*)(coo.r.p + coo.off[w]), (const uint32_t*)(coo.c.p + coo.off[w]), nw, (uint64_t)nb,
+                    key.p);
+        {
+          DBuf<uint32_t> id = ds.iota(nw, (uint32_t)coo.off[w]);
+          ds.template sort_pairs<uint64_t>(key.p, skey.p, id.p, perm.p, nw, bits_for(32 * (uint64_t)nb * nb - 1));
         }
-        std::sort(items.begin(), items.end(), [](const Cell& a, const Cell& b) { return a.key < b.key; });
-        cell_base[w] = base;
-        for (size_t i = 0; i < items.size(); i++) {
-          if (i == 0 || items[i].key != items[i - 1].key) {
-            uint64_t key = items[i].key;
-            cl[w].push_back((uint32_t)(key / (nb * nb)));
-            cp[w].push_back((uint32_t)((key / nb) % nb));
-            cq[w].push_back((uint32_t)(key % nb));
-            cslots.resize(cslots.size() + 32, (L)0);
-          }
-          L& s = cslots[cslots.size() - 32 + items[i].slot];
-          s = (L)(s + (L)items[i].val);
-        }
-        ncell[w] = cl[w].size();
-        base += ncell[w];
+        eng->launch("k_run_heads", 0, 0, (k_run_heads<uint64_t>), dim3(ds.grid(nw)), dim3(256), (const uint64_t*)skey.p,
+                    nw, head.p);
+        uint32_t* id = reinterpret_cast<uint32_t*>(key.p);  // run numbers (1-based), over the unsorted keys
+        ds.scan(head.p, id, nw, true);
+        const uint64_t nc = ds.last_of(id, nw);
+        eng->launch("k_cell_sum", 0, 0, (k_cell_sum<L>), dim3(ds.grid(nw)), dim3(256), (const uint64_t*)skey.p,
+                    (const uint32_t*)perm.p, (const uint32_t*)id, nw, (const uint8_t*)slot.p, (const L*)coo.v.p,
+                    ckey.p + cbase[w], cslots.p + cbase[w] * 32);
+        ncell[w] = nc;
+        cbase[w + 1] = cbase[w] + nc;
       }
-      size_t total = base;
-      DBuf<L> dsl(&pool, std::max<size_t>(total, 1) * 32);
-      dhat = DBuf<L>(&pool, std::max<size_t>(total, 1) * E);
-      if (total) {
-        eng->h2d(dsl.p, cslots.data(), cslots.size() * sizeof(L));
-        eng->pack(dsl.p, dhat.p, total, 1);
-      }
+      const uint64_t total = cbase[3];
+      dhat = DBuf<L>(&pool, std::max<uint64_t>(total, 1) * E);
+      if (total) eng->pack(cslots.p, dhat.p, total, 1);
       // build_layer_rows CSR over (l, q) merging A, B, C (xsel = matrix id)
-      {
-        std::vector<uint64_t> ptr(32 * nb + 1, 0);
-        for (int w = 0; w < 3; w++)
-          for (size_t i = 0; i < ncell[w]; i++) ptr[(size_t)cl[w][i] * nb + cq[w][i] + 1]++;
-        for (size_t o = 0; o < 32 * nb; o++) ptr[o + 1] += ptr[o];
-        std::vector<uint64_t> pos(ptr.begin(), ptr.end() - 1);
-        std::vector<uint32_t> dk(total), sel(total), idx(total);
-        for (int w = 0; w < 3; w++)
-          for (size_t i = 0; i < ncell[w]; i++) {
-            uint64_t o = pos[(size_t)cl[w][i] * nb + cq[w][i]]++;
-            dk[o] = (uint32_t)(cell_base[w] + i);
-            sel[o] = (uint32_t)w;
-            idx[o] = cp[w][i];
-          }
-        upload_csr(*eng, pool, ptr, dk, sel, idx, rptr, rdk, rsel, ridx);
+      const uint64_t nt = std::max<uint64_t>(total, 1);
+      rptr = DBuf<uint64_t>(&pool, 32 * nb + 1);
+      rdk = DBuf<uint32_t>(&pool, nt);
+      rsel = DBuf<uint32_t>(&pool, nt);
+      ridx = DBuf<uint32_t>(&pool, nt);
+      DBuf<uint32_t> key(&pool, nt), skey(&pool, nt), perm(&pool, nt);
+      if (total) {
+        eng->launch("k_cell_lq", 0, 0, k_cell_lq, dim3(ds.grid(total)), dim3(256), (const uint64_t*)ckey.p, total,
+                    (uint64_t)nb, key.p);
+        DBuf<uint32_t> id = ds.iota(total);
+        ds.template sort_pairs<uint32_t>(key.p, skey.p, id.p, perm.p, total, bits_for(32 * (uint64_t)nb - 1));
+        eng->launch("k_cell_rows_place", 0, 0, k_cell_rows_place, dim3(ds.grid(total)), dim3(256),
+                    (const uint32_t*)perm.p, (const uint64_t*)ckey.p, total, (uint64_t)nb, cbase[1], cbase[2], rdk.p,
+                    rsel.p, ridx.p);
       }
+      eng->launch("k_group_ptr", 0, 0, (k_group_ptr<uint32_t, uint64_t>), dim3(ds.grid(32 * nb + 1)), dim3(256),
+                  (const uint32_t*)skey.p, total, (uint64_t)(32 * nb), (uint64_t)0, rptr.p);
+      eng->sync();
     }
     if (kMx<T>) {
       // matrix-core layer rows: entries (row 32p+s of matrix w, value v) by
       // output (l = (c - r) mod 32, q = c / 32), code p << 7 | w << 5 | s
       if (nb > (size_t(1) << 25)) throw Error(NENYA_E_SHAPE, "too many blocks for the layer-row entry code");
-      auto out_of = [&](uint64_t r, uint64_t c) { return (((c % 32) + 32 - (r % 32)) % 32) * nb + c / 32; };
-      std::vector<uint64_t> ptr(32 * nb + 1, 0);
-      size_t ne = 0;
-      for (int w = 0; w < 3; w++) {
-        ne += m3[w].nnz;
-        for (size_t k2 = 0; k2 < m3[w].nnz; k2++) ptr[out_of(m3[w].r[k2], m3[w].c[k2]) + 1]++;
+      const uint64_t nn = std::max<uint64_t>(ne, 1);
+      eptr = DBuf<uint64_t>(&pool, 32 * nb + 1);
+      ecode = DBuf<uint32_t>(&pool, nn);
+      eval = DBuf<L>(&pool, nn);
+      DBuf<uint32_t> key(&pool, nn), skey(&pool, nn), perm(&pool, nn);
+      if (ne) {
+        eng->launch("k_layer_keys", 0, 0, k_layer_keys, dim3(ds.grid(ne)), dim3(256), (const uint32_t*)coo.r.p,
+                    (const uint32_t*)coo.c.p, ne, (uint64_t)nb, key.p);
+        DBuf<uint32_t> id = ds.iota(ne);
+        ds.template sort_pairs<uint32_t>(key.p, skey.p, id.p, perm.p, ne, bits_for(32 * (uint64_t)nb - 1));
+        eng->launch("k_layer_place", 0, 0, (k_layer_place<L>), dim3(ds.grid(ne)), dim3(256), (const uint32_t*)perm.p,
+                    (const uint32_t*)coo.r.p, (const L*)coo.v.p, ne, coo.off[1], coo.off[2], ecode.p, eval.p);
       }
-      for (size_t o = 0; o < 32 * nb; o++) ptr[o + 1] += ptr[o];
-      std::vector<uint64_t> pos(ptr.begin(), ptr.end() - 1);
-      std::vector<uint32_t> code(std::max<size_t>(ne, 1));
-      std::vector<L> ev(std::max<size_t>(ne, 1));
-      for (int w = 0; w < 3; w++) {
-        const L* vals = (const L*)m3[w].v;
-        for (size_t k2 = 0; k2 < m3[w].nnz; k2++) {
-          const uint64_t r = m3[w].r[k2], c = m3[w].c[k2];
-          const uint64_t o = pos[out_of(r, c)]++;
-          code[o] = (uint32_t)(((r / 32) << 7) | ((uint64_t)w << 5) | (r % 32));
-          ev[o] = vals[k2];
-        }
-      }
-      eptr = DBuf<uint64_t>(&pool, ptr.size());
-      ecode = DBuf<uint32_t>(&pool, code.size());
-      eval = DBuf<L>(&pool, ev.size());
-      eng->h2d(eptr.p, ptr.data(), ptr.size() * 8);
-      eng->h2d(ecode.p, code.data(), code.size() * 4);
-      eng->h2d(eval.p, ev.data(), ev.size() * sizeof(L));
+      eng->launch("k_group_ptr", 0, 0, (k_group_ptr<uint32_t, uint64_t>), dim3(ds.grid(32 * nb + 1)), dim3(256),
+                  (const uint32_t*)skey.p, ne, (uint64_t)(32 * nb), (uint64_t)0, eptr.p);
       eng->sync();
     }
+    setup_mark("matrices", st);
     // the proof buffer and the pinned staging areas exist before the first
     // proof (a no-alloc call may be the first)
     cx0.proofbuf.get(proof_size());
     eng->reserve_staging();
     eng->sync();
+    setup_mark("staging", st);
   }
   ~GeneralParams() override {
     pw = PcsDev<T>();

@@ -549,9 +549,13 @@ template <class T> struct Verifier {
       e.prof = nullptr;
       Pool& pool = cx.pool;
       Csr<T> ac, bc, cc;
-      ac.build(e, A, true);
-      bc.build(e, B, true);
-      cc.build(e, Cm, true);
+      {
+        DevSetup<T> ds(e);
+        DevCoo<T> coo(e, {&A, &B, &Cm});
+        ds.csr(ac, coo.c.p, coo.r.p, coo.v.p, A.nnz, m);
+        ds.csr(bc, coo.c.p + coo.off[1], coo.r.p + coo.off[1], coo.v.p + coo.off[1], B.nnz, m);
+        ds.csr(cc, coo.c.p + coo.off[2], coo.r.p + coo.off[2], coo.v.p + coo.off[2], Cm.nnz, m);
+      }
       DBuf<L> eq_s1 = e.eq_table(s1.data(), sp.log_n), eq_r = e.eq_table(rt.data(), sp.log_n);
       DBuf<L> eq_s2 = e.eq_table(s2.data(), sp.log_m);
       DBuf<L> geq(&pool, n * E), g2eq(&pool, n * E), mrow(&pool, m * E);
