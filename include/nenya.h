@@ -154,7 +154,8 @@ int nenya_prove_general_device_noalloc(const nenya_params* p, const void* d_w, s
 int nenya_set_timing(nenya_params* p, int enabled);
 int nenya_last_timing(const nenya_params* p, double* ms, int cap, char* names, size_t names_cap);
 /* Wall-clock timing (ms) of the stages of the setup that made `p`
- * (pcs_params: the PCS parameters with the RAA permutations and diagonals,
+ * (context: the params' stream, pool and device constants; pcs_params: the
+ * PCS parameters with the RAA permutations and diagonals,
  * raa.rs:43-63; pcs_upload; matrices: the matrix packing of
  * pack_general_matrices / pack_mimd_matrices, reduce.rs:249-273, 372-438,
  * 618-651; staging), in the same format as nenya_last_timing (no reference

@@ -159,6 +159,7 @@ template <class T> struct SimdParams : ParamsBase {
     seeds[1] = seed_q;
     log_n = ln;
     log_m = lm;
+    setup_mark("context", cx0.st);  // the params' stream, pool and constants
     pw.P = PcsParams::setup(lm, seed_w, T::K);
     pq.P = PcsParams::setup_r2(ln, seed_q, T::K);
     setup_mark("pcs_params", cx0.st);
@@ -386,6 +387,7 @@ template <class T> struct MimdParams : ParamsBase {
     m = size_t(1) << lm;
     Pool& pool = cx0.pool;
     Engine<T>* eng = cx0.eng.get();
+    setup_mark("context", cx0.st);  // the params' stream, pool and constants
     pw.P = PcsParams::setup(lm, seed_w, T::K);
     pabc.P = PcsParams::setup_stacked(ln, seed_w ^ 0xABC, 3, T::K);
     pe.P = PcsParams::setup_r2(ln, seed_e, T::K);
@@ -771,6 +773,7 @@ template <class T> struct GeneralParams : ParamsBase {
     hipStream_t st = cx0.st;
     Engine<T>* eng = cx0.eng.get();
     footprint = (size_t)(170.0 * E * sizeof(L) * nb) + (size_t)nb * 1024 * sizeof(L);
+    setup_mark("context", st);  // the params' stream, pool and constants
     pw.P = PcsParams::setup_batched(lb, seed_w, 32, T::K);
     pabc.P = PcsParams::setup_stacked(lb, seed_w ^ 0xABC, 3, T::K);
     pe.P = PcsParams::setup_r2(lb, seed_e, T::K);
