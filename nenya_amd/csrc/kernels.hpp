@@ -1162,41 +1162,6 @@ __global__ void __launch_bounds__(E) k_spmv_heavy_sum(const LT<T>* part, uint32_
   out[o * E + l] = (LT<T>)s;
 }
 
-// Sparse R2 x R2 accumulation: out[o] = sum_{k in ptr[o]..} D[k] * X[xsel[k]][xidx[k]]
-// with D low-degree (packed diagonal cells).  Used for layered_matvec
-// (general.rs:211-230; X = rho layers) and build_layer_rows (general.rs:233-271;
-// X = {eq, gamma*eq, gamma^2*eq}).
-template <class T>
-__global__ void __launch_bounds__(GB) k_spmv_gr(const uint64_t* ptr, const uint32_t* dk, const uint32_t* xsel,
-                                                const uint32_t* xidx, const LT<T>* D, const LT<T>* X,
-                                                uint64_t xstride, LT<T>* out, uint64_t nout) {
-  __shared__ __attribute__((aligned(16))) LT<T> sh[GPB][2][E];
-  int g = threadIdx.x >> 5, t = threadIdx.x & 31;
-  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < nout; base += (uint64_t)gridDim.x * GPB) {
-    uint64_t o = base + g;
-    bool ok = o < nout;
-    uint64_t k0 = ok ? ptr[o] : 0, k1 = ok ? ptr[o + 1] : 0;
-    // both groups of the wave iterate the max count to keep barriers uniform
-    uint64_t cnt = k1 - k0, other = __shfl_xor(cnt, 32, 64), mx = cnt > other ? cnt : other;
-    Acc<T> acc;
-    acc.clear();
-    for (uint64_t j = 0; j < mx; j++) {
-      bool act = j < cnt;
-      if (act) {
-        uint64_t k = k0 + j;
-        stage<T>(sh[g][0], D + (uint64_t)dk[k] * E, t);
-        stage<T>(sh[g][1], X + (uint64_t)xsel[k] * xstride * E + (uint64_t)xidx[k] * E, t);
-      }
-      __syncthreads();
-      if (act) acc_mul<T, 16>(acc, sh[g][0], sh[g][1], t);
-      __syncthreads();
-    }
-    CT<T> r[4];
-    acc_final<T>(acc, t, r);
-    if (ok) st4<T>(out + o * E + 4 * t, r);
-  }
-}
-
 // m-bar (pcs.rs:554-561): mbar[j] = sum_{i<rows} e0s[i] * M[i*k2 + j]  (M low-degree if NA=16)
 template <class T, int NA>
 __global__ void __launch_bounds__(GB) k_mbar(const LT<T>* e0s, const LT<T>* M, uint64_t rows, uint64_t k2,

@@ -27,8 +27,12 @@ namespace nenya {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-template <class T> constexpr bool kMx = T::K <= 32;  // 64-bit limbs stay on the vector ALUs
+template <class T> constexpr bool kMx = T::K <= 32;  // Horner-accumulated digit GEMMs (mod 2^32)
 template <class T> constexpr int kND = T::K / 8;     // balanced base-256 digits per limb
+// 64-bit limbs: fixed-multiplier products (folds, scalings) on the matrix
+// cores with exact digit classes (k_fold_mx64, k_mulfix_mx64)
+template <class T> constexpr bool kMx64 = T::K == 64;
+template <class T> constexpr bool kFrag = kMx<T> || kMx64<T>;  // challenges carry MFMA fragments
 
 // Balanced digits of v (mod 2^(8 ND)): byte p of the result is digit p.
 template <int ND> __device__ __forceinline__ uint32_t bal_digits(uint32_t v) {
@@ -641,6 +645,220 @@ __global__ void __launch_bounds__(256) k_cgemm_valu(const LT<T>* b2, const LT<T>
     for (int i = 0; i < PG; i++) {
       const uint64_t p = p0 + g * PG + i;
       if (p < nb) out[p * E + limb] = (LT<T>)acc[i];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fixed-multiplier products with 64-bit limbs on the matrix cores.
+//
+// The Horner accumulation above relies on the i32 accumulators wrapping mod
+// 2^32.  With k = 64 every digit class
+//   Y_s = sum_{p+q=s} M_q x_p     (s < 8, balanced base-256 digits)
+// is kept exact instead: |M_q x_p| <= 128 * 2^14 = 2^21 per 128-term row and
+// at most 8 pairs per class, so |Y_s| <= 2^24 fits an i32 accumulator, and
+// M x = sum_s 256^s Y_s (mod 2^64) is formed in 64-bit Horner on the vector
+// ALUs.  36 digit-pair MFMAs per k-block and row-block (10 at k = 32).  The
+// fragments of M (8 digit planes, 128 KB) stay in LDS for the block's
+// lifetime; each fragment is read once per row-block and applied to the
+// 8 - q classes it feeds (eight class accumulators, 128 AGPRs).
+__device__ __forceinline__ uint64_t bal_digits64(uint64_t v) {
+  return (v + 0x8080808080808080ull) ^ 0x8080808080808080ull;
+}
+// 16 limbs -> 8 digit planes of 16 int8 each
+__device__ __forceinline__ void planes16_64(const uint64_t v[16], v4i out[8]) {
+  uint64_t z[16];
+#pragma unroll
+  for (int b = 0; b < 16; b++) z[b] = bal_digits64(v[b]);
+#pragma unroll
+  for (int p = 0; p < 8; p++) {
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+      uint32_t x = 0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) x |= (uint32_t)((z[4 * w + j] >> (8 * p)) & 0xffu) << (8 * j);
+      out[p][w] = (int)x;
+    }
+  }
+}
+// The 16 limbs of half h in k-block kb (kappa order), 64-bit.
+__device__ __forceinline__ void ld16k64(const uint64_t* el, int kb, int h, uint64_t v[16]) {
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const ulonglong2* q = reinterpret_cast<const ulonglong2*>(el + 32 * kb + 8 * j + 4 * h);
+    const ulonglong2 a = q[0], b = q[1];
+    v[4 * j] = a.x;
+    v[4 * j + 1] = a.y;
+    v[4 * j + 2] = b.x;
+    v[4 * j + 3] = b.y;
+  }
+}
+// Fragments of M_r (8 digit planes) from v[i] = Y^i r, as mulmat_frag;
+// items [id0, id1) of the E * 8 (row, k-block, half) items.
+__device__ __forceinline__ void mulmat_frag64(const uint64_t (*v)[E], uint32_t* frag, int id0, int id1) {
+  for (int id = id0 + threadIdx.x; id < id1; id += blockDim.x) {
+    const int row = id & (E - 1), kb = (id >> 7) & 3, h = id >> 9;
+    const int t = row >> 2, cp = row & 3;
+    uint32_t words[8][4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int i = 8 * kb + 2 * j + h;
+      uint64_t a[4] = {v[i][4 * t], v[i][4 * t + 1], v[i][4 * t + 2], v[i][4 * t + 3]};
+      uint64_t z[4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        z[c] = bal_digits64(a[cp]);
+        const uint64_t n0 = 0 - a[3], n1 = a[0] - a[3], n2 = a[1], n3 = a[2];
+        a[0] = n0; a[1] = n1; a[2] = n2; a[3] = n3;
+      }
+#pragma unroll
+      for (int p = 0; p < 8; p++) {
+        uint32_t wd = 0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) wd |= (uint32_t)((z[c] >> (8 * p)) & 0xffu) << (8 * c);
+        words[p][j] = wd;
+      }
+    }
+    const int rb = row >> 5, lane = (row & 31) + 32 * h;
+#pragma unroll
+    for (int p = 0; p < 8; p++)
+      reinterpret_cast<uint4*>(frag)[frag_at(p, kb, rb) * 64 + lane] =
+          make_uint4(words[p][0], words[p][1], words[p][2], words[p][3]);
+  }
+}
+// M_r fragments for r in device memory; blocks (b, y) handle r + b
+// (fragments at frag + b * 8 * 4096 words), block y the y-th share of the
+// fragment items (each block forms the Y^i r chain itself: it is cheap, the
+// digit planes are not).
+__global__ void __launch_bounds__(256) k_mulmat64(const uint64_t* r, uint32_t* frag) {
+  __shared__ uint64_t v[32][E];
+  r += (uint64_t)blockIdx.x * E;
+  frag += (uint64_t)blockIdx.x * 8 * 4096;
+  if (threadIdx.x < E) v[0][threadIdx.x] = r[threadIdx.x];
+  __syncthreads();
+  mulmat_chain<uint64_t>(v);
+  const int per = E * 8 / gridDim.y;
+  mulmat_frag64(v, frag, blockIdx.y * per, (blockIdx.y + 1) * per);
+}
+// Row-block rb of M x for one 32-element tile: R[i] (lane (col, h), rows
+// 8(i/4) + 4h + i%4 of the row-block) from the digit planes B of x.
+__device__ __forceinline__ void mul_rb64(const v4i* A, const v4i B[4][8], int rb, uint64_t R[16]) {
+  const int l = threadIdx.x & 63;
+  v16i acc[8];
+#pragma unroll
+  for (int s = 0; s < 8; s++) acc[s] = (v16i)0;
+#pragma unroll
+  for (int q = 0; q < 8; q++)
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++) {
+      const v4i af = A[frag_at(q, kb, rb) * 64 + l];
+#pragma unroll
+      for (int p = 0; p + q < 8; p++) acc[q + p] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af, B[kb][p], acc[q + p], 0, 0, 0);
+    }
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    uint64_t x = (uint64_t)(int64_t)acc[7][i];
+#pragma unroll
+    for (int s = 6; s >= 0; s--) x = (x << 8) + (uint64_t)(int64_t)acc[s][i];
+    R[i] = x;
+  }
+}
+// Digit planes of a tile's 32 elements (x[e] at x + e * E, or their
+// differences hi - lo when lo is given); zero for elements past the end.
+__device__ __forceinline__ void tile_planes64(const uint64_t* x, const uint64_t* lo, bool ok, v4i B[4][8]) {
+  const int h = (threadIdx.x & 63) >> 5;
+#pragma unroll
+  for (int kb = 0; kb < 4; kb++) {
+    uint64_t d[16];
+    if (ok) {
+      ld16k64(x, kb, h, d);
+      if (lo) {
+        uint64_t a[16];
+        ld16k64(lo, kb, h, a);
+#pragma unroll
+        for (int b = 0; b < 16; b++) d[b] -= a[b];
+      }
+    } else {
+#pragma unroll
+      for (int b = 0; b < 16; b++) d[b] = 0;
+    }
+    planes16_64(d, B[kb]);
+  }
+}
+
+// Sumcheck fold with 64-bit limbs on the matrix cores (same result as
+// k_fold): dst[tab][i] = lo + r (hi - lo).  Persistent blocks of 4 waves,
+// a wave per 32-element tile; the fold may be in place (a wave reloads the
+// lo limbs of a row-block before it stores that row-block, and no other
+// wave touches its tile).  With a side task the last block runs it.
+template <class T>
+__global__ void __launch_bounds__(256) k_fold_mx64(const uint32_t* frag, FoldJob<T> J) {
+  static_assert(T::K == 64, "64-bit limbs");
+  __shared__ v4i sA[8 * 16 * 64];
+  const uint64_t total = J.m * (J.ntab[0] + J.ntab[1]), ntiles = (total + 31) / 32;
+  const unsigned nfold = gridDim.x - (J.side ? 1 : 0);
+  if (J.side && blockIdx.x == nfold) {
+    if (J.side == 1) eq_one_256<T>(J.sa, J.sb, J.so);
+    else lin_claim_256<T>(J.sa, J.sb, J.so);
+    return;
+  }
+  stage_frags<8>(reinterpret_cast<const v4i*>(frag), sA);
+  __syncthreads();
+  const int l = threadIdx.x & 63, col = l & 31, h = l >> 5;
+  for (uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntiles; tile += (uint64_t)nfold * 4) {
+    asm volatile("" ::: "memory");
+    const uint64_t* lo;
+    uint64_t* out;
+    const bool ok = fold_locate<T>(J, tile * 32 + col, lo, out);
+    v4i B[4][8];
+    tile_planes64(lo + J.m * E, lo, ok, B);
+#pragma unroll 1
+    for (int rb = 0; rb < 4; rb++) {
+      uint64_t R[16];
+      mul_rb64(sA, B, rb, R);
+      if (ok) {
+        uint64_t a[16];
+        ld16k64(lo, rb, h, a);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          ulonglong2* o = reinterpret_cast<ulonglong2*>(out + 32 * rb + 8 * j + 4 * h);
+          o[0] = make_ulonglong2(a[4 * j] + R[4 * j], a[4 * j + 1] + R[4 * j + 1]);
+          o[1] = make_ulonglong2(a[4 * j + 2] + R[4 * j + 2], a[4 * j + 3] + R[4 * j + 3]);
+        }
+      }
+    }
+  }
+}
+
+// out[e * nmat + i] = M_i x[e] for the nmat fixed matrices of frag (matrix i
+// at fragment offset i * 8 * 4096 words); block (bx, i) keeps M_i in LDS.
+__global__ void __launch_bounds__(256) k_mulfix_mx64(const uint32_t* frag, const uint64_t* x, uint64_t n,
+                                                     uint64_t* out) {
+  __shared__ v4i sA[8 * 16 * 64];
+  const int i = blockIdx.y, nmat = gridDim.y;
+  stage_frags<8>(reinterpret_cast<const v4i*>(frag) + (size_t)i * 8 * 16 * 64, sA);
+  __syncthreads();
+  const int l = threadIdx.x & 63, col = l & 31, h = l >> 5;
+  const uint64_t ntiles = (n + 31) / 32;
+  for (uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntiles; tile += (uint64_t)gridDim.x * 4) {
+    asm volatile("" ::: "memory");
+    const uint64_t e = tile * 32 + col;
+    const bool ok = e < n;
+    v4i B[4][8];
+    tile_planes64(x + (ok ? e : 0) * E, nullptr, ok, B);
+#pragma unroll 1
+    for (int rb = 0; rb < 4; rb++) {
+      uint64_t R[16];
+      mul_rb64(sA, B, rb, R);
+      if (ok) {
+        uint64_t* o = out + (e * nmat + i) * E + 32 * rb + 4 * h;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          ulonglong2* q = reinterpret_cast<ulonglong2*>(o + 8 * j);
+          q[0] = make_ulonglong2(R[4 * j], R[4 * j + 1]);
+          q[1] = make_ulonglong2(R[4 * j + 2], R[4 * j + 3]);
+        }
+      }
     }
   }
 }

@@ -369,6 +369,15 @@ template <class T> struct Engine {
       d_lbfrag = DBuf<uint32_t>(pool, K.lbfrag.size());
       h2d(d_lbfrag.p, K.lbfrag.data(), K.lbfrag.size() * 4);
     }
+    if constexpr (kMx64<T>) {
+      // x -> l_beta_i x fragments (8 digit planes each), made on the device
+      DBuf<L> lb(pool, 16 * E);
+      for (int i = 0; i < 16; i++) h2d(lb.p + (size_t)i * E, K.lbeta[i].v, E * sizeof(L));
+      d_lbfrag = DBuf<uint32_t>(pool, (size_t)16 * 8 * 4096);
+      hipLaunchKernelGGL(k_mulmat64, dim3(16, 4), dim3(256), 0, st, (const uint64_t*)lb.p, d_lbfrag.p);
+      HIP_CHECK(hipGetLastError());
+      sync();
+    }
     sync();
   }
 
@@ -557,7 +566,28 @@ template <class T> struct Engine {
         return;
       }
     }
+    if constexpr (kMx64<T>) {
+      if (n >= 512) {
+        DBuf<uint32_t> fr(pool, (size_t)8 * 4096);
+        mulmat64(s, fr.p);
+        launch("k_scale_mx64", n * 2 * EB, (double)n * E * E, k_mulfix_mx64, dim3(grid_for((n + 31) / 32, 4, 256), 1),
+               dim3(256), (const uint32_t*)fr.p, (const uint64_t*)X, (uint64_t)n, (uint64_t*)out);
+        return;
+      }
+    }
     launch("k_scale", n * 2 * EB, n * 32 * PM, (k_scale<T>), dim3(grid_for(n, GPB)), dim3(GB), s, X, out, n);
+  }
+  // G[e][i] = l_beta_i x[e] (i < 16) for nx elements: the products of the
+  // packing basis b_{2i+h} = l_beta_i u_h that the layer rows and M_gamma
+  // rows gather (a fixed-matrix product per i on the matrix cores)
+  void mulfix_lb(const L* x, uint64_t nx, L* G) {
+    const uint64_t ntiles = (nx + 31) / 32;
+    if constexpr (kMx64<T>)
+      launch("k_mulfix_mx64", nx * 17.0 * EB, nx * 16.0 * E * E, k_mulfix_mx64, dim3(grid_for(ntiles, 4, 16), 16),
+             dim3(256), (const uint32_t*)d_lbfrag.p, (const uint64_t*)x, nx, (uint64_t*)G);
+    else
+      launch("k_mulfix_mx", nx * 17.0 * EB, nx * 16.0 * E * E, (k_mulfix_mx<T>), dim3(grid_for(ntiles, 4, 32), 16),
+             dim3(256), (const uint32_t*)d_lbfrag.p, x, nx, G);
   }
   void sub(const L* a, const L* b, L* o, uint64_t nl) {
     launch("k_sub", nl * 3.0 * sizeof(L), 0, (k_sub<T>), dim3(grid_for(nl, 256, 1u << 20)), dim3(256), a, b, o, nl);
@@ -581,8 +611,25 @@ template <class T> struct Engine {
       mm.frag = DBuf<uint32_t>(pool, (size_t)kND<T> * 4096);
       mm.fp = mm.frag.p;
       launch("k_mulmat", EB + kND<T> * 16384.0, 0, (k_mulmat<T>), dim3(1), dim3(256), r, mm.frag.p);
+    } else if constexpr (kMx64<T>) {
+      mm.frag = DBuf<uint32_t>(pool, (size_t)8 * 4096);
+      mm.fp = mm.frag.p;
+      mulmat64(r, mm.frag.p);
     }
     return mm;
+  }
+  // The fragment slot of sumcheck round r for a fold of fold_elems elements
+  // (64-bit limbs: none below MX64_MIN, where the fold stays on the vector
+  // ALUs and the fragments would not be read).
+  static uint32_t* round_frag(uint32_t* frags, size_t r, uint64_t fold_elems) {
+    if constexpr (kMx64<T>) return fold_elems >= MX64_MIN ? frags + r * 8 * 4096 : nullptr;
+    else return frags + (kMx<T> ? r * kND<T> * 4096 : 0);
+  }
+  // 64-bit limbs: folds of at least MX64_MIN elements use the matrix cores
+  // (below that the fragments, 128 KB, cost more than they save)
+  static constexpr uint64_t MX64_MIN = 4096;
+  void mulmat64(const L* r, uint32_t* frag) {
+    launch("k_mulmat64", EB + 8 * 16384.0, 0, k_mulmat64, dim3(1, 4), dim3(256), (const uint64_t*)r, frag);
   }
   // A scalar update that needs nothing from the fold it accompanies: on the
   // matrix-core path it runs in an extra block of the fold launch, otherwise
@@ -621,6 +668,18 @@ template <class T> struct Engine {
       launch("k_fold_mx", m * (J.ntab[0] + J.ntab[1]) * 3 * EB, m * (J.ntab[0] + J.ntab[1]) * 32 * PM, (k_fold_mx<T>),
              dim3(nb + (sd.kind ? 1 : 0)), dim3(256), mm.fp, J, split);
     } else {
+      if constexpr (kMx64<T>) {
+        const uint64_t tot = m * (n0 + (s1 ? n1 : 0));
+        if (mm.fp && tot >= MX64_MIN) {
+          FoldJob<T> J{{s0, s1 ? s1 : s0}, {d0, d1 ? d1 : d0}, {n0, s1 ? n1 : 0}, {ss0, ss1}, {ds0, ds1}, m,
+                       sd.kind, sd.sa, sd.sb, sd.so};
+          // one block per CU at most (each keeps the 128 KB of fragments in LDS)
+          const unsigned nb = grid_for((tot + 31) / 32, 4, 256);
+          launch("k_fold_mx64", tot * 3 * EB, tot * 32 * PM, (k_fold_mx64<T>), dim3(nb + (sd.kind ? 1 : 0)), dim3(256),
+                 mm.fp, J);
+          return;
+        }
+      }
       fold(mm.r, s0, d0, m, n0, ss0, ds0);
       if (s1) fold(mm.r, s1, d1, m, n1, ss1, ds1);
       side_kernel(sd);
@@ -954,6 +1013,12 @@ template <class T> struct Engine {
       launch("k_lift_mulmat_fs", 5 * EB, lm, (k_lift_mulmat_fs<T>), dim3(1), dim3(256), f, d, d_frob.p, T::K - 1, frag);
     } else {
       launch("k_lift_fs", 5 * EB, lm, (k_lift_fs<T>), dim3(1), dim3(256), f, d, d_frob.p, T::K - 1);
+      if constexpr (kMx64<T>) {
+        if (frag) {
+          mm->fp = frag;
+          mulmat64(d, frag);
+        }
+      }
     }
   }
   // The device transcript of a sumcheck: the host transcript's state
@@ -1005,6 +1070,12 @@ template <class T> struct Engine {
              w);
     } else {
       launch("k_lift", 2 * EB, lm, (k_lift_from<T>), dim3(1), dim3(256), src, d, d_frob.p, T::K - 1, w);
+      if constexpr (kMx64<T>) {
+        if (frag) {
+          mm->fp = frag;
+          mulmat64(d, frag);
+        }
+      }
     }
   }
   // Host: sample the challenge (the transcript's sampler) and post it.
@@ -1404,7 +1475,7 @@ template <class T> struct Engine {
     unsigned nbk_max = 1;
     for (size_t m = N / 2; m >= 1; m /= 2) nbk_max = std::max(nbk_max, blocks(m));
     DBuf<L> part(pool, (size_t)nbk_max * 3 * E);
-    DBuf<uint32_t> frags(pool, kMx<T> ? nv * kND<T> * 4096 : 1);
+    DBuf<uint32_t> frags(pool, kFrag<T> ? nv * kND<T> * 4096 : 1);
     MbGuard guard(*this);
     const bool pipe = pipeline_rounds();
     // contexts that do not pipeline: Fiat-Shamir of the rounds on the device
@@ -1448,10 +1519,10 @@ template <class T> struct Engine {
       };
       if (dfs) {
         checkpoint();
-        lift_device_fs(dtr.p, drounds.p + r * 3 * E, rc, frags.p + (kMx<T> ? r * kND<T> * 4096 : 0), &mr);
+        lift_device_fs(dtr.p, drounds.p + r * 3 * E, rc, round_frag(frags.p, r, (uint64_t)m * ntab), &mr);
       } else {
         if (!pipe) host_round();
-        lift_posted((int)r, vup, rc, frags.p + (kMx<T> ? r * kND<T> * 4096 : 0), &mr);
+        lift_posted((int)r, vup, rc, round_frag(frags.p, r, (uint64_t)m * ntab), &mr);
       }
       // the next round's eq factor eq_one(rt[r], r_r) rides along with the fold
       Side sd;
@@ -1512,7 +1583,7 @@ template <class T> struct Engine {
     // allocated before the first waiting kernel (see mult_check)
     const size_t nparts = kMx<T> ? GRAM_BLOCKS_MAX : grid_for((uint64_t)(N / 2 > 0 ? N / 2 : 1) * ntab, GPB, 2048);
     DBuf<L> part(pool, nparts * 2 * E);
-    DBuf<uint32_t> frags(pool, kMx<T> ? nv * kND<T> * 4096 : 1);
+    DBuf<uint32_t> frags(pool, kFrag<T> ? nv * kND<T> * 4096 : 1);
     MbGuard guard(*this);
     const bool pipe = pipeline_rounds();
     const bool dfs = !pipe && devfs;  // as in mult_check
@@ -1559,10 +1630,10 @@ template <class T> struct Engine {
       };
       if (dfs) {
         checkpoint();
-        lift_device_fs(dtr.p, drounds.p + r * 3 * E, rc, frags.p + (kMx<T> ? r * kND<T> * 4096 : 0), &mr);
+        lift_device_fs(dtr.p, drounds.p + r * 3 * E, rc, round_frag(frags.p, r, (uint64_t)m * 2 * ntab), &mr);
       } else {
         if (!pipe) host_round();
-        lift_posted((int)r, vup, rc, frags.p + (kMx<T> ? r * kND<T> * 4096 : 0), &mr);
+        lift_posted((int)r, vup, rc, round_frag(frags.p, r, (uint64_t)m * 2 * ntab), &mr);
       }
       // the next claim q_r(r_r) rides along with the folds
       fold2(mr, m, f, fw.p, ntab, fs, H, g, gw.p, ntab, gs, H, Side{2, dq.p, rc, dc.p});

@@ -7,7 +7,7 @@
 // (rocPRIM's LSD radix sort is stable), so entries keep their input order
 // within a group exactly as the host's counting sorts and stable sorts left
 // them; group pointers are lower bounds of the group ids in the sorted keys;
-// merges of equal keys (diagonal cells, MIMD cells) are runs of the sorted
+// merges of equal keys (the MIMD union-support cells) are runs of the sorted
 // keys, numbered by a scan of run heads.
 #pragma once
 #include <cstring>
@@ -88,15 +88,6 @@ __global__ void __launch_bounds__(256) k_layer_place(const uint32_t* perm, const
   }
 }
 
-// Diagonal-cell keys of General (64-bit limbs): (l * nb + p) * nb + q of
-// entry (r, c) with l = (c - r) mod 32, p = r / 32, q = c / 32.
-__global__ void __launch_bounds__(256) k_cell_keys(const uint32_t* r, const uint32_t* c, uint64_t n, uint64_t nb,
-                                                   uint64_t* key) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t rr = r[i], cc = c[i];
-    key[i] = ((uint64_t)((cc - rr) & 31u) * nb + rr / 32) * nb + cc / 32;
-  }
-}
 // MIMD cell keys r * m + c
 __global__ void __launch_bounds__(256) k_rc_keys(const uint32_t* r, const uint32_t* c, uint64_t n, uint64_t m,
                                                  uint64_t* key) {
@@ -113,8 +104,8 @@ __global__ void __launch_bounds__(256) k_run_heads(const K* key, uint64_t n, uin
 
 // Cells from runs: run j (heads scanned inclusively: run id = id[i] - 1)
 // owns positions [start_j, start_{j+1}); the thread at a run head sums the
-// run's values into the cell's 32 slots (slot of entry e: slot[e]) and
-// records the cell's key.  slots must start zeroed.
+// run's values into the cell's 32 slots (slot of entry e: slot[e], its
+// instance) and records the cell's key.  slots must start zeroed.
 template <class L>
 __global__ void __launch_bounds__(256) k_cell_sum(const uint64_t* skey, const uint32_t* perm, const uint32_t* id,
                                                   uint64_t n, const uint8_t* slot, const L* v, uint64_t* ckey, L* slots) {
@@ -129,31 +120,6 @@ __global__ void __launch_bounds__(256) k_cell_sum(const uint64_t* skey, const ui
       L& x = out[slot[e]];
       x = (L)((C)x + (C)v[e]);
     }
-  }
-}
-
-// Slot of each General entry: r % 32
-__global__ void __launch_bounds__(256) k_row_slot(const uint32_t* r, uint64_t n, uint8_t* slot) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    slot[i] = (uint8_t)(r[i] & 31u);
-}
-
-// General cells -> the layer-row CSR entries over (l, q): key l * nb + q,
-// and per cell (global index g) its block row p and matrix w
-__global__ void __launch_bounds__(256) k_cell_lq(const uint64_t* ckey, uint64_t ncell, uint64_t nb, uint32_t* key) {
-  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ncell; g += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t k = ckey[g];
-    key[g] = (uint32_t)((k / (nb * nb)) * nb + k % nb);
-  }
-}
-__global__ void __launch_bounds__(256) k_cell_rows_place(const uint32_t* perm, const uint64_t* ckey, uint64_t n,
-                                                         uint64_t nb, uint64_t c1, uint64_t c2, uint32_t* dk,
-                                                         uint32_t* sel, uint32_t* idx) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t g = perm[i];
-    dk[i] = g;
-    sel[i] = g >= c2 ? 2u : g >= c1 ? 1u : 0u;
-    idx[i] = (uint32_t)((ckey[g] / nb) % nb);
   }
 }
 
@@ -190,18 +156,6 @@ __global__ void __launch_bounds__(256) k_nz_codes(const uint32_t* perm, const ui
     const uint32_t w = g >= c2 ? 2u : g >= c1 ? 1u : 0u;
     code[i] = (row[g] << 7) | (w << 5) | sl;
     val[i] = cval[x];
-  }
-}
-// MIMD M_gamma-row entries (vector ALUs): per cell by column (dk = g,
-// sel = w, idx = row)
-__global__ void __launch_bounds__(256) k_cellcol_place(const uint32_t* perm, const uint32_t* row, uint64_t n,
-                                                       uint64_t c1, uint64_t c2, uint32_t* dk, uint32_t* sel,
-                                                       uint32_t* idx) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t g = perm[i];
-    dk[i] = g;
-    sel[i] = g >= c2 ? 2u : g >= c1 ? 1u : 0u;
-    idx[i] = row[g];
   }
 }
 // MIMD entries' instance numbers from the per-instance offsets

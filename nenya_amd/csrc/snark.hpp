@@ -356,17 +356,13 @@ template <class T> struct MimdParams : ParamsBase {
   Ctx<T> cx0;
   PcsDev<T> pw, pabc, pe;
   // union-support cells of A, B, C (pack_matrix_set, reduce.rs:119-141),
-  // concatenated: 32 instance values and the packed entry per cell
+  // concatenated: the 32 instance values per cell
   size_t ncell[3] = {0, 0, 0};
   DBuf<uint64_t> cptr;  // 3 x (n+1) row pointers (global cell indices)
   DBuf<uint32_t> ccol;
-  DBuf<L> cval, dhat;
-  // vector-ALU path (k = 64): matvec entries by row (k_spmv_gr layout) and
-  // M_gamma-row entries by column, over the packed cells dhat
-  DBuf<uint64_t> rwptr;
-  DBuf<uint32_t> rwdk, rwsel, rwidx;
-  // matrix-core path (k <= 32): M_gamma-row entries by column, one per
-  // nonzero instance value, code = y << 7 | w << 5 | s (k_rows_gather)
+  DBuf<L> cval;
+  // M_gamma-row entries by column, one per nonzero instance value,
+  // code = y << 7 | w << 5 | s (k_rows_gather)
   DBuf<uint64_t> eptr;
   DBuf<uint32_t> ecode;
   DBuf<L> eval;
@@ -470,7 +466,7 @@ template <class T> struct MimdParams : ParamsBase {
       eng->launch("k_group_ptr", 0, 0, (k_group_ptr<uint32_t, uint64_t>), dim3(ds.grid(n + 1)), dim3(256),
                   (const uint32_t*)(crow.p + cbase[w]), (uint64_t)ncell[w], (uint64_t)n, (uint64_t)cbase[w],
                   cptr.p + w * (n + 1));
-    if constexpr (kMx<T>) {
+    {
       if (n > (size_t(1) << 25)) throw Error(NENYA_E_SHAPE, "too many rows for the row-entry code");
       // M_gamma-row entries: the nonzero instance values by column, in
       // (cell, slot) order within a column
@@ -501,40 +497,13 @@ template <class T> struct MimdParams : ParamsBase {
       setup_mark("matrices", cx0.st);
       footprint = (size_t)((24.0 * E + 3 * 32) * sizeof(L) * n + (8.0 * E + 64) * sizeof(L) * m +
                            (48.0 * E + 1024) * sizeof(L) * n);
-      return;
     }
-    dhat = DBuf<L>(&pool, std::max<size_t>(total, 1) * E);
-    if (total) eng->pack(cval.p, dhat.p, total, 1);
-    // M_gamma row entries by column: dhat[g] * eqs[w][row(g)] (mimd.rs:196-219)
-    {
-      rwptr = DBuf<uint64_t>(&pool, m + 1);
-      const uint64_t nt = std::max<uint64_t>(total, 1);
-      rwdk = DBuf<uint32_t>(&pool, nt);
-      rwsel = DBuf<uint32_t>(&pool, nt);
-      rwidx = DBuf<uint32_t>(&pool, nt);
-      DBuf<uint32_t> skey(&pool, nt), perm(&pool, nt);
-      if (total) {
-        DBuf<uint32_t> id = ds.iota(total);
-        ds.template sort_pairs<uint32_t>(ccol.p, skey.p, id.p, perm.p, total, bits_for(m - 1));
-        eng->launch("k_cellcol_place", 0, 0, k_cellcol_place, dim3(ds.grid(total)), dim3(256), (const uint32_t*)perm.p,
-                    (const uint32_t*)crow.p, (uint64_t)total, (uint64_t)cbase[1], (uint64_t)cbase[2], rwdk.p, rwsel.p,
-                    rwidx.p);
-      }
-      eng->launch("k_group_ptr", 0, 0, (k_group_ptr<uint32_t, uint64_t>), dim3(ds.grid(m + 1)), dim3(256),
-                  (const uint32_t*)skey.p, (uint64_t)total, (uint64_t)m, (uint64_t)0, rwptr.p);
-    }
-    setup_mark("matrices", cx0.st);
-    // setup data is read by other streams (server workers): finish it here
-    eng->sync();
-    footprint = (size_t)((24.0 * E + 3 * 32) * sizeof(L) * n + (8.0 * E + 64) * sizeof(L) * m +
-                         1024 * sizeof(L) * n);
   }
   ~MimdParams() override {
     pw = PcsDev<T>();
     pabc = PcsDev<T>();
     pe = PcsDev<T>();
-    cptr.release(); ccol.release(); cval.release(); dhat.release();
-    rwptr.release(); rwdk.release(); rwsel.release(); rwidx.release();
+    cptr.release(); ccol.release(); cval.release();
     eptr.release(); ecode.release(); eval.release();
   }
 
@@ -575,21 +544,12 @@ template <class T> struct MimdParams : ParamsBase {
   // build_combined_row (mimd.rs:196-219): M[col] = sum over the three
   // matrices of eqs[w][y] Xhat[y, col], eqs = {eq, gamma eq, gamma^2 eq} (3 x n)
   void m_row(Engine<T>& e, const L* eqs, L* mrow) {
-    if constexpr (kMx<T>) {
-      // G[w][y][i] = eqs[w][y] l_beta_i, then entries v * G[w][y][s/2] u_{s%2}
-      DBuf<L> G(e.pool, 3 * n * 16 * E);
-      const uint64_t nx = 3 * n, ntiles = (nx + 31) / 32;
-      e.launch("k_mulfix_mx", nx * 17.0 * Engine<T>::EB, nx * 16.0 * E * E, (k_mulfix_mx<T>),
-               dim3(grid_for(ntiles, 4, 32), 16), dim3(256), (const uint32_t*)e.d_lbfrag.p, eqs, nx, G.p);
-      e.launch("k_rows_gather", 0, 0, (k_rows_gather<T>), dim3(grid_for(m, 8, 1u << 16)), dim3(256),
-               (const uint64_t*)eptr.p, (const uint32_t*)ecode.p, (const L*)eval.p, (const L*)G.p, (uint64_t)n, mrow,
-               (uint64_t)m);
-    } else {
-      const double ncl = (double)(ncell[0] + ncell[1] + ncell[2]);
-      e.launch("k_spmv_gr", ncl * 2 * Engine<T>::EB, ncl * 16 * Engine<T>::PM, (k_spmv_gr<T>), dim3(grid_for(m, GPB)),
-               dim3(GB), (const uint64_t*)rwptr.p, (const uint32_t*)rwdk.p, (const uint32_t*)rwsel.p,
-               (const uint32_t*)rwidx.p, (const L*)dhat.p, eqs, (uint64_t)n, mrow, (uint64_t)m);
-    }
+    // G[w][y][i] = eqs[w][y] l_beta_i, then entries v * G[w][y][s/2] u_{s%2}
+    DBuf<L> G(e.pool, 3 * n * 16 * E);
+    e.mulfix_lb(eqs, 3 * n, G.p);
+    e.launch("k_rows_gather", 0, 0, (k_rows_gather<T>), dim3(grid_for(m, 8, 1u << 16)), dim3(256),
+             (const uint64_t*)eptr.p, (const uint32_t*)ecode.p, (const L*)eval.p, (const L*)G.p, (uint64_t)n, mrow,
+             (uint64_t)m);
   }
 
   ProofView prove(Ctx<T>& cx, const L* wit /* 32 x m */) {
@@ -749,11 +709,7 @@ template <class T> struct GeneralParams : ParamsBase {
   Ctx<T> cx0;
   PcsDev<T> pw, pabc, pe;
   Csr<T> mr[3];  // A, B, C by row (base-ring matvecs)
-  DBuf<L> dhat;  // packed diagonal cells, A then B then C
-  size_t ncell[3] = {0, 0, 0};
-  DBuf<uint64_t> rptr;
-  DBuf<uint32_t> rdk, rsel, ridx;
-  DBuf<uint64_t> eptr;  // layer-row entries by output (matrix-core path)
+  DBuf<uint64_t> eptr;  // layer-row entries by output
   DBuf<uint32_t> ecode;
   DBuf<L> eval;
 
@@ -784,74 +740,18 @@ template <class T> struct GeneralParams : ParamsBase {
     setup_mark("pcs_upload", st);
     for (int w = 0; w < 3; w++) check_coo(m3[w], N, N);
     // pack_general_matrices on the device (stable sorts of the uploaded
-    // entries): the row CSRs, then the diagonal cells (vector-ALU width) or
-    // the layer-row entries (matrix-core widths)
+    // entries): the row CSRs, then the layer-row entries (the diagonal cells
+    // of reduce.rs:387-397 are never formed: the layer rows gather the
+    // entries against the basis products instead)
     DevSetup<T> ds(*eng);
     DevCoo<T> coo(*eng, {&m3[0], &m3[1], &m3[2]});
     const uint64_t ne = coo.off[3];
     if (ne >= (uint64_t(1) << 32)) throw Error(NENYA_E_SHAPE, "too many matrix entries");
     for (int w = 0; w < 3; w++)
       ds.csr(mr[w], coo.r.p + coo.off[w], coo.c.p + coo.off[w], coo.v.p + coo.off[w], m3[w].nnz, N);
-    if (!kMx<T>) {
-      // diagonal cells: nonzero (r,c) -> layer l = (c%32 - r%32) mod 32, p = r/32,
-      // q = c/32, slot i = r%32 (reduce.rs:387-397); cells of A, B, C
-      // concatenated, each matrix's in (l, p, q) order
-      DBuf<uint8_t> slot(&pool, std::max<uint64_t>(ne, 1));
-      eng->launch("k_row_slot", 0, 0, k_row_slot, dim3(ds.grid(ne)), dim3(256), (const uint32_t*)coo.r.p, ne, slot.p);
-      DBuf<uint64_t> ckey(&pool, std::max<uint64_t>(ne, 1));
-      DBuf<L> cslots(&pool, std::max<uint64_t>(ne, 1) * 32);
-      HIP_CHECK(hipMemsetAsync(cslots.p, 0, std::max<uint64_t>(ne, 1) * 32 * sizeof(L), st));
-      uint64_t cbase[4] = {0, 0, 0, 0};
-      for (int w = 0; w < 3; w++) {
-        const uint64_t nw = m3[w].nnz;
-        cbase[w + 1] = cbase[w];
-        if (!nw) continue;
-        DBuf<uint64_t> key(&pool, nw), skey(&pool, nw);
-        DBuf<uint32_t> perm(&pool, nw), head(&pool, nw);
-        eng->launch("k_cell_keys", 0, 0, k_cell_keys, dim3(ds.grid(nw)), dim3(256),
-                    (const uint32_t*)(coo.r.p + coo.off[w]), (const uint32_t*)(coo.c.p + coo.off[w]), nw, (uint64_t)nb,
-                    key.p);
-        {
-          DBuf<uint32_t> id = ds.iota(nw, (uint32_t)coo.off[w]);
-          ds.template sort_pairs<uint64_t>(key.p, skey.p, id.p, perm.p, nw, bits_for(32 * (uint64_t)nb * nb - 1));
-        }
-        eng->launch("k_run_heads", 0, 0, (k_run_heads<uint64_t>), dim3(ds.grid(nw)), dim3(256), (const uint64_t*)skey.p,
-                    nw, head.p);
-        uint32_t* id = reinterpret_cast<uint32_t*>(key.p);  // run numbers (1-based), over the unsorted keys
-        ds.scan(head.p, id, nw, true);
-        const uint64_t nc = ds.last_of(id, nw);
-        eng->launch("k_cell_sum", 0, 0, (k_cell_sum<L>), dim3(ds.grid(nw)), dim3(256), (const uint64_t*)skey.p,
-                    (const uint32_t*)perm.p, (const uint32_t*)id, nw, (const uint8_t*)slot.p, (const L*)coo.v.p,
-                    ckey.p + cbase[w], cslots.p + cbase[w] * 32);
-        ncell[w] = nc;
-        cbase[w + 1] = cbase[w] + nc;
-      }
-      const uint64_t total = cbase[3];
-      dhat = DBuf<L>(&pool, std::max<uint64_t>(total, 1) * E);
-      if (total) eng->pack(cslots.p, dhat.p, total, 1);
-      // build_layer_rows CSR over (l, q) merging A, B, C (xsel = matrix id)
-      const uint64_t nt = std::max<uint64_t>(total, 1);
-      rptr = DBuf<uint64_t>(&pool, 32 * nb + 1);
-      rdk = DBuf<uint32_t>(&pool, nt);
-      rsel = DBuf<uint32_t>(&pool, nt);
-      ridx = DBuf<uint32_t>(&pool, nt);
-      DBuf<uint32_t> key(&pool, nt), skey(&pool, nt), perm(&pool, nt);
-      if (total) {
-        eng->launch("k_cell_lq", 0, 0, k_cell_lq, dim3(ds.grid(total)), dim3(256), (const uint64_t*)ckey.p, total,
-                    (uint64_t)nb, key.p);
-        DBuf<uint32_t> id = ds.iota(total);
-        ds.template sort_pairs<uint32_t>(key.p, skey.p, id.p, perm.p, total, bits_for(32 * (uint64_t)nb - 1));
-        eng->launch("k_cell_rows_place", 0, 0, k_cell_rows_place, dim3(ds.grid(total)), dim3(256),
-                    (const uint32_t*)perm.p, (const uint64_t*)ckey.p, total, (uint64_t)nb, cbase[1], cbase[2], rdk.p,
-                    rsel.p, ridx.p);
-      }
-      eng->launch("k_group_ptr", 0, 0, (k_group_ptr<uint32_t, uint64_t>), dim3(ds.grid(32 * nb + 1)), dim3(256),
-                  (const uint32_t*)skey.p, total, (uint64_t)(32 * nb), (uint64_t)0, rptr.p);
-      eng->sync();
-    }
-    if (kMx<T>) {
-      // matrix-core layer rows: entries (row 32p+s of matrix w, value v) by
-      // output (l = (c - r) mod 32, q = c / 32), code p << 7 | w << 5 | s
+    {
+      // layer-row entries (row 32p+s of matrix w, value v) by output
+      // (l = (c - r) mod 32, q = c / 32), code p << 7 | w << 5 | s
       if (nb > (size_t(1) << 25)) throw Error(NENYA_E_SHAPE, "too many blocks for the layer-row entry code");
       const uint64_t nn = std::max<uint64_t>(ne, 1);
       eptr = DBuf<uint64_t>(&pool, 32 * nb + 1);
@@ -883,48 +783,24 @@ template <class T> struct GeneralParams : ParamsBase {
     pabc = PcsDev<T>();
     pe = PcsDev<T>();
     for (auto& m : mr) m = Csr<T>();
-    dhat.release();
-    rptr.release(); rdk.release(); rsel.release(); ridx.release();
     eptr.release(); ecode.release(); eval.release();
   }
 
   // build_layer_rows (general.rs:233-271): rows[l][q] (32 x nb) from
-  // eqs = {eq(s1), gamma eq(s1), gamma^2 eq(s1)} (3 x nb), on e's stream
-  void layer_rows(Engine<T>& e, const L* eqs, L* rows) {
-    if constexpr (kMx<T>) {
-      // G[w][p][i] = eqs[w][p] l_beta_i on the matrix cores, then the gather
-      DBuf<L> G(e.pool, 3 * nb * 16 * E);
-      const uint64_t nx = 3 * nb, ntiles = (nx + 31) / 32;
-      e.launch("k_mulfix_mx", nx * 17.0 * Engine<T>::EB, nx * 16.0 * E * E, (k_mulfix_mx<T>),
-               dim3(grid_for(ntiles, 4, 32), 16), dim3(256), (const uint32_t*)e.d_lbfrag.p, eqs, nx, G.p);
-      const double ne = (double)(mr[0].nnz + mr[1].nnz + mr[2].nnz);
-      e.launch("k_rows_gather", ne * (Engine<T>::EB + 8) + 32.0 * nb * Engine<T>::EB, ne * E, (k_rows_gather<T>),
-               dim3(grid_for(32 * nb, 8, 1u << 16)), dim3(256), (const uint64_t*)eptr.p, (const uint32_t*)ecode.p,
-               (const L*)eval.p, (const L*)G.p, (uint64_t)nb, rows, (uint64_t)(32 * nb));
-    } else {
-      double tc = (double)(ncell[0] + ncell[1] + ncell[2]);
-      e.launch("k_spmv_gr", tc * 2.0 * Engine<T>::EB, tc * 16 * Engine<T>::PM, (k_spmv_gr<T>),
-               dim3(grid_for(32 * nb, GPB)), dim3(GB), (const uint64_t*)rptr.p, (const uint32_t*)rdk.p,
-               (const uint32_t*)rsel.p, (const uint32_t*)ridx.p, (const L*)dhat.p, eqs, (uint64_t)nb, rows,
-               (uint64_t)(32 * nb));
-    }
-  }
-
-  // layer_rows with eqs = {eq_s1, gamma eq_s1, gamma^2 eq_s1} formed here
-  // (k <= 32): each w's third of eqs and of the products G on its own
-  // stream, so the scalings overlap the other thirds' products.
+  // eqs = {eq_s1, gamma eq_s1, gamma^2 eq_s1} (3 x nb, formed here): each
+  // w's third of eqs and of the products G[w][p][i] = eqs[w][p] l_beta_i on
+  // its own stream, so the scalings overlap the other thirds' products; then
+  // an entry (row 32p+s of matrix w, value v) adds v G[w][p][s/2] u_{s%2} to
+  // the row of its output (l = (c - r) mod 32, q = c / 32).
   void layer_rows_from(Engine<T>& e, const L* eq1, const L* dg, const L* dg2, L* eqs, L* rows) {
     DBuf<L> G(e.pool, 3 * nb * 16 * E);
-    const uint64_t ntiles = (nb + 31) / 32;
     e.fork(3);
     for (int w = 0; w < 3; w++) {
       e.on(w);
       L* ew = eqs + (size_t)w * nb * E;
       if (w == 0) e.d2d(ew, eq1, nb * E * sizeof(L));
       else e.scale(w == 1 ? dg : dg2, eq1, ew, nb);
-      e.launch("k_mulfix_mx", nb * 17.0 * Engine<T>::EB, nb * 16.0 * E * E, (k_mulfix_mx<T>),
-               dim3(grid_for(ntiles, 4, 32), 16), dim3(256), (const uint32_t*)e.d_lbfrag.p, (const L*)ew, (uint64_t)nb,
-               G.p + (size_t)w * nb * 16 * E);
+      e.mulfix_lb(ew, nb, G.p + (size_t)w * nb * 16 * E);
     }
     e.join();
     const double ne = (double)(mr[0].nnz + mr[1].nnz + mr[2].nnz);
@@ -1080,14 +956,7 @@ template <class T> struct GeneralParams : ParamsBase {
     {
       DBuf<L> eq1 = e.eq_table(mp.chal.data(), log_b);
       DBuf<L> dg = e.put(&gam, 1), dg2 = e.put(&gam2, 1);
-      if constexpr (kMx<T>) {
-        layer_rows_from(e, eq1.p, dg.p, dg2.p, eqs.p, rows.p);
-      } else {
-        e.d2d(eqs.p, eq1.p, nb * E * sizeof(L));
-        e.scale(dg.p, eq1.p, eqs.p + nb * E, nb);
-        e.scale(dg2.p, eq1.p, eqs.p + 2 * nb * E, nb);
-        layer_rows(e, eqs.p, rows.p);
-      }
+      layer_rows_from(e, eq1.p, dg.p, dg2.p, eqs.p, rows.p);
     }
     eqs.release();
     stg.mark("layer_rows");

@@ -682,12 +682,9 @@ template <class T> struct Verifier {
       DBuf<L> eqs(&pool, 3 * nb * E), rows(&pool, 32 * nb * E);
       {
         DBuf<L> eq1 = e.eq_table(s1.data(), gp.log_b);
-        e.d2d(eqs.p, eq1.p, nb * E * sizeof(L));
         DBuf<L> dg = e.put(&gam, 1), dg2 = e.put(&gam2, 1);
-        e.scale(dg.p, eq1.p, eqs.p + nb * E, nb);
-        e.scale(dg2.p, eq1.p, eqs.p + 2 * nb * E, nb);
+        gp.layer_rows_from(e, eq1.p, dg.p, dg2.p, eqs.p, rows.p);
       }
-      gp.layer_rows(e, eqs.p, rows.p);
       DBuf<L> eq_s2 = e.eq_table(s2.data(), gp.log_b);
       at_s2 = e.template dots<32>(rows.p, nb, 32, eq_s2.p, nb);
     }
