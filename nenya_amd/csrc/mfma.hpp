@@ -863,6 +863,127 @@ __global__ void __launch_bounds__(256) k_mulfix_mx64(const uint32_t* frag, const
   }
 }
 
+// Linear-check round sums with 64-bit limbs (as k_gram_lin): the two Gram
+// matrices over (f_lo, g_lo) and (f_hi - f_lo, g_hi - g_lo) with exact digit
+// classes.  Per 32-element chunk and 32 x 32 block, class s of the digit
+// products, sum_{p+q=s} X_p^T Y_q, is exact in i32 (|.| <= 8 * 32 * 2^14),
+// and 256^s (class s) is added to the wave's running 64-bit totals.  512 threads (8 waves, 2 per SIMD, the digit
+// planes of the chunk's four 128 x 32 limb matrices fill 128 KB of LDS);
+// wave w takes the (gram, bx, by) blocks w, w + 8, w + 16, w + 24.  The
+// contraction into 128 limbs is k_gram_lin's, in 64-bit.
+__global__ void __launch_bounds__(512) k_gram_lin64(const uint64_t* f, uint64_t fstride, const uint64_t* gt,
+                                                    uint64_t gstride, uint64_t ntab, uint64_t m, int xrows,
+                                                    uint64_t* partials) {
+  __shared__ __attribute__((aligned(16))) uint32_t lds[32768];
+  __shared__ unsigned long long raw[2][63][8];
+  __shared__ uint64_t q[2][63][4];
+  uint32_t (*sF)[8][4][64][4] = reinterpret_cast<uint32_t (*)[8][4][64][4]>(lds);
+  const int tid = threadIdx.x, j = tid & 127, eg0 = tid >> 7;
+  const int w = tid >> 6, l = tid & 63;
+  const int xb = xrows / 32, npairs = 2 * xb * 4;
+  const uint64_t total = m * ntab, nchunks = (total + 31) / 32;
+  for (int i = tid; i < 2 * 63 * 8; i += 512) (&raw[0][0][0])[i] = 0;
+  uint64_t tot[4][16];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int i = 0; i < 16; i++) tot[u][i] = 0;
+  const int rb = j >> 5;
+  for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    // thread (j, eg0): limb j of the element groups eg = eg0 and eg0 + 4
+    // (elements 4eg .. 4eg+3, k = kappa(eg & 1, 4(eg >> 1) + c))
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+      const int eg = eg0 + 4 * half;
+      const int lane = (j & 31) + 32 * (eg & 1), wd = eg >> 1;
+      uint64_t z[4][4];  // [x0, y0, x1, y1][element]
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const uint64_t e = ch * 32 + 4 * eg + c;
+        uint64_t f0 = 0, f1 = 0, g0 = 0, g1 = 0;
+        if (e < total) {
+          const uint64_t tab = e / m, i = e % m;
+          const uint64_t* F = f + (tab * fstride + i) * E;
+          const uint64_t* G = gt + (tab * gstride + i) * E;
+          f0 = F[j];
+          f1 = F[m * E + j];
+          g0 = G[j];
+          g1 = G[m * E + j];
+        }
+        z[0][c] = bal_digits64(f0);
+        z[1][c] = bal_digits64(g0);
+        z[2][c] = bal_digits64(f1 - f0);
+        z[3][c] = bal_digits64(g1 - g0);
+      }
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int p = 0; p < 8; p++) {
+          uint32_t wv = 0;
+#pragma unroll
+          for (int c = 0; c < 4; c++) wv |= (uint32_t)((z[mt][c] >> (8 * p)) & 0xffu) << (8 * c);
+          sF[mt][p][rb][lane][wd] = wv;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int pr = w + 8 * u;
+      if (pr >= npairs) continue;
+      const int pg = pr / (xb * 4), pbx = (pr % (xb * 4)) >> 2, pby = pr & 3;
+      const int xm = 2 * pg, ym = 2 * pg + 1;
+#pragma unroll
+      for (int sd = 7; sd >= 0; sd--) {
+        v16i tmp = (v16i)0;
+#pragma unroll
+        for (int pd = 0; pd <= sd; pd++) {
+          const v4i a = *reinterpret_cast<const v4i*>(sF[xm][pd][pbx][l]);
+          const v4i b = *reinterpret_cast<const v4i*>(sF[ym][sd - pd][pby][l]);
+          tmp = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, tmp, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) tot[u][i] += (uint64_t)(int64_t)tmp[i] << (8 * sd);
+      }
+    }
+    __syncthreads();
+  }
+  // accumulator entry i of lane l in block (bx, by): row 32bx + 8(i>>2) +
+  // 4(l>>5) + (i&3), column 32by + (l&31)
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int pr = w + 8 * u;
+    if (pr >= npairs) continue;
+    const int pg = pr / (xb * 4), pbx = (pr % (xb * 4)) >> 2, pby = pr & 3;
+    const int mp0 = 8 * (pbx + pby) + (l >> 5) + ((l & 31) >> 2), cp = l & 3;
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+      atomicAdd(&raw[pg][mp0 + 2 * (i >> 2)][(i & 3) + cp], (unsigned long long)tot[u][i]);
+  }
+  __syncthreads();
+  if (tid < 2 * 63) {
+    const int g = tid / 63, p = tid % 63;
+    const unsigned long long* r = raw[g][p];
+    q[g][p][0] = r[0] - r[4];
+    q[g][p][1] = r[1] - r[4] - r[5];
+    q[g][p][2] = r[2] - r[5] - r[6];
+    q[g][p][3] = r[3] - r[6];
+  }
+  __syncthreads();
+  if (tid < 2 * E) {
+    const int g = tid >> 7, u = (tid & 127) >> 2, c = tid & 3;
+    auto hc = [&](int s2, int cc) -> uint64_t { return q[g][32 + s2][cc] - (s2 < 2 ? q[g][61 + s2][cc] : 0ull); };
+    uint64_t v = q[g][u][c];
+    if (u >= 1) v -= hc(u - 1, c);
+    if (u >= 3) v -= hc(u - 3, c);
+    if (u <= 30) {
+      if (c == 0) v += hc(u, 3);
+      else if (c == 1) v += hc(u, 3) - hc(u, 0);
+      else v -= hc(u, c - 1);
+    }
+    partials[((size_t)blockIdx.x * 2 + g) * E + (tid & 127)] = v;
+  }
+}
+
 // m-bars on the matrix cores (pcs.rs:554-561, 984-1001):
 //   out[l][j] = sum_{i < rows} e_i X[l * lstride + i * k2 + j]
 // with the M_{e_i} fragments (frag + i * ND * 4096 words, from k_mulmat).

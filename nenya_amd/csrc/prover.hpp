@@ -699,6 +699,15 @@ template <class T> struct Engine {
            gs, ntab, m, xrows, part);
     return nwg;
   }
+  // The same with 64-bit limbs (k_gram_lin64: one block per CU, its LDS
+  // holds a chunk's digit planes).
+  unsigned gram_lin64(const L* f, uint64_t fs, const L* g, uint64_t gs, uint64_t ntab, uint64_t m, int xrows, L* part) {
+    const uint64_t nch = (m * ntab + 31) / 32;
+    const unsigned nwg = (unsigned)std::min<uint64_t>(nch, GRAM_BLOCKS);
+    launch("k_gram_lin64", m * ntab * 4.0 * EB, m * ntab * 2.0 * E * E, k_gram_lin64, dim3(nwg), dim3(512),
+           (const uint64_t*)f, fs, (const uint64_t*)g, gs, ntab, m, xrows, (uint64_t*)part);
+    return nwg;
+  }
   void marg(const L* s, L* eg, uint64_t m) {
     launch("k_marg", m * 3 * EB, m * 32 * PM, (k_marg<T>), dim3(grid_for(m, GPB)), dim3(GB), s, eg, m);
   }
@@ -1581,7 +1590,7 @@ template <class T> struct Engine {
     DBuf<L> dc = put(&claim, 1), dq(pool, 3 * E), dch(pool, nv * E);
     size_t len = N;
     // allocated before the first waiting kernel (see mult_check)
-    const size_t nparts = kMx<T> ? GRAM_BLOCKS_MAX : grid_for((uint64_t)(N / 2 > 0 ? N / 2 : 1) * ntab, GPB, 2048);
+    const size_t nparts = kFrag<T> ? GRAM_BLOCKS_MAX : grid_for((uint64_t)(N / 2 > 0 ? N / 2 : 1) * ntab, GPB, 2048);
     DBuf<L> part(pool, nparts * 2 * E);
     DBuf<uint32_t> frags(pool, kFrag<T> ? nv * kND<T> * 4096 : 1);
     MbGuard guard(*this);
@@ -1601,6 +1610,10 @@ template <class T> struct Engine {
     auto round_sums = [&](size_t r, size_t m) -> uint32_t {
       if constexpr (kMx<T>) {
         const unsigned nwg = gram_lin(f, fs, g, gs, (uint64_t)ntab, (uint64_t)m, (r == 0 && NAF0 == 16) ? 64 : 128, part.p);
+        return post(r, nwg);
+      } else if constexpr (kMx64<T>) {
+        const unsigned nwg = gram_lin64(f, fs, g, gs, (uint64_t)ntab, (uint64_t)m, (r == 0 && NAF0 == 16) ? 64 : 128,
+                                        part.p);
         return post(r, nwg);
       } else {
         const unsigned nbk = grid_for(m * ntab, GPB, 2048);
