@@ -411,10 +411,12 @@ __device__ __forceinline__ void mul_tile(const v4i* A, const LT<T>* x, uint64_t 
   }
 }
 
-// out[e*nmat + i] = M_i x[e] for the nmat fixed matrices of frag (matrix i
-// at fragment offset i*ND*16); block (bx, i) keeps matrix i in LDS.
+// out[e*nmat + i] = M_i x[e] (or out[i*n + e] when imajor) for the nmat
+// fixed matrices of frag (matrix i at fragment offset i*ND*16); block
+// (bx, i) keeps matrix i in LDS.
 template <class T>
-__global__ void __launch_bounds__(256) k_mulfix_mx(const uint32_t* frag, const LT<T>* x, uint64_t n, LT<T>* out) {
+__global__ void __launch_bounds__(256) k_mulfix_mx(const uint32_t* frag, const LT<T>* x, uint64_t n, LT<T>* out,
+                                                   int imajor) {
   constexpr int ND = kND<T>;
   __shared__ v4i sA[ND * 16 * 64];
   const int i = blockIdx.y, nmat = gridDim.y;
@@ -424,7 +426,8 @@ __global__ void __launch_bounds__(256) k_mulfix_mx(const uint32_t* frag, const L
   const uint64_t ntiles = (n + 31) / 32;
   for (uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntiles; tile += (uint64_t)gridDim.x * 4) {
     asm volatile("" ::: "memory");
-    mul_tile<T>(sA, x, n, out + (size_t)i * E, (uint64_t)nmat, tile);
+    if (imajor) mul_tile<T>(sA, x, n, out + (size_t)i * n * E, 1, tile);
+    else mul_tile<T>(sA, x, n, out + (size_t)i * E, (uint64_t)nmat, tile);
   }
 }
 
@@ -830,10 +833,11 @@ __global__ void __launch_bounds__(256) k_fold_mx64(const uint32_t* frag, FoldJob
   }
 }
 
-// out[e * nmat + i] = M_i x[e] for the nmat fixed matrices of frag (matrix i
-// at fragment offset i * 8 * 4096 words); block (bx, i) keeps M_i in LDS.
+// out[e * nmat + i] = M_i x[e] (or out[i * n + e] when imajor) for the nmat
+// fixed matrices of frag (matrix i at fragment offset i * 8 * 4096 words);
+// block (bx, i) keeps M_i in LDS.
 __global__ void __launch_bounds__(256) k_mulfix_mx64(const uint32_t* frag, const uint64_t* x, uint64_t n,
-                                                     uint64_t* out) {
+                                                     uint64_t* out, int imajor) {
   __shared__ v4i sA[8 * 16 * 64];
   const int i = blockIdx.y, nmat = gridDim.y;
   stage_frags<8>(reinterpret_cast<const v4i*>(frag) + (size_t)i * 8 * 16 * 64, sA);
@@ -851,7 +855,7 @@ __global__ void __launch_bounds__(256) k_mulfix_mx64(const uint32_t* frag, const
       uint64_t R[16];
       mul_rb64(sA, B, rb, R);
       if (ok) {
-        uint64_t* o = out + (e * nmat + i) * E + 32 * rb + 4 * h;
+        uint64_t* o = out + (imajor ? (uint64_t)i * n + e : e * nmat + i) * E + 32 * rb + 4 * h;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           ulonglong2* q = reinterpret_cast<ulonglong2*>(o + 8 * j);

@@ -562,7 +562,7 @@ template <class T> struct Engine {
         launch("k_mulmat", EB + kND<T> * 16384.0, 0, (k_mulmat<T>), dim3(1), dim3(256), s, fr.p);
         const uint64_t ntiles = (n + 31) / 32;
         launch("k_scale_mx", n * 2 * EB, (double)n * E * E, (k_mulfix_mx<T>), dim3(grid_for(ntiles, 4, 1024), 1),
-               dim3(256), (const uint32_t*)fr.p, X, n, out);
+               dim3(256), (const uint32_t*)fr.p, X, n, out, 0);
         return;
       }
     }
@@ -571,7 +571,7 @@ template <class T> struct Engine {
         DBuf<uint32_t> fr(pool, (size_t)8 * 4096);
         mulmat64(s, fr.p);
         launch("k_scale_mx64", n * 2 * EB, (double)n * E * E, k_mulfix_mx64, dim3(grid_for((n + 31) / 32, 4, 256), 1),
-               dim3(256), (const uint32_t*)fr.p, (const uint64_t*)X, (uint64_t)n, (uint64_t*)out);
+               dim3(256), (const uint32_t*)fr.p, (const uint64_t*)X, (uint64_t)n, (uint64_t*)out, 0);
         return;
       }
     }
@@ -584,10 +584,10 @@ template <class T> struct Engine {
     const uint64_t ntiles = (nx + 31) / 32;
     if constexpr (kMx64<T>)
       launch("k_mulfix_mx64", nx * 17.0 * EB, nx * 16.0 * E * E, k_mulfix_mx64, dim3(grid_for(ntiles, 4, 16), 16),
-             dim3(256), (const uint32_t*)d_lbfrag.p, (const uint64_t*)x, nx, (uint64_t*)G);
+             dim3(256), (const uint32_t*)d_lbfrag.p, (const uint64_t*)x, nx, (uint64_t*)G, 0);
     else
       launch("k_mulfix_mx", nx * 17.0 * EB, nx * 16.0 * E * E, (k_mulfix_mx<T>), dim3(grid_for(ntiles, 4, 32), 16),
-             dim3(256), (const uint32_t*)d_lbfrag.p, x, nx, G);
+             dim3(256), (const uint32_t*)d_lbfrag.p, x, nx, G, 0);
   }
   void sub(const L* a, const L* b, L* o, uint64_t nl) {
     launch("k_sub", nl * 3.0 * sizeof(L), 0, (k_sub<T>), dim3(grid_for(nl, 256, 1u << 20)), dim3(256), a, b, o, nl);
@@ -770,6 +770,8 @@ template <class T> struct Engine {
     return out;
   }
 
+  // outer-product steps of at least this many entries use the matrix cores
+  static constexpr size_t EQ_MX_MIN = 8192;
   // build_eq_table (eq.rs:35-58) for host points r[0..n): 2^n entries
   DBuf<L> eq_table(const G* r, size_t n) {
     if (n == 0) {
@@ -796,8 +798,32 @@ template <class T> struct Engine {
     for (int j = k - 2; j >= 0; j--) {
       const size_t na = size_t(1) << sz[j];
       DBuf<L> nx(pool, na * len * E);
+      const L* hj = heads.p + off[j] * E;
+      if (kFrag<T> && na * len >= EQ_MX_MIN) {
+        // each head entry is a fixed multiplier of the whole tail: its
+        // fragments, then one fixed-matrix product per head entry
+        const uint64_t ntiles = (len + 31) / 32;
+        const unsigned per = (unsigned)std::max<size_t>(1, 512 / na);  // ~512 blocks in all
+        DBuf<uint32_t> fr(pool, na * kND<T> * 4096);
+        if constexpr (kMx64<T>) {
+          launch("k_mulmat64", na * (EB + 8 * 16384.0), 0, k_mulmat64, dim3((unsigned)na, 4), dim3(256),
+                 (const uint64_t*)hj, fr.p);
+          launch("k_eq_outer_mx64", na * len * EB, (double)na * len * E * E, k_mulfix_mx64,
+                 dim3(grid_for(ntiles, 4, per), (unsigned)na), dim3(256), (const uint32_t*)fr.p, (const uint64_t*)tail,
+                 (uint64_t)len, (uint64_t*)nx.p, 1);
+        } else if constexpr (kMx<T>) {
+          launch("k_mulmat", na * (EB + kND<T> * 16384.0), 0, (k_mulmat<T>), dim3((unsigned)na), dim3(256), hj, fr.p);
+          launch("k_eq_outer_mx", na * len * EB, (double)na * len * E * E, (k_mulfix_mx<T>),
+                 dim3(grid_for(ntiles, 4, per), (unsigned)na), dim3(256), (const uint32_t*)fr.p, tail, (uint64_t)len,
+                 nx.p, 1);
+        }
+        cur = std::move(nx);
+        tail = cur.p;
+        len *= na;
+        continue;
+      }
       launch("k_eq_outer", na * len * EB, na * len * 32 * PM, (k_eq_outer<T>), dim3(grid_for(na * len, GPB)), dim3(GB),
-             (const L*)(heads.p + off[j] * E), (uint64_t)na, tail, (uint64_t)len, nx.p);
+             hj, (uint64_t)na, tail, (uint64_t)len, nx.p);
       cur = std::move(nx);
       tail = cur.p;
       len *= na;
