@@ -876,8 +876,8 @@ __global__ void __launch_bounds__(256) k_mulfix_mx64(const uint32_t* frag, const
 // wave w takes the (gram, bx, by) blocks w, w + 8, w + 16, w + 24.  The
 // contraction into 128 limbs is k_gram_lin's, in 64-bit.
 __global__ void __launch_bounds__(512) k_gram_lin64(const uint64_t* f, uint64_t fstride, const uint64_t* gt,
-                                                    uint64_t gstride, uint64_t ntab, uint64_t m, int xrows,
-                                                    uint64_t* partials) {
+                                                    uint64_t gstride, uint64_t ntab, uint64_t m, uint64_t hoff,
+                                                    int xrows, uint64_t* partials) {
   __shared__ __attribute__((aligned(16))) uint32_t lds[32768];
   __shared__ unsigned long long raw[2][63][8];
   __shared__ uint64_t q[2][63][4];
@@ -910,9 +910,9 @@ __global__ void __launch_bounds__(512) k_gram_lin64(const uint64_t* f, uint64_t 
           const uint64_t* F = f + (tab * fstride + i) * E;
           const uint64_t* G = gt + (tab * gstride + i) * E;
           f0 = F[j];
-          f1 = F[m * E + j];
+          f1 = F[hoff * E + j];
           g0 = G[j];
-          g1 = G[m * E + j];
+          g1 = G[hoff * E + j];
         }
         z[0][c] = bal_digits64(f0);
         z[1][c] = bal_digits64(g0);
@@ -1149,7 +1149,9 @@ __global__ void k_sum_split(const LT<T>* part, uint64_t nsplit, uint64_t cnt, LT
 // ---------------------------------------------------------------------------
 
 // Linear-check round (as k_lin_round): gram 0 over (f_lo, g_lo), gram 1
-// over (f_hi - f_lo, g_hi - g_lo), both from one read of the tables.  A
+// over (f_hi - f_lo, g_hi - g_lo), both from one read of the tables (hi at
+// hoff elements past lo; hoff = 0 makes gram 1 zero and gram 0 a plain sum
+// of products, the inner products of Engine::dots).  A
 // block (16 waves) walks chunks of 32 elements: its threads load the four
 // 128 x 32 limb matrices (thread (j, eg): limb j of elements 4eg..4eg+3,
 // which are k = kappa(eg&1, 4(eg>>1) + c)) and store their digit planes in
@@ -1164,7 +1166,7 @@ __global__ void k_sum_split(const LT<T>* part, uint64_t nsplit, uint64_t cnt, LT
 // bins to 128 limbs per matrix, written as partials [block][gram].
 template <class T>
 __global__ void __launch_bounds__(1024) k_gram_lin(const LT<T>* f, uint64_t fstride, const LT<T>* gt, uint64_t gstride,
-                                                   uint64_t ntab, uint64_t m, int xrows, LT<T>* partials) {
+                                                   uint64_t ntab, uint64_t m, uint64_t hoff, int xrows, LT<T>* partials) {
   using L = LT<T>;
   constexpr int ND = kND<T>;
   // digit planes [x0|y0|x1|y1][plane][rb][lane][word] in the loop, a Gram matrix after
@@ -1202,9 +1204,9 @@ __global__ void __launch_bounds__(1024) k_gram_lin(const LT<T>* f, uint64_t fstr
         const L* F = f + (tab * fstride + i) * E;
         const L* G = gt + (tab * gstride + i) * E;
         rf[c][0] = (uint32_t)F[j];
-        rf[c][1] = (uint32_t)F[m * E + j];
+        rf[c][1] = (uint32_t)F[hoff * E + j];
         rg[c][0] = (uint32_t)G[j];
-        rg[c][1] = (uint32_t)G[m * E + j];
+        rg[c][1] = (uint32_t)G[hoff * E + j];
       }
     }
   };

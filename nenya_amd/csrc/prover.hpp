@@ -696,7 +696,7 @@ template <class T> struct Engine {
     const int tb = Tuning::get().gram_blocks[Tuning::kidx(T::K)][Tuning::lg(nch)].load(std::memory_order_relaxed);
     const unsigned nwg = (unsigned)std::min<uint64_t>(nch, tb ? (unsigned)tb : GRAM_BLOCKS);
     launch("k_gram_lin", m * ntab * 4.0 * EB, m * ntab * 2.0 * E * E, (k_gram_lin<T>), dim3(nwg), dim3(1024), f, fs, g,
-           gs, ntab, m, xrows, part);
+           gs, ntab, m, m, xrows, part);
     return nwg;
   }
   // The same with 64-bit limbs (k_gram_lin64: one block per CU, its LDS
@@ -705,7 +705,7 @@ template <class T> struct Engine {
     const uint64_t nch = (m * ntab + 31) / 32;
     const unsigned nwg = (unsigned)std::min<uint64_t>(nch, GRAM_BLOCKS);
     launch("k_gram_lin64", m * ntab * 4.0 * EB, m * ntab * 2.0 * E * E, k_gram_lin64, dim3(nwg), dim3(512),
-           (const uint64_t*)f, fs, (const uint64_t*)g, gs, ntab, m, xrows, (uint64_t*)part);
+           (const uint64_t*)f, fs, (const uint64_t*)g, gs, ntab, m, m, xrows, (uint64_t*)part);
     return nwg;
   }
   void marg(const L* s, L* eg, uint64_t m) {
@@ -720,6 +720,23 @@ template <class T> struct Engine {
   // ntab inner products <A_y, B> (A_y = A + y*astride elements), left on
   // the device (ntab contiguous elements).
   template <int NA> DBuf<L> dots_dev(const L* A, uint64_t astride, int ntab, const L* B, uint64_t n) {
+    if (kFrag<T> && ntab == 1 && n >= DOT_MX_MIN) {
+      // one long inner product: the Gram-matrix round kernel with no hi
+      // half (its second sum is then zero), contracted once
+      DBuf<L> part(pool, (size_t)GRAM_BLOCKS_MAX * 2 * E);
+      const int xrows = NA == 16 ? 64 : 128;
+      unsigned nwg;
+      if constexpr (kMx64<T>) {
+        nwg = (unsigned)std::min<uint64_t>((n + 31) / 32, GRAM_BLOCKS);
+        launch("k_dot_gram64", n * 2.0 * EB, n * 2.0 * E * E, k_gram_lin64, dim3(nwg), dim3(512), (const uint64_t*)A,
+               (uint64_t)0, (const uint64_t*)B, (uint64_t)0, (uint64_t)1, n, (uint64_t)0, xrows, (uint64_t*)part.p);
+      } else {
+        nwg = (unsigned)std::min<uint64_t>((n + 31) / 32, GRAM_BLOCKS);
+        launch("k_dot_gram", n * 2.0 * EB, n * 2.0 * E * E, (k_gram_lin<T>), dim3(nwg), dim3(1024), A, (uint64_t)0, B,
+               (uint64_t)0, (uint64_t)1, n, (uint64_t)0, xrows, part.p);
+      }
+      return reduce(part, nwg, 2);  // element 0: the inner product
+    }
     unsigned nb = grid_for(n, GPB, ntab > 1 ? 256 : 2048);
     DBuf<L> part(pool, (size_t)nb * ntab * E);
     launch("k_dot", n * (ntab + 1.0) * EB, n * ntab * NA * PM, (k_dot<T, NA>), dim3(nb, ntab), dim3(GB), A, astride, B, n,
@@ -770,6 +787,8 @@ template <class T> struct Engine {
     return out;
   }
 
+  // inner products of at least this many terms use the matrix cores
+  static constexpr uint64_t DOT_MX_MIN = 8192;
   // outer-product steps of at least this many entries use the matrix cores
   static constexpr size_t EQ_MX_MIN = 8192;
   // build_eq_table (eq.rs:35-58) for host points r[0..n): 2^n entries
