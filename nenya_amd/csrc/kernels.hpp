@@ -457,6 +457,58 @@ __global__ void __launch_bounds__(GB) k_mult_round(const LT<T>* eg, const LT<T>*
   block_emit<T, 3>(acc, partials, t, g);
 }
 
+// The element products of a Gruen round, for the matrix-core form of its
+// sums (sum_i eg_i P_s[i] as Gram contractions):
+//   P[0][i] = a_lo b_lo - c_lo - e_lo, P[1][i] = a_hi b_hi - c_hi - e_hi,
+//   P[2][i] = (a_hi - a_lo)(b_hi - b_lo)    (c, e may be null).
+template <class T>
+__global__ void __launch_bounds__(GB) k_mult_prod(const LT<T>* a, const LT<T>* b, const LT<T>* cc, const LT<T>* ee,
+                                                  uint64_t m, LT<T>* P) {
+  __shared__ __attribute__((aligned(16))) LT<T> sh[GPB][2][E];
+  const int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < 3 * m; base += (uint64_t)gridDim.x * GPB) {
+    const uint64_t x = base + g;
+    const bool ok = x < 3 * m;
+    const int s = ok ? (int)(x / m) : 0;
+    const uint64_t i = ok ? x % m : 0;
+    if (ok) {
+      CT<T> lo[4], hi[4], xv[4], yv[4];
+      if (s != 1) {
+        ld4<T>(a + i * E + 4 * t, lo);
+        ld4<T>(b + i * E + 4 * t, yv);
+      }
+      if (s != 0) {
+        ld4<T>(a + (i + m) * E + 4 * t, hi);
+        ld4<T>(b + (i + m) * E + 4 * t, xv);
+      }
+      CT<T> f[4], h[4];
+      for (int c = 0; c < 4; c++) {
+        f[c] = s == 0 ? lo[c] : s == 1 ? hi[c] : hi[c] - lo[c];
+        h[c] = s == 0 ? yv[c] : s == 1 ? xv[c] : xv[c] - yv[c];
+      }
+      st4<T>(sh[g][0] + 4 * t, f);
+      st4<T>(sh[g][1] + 4 * t, h);
+    }
+    __syncthreads();
+    Acc<T> pa;
+    pa.clear();
+    if (ok) acc_mul<T>(pa, sh[g][0], sh[g][1], t);
+    CT<T> p[4];
+    acc_final<T>(pa, t, p);
+    if (ok) {
+      if (s < 2 && cc) {
+        CT<T> c4[4], e4[4];
+        const uint64_t off = (s == 0 ? i : i + m) * E + 4 * t;
+        ld4<T>(cc + off, c4);
+        ld4<T>(ee + off, e4);
+        for (int c = 0; c < 4; c++) p[c] -= c4[c] + e4[c];
+      }
+      st4<T>(P + ((uint64_t)s * m + i) * E + 4 * t, p);
+    }
+    __syncthreads();
+  }
+}
+
 // Linear-check round over `ntab` table pairs (linear_check_simd.rs:264-286,
 // linear_check_general.rs:289-334): s0 += f_lo g_lo, sinf += df dg.
 // f, g: ntab tables of stride `stride` elements, current length 2m.

@@ -877,14 +877,14 @@ __global__ void __launch_bounds__(256) k_mulfix_mx64(const uint32_t* frag, const
 // contraction into 128 limbs is k_gram_lin's, in 64-bit.
 __global__ void __launch_bounds__(512) k_gram_lin64(const uint64_t* f, uint64_t fstride, const uint64_t* gt,
                                                     uint64_t gstride, uint64_t ntab, uint64_t m, uint64_t hoff,
-                                                    int xrows, uint64_t* partials) {
+                                                    int xrows, uint64_t* partials, int pstride, int poff) {
   __shared__ __attribute__((aligned(16))) uint32_t lds[32768];
   __shared__ unsigned long long raw[2][63][8];
   __shared__ uint64_t q[2][63][4];
   uint32_t (*sF)[8][4][64][4] = reinterpret_cast<uint32_t (*)[8][4][64][4]>(lds);
   const int tid = threadIdx.x, j = tid & 127, eg0 = tid >> 7;
   const int w = tid >> 6, l = tid & 63;
-  const int xb = xrows / 32, npairs = 2 * xb * 4;
+  const int xb = xrows / 32, npairs = (hoff ? 2 : 1) * xb * 4;  // gram 1 only with a hi half
   const uint64_t total = m * ntab, nchunks = (total + 31) / 32;
   for (int i = tid; i < 2 * 63 * 8; i += 512) (&raw[0][0][0])[i] = 0;
   uint64_t tot[4][16];
@@ -984,7 +984,7 @@ __global__ void __launch_bounds__(512) k_gram_lin64(const uint64_t* f, uint64_t 
       else if (c == 1) v += hc(u, 3) - hc(u, 0);
       else v -= hc(u, c - 1);
     }
-    partials[((size_t)blockIdx.x * 2 + g) * E + (tid & 127)] = v;
+    if (g == 0 || hoff) partials[((size_t)blockIdx.x * pstride + poff + g) * E + (tid & 127)] = v;
   }
 }
 
@@ -1151,7 +1151,8 @@ __global__ void k_sum_split(const LT<T>* part, uint64_t nsplit, uint64_t cnt, LT
 // Linear-check round (as k_lin_round): gram 0 over (f_lo, g_lo), gram 1
 // over (f_hi - f_lo, g_hi - g_lo), both from one read of the tables (hi at
 // hoff elements past lo; hoff = 0 makes gram 1 zero and gram 0 a plain sum
-// of products, the inner products of Engine::dots).  A
+// of products, the inner products of Engine::dots).  Partials of block b:
+// gram g at partials[b * pstride + poff + g] (gram 1 only when hoff != 0).  A
 // block (16 waves) walks chunks of 32 elements: its threads load the four
 // 128 x 32 limb matrices (thread (j, eg): limb j of elements 4eg..4eg+3,
 // which are k = kappa(eg&1, 4(eg>>1) + c)) and store their digit planes in
@@ -1166,7 +1167,8 @@ __global__ void k_sum_split(const LT<T>* part, uint64_t nsplit, uint64_t cnt, LT
 // bins to 128 limbs per matrix, written as partials [block][gram].
 template <class T>
 __global__ void __launch_bounds__(1024) k_gram_lin(const LT<T>* f, uint64_t fstride, const LT<T>* gt, uint64_t gstride,
-                                                   uint64_t ntab, uint64_t m, uint64_t hoff, int xrows, LT<T>* partials) {
+                                                   uint64_t ntab, uint64_t m, uint64_t hoff, int xrows, LT<T>* partials,
+                                                   int pstride, int poff) {
   using L = LT<T>;
   constexpr int ND = kND<T>;
   // digit planes [x0|y0|x1|y1][plane][rb][lane][word] in the loop, a Gram matrix after
@@ -1176,7 +1178,7 @@ __global__ void __launch_bounds__(1024) k_gram_lin(const LT<T>* f, uint64_t fstr
   const int tid = threadIdx.x, j = tid & 127, eg = tid >> 7;
   const int w = tid >> 6, l = tid & 63;
   // (gram, bx, by) pairs: xb = xrows/32 row-blocks of x per gram
-  const int xb = xrows / 32, npairs = 2 * xb * 4;
+  const int xb = xrows / 32, npairs = (hoff ? 2 : 1) * xb * 4;  // gram 1 only with a hi half
   int pg[2], pbx[2], pby[2];
   bool pon[2];
 #pragma unroll
@@ -1282,7 +1284,7 @@ __global__ void __launch_bounds__(1024) k_gram_lin(const LT<T>* f, uint64_t fstr
       else if (c == 1) v += hc(u, 3) - hc(u, 0);
       else v -= hc(u, c - 1);
     }
-    partials[((size_t)blockIdx.x * 2 + g) * E + (tid & 127)] = (L)v;
+    if (g == 0 || hoff) partials[((size_t)blockIdx.x * pstride + poff + g) * E + (tid & 127)] = (L)v;
   }
 }
 

@@ -696,7 +696,7 @@ template <class T> struct Engine {
     const int tb = Tuning::get().gram_blocks[Tuning::kidx(T::K)][Tuning::lg(nch)].load(std::memory_order_relaxed);
     const unsigned nwg = (unsigned)std::min<uint64_t>(nch, tb ? (unsigned)tb : GRAM_BLOCKS);
     launch("k_gram_lin", m * ntab * 4.0 * EB, m * ntab * 2.0 * E * E, (k_gram_lin<T>), dim3(nwg), dim3(1024), f, fs, g,
-           gs, ntab, m, m, xrows, part);
+           gs, ntab, m, m, xrows, part, 2, 0);
     return nwg;
   }
   // The same with 64-bit limbs (k_gram_lin64: one block per CU, its LDS
@@ -705,7 +705,7 @@ template <class T> struct Engine {
     const uint64_t nch = (m * ntab + 31) / 32;
     const unsigned nwg = (unsigned)std::min<uint64_t>(nch, GRAM_BLOCKS);
     launch("k_gram_lin64", m * ntab * 4.0 * EB, m * ntab * 2.0 * E * E, k_gram_lin64, dim3(nwg), dim3(512),
-           (const uint64_t*)f, fs, (const uint64_t*)g, gs, ntab, m, m, xrows, (uint64_t*)part);
+           (const uint64_t*)f, fs, (const uint64_t*)g, gs, ntab, m, m, xrows, (uint64_t*)part, 2, 0);
     return nwg;
   }
   void marg(const L* s, L* eg, uint64_t m) {
@@ -722,20 +722,21 @@ template <class T> struct Engine {
   template <int NA> DBuf<L> dots_dev(const L* A, uint64_t astride, int ntab, const L* B, uint64_t n) {
     if (kFrag<T> && ntab == 1 && n >= DOT_MX_MIN) {
       // one long inner product: the Gram-matrix round kernel with no hi
-      // half (its second sum is then zero), contracted once
-      DBuf<L> part(pool, (size_t)GRAM_BLOCKS_MAX * 2 * E);
+      // half (only its first sum is formed), contracted once
+      DBuf<L> part(pool, (size_t)GRAM_BLOCKS * E);
       const int xrows = NA == 16 ? 64 : 128;
       unsigned nwg;
       if constexpr (kMx64<T>) {
         nwg = (unsigned)std::min<uint64_t>((n + 31) / 32, GRAM_BLOCKS);
         launch("k_dot_gram64", n * 2.0 * EB, n * 2.0 * E * E, k_gram_lin64, dim3(nwg), dim3(512), (const uint64_t*)A,
-               (uint64_t)0, (const uint64_t*)B, (uint64_t)0, (uint64_t)1, n, (uint64_t)0, xrows, (uint64_t*)part.p);
+               (uint64_t)0, (const uint64_t*)B, (uint64_t)0, (uint64_t)1, n, (uint64_t)0, xrows, (uint64_t*)part.p, 1,
+               0);
       } else {
         nwg = (unsigned)std::min<uint64_t>((n + 31) / 32, GRAM_BLOCKS);
         launch("k_dot_gram", n * 2.0 * EB, n * 2.0 * E * E, (k_gram_lin<T>), dim3(nwg), dim3(1024), A, (uint64_t)0, B,
-               (uint64_t)0, (uint64_t)1, n, (uint64_t)0, xrows, part.p);
+               (uint64_t)0, (uint64_t)1, n, (uint64_t)0, xrows, part.p, 1, 0);
       }
-      return reduce(part, nwg, 2);  // element 0: the inner product
+      return reduce(part, nwg, 1);
     }
     unsigned nb = grid_for(n, GPB, ntab > 1 ? 256 : 2048);
     DBuf<L> part(pool, (size_t)nb * ntab * E);
@@ -787,6 +788,9 @@ template <class T> struct Engine {
     return out;
   }
 
+  // Gruen rounds over at least this many element pairs form their sums on
+  // the matrix cores
+  static constexpr uint64_t MULT_MX_MIN = 8192;
   // inner products of at least this many terms use the matrix cores
   static constexpr uint64_t DOT_MX_MIN = 8192;
   // outer-product steps of at least this many entries use the matrix cores
@@ -1528,7 +1532,9 @@ template <class T> struct Engine {
     };
     unsigned nbk_max = 1;
     for (size_t m = N / 2; m >= 1; m /= 2) nbk_max = std::max(nbk_max, blocks(m));
-    DBuf<L> part(pool, (size_t)nbk_max * 3 * E);
+    DBuf<L> part(pool, (size_t)std::max<unsigned>(nbk_max, GRAM_BLOCKS) * 3 * E);
+    // the rounds' element products (matrix-core form of the sums)
+    DBuf<L> prods(pool, kFrag<T> && N / 2 >= MULT_MX_MIN ? 3 * (N / 2) * E : 1);
     DBuf<uint32_t> frags(pool, kFrag<T> ? nv * kND<T> * 4096 : 1);
     MbGuard guard(*this);
     const bool pipe = pipeline_rounds();
@@ -1544,6 +1550,28 @@ template <class T> struct Engine {
       const unsigned nbk = blocks(m);
       const L* c3 = ntab > 2 ? cur[2] : (const L*)nullptr;
       const L* e3 = ntab > 2 ? cur[3] : (const L*)nullptr;
+      if (kFrag<T> && m >= MULT_MX_MIN) {
+        // the element products P_s on the vector ALUs, then the three sums
+        // sum_i eg_i P_s[i] as Gram contractions on the matrix cores
+        launch("k_mult_prod", m * (2.0 * ntab + 3) * EB, m * 3 * 32 * PM, (k_mult_prod<T>),
+               dim3(grid_for(3 * m, GPB, 4096)), dim3(GB), cur[0], cur[1], c3, e3, (uint64_t)m, prods.p);
+        const unsigned nwg = (unsigned)std::min<uint64_t>((m + 31) / 32, GRAM_BLOCKS);
+        for (int s = 0; s < 3; s++) {
+          if constexpr (kMx64<T>)
+            launch("k_mult_gram64", m * 2.0 * EB, m * 1.0 * E * E, k_gram_lin64, dim3(nwg), dim3(512),
+                   (const uint64_t*)eg.p, (uint64_t)0, (const uint64_t*)(prods.p + (size_t)s * m * E), (uint64_t)0,
+                   (uint64_t)1, (uint64_t)m, (uint64_t)0, 128, (uint64_t*)part.p, 3, s);
+          else
+            launch("k_mult_gram", m * 2.0 * EB, m * 1.0 * E * E, (k_gram_lin<T>), dim3(nwg), dim3(1024), (const L*)eg.p,
+                   (uint64_t)0, (const L*)(prods.p + (size_t)s * m * E), (uint64_t)0, (uint64_t)1, (uint64_t)m,
+                   (uint64_t)0, 128, part.p, 3, s);
+        }
+        if (dfs) {
+          post_round_dev(part.p, nwg, 3, 0, nullptr, nullptr, drounds.p + r * 3 * E);
+          return 0u;
+        }
+        return post_round(part.p, nwg, 3, 0, nullptr, nullptr, (int)r);
+      }
       if (split)
         launch("k_mult_round", m * (2.0 * ntab + 1) * EB, m * 6 * 32 * PM, (k_mult_round<T, true>), dim3(nbk),
                dim3(GB), (const L*)eg.p, cur[0], cur[1], c3, e3, (uint64_t)m, part.p);

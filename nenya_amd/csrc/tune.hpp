@@ -94,7 +94,7 @@ template <class T> void tune_width(std::ostringstream& js, int lmin, int lmax) {
       auto run = [&](unsigned nwg) {
         return time_launch(cx.st, [&] {
           hipLaunchKernelGGL(k_gram_lin<T>, dim3(nwg), dim3(1024), 0, cx.st, (const L*)src.p, (uint64_t)0,
-                             (const L*)src.p, (uint64_t)0, (uint64_t)1, m, m, 128, part.p);
+                             (const L*)src.p, (uint64_t)0, (uint64_t)1, m, m, 128, part.p, 2, 0);
         });
       };
       const unsigned defn = (unsigned)std::min<uint64_t>(nch, Engine<T>::GRAM_BLOCKS);
