@@ -611,6 +611,106 @@ __global__ void __launch_bounds__(256) k_cgemm(const uint32_t* frag, const LT<T>
   }
 }
 
+// Split-K form of k_cgemm: block (bx, z) keeps the fragments of k-blocks
+// [8z, 8z + 8) (all four row-blocks, ND planes, ND * 32 KiB) in LDS for its
+// lifetime, so no fragment load waits on L2 inside the loop; each wave takes
+// 32-row tiles with all four row-blocks (a k-block's digit planes feed four
+// row-blocks), Horner over the digit classes per k-block (i32 wrap =
+// mod 2^k), and writes the slice's partial sums to part + z * nb * E
+// (k_sum_split adds the four slices).  One block per CU (LDS), one wave per
+// SIMD: the C loads run four k-blocks ahead to cover HBM latency.
+template <class T>
+__global__ void __launch_bounds__(256) k_cgemm_sk(const uint32_t* frag, const LT<T>* cmat, LT<T>* part, uint64_t nb) {
+  using C = CT<T>;
+  constexpr int ND = kND<T>, KS = 8;
+  __shared__ v4i sA[ND * KS * 4 * 64];  // [pd][kbl][rb][lane]
+  const v4i* A = reinterpret_cast<const v4i*>(frag);
+  const int z = blockIdx.y;
+  {
+    v4i t[ND * KS * 4 * 64 / 256];
+#pragma unroll
+    for (int u = 0; u < ND * KS * 4 * 64 / 256; u++) {
+      const int i = threadIdx.x + 256 * u;
+      const int l = i & 63, rb = (i >> 6) & 3, kbl = (i >> 8) % KS, pd = (i >> 8) / KS;
+      t[u] = A[((pd * 32 + z * KS + kbl) * 4 + rb) * 64 + l];
+    }
+#pragma unroll
+    for (int u = 0; u < ND * KS * 4 * 64 / 256; u++) sA[threadIdx.x + 256 * u] = t[u];
+  }
+  __syncthreads();
+  const int l = threadIdx.x & 63, col = l & 31, h = l >> 5;
+  const uint64_t ntiles = (nb + 31) / 32, stride = (uint64_t)gridDim.x * 4;
+  // k-blocks of C four steps ahead, across tile boundaries: at step
+  // kbl = 4 half + q the wave computes ring slot q and refills it with
+  // k-block kbl + 4 (of the next tile in the second half)
+  uint32_t ring[4][16];
+  auto load = [&](uint64_t tile, int kbl, uint32_t (&v)[16]) {
+    const uint64_t p = tile * 32 + col;
+    if (tile < ntiles && p < nb) {
+      ld16k<T>(cmat + p * 32 + (uint64_t)(z * KS + kbl) * nb * 32, 0, h, v);
+    } else {
+#pragma unroll
+      for (int b = 0; b < 16; b++) v[b] = 0;
+    }
+  };
+  uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+#pragma unroll
+  for (int q = 0; q < 4; q++) load(tile, q, ring[q]);
+  for (; tile < ntiles; tile += stride) {
+    const uint64_t p = tile * 32 + col;
+    const bool ok = p < nb;
+    v16i acc[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) acc[rb] = (v16i)0;
+#pragma unroll 1
+    for (int half = 0; half < 2; half++) {
+#pragma unroll
+     for (int q = 0; q < 4; q++) {
+      const int kbl = 4 * half + q;
+      v4i B[ND];
+      planes16<ND>(ring[q], B);
+      if (half == 0) load(tile, q + 4, ring[q]);
+      else load(tile + stride, q, ring[q]);
+#pragma unroll
+      for (int rb = 0; rb < 4; rb += 2) {
+        // keeps each row-block pair's fragment reads in place (hoisted, the
+        // unrolled loop's reads would pin hundreds of registers); the pair's
+        // two MFMA chains interleave, so neither waits on its own results
+        asm volatile("" ::: "memory");
+        v16i t0 = (v16i)0, t1 = (v16i)0;
+#pragma unroll
+        for (int sd = ND - 1; sd >= 0; sd--) {
+          if (sd != ND - 1) {
+            t0 <<= 8;
+            t1 <<= 8;
+          }
+#pragma unroll
+          for (int pd = 0; pd <= sd; pd++) {
+            t0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(sA[((pd * KS + kbl) * 4 + rb) * 64 + l], B[sd - pd], t0, 0, 0, 0);
+            t1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(sA[((pd * KS + kbl) * 4 + rb + 1) * 64 + l], B[sd - pd], t1, 0, 0,
+                                                       0);
+          }
+        }
+        acc[rb] += t0;
+        acc[rb + 1] += t1;
+      }
+     }
+    }
+    if (ok) {
+      LT<T>* o = part + ((uint64_t)z * nb + p) * E;
+#pragma unroll
+      for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          C x[4];
+#pragma unroll
+          for (int c = 0; c < 4; c++) x[c] = (C)(uint32_t)acc[rb][4 * j + c];
+          st4<T>(o + 32 * rb + 8 * j + 4 * h, x);
+        }
+    }
+  }
+}
+
 // The same product on the vector ALUs (64-bit limbs):
 //   out[p][limb] = sum_{j < 1024} B2[j][limb] C_p[j],  j = 32 s + s',
 // B2 the basis products as plain limbs ([j][limb]).  A block takes 32 block

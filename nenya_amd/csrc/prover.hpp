@@ -589,6 +589,17 @@ template <class T> struct Engine {
       launch("k_mulfix_mx", nx * 17.0 * EB, nx * 16.0 * E * E, (k_mulfix_mx<T>), dim3(grid_for(ntiles, 4, 32), 16),
              dim3(256), (const uint32_t*)d_lbfrag.p, x, nx, G, 0);
   }
+  // out[p] = sum_{s,s'} C_p[s][s'] b_s b_s' for nb block rows (k <= 32):
+  // the basis-product GEMM split four ways over its 1024-long sum (a slice
+  // of the fragments in each block's LDS), then the slices added
+  void cgemm(const L* cm, L* out, uint64_t nb) {
+    const uint64_t ntiles = (nb + 31) / 32;
+    DBuf<L> part(pool, 4 * nb * E);
+    launch("k_cgemm", (double)nb * (1024 * sizeof(L) + 4 * EB), (double)nb * 1024 * E, (k_cgemm_sk<T>),
+           dim3(grid_for(ntiles, 4, kND<T> <= 2 ? 256 : 64), 4), dim3(256), (const uint32_t*)d_b2frag.p, cm, part.p, nb);
+    launch("k_sum_split", 5.0 * nb * EB, 0, (k_sum_split<T>), dim3(grid_for(nb * E, 256, 4096)), dim3(256),
+           (const L*)part.p, (uint64_t)4, nb, out);
+  }
   void sub(const L* a, const L* b, L* o, uint64_t nl) {
     launch("k_sub", nl * 3.0 * sizeof(L), 0, (k_sub<T>), dim3(grid_for(nl, 256, 1u << 20)), dim3(256), a, b, o, nl);
   }

@@ -521,9 +521,7 @@ template <class T> struct MimdParams : ParamsBase {
         e.launch("k_cmat_mimd", (double)n * 1024 * sizeof(L), (double)ncell[w] * 1024, (k_cmat_mimd<T>),
                  dim3(grid_for(n, 8, 4096)), dim3(256), (const uint64_t*)(cptr.p + w * (n + 1)), (const uint32_t*)ccol.p,
                  (const L*)cval.p, slots, cmw, (uint64_t)n);
-        e.launch("k_cgemm", (double)n * (1024 * sizeof(L) + Engine<T>::EB), (double)n * 1024 * E, (k_cgemm<T>),
-                 dim3(grid_for(n, 32, 1024)), dim3(256), (const uint32_t*)e.d_b2frag.p, (const L*)cmw,
-                 out + w * n * E, (uint64_t)n);
+        e.cgemm(cmw, out + w * n * E, n);
       }
       e.join();
     } else {
@@ -866,9 +864,7 @@ template <class T> struct GeneralParams : ParamsBase {
         e.launch("k_cmat", (double)nb * 1024 * sizeof(L), 0, (k_cmat<T>), dim3(grid_for(nb, 8, 4096)), dim3(256),
                  (const uint32_t*)mr[m].ptr.p, (const uint32_t*)mr[m].idx.p, (const L*)mr[m].val.p, w, cmm,
                  (uint64_t)nb);
-        e.launch("k_cgemm", (double)nb * (1024 * sizeof(L) + Engine<T>::EB), (double)nb * 1024 * E, (k_cgemm<T>),
-                 dim3(grid_for(nb, 32, 1024)), dim3(256), (const uint32_t*)e.d_b2frag.p, (const L*)cmm,
-                 sums.p + m * nb * E, (uint64_t)nb);
+        e.cgemm(cmm, sums.p + m * nb * E, nb);
       }
       e.join();
     } else {
