@@ -923,36 +923,13 @@ constexpr int DPP_HALF_MIRROR = 0x141; // lane i <-> 7-i within 8 lanes
 // sum their circ (all) and hi (wrapped) terms with a DPP transpose-reduction;
 // the reduction Y^32 = -Y^3 - Y - alpha is applied by 128 threads reading the
 // sums from LDS.
-// Lifts src (device or host-pinned memory) into el with a 256-thread block;
-// the lifted value is left in sx (LDS, C-typed, 16-byte aligned) as well.
-template <class T>
-__device__ __forceinline__ void teich_lift_block(const LT<T>* src, LT<T>* el, const LT<T>* frob_t, int squarings,
-                                                 CT<T>* sx) {
-  using C = CT<T>;
-  __shared__ C part[E];
-  __shared__ __attribute__((aligned(16))) C sx2[E];
-  __shared__ C sred[2][32][4];  // [0] reduced circ, [1] reduced hi (before the cascade)
+//
+// square_times: `count` squarings of the element in sx (sx2 = 2 sx beside
+// it), 256 threads, C the arithmetic width (the ring's, or a narrower one
+// where only the low bits are needed).  Ends with sx, sx2 updated.
+template <class C>
+__device__ __forceinline__ void square_times(C* sx, C* sx2, C (*sred)[32][4], int count) {
   const int tid = threadIdx.x;
-  if (tid < E) sx[tid] = (C)src[tid];
-  __syncthreads();
-  {
-    const int o = tid & (E - 1), h = tid >> 7;
-    // all 64 matrix entries in flight at once (one memory latency)
-    C f[64];
-#pragma unroll
-    for (int j = 0; j < 64; j++) f[j] = (C)frob_t[(size_t)(64 * h + j) * E + o];
-    C s = 0;
-#pragma unroll
-    for (int j = 0; j < 64; j++) s += f[j] * sx[64 * h + j];
-    if (h) part[o] = s;
-    __syncthreads();
-    if (!h) {
-      const C y = s + part[o];
-      sx[o] = y;
-      sx2[o] = y + y;
-    }
-    __syncthreads();
-  }
   const int lane = tid & 63, dl = lane & 7;
   const int t = 8 * (tid >> 6) + (lane >> 3);
   const bool b2 = dl & 4, b1 = dl & 2, b0 = dl & 1;
@@ -960,7 +937,7 @@ __device__ __forceinline__ void teich_lift_block(const LT<T>* src, LT<T>* el, co
   const bool wrap = dl >= a0;
   const int q0 = 3 * (wrap ? dl - a0 : dl), nq = wrap ? n1 : n0;
   const C hm = wrap ? (C)~(C)0 : (C)0;
-  for (int it = 0; it < squarings; it++) {
+  for (int it = 0; it < count; it++) {
     C ac[7];
 #pragma unroll
     for (int c = 0; c < 7; c++) ac[c] = 0;
@@ -1012,6 +989,64 @@ __device__ __forceinline__ void teich_lift_block(const LT<T>* src, LT<T>* el, co
       sx2[tid] = y + y;
     }
     __syncthreads();
+  }
+}
+
+// Lifts src (device or host-pinned memory) into el with a 256-thread block;
+// the lifted value is left in sx (LDS, C-typed, 16-byte aligned) as well.
+template <class T>
+__device__ __forceinline__ void teich_lift_block(const LT<T>* src, LT<T>* el, const LT<T>* frob_t, int squarings,
+                                                 CT<T>* sx) {
+  using C = CT<T>;
+  __shared__ C part[E];
+  __shared__ __attribute__((aligned(16))) C sx2[E];
+  __shared__ C sred[2][32][4];  // [0] reduced circ, [1] reduced hi (before the cascade)
+  const int tid = threadIdx.x;
+  if (tid < E) sx[tid] = (C)src[tid];
+  __syncthreads();
+  {
+    const int o = tid & (E - 1), h = tid >> 7;
+    // all 64 matrix entries in flight at once (one memory latency)
+    C f[64];
+#pragma unroll
+    for (int j = 0; j < 64; j++) f[j] = (C)frob_t[(size_t)(64 * h + j) * E + o];
+    C s = 0;
+#pragma unroll
+    for (int j = 0; j < 64; j++) s += f[j] * sx[64 * h + j];
+    if (h) part[o] = s;
+    __syncthreads();
+    if (!h) {
+      const C y = s + part[o];
+      sx[o] = y;
+      sx2[o] = y + y;
+    }
+    __syncthreads();
+  }
+  if constexpr (sizeof(C) == 8) {
+    // 64-bit limbs: z_i = y^(2^i) is needed mod 2^64 only at the end, and
+    // u = v mod 2^s gives u^2 = v^2 mod 2^(s+1); so the first squarings run
+    // mod 2^32 (z_31 exact mod 2^32) and the last 32, from z_31's low words,
+    // leave z_63 exact mod 2^64
+    __shared__ __attribute__((aligned(16))) uint32_t lx[E], lx2[E];
+    __shared__ uint32_t lred[2][32][4];
+    const int lo = squarings > 32 ? squarings - 32 : 0;
+    if (lo) {
+      if (tid < E) {
+        lx[tid] = (uint32_t)sx[tid];
+        lx2[tid] = (uint32_t)sx2[tid];
+      }
+      __syncthreads();
+      square_times<uint32_t>(lx, lx2, lred, lo);
+      if (tid < E) {
+        // sx2 = 2 sx in 64 bits (the 32-bit double dropped its carry)
+        sx[tid] = (C)lx[tid];
+        sx2[tid] = (C)lx[tid] + (C)lx[tid];
+      }
+      __syncthreads();
+    }
+    square_times<C>(sx, sx2, sred, squarings - lo);
+  } else {
+    square_times<C>(sx, sx2, sred, squarings);
   }
   if (tid < E) el[tid] = (LT<T>)sx[tid];
 }
