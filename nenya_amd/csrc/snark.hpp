@@ -561,14 +561,12 @@ template <class T> struct MimdParams : ParamsBase {
     stg.on = cx.timing;
     stg.st = cx.st;
     stg.start();
-    // witness slots [m][32] (the slot-form commitment message)
-    DBuf<L> slots(&pool, m * 32);
-    {
-      std::vector<L> tr(m * 32);
-      for (size_t j = 0; j < m; j++)
-        for (int s = 0; s < 32; s++) tr[j * 32 + s] = wit[s * m + j];
-      e.h2d(slots.p, tr.data(), tr.size() * sizeof(L));
-    }
+    // witnesses [32][m] and their slot transpose [m][32] (the slot-form
+    // commitment message), made on the device
+    DBuf<L> dw(&pool, 32 * m), slots(&pool, m * 32);
+    e.h2d(dw.p, wit, 32 * m * sizeof(L));
+    e.launch("k_slots", 64.0 * m * sizeof(L), 0, (k_slots<T>), dim3(grid_for(m, 64, 1u << 16)), dim3(256),
+             (const L*)dw.p, (uint64_t)m, slots.p);
     stg.mark("upload");
     // 1. w_phi = pack_r2(witness column j) (reduce.rs:294-312)
     DBuf<L> wphi(&pool, m * E);
