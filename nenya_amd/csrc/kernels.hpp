@@ -246,13 +246,14 @@ __global__ void __launch_bounds__(GB) k_fold(const LT<T>* r, const LT<T>* src, L
 }
 
 // marginalize_and_scale (mult_check.rs:302-313): eg[i] = scale*(eg[i]+eg[i+m])
+// (block blk of nblk; the kernel runs it as block blockIdx.x of gridDim.x)
 template <class T>
-__global__ void __launch_bounds__(GB) k_marg(const LT<T>* scale, LT<T>* eg, uint64_t m) {
+__device__ __forceinline__ void marg_body(const LT<T>* scale, LT<T>* eg, uint64_t m, unsigned blk, unsigned nblk) {
   __shared__ __attribute__((aligned(16))) LT<T> ss[E];
   __shared__ __attribute__((aligned(16))) LT<T> sh[GPB][E];
   int g = threadIdx.x >> 5, t = threadIdx.x & 31;
   if (threadIdx.x < 32) stage<T>(ss, scale, t);
-  for (uint64_t base = (uint64_t)blockIdx.x * GPB; base < m; base += (uint64_t)gridDim.x * GPB) {
+  for (uint64_t base = (uint64_t)blk * GPB; base < m; base += (uint64_t)nblk * GPB) {
     uint64_t i = base + g;
     bool ok = i < m;
     if (ok) {
@@ -271,6 +272,10 @@ __global__ void __launch_bounds__(GB) k_marg(const LT<T>* scale, LT<T>* eg, uint
     if (ok) st4<T>(eg + i * E + 4 * t, o);
     __syncthreads();
   }
+}
+template <class T>
+__global__ void __launch_bounds__(GB) k_marg(const LT<T>* scale, LT<T>* eg, uint64_t m) {
+  marg_body<T>(scale, eg, m, blockIdx.x, gridDim.x);
 }
 
 // build_eq_table (eq.rs:35-58) by chunks: the table over variables
@@ -373,8 +378,45 @@ __global__ void __launch_bounds__(GB) k_eq_outer(const LT<T>* A, uint64_t na, co
 
 // Write the block's reduced sum of nsum accumulators.  Groups' raw sums are
 // added through LDS (linear), then group 0 reduces.
+// Integer atomic add (wrap-around) for 32- and 64-bit words.
+__device__ __forceinline__ void atom_add(uint32_t* p, uint32_t v) { atomicAdd(p, v); }
+__device__ __forceinline__ void atom_add(uint64_t* p, uint64_t v) {
+  atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
+}
+
+// The block's NS sums: written as partials of block blk, or (sums != null)
+// added with integer atomics into sums[s] (exact: the sums are taken mod
+// 2^32 or 2^64, and 2^k divides both); only_s >= 0 emits that sum alone
+// (the others are zero).
+// One sum of a block, as sum s of NSO: partials of block blk (the other
+// NSO - 1 sums written as zero), or added atomically into sums[s].
+template <class T>
+__device__ __forceinline__ void emit_one(const Acc<T>& acc, int s, int nso, LT<T>* partials, int t, int g,
+                                         unsigned blk, CT<T>* sums) {
+  __shared__ CT<T> red[GPB][14][32];
+  for (int c = 0; c < 7; c++) {
+    red[g][c][t] = acc.c[c];
+    red[g][7 + c][t] = acc.h[c];
+  }
+  __syncthreads();
+  if (g == 0) {
+    Acc<T> a = acc;
+    for (int gg = 1; gg < GPB; gg++)
+      for (int c = 0; c < 7; c++) { a.c[c] += red[gg][c][t]; a.h[c] += red[gg][7 + c][t]; }
+    CT<T> o[4];
+    acc_final<T>(a, t, o);
+    if (sums) {
+      for (int c = 0; c < 4; c++) atom_add(sums + s * E + 4 * t + c, o[c]);
+    } else {
+      const CT<T> z[4] = {0, 0, 0, 0};
+      for (int so = 0; so < nso; so++) st4<T>(partials + ((uint64_t)blk * nso + so) * E + 4 * t, so == s ? o : z);
+    }
+  }
+}
+
 template <class T, int NS>
-__device__ __forceinline__ void block_emit(Acc<T> (&acc)[NS], LT<T>* partials, int t, int g) {
+__device__ __forceinline__ void block_emit(Acc<T> (&acc)[NS], LT<T>* partials, int t, int g, unsigned blk,
+                                           CT<T>* sums = nullptr, int only_s = -1) {
   __shared__ CT<T> red[GPB][NS][14][32];
   for (int s = 0; s < NS; s++)
     for (int c = 0; c < 7; c++) {
@@ -384,12 +426,17 @@ __device__ __forceinline__ void block_emit(Acc<T> (&acc)[NS], LT<T>* partials, i
   __syncthreads();
   if (g == 0) {
     for (int s = 0; s < NS; s++) {
+      if (only_s >= 0 && s != only_s) continue;
       Acc<T> a = acc[s];
       for (int gg = 1; gg < GPB; gg++)
         for (int c = 0; c < 7; c++) { a.c[c] += red[gg][s][c][t]; a.h[c] += red[gg][s][7 + c][t]; }
       CT<T> o[4];
       acc_final<T>(a, t, o);
-      st4<T>(partials + ((uint64_t)blockIdx.x * NS + s) * E + 4 * t, o);
+      if (sums) {
+        for (int c = 0; c < 4; c++) atom_add(sums + s * E + 4 * t + c, o[c]);
+      } else {
+        st4<T>(partials + ((uint64_t)blk * NS + s) * E + 4 * t, o);
+      }
     }
   }
 }
@@ -400,15 +447,21 @@ __device__ __forceinline__ void block_emit(Acc<T> (&acc)[NS], LT<T>* partials, i
 // SPLIT (small m, latency-bound): block b computes only sum s = b % 3 over
 // element groups b / 3, so an element's three product chains run on three
 // blocks at once; the other two partial sums of the block are zero.
+// (block blk of nblk, a multiple of 3 with SPLIT; sums != null: the block's
+// sums are added atomically into sums[3] instead of written as partials)
 template <class T, bool SPLIT>
-__global__ void __launch_bounds__(GB) k_mult_round(const LT<T>* eg, const LT<T>* a, const LT<T>* b, const LT<T>* cc,
-                                                   const LT<T>* ee, uint64_t m, LT<T>* partials) {
+__device__ __forceinline__ void mult_round_body(const LT<T>* eg, const LT<T>* a, const LT<T>* b, const LT<T>* cc,
+                                                const LT<T>* ee, uint64_t m, LT<T>* partials, CT<T>* sums,
+                                                unsigned blk, unsigned nblk) {
   __shared__ __attribute__((aligned(16))) LT<T> sh[GPB][3][E];
   int g = threadIdx.x >> 5, t = threadIdx.x & 31;
-  const int sblk = SPLIT ? (int)(blockIdx.x % 3) : 0;
-  const uint64_t bid = SPLIT ? blockIdx.x / 3 : blockIdx.x, nbid = SPLIT ? gridDim.x / 3 : gridDim.x;
-  Acc<T> acc[3];
-  for (int s = 0; s < 3; s++) acc[s].clear();
+  const int sblk = SPLIT ? (int)(blk % 3) : 0;
+  const uint64_t bid = SPLIT ? blk / 3 : blk, nbid = SPLIT ? nblk / 3 : nblk;
+  // SPLIT: one accumulator, for sum sblk (a runtime index into three would
+  // put them in scratch memory)
+  constexpr int NACC = SPLIT ? 1 : 3;
+  Acc<T> acc[NACC];
+  for (int s = 0; s < NACC; s++) acc[s].clear();
   for (uint64_t base = bid * GPB; base < m; base += nbid * GPB) {
     uint64_t i = base + g;
     bool ok = i < m;
@@ -450,11 +503,17 @@ __global__ void __launch_bounds__(GB) k_mult_round(const LT<T>* eg, const LT<T>*
       __syncthreads();
       st4<T>(sh[g][0] + 4 * t, p);
       __syncthreads();
-      if (ok) acc_mul<T>(acc[s], sh[g][2], sh[g][0], t);
+      if (ok) acc_mul<T>(acc[SPLIT ? 0 : s], sh[g][2], sh[g][0], t);
       __syncthreads();
     }
   }
-  block_emit<T, 3>(acc, partials, t, g);
+  if constexpr (SPLIT) emit_one<T>(acc[0], sblk, 3, partials, t, g, blk, sums);
+  else block_emit<T, 3>(acc, partials, t, g, blk, sums);
+}
+template <class T, bool SPLIT>
+__global__ void __launch_bounds__(GB) k_mult_round(const LT<T>* eg, const LT<T>* a, const LT<T>* b, const LT<T>* cc,
+                                                   const LT<T>* ee, uint64_t m, LT<T>* partials) {
+  mult_round_body<T, SPLIT>(eg, a, b, cc, ee, m, partials, nullptr, blockIdx.x, gridDim.x);
 }
 
 // The element products of a Gruen round, for the matrix-core form of its
@@ -546,7 +605,7 @@ __global__ void __launch_bounds__(GB) k_lin_round(const LT<T>* f, uint64_t fstri
     if (ok) acc_mul<T, NAF>(acc[1], sh[g][0], sh[g][1], t);
     __syncthreads();
   }
-  block_emit<T, 2>(acc, partials, t, g);
+  block_emit<T, 2>(acc, partials, t, g, blockIdx.x);
 }
 
 // partial sums of sum_i A_y[i]*B[i] for tables y = blockIdx.y (A_y = A + y*astride

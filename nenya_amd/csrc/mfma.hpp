@@ -975,9 +975,11 @@ __global__ void __launch_bounds__(256) k_mulfix_mx64(const uint32_t* frag, const
 // planes of the chunk's four 128 x 32 limb matrices fill 128 KB of LDS);
 // wave w takes the (gram, bx, by) blocks w, w + 8, w + 16, w + 24.  The
 // contraction into 128 limbs is k_gram_lin's, in 64-bit.
-__global__ void __launch_bounds__(512) k_gram_lin64(const uint64_t* f, uint64_t fstride, const uint64_t* gt,
-                                                    uint64_t gstride, uint64_t ntab, uint64_t m, uint64_t hoff,
-                                                    int xrows, uint64_t* partials, int pstride, int poff) {
+// (block blk of nblk; sums != null as in gram_lin_body)
+__device__ __forceinline__ void gram_lin64_body(const uint64_t* f, uint64_t fstride, const uint64_t* gt,
+                                                uint64_t gstride, uint64_t ntab, uint64_t m, uint64_t hoff, int xrows,
+                                                uint64_t* partials, int pstride, int poff, uint64_t* sums, unsigned blk,
+                                                unsigned nblk) {
   __shared__ __attribute__((aligned(16))) uint32_t lds[32768];
   __shared__ unsigned long long raw[2][63][8];
   __shared__ uint64_t q[2][63][4];
@@ -993,7 +995,7 @@ __global__ void __launch_bounds__(512) k_gram_lin64(const uint64_t* f, uint64_t 
 #pragma unroll
     for (int i = 0; i < 16; i++) tot[u][i] = 0;
   const int rb = j >> 5;
-  for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+  for (uint64_t ch = blk; ch < nchunks; ch += nblk) {
     // thread (j, eg0): limb j of the element groups eg = eg0 and eg0 + 4
     // (elements 4eg .. 4eg+3, k = kappa(eg & 1, 4(eg >> 1) + c))
 #pragma unroll
@@ -1084,8 +1086,17 @@ __global__ void __launch_bounds__(512) k_gram_lin64(const uint64_t* f, uint64_t 
       else if (c == 1) v += hc(u, 3) - hc(u, 0);
       else v -= hc(u, c - 1);
     }
-    if (g == 0 || hoff) partials[((size_t)blockIdx.x * pstride + poff + g) * E + (tid & 127)] = v;
+    if (g == 0 || hoff) {
+      if (sums) atom_add(sums + (size_t)(poff + g) * E + (tid & 127), v);
+      else partials[((size_t)blk * pstride + poff + g) * E + (tid & 127)] = v;
+    }
   }
+}
+__global__ void __launch_bounds__(512) k_gram_lin64(const uint64_t* f, uint64_t fstride, const uint64_t* gt,
+                                                    uint64_t gstride, uint64_t ntab, uint64_t m, uint64_t hoff,
+                                                    int xrows, uint64_t* partials, int pstride, int poff) {
+  gram_lin64_body(f, fstride, gt, gstride, ntab, m, hoff, xrows, partials, pstride, poff, nullptr, blockIdx.x,
+                  gridDim.x);
 }
 
 // m-bars on the matrix cores (pcs.rs:554-561, 984-1001):
@@ -1265,10 +1276,12 @@ __global__ void k_sum_split(const LT<T>* part, uint64_t nsplit, uint64_t cnt, LT
 // the 63 x 7 bins raw[i+i'][c+c'] with LDS atomics (both Gram matrices at
 // once); then alpha^4 = -alpha - 1 and Y^32 = -Y^3 - Y - alpha reduce the
 // bins to 128 limbs per matrix, written as partials [block][gram].
+// (block blk of nblk; sums != null: gram g is added atomically into
+// sums[poff + g] instead of written as a partial)
 template <class T>
-__global__ void __launch_bounds__(1024) k_gram_lin(const LT<T>* f, uint64_t fstride, const LT<T>* gt, uint64_t gstride,
-                                                   uint64_t ntab, uint64_t m, uint64_t hoff, int xrows, LT<T>* partials,
-                                                   int pstride, int poff) {
+__device__ __forceinline__ void gram_lin_body(const LT<T>* f, uint64_t fstride, const LT<T>* gt, uint64_t gstride,
+                                              uint64_t ntab, uint64_t m, uint64_t hoff, int xrows, LT<T>* partials,
+                                              int pstride, int poff, CT<T>* sums, unsigned blk, unsigned nblk) {
   using L = LT<T>;
   constexpr int ND = kND<T>;
   // digit planes [x0|y0|x1|y1][plane][rb][lane][word] in the loop, a Gram matrix after
@@ -1312,8 +1325,8 @@ __global__ void __launch_bounds__(1024) k_gram_lin(const LT<T>* f, uint64_t fstr
       }
     }
   };
-  load(blockIdx.x);
-  for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+  load(blk);
+  for (uint64_t ch = blk; ch < nchunks; ch += nblk) {
     uint32_t z[4][4];  // [x0, y0, x1, y1][element]
 #pragma unroll
     for (int c = 0; c < 4; c++) {
@@ -1332,7 +1345,7 @@ __global__ void __launch_bounds__(1024) k_gram_lin(const LT<T>* f, uint64_t fstr
         sF[mt][p][rb][lane][wd] = wv;
       }
     __syncthreads();
-    load(ch + gridDim.x);
+    load(ch + nblk);
 #pragma unroll
     for (int u = 0; u < 2; u++) {
       if (!pon[u]) continue;
@@ -1384,8 +1397,18 @@ __global__ void __launch_bounds__(1024) k_gram_lin(const LT<T>* f, uint64_t fstr
       else if (c == 1) v += hc(u, 3) - hc(u, 0);
       else v -= hc(u, c - 1);
     }
-    if (g == 0 || hoff) partials[((size_t)blockIdx.x * pstride + poff + g) * E + (tid & 127)] = (L)v;
+    if (g == 0 || hoff) {
+      if (sums) atom_add(sums + (size_t)(poff + g) * E + (tid & 127), v);
+      else partials[((size_t)blk * pstride + poff + g) * E + (tid & 127)] = (L)v;
+    }
   }
+}
+template <class T>
+__global__ void __launch_bounds__(1024) k_gram_lin(const LT<T>* f, uint64_t fstride, const LT<T>* gt, uint64_t gstride,
+                                                   uint64_t ntab, uint64_t m, uint64_t hoff, int xrows, LT<T>* partials,
+                                                   int pstride, int poff) {
+  gram_lin_body<T>(f, fstride, gt, gstride, ntab, m, hoff, xrows, partials, pstride, poff, nullptr, blockIdx.x,
+                   gridDim.x);
 }
 
 }  // namespace nenya

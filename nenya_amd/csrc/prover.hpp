@@ -238,6 +238,13 @@ struct OpenOut {
   std::vector<uint8_t> bytes;
 };
 
+// The round hub of a server (hub.hpp): served contexts hand a sumcheck's
+// small tail rounds to it.
+struct Hub;
+template <class T> struct HubTask;
+template <class T> void hub_run(Hub* h, HubTask<T>& t, hipStream_t st);
+constexpr uint64_t HUB_FOLD_MAX = 4096;  // rounds whose folds touch fewer elements go to the hub
+
 template <class T> struct Engine {
   using L = typename T::L;
   using C = typename T::C;
@@ -295,6 +302,18 @@ template <class T> struct Engine {
   // transcript's BLAKE3 work in the lift kernel (servers created with
   // NENYA_DEVICE_FS=1).
   bool devfs = false;
+  // Served contexts of a server with a round hub: the sumchecks' rounds use
+  // the device transcript, and once a round's folds touch fewer than
+  // HUB_FOLD_MAX elements the remaining rounds run on the hub, fused with
+  // other proofs' rounds (hev: the hand-over's ready / done events).
+  Hub* hub = nullptr;
+  hipEvent_t hev[2] = {};
+  void hub_events() {
+    if (!hev[0]) {
+      HIP_CHECK(hipEventCreateWithFlags(&hev[0], hipEventDisableTiming));
+      HIP_CHECK(hipEventCreateWithFlags(&hev[1], hipEventDisableTiming));
+    }
+  }
   // No-alloc calls: one stream only (no forks, downloads in order on the
   // prover stream), so no stream or event is created during the call.
   bool serial = false;
@@ -1020,6 +1039,8 @@ template <class T> struct Engine {
     held.clear();
   }
   ~Engine() {
+    for (auto& e : hev)
+      if (e) (void)hipEventDestroy(e);
     if (sbg) {
       (void)hipStreamSynchronize(sbg);
       (void)hipStreamDestroy(sbg);
@@ -1550,9 +1571,12 @@ template <class T> struct Engine {
     MbGuard guard(*this);
     const bool pipe = pipeline_rounds();
     // contexts that do not pipeline: Fiat-Shamir of the rounds on the device
-    const bool dfs = !pipe && devfs;
+    const bool use_hub = hub && !pipe;
+    const bool dfs = !pipe && (devfs || use_hub);
     DBuf<L> drounds(pool, dfs ? 3 * nv * E : 1);
     DBuf<b3::State> dtr(pool, 1);
+    DBuf<C> hsums(pool, use_hub ? 3 * E : 1);
+    size_t hub_from = nv;  // first round run on the hub
     if (dfs) h2d(dtr.p, static_cast<const b3::State*>(&tr.h), sizeof(b3::State));
     wait_ticks = spin_ticks();
     // enqueue the sums of round r over tables of 2m entries; returns the flag value
@@ -1634,10 +1658,40 @@ template <class T> struct Engine {
       uint32_t vnext = 0;
       if (r + 1 < nv) {
         marg(dsc.p, eg.p, m / 2);
+        if (use_hub && (m / 2) * ntab < HUB_FOLD_MAX) {
+          hub_from = r + 1;
+          break;
+        }
         vnext = round_sums(r + 1, m / 2);
       }
       if (pipe) host_round();
       vdown = vnext;
+    }
+    if (hub_from < nv) {
+      checkpoint();
+      HubTask<T> t;
+      t.type = 0;
+      t.r = (int)hub_from;
+      t.nv = (int)nv;
+      t.m = len / 2;
+      t.ntab = (uint64_t)ntab;
+      t.work = work.p;
+      t.stride = N / 2;
+      t.eg = eg.p;
+      t.drt = drt.p;
+      t.dsc = dsc.p;
+      t.dch = dch.p;
+      t.drounds = drounds.p;
+      t.dtr = dtr.p;
+      t.sums = hsums.p;
+      t.frob = d_frob.p;
+      t.cancel = cancel;
+      hub_events();
+      t.ready = hev[0];
+      t.done = hev[1];
+      HIP_CHECK(hipMemsetAsync(hsums.p, 0, 3 * E * sizeof(C), st));
+      HIP_CHECK(hipEventRecord(t.ready, st));
+      hub_run<T>(hub, t, st);
     }
     // challenges and finals (and, with the device transcript, the round
     // polynomials and the transcript state) with one synchronization
@@ -1679,9 +1733,12 @@ template <class T> struct Engine {
     DBuf<uint32_t> frags(pool, kFrag<T> ? nv * kND<T> * 4096 : 1);
     MbGuard guard(*this);
     const bool pipe = pipeline_rounds();
-    const bool dfs = !pipe && devfs;  // as in mult_check
+    const bool use_hub = hub && !pipe;  // as in mult_check
+    const bool dfs = !pipe && (devfs || use_hub);
     DBuf<L> drounds(pool, dfs ? 3 * nv * E : 1);
     DBuf<b3::State> dtr(pool, 1);
+    DBuf<C> hsums(pool, use_hub ? 3 * E : 1);
+    size_t hub_from = nv;
     if (dfs) h2d(dtr.p, static_cast<const b3::State*>(&tr.h), sizeof(b3::State));
     wait_ticks = spin_ticks();
     auto post = [&](size_t r, unsigned nparts) -> uint32_t {
@@ -1738,9 +1795,39 @@ template <class T> struct Engine {
       g = gw.p;
       fs = gs = H;
       len = m;
+      if (r + 1 < nv && use_hub && 2 * (m / 2) * (uint64_t)ntab < HUB_FOLD_MAX) {
+        hub_from = r + 1;
+        break;
+      }
       const uint32_t vnext = r + 1 < nv ? round_sums(r + 1, m / 2) : 0;
       if (pipe) host_round();
       vdown = vnext;
+    }
+    if (hub_from < nv) {
+      checkpoint();
+      HubTask<T> t;
+      t.type = 1;
+      t.r = (int)hub_from;
+      t.nv = (int)nv;
+      t.m = len / 2;
+      t.ntab = (uint64_t)ntab;
+      t.fw = fw.p;
+      t.gw = gw.p;
+      t.H = H;
+      t.dc = dc.p;
+      t.dq = dq.p;
+      t.dch = dch.p;
+      t.drounds = drounds.p;
+      t.dtr = dtr.p;
+      t.sums = hsums.p;
+      t.frob = d_frob.p;
+      t.cancel = cancel;
+      hub_events();
+      t.ready = hev[0];
+      t.done = hev[1];
+      HIP_CHECK(hipMemsetAsync(hsums.p, 0, 3 * E * sizeof(C), st));
+      HIP_CHECK(hipEventRecord(t.ready, st));
+      hub_run<T>(hub, t, st);
     }
     if (dfs) fetch_transcript_async(dtr.p);
     std::vector<G> tail = pick_many({{dch.p, 1, nv}, {f, fs, (uint64_t)ntab}, {g, gs, (uint64_t)ntab},
@@ -1757,3 +1844,5 @@ template <class T> struct Engine {
 };
 
 }  // namespace nenya
+
+#include "hub.hpp"  // the round hub's kernels and thread (uses the declarations above)

@@ -52,7 +52,8 @@ struct Worker {
   hipStream_t st = nullptr;
   std::unique_ptr<CtxBase> ctx[4];  // per width: 8, 16, 32, 64
   bool devfs = false;               // round challenges drawn on the device
-  explicit Worker(bool dfs) : devfs(dfs) { HIP_CHECK(hipStreamCreate(&st)); }
+  Hub* hub = nullptr;               // the server's round hub, if any
+  Worker(bool dfs, Hub* h) : devfs(dfs), hub(h) { HIP_CHECK(hipStreamCreate(&st)); }
   ~Worker() {
     for (auto& c : ctx) c.reset();
     if (st) (void)hipStreamDestroy(st);
@@ -64,6 +65,10 @@ struct Worker {
       // several workers drive the device: no kernel waits on a host thread
       static_cast<Ctx<T>&>(*ctx[i]).eng->pipelined = false;
       static_cast<Ctx<T>&>(*ctx[i]).eng->devfs = devfs;
+      static_cast<Ctx<T>&>(*ctx[i]).eng->hub = hub;
+      // NENYA_SERVER_TIMING=1: stage times of every served proof to stderr
+      const char* tm = std::getenv("NENYA_SERVER_TIMING");
+      static_cast<Ctx<T>&>(*ctx[i]).timing = tm && tm[0] == '1';
     }
     return static_cast<Ctx<T>&>(*ctx[i]);
   }
@@ -75,6 +80,8 @@ struct Server {
   std::deque<Job*> queue;
   std::unordered_map<uint64_t, std::unique_ptr<Job>> jobs;
   std::vector<Pinned> spare;  // proof buffers returned by released jobs
+  // declared before the workers: destroyed after their threads have ended
+  std::unique_ptr<Hub> hub;
   std::vector<std::unique_ptr<Worker>> workers;
   std::vector<std::thread> threads;
   size_t budget = 0, in_use = 0;
@@ -104,7 +111,9 @@ struct Server {
       }
     }
     const char* dfs = std::getenv("NENYA_DEVICE_FS");
-    for (int i = 0; i < nworkers; i++) workers.emplace_back(new Worker(dfs && dfs[0] == '1'));
+    const char* hb = std::getenv("NENYA_HUB");
+    if (hb && hb[0] == '1') hub.reset(new Hub());
+    for (int i = 0; i < nworkers; i++) workers.emplace_back(new Worker(dfs && dfs[0] == '1', hub.get()));
     live_servers()++;  // before any job can be submitted
     for (int i = 0; i < nworkers; i++) threads.emplace_back([this, i] { loop(i); });
   }
@@ -205,6 +214,12 @@ struct Server {
     } else {
       auto* sp = static_cast<SimdParams<T>*>(j.params);
       v = sp->prove(cx, j.m3[0], j.m3[1], j.m3[2], (const typename T::L*)j.w);
+    }
+    if (cx.timing) {
+      std::string line = "job " + std::to_string(j.id) + " kind " + std::to_string(j.kind) + " k " +
+                         std::to_string(T::K) + " start " + std::to_string(now_ms() - j.t_start) + ":";
+      for (auto& st : cx.last_timing) line += " " + st.first + " " + std::to_string(st.second);
+      std::fprintf(stderr, "%s\n", line.c_str());
     }
     j.len = v.second;
     size_t cur = j.params->proof_len.load();
