@@ -51,7 +51,8 @@ template <class T> struct HubSumsM {
 template <class T> __global__ void __launch_bounds__(GB) k_hub_mult_sums(HubMulti<HubSumsM<T>> M) {
   const int t = hub_task(M, blockIdx.x);
   const HubSumsM<T>& d = M.d[t];
-  mult_round_body<T, true>(d.eg, d.a, d.b, d.c, d.e, d.m, nullptr, d.sums, blockIdx.x - M.start[t],
+  const unsigned lb = blockIdx.x - M.start[t];
+  mult_round_body<T, true>(d.eg, d.a, d.b, d.c, d.e, d.m, nullptr, d.sums + (size_t)(lb % SUM_COPIES) * 3 * E, lb,
                            M.start[t + 1] - M.start[t]);
 }
 
@@ -66,14 +67,16 @@ template <class T> struct HubSumsL {
 template <class T> __global__ void __launch_bounds__(1024) k_hub_lin_sums(HubMulti<HubSumsL<T>> M) {
   const int t = hub_task(M, blockIdx.x);
   const HubSumsL<T>& d = M.d[t];
-  gram_lin_body<T>(d.f, d.fs, d.g, d.gs, d.ntab, d.m, d.m, d.xrows, nullptr, 2, 0, d.sums, blockIdx.x - M.start[t],
-                   M.start[t + 1] - M.start[t]);
+  const unsigned lb = blockIdx.x - M.start[t];
+  gram_lin_body<T>(d.f, d.fs, d.g, d.gs, d.ntab, d.m, d.m, d.xrows, nullptr, 2, 0,
+                   d.sums + (size_t)(lb % SUM_COPIES) * 3 * E, lb, M.start[t + 1] - M.start[t]);
 }
 __global__ void __launch_bounds__(512) k_hub_lin_sums64(HubMulti<HubSumsL<W<64>>> M) {
   const int t = hub_task(M, blockIdx.x);
   const HubSumsL<W<64>>& d = M.d[t];
-  gram_lin64_body(d.f, d.fs, d.g, d.gs, d.ntab, d.m, d.m, d.xrows, nullptr, 2, 0, d.sums, blockIdx.x - M.start[t],
-                  M.start[t + 1] - M.start[t]);
+  const unsigned lb = blockIdx.x - M.start[t];
+  gram_lin64_body(d.f, d.fs, d.g, d.gs, d.ntab, d.m, d.m, d.xrows, nullptr, 2, 0,
+                  d.sums + (size_t)(lb % SUM_COPIES) * 3 * E, lb, M.start[t + 1] - M.start[t]);
 }
 
 // One block per task: the round polynomial from the sums (k_sum_post's
@@ -97,19 +100,21 @@ template <class T> __global__ void __launch_bounds__(256) k_hub_post_lift(HubMul
   __shared__ __attribute__((aligned(16))) L nat[E];
   __shared__ __attribute__((aligned(16))) L q[3 * E];
   __shared__ fsd::RoundLds<T> fl;
+  __shared__ C ssum[3][E];
   const int t = hub_task(M, blockIdx.x);
   const HubPost<T>& d = M.d[t];
+  gather_sums<T>(d.sums, 3, ssum);
+  __syncthreads();
   for (int i = threadIdx.x; i < 3 * E; i += blockDim.x) {
     const int k = i / E, li = i % E;
     C v;
-    if (d.mode == 0) v = k == 1 ? (C)(d.sums[E + li] - d.sums[li] - d.sums[2 * E + li]) : d.sums[k * E + li];
-    else v = k == 0 ? d.sums[li] : k == 1 ? (C)((C)d.c[li] - d.sums[li] - d.sums[li] - d.sums[E + li]) : d.sums[E + li];
+    if (d.mode == 0) v = k == 1 ? (C)(ssum[1][li] - ssum[0][li] - ssum[2][li]) : ssum[k][li];
+    else v = k == 0 ? ssum[0][li] : k == 1 ? (C)((C)d.c[li] - ssum[0][li] - ssum[0][li] - ssum[1][li]) : ssum[1][li];
     q[i] = (L)v;
     d.qh[i] = (L)v;
     if (d.qd) d.qd[i] = (L)v;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < 3 * E; i += blockDim.x) d.sums[i] = 0;
   if (threadIdx.x < 64) fsd::round_challenge<T>(d.st, q, fl, nat);
   __syncthreads();
   teich_lift_block<T>(nat, d.x, d.frob, T::K - 1, sx);
@@ -181,8 +186,8 @@ template <class T> __global__ void __launch_bounds__(GB) k_hub_marg(HubMulti<Hub
 // on the device.  Mult check (type 0): tables work + t*stride (t < ntab),
 // eq table eg, point drt, eq factor dsc.  Linear check (type 1): tables fw,
 // gw (ntab each, stride H), claim dc, round polynomial dq.  Both: challenges
-// dch, round polynomials drounds, device transcript dtr, sums (3 elements of
-// the compute type, zero on entry).
+// dch, round polynomials drounds, device transcript dtr, sums (SUM_COPIES x
+// 3 elements of the compute type, zero on entry).
 template <class T> struct HubTask {
   using L = typename T::L;
   using C = typename T::C;

@@ -378,6 +378,11 @@ __global__ void __launch_bounds__(GB) k_eq_outer(const LT<T>* A, uint64_t na, co
 
 // Write the block's reduced sum of nsum accumulators.  Groups' raw sums are
 // added through LDS (linear), then group 0 reduces.
+// Sums that many blocks add atomically are spread over SUM_COPIES copies
+// (block b adds into copy b % SUM_COPIES): same-address atomics serialize at
+// the L2, so one copy would make every block wait for all the others.
+constexpr int SUM_COPIES = 16;
+
 // Integer atomic add (wrap-around) for 32- and 64-bit words.
 __device__ __forceinline__ void atom_add(uint32_t* p, uint32_t v) { atomicAdd(p, v); }
 __device__ __forceinline__ void atom_add(uint64_t* p, uint64_t v) {
@@ -722,6 +727,31 @@ __device__ __forceinline__ void mb_fetch(const uint32_t* src, int words, uint32_
 //                                   (mult_check.rs:250-296)
 //   mode 1 (linear check, ns = 2): q = (s0, c - 2 s0 - s1, s1)
 //                                   (linear_check_general.rs:289-334)
+// The round polynomial q from the round's sums ssum (LDS) by the whole
+// block (modes as below), written to qh (and qd); then the flag rises
+// (none: the round's Fiat-Shamir runs on the device).
+template <class T>
+__device__ __forceinline__ void post_q(const CT<T> (*ssum)[E], int mode, const LT<T>* c, LT<T>* qd, LT<T>* qh,
+                                       uint32_t* flag, uint32_t val) {
+  using C = CT<T>;
+  for (int i = threadIdx.x; i < 3 * E; i += blockDim.x) {
+    const int k = i / E, li = i % E;
+    C v;
+    if (mode == 0) v = k == 1 ? (C)(ssum[1][li] - ssum[0][li] - ssum[2][li]) : ssum[k][li];
+    else v = k == 0 ? ssum[0][li] : k == 1 ? (C)((C)c[li] - ssum[0][li] - ssum[0][li] - ssum[1][li]) : ssum[1][li];
+    if (qd) qd[i] = (LT<T>)v;
+    qh[i] = (LT<T>)v;
+  }
+  // each writer's stores have completed before one system-scope release
+  // orders them ahead of the flag
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0 && flag) {
+    __threadfence_system();
+    __hip_atomic_store(flag, val, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(1024) k_sum_post(const LT<T>* partials, uint64_t nblocks, int ns, int mode,
                                                   const LT<T>* c, LT<T>* sums, LT<T>* qd, LT<T>* qh, uint32_t* count,
@@ -781,26 +811,298 @@ __global__ void __launch_bounds__(1024) k_sum_post(const LT<T>* partials, uint64
     for (int i = threadIdx.x; i < ns * E; i += blockDim.x) ssum[i / E][i % E] = (C)sums[i];
     __syncthreads();
   }
-  const int i = threadIdx.x;
-  if (i < 3 * E) {
-    const int k = i / E, li = i % E;
-    C v;
-    if (mode == 0) v = k == 1 ? (C)(ssum[1][li] - ssum[0][li] - ssum[2][li]) : ssum[k][li];
-    else v = k == 0 ? ssum[0][li] : k == 1 ? (C)((C)c[li] - ssum[0][li] - ssum[0][li] - ssum[1][li]) : ssum[1][li];
-    if (qd) qd[i] = (LT<T>)v;
-    qh[i] = (LT<T>)v;
+  if (!single && threadIdx.x == 0) *count = 0;
+  post_q<T>(ssum, mode, c, qd, qh, flag, val);
+}
+
+// ============================================================================
+// Small sumcheck rounds in one kernel: the fold of round r, the eq table's
+// marginalization and round r+1's sums (then posted by the last block), so a
+// latency-bound round costs this kernel and the challenge's lift.  A block
+// takes pairs i of round r+1 (half mh = m/2) in turn; its 32-lane groups
+// each run one group product of a pair at a time, so a pair costs a few
+// product latencies, not the sum of its products: the folds of positions
+// i and i+mh of every table (from positions i, i+m, i+mh, i+mh+m; each
+// position is read and written by this block alone, so in place is safe)
+// run side by side, then the products of the round's sums.  Block sums go
+// to `sums` with integer atomics (mod 2^32 or 2^64); the block that
+// finishes last (count) forms q_{r+1}, clears sums and count, and posts.
+// ============================================================================
+
+// out = c (hi - lo) + lo for one group (lane t: coefficients 4t..4t+3); sc
+// the challenge, shd the group's scratch element (LDS); the caller syncs
+// the block between calls that share shd.
+template <class T>
+__device__ __forceinline__ void fold_one(const LT<T>* sc, LT<T>* shd, const LT<T>* src, uint64_t lo_at, uint64_t hi_at,
+                                         int t, CT<T> out[4]) {
+  using C = CT<T>;
+  C lo[4], hi[4];
+  ld4<T>(src + lo_at * E + 4 * t, lo);
+  ld4<T>(src + hi_at * E + 4 * t, hi);
+  C d[4] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], hi[3] - lo[3]};
+  st4<T>(shd + 4 * t, d);
+  __builtin_amdgcn_wave_barrier();  // the group's 32 lanes sit in one wave
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  Acc<T> pa;
+  pa.clear();
+  acc_mul<T>(pa, sc, shd, t);
+  acc_final<T>(pa, t, out);
+  for (int c = 0; c < 4; c++) out[c] += lo[c];
+}
+
+// ns sums from their SUM_COPIES atomic copies (each 3 elements apart) into
+// ssum (LDS), the copies cleared for the next launch; the whole block calls
+// it, and syncs before ssum is read.
+template <class T> __device__ __forceinline__ void gather_sums(CT<T>* sums, int ns, CT<T> (*ssum)[E]) {
+  using C = CT<T>;
+  for (int k = threadIdx.x; k < ns * E; k += blockDim.x) {
+    // all copies' loads in flight at once (one L2 latency), then the sum
+    C v[SUM_COPIES];
+#pragma unroll
+    for (int cp = 0; cp < SUM_COPIES; cp++)
+      v[cp] = __hip_atomic_load(sums + (size_t)cp * 3 * E + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    C acc = 0;
+#pragma unroll
+    for (int cp = 0; cp < SUM_COPIES; cp++) acc += v[cp];
+    ssum[k / E][k % E] = acc;
   }
-  // each writer's stores have completed before one system-scope release
-  // orders them ahead of the flag
-  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();  // every copy read before any is cleared
+  for (int k = threadIdx.x; k < SUM_COPIES * 3 * E; k += blockDim.x) sums[k] = 0;
+}
+
+// Last block of a tail launch: true for the block that sees every other
+// block's atomic sums in.
+__device__ __forceinline__ bool tail_last(uint32_t* count, bool* last) {
+  __threadfence();
   __syncthreads();
-  if (threadIdx.x == 0) {
-    if (!single) *count = 0;
-    if (flag) {  // no flag: the round's Fiat-Shamir runs on the device
-      __threadfence_system();
-      __hip_atomic_store(flag, val, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  if (threadIdx.x == 0) *last = atomicAdd(count, 1u) == gridDim.x - 1;
+  __syncthreads();
+  return *last;
+}
+
+template <class T> struct TailM {
+  const LT<T>* src[4];  // the round's tables (2m entries each)
+  LT<T>* dst[4];        // the folded tables (m entries; dst may be src)
+  uint64_t m;
+  const LT<T>* rc;  // the round's challenge
+  const LT<T>* rt;  // rt[r]: the marginalization scale is eq_one(rt[r], c)
+  LT<T>* eg;        // eq table (m entries in, m/2 out)
+  CT<T>* sums;      // SUM_COPIES x 3 elements, zero on entry
+  uint32_t* count;  // zero on entry
+  LT<T>* qh;        // q_{r+1}
+  uint32_t* flag;   // mailbox flag (null: device transcript)
+  uint32_t val;
+};
+// Gruen rounds (as k_fold + k_marg + k_mult_round + k_sum_post), NT = 2
+// tables (a, b) or 4 (a, b, c, e); 2 NT + 1 groups: group 2tb+h folds half
+// h of table tb (group 2NT forms the scale first), then group 2NT the new
+// eq entry and groups 0..2 the products P_s, then groups 0..2 add eg' P_s.
+template <class T, int NT> __global__ void __launch_bounds__(32 * (2 * NT + 1)) k_tail_mult(TailM<T> A) {
+  using C = CT<T>;
+  using L = LT<T>;
+  constexpr int NG = 2 * NT + 1;
+  __shared__ __attribute__((aligned(16))) L sc[E], srt[E], sscale[E], seg[E];
+  __shared__ __attribute__((aligned(16))) L sf[2 * NT][E];  // folded: [2tb] entry i, [2tb+1] entry i+mh
+  __shared__ __attribute__((aligned(16))) L sp[3][E];       // P_s
+  __shared__ __attribute__((aligned(16))) L sx[NG][E];      // per-group scratch
+  __shared__ C ssum[3][E];
+  __shared__ bool last;
+  const int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  const uint64_t m = A.m, mh = m / 2;
+  if (g == 0) {
+    stage<T>(sc, A.rc, t);
+    stage<T>(srt, A.rt, t);
+  }
+  __syncthreads();
+  if (g == 2 * NT) {  // eq_one(rt[r], c) = 2 rt c - rt - c + 1
+    Acc<T> pa;
+    pa.clear();
+    acc_mul<T>(pa, srt, sc, t);
+    C o[4], a[4], b[4];
+    acc_final<T>(pa, t, o);
+    ld4<T>(srt + 4 * t, a);
+    ld4<T>(sc + 4 * t, b);
+    for (int c = 0; c < 4; c++) o[c] = o[c] + o[c] - a[c] - b[c];
+    if (t == 0) o[0] += 1;
+    st4<T>(sscale + 4 * t, o);
+  }
+  Acc<T> acc;  // groups 0..2: sum s = g
+  acc.clear();
+  for (uint64_t i = blockIdx.x; i < mh; i += gridDim.x) {
+    if (g < 2 * NT) {  // stage 1: the folds
+      const int tb = g >> 1, h = g & 1;
+      C x[4];
+      fold_one<T>(sc, sx[g], A.src[tb], i + h * mh, i + h * mh + m, t, x);
+      st4<T>(sf[g] + 4 * t, x);
+      st4<T>(A.dst[tb] + (i + h * mh) * E + 4 * t, x);
+    }
+    __syncthreads();
+    // stage 2 (groups in different waves where they can be, so their
+    // products run side by side): the last group eg'[i] = scale (eg[i] +
+    // eg[i + mh]), groups 0..2 P_s (s = g) from the folded a, b (c, e)
+    if (g == 2 * NT) {
+      C e0[4], e1[4];
+      ld4<T>(A.eg + i * E + 4 * t, e0);
+      ld4<T>(A.eg + (i + mh) * E + 4 * t, e1);
+      for (int c = 0; c < 4; c++) e0[c] += e1[c];
+      st4<T>(sx[g] + 4 * t, e0);
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      Acc<T> pa;
+      pa.clear();
+      acc_mul<T>(pa, sscale, sx[g], t);
+      C eo[4];
+      acc_final<T>(pa, t, eo);
+      st4<T>(seg + 4 * t, eo);
+      st4<T>(A.eg + i * E + 4 * t, eo);
+    } else if (g < 3) {
+      const int s = g;
+      C a0[4], a1[4], b0[4], b1[4], x[4], y[4];
+      ld4<T>(sf[0] + 4 * t, a0);
+      ld4<T>(sf[1] + 4 * t, a1);
+      ld4<T>(sf[2] + 4 * t, b0);
+      ld4<T>(sf[3] + 4 * t, b1);
+      for (int c = 0; c < 4; c++) {
+        x[c] = s == 0 ? a0[c] : s == 1 ? a1[c] : a1[c] - a0[c];
+        y[c] = s == 0 ? b0[c] : s == 1 ? b1[c] : b1[c] - b0[c];
+      }
+      st4<T>(sx[g] + 4 * t, x);
+      st4<T>(sp[s] + 4 * t, y);
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      Acc<T> pa;
+      pa.clear();
+      acc_mul<T>(pa, sx[g], sp[s], t);
+      C p[4];
+      acc_final<T>(pa, t, p);
+      if constexpr (NT == 4) {
+        if (s < 2) {
+          C c4[4], e4[4];
+          ld4<T>(sf[4 + s] + 4 * t, c4);
+          ld4<T>(sf[6 + s] + 4 * t, e4);
+          for (int c = 0; c < 4; c++) p[c] -= c4[c] + e4[c];
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      st4<T>(sp[s] + 4 * t, p);
+    }
+    __syncthreads();
+    if (g < 3) acc_mul<T>(acc, seg, sp[g], t);  // stage 3
+    __syncthreads();
+  }
+  if (g < 3) {
+    C o[4];
+    acc_final<T>(acc, t, o);
+    C* dst = A.sums + (size_t)(blockIdx.x % SUM_COPIES) * 3 * E + g * E + 4 * t;
+    for (int c = 0; c < 4; c++) atom_add(dst + c, o[c]);
+  }
+  if (!tail_last(A.count, &last)) return;
+  __threadfence();
+  gather_sums<T>(A.sums, 3, ssum);
+  if (threadIdx.x == 0) *A.count = 0;
+  __syncthreads();
+  post_q<T>(ssum, 0, nullptr, nullptr, A.qh, A.flag, A.val);
+}
+
+template <class T> struct TailL {
+  const LT<T>* f;  // the round's tables: f + t*fs, g + t*gs (2m entries each)
+  const LT<T>* gt;
+  uint64_t fs, gs;
+  LT<T>* fw;  // folded tables at stride H (may be the sources)
+  LT<T>* gw;
+  uint64_t H, ntab, m;
+  const LT<T>* rc;  // the round's challenge
+  LT<T>* dq;        // q_r in, q_{r+1} out
+  LT<T>* dc;        // the claim lin_claim(q_r, c) out
+  CT<T>* sums;      // SUM_COPIES x 3 elements (2 used), zero on entry
+  uint32_t* count;  // zero on entry
+  LT<T>* qh;        // q_{r+1}
+  uint32_t* flag;
+  uint32_t val;
+};
+// {0,1,inf} linear rounds (as fold2 + k_lin_claim + k_gram_lin +
+// k_sum_post): a block takes (table, pair) items in turn; groups 0..3 fold
+// f at i, i+mh and g at i, i+mh, then groups 0, 2 add f'g' and df' dg'.
+template <class T> __global__ void __launch_bounds__(128) k_tail_lin(TailL<T> A) {
+  using C = CT<T>;
+  using L = LT<T>;
+  __shared__ __attribute__((aligned(16))) L sc[E], sq[E];
+  __shared__ __attribute__((aligned(16))) L sf[4][E];  // f'_i, f'_{i+mh}, g'_i, g'_{i+mh}
+  __shared__ __attribute__((aligned(16))) L sx[4][E];
+  __shared__ C ssum[3][E];
+  __shared__ bool last;
+  const int g = threadIdx.x >> 5, t = threadIdx.x & 31;
+  const uint64_t m = A.m, mh = m / 2, total = A.ntab * mh;
+  if (g == 0) stage<T>(sc, A.rc, t);
+  __syncthreads();
+  Acc<T> acc;  // groups 0, 2: sum s = g / 2
+  acc.clear();
+  for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {
+    const uint64_t tab = w / mh, i = w % mh;
+    {  // stage 1: group g folds table (g < 2 ? f : g), entry i + (g & 1) mh
+      const int h = g & 1;
+      const L* src = g < 2 ? A.f + tab * A.fs * E : A.gt + tab * A.gs * E;
+      L* dst = g < 2 ? A.fw + tab * A.H * E : A.gw + tab * A.H * E;
+      C x[4];
+      fold_one<T>(sc, sx[g], src, i + h * mh, i + h * mh + m, t, x);
+      st4<T>(sf[g] + 4 * t, x);
+      st4<T>(dst + (i + h * mh) * E + 4 * t, x);
+    }
+    __syncthreads();
+    if (g == 0 || g == 2) {  // stage 2 (groups 0 and 2 sit in different waves)
+      if (g == 0) {
+        acc_mul<T>(acc, sf[0], sf[2], t);
+      } else {
+        C f0[4], f1[4], g0[4], g1[4], x[4], y[4];
+        ld4<T>(sf[0] + 4 * t, f0);
+        ld4<T>(sf[1] + 4 * t, f1);
+        ld4<T>(sf[2] + 4 * t, g0);
+        ld4<T>(sf[3] + 4 * t, g1);
+        for (int c = 0; c < 4; c++) {
+          x[c] = f1[c] - f0[c];
+          y[c] = g1[c] - g0[c];
+        }
+        st4<T>(sx[2] + 4 * t, x);
+        st4<T>(sx[3] + 4 * t, y);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        acc_mul<T>(acc, sx[2], sx[3], t);
+      }
+    }
+    __syncthreads();
+  }
+  if (g == 0 || g == 2) {
+    C o[4];
+    acc_final<T>(acc, t, o);
+    C* dst = A.sums + (size_t)(blockIdx.x % SUM_COPIES) * 3 * E + (g / 2) * E + 4 * t;
+    for (int c = 0; c < 4; c++) atom_add(dst + c, o[c]);
+  }
+  if (!tail_last(A.count, &last)) return;
+  __threadfence();
+  gather_sums<T>(A.sums, 2, ssum);
+  if (threadIdx.x == 0) *A.count = 0;
+  // the next claim c' = q0 + c (q1 + c q2) from q_r, kept in sq and written to dc
+  if (g == 0) stage<T>(sq, A.dq + 2 * E, t);
+  for (int step = 1; step >= 0; step--) {
+    __syncthreads();
+    C o[4], a[4];
+    if (g == 0) {
+      Acc<T> pa;
+      pa.clear();
+      acc_mul<T>(pa, sc, sq, t);
+      acc_final<T>(pa, t, o);
+      ld4<T>(A.dq + step * E + 4 * t, a);
+      for (int k = 0; k < 4; k++) o[k] += a[k];
+    }
+    __syncthreads();
+    if (g == 0) {
+      st4<T>(sq + 4 * t, o);
+      if (!step) st4<T>(A.dc + 4 * t, o);
     }
   }
+  __syncthreads();
+  post_q<T>(ssum, 1, sq, A.dq, A.qh, A.flag, A.val);
 }
 
 // ============================================================================

@@ -244,6 +244,15 @@ struct Hub;
 template <class T> struct HubTask;
 template <class T> void hub_run(Hub* h, HubTask<T>& t, hipStream_t st);
 constexpr uint64_t HUB_FOLD_MAX = 4096;  // rounds whose folds touch fewer elements go to the hub
+// Rounds whose folds touch fewer elements (latency-bound) run as one tail
+// kernel (k_tail_mult / k_tail_lin: fold, marginalization, the next round's
+// sums and its post) after the challenge's lift.
+constexpr uint64_t TAIL_FOLD_MAX = 4096;
+// Block caps of the tail kernels (one block per pair, or per (table, pair)):
+// past these, the matrix-core folds and Gram sums of the ordinary rounds are
+// faster (measured at 2^20 constraints: a 512-block linear tail 41 us
+// against 22 us for k_fold_mx + k_gram_lin + k_sum_post).
+constexpr uint64_t TAIL_MULT_BLOCKS = 256, TAIL_LIN_BLOCKS = 64;
 
 template <class T> struct Engine {
   using L = typename T::L;
@@ -1100,7 +1109,10 @@ template <class T> struct Engine {
     mm->r = d;
     if constexpr (kMx<T>) {
       mm->fp = frag;
-      launch("k_lift_mulmat_fs", 5 * EB, lm, (k_lift_mulmat_fs<T>), dim3(1), dim3(256), f, d, d_frob.p, T::K - 1, frag);
+      if (frag)
+        launch("k_lift_mulmat_fs", 5 * EB, lm, (k_lift_mulmat_fs<T>), dim3(1), dim3(256), f, d, d_frob.p, T::K - 1, frag);
+      else  // a tail round's fold needs no fragments
+        launch("k_lift_fs", 5 * EB, lm, (k_lift_fs<T>), dim3(1), dim3(256), f, d, d_frob.p, T::K - 1);
     } else {
       launch("k_lift_fs", 5 * EB, lm, (k_lift_fs<T>), dim3(1), dim3(256), f, d, d_frob.p, T::K - 1);
       if constexpr (kMx64<T>) {
@@ -1156,8 +1168,11 @@ template <class T> struct Engine {
     mm->r = d;
     if constexpr (kMx<T>) {
       mm->fp = frag;
-      launch("k_lift_mulmat", 2 * EB, lm, (k_lift_mulmat<T>), dim3(1), dim3(256), src, d, d_frob.p, T::K - 1, frag,
-             w);
+      if (frag)
+        launch("k_lift_mulmat", 2 * EB, lm, (k_lift_mulmat<T>), dim3(1), dim3(256), src, d, d_frob.p, T::K - 1, frag,
+               w);
+      else  // a tail round's fold needs no fragments
+        launch("k_lift", 2 * EB, lm, (k_lift_from<T>), dim3(1), dim3(256), src, d, d_frob.p, T::K - 1, w);
     } else {
       launch("k_lift", 2 * EB, lm, (k_lift_from<T>), dim3(1), dim3(256), src, d, d_frob.p, T::K - 1, w);
       if constexpr (kMx64<T>) {
@@ -1575,8 +1590,13 @@ template <class T> struct Engine {
     const bool dfs = !pipe && (devfs || use_hub);
     DBuf<L> drounds(pool, dfs ? 3 * nv * E : 1);
     DBuf<b3::State> dtr(pool, 1);
-    DBuf<C> hsums(pool, use_hub ? 3 * E : 1);
+    DBuf<C> hsums(pool, use_hub ? (size_t)SUM_COPIES * 3 * E : 1);
     size_t hub_from = nv;  // first round run on the hub
+    // tail rounds: their sums accumulate in tsums, tcount finds the last block
+    DBuf<C> tsums(pool, (size_t)SUM_COPIES * 3 * E);
+    DBuf<uint32_t> tcount(pool, 1);
+    HIP_CHECK(hipMemsetAsync(tsums.p, 0, (size_t)SUM_COPIES * 3 * E * sizeof(C), st));
+    HIP_CHECK(hipMemsetAsync(tcount.p, 0, sizeof(uint32_t), st));
     if (dfs) h2d(dtr.p, static_cast<const b3::State*>(&tr.h), sizeof(b3::State));
     wait_ticks = spin_ticks();
     // enqueue the sums of round r over tables of 2m entries; returns the flag value
@@ -1634,12 +1654,51 @@ template <class T> struct Engine {
         for (int i = 0; i < 3; i++) o.rounds.push_back(q[i]);
         post_challenge(tr, "sc/challenge", (int)r, vup);
       };
+      // a small round: its fold, the marginalization and round r+1's sums
+      // and post in one tail kernel (the fold needs no fragments)
+      const bool tail = !use_hub && (uint64_t)m * ntab < TAIL_FOLD_MAX && m / 2 <= TAIL_MULT_BLOCKS && r + 1 < nv;
+      uint32_t* rfrag = tail ? nullptr : round_frag(frags.p, r, (uint64_t)m * ntab);
       if (dfs) {
         checkpoint();
-        lift_device_fs(dtr.p, drounds.p + r * 3 * E, rc, round_frag(frags.p, r, (uint64_t)m * ntab), &mr);
+        lift_device_fs(dtr.p, drounds.p + r * 3 * E, rc, rfrag, &mr);
       } else {
         if (!pipe) host_round();
-        lift_posted((int)r, vup, rc, round_frag(frags.p, r, (uint64_t)m * ntab), &mr);
+        lift_posted((int)r, vup, rc, rfrag, &mr);
+      }
+      if (tail) {
+        TailM<T> A{};
+        for (int t = 0; t < ntab; t++) {
+          A.src[t] = cur[t];
+          A.dst[t] = work.p + (size_t)t * (N / 2) * E;
+        }
+        A.m = m;
+        A.rc = rc;
+        A.rt = drt.p + r * E;
+        A.eg = eg.p;
+        A.sums = tsums.p;
+        A.count = tcount.p;
+        uint32_t vnext = 0;
+        if (dfs) {
+          A.qh = drounds.p + (r + 1) * 3 * E;
+        } else {
+          vnext = ++mseq;
+          A.qh = mbd<L>(mb_down((int)r + 1));
+          A.flag = mbd<uint32_t>(mb_down_flag((int)r + 1));
+          A.val = vnext;
+        }
+        const double tb = (m * ntab * 3.0 + m * 1.5) * EB;
+        // one block per pair of round r+1
+        if (ntab == 4)
+          launch("k_tail_mult", tb, m * (2.0 * ntab + 4) * 32 * PM, (k_tail_mult<T, 4>), dim3((unsigned)(m / 2)),
+                 dim3(32 * 9), A);
+        else
+          launch("k_tail_mult", tb, m * (2.0 * ntab + 4) * 32 * PM, (k_tail_mult<T, 2>), dim3((unsigned)(m / 2)),
+                 dim3(32 * 5), A);
+        for (int t = 0; t < ntab; t++) cur[t] = work.p + (size_t)t * (N / 2) * E;
+        len = m;
+        if (pipe) host_round();
+        vdown = vnext;
+        continue;
       }
       // the next round's eq factor eq_one(rt[r], r_r) rides along with the fold
       Side sd;
@@ -1689,7 +1748,7 @@ template <class T> struct Engine {
       hub_events();
       t.ready = hev[0];
       t.done = hev[1];
-      HIP_CHECK(hipMemsetAsync(hsums.p, 0, 3 * E * sizeof(C), st));
+      HIP_CHECK(hipMemsetAsync(hsums.p, 0, (size_t)SUM_COPIES * 3 * E * sizeof(C), st));
       HIP_CHECK(hipEventRecord(t.ready, st));
       hub_run<T>(hub, t, st);
     }
@@ -1737,8 +1796,12 @@ template <class T> struct Engine {
     const bool dfs = !pipe && (devfs || use_hub);
     DBuf<L> drounds(pool, dfs ? 3 * nv * E : 1);
     DBuf<b3::State> dtr(pool, 1);
-    DBuf<C> hsums(pool, use_hub ? 3 * E : 1);
+    DBuf<C> hsums(pool, use_hub ? (size_t)SUM_COPIES * 3 * E : 1);
     size_t hub_from = nv;
+    DBuf<C> tsums(pool, (size_t)SUM_COPIES * 3 * E);  // tail rounds (as in mult_check)
+    DBuf<uint32_t> tcount(pool, 1);
+    HIP_CHECK(hipMemsetAsync(tsums.p, 0, (size_t)SUM_COPIES * 3 * E * sizeof(C), st));
+    HIP_CHECK(hipMemsetAsync(tcount.p, 0, sizeof(uint32_t), st));
     if (dfs) h2d(dtr.p, static_cast<const b3::State*>(&tr.h), sizeof(b3::State));
     wait_ticks = spin_ticks();
     auto post = [&](size_t r, unsigned nparts) -> uint32_t {
@@ -1782,12 +1845,51 @@ template <class T> struct Engine {
         for (int i = 0; i < 3; i++) o.rounds.push_back(q[i]);
         post_challenge(tr, "sc/challenge", (int)r, vup);
       };
+      const bool tail =
+          !use_hub && 2 * m * (uint64_t)ntab < TAIL_FOLD_MAX && (uint64_t)ntab * (m / 2) <= TAIL_LIN_BLOCKS && r + 1 < nv;
+      uint32_t* rfrag = tail ? nullptr : round_frag(frags.p, r, (uint64_t)m * 2 * ntab);
       if (dfs) {
         checkpoint();
-        lift_device_fs(dtr.p, drounds.p + r * 3 * E, rc, round_frag(frags.p, r, (uint64_t)m * 2 * ntab), &mr);
+        lift_device_fs(dtr.p, drounds.p + r * 3 * E, rc, rfrag, &mr);
       } else {
         if (!pipe) host_round();
-        lift_posted((int)r, vup, rc, round_frag(frags.p, r, (uint64_t)m * 2 * ntab), &mr);
+        lift_posted((int)r, vup, rc, rfrag, &mr);
+      }
+      if (tail) {
+        TailL<T> A{};
+        A.f = f;
+        A.gt = g;
+        A.fs = fs;
+        A.gs = gs;
+        A.fw = fw.p;
+        A.gw = gw.p;
+        A.H = H;
+        A.ntab = (uint64_t)ntab;
+        A.m = m;
+        A.rc = rc;
+        A.dq = dq.p;
+        A.dc = dc.p;
+        A.sums = tsums.p;
+        A.count = tcount.p;
+        uint32_t vnext = 0;
+        if (dfs) {
+          A.qh = drounds.p + (r + 1) * 3 * E;
+        } else {
+          vnext = ++mseq;
+          A.qh = mbd<L>(mb_down((int)r + 1));
+          A.flag = mbd<uint32_t>(mb_down_flag((int)r + 1));
+          A.val = vnext;
+        }
+        // one block per (table, pair) of round r+1
+        launch("k_tail_lin", m * ntab * 6.0 * EB, m * ntab * 5.0 * 32 * PM, (k_tail_lin<T>),
+               dim3((unsigned)((uint64_t)ntab * (m / 2))), dim3(128), A);
+        f = fw.p;
+        g = gw.p;
+        fs = gs = H;
+        len = m;
+        if (pipe) host_round();
+        vdown = vnext;
+        continue;
       }
       // the next claim q_r(r_r) rides along with the folds
       fold2(mr, m, f, fw.p, ntab, fs, H, g, gw.p, ntab, gs, H, Side{2, dq.p, rc, dc.p});
@@ -1825,7 +1927,7 @@ template <class T> struct Engine {
       hub_events();
       t.ready = hev[0];
       t.done = hev[1];
-      HIP_CHECK(hipMemsetAsync(hsums.p, 0, 3 * E * sizeof(C), st));
+      HIP_CHECK(hipMemsetAsync(hsums.p, 0, (size_t)SUM_COPIES * 3 * E * sizeof(C), st));
       HIP_CHECK(hipEventRecord(t.ready, st));
       hub_run<T>(hub, t, st);
     }
