@@ -711,47 +711,6 @@ __global__ void __launch_bounds__(256) k_cgemm_sk(const uint32_t* frag, const LT
   }
 }
 
-// The same product on the vector ALUs (64-bit limbs):
-//   out[p][limb] = sum_{j < 1024} B2[j][limb] C_p[j],  j = 32 s + s',
-// B2 the basis products as plain limbs ([j][limb]).  A block takes 32 block
-// rows: thread (g, limb) accumulates 16 of them; per s it stages the 32
-// C_p[s][.] values of the block's rows in LDS and reads its 32 B2 limbs.
-template <class T>
-__global__ void __launch_bounds__(256) k_cgemm_valu(const LT<T>* b2, const LT<T>* cmat, LT<T>* out, uint64_t nb) {
-  using C = CT<T>;
-  constexpr int PB = 32, PG = 16;
-  __shared__ C cs[PB][32];
-  const int limb = threadIdx.x & (E - 1), g = threadIdx.x >> 7;
-  for (uint64_t p0 = (uint64_t)blockIdx.x * PB; p0 < nb; p0 += (uint64_t)gridDim.x * PB) {
-    C acc[PG];
-#pragma unroll
-    for (int i = 0; i < PG; i++) acc[i] = 0;
-    for (int s = 0; s < 32; s++) {
-      __syncthreads();
-      for (int t = threadIdx.x; t < PB * 32; t += blockDim.x) {
-        const uint64_t p = p0 + t / 32;
-        cs[t / 32][t % 32] = p < nb ? (C)cmat[((uint64_t)s * nb + p) * 32 + (t % 32)] : (C)0;
-      }
-      __syncthreads();
-      C bv[32];
-#pragma unroll
-      for (int sp = 0; sp < 32; sp++) bv[sp] = (C)b2[(size_t)(32 * s + sp) * E + limb];
-#pragma unroll
-      for (int i = 0; i < PG; i++) {
-        C a = acc[i];
-#pragma unroll
-        for (int sp = 0; sp < 32; sp++) a += bv[sp] * cs[g * PG + i][sp];
-        acc[i] = a;
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < PG; i++) {
-      const uint64_t p = p0 + g * PG + i;
-      if (p < nb) out[p * E + limb] = (LT<T>)acc[i];
-    }
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Fixed-multiplier products with 64-bit limbs on the matrix cores.
 //
@@ -962,6 +921,93 @@ __global__ void __launch_bounds__(256) k_mulfix_mx64(const uint32_t* frag, const
           q[0] = make_ulonglong2(R[4 * j], R[4 * j + 1]);
           q[1] = make_ulonglong2(R[4 * j + 2], R[4 * j + 3]);
         }
+      }
+    }
+  }
+}
+
+// A-fragments (8 digit planes) of the basis products B2[k][row] = (b_s
+// b_s')[row], k = 32 s + s' (1024 columns), with 64-bit limbs: thread
+// (kb, rb, lane, word) forms the digits of its four k values, as mx_frags
+// lays them out ([plane][kb][rb][lane][16 bytes], byte c of word w at
+// k = 32 kb + 8 w + 4 h + c, h = lane / 32).
+__global__ void __launch_bounds__(256) k_b2frag64(const uint64_t* b2, uint32_t* frag) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;  // 32 * 4 * 64 * 4 items
+  if (id >= 32 * 4 * 64 * 4) return;
+  const int wd = id & 3, lane = (id >> 2) & 63, rb = (id >> 8) & 3, kb = id >> 10;
+  const int h = lane >> 5, row = 32 * rb + (lane & 31);
+  uint64_t z[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) z[c] = bal_digits64(b2[(size_t)(32 * kb + 8 * wd + 4 * h + c) * E + row]);
+#pragma unroll
+  for (int p = 0; p < 8; p++) {
+    uint32_t word = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) word |= (uint32_t)((z[c] >> (8 * p)) & 0xffu) << (8 * c);
+    frag[((((size_t)p * 32 + kb) * 4 + rb) * 64 + lane) * 4 + wd] = word;
+  }
+}
+
+// The layered matvec's GEMM with 64-bit limbs on the matrix cores:
+//   out[p] = sum_{k < 1024} B2[k] C_p[k]   (k = 32 s + s', C_p as k_cmat
+// stores it, k-block s at cmat + (s nb + p) 32).  Every digit class Y_s
+// (s < 8) is exact in i32 over the whole sum: |Y_s| <= 8 * 1024 * 2^14 =
+// 2^27.  Block: 4 waves on 32 block rows, wave w output rows 32w..32w+31
+// (one row-block); per k-block 36 digit-pair MFMAs, C and the fragments of
+// k-block kb+1 loaded during kb's matrix work; then a 64-bit Horner over
+// the classes.
+__global__ void __launch_bounds__(256) k_cgemm64(const uint32_t* frag, const uint64_t* cmat, uint64_t* out,
+                                                 uint64_t nb) {
+  const v4i* A = reinterpret_cast<const v4i*>(frag);
+  const int l = threadIdx.x & 63, col = l & 31, h = l >> 5, rb = threadIdx.x >> 6;
+  const uint64_t ntiles = (nb + 31) / 32;
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint64_t p = tile * 32 + col;
+    const bool ok = p < nb;
+    const uint64_t* cp = cmat + (ok ? p : 0) * 32;
+    v16i acc[8];
+#pragma unroll
+    for (int s = 0; s < 8; s++) acc[s] = (v16i)0;
+    uint64_t v[16];
+    v4i a[8];
+    auto load_kb = [&](int kb) {
+      if (ok && kb < 32) {
+        ld16k64(cp + (uint64_t)kb * nb * 32, 0, h, v);
+      } else {
+#pragma unroll
+        for (int b = 0; b < 16; b++) v[b] = 0;
+      }
+      const int kk = kb & 31;
+#pragma unroll
+      for (int q = 0; q < 8; q++) a[q] = A[((q * 32 + kk) * 4 + rb) * 64 + l];
+    };
+    load_kb(0);
+    for (int kb = 0; kb < 32; kb++) {
+      v4i B[8], ac[8];
+      planes16_64(v, B);
+#pragma unroll
+      for (int q = 0; q < 8; q++) ac[q] = a[q];
+      load_kb(kb + 1);
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+#pragma unroll
+        for (int pd = 0; pd + q < 8; pd++)
+          acc[q + pd] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ac[q], B[pd], acc[q + pd], 0, 0, 0);
+    }
+    if (ok) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        uint64_t x[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          uint64_t y = (uint64_t)(int64_t)acc[7][4 * j + c];
+#pragma unroll
+          for (int s = 6; s >= 0; s--) y = (y << 8) + (uint64_t)(int64_t)acc[s][4 * j + c];
+          x[c] = y;
+        }
+        ulonglong2* o = reinterpret_cast<ulonglong2*>(out + p * E + 32 * rb + 8 * j + 4 * h);
+        o[0] = make_ulonglong2(x[0], x[1]);
+        o[1] = make_ulonglong2(x[2], x[3]);
       }
     }
   }

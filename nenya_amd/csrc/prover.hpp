@@ -404,6 +404,11 @@ template <class T> struct Engine {
       d_lbfrag = DBuf<uint32_t>(pool, (size_t)16 * 8 * 4096);
       hipLaunchKernelGGL(k_mulmat64, dim3(16, 4), dim3(256), 0, st, (const uint64_t*)lb.p, d_lbfrag.p);
       HIP_CHECK(hipGetLastError());
+      // the basis products' fragments (8 digit planes, 1 MiB) for k_cgemm64
+      d_b2frag = DBuf<uint32_t>(pool, (size_t)8 * 32 * 4 * 64 * 4);
+      hipLaunchKernelGGL(k_b2frag64, dim3(32 * 4 * 64 * 4 / 256), dim3(256), 0, st, (const uint64_t*)d_b2.p,
+                         d_b2frag.p);
+      HIP_CHECK(hipGetLastError());
       sync();
     }
     sync();
@@ -627,6 +632,12 @@ template <class T> struct Engine {
            dim3(grid_for(ntiles, 4, kND<T> <= 2 ? 256 : 64), 4), dim3(256), (const uint32_t*)d_b2frag.p, cm, part.p, nb);
     launch("k_sum_split", 5.0 * nb * EB, 0, (k_sum_split<T>), dim3(grid_for(nb * E, 256, 4096)), dim3(256),
            (const L*)part.p, (uint64_t)4, nb, out);
+  }
+  // The same GEMM with 64-bit limbs (exact digit classes, no split).
+  void cgemm64(const L* cm, L* out, uint64_t nb) {
+    launch("k_cgemm64", (double)nb * (1024 * sizeof(L) + EB), (double)nb * 1024 * E, k_cgemm64,
+           dim3(grid_for((nb + 31) / 32, 1, 1024)), dim3(256), (const uint32_t*)d_b2frag.p, (const uint64_t*)cm,
+           (uint64_t*)out, nb);
   }
   void sub(const L* a, const L* b, L* o, uint64_t nl) {
     launch("k_sub", nl * 3.0 * sizeof(L), 0, (k_sub<T>), dim3(grid_for(nl, 256, 1u << 20)), dim3(256), a, b, o, nl);

@@ -526,16 +526,14 @@ template <class T> struct MimdParams : ParamsBase {
       e.join();
     } else {
       // C_r from the cells' instance values and the witness slots, then the
-      // basis products on the vector ALUs (as the layered matvec)
+      // basis-product GEMM with 64-bit limbs (as the layered matvec)
       (void)wphi;
       DBuf<L> cm(e.pool, n * 1024);
       for (int w = 0; w < 3; w++) {
         e.launch("k_cmat_mimd", (double)n * 1024 * sizeof(L), (double)ncell[w] * 1024, (k_cmat_mimd<T>),
                  dim3(grid_for(n, 8, 4096)), dim3(256), (const uint64_t*)(cptr.p + w * (n + 1)), (const uint32_t*)ccol.p,
                  (const L*)cval.p, slots, cm.p, (uint64_t)n);
-        e.launch("k_cgemm_valu", (double)n * (1024 * sizeof(L) + Engine<T>::EB), (double)n * 1024 * E,
-                 (k_cgemm_valu<T>), dim3(grid_for(n, 32, 2048)), dim3(256), (const L*)e.d_b2.p, (const L*)cm.p,
-                 out + w * n * E, (uint64_t)n);
+        e.cgemm64(cm.p, out + w * n * E, (uint64_t)n);
       }
     }
   }
@@ -866,17 +864,14 @@ template <class T> struct GeneralParams : ParamsBase {
       }
       e.join();
     } else {
-      // the same factorization on the vector ALUs: C_p, then the products
-      // b_s b_s' (131072 limb-MACs per block row instead of a full ring
-      // product per diagonal cell)
+      // the same factorization with 64-bit limbs: C_p, then the products
+      // b_s b_s' on the matrix cores with exact digit classes (k_cgemm64)
       DBuf<L> cm(&pool, nb * 1024);
       for (int m = 0; m < 3; m++) {
         e.launch("k_cmat", (double)nb * 1024 * sizeof(L), 0, (k_cmat<T>), dim3(grid_for(nb, 8, 4096)), dim3(256),
                  (const uint32_t*)mr[m].ptr.p, (const uint32_t*)mr[m].idx.p, (const L*)mr[m].val.p, w, cm.p,
                  (uint64_t)nb);
-        e.launch("k_cgemm_valu", (double)nb * (1024 * sizeof(L) + Engine<T>::EB), (double)nb * 1024 * E,
-                 (k_cgemm_valu<T>), dim3(grid_for(nb, 32, 2048)), dim3(256), (const L*)e.d_b2.p, (const L*)cm.p,
-                 sums.p + m * nb * E, (uint64_t)nb);
+        e.cgemm64(cm.p, sums.p + m * nb * E, (uint64_t)nb);
       }
     }
     // 4. c_hat = pack_r2(psi(c_sum)) (general.rs:410-414); a_hat, b_hat = phi(slots)
