@@ -1013,6 +1013,80 @@ __global__ void __launch_bounds__(256) k_cgemm64(const uint32_t* frag, const uin
   }
 }
 
+// A-fragments (8 digit planes) of pack_r2 o rot_L for L < 32 with 64-bit
+// limbs: P_L[row][k] = b_{(k - L) mod 32}[row] (one k-block; layout as
+// k_pack_mx reads it: [L][plane][rb][lane][16 bytes]).
+__global__ void __launch_bounds__(256) k_packfrag64(const uint64_t* basis, uint32_t* frag) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;  // 32 * 4 * 64 * 4 items
+  if (id >= 32 * 4 * 64 * 4) return;
+  const int wd = id & 3, lane = (id >> 2) & 63, rb = (id >> 8) & 3, L = id >> 10;
+  const int h = lane >> 5, row = 32 * rb + (lane & 31);
+  uint64_t z[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) z[c] = bal_digits64(basis[(size_t)((8 * wd + 4 * h + c - L) & 31) * E + row]);
+#pragma unroll
+  for (int p = 0; p < 8; p++) {
+    uint32_t word = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) word |= (uint32_t)((z[c] >> (8 * p)) & 0xffu) << (8 * c);
+    frag[((((size_t)L * 8 + p) * 4 + rb) * 64 + lane) * 4 + wd] = word;
+  }
+}
+
+// pack_r2 on the matrix cores with 64-bit limbs (as k_pack_mx):
+// out[r * n + e] = P_L slots[e], L = (r + rot0) mod 32; the 32-long sum of
+// one k-block keeps every digit class exact in i32 (|Y_s| <= 8 * 32 * 2^14),
+// 36 MFMAs per row-block, then a 64-bit Horner.  Block: 32 elements, wave w
+// output rows 32w..32w+31; the slots are read once for all nrot rotations.
+__global__ void __launch_bounds__(256) k_pack_mx64(const uint32_t* frag, const uint64_t* slots, uint64_t n, int nrot,
+                                                   int rot0, uint64_t* out) {
+  const int l64 = threadIdx.x & 63, col = l64 & 31, h = l64 >> 5, rb = threadIdx.x >> 6;
+  const v4i* A = reinterpret_cast<const v4i*>(frag);
+  const uint64_t ntiles = (n + 31) / 32;
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint64_t e = tile * 32 + col;
+    const bool ok = e < n;
+    uint64_t v[16];
+    if (ok) {
+      ld16k64(slots + e * 32, 0, h, v);
+    } else {
+#pragma unroll
+      for (int b = 0; b < 16; b++) v[b] = 0;
+    }
+    v4i B[8];
+    planes16_64(v, B);
+    for (int r = 0; r < nrot; r++) {
+      const int L = (r + rot0) & 31;
+      v16i acc[8];
+#pragma unroll
+      for (int d = 0; d < 8; d++) acc[d] = (v16i)0;
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const v4i a = A[(((size_t)L * 8 + q) * 4 + rb) * 64 + l64];
+#pragma unroll
+        for (int pd = 0; pd + q < 8; pd++)
+          acc[q + pd] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, B[pd], acc[q + pd], 0, 0, 0);
+      }
+      if (ok) {
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+          uint64_t y4[4];
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            uint64_t y = (uint64_t)(int64_t)acc[7][4 * jj + c];
+#pragma unroll
+            for (int d = 6; d >= 0; d--) y = (y << 8) + (uint64_t)(int64_t)acc[d][4 * jj + c];
+            y4[c] = y;
+          }
+          ulonglong2* o = reinterpret_cast<ulonglong2*>(out + ((uint64_t)r * n + e) * E + 32 * rb + 8 * jj + 4 * h);
+          o[0] = make_ulonglong2(y4[0], y4[1]);
+          o[1] = make_ulonglong2(y4[2], y4[3]);
+        }
+      }
+    }
+  }
+}
+
 // Linear-check round sums with 64-bit limbs (as k_gram_lin): the two Gram
 // matrices over (f_lo, g_lo) and (f_hi - f_lo, g_hi - g_lo) with exact digit
 // classes.  Per 32-element chunk and 32 x 32 block, class s of the digit

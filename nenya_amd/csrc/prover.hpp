@@ -409,6 +409,11 @@ template <class T> struct Engine {
       hipLaunchKernelGGL(k_b2frag64, dim3(32 * 4 * 64 * 4 / 256), dim3(256), 0, st, (const uint64_t*)d_b2.p,
                          d_b2frag.p);
       HIP_CHECK(hipGetLastError());
+      // pack_r2 o rot_L fragments (8 digit planes, 32 rotations, 1 MiB) for k_pack_mx64
+      d_packfrag = DBuf<uint32_t>(pool, (size_t)32 * 8 * 4 * 64 * 4);
+      hipLaunchKernelGGL(k_packfrag64, dim3(32 * 4 * 64 * 4 / 256), dim3(256), 0, st, (const uint64_t*)d_basis.p,
+                         d_packfrag.p);
+      HIP_CHECK(hipGetLastError());
       sync();
     }
     sync();
@@ -574,8 +579,10 @@ template <class T> struct Engine {
              dim3((unsigned)std::min<uint64_t>(ntiles, 8192)), dim3(256), (const uint32_t*)d_packfrag.p, slots, n, nrot,
              rot0, out);
     } else {
-      launch("k_pack", n * nrot * (32.0 * sizeof(L) + EB), n * nrot * 4096.0, (k_pack<T>), dim3(grid_for(n * nrot, GPB)),
-             dim3(GB), slots, d_basis.p, out, n, nrot, rot0);
+      const uint64_t ntiles = (n + 31) / 32;
+      launch("k_pack_mx64", n * (32.0 * sizeof(L) + nrot * EB), n * nrot * 4096.0, k_pack_mx64,
+             dim3((unsigned)std::min<uint64_t>(ntiles, 8192)), dim3(256), (const uint32_t*)d_packfrag.p,
+             (const uint64_t*)slots, n, nrot, rot0, (uint64_t*)out);
     }
   }
   void psi(const L* x, L* slots, uint64_t n) {
