@@ -1087,6 +1087,107 @@ __global__ void __launch_bounds__(256) k_pack_mx64(const uint32_t* frag, const u
   }
 }
 
+// m-bar with 64-bit limbs on the matrix cores (as k_mbar_mx):
+// out[l][j] = sum_{i in rows of split s} M_{e_i} X[l * lstride + i * k2 + j],
+// frag: the fragments of every M_{e_i} (k_mulmat64, 8 planes, 128 KB each).
+// Block: one tile of 32 consecutive outputs of one layer, wave w output
+// rows 32w..32w+31, 36 digit-pair MFMAs per (row, k-block).  The tile's X
+// row is staged once per block as digit planes in LDS (thread (kb, lane)
+// forms the 16 limbs of its k-block; two buffers, one barrier per row), its
+// global loads issued one row ahead; each wave's fragments are loaded one
+// k-block ahead.  The digit classes add up exactly in i32 over 64 rows
+// (|Y_s| <= 8 * 128 * 64 * 2^14 = 2^30) and are folded into 64-bit totals
+// by a Horner step every 64 rows.  Block b takes split b % nsplit (rows
+// [s*rows/nsplit, (s+1)*rows/nsplit)) and writes out + s * nl * k2 elements;
+// workgroups go to the eight XCDs round-robin, so with nsplit a multiple of
+// 8 an XCD reads the fragments of one eighth of the rows.
+__global__ void __launch_bounds__(256) k_mbar_mx64(const uint32_t* frag, const uint64_t* X, uint64_t rows, uint64_t k2,
+                                                   uint64_t lstride, uint64_t nl, uint64_t nsplit, uint64_t* out) {
+  __shared__ v4i sB[2][4 * 8 * 64];  // [buffer][k-block][plane][lane], 2 x 32 KB
+  const v4i* A = reinterpret_cast<const v4i*>(frag);
+  const int l64 = threadIdx.x & 63, col = l64 & 31, h = l64 >> 5, w = threadIdx.x >> 6;
+  const uint64_t tpl = (k2 + 31) / 32, ntiles = tpl * nl;
+  const uint64_t sp = blockIdx.x % nsplit, nq = gridDim.x / nsplit;
+  const uint64_t i0 = rows * sp / nsplit, i1 = rows * (sp + 1) / nsplit;
+  out += sp * nl * k2 * E;
+  for (uint64_t tile = blockIdx.x / nsplit; tile < ntiles; tile += nq) {
+    const uint64_t l = tile / tpl, j = (tile % tpl) * 32 + col;
+    const bool ok = j < k2;
+    const uint64_t* xl = X + (l * lstride + (ok ? j : 0)) * E;
+    uint64_t tot[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) tot[c] = 0;
+    uint64_t v[16];
+    // this thread's share of row i: k-block w of its lane's column
+    auto ldrow = [&](uint64_t i, uint64_t ie) {
+      if (ok && i < ie) {
+        ld16k64(xl + i * k2 * E, w, h, v);
+      } else {
+#pragma unroll
+        for (int b = 0; b < 16; b++) v[b] = 0;
+      }
+    };
+    auto strow = [&](int buf) {
+      v4i P[8];
+      planes16_64(v, P);
+#pragma unroll
+      for (int p = 0; p < 8; p++) sB[buf][(w * 8 + p) * 64 + l64] = P[p];
+    };
+    for (uint64_t ib = i0; ib < i1; ib += 64) {
+      const uint64_t ie = i1 < ib + 64 ? i1 : ib + 64, nt = 4 * (ie - ib);
+      v16i acc[8];
+#pragma unroll
+      for (int s = 0; s < 8; s++) acc[s] = (v16i)0;
+      ldrow(ib, ie);
+      lds_barrier();  // the buffers are no longer read
+      strow(0);
+      ldrow(ib + 1, ie);
+      lds_barrier();
+      v4i a[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) a[q] = A[ib * 8192 + frag_at(q, 0, w) * 64 + l64];
+      for (uint64_t t = 0; t < nt; t++) {
+        const uint64_t i = ib + (t >> 2), tn = t + 1;
+        const int kb = (int)(t & 3), buf = (int)((i - ib) & 1);
+        v4i an[8];
+        const uint64_t in = tn < nt ? ib + (tn >> 2) : ib;
+#pragma unroll
+        for (int q = 0; q < 8; q++) an[q] = A[in * 8192 + frag_at(q, (int)(tn & 3), w) * 64 + l64];
+        const v4i* Bs = &sB[buf][kb * 8 * 64 + l64];
+#pragma unroll
+        for (int p = 0; p < 8; p++) {
+          const v4i B = Bs[p * 64];
+#pragma unroll
+          for (int q = 0; p + q < 8; q++)
+            acc[q + p] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[q], B, acc[q + p], 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) a[q] = an[q];
+        if (kb == 3) {  // row i done: stage row i + 1, load row i + 2
+          if (i + 1 < ie) strow(buf ^ 1);
+          ldrow(i + 2, ie);
+          lds_barrier();
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 16; c++) {
+        uint64_t y = (uint64_t)(int64_t)acc[7][c];
+#pragma unroll
+        for (int s = 6; s >= 0; s--) y = (y << 8) + (uint64_t)(int64_t)acc[s][c];
+        tot[c] += y;
+      }
+    }
+    if (ok) {
+#pragma unroll
+      for (int jj = 0; jj < 4; jj++) {
+        ulonglong2* o = reinterpret_cast<ulonglong2*>(out + (l * k2 + j) * E + 32 * w + 8 * jj + 4 * h);
+        o[0] = make_ulonglong2(tot[4 * jj], tot[4 * jj + 1]);
+        o[1] = make_ulonglong2(tot[4 * jj + 2], tot[4 * jj + 3]);
+      }
+    }
+  }
+}
+
 // Linear-check round sums with 64-bit limbs (as k_gram_lin): the two Gram
 // matrices over (f_lo, g_lo) and (f_hi - f_lo, g_hi - g_lo) with exact digit
 // classes.  Per 32-element chunk and 32 x 32 block, class s of the digit

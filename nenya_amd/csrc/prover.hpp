@@ -1481,6 +1481,26 @@ template <class T> struct Engine {
              dim3(256), (const L*)part.p, nsplit, nl * k2, out);
   }
 
+  // mbar_mx with 64-bit limbs (k_mbar_mx64): the rows are split eight ways
+  // (one share of the fragments per XCD's L2) when their fragments exceed
+  // 2 MB, and further while there are fewer than 1024 blocks.
+  void mbar_mx64(const L* e, uint64_t rows, const L* X, uint64_t k2, uint64_t lstride, uint64_t nl, L* out) {
+    DBuf<uint32_t> fr(pool, (size_t)rows * 8 * 4096);
+    launch("k_mulmat64", rows * (EB + 8 * 16384.0), 0, k_mulmat64, dim3((unsigned)rows, 4), dim3(256),
+           (const uint64_t*)e, fr.p);
+    const uint64_t tiles = nl * ((k2 + 31) / 32);
+    uint64_t nsplit = rows * 131072 > (2u << 20) ? std::min<uint64_t>(rows, 8) : 1;
+    while (nsplit * 2 <= rows && tiles * nsplit < 1024) nsplit *= 2;
+    const uint64_t ntb = std::min<uint64_t>(tiles, 65535 / nsplit);
+    DBuf<L> part(pool, nsplit > 1 ? nsplit * nl * k2 * E : 1);
+    launch("k_mbar_mx64", (double)nl * rows * k2 * EB + nsplit * nl * k2 * EB, (double)nl * rows * k2 * E * E,
+           k_mbar_mx64, dim3((unsigned)(ntb * nsplit)), dim3(256), (const uint32_t*)fr.p, (const uint64_t*)X, rows, k2,
+           lstride, nl, nsplit, (uint64_t*)(nsplit > 1 ? part.p : out));
+    if (nsplit > 1)
+      launch("k_sum_split", (nsplit + 1.0) * nl * k2 * EB, 0, (k_sum_split<T>), dim3(grid_for(nl * k2 * E, 256, 4096)),
+             dim3(256), (const L*)part.p, nsplit, nl * k2, out);
+  }
+
   G open(const PcsDev<T>& pc, const CommitDev<T>& c, const G* z, const G* weights, size_t s, Sink& out,
          bool bg_tail = false) {
     DBuf<L> y = open_dev(pc, c, z, weights, s, out, bg_tail);
@@ -1510,6 +1530,8 @@ template <class T> struct Engine {
     double mbb = (rows * P.k2 + P.k2) * EB;
     if constexpr (kMx<T>) {
       mbar_mx(e0s.p, rows, c.msg, P.k2, 0, 1, mbar.p);
+    } else if constexpr (kMx64<T>) {
+      mbar_mx64(e0s.p, rows, c.msg, P.k2, 0, 1, mbar.p);
     } else if (P.packed)
       launch("k_mbar", mbb, rows * P.k2 * 16 * PM, (k_mbar<T, 16>), dim3(grid_for(P.k2, GPB)), dim3(GB), (const L*)e0s.p,
              c.msg, rows, (uint64_t)P.k2, mbar.p, (uint64_t)0);
@@ -1544,6 +1566,8 @@ template <class T> struct Engine {
     DBuf<L> mbar(pool, (size_t)nl * P.k2 * E);
     if constexpr (kMx<T>)
       mbar_mx(e0.p, P.k1, rho, P.k2, nb, nl, mbar.p);
+    else if constexpr (kMx64<T>)
+      mbar_mx64(e0.p, P.k1, rho, P.k2, nb, nl, mbar.p);
     else
       launch("k_rot_mbar", (double)nl * P.k1 * P.k2 * EB, (double)nl * P.k1 * P.k2 * 16 * PM, (k_rot_mbar<T>),
              dim3(grid_for((uint64_t)nl * P.k2, GPB)), dim3(GB), (const L*)e0.p, rho, nb, (uint64_t)P.k1, (uint64_t)P.k2,
