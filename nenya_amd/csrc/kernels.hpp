@@ -1287,8 +1287,25 @@ constexpr int DPP_HALF_MIRROR = 0x141; // lane i <-> 7-i within 8 lanes
 //
 // square_times: `count` squarings of the element in sx (sx2 = 2 sx beside
 // it), 256 threads, C the arithmetic width (the ring's, or a narrower one
-// where only the low bits are needed).  Ends with sx, sx2 updated.
-template <class C>
+// where only the low bits are needed).  Ends with sx, sx2 updated.  M24:
+// the products on the 24-bit multiplier (C 32-bit; every product and sum is
+// then exact mod 2^24 only, the limbs' upper bits are left undefined).
+template <bool M24, class C> __device__ __forceinline__ void d_raw_sq(const C a[4], const C b[4], C p[7]) {
+  if constexpr (M24) {
+    static_assert(sizeof(C) == 4, "24-bit products on 32-bit limbs");
+    auto m = [](C x, C y) -> C { return (C)__umul24((unsigned)x, (unsigned)y); };
+    p[0] = m(a[0], b[0]);
+    p[1] = m(a[0], b[1]) + m(a[1], b[0]);
+    p[2] = m(a[0], b[2]) + m(a[1], b[1]) + m(a[2], b[0]);
+    p[3] = m(a[0], b[3]) + m(a[1], b[2]) + m(a[2], b[1]) + m(a[3], b[0]);
+    p[4] = m(a[1], b[3]) + m(a[2], b[2]) + m(a[3], b[1]);
+    p[5] = m(a[2], b[3]) + m(a[3], b[2]);
+    p[6] = m(a[3], b[3]);
+  } else {
+    d_raw(a, b, p);
+  }
+}
+template <class C, bool M24 = false>
 __device__ __forceinline__ void square_times(C* sx, C* sx2, C (*sred)[32][4], int count) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, dl = lane & 7;
@@ -1302,18 +1319,27 @@ __device__ __forceinline__ void square_times(C* sx, C* sx2, C (*sred)[32][4], in
     C ac[7];
 #pragma unroll
     for (int c = 0; c < 7; c++) ac[c] = 0;
+    // the three pairs' factors are read at once (one LDS latency): unused
+    // pairs read clamped indices and contribute zero
+    C a[3][4], b[3][4];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       const int q = q0 + k;
-      if (q < nq) {
-        const int i = wrap ? t + 1 + q : q, j = wrap ? 31 - q : t - q;
-        C a[4], b[4], p[7];
-        ldc4((i != j ? sx2 : sx) + 4 * i, a);
-        ldc4(sx + 4 * j, b);
-        d_raw(a, b, p);
+      int i = wrap ? t + 1 + q : q, j = wrap ? 31 - q : t - q;
+      i = i < 0 ? 0 : (i > 31 ? 31 : i);
+      j = j < 0 ? 0 : (j > 31 ? 31 : j);
+      ldc4((i != j ? sx2 : sx) + 4 * i, a[k]);
+      ldc4(sx + 4 * j, b[k]);
+    }
 #pragma unroll
-        for (int c = 0; c < 7; c++) ac[c] += p[c];
-      }
+    for (int k = 0; k < 3; k++) {
+      const C m = q0 + k < nq ? (C)~(C)0 : (C)0;
+      C bm[4], p[7];
+#pragma unroll
+      for (int c = 0; c < 4; c++) bm[c] = b[k][c] & m;
+      d_raw_sq<M24>(a[k], bm, p);
+#pragma unroll
+      for (int c = 0; c < 7; c++) ac[c] += p[c];
     }
     C v[8];
     d_red(ac, v);
@@ -1383,11 +1409,15 @@ __device__ __forceinline__ void teich_lift_block(const LT<T>* src, LT<T>* el, co
     }
     __syncthreads();
   }
+  // z_i = y^(2^i) is needed exact mod 2^K only at the end, and u = v mod 2^s
+  // gives u^2 = v^2 mod 2^(s+1): z_{n-j} needs only K - j bits.  So squarings
+  // run on the narrowest arithmetic that keeps this: the first ones on the
+  // 24-bit multiplier (exact mod 2^24) as long as the squarings after them
+  // bring the precision to K (32 at the end of the 32-bit stage below).
+  auto n24 = [](int n, int bits) { return n + 24 - bits < 0 ? 0 : (n + 24 - bits < n ? n + 24 - bits : n); };
   if constexpr (sizeof(C) == 8) {
-    // 64-bit limbs: z_i = y^(2^i) is needed mod 2^64 only at the end, and
-    // u = v mod 2^s gives u^2 = v^2 mod 2^(s+1); so the first squarings run
-    // mod 2^32 (z_31 exact mod 2^32) and the last 32, from z_31's low words,
-    // leave z_63 exact mod 2^64
+    // 64-bit limbs: the first squarings run mod 2^32 (z_31 exact mod 2^32)
+    // and the last 32, from z_31's low words, leave z_63 exact mod 2^64
     __shared__ __attribute__((aligned(16))) uint32_t lx[E], lx2[E];
     __shared__ uint32_t lred[2][32][4];
     const int lo = squarings > 32 ? squarings - 32 : 0;
@@ -1397,7 +1427,9 @@ __device__ __forceinline__ void teich_lift_block(const LT<T>* src, LT<T>* el, co
         lx2[tid] = (uint32_t)sx2[tid];
       }
       __syncthreads();
-      square_times<uint32_t>(lx, lx2, lred, lo);
+      const int a = n24(lo, 32);
+      square_times<uint32_t, true>(lx, lx2, lred, a);
+      square_times<uint32_t>(lx, lx2, lred, lo - a);
       if (tid < E) {
         // sx2 = 2 sx in 64 bits (the 32-bit double dropped its carry)
         sx[tid] = (C)lx[tid];
@@ -1407,7 +1439,9 @@ __device__ __forceinline__ void teich_lift_block(const LT<T>* src, LT<T>* el, co
     }
     square_times<C>(sx, sx2, sred, squarings - lo);
   } else {
-    square_times<C>(sx, sx2, sred, squarings);
+    const int a = n24(squarings, T::K);
+    square_times<C, true>(sx, sx2, sred, a);
+    square_times<C>(sx, sx2, sred, squarings - a);
   }
   if (tid < E) el[tid] = (LT<T>)sx[tid];
 }
@@ -1674,86 +1708,209 @@ __global__ void __launch_bounds__(GB) k_rot_mbar(const LT<T>* e0, const LT<T>* r
 //   scan    each block scans its segment (read once into registers) from its
 //           segment's offset
 // Encode = pass(msg -> t, perm1, d1, q) then pass(t -> z, perm2, d2, 1).
-// A segment is 32 entries per thread: RAA_TB / W threads share a limb.
+// Each thread moves V limbs of an entry with one access of up to 16 bytes
+// (TPE threads per entry), SUB entries lie side by side and a thread takes
+// PER consecutive entries: a block's segment is SEG = SUB * PER entries.
 // ============================================================================
 constexpr int RAA_TB = 256;  // threads per block
-template <int W> constexpr int raa_seg() { return 32 * (RAA_TB / W); }
+template <class T, int W> struct Raa {
+  static constexpr int LB = (int)sizeof(LT<T>);
+  static constexpr int V = (16 / LB) < (W / 8) ? 16 / LB : W / 8;  // limbs per access
+  static constexpr int TPE = W / V;                                 // threads per entry
+  static constexpr int SUB = RAA_TB / TPE;                          // entries side by side
+  static constexpr int PB = 128 / (V * (int)sizeof(CT<T>));
+  static constexpr int PER = PB > 4 ? PB : 4;  // entries per thread (about 32 registers of values)
+  static constexpr int SEG = SUB * PER;                             // entries per block
+};
+template <class L, int V> struct alignas(V * sizeof(L)) LimbVec {
+  L v[V];
+};
+// V limbs at p (one aligned access) widened to C, and back (truncated to L)
+template <class L, class C, int V> __device__ __forceinline__ void ld_limbs(const L* p, C out[V]) {
+  const LimbVec<L, V> x = *reinterpret_cast<const LimbVec<L, V>*>(p);
+#pragma unroll
+  for (int u = 0; u < V; u++) out[u] = (C)x.v[u];
+}
+template <class L, class C, int V> __device__ __forceinline__ void st_limbs(L* p, const C in[V]) {
+  LimbVec<L, V> x;
+#pragma unroll
+  for (int u = 0; u < V; u++) x.v[u] = (L)in[u];
+  *reinterpret_cast<LimbVec<L, V>*>(p) = x;
+}
 
 template <class T, int W>
 __global__ void __launch_bounds__(RAA_TB) k_raa_gather(const LT<T>* src, uint64_t src_row, const uint32_t* perm,
                                                        const LT<T>* d, int qs, uint64_t n, LT<T>* dst,
                                                        CT<T>* segsum, uint64_t nseg) {
-  constexpr int SUB = RAA_TB / W;
-  constexpr int PER = 32;
-  __shared__ CT<T> red[SUB][W];
-  uint64_t row = blockIdx.y, seg = blockIdx.x;
-  int s = threadIdx.x % W, sub = threadIdx.x / W;
-  const LT<T>* sr = src + row * src_row * W;
-  LT<T>* dr = dst + row * n * W;
-  uint64_t b = seg * raa_seg<W>() + (uint64_t)sub * PER;
-  CT<T> acc = 0;
-#pragma unroll 8
-  for (int k = 0; k < PER; k++) {
-    uint64_t i = b + k;
-    if (i < n) {
-      CT<T> v = (CT<T>)sr[(uint64_t)(perm[i] >> qs) * W + s] * (CT<T>)d[i];
-      dr[i * W + s] = (LT<T>)v;
-      acc += v;
+  using R = Raa<T, W>;
+  using L = LT<T>;
+  using C = CT<T>;
+  constexpr int V = R::V, PER = R::PER;
+  __shared__ C red[R::SUB][W];
+  const uint64_t row = blockIdx.y, seg = blockIdx.x;
+  const int tq = threadIdx.x % R::TPE, sub = threadIdx.x / R::TPE;
+  const L* sr = src + row * src_row * W + tq * V;
+  L* dr = dst + row * n * W + tq * V;
+  const uint64_t b = seg * R::SEG + (uint64_t)sub * PER;
+  C acc[V];
+#pragma unroll
+  for (int u = 0; u < V; u++) acc[u] = 0;
+  if (b + PER <= n) {
+    // all PER indices, scalars and source entries in flight at once
+    uint32_t pi[PER];
+    C dv[PER], x[PER][V];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      pi[k] = perm[b + k] >> qs;
+      dv[k] = (C)d[b + k];
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) ld_limbs<L, C, V>(sr + (uint64_t)pi[k] * W, x[k]);
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+#pragma unroll
+      for (int u = 0; u < V; u++) {
+        x[k][u] *= dv[k];
+        acc[u] += x[k][u];
+      }
+      st_limbs<L, C, V>(dr + (b + k) * W, x[k]);
+    }
+  } else {
+    for (int k = 0; k < PER && b + k < n; k++) {
+      C x[V];
+      ld_limbs<L, C, V>(sr + (uint64_t)(perm[b + k] >> qs) * W, x);
+      const C dv = (C)d[b + k];
+#pragma unroll
+      for (int u = 0; u < V; u++) {
+        x[u] *= dv;
+        acc[u] += x[u];
+      }
+      st_limbs<L, C, V>(dr + (b + k) * W, x);
     }
   }
-  red[sub][s] = acc;
+#pragma unroll
+  for (int u = 0; u < V; u++) red[sub][tq * V + u] = acc[u];
   __syncthreads();
-  if (sub == 0) {
-    for (int k = 1; k < SUB; k++) acc += red[k][s];
-    segsum[(row * nseg + seg) * W + s] = acc;
+  for (int s = threadIdx.x; s < W; s += RAA_TB) {
+    C t = 0;
+#pragma unroll 8
+    for (int k = 0; k < R::SUB; k++) t += red[k][s];
+    segsum[(row * nseg + seg) * W + s] = t;
   }
 }
-// Exclusive scan of one row's segment sums (grid: rows): each of the
-// 1024 / W threads of a limb takes a contiguous range of segments.
+// Exclusive scan of one row's segment sums (grid: rows): thread (sub, q)
+// takes limbs [q * LPT, (q + 1) * LPT) of a contiguous range of segments,
+// eight segments' loads in flight at a time; the 32 ranges' totals are
+// scanned in LDS.
 template <class T, int W>
 __global__ void __launch_bounds__(1024) k_raa_segscan(CT<T>* segsum, uint64_t nseg) {
-  constexpr int SUBS = 1024 / W;
-  __shared__ CT<T> part[SUBS][W];
-  const int s = threadIdx.x % W, sub = threadIdx.x / W;
+  using C = CT<T>;
+  constexpr int LPT = W / 32, SUBS = 32;
+  __shared__ C part[SUBS][W];
+  const int q = threadIdx.x % 32, sub = threadIdx.x / 32;
   const uint64_t per = (nseg + SUBS - 1) / SUBS, g0 = (uint64_t)sub * per;
   const uint64_t g1 = g0 + per < nseg ? g0 + per : nseg;
-  CT<T>* base = segsum + (uint64_t)blockIdx.x * nseg * W;
-  CT<T> tot = 0;
-  for (uint64_t g = g0; g < g1; g++) tot += base[g * W + s];
-  part[sub][s] = tot;
+  C* base = segsum + (uint64_t)blockIdx.x * nseg * W + q * LPT;
+  C tot[LPT];
+#pragma unroll
+  for (int u = 0; u < LPT; u++) tot[u] = 0;
+  for (uint64_t g = g0; g < g1; g += 8) {
+    C x[8][LPT];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (g + k < g1) {
+        ld_limbs<C, C, LPT>(base + (g + k) * W, x[k]);
+      } else {
+#pragma unroll
+        for (int u = 0; u < LPT; u++) x[k][u] = 0;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+#pragma unroll
+      for (int u = 0; u < LPT; u++) tot[u] += x[k][u];
+  }
+#pragma unroll
+  for (int u = 0; u < LPT; u++) part[sub][q * LPT + u] = tot[u];
   __syncthreads();
-  CT<T> run = 0;
-  for (int k = 0; k < sub; k++) run += part[k][s];
-  for (uint64_t g = g0; g < g1; g++) {
-    const CT<T> v = base[g * W + s];
-    base[g * W + s] = run;
-    run += v;
+  if (threadIdx.x < W) {
+    C run = 0;
+    for (int k = 0; k < SUBS; k++) {
+      const C t = part[k][threadIdx.x];
+      part[k][threadIdx.x] = run;
+      run += t;
+    }
+  }
+  __syncthreads();
+  C run[LPT];
+#pragma unroll
+  for (int u = 0; u < LPT; u++) run[u] = part[sub][q * LPT + u];
+  for (uint64_t g = g0; g < g1; g += 8) {
+    C x[8][LPT];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (g + k < g1) {
+        ld_limbs<C, C, LPT>(base + (g + k) * W, x[k]);
+      } else {
+#pragma unroll
+        for (int u = 0; u < LPT; u++) x[k][u] = 0;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (g + k < g1) st_limbs<C, C, LPT>(base + (g + k) * W, run);
+#pragma unroll
+      for (int u = 0; u < LPT; u++) run[u] += x[k][u];
+    }
   }
 }
 template <class T, int W>
 __global__ void __launch_bounds__(RAA_TB) k_raa_scan(LT<T>* buf, uint64_t n, const CT<T>* segsum, uint64_t nseg) {
-  constexpr int SUB = RAA_TB / W;
-  constexpr int PER = 32;
-  __shared__ CT<T> red[SUB][W];
-  uint64_t row = blockIdx.y, seg = blockIdx.x;
-  int s = threadIdx.x % W, sub = threadIdx.x / W;
-  LT<T>* br = buf + row * n * W;
-  uint64_t b = seg * raa_seg<W>() + (uint64_t)sub * PER;
-  CT<T> v[PER];
-  CT<T> tot = 0;
+  using R = Raa<T, W>;
+  using L = LT<T>;
+  using C = CT<T>;
+  constexpr int V = R::V, PER = R::PER;
+  __shared__ C red[R::SUB][W];
+  const uint64_t row = blockIdx.y, seg = blockIdx.x;
+  const int tq = threadIdx.x % R::TPE, sub = threadIdx.x / R::TPE;
+  L* br = buf + row * n * W + tq * V;
+  const uint64_t b = seg * R::SEG + (uint64_t)sub * PER;
+  C v[PER][V], tot[V];
+#pragma unroll
+  for (int u = 0; u < V; u++) tot[u] = 0;
 #pragma unroll
   for (int k = 0; k < PER; k++) {
-    v[k] = b + k < n ? (CT<T>)br[(b + k) * W + s] : (CT<T>)0;
-    tot += v[k];
+    if (b + k < n) {
+      ld_limbs<L, C, V>(br + (b + k) * W, v[k]);
+    } else {
+#pragma unroll
+      for (int u = 0; u < V; u++) v[k][u] = 0;
+    }
+#pragma unroll
+    for (int u = 0; u < V; u++) tot[u] += v[k][u];
   }
-  red[sub][s] = tot;
+#pragma unroll
+  for (int u = 0; u < V; u++) red[sub][tq * V + u] = tot[u];
   __syncthreads();
-  CT<T> run = segsum[(row * nseg + seg) * W + s];
-  for (int k = 0; k < sub; k++) run += red[k][s];
+  // per limb: the segment's offset plus the totals of the entries before
+  // this thread's within the segment
+  for (int s = threadIdx.x; s < W; s += RAA_TB) {
+    C run = segsum[(row * nseg + seg) * W + s];
+    for (int k = 0; k < R::SUB; k++) {
+      const C t = red[k][s];
+      red[k][s] = run;
+      run += t;
+    }
+  }
+  __syncthreads();
+  C run[V];
+#pragma unroll
+  for (int u = 0; u < V; u++) run[u] = red[sub][tq * V + u];
 #pragma unroll
   for (int k = 0; k < PER; k++) {
-    run += v[k];
-    if (b + k < n) br[(b + k) * W + s] = (LT<T>)run;
+#pragma unroll
+    for (int u = 0; u < V; u++) run[u] += v[k][u];
+    if (b + k < n) st_limbs<L, C, V>(br + (b + k) * W, run);
   }
 }
 

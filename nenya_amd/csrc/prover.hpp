@@ -1345,7 +1345,7 @@ template <class T> struct Engine {
     const PcsParams& P = pc.P;
     if (P.q & (P.q - 1)) throw Error(NENYA_E_SHAPE, "RAA repetition must be a power of two");
     const int qs = __builtin_ctzll((uint64_t)P.q);
-    uint64_t nseg = (P.n + raa_seg<W>() - 1) / raa_seg<W>();
+    uint64_t nseg = (P.n + Raa<T, W>::SEG - 1) / Raa<T, W>::SEG;
     DBuf<C> seg(pool, rows * nseg * W);
     dim3 grid((unsigned)nseg, (unsigned)rows);
     double eb = W * sizeof(L), rowb = (double)rows * P.n * eb;
