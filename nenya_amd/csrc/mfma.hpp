@@ -41,20 +41,34 @@ template <int ND> __device__ __forceinline__ uint32_t bal_digits(uint32_t v) {
   else return (v + 0x80808080u) ^ 0x80808080u;
 }
 
+// 4 x 4 byte transpose by byte permutes: w[p] holds byte p of z0, z1, z2,
+// z3 (in that order), for p < NB.
+template <int NB>
+__device__ __forceinline__ void bytes_t4(uint32_t z0, uint32_t z1, uint32_t z2, uint32_t z3, uint32_t w[NB]) {
+  // perm(hi, lo, sel): byte i of the result is byte sel_i of hi:lo (lo = 0..3)
+  const uint32_t t0 = __builtin_amdgcn_perm(z1, z0, 0x05010400u);  // z0.0 z1.0 z0.1 z1.1
+  const uint32_t t1 = __builtin_amdgcn_perm(z3, z2, 0x05010400u);  // z2.0 z3.0 z2.1 z3.1
+  w[0] = __builtin_amdgcn_perm(t1, t0, 0x05040100u);
+  if constexpr (NB > 1) w[1] = __builtin_amdgcn_perm(t1, t0, 0x07060302u);
+  if constexpr (NB > 2) {
+    const uint32_t t2 = __builtin_amdgcn_perm(z1, z0, 0x07030602u);  // z0.2 z1.2 z0.3 z1.3
+    const uint32_t t3 = __builtin_amdgcn_perm(z3, z2, 0x07030602u);
+    w[2] = __builtin_amdgcn_perm(t3, t2, 0x05040100u);
+    if constexpr (NB > 3) w[3] = __builtin_amdgcn_perm(t3, t2, 0x07060302u);
+  }
+}
+
 // 16 values (k = 16 consecutive limbs) -> ND digit planes of 16 int8 each.
 template <int ND> __device__ __forceinline__ void planes16(const uint32_t v[16], v4i out[ND]) {
   uint32_t z[16];
 #pragma unroll
   for (int b = 0; b < 16; b++) z[b] = bal_digits<ND>(v[b]);
 #pragma unroll
-  for (int p = 0; p < ND; p++) {
+  for (int w = 0; w < 4; w++) {
+    uint32_t x[ND];
+    bytes_t4<ND>(z[4 * w], z[4 * w + 1], z[4 * w + 2], z[4 * w + 3], x);
 #pragma unroll
-    for (int w = 0; w < 4; w++) {
-      uint32_t x = 0;
-#pragma unroll
-      for (int j = 0; j < 4; j++) x |= ((z[4 * w + j] >> (8 * p)) & 0xffu) << (8 * j);
-      out[p][w] = (int)x;
-    }
+    for (int p = 0; p < ND; p++) out[p][w] = (int)x[p];
   }
 }
 
@@ -733,13 +747,15 @@ __device__ __forceinline__ void planes16_64(const uint64_t v[16], v4i out[8]) {
 #pragma unroll
   for (int b = 0; b < 16; b++) z[b] = bal_digits64(v[b]);
 #pragma unroll
-  for (int p = 0; p < 8; p++) {
+  for (int w = 0; w < 4; w++) {
+    uint32_t lo[4], hi[4];
+    bytes_t4<4>((uint32_t)z[4 * w], (uint32_t)z[4 * w + 1], (uint32_t)z[4 * w + 2], (uint32_t)z[4 * w + 3], lo);
+    bytes_t4<4>((uint32_t)(z[4 * w] >> 32), (uint32_t)(z[4 * w + 1] >> 32), (uint32_t)(z[4 * w + 2] >> 32),
+                (uint32_t)(z[4 * w + 3] >> 32), hi);
 #pragma unroll
-    for (int w = 0; w < 4; w++) {
-      uint32_t x = 0;
-#pragma unroll
-      for (int j = 0; j < 4; j++) x |= (uint32_t)((z[4 * w + j] >> (8 * p)) & 0xffu) << (8 * j);
-      out[p][w] = (int)x;
+    for (int p = 0; p < 4; p++) {
+      out[p][w] = (int)lo[p];
+      out[4 + p][w] = (int)hi[p];
     }
   }
 }
@@ -1243,14 +1259,17 @@ __device__ __forceinline__ void gram_lin64_body(const uint64_t* f, uint64_t fstr
         z[3][c] = bal_digits64(g1 - g0);
       }
 #pragma unroll
-      for (int mt = 0; mt < 4; mt++)
+      for (int mt = 0; mt < 4; mt++) {
+        uint32_t lo[4], hi[4];
+        bytes_t4<4>((uint32_t)z[mt][0], (uint32_t)z[mt][1], (uint32_t)z[mt][2], (uint32_t)z[mt][3], lo);
+        bytes_t4<4>((uint32_t)(z[mt][0] >> 32), (uint32_t)(z[mt][1] >> 32), (uint32_t)(z[mt][2] >> 32),
+                    (uint32_t)(z[mt][3] >> 32), hi);
 #pragma unroll
-        for (int p = 0; p < 8; p++) {
-          uint32_t wv = 0;
-#pragma unroll
-          for (int c = 0; c < 4; c++) wv |= (uint32_t)((z[mt][c] >> (8 * p)) & 0xffu) << (8 * c);
-          sF[mt][p][rb][lane][wd] = wv;
+        for (int p = 0; p < 4; p++) {
+          sF[mt][p][rb][lane][wd] = lo[p];
+          sF[mt][4 + p][rb][lane][wd] = hi[p];
         }
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -1480,6 +1499,56 @@ __global__ void k_sum_split(const LT<T>* part, uint64_t nsplit, uint64_t cnt, LT
 // meet at beta^(i+i') alpha^(c+c').
 // ---------------------------------------------------------------------------
 
+// The end of a linear-check Gram block: the wave accumulators (NACC
+// (gram, bx, by) blocks per wave) are contracted into 128 limbs per Gram
+// matrix (bins raw[i+i'][c+c'] by LDS atomics, alpha^4 = -alpha - 1, then
+// Y^32 = -Y^3 - Y - alpha) and written as the block's partials (or added
+// atomically into sums).  raw must be zero on entry.
+template <class T, int NACC>
+__device__ __forceinline__ void gram_finish(const v16i* tot, const bool* pon, const int* pg, const int* pbx,
+                                            const int* pby, uint32_t (*raw)[63][8], uint32_t (*q)[63][4], uint64_t hoff,
+                                            LT<T>* partials, int pstride, int poff, CT<T>* sums, unsigned blk) {
+  using L = LT<T>;
+  const int tid = threadIdx.x, l = tid & 63;
+  __syncthreads();
+  // accumulator entry i of lane l in block (bx, by): row 32bx + 8(i>>2) +
+  // 4(l>>5) + (i&3), column 32by + (l&31)
+#pragma unroll
+  for (int u = 0; u < NACC; u++) {
+    if (!pon[u]) continue;
+    const int mp0 = 8 * (pbx[u] + pby[u]) + (l >> 5) + ((l & 31) >> 2), cp = l & 3;
+#pragma unroll
+    for (int i = 0; i < 16; i++) atomicAdd(&raw[pg[u]][mp0 + 2 * (i >> 2)][(i & 3) + cp], (uint32_t)tot[u][i]);
+  }
+  __syncthreads();
+  if (tid < 2 * 63) {
+    const int g = tid / 63, p = tid % 63;
+    const uint32_t* r = raw[g][p];
+    q[g][p][0] = r[0] - r[4];
+    q[g][p][1] = r[1] - r[4] - r[5];
+    q[g][p][2] = r[2] - r[5] - r[6];
+    q[g][p][3] = r[3] - r[6];
+  }
+  __syncthreads();
+  if (tid < 2 * E) {
+    const int g = tid >> 7, u = (tid & 127) >> 2, c = tid & 3;
+    // position 32 + s after positions 61 and 62 have folded into 32 and 33
+    auto hc = [&](int s, int cc) -> uint32_t { return q[g][32 + s][cc] - (s < 2 ? q[g][61 + s][cc] : 0u); };
+    uint32_t v = q[g][u][c];
+    if (u >= 1) v -= hc(u - 1, c);  // -Y from position u + 31
+    if (u >= 3) v -= hc(u - 3, c);  // -Y^3 from position u + 29
+    if (u <= 30) {                  // -alpha from position u + 32
+      if (c == 0) v += hc(u, 3);
+      else if (c == 1) v += hc(u, 3) - hc(u, 0);
+      else v -= hc(u, c - 1);
+    }
+    if (g == 0 || hoff) {
+      if (sums) atom_add(sums + (size_t)(poff + g) * E + (tid & 127), v);
+      else partials[((size_t)blk * pstride + poff + g) * E + (tid & 127)] = (L)v;
+    }
+  }
+}
+
 // Linear-check round (as k_lin_round): gram 0 over (f_lo, g_lo), gram 1
 // over (f_hi - f_lo, g_hi - g_lo), both from one read of the tables (hi at
 // hoff elements past lo; hoff = 0 makes gram 1 zero and gram 0 a plain sum
@@ -1531,12 +1600,24 @@ __device__ __forceinline__ void gram_lin_body(const LT<T>* f, uint64_t fstride, 
   // MFMA work of chunk ch, so HBM latency hides behind the matrix cores
   uint32_t rf[4][2], rg[4][2];  // [element][lo, hi]
   auto load = [&](uint64_t ch) {
+    // one division per chunk (block-uniform): with m >= 32 a chunk spans at
+    // most two tables
+    const uint64_t base = ch * 32, tab0 = base / m, i0 = base - tab0 * m;
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      const uint64_t e = ch * 32 + 4 * eg + c;
+      const uint64_t e = base + 4 * eg + c;
       rf[c][0] = rf[c][1] = rg[c][0] = rg[c][1] = 0;
       if (ch < nchunks && e < total) {
-        const uint64_t tab = e / m, i = e % m;
+        uint64_t tab, i;
+        if (m >= 32) {
+          const uint64_t ii = i0 + 4 * eg + c;
+          const bool wr = ii >= m;
+          tab = tab0 + (wr ? 1 : 0);
+          i = wr ? ii - m : ii;
+        } else {
+          tab = e / m;
+          i = e % m;
+        }
         const L* F = f + (tab * fstride + i) * E;
         const L* G = gt + (tab * gstride + i) * E;
         rf[c][0] = (uint32_t)F[j];
@@ -1557,72 +1638,190 @@ __device__ __forceinline__ void gram_lin_body(const LT<T>* f, uint64_t fstride, 
       z[3][c] = bal_digits<ND>(rg[c][1] - rg[c][0]);
     }
 #pragma unroll
-    for (int mt = 0; mt < 4; mt++)
+    for (int mt = 0; mt < 4; mt++) {
+      uint32_t wv[ND];
+      bytes_t4<ND>(z[mt][0], z[mt][1], z[mt][2], z[mt][3], wv);
 #pragma unroll
-      for (int p = 0; p < ND; p++) {
-        uint32_t wv = 0;
-#pragma unroll
-        for (int c = 0; c < 4; c++) wv |= ((z[mt][c] >> (8 * p)) & 0xffu) << (8 * c);
-        sF[mt][p][rb][lane][wd] = wv;
-      }
+      for (int p = 0; p < ND; p++) sF[mt][p][rb][lane][wd] = wv[p];
+    }
     __syncthreads();
     load(ch + nblk);
 #pragma unroll
     for (int u = 0; u < 2; u++) {
       if (!pon[u]) continue;
       const int xm = 2 * pg[u], ym = 2 * pg[u] + 1;
+      v4i a[ND], b[ND];
+#pragma unroll
+      for (int p = 0; p < ND; p++) {
+        a[p] = *reinterpret_cast<const v4i*>(sF[xm][p][pbx[u]][l]);
+        b[p] = *reinterpret_cast<const v4i*>(sF[ym][p][pby[u]][l]);
+      }
       v16i tmp = (v16i)0;
 #pragma unroll
       for (int sd = ND - 1; sd >= 0; sd--) {
         if (sd != ND - 1) tmp <<= 8;
 #pragma unroll
-        for (int pd = 0; pd <= sd; pd++) {
-          const v4i a = *reinterpret_cast<const v4i*>(sF[xm][pd][pbx[u]][l]);
-          const v4i b = *reinterpret_cast<const v4i*>(sF[ym][sd - pd][pby[u]][l]);
-          tmp = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, tmp, 0, 0, 0);
-        }
+        for (int pd = 0; pd <= sd; pd++) tmp = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[pd], b[sd - pd], tmp, 0, 0, 0);
       }
       tot[u] += tmp;
     }
     __syncthreads();
   }
-  __syncthreads();
-  // accumulator entry i of lane l in block (bx, by): row 32bx + 8(i>>2) +
-  // 4(l>>5) + (i&3), column 32by + (l&31)
+  gram_finish<T, 2>(tot, pon, pg, pbx, pby, raw, q, hoff, partials, pstride, poff, sums, blk);
+}
+// The same sums with 512 threads, two chunks of loads in flight and two LDS
+// buffers: thread (j, eg2) loads limb j of elements 8eg2..8eg2+7 of a chunk
+// (f and g at lo and hi: 32 values), so the raw limbs of the next two
+// chunks are in registers (128 KB per CU); while the waves' MFMAs read one
+// chunk's digit planes from one LDS buffer, its threads write the next
+// chunk's into the other (one barrier per chunk).  Wave w accumulates the
+// (gram, bx, by) pairs w, w+8, w+16, w+24.  Chunks of block b: b, b + nblk,
+// ...
+template <class T>
+__global__ void __launch_bounds__(512) k_gram_lin2(const LT<T>* f, uint64_t fstride, const LT<T>* gt, uint64_t gstride,
+                                                   uint64_t ntab, uint64_t m, uint64_t hoff, int xrows, LT<T>* partials,
+                                                   int pstride, int poff) {
+  using L = LT<T>;
+  constexpr int ND = kND<T>;
+  constexpr int BUFW = 4 * ND * 4 * 64 * 4;  // words of one chunk's planes
+  __shared__ __attribute__((aligned(16))) uint32_t lds[2 * BUFW];
+  __shared__ uint32_t raw[2][63][8], q[2][63][4];
+  const int tid = threadIdx.x, j = tid & 127, eg2 = tid >> 7, rb = j >> 5;
+  const int w = tid >> 6, l = tid & 63;
+  const int xb = xrows / 32, npairs = (hoff ? 2 : 1) * xb * 4;
+  int pg[4], pbx[4], pby[4];
+  bool pon[4];
 #pragma unroll
-  for (int u = 0; u < 2; u++) {
-    if (!pon[u]) continue;
-    const int mp0 = 8 * (pbx[u] + pby[u]) + (l >> 5) + ((l & 31) >> 2), cp = l & 3;
+  for (int u = 0; u < 4; u++) {
+    const int pr = w + 8 * u;
+    pon[u] = pr < npairs;
+    pg[u] = pr / (xb * 4);
+    pbx[u] = (pr % (xb * 4)) >> 2;
+    pby[u] = pr & 3;
+  }
+  const uint64_t total = m * ntab, nchunks = (total + 31) / 32, nblk = gridDim.x;
+  for (int i = tid; i < 2 * 63 * 8; i += 512) (&raw[0][0][0])[i] = 0;
+  v16i tot[4];
 #pragma unroll
-    for (int i = 0; i < 16; i++) atomicAdd(&raw[pg[u]][mp0 + 2 * (i >> 2)][(i & 3) + cp], (uint32_t)tot[u][i]);
-  }
-  __syncthreads();
-  if (tid < 2 * 63) {
-    const int g = tid / 63, p = tid % 63;
-    const uint32_t* r = raw[g][p];
-    q[g][p][0] = r[0] - r[4];
-    q[g][p][1] = r[1] - r[4] - r[5];
-    q[g][p][2] = r[2] - r[5] - r[6];
-    q[g][p][3] = r[3] - r[6];
-  }
-  __syncthreads();
-  if (tid < 2 * E) {
-    const int g = tid >> 7, u = (tid & 127) >> 2, c = tid & 3;
-    // position 32 + s after positions 61 and 62 have folded into 32 and 33
-    auto hc = [&](int s, int cc) -> uint32_t { return q[g][32 + s][cc] - (s < 2 ? q[g][61 + s][cc] : 0u); };
-    uint32_t v = q[g][u][c];
-    if (u >= 1) v -= hc(u - 1, c);  // -Y from position u + 31
-    if (u >= 3) v -= hc(u - 3, c);  // -Y^3 from position u + 29
-    if (u <= 30) {                  // -alpha from position u + 32
-      if (c == 0) v += hc(u, 3);
-      else if (c == 1) v += hc(u, 3) - hc(u, 0);
-      else v -= hc(u, c - 1);
+  for (int u = 0; u < 4; u++) tot[u] = (v16i)0;
+  uint32_t R0[8][4], R1[8][4];  // [element][f lo, f hi, g lo, g hi]
+  // Loads are unconditional, so they stay in flight across the stage and
+  // MFMA work.  With m a multiple of 32 a whole chunk lies in one table and
+  // its elements are consecutive: one base pointer per array (uniform table
+  // and index), constant offsets per element.  Otherwise (short tables, the
+  // tail chunk) each element finds its table and index, elements past the
+  // end read element 0 and are zeroed.
+  const bool whole = m % 32 == 0 && total % 32 == 0;
+  auto load = [&](uint64_t ch, uint32_t (*R)[4]) {
+    const uint64_t base = ch * 32;
+    if (whole && ch < nchunks) {
+      const uint64_t tab = base / m, i0 = base - tab * m + 8 * eg2;
+      const L* pf = f + (tab * fstride + i0) * E + j;
+      const L* pg = gt + (tab * gstride + i0) * E + j;
+      const L* pfh = pf + hoff * E;
+      const L* pgh = pg + hoff * E;
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        R[c][0] = (uint32_t)pf[c * E];
+        R[c][1] = (uint32_t)pfh[c * E];
+        R[c][2] = (uint32_t)pg[c * E];
+        R[c][3] = (uint32_t)pgh[c * E];
+      }
+    } else {
+      uint64_t fo[8], go[8];
+      bool ok[8];
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        const uint64_t e = base + 8 * eg2 + c, tab = e / m, i = e % m;
+        ok[c] = ch < nchunks && e < total;
+        fo[c] = ok[c] ? (tab * fstride + i) * E : 0;
+        go[c] = ok[c] ? (tab * gstride + i) * E : 0;
+      }
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        const uint32_t x0 = (uint32_t)f[fo[c] + j], x1 = (uint32_t)f[fo[c] + hoff * E + j];
+        const uint32_t y0 = (uint32_t)gt[go[c] + j], y1 = (uint32_t)gt[go[c] + hoff * E + j];
+        R[c][0] = ok[c] ? x0 : 0u;
+        R[c][1] = ok[c] ? x1 : 0u;
+        R[c][2] = ok[c] ? y0 : 0u;
+        R[c][3] = ok[c] ? y1 : 0u;
+      }
     }
-    if (g == 0 || hoff) {
-      if (sums) atom_add(sums + (size_t)(poff + g) * E + (tid & 127), v);
-      else partials[((size_t)blk * pstride + poff + g) * E + (tid & 127)] = (L)v;
+  };
+  // digit planes of a chunk into buffer sF ([x0|y0|x1|y1][plane][rb][lane][word]):
+  // elements 4k..4k+3 of this thread's eight are k = 8 eg2 + 4k + c of the
+  // operands (lane (j%32) + 32k, word eg2)
+  auto stage = [&](const uint32_t (*R)[4], uint32_t* buf) {
+    uint32_t (*sF)[ND][4][64][4] = reinterpret_cast<uint32_t (*)[ND][4][64][4]>(buf);
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      uint32_t z[4][4];  // [x0, y0, x1, y1][element]
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const uint32_t* r = R[4 * k + c];
+        z[0][c] = bal_digits<ND>(r[0]);
+        z[1][c] = bal_digits<ND>(r[2]);
+        z[2][c] = bal_digits<ND>(r[1] - r[0]);
+        z[3][c] = bal_digits<ND>(r[3] - r[2]);
+      }
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
+        uint32_t wv[ND];
+        bytes_t4<ND>(z[mt][0], z[mt][1], z[mt][2], z[mt][3], wv);
+#pragma unroll
+        for (int p = 0; p < ND; p++) sF[mt][p][rb][(j & 31) + 32 * k][eg2] = wv[p];
+      }
     }
+  };
+  // each operand plane is read from LDS once per pair (2 ND reads for
+  // ND (ND + 1) / 2 MFMAs)
+  auto grams = [&](const uint32_t* buf) {
+    const uint32_t (*sF)[ND][4][64][4] = reinterpret_cast<const uint32_t (*)[ND][4][64][4]>(buf);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (!pon[u]) continue;
+      const int xm = 2 * pg[u], ym = 2 * pg[u] + 1;
+      v4i a[ND], b[ND];
+#pragma unroll
+      for (int p = 0; p < ND; p++) {
+        a[p] = *reinterpret_cast<const v4i*>(sF[xm][p][pbx[u]][l]);
+        b[p] = *reinterpret_cast<const v4i*>(sF[ym][p][pby[u]][l]);
+      }
+      v16i tmp = (v16i)0;
+#pragma unroll
+      for (int sd = ND - 1; sd >= 0; sd--) {
+        if (sd != ND - 1) tmp <<= 8;
+#pragma unroll
+        for (int pd = 0; pd <= sd; pd++) tmp = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[pd], b[sd - pd], tmp, 0, 0, 0);
+      }
+      tot[u] += tmp;
+    }
+  };
+  uint64_t ch = blockIdx.x;
+  int cur = 0;
+  load(ch, R0);
+  load(ch + nblk, R1);
+  stage(R0, lds);
+  load(ch + 2 * nblk, R0);
+  __syncthreads();
+  // chunk ch's planes in buffer cur; the register set that holds chunk
+  // ch + nblk is staged into the other buffer and reloaded with ch + 3 nblk
+  while (ch < nchunks) {
+    grams(lds + cur * BUFW);
+    stage(R1, lds + (cur ^ 1) * BUFW);
+    load(ch + 3 * nblk, R1);
+    __syncthreads();
+    ch += nblk;
+    cur ^= 1;
+    if (ch >= nchunks) break;
+    grams(lds + cur * BUFW);
+    stage(R0, lds + (cur ^ 1) * BUFW);
+    load(ch + 3 * nblk, R0);
+    __syncthreads();
+    ch += nblk;
+    cur ^= 1;
   }
+  gram_finish<T, 4>(tot, pon, pg, pbx, pby, raw, q, hoff, partials, pstride, poff, nullptr, blockIdx.x);
 }
 template <class T>
 __global__ void __launch_bounds__(1024) k_gram_lin(const LT<T>* f, uint64_t fstride, const LT<T>* gt, uint64_t gstride,

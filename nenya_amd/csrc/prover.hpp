@@ -752,7 +752,7 @@ template <class T> struct Engine {
     const uint64_t nch = (m * ntab + 31) / 32;
     const int tb = Tuning::get().gram_blocks[Tuning::kidx(T::K)][Tuning::lg(nch)].load(std::memory_order_relaxed);
     const unsigned nwg = (unsigned)std::min<uint64_t>(nch, tb ? (unsigned)tb : GRAM_BLOCKS);
-    launch("k_gram_lin", m * ntab * 4.0 * EB, m * ntab * 2.0 * E * E, (k_gram_lin<T>), dim3(nwg), dim3(1024), f, fs, g,
+    launch("k_gram_lin", m * ntab * 4.0 * EB, m * ntab * 2.0 * E * E, (k_gram_lin2<T>), dim3(nwg), dim3(512), f, fs, g,
            gs, ntab, m, m, xrows, part, 2, 0);
     return nwg;
   }
@@ -790,7 +790,7 @@ template <class T> struct Engine {
                0);
       } else {
         nwg = (unsigned)std::min<uint64_t>((n + 31) / 32, GRAM_BLOCKS);
-        launch("k_dot_gram", n * 2.0 * EB, n * 2.0 * E * E, (k_gram_lin<T>), dim3(nwg), dim3(1024), A, (uint64_t)0, B,
+        launch("k_dot_gram", n * 2.0 * EB, n * 2.0 * E * E, (k_gram_lin2<T>), dim3(nwg), dim3(512), A, (uint64_t)0, B,
                (uint64_t)0, (uint64_t)1, n, (uint64_t)0, xrows, part.p, 1, 0);
       }
       return reduce(part, nwg, 1);
@@ -1659,7 +1659,7 @@ template <class T> struct Engine {
                    (const uint64_t*)eg.p, (uint64_t)0, (const uint64_t*)(prods.p + (size_t)s * m * E), (uint64_t)0,
                    (uint64_t)1, (uint64_t)m, (uint64_t)0, 128, (uint64_t*)part.p, 3, s);
           else
-            launch("k_mult_gram", m * 2.0 * EB, m * 1.0 * E * E, (k_gram_lin<T>), dim3(nwg), dim3(1024), (const L*)eg.p,
+            launch("k_mult_gram", m * 2.0 * EB, m * 1.0 * E * E, (k_gram_lin2<T>), dim3(nwg), dim3(512), (const L*)eg.p,
                    (uint64_t)0, (const L*)(prods.p + (size_t)s * m * E), (uint64_t)0, (uint64_t)1, (uint64_t)m,
                    (uint64_t)0, 128, part.p, 3, s);
         }

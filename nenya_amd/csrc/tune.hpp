@@ -93,7 +93,7 @@ template <class T> void tune_width(std::ostringstream& js, int lmin, int lmax) {
       const uint64_t m = uint64_t(1) << (lg - 1), nch = m / 32;  // 2^lg elements read per table pair half
       auto run = [&](unsigned nwg) {
         return time_launch(cx.st, [&] {
-          hipLaunchKernelGGL(k_gram_lin<T>, dim3(nwg), dim3(1024), 0, cx.st, (const L*)src.p, (uint64_t)0,
+          hipLaunchKernelGGL(k_gram_lin2<T>, dim3(nwg), dim3(512), 0, cx.st, (const L*)src.p, (uint64_t)0,
                              (const L*)src.p, (uint64_t)0, (uint64_t)1, m, m, 128, part.p, 2, 0);
         });
       };
