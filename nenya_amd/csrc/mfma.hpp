@@ -971,8 +971,9 @@ __global__ void __launch_bounds__(256) k_b2frag64(const uint64_t* b2, uint32_t* 
 // 2^27.  Block: 4 waves on 32 block rows, wave w output rows 32w..32w+31
 // (one row-block); per k-block 36 digit-pair MFMAs, C and the fragments of
 // k-block kb+1 loaded during kb's matrix work; then a 64-bit Horner over
-// the classes.
-__global__ void __launch_bounds__(256) k_cgemm64(const uint32_t* frag, const uint64_t* cmat, uint64_t* out,
+// the classes.  Two blocks per CU: a few registers spill, but the second wave
+// per SIMD hides more load latency than the spills cost.
+__global__ void __launch_bounds__(256, 2) k_cgemm64(const uint32_t* frag, const uint64_t* cmat, uint64_t* out,
                                                  uint64_t nb) {
   const v4i* A = reinterpret_cast<const v4i*>(frag);
   const int l = threadIdx.x & 63, col = l & 31, h = l >> 5, rb = threadIdx.x >> 6;
@@ -1116,8 +1117,9 @@ __global__ void __launch_bounds__(256) k_pack_mx64(const uint32_t* frag, const u
 // by a Horner step every 64 rows.  Block b takes split b % nsplit (rows
 // [s*rows/nsplit, (s+1)*rows/nsplit)) and writes out + s * nl * k2 elements;
 // workgroups go to the eight XCDs round-robin, so with nsplit a multiple of
-// 8 an XCD reads the fragments of one eighth of the rows.
-__global__ void __launch_bounds__(256) k_mbar_mx64(const uint32_t* frag, const uint64_t* X, uint64_t rows, uint64_t k2,
+// 8 an XCD reads the fragments of one eighth of the rows.  Two blocks per
+// CU (a few registers spill; the second wave per SIMD hides load latency).
+__global__ void __launch_bounds__(256, 2) k_mbar_mx64(const uint32_t* frag, const uint64_t* X, uint64_t rows, uint64_t k2,
                                                    uint64_t lstride, uint64_t nl, uint64_t nsplit, uint64_t* out) {
   __shared__ v4i sB[2][4 * 8 * 64];  // [buffer][k-block][plane][lane], 2 x 32 KB
   const v4i* A = reinterpret_cast<const v4i*>(frag);
