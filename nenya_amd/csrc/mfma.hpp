@@ -1234,9 +1234,45 @@ __device__ __forceinline__ void gram_lin64_body(const uint64_t* f, uint64_t fstr
 #pragma unroll
     for (int i = 0; i < 16; i++) tot[u][i] = 0;
   const int rb = j >> 5;
+  // with m a multiple of 32 a chunk lies in one table: one base pointer per
+  // array (uniform table and index), unconditional loads at constant offsets
+  const bool whole = m % 32 == 0 && total % 32 == 0;
   for (uint64_t ch = blk; ch < nchunks; ch += nblk) {
     // thread (j, eg0): limb j of the element groups eg = eg0 and eg0 + 4
     // (elements 4eg .. 4eg+3, k = kappa(eg & 1, 4(eg >> 1) + c))
+    uint64_t R[2][4][4];  // [half][element][f lo, f hi, g lo, g hi]
+    if (whole) {
+      const uint64_t tab = ch * 32 / m, i0 = ch * 32 - tab * m + 4 * eg0;
+      const uint64_t* pf = f + (tab * fstride + i0) * E + j;
+      const uint64_t* pg = gt + (tab * gstride + i0) * E + j;
+#pragma unroll
+      for (int half = 0; half < 2; half++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const uint64_t o = (uint64_t)(16 * half + c) * E;
+          R[half][c][0] = pf[o];
+          R[half][c][1] = pf[hoff * E + o];
+          R[half][c][2] = pg[o];
+          R[half][c][3] = pg[hoff * E + o];
+        }
+    } else {
+#pragma unroll
+      for (int half = 0; half < 2; half++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const uint64_t e = ch * 32 + 4 * (eg0 + 4 * half) + c;
+          R[half][c][0] = R[half][c][1] = R[half][c][2] = R[half][c][3] = 0;
+          if (e < total) {
+            const uint64_t tab = e / m, i = e % m;
+            const uint64_t* F = f + (tab * fstride + i) * E;
+            const uint64_t* G = gt + (tab * gstride + i) * E;
+            R[half][c][0] = F[j];
+            R[half][c][1] = F[hoff * E + j];
+            R[half][c][2] = G[j];
+            R[half][c][3] = G[hoff * E + j];
+          }
+        }
+    }
 #pragma unroll
     for (int half = 0; half < 2; half++) {
       const int eg = eg0 + 4 * half;
@@ -1244,17 +1280,7 @@ __device__ __forceinline__ void gram_lin64_body(const uint64_t* f, uint64_t fstr
       uint64_t z[4][4];  // [x0, y0, x1, y1][element]
 #pragma unroll
       for (int c = 0; c < 4; c++) {
-        const uint64_t e = ch * 32 + 4 * eg + c;
-        uint64_t f0 = 0, f1 = 0, g0 = 0, g1 = 0;
-        if (e < total) {
-          const uint64_t tab = e / m, i = e % m;
-          const uint64_t* F = f + (tab * fstride + i) * E;
-          const uint64_t* G = gt + (tab * gstride + i) * E;
-          f0 = F[j];
-          f1 = F[hoff * E + j];
-          g0 = G[j];
-          g1 = G[hoff * E + j];
-        }
+        const uint64_t f0 = R[half][c][0], f1 = R[half][c][1], g0 = R[half][c][2], g1 = R[half][c][3];
         z[0][c] = bal_digits64(f0);
         z[1][c] = bal_digits64(g0);
         z[2][c] = bal_digits64(f1 - f0);
