@@ -336,20 +336,47 @@ __device__ __forceinline__ void fold_tile(const v4i* A, const FoldJob<T>& J, uin
   }
 }
 
-// Sumcheck fold on the matrix cores (same result as k_fold):
+// Sumcheck fold on the matrix cores (same result as k_fold), split mode
+// (few tiles):
 //   dst[tab][i] = lo + r (hi - lo),  lo = src[tab][i], hi = src[tab][i+m].
-// Bulk mode: persistent blocks of 4 waves, a wave folds 32 elements per
-// tile, the M_r fragments (ND*16 KB) sit in LDS for the block's lifetime.
-// Split mode (few tiles): block b takes tile b and wave w its row-block w,
-// with its fragments loaded from global memory (L2-resident) up front; a
-// barrier between the loads and the stores keeps in-place folds exact.
-// Bulk mode needs none: a wave loads and stores only its own tile.
-// With a side task the last block runs it instead of folding.
+// Block b takes tile b and wave w its row-block w, with its fragments loaded
+// from global memory (L2-resident) up front; a barrier between the loads and
+// the stores keeps in-place folds exact.  With a side task the last block
+// runs it instead of folding.  Many tiles: k_fold_mx_bulk.
 template <class T>
-__global__ void __launch_bounds__(256) k_fold_mx(const uint32_t* frag, FoldJob<T> J, int split) {
-  constexpr int ND = kND<T>;
-  __shared__ v4i sA[ND * 16 * 64];
+__global__ void __launch_bounds__(256) k_fold_mx(const uint32_t* frag, FoldJob<T> J) {
   const v4i* gA = reinterpret_cast<const v4i*>(frag);
+  const unsigned nfold = gridDim.x - (J.side ? 1 : 0);
+  if (J.side && blockIdx.x == nfold) {
+    if (J.side == 1) eq_one_256<T>(J.sa, J.sb, J.so);
+    else lin_claim_256<T>(J.sa, J.sb, J.so);
+    return;
+  }
+  fold_tile<T, 1, true>(gA, J, blockIdx.x, threadIdx.x >> 6);
+}
+
+// Sumcheck fold on the matrix cores with loads in flight (many tiles):
+// persistent blocks of 4 waves, wave w owns output row-block w
+// and keeps its fragments of M_r (ND planes x 4 k-blocks) in registers for
+// the block's lifetime.  A tile's 32 elements are shared by the block: wave
+// w loads k-block w of lo and hi (one 16-limb access pair per lane), forms
+// the digit planes of hi - lo for that k-block into one of two LDS buffers,
+// and keeps lo's k-block w, which is exactly the lo part of its output rows.
+// After one barrier every wave reads the tile's four k-blocks of planes and
+// runs its row-block's MFMAs.  The raw loads of the block's next two tiles
+// are issued unconditionally (past-the-end elements read element 0 and are
+// zeroed) and stay in flight across the barrier (lds_barrier orders LDS
+// only).  In-place folds are safe: wave w overwrites only the lo limbs it
+// read itself.  With m a multiple of 32 a tile lies in one table (one
+// division per tile).  Two blocks per CU.  Tiles of block b: b, b + nfold,
+// ...
+template <class T>
+__global__ void __launch_bounds__(256, 2) k_fold_mx_bulk(const uint32_t* frag, FoldJob<T> J) {
+  using C = CT<T>;
+  using L = LT<T>;
+  constexpr int ND = kND<T>;
+  __shared__ v4i sD[2][4 * ND * 64];  // [buffer][k-block][plane][lane]
+  const int l64 = threadIdx.x & 63, col = l64 & 31, h = l64 >> 5, w = threadIdx.x >> 6;
   const uint64_t total = J.m * (J.ntab[0] + J.ntab[1]), ntiles = (total + 31) / 32;
   const unsigned nfold = gridDim.x - (J.side ? 1 : 0);
   if (J.side && blockIdx.x == nfold) {
@@ -357,17 +384,97 @@ __global__ void __launch_bounds__(256) k_fold_mx(const uint32_t* frag, FoldJob<T
     else lin_claim_256<T>(J.sa, J.sb, J.so);
     return;
   }
-  if (split) {
-    fold_tile<T, 1, true>(gA, J, blockIdx.x, threadIdx.x >> 6);
-    return;
-  }
-  stage_frags<ND>(gA, sA);
-  __syncthreads();
-  for (uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntiles; tile += (uint64_t)nfold * 4) {
-    // keeps the A-fragment reads inside the loop (hoisted, they would pin
-    // ND*64 registers)
-    asm volatile("" ::: "memory");
-    fold_tile<T, 4, false>(sA, J, tile, 0);
+  const v4i* gA = reinterpret_cast<const v4i*>(frag);
+  v4i A[ND][4];
+#pragma unroll
+  for (int p = 0; p < ND; p++)
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++) A[p][kb] = gA[frag_at(p, kb, w) * 64 + l64];
+  const bool whole = J.m % 32 == 0;
+  // element pointers of a tile: lo (hi = lo + m) and out; ok false past the end
+  auto locate = [&](uint64_t tile, const L*& lo, L*& out, bool& ok) {
+    const uint64_t x0 = tile * 32;
+    ok = tile < ntiles && x0 + col < total;
+    if (whole && tile < ntiles) {
+      const uint64_t n0 = J.m * J.ntab[0];
+      const bool g1 = x0 >= n0;
+      const uint64_t x = g1 ? x0 - n0 : x0, tab = x / J.m, i = x - tab * J.m + col;
+      lo = (g1 ? J.src[1] : J.src[0]) + (tab * (g1 ? J.sstride[1] : J.sstride[0]) + i) * E;
+      out = (g1 ? J.dst[1] : J.dst[0]) + (tab * (g1 ? J.dstride[1] : J.dstride[0]) + i) * E;
+    } else {
+      ok = fold_locate<T>(J, tile < ntiles ? x0 + col : total, lo, out) && ok;
+    }
+    if (!ok) lo = J.src[0];
+  };
+  // raw limbs of k-block w of lo and hi
+  auto load = [&](uint64_t tile, uint32_t (*R)[16], L*& out, bool& ok) {
+    const L* lo;
+    locate(tile, lo, out, ok);
+    ld16k<T>(lo, w, h, R[0]);
+    ld16k<T>(lo + (ok ? J.m : 0) * E, w, h, R[1]);
+  };
+  auto stage = [&](uint32_t (*R)[16], bool ok, int buf) {
+    uint32_t d[16];
+#pragma unroll
+    for (int b = 0; b < 16; b++) {
+      if (!ok) R[0][b] = R[1][b] = 0;
+      d[b] = R[1][b] - R[0][b];
+    }
+    v4i P[ND];
+    planes16<ND>(d, P);
+#pragma unroll
+    for (int p = 0; p < ND; p++) sD[buf][(w * ND + p) * 64 + l64] = P[p];
+  };
+  // the B operands are read from LDS next to their MFMAs (registers go to
+  // the loads in flight instead)
+  auto compute = [&](const uint32_t* lo, L* out, bool ok, int buf) {
+    v16i acc = (v16i)0;
+#pragma unroll
+    for (int sd = ND - 1; sd >= 0; sd--) {
+      if (sd != ND - 1) acc <<= 8;
+#pragma unroll
+      for (int p = 0; p <= sd; p++)
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++)
+          acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[p][kb], sD[buf][(kb * ND + sd - p) * 64 + l64], acc, 0, 0, 0);
+    }
+    if (ok) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        C x[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) x[c] = (C)(lo[4 * j + c] + (uint32_t)acc[4 * j + c]);
+        st4<T>(out + 32 * w + 8 * j + 4 * h, x);
+      }
+    }
+  };
+  // two tiles' raw loads in flight: the register set of tile t is staged,
+  // then reloaded with tile t + 2 nfold (a third set spills registers and
+  // ran slower)
+  constexpr int NP = 2;
+  uint32_t R[NP][2][16], lo[16];
+  L* o[NP];
+  bool k[NP];
+  uint64_t tile = blockIdx.x;
+  int buf = 0;
+#pragma unroll
+  for (int u = 0; u < NP; u++) load(tile + u * (uint64_t)nfold, R[u], o[u], k[u]);
+  while (true) {
+#pragma unroll
+    for (int u = 0; u < NP; u++) {
+      if (tile >= ntiles) break;
+      stage(R[u], k[u], buf);
+#pragma unroll
+      for (int b = 0; b < 16; b++) lo[b] = R[u][0][b];
+      L* oc = o[u];
+      const bool kc = k[u];
+      load(tile + NP * (uint64_t)nfold, R[u], o[u], k[u]);
+      lds_barrier();  // the tile's planes are in LDS
+      compute(lo, oc, kc, buf);
+      buf ^= 1;
+      tile += nfold;
+    }
+    if (tile >= ntiles) break;
   }
 }
 

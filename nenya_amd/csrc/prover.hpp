@@ -714,16 +714,21 @@ template <class T> struct Engine {
       FoldJob<T> J{{s0, s1 ? s1 : s0}, {d0, d1 ? d1 : d0}, {n0, s1 ? n1 : 0}, {ss0, ss1}, {ds0, ds1}, m,
                    sd.kind, sd.sa, sd.sb, sd.so};
       const uint64_t ntiles = (m * (J.ntab[0] + J.ntab[1]) + 31) / 32;
-      // few tiles: 4 waves per tile, fragments from L2 (the autotuner's
-      // picks, when it ran, replace the built-in choice)
+      // few tiles: 4 waves per tile, fragments from L2 (k_fold_mx); many:
+      // persistent blocks with loads in flight (k_fold_mx_bulk); the
+      // autotuner's picks, when it ran, replace the built-in choice
       const Tuning& tu = Tuning::get();
       const int lt = Tuning::lg(ntiles), kx = Tuning::kidx(T::K);
       const int tmode = tu.fold_mode[kx][lt].load(std::memory_order_relaxed);
       const int tcap = tu.fold_cap[kx][lt].load(std::memory_order_relaxed);
       const int split = tmode ? tmode == 1 : ntiles <= 256;
-      const unsigned nb = split ? (unsigned)ntiles : grid_for(ntiles, 4, tcap ? (unsigned)tcap : 512u);
-      launch("k_fold_mx", m * (J.ntab[0] + J.ntab[1]) * 3 * EB, m * (J.ntab[0] + J.ntab[1]) * 32 * PM, (k_fold_mx<T>),
-             dim3(nb + (sd.kind ? 1 : 0)), dim3(256), mm.fp, J, split);
+      const double fb = m * (J.ntab[0] + J.ntab[1]) * 3 * EB, fm = m * (J.ntab[0] + J.ntab[1]) * 32 * PM;
+      if (split)
+        launch("k_fold_mx", fb, fm, (k_fold_mx<T>), dim3((unsigned)ntiles + (sd.kind ? 1 : 0)), dim3(256), mm.fp, J);
+      else
+        launch("k_fold_mx", fb, fm, (k_fold_mx_bulk<T>),
+               dim3((unsigned)std::min<uint64_t>(ntiles, tcap ? (unsigned)tcap : 512u) + (sd.kind ? 1 : 0)), dim3(256),
+               mm.fp, J);
     } else {
       if constexpr (kMx64<T>) {
         const uint64_t tot = m * (n0 + (s1 ? n1 : 0));

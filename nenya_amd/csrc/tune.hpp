@@ -3,10 +3,11 @@
 // For each limb width and each power-of-two work size it times the candidate
 // shapes on synthetic tables (the kernels' cost does not depend on the
 // values) and records the fastest in Tuning, which every later launch reads:
-//   * k_fold_mx (k <= 32): split mode (a workgroup per 32-element tile, the
-//     fragments read from L2) or bulk mode (fragments in LDS, persistent
-//     workgroups) with 256 / 512 / 1024 / 2048 workgroups;
-//   * k_gram_lin (k <= 32): 128 / 256 / 512 / 1024 workgroups;
+//   * folds (k <= 32): split mode (k_fold_mx: a workgroup per 32-element
+//     tile, the fragments read from L2) or bulk mode (k_fold_mx_bulk:
+//     persistent workgroups, a wave per row-block, loads two tiles ahead)
+//     with 256 / 512 / 1024 workgroups;
+//   * k_gram_lin2 (k <= 32): 128 / 256 / 512 / 1024 workgroups;
 //   * k_fold (k = 64, vector ALUs): a workgroup cap of 2048 .. 16384.
 // The report lists, per width and size, the pick with its time and the
 // built-in choice's time.
@@ -59,9 +60,12 @@ template <class T> void tune_width(std::ostringstream& js, int lmin, int lmax) {
       const uint64_t m = uint64_t(1) << lg, ntiles = m / 32;
       FoldJob<T> J{{src.p, src.p}, {dst.p, dst.p}, {1, 0}, {0, 0}, {0, 0}, m, 0, nullptr, nullptr, nullptr};
       auto run = [&](int split, unsigned cap) {
-        const unsigned nb = split ? (unsigned)ntiles : grid_for(ntiles, 4, cap);
         return time_launch(cx.st, [&] {
-          hipLaunchKernelGGL(k_fold_mx<T>, dim3(nb), dim3(256), 0, cx.st, (const uint32_t*)fr.p, J, split);
+          if (split)
+            hipLaunchKernelGGL(k_fold_mx<T>, dim3((unsigned)ntiles), dim3(256), 0, cx.st, (const uint32_t*)fr.p, J);
+          else
+            hipLaunchKernelGGL(k_fold_mx_bulk<T>, dim3((unsigned)std::min<uint64_t>(ntiles, cap)), dim3(256), 0,
+                               cx.st, (const uint32_t*)fr.p, J);
         });
       };
       // the built-in choice (split up to 256 tiles, else 512 workgroups) is
@@ -76,7 +80,7 @@ template <class T> void tune_width(std::ostringstream& js, int lmin, int lmax) {
         if (def_split) def = t;
         if (t < best) { best = t; bmode = 1; bcap = 0; }
       }
-      for (unsigned cap : {256u, 512u, 1024u, 2048u}) {
+      for (unsigned cap : {256u, 512u, 1024u}) {
         const double t = run(0, cap);
         if (!def_split && cap == 512u) def = t;
         if (t < best) { best = t; bmode = 2; bcap = cap; }
@@ -85,7 +89,7 @@ template <class T> void tune_width(std::ostringstream& js, int lmin, int lmax) {
       tu.fold_mode[kx][lt] = bmode;
       tu.fold_cap[kx][lt] = (int)bcap;
       js << (lg > lmin ? "," : "") << "\"2^" << lg << "\":{\"pick\":\"" << (bmode == 1 ? "split" : "bulk")
-         << "\",\"workgroups\":" << (bmode == 1 ? ntiles : grid_for(ntiles, 4, bcap)) << ",\"us\":" << best * 1e3
+         << "\",\"workgroups\":" << (bmode == 1 ? ntiles : std::min<uint64_t>(ntiles, bcap)) << ",\"us\":" << best * 1e3
          << ",\"builtin_us\":" << def * 1e3 << "}";
     }
     js << "},\"gram\":{";
