@@ -167,7 +167,7 @@ inline void par_memcpy(uint8_t* dst, const uint8_t* src, size_t n) {
 inline unsigned dl_blocks() {
   static const unsigned v = [] {
     const char* e = std::getenv("NENYA_DL_BLOCKS");
-    return e ? (unsigned)std::max(1, std::atoi(e)) : 16u;
+    return e ? (unsigned)std::max(1, std::atoi(e)) : 32u;
   }();
   return v;
 }
