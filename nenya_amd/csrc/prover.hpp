@@ -929,16 +929,37 @@ template <class T> struct Engine {
   // one-block kernel on another stream for 1.1 ms; with 16, for 70 us, at
   // the same copy time).
 
+  // Sections of at least dl_dma_min() bytes (256 KiB) go through the DMA engines
+  // (hipMemcpyAsync into the pinned buffer): a copy kernel's writes to host
+  // memory slow the HBM kernels it overlaps (a 2 GB device copy beside a
+  // 64 MiB copy-kernel download took 1.3-3.3x its time, beside the DMA copy
+  // 1.0x), while small sections keep the copy kernel (lower per-call cost).
+  // NENYA_DL_DMA_MIN overrides the threshold (bytes).
+  static size_t dl_dma_min() {
+    static const size_t v = [] {
+      const char* e = std::getenv("NENYA_DL_DMA_MIN");
+      return e ? (size_t)std::strtoull(e, nullptr, 10) : (size_t)(256 << 10);
+    }();
+    return v;
+  }
   template <class X> void download(uint8_t* dst, DBuf<X>& src, size_t bytes) {
     if (!bytes) return;
     uint8_t* ddst = nullptr;
     HIP_CHECK(hipHostGetDevicePointer((void**)&ddst, dst, 0));
     const dim3 grid((unsigned)std::min<uint64_t>((bytes + 4095) / 4096, dl_blocks()));
+    auto copy = [&](hipStream_t s, bool dma) {
+      if (dma) {
+        HIP_CHECK(hipMemcpyAsync(dst, src.p, bytes, hipMemcpyDeviceToHost, s));
+      } else {
+        hipLaunchKernelGGL(k_copy_out, grid, dim3(256), 0, s, (const uint8_t*)src.p, ddst, (uint64_t)bytes);
+        HIP_CHECK(hipGetLastError());
+      }
+    };
     if (!pipelined || serial) {
-      // contexts of a multi-worker server and no-alloc calls: in order on the prover stream
-      // (the workers' streams already overlap each other)
-      hipLaunchKernelGGL(k_copy_out, grid, dim3(256), 0, st, (const uint8_t*)src.p, ddst, (uint64_t)bytes);
-      HIP_CHECK(hipGetLastError());
+      // contexts of a multi-worker server and no-alloc calls: in order on the
+      // prover stream (the workers' streams already overlap each other); the
+      // no-alloc calls keep the copy kernel for every section
+      copy(st, !serial && bytes >= dl_dma_min());
       return;
     }
     if (!cst) {
@@ -947,8 +968,7 @@ template <class T> struct Engine {
     }
     HIP_CHECK(hipEventRecord(cev, st));
     HIP_CHECK(hipStreamWaitEvent(cst, cev, 0));
-    hipLaunchKernelGGL(k_copy_out, grid, dim3(256), 0, cst, (const uint8_t*)src.p, ddst, (uint64_t)bytes);
-    HIP_CHECK(hipGetLastError());
+    copy(cst, bytes >= dl_dma_min());
     held.push_back({src.p, src.n * sizeof(X)});
     src.p = nullptr;
     src.n = 0;
