@@ -557,27 +557,52 @@ __global__ void __launch_bounds__(256) k_mulfix_mx(const uint32_t* frag, const L
 // u_0 = 1 - alpha, u_1 = alpha (R1 scalars acting per coefficient), so an
 // entry (row 32p+s of matrix w, value v) adds v * G[w][p][s/2] * u_{s%2} to
 // rows[l][q].  One 32-lane group per output (l, q); entries CSR by output,
-// code = p << 7 | w << 5 | s.
+// code = p << 7 | w << 5 | s.  The group reads up to 32 entries' codes and
+// values at once (lane i: entry k0 + i), then gathers their G elements four
+// at a time: the four loads are issued before any is used (indices past the
+// output's last entry repeat that entry with value 0, so no load is
+// conditional), keeping 2 KiB per group in flight instead of one element
+// behind a dependent code read.
 template <class T>
 __global__ void __launch_bounds__(256) k_rows_gather(const uint64_t* ptr, const uint32_t* code, const LT<T>* val,
                                                      const LT<T>* G, uint64_t nb, LT<T>* rows, uint64_t nout) {
   using C = CT<T>;
+  constexpr int U = 4;
   const int t = threadIdx.x & 31;
   for (uint64_t o = (uint64_t)blockIdx.x * 8 + (threadIdx.x >> 5); o < nout; o += (uint64_t)gridDim.x * 8) {
     C acc[4] = {0, 0, 0, 0};
-    for (uint64_t k = ptr[o]; k < ptr[o + 1]; k++) {
-      const uint32_t cd = code[k];
-      const uint64_t p = cd >> 7, w = (cd >> 5) & 3, s = cd & 31;
-      C g[4], f[4];
-      ld4<T>(G + ((w * nb + p) * 16 + (s >> 1)) * E + 4 * t, g);
-      if (s & 1) {  // alpha g
-        f[0] = 0 - g[3]; f[1] = g[0] - g[3]; f[2] = g[1]; f[3] = g[2];
-      } else {      // (1 - alpha) g
-        f[0] = g[0] + g[3]; f[1] = g[1] - g[0] + g[3]; f[2] = g[2] - g[1]; f[3] = g[3] - g[2];
-      }
-      const C v = (C)val[k];
+    const uint64_t k0 = ptr[o], k1 = ptr[o + 1];
+    for (uint64_t kb = k0; kb < k1; kb += 32) {
+      const uint32_t ne = k1 - kb < 32 ? (uint32_t)(k1 - kb) : 32u;
+      const uint64_t ke = kb + ((uint32_t)t < ne ? (uint32_t)t : ne - 1);
+      const uint32_t cdl = code[ke];
+      const C vl = (uint32_t)t < ne ? (C)val[ke] : (C)0;
+      for (uint32_t j = 0; j < ne; j += U) {
+        C g[U][4];
+        uint32_t sj[U];
 #pragma unroll
-      for (int c = 0; c < 4; c++) acc[c] += v * f[c];
+        for (int u = 0; u < U; u++) {
+          // lanes j + u >= ne hold the last entry's code (value 0 below)
+          const uint32_t cd = __shfl(cdl, (int)(j + u < ne ? j + u : ne - 1), 32);
+          const uint64_t p = cd >> 7, w = (cd >> 5) & 3;
+          sj[u] = cd & 31;
+          ld4<T>(G + ((w * nb + p) * 16 + (sj[u] >> 1)) * E + 4 * t, g[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          // j + u <= 31: lanes at or past ne carry value 0
+          const C v = __shfl(vl, (int)(j + u), 32);
+          C f[4];
+          if (sj[u] & 1) {  // alpha g
+            f[0] = 0 - g[u][3]; f[1] = g[u][0] - g[u][3]; f[2] = g[u][1]; f[3] = g[u][2];
+          } else {          // (1 - alpha) g
+            f[0] = g[u][0] + g[u][3]; f[1] = g[u][1] - g[u][0] + g[u][3]; f[2] = g[u][2] - g[u][1];
+            f[3] = g[u][3] - g[u][2];
+          }
+#pragma unroll
+          for (int c = 0; c < 4; c++) acc[c] += v * f[c];
+        }
+      }
     }
     st4<T>(rows + o * E + 4 * t, acc);
   }
