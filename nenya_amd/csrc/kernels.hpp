@@ -1379,8 +1379,9 @@ __device__ __forceinline__ void square_times(C* sx, C* sx2, C (*sred)[32][4], in
   }
 }
 
-// Lifts src (device or host-pinned memory) into el with a 256-thread block;
-// the lifted value is left in sx (LDS, C-typed, 16-byte aligned) as well.
+// Lifts src (device or host-pinned memory) into el (if not null) with a
+// 256-thread block; the lifted value is left in sx (LDS, C-typed, 16-byte
+// aligned) as well.
 template <class T>
 __device__ __forceinline__ void teich_lift_block(const LT<T>* src, LT<T>* el, const LT<T>* frob_t, int squarings,
                                                  CT<T>* sx) {
@@ -1443,7 +1444,7 @@ __device__ __forceinline__ void teich_lift_block(const LT<T>* src, LT<T>* el, co
     square_times<C, true>(sx, sx2, sred, a);
     square_times<C>(sx, sx2, sred, squarings - a);
   }
-  if (tid < E) el[tid] = (LT<T>)sx[tid];
+  if (el && tid < E) el[tid] = (LT<T>)sx[tid];
 }
 
 template <class T>

@@ -102,20 +102,24 @@ __device__ __forceinline__ int frag_at(int p, int kb, int rb) { return (p * 4 + 
 // levels with one barrier each.  Multiplying by Y^k (k <= 16) shifts by k
 // and sends each of the k top elements h = o[s + 32 - k] (s < k) to
 // -alpha h at Y^s, -h at Y^(s+1) and -h at Y^(s+3), all below Y^32.
+// Limb l of Y^k o (k <= 29: the wrapped terms o_t Y^(t+k), t + k >= 32, go
+// to Y^s, Y^(s+1), Y^(s+3) with s = t + k - 32 < k, all below Y^32).
+template <class C> __device__ __forceinline__ C mul_yk_limb(const C* o, int k, int l) {
+  const int t = l >> 2, c = l & 3;
+  C x = t >= k ? o[l - 4 * k] : (C)0;
+  if (t < k) {  // -alpha * h = (h3, h3 - h0, -h1, -h2)
+    const C* h = o + 4 * (t + 32 - k);
+    x += (c == 0) ? h[3] : (c == 1) ? (C)(h[3] - h[0]) : (c == 2) ? (C)(0 - h[1]) : (C)(0 - h[2]);
+  }
+  if (t >= 1 && t <= k) x -= o[4 * (t - 1 + 32 - k) + c];
+  if (t >= 3 && t <= k + 2) x -= o[4 * (t - 3 + 32 - k) + c];
+  return x;
+}
 template <class C> __device__ __forceinline__ void mulmat_chain(C (*v)[E]) {
   for (int k = 1; k < 32; k <<= 1) {
     for (int id = threadIdx.x; id < k * E; id += blockDim.x) {
       const int i = k + id / E, l = id % E;
-      const int t = l >> 2, c = l & 3;
-      const C* o = v[i - k];
-      C x = t >= k ? o[l - 4 * k] : (C)0;
-      if (t < k) {  // -alpha * h = (h3, h3 - h0, -h1, -h2)
-        const C* h = o + 4 * (t + 32 - k);
-        x += (c == 0) ? h[3] : (c == 1) ? (C)(h[3] - h[0]) : (c == 2) ? (C)(0 - h[1]) : (C)(0 - h[2]);
-      }
-      if (t >= 1 && t <= k) x -= o[4 * (t - 1 + 32 - k) + c];
-      if (t >= 3 && t <= k + 2) x -= o[4 * (t - 3 + 32 - k) + c];
-      v[i][l] = x;
+      v[i][l] = mul_yk_limb<C>(v[i - k], k, l);
     }
     __syncthreads();
   }
@@ -124,38 +128,44 @@ template <class C> __device__ __forceinline__ void mulmat_chain(C (*v)[E]) {
 // A-fragments of M_r from v[i] = Y^i r: column 4i+c of M_r is X^c v_i with
 // X*(a0,a1,a2,a3) = (-a3, a0-a3, a1, a2) (X^4 = -X - 1).  Item (row, kb, h)
 // writes the 16 bytes of lane (row%32)+32h in fragment (p, kb, row/32): word
-// j holds columns 4i..4i+3 with i = 8kb + 2j + h (k = 32kb + kappa(h, 4j+c)).
-template <class T> __device__ __forceinline__ void mulmat_frag(const CT<T> (*v)[E], uint32_t* frag) {
+// j holds columns 4i..4i+3 with i = 8kb + 2j + h (k = 32kb + kappa(h, 4j+c)),
+// read from vj[j] = v[8kb + 2j + h].
+template <class T>
+__device__ __forceinline__ void mulmat_frag_item(const CT<T>* const (&vj)[4], int row, int kb, int h, uint32_t* frag) {
   using C = CT<T>;
   constexpr int ND = kND<T>;
+  const int t = row >> 2, cp = row & 3;
+  uint32_t words[ND][4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const C* vi = vj[j];
+    C a[4] = {vi[4 * t], vi[4 * t + 1], vi[4 * t + 2], vi[4 * t + 3]};
+    uint32_t z[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      z[c] = bal_digits<ND>((uint32_t)a[cp]);
+      C n0 = 0 - a[3], n1 = a[0] - a[3], n2 = a[1], n3 = a[2];
+      a[0] = n0; a[1] = n1; a[2] = n2; a[3] = n3;
+    }
+#pragma unroll
+    for (int p = 0; p < ND; p++) {
+      uint32_t wd = 0;
+#pragma unroll
+      for (int c = 0; c < 4; c++) wd |= ((z[c] >> (8 * p)) & 0xffu) << (8 * c);
+      words[p][j] = wd;
+    }
+  }
+  const int rb = row >> 5, lane = (row & 31) + 32 * h;
+#pragma unroll
+  for (int p = 0; p < ND; p++)
+    reinterpret_cast<uint4*>(frag)[frag_at(p, kb, rb) * 64 + lane] =
+        make_uint4(words[p][0], words[p][1], words[p][2], words[p][3]);
+}
+template <class T> __device__ __forceinline__ void mulmat_frag(const CT<T> (*v)[E], uint32_t* frag) {
   for (int id = threadIdx.x; id < E * 8; id += blockDim.x) {
     const int row = id & (E - 1), kb = (id >> 7) & 3, h = id >> 9;
-    const int t = row >> 2, cp = row & 3;
-    uint32_t words[ND][4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int i = 8 * kb + 2 * j + h;
-      C a[4] = {v[i][4 * t], v[i][4 * t + 1], v[i][4 * t + 2], v[i][4 * t + 3]};
-      uint32_t z[4];
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        z[c] = bal_digits<ND>((uint32_t)a[cp]);
-        C n0 = 0 - a[3], n1 = a[0] - a[3], n2 = a[1], n3 = a[2];
-        a[0] = n0; a[1] = n1; a[2] = n2; a[3] = n3;
-      }
-#pragma unroll
-      for (int p = 0; p < ND; p++) {
-        uint32_t wd = 0;
-#pragma unroll
-        for (int c = 0; c < 4; c++) wd |= ((z[c] >> (8 * p)) & 0xffu) << (8 * c);
-        words[p][j] = wd;
-      }
-    }
-    const int rb = row >> 5, lane = (row & 31) + 32 * h;
-#pragma unroll
-    for (int p = 0; p < ND; p++)
-      reinterpret_cast<uint4*>(frag)[frag_at(p, kb, rb) * 64 + lane] =
-          make_uint4(words[p][0], words[p][1], words[p][2], words[p][3]);
+    const CT<T>* const vj[4] = {v[8 * kb + h], v[8 * kb + 2 + h], v[8 * kb + 4 + h], v[8 * kb + 6 + h]};
+    mulmat_frag_item<T>(vj, row, kb, h, frag);
   }
 }
 
@@ -175,7 +185,16 @@ __global__ void __launch_bounds__(256) k_mulmat(const LT<T>* r, uint32_t* frag) 
 
 // Lift one challenge (natural sample src, may be host-pinned; with w.flag
 // set, posted by the host into the mailbox) into x and emit the M_r
-// fragments of the result.
+// fragments of the result.  With one block, that block makes all of them.
+// With LIFT_MM_BLOCKS blocks every block lifts the same sample (the lift is
+// one block's dependent chain, so the copies run side by side and add no
+// latency), block 0 stores it, and block b makes the fragments of k-block
+// b/2, half b%2 from its four vectors Y^i r (i = 8kb + h + 2j), reached in
+// two single-vector steps (Y^(8kb+h) r, then its Y^2, Y^4 and Y^6 multiples)
+// instead of the five levels of the 31-vector chain.  src
+// must not alias x when several blocks run (a block may still read src
+// after block 0 has stored x).
+constexpr int LIFT_MM_BLOCKS = 8;
 template <class T>
 __global__ void __launch_bounds__(256) k_lift_mulmat(const LT<T>* src, LT<T>* x, const LT<T>* frob_t, int squarings,
                                                      uint32_t* frag, MbWait w) {
@@ -187,10 +206,26 @@ __global__ void __launch_bounds__(256) k_lift_mulmat(const LT<T>* src, LT<T>* x,
     mb_fetch(reinterpret_cast<const uint32_t*>(src), E * sizeof(LT<T>) / 4, posted);
     src = reinterpret_cast<const LT<T>*>(posted);
   }
-  teich_lift_block<T>(src, x, frob_t, squarings, v[0]);
+  teich_lift_block<T>(src, blockIdx.x == 0 ? x : nullptr, frob_t, squarings, v[0]);
   __syncthreads();
-  mulmat_chain<C>(v);
-  mulmat_frag<T>(v, frag);
+  if (gridDim.x == 1) {
+    mulmat_chain<C>(v);
+    mulmat_frag<T>(v, frag);
+    return;
+  }
+  const int kb = (int)blockIdx.x >> 1, h = (int)blockIdx.x & 1, tid = threadIdx.x, i0 = 8 * kb + h;
+  // v[1] = Y^i0 r (i0 <= 25), then v[2 + j] = Y^(2j + 2) v[1]
+  if (i0 > 0) {
+    if (tid < E) v[1][tid] = mul_yk_limb<C>(v[0], i0, tid);
+    __syncthreads();
+  }
+  const C* u = v[i0 > 0 ? 1 : 0];
+  for (int id = tid; id < 3 * E; id += blockDim.x) v[2 + id / E][id % E] = mul_yk_limb<C>(u, 2 * (id / E) + 2, id % E);
+  __syncthreads();
+  if (tid < E) {
+    const C* const vj[4] = {u, v[2], v[3], v[4]};
+    mulmat_frag_item<T>(vj, tid, kb, h, frag);
+  }
 }
 
 // The 16 limbs of half h in k-block kb (kappa order): groups of 4 at 8j + 4h.

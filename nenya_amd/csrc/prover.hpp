@@ -1212,8 +1212,8 @@ template <class T> struct Engine {
     if constexpr (kMx<T>) {
       mm->fp = frag;
       if (frag)
-        launch("k_lift_mulmat", 2 * EB, lm, (k_lift_mulmat<T>), dim3(1), dim3(256), src, d, d_frob.p, T::K - 1, frag,
-               w);
+        launch("k_lift_mulmat", 2 * EB, lm, (k_lift_mulmat<T>), dim3(LIFT_MM_BLOCKS), dim3(256), src, d, d_frob.p,
+               T::K - 1, frag, w);
       else  // a tail round's fold needs no fragments
         launch("k_lift", 2 * EB, lm, (k_lift_from<T>), dim3(1), dim3(256), src, d, d_frob.p, T::K - 1, w);
     } else {
@@ -1262,6 +1262,7 @@ template <class T> struct Engine {
       if (mm && cnt == 1) {
         mm->frag = DBuf<uint32_t>(pool, (size_t)kND<T> * 4096);
         mm->fp = mm->frag.p;
+        // in place (src = d): one block (see k_lift_mulmat)
         launch("k_lift_mulmat", 2 * EB, lm, (k_lift_mulmat<T>), dim3(1), dim3(256), (const L*)d.p, d.p, d_frob.p,
                T::K - 1, mm->frag.p, MbWait{});
         fused = true;
@@ -1291,8 +1292,8 @@ template <class T> struct Engine {
       if (mm) {
         mm->frag = DBuf<uint32_t>(pool, (size_t)kND<T> * 4096);
         mm->fp = mm->frag.p;
-        launch("k_lift_mulmat", 2 * EB, lm, (k_lift_mulmat<T>), dim3(1), dim3(256), src, d, d_frob.p, T::K - 1,
-               mm->frag.p, MbWait{});
+        launch("k_lift_mulmat", 2 * EB, lm, (k_lift_mulmat<T>), dim3(LIFT_MM_BLOCKS), dim3(256), src, d, d_frob.p,
+               T::K - 1, mm->frag.p, MbWait{});
         fused = true;
       }
     }
