@@ -940,35 +940,41 @@ __device__ __forceinline__ void ld16k64(const uint64_t* el, int kb, int h, uint6
 }
 // Fragments of M_r (8 digit planes) from v[i] = Y^i r, as mulmat_frag;
 // items [id0, id1) of the E * 8 (row, k-block, half) items.
+// One fragment item (row, kb, h) of mulmat_frag64, from vj[j] = v[8kb + 2j + h].
+__device__ __forceinline__ void mulmat_frag64_item(const uint64_t* const (&vj)[4], int row, int kb, int h,
+                                                   uint32_t* frag) {
+  const int t = row >> 2, cp = row & 3;
+  uint32_t words[8][4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const uint64_t* vi = vj[j];
+    uint64_t a[4] = {vi[4 * t], vi[4 * t + 1], vi[4 * t + 2], vi[4 * t + 3]};
+    uint64_t z[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      z[c] = bal_digits64(a[cp]);
+      const uint64_t n0 = 0 - a[3], n1 = a[0] - a[3], n2 = a[1], n3 = a[2];
+      a[0] = n0; a[1] = n1; a[2] = n2; a[3] = n3;
+    }
+#pragma unroll
+    for (int p = 0; p < 8; p++) {
+      uint32_t wd = 0;
+#pragma unroll
+      for (int c = 0; c < 4; c++) wd |= (uint32_t)((z[c] >> (8 * p)) & 0xffu) << (8 * c);
+      words[p][j] = wd;
+    }
+  }
+  const int rb = row >> 5, lane = (row & 31) + 32 * h;
+#pragma unroll
+  for (int p = 0; p < 8; p++)
+    reinterpret_cast<uint4*>(frag)[frag_at(p, kb, rb) * 64 + lane] =
+        make_uint4(words[p][0], words[p][1], words[p][2], words[p][3]);
+}
 __device__ __forceinline__ void mulmat_frag64(const uint64_t (*v)[E], uint32_t* frag, int id0, int id1) {
   for (int id = id0 + threadIdx.x; id < id1; id += blockDim.x) {
     const int row = id & (E - 1), kb = (id >> 7) & 3, h = id >> 9;
-    const int t = row >> 2, cp = row & 3;
-    uint32_t words[8][4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int i = 8 * kb + 2 * j + h;
-      uint64_t a[4] = {v[i][4 * t], v[i][4 * t + 1], v[i][4 * t + 2], v[i][4 * t + 3]};
-      uint64_t z[4];
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        z[c] = bal_digits64(a[cp]);
-        const uint64_t n0 = 0 - a[3], n1 = a[0] - a[3], n2 = a[1], n3 = a[2];
-        a[0] = n0; a[1] = n1; a[2] = n2; a[3] = n3;
-      }
-#pragma unroll
-      for (int p = 0; p < 8; p++) {
-        uint32_t wd = 0;
-#pragma unroll
-        for (int c = 0; c < 4; c++) wd |= (uint32_t)((z[c] >> (8 * p)) & 0xffu) << (8 * c);
-        words[p][j] = wd;
-      }
-    }
-    const int rb = row >> 5, lane = (row & 31) + 32 * h;
-#pragma unroll
-    for (int p = 0; p < 8; p++)
-      reinterpret_cast<uint4*>(frag)[frag_at(p, kb, rb) * 64 + lane] =
-          make_uint4(words[p][0], words[p][1], words[p][2], words[p][3]);
+    const uint64_t* const vj[4] = {v[8 * kb + h], v[8 * kb + 2 + h], v[8 * kb + 4 + h], v[8 * kb + 6 + h]};
+    mulmat_frag64_item(vj, row, kb, h, frag);
   }
 }
 // M_r fragments for r in device memory; blocks (b, y) handle r + b
@@ -984,6 +990,37 @@ __global__ void __launch_bounds__(256) k_mulmat64(const uint64_t* r, uint32_t* f
   mulmat_chain<uint64_t>(v);
   const int per = E * 8 / gridDim.y;
   mulmat_frag64(v, frag, blockIdx.y * per, (blockIdx.y + 1) * per);
+}
+
+// The 64-bit counterpart of k_lift_mulmat with LIFT_MM_BLOCKS blocks: every
+// block lifts the sample (posted by the host when w.flag is set), block 0
+// stores it in x, and block b makes the fragments of k-block b/2, half b%2
+// (8 digit planes) from Y^(8kb+h) r and its Y^2, Y^4, Y^6 multiples.  One
+// launch instead of the lift and k_mulmat64; src must not alias x.
+__global__ void __launch_bounds__(256) k_lift_mulmat64(const uint64_t* src, uint64_t* x, const uint64_t* frob_t,
+                                                       int squarings, uint32_t* frag, MbWait w) {
+  __shared__ __attribute__((aligned(16))) uint64_t v[5][E];
+  __shared__ uint32_t posted[E * 2];
+  if (w.flag) {
+    mb_wait(w);
+    mb_fetch(reinterpret_cast<const uint32_t*>(src), E * 2, posted);
+    src = reinterpret_cast<const uint64_t*>(posted);
+  }
+  teich_lift_block<W<64>>(src, blockIdx.x == 0 ? x : nullptr, frob_t, squarings, v[0]);
+  __syncthreads();
+  const int kb = (int)blockIdx.x >> 1, h = (int)blockIdx.x & 1, tid = threadIdx.x, i0 = 8 * kb + h;
+  if (i0 > 0) {
+    if (tid < E) v[1][tid] = mul_yk_limb<uint64_t>(v[0], i0, tid);
+    __syncthreads();
+  }
+  const uint64_t* u = v[i0 > 0 ? 1 : 0];
+  for (int id = tid; id < 3 * E; id += blockDim.x)
+    v[2 + id / E][id % E] = mul_yk_limb<uint64_t>(u, 2 * (id / E) + 2, id % E);
+  __syncthreads();
+  if (tid < E) {
+    const uint64_t* const vj[4] = {u, v[2], v[3], v[4]};
+    mulmat_frag64_item(vj, tid, kb, h, frag);
+  }
 }
 // Row-block rb of M x for one 32-element tile: R[i] (lane (col, h), rows
 // 8(i/4) + 4h + i%4 of the row-block) from the digit planes B of x.

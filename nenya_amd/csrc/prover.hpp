@@ -1216,14 +1216,16 @@ template <class T> struct Engine {
                T::K - 1, frag, w);
       else  // a tail round's fold needs no fragments
         launch("k_lift", 2 * EB, lm, (k_lift_from<T>), dim3(1), dim3(256), src, d, d_frob.p, T::K - 1, w);
+    } else if constexpr (kMx64<T>) {
+      if (frag) {
+        mm->fp = frag;
+        launch("k_lift_mulmat64", 2 * EB, lm, k_lift_mulmat64, dim3(LIFT_MM_BLOCKS), dim3(256), (const uint64_t*)src,
+               (uint64_t*)d, (const uint64_t*)d_frob.p, T::K - 1, frag, w);
+      } else {
+        launch("k_lift", 2 * EB, lm, (k_lift_from<T>), dim3(1), dim3(256), src, d, d_frob.p, T::K - 1, w);
+      }
     } else {
       launch("k_lift", 2 * EB, lm, (k_lift_from<T>), dim3(1), dim3(256), src, d, d_frob.p, T::K - 1, w);
-      if constexpr (kMx64<T>) {
-        if (frag) {
-          mm->fp = frag;
-          mulmat64(d, frag);
-        }
-      }
     }
   }
   // Host: sample the challenge (the transcript's sampler) and post it.
@@ -1294,6 +1296,14 @@ template <class T> struct Engine {
         mm->fp = mm->frag.p;
         launch("k_lift_mulmat", 2 * EB, lm, (k_lift_mulmat<T>), dim3(LIFT_MM_BLOCKS), dim3(256), src, d, d_frob.p,
                T::K - 1, mm->frag.p, MbWait{});
+        fused = true;
+      }
+    } else if constexpr (kMx64<T>) {
+      if (mm) {
+        mm->frag = DBuf<uint32_t>(pool, (size_t)8 * 4096);
+        mm->fp = mm->frag.p;
+        launch("k_lift_mulmat64", 2 * EB, lm, k_lift_mulmat64, dim3(LIFT_MM_BLOCKS), dim3(256), (const uint64_t*)src,
+               (uint64_t*)d, (const uint64_t*)d_frob.p, T::K - 1, mm->frag.p, MbWait{});
         fused = true;
       }
     }
