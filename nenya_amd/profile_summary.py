@@ -122,10 +122,16 @@ def traffic(args):
     fr, wr = load_counters(args.fetch), load_counters(args.write)
     f_scale, w_scale, cal_f, cal_w = calib_scales(fr, wr, args.calib_kernel, args.calib_bytes)
     cal_id = cal_w["Dispatch_Id"]
-    fk = [float(r["Counter_Value"]) for r in fr if short(r["Kernel_Name"]).startswith(args.kernel + "<")
-          or short(r["Kernel_Name"]) == args.kernel]
-    wk = [float(r["Counter_Value"]) for r in wr if short(r["Kernel_Name"]).startswith(args.kernel + "<")
-          or short(r["Kernel_Name"]) == args.kernel]
+    # a comma-separated list averages over the dispatches of several kernels
+    # (the split and persistent forms of one operation)
+    names = args.kernel.split(",")
+
+    def hit(r):
+        n = short(r["Kernel_Name"])
+        return any(n == k or n.startswith(k + "<") for k in names)
+
+    fk = [float(r["Counter_Value"]) for r in fr if hit(r)]
+    wk = [float(r["Counter_Value"]) for r in wr if hit(r)]
     if not fk or len(fk) != len(wk):
         sys.exit(f"kernel {args.kernel}: {len(fk)} FETCH and {len(wk)} WRITE dispatches")
     fetch = sum(fk) / len(fk) * f_scale * 1024
